@@ -1,0 +1,988 @@
+/*
+ * oracle.cpp -- CPU restatement of rgbd-flow's coarse-to-fine scene-flow path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  The product library never links,
+ * loads or calls this file.  Built by oracle/Makefile with g++ and
+ * -ffp-contract=off so every product and sum is rounded exactly where the
+ * C++ expression says.
+ *
+ * Two precisions:
+ *   ORC_PREC_REF  the reference's storage: images, derivatives, products,
+ *                 Laplacians and u in f64; the system, du, q, the device
+ *                 weight copies and the "sqrCU" squares in f32
+ *                 (src/sceneFlowSingleScaleJacobiCUDA.cu:44-47,680-747).
+ *   ORC_PREC_F64  the same algorithm with every buffer and product in f64.
+ *
+ * Parity status of each function is recorded in tests/test_oracle_*.py
+ * (closed forms, invariants, brute force).  The reference has no tests and
+ * no golden vectors, so nothing here is compared with reference output.
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <unordered_set>
+#include <vector>
+
+namespace {
+
+using std::vector;
+
+/* project.h:29-39 -- __min/__max return the FIRST argument's type */
+template <class A, class B> A liu_min(A a, B b) { return (a > b) ? (A)b : a; }
+template <class A, class B> A liu_max(A a, B b) { return (a < b) ? (A)b : a; }
+/* ImageProcessing.h:25 */
+inline int enforce_range(int x, int maxv) { return liu_min(liu_max(x, 0), maxv - 1); }
+
+inline double dsq(double x) { return x * x; }
+
+/* primesensorUtils.cpp:53-64 */
+double stereo_error(double z) { return dsq(z) / (1 / .00285 + z); }
+double stereo_error_ratio(double z) {
+  static const double err1 = stereo_error(1);
+  return stereo_error(z) / err1;
+}
+
+/* --------------------------------------------------------------------------
+ * ImageProcessing.h:128-148 BilinearInterpolate (accumulating into result)
+ * Truncating base, f64 weights clamped to [0,1], taps in (m,n) order
+ * (0,0),(0,1),(1,0),(1,1).
+ */
+void bilerp_acc(const double* img, int W, int H, int C, double x, double y, double* result) {
+  int xx = (int)x, yy = (int)y;
+  double dx = liu_max(liu_min(x - xx, 1), 0);
+  double dy = liu_max(liu_min(y - yy, 1), 0);
+  for (int m = 0; m <= 1; m++)
+    for (int n = 0; n <= 1; n++) {
+      int uu = enforce_range(xx + m, W);
+      int vv = enforce_range(yy + n, H);
+      int off = (vv * W + uu) * C;
+      double s = std::fabs(1 - m - dx) * std::fabs(1 - n - dy);
+      for (int l = 0; l < C; l++) result[l] += img[off + l] * s;
+    }
+}
+
+/* ImageProcessing.h:203-222 ResizeImage(ratio) */
+void resize_ratio(const double* src, int W, int H, int C, double ratio, double* dst) {
+  int DW = (double)W * ratio, DH = (double)H * ratio;
+  std::fill(dst, dst + (size_t)DW * DH * C, 0.0);
+  for (int i = 0; i < DH; i++)
+    for (int j = 0; j < DW; j++) {
+      double x = (double)(j + 1) / ratio - 1;
+      double y = (double)(i + 1) / ratio - 1;
+      bilerp_acc(src, W, H, C, x, y, dst + ((size_t)i * DW + j) * C);
+    }
+}
+
+/* ImageProcessing.h:224-242 ResizeImage(dst dims) */
+void resize_dims(const double* src, int Ws, int Hs, int C, int Wd, int Hd, double* dst) {
+  double xr = (double)Wd / Ws, yr = (double)Hd / Hs;
+  std::fill(dst, dst + (size_t)Wd * Hd * C, 0.0);
+  for (int i = 0; i < Hd; i++)
+    for (int j = 0; j < Wd; j++) {
+      double x = (double)(j + 1) / xr - 1;
+      double y = (double)(i + 1) / yr - 1;
+      bilerp_acc(src, Ws, Hs, C, x, y, dst + ((size_t)i * Wd + j) * C);
+    }
+}
+
+/* ImageProcessing.h:247-269 hfiltering (clamp-to-edge, taps l=-f..f in order) */
+void hfilter(const double* src, double* dst, int W, int H, int C, const double* f, int fs) {
+  std::fill(dst, dst + (size_t)W * H * C, 0.0);
+  for (int i = 0; i < H; i++)
+    for (int j = 0; j < W; j++) {
+      size_t off = (size_t)i * W * C;
+      double* b = dst + off + (size_t)j * C;
+      for (int l = -fs; l <= fs; l++) {
+        double wgt = f[l + fs];
+        int jj = enforce_range(j + l, W);
+        for (int k = 0; k < C; k++) b[k] += src[off + (size_t)jj * C + k] * wgt;
+      }
+    }
+}
+
+/* ImageProcessing.h:339-358 vfiltering */
+void vfilter(const double* src, double* dst, int W, int H, int C, const double* f, int fs) {
+  std::fill(dst, dst + (size_t)W * H * C, 0.0);
+  for (int i = 0; i < H; i++)
+    for (int j = 0; j < W; j++) {
+      double* b = dst + ((size_t)i * W + j) * C;
+      for (int l = -fs; l <= fs; l++) {
+        double wgt = f[l + fs];
+        int ii = enforce_range(i + l, H);
+        for (int k = 0; k < C; k++) b[k] += src[((size_t)ii * W + j) * C + k] * wgt;
+      }
+    }
+}
+
+/* Image.h:1179-1199 GaussianSmoothing + Image.h:1321-1330 imfilter_hv */
+void gaussian_smooth(const double* img, int W, int H, int C, double sigma, int fsize, double* out) {
+  vector<double> g(2 * fsize + 1);
+  double sum = 0;
+  double s2 = sigma * sigma * 2;
+  for (int i = -fsize; i <= fsize; i++) {
+    g[i + fsize] = std::exp(-(double)(i * i) / s2);
+    sum += g[i + fsize];
+  }
+  for (int i = 0; i < 2 * fsize + 1; i++) g[i] /= sum;
+  vector<double> tmp((size_t)W * H * C);
+  hfilter(img, tmp.data(), W, H, C, g.data(), fsize);
+  vfilter(tmp.data(), out, W, H, C, g.data(), fsize);
+}
+
+struct Level { int W, H; vector<double> d; };
+
+/* GaussianPyramid.cpp:20-63 */
+vector<Level> gaussian_pyramid(const double* img, int W, int H, int C, double ratio, int min_width) {
+  if (ratio > 0.98 || ratio < 0.4) ratio = 0.75;
+  double w = W * ratio;
+  int nLevels = 1;
+  while (w >= min_width) { nLevels++; w *= ratio; }
+  vector<Level> P(nLevels);
+  P[0].W = W; P[0].H = H; P[0].d.assign(img, img + (size_t)W * H * C);
+  double baseSigma = (1 / ratio - 1);
+  int n = std::log(0.25) / std::log(ratio);
+  double nSigma = baseSigma * n;
+  for (int i = 1; i < nLevels; i++) {
+    if (i <= n) {
+      double sigma = baseSigma * i;
+      vector<double> foo((size_t)W * H * C);
+      gaussian_smooth(img, W, H, C, sigma, (int)(sigma * 3), foo.data());
+      double r = std::pow(ratio, i);
+      P[i].W = (double)W * r; P[i].H = (double)H * r;
+      P[i].d.resize((size_t)P[i].W * P[i].H * C);
+      resize_ratio(foo.data(), W, H, C, r, P[i].d.data());
+    } else {
+      const Level& src = P[i - n];
+      vector<double> foo((size_t)src.W * src.H * C);
+      gaussian_smooth(src.d.data(), src.W, src.H, C, nSigma, (int)(nSigma * 3), foo.data());
+      double rate = (double)std::pow(ratio, i) * W / src.W;
+      P[i].W = (double)src.W * rate; P[i].H = (double)src.H * rate;
+      P[i].d.resize((size_t)P[i].W * P[i].H * C);
+      resize_ratio(foo.data(), src.W, src.H, C, rate, P[i].d.data());
+    }
+  }
+  return P;
+}
+
+/* depthMapPyramid.cpp:15-140 (validity-aware 2x2 component average) */
+vector<Level> depth_map_pyramid(const double* img, int W, int H, double ratio, int min_width) {
+  if (ratio > 0.98 || ratio < 0.4) ratio = 0.75;
+  double wd = W * ratio;
+  int nLevels = 1;
+  while (wd >= min_width) { nLevels++; wd *= ratio; }
+  vector<Level> P(nLevels);
+  P[0].W = W; P[0].H = H; P[0].d.assign(img, img + (size_t)W * H);
+  for (int m = 1; m < nLevels; m++) {
+    const Level& L = P[m - 1];
+    const int LW = L.W, LH = L.H;
+    auto at = [&](int y, int x) { return L.d[(size_t)y * LW + x]; };
+    double lr = std::pow(ratio, m);
+    int FW = W * lr, FH = H * lr;
+    vector<double> foo((size_t)FW * FH, 0.0);
+    const float factor = (float)LH / FH;
+    const float maxDZ = .005;
+    for (int i = 0; i < FH; i++)
+      for (int j = 0; j < FW; j++) {
+        const int x = std::max(0, std::min(LW - 1, (int)std::rint(factor * j)));
+        const int y = std::max(0, std::min(LH - 1, (int)std::rint(factor * i)));
+        double& o = foo[(size_t)i * FW + j];
+        if (at(y, x) > 0) {
+          int xs[2], ys[2];
+          xs[0] = std::max(0, std::min(LW - 1, (int)std::floor(factor * j)));
+          xs[1] = std::min(LW - 1, xs[0] + 1);
+          ys[0] = std::max(0, std::min(LH - 1, (int)std::floor(factor * i)));
+          ys[1] = std::min(LH - 1, ys[0] + 1);
+          const int xi = (x == xs[0]) ? 0 : 1, yi = (y == ys[0]) ? 0 : 1;
+          const double c = at(y, x);
+          vector<float> comp{(float)c};
+          if (std::fabs(at(ys[yi], xs[!xi]) - c) < maxDZ * stereo_error_ratio(c)) comp.push_back(at(ys[yi], xs[!xi]));
+          if (std::fabs(at(ys[!yi], xs[xi]) - c) < maxDZ * stereo_error_ratio(c)) comp.push_back(at(ys[!yi], xs[xi]));
+          if (std::fabs(at(ys[!yi], xs[!xi]) - c) < maxDZ * stereo_error_ratio(c)) comp.push_back(at(ys[!yi], xs[!xi]));
+          if (comp.size() > 1) {
+            float avg = 0;
+            for (float z : comp) avg += z;
+            o = avg / comp.size();
+          } else {
+            const double c2 = at(ys[!yi], xs[!xi]);
+            vector<float> other{(float)c2};
+            if (std::fabs(at(ys[yi], xs[!xi]) - c2) < maxDZ * stereo_error_ratio(c2)) other.push_back(at(ys[yi], xs[!xi]));
+            if (std::fabs(at(ys[!yi], xs[xi]) - c2) < maxDZ * stereo_error_ratio(c2)) other.push_back(at(ys[!yi], xs[xi]));
+            if (other.size() > 1) {
+              float avg = 0;
+              for (float z : other) avg += z;
+              o = avg / other.size();
+            } else {
+              o = c;
+            }
+          }
+        } else {
+          o = -1;
+        }
+      }
+    P[m].W = FW; P[m].H = FH; P[m].d.swap(foo);
+  }
+  return P;
+}
+
+/* OpticalFlow.cpp:148-220 im2feature(RGB_WITH_GRADIENT); Image.h:955-1016 dx/dy
+ * with the 5-tap [1,-8,0,8,-1]/12 filter; Image.h:1435-1468 desaturate. */
+void im2feature(const double* im, int W, int H, double* out) {
+  size_t N = (size_t)W * H;
+  vector<double> gray(N), gx(N), gy(N);
+  for (size_t i = 0; i < N; i++) {
+    size_t o = i * 3;
+    gray[i] = (double)im[o] * .114 + im[o + 1] * .587 + im[o + 2] * .299;
+  }
+  double f[5] = {1, -8, 0, 8, -1};
+  for (int i = 0; i < 5; i++) f[i] /= 12;
+  hfilter(gray.data(), gx.data(), W, H, 1, f, 2);
+  vfilter(gray.data(), gy.data(), W, H, 1, f, 2);
+  for (size_t i = 0; i < N; i++) {
+    out[i * 5 + 0] = gray[i];
+    out[i * 5 + 1] = gx[i];
+    out[i * 5 + 2] = gy[i];
+    out[i * 5 + 3] = im[i * 3 + 1] - im[i * 3];
+    out[i * 5 + 4] = im[i * 3 + 1] - im[i * 3 + 2];
+  }
+}
+
+/* opticalFlowUtils.cpp:76-99 warpBy3DFlow (bilinear branch) via
+ * ImageProcessing.h:471-491 warpImage: out-of-bounds -> frame-1 value. */
+void warp_cpu(const double* I1, const double* D1, const double* I2, const double* D2,
+              const double* u, const double* v, const double* w, int W, int H, int C,
+              double* wI, double* wD) {
+  std::fill(wI, wI + (size_t)W * H * C, 0.0);
+  std::fill(wD, wD + (size_t)W * H, 0.0);
+  for (int i = 0; i < H; i++)
+    for (int j = 0; j < W; j++) {
+      size_t l = (size_t)i * W + j;
+      double y = i + v[l], x = j + u[l];
+      if (x < 0 || x > W - 1 || y < 0 || y > H - 1) {
+        for (int k = 0; k < C; k++) wI[l * C + k] = I1[l * C + k];
+        wD[l] = D1[l];
+      } else {
+        bilerp_acc(I2, W, H, C, x, y, wI + l * C);
+        bilerp_acc(D2, W, H, 1, x, y, wD + l);
+      }
+    }
+  for (size_t l = 0; l < (size_t)W * H; l++) wD[l] -= w[l];
+}
+
+/* recursiveMedianFiltering.cpp:34-99 -- greedy fill in order of most known
+ * neighbours; the visiting order is std::unordered_set's (same container, same
+ * insert/erase sequence as the reference); values pass through a float buffer. */
+void median_fill(double* data, int W, int H, int hw) {
+  const int win = (2 * hw + 1) * (2 * hw + 1);
+  vector<std::unordered_set<uint32_t>> byCount(win);
+  vector<int32_t> countOf((size_t)W * H, -1);
+  uint32_t left = 0;
+  for (int i = 0, l = 0; i < H; i++)
+    for (int j = 0; j < W; j++, l++)
+      if (data[l] <= 0) {
+        uint32_t c = 0;
+        for (int ii = std::max(0, i - hw); ii <= std::min(H - 1, i + hw); ii++)
+          for (int jj = std::max(0, j - hw); jj <= std::min(W - 1, j + hw); jj++)
+            if (data[ii * W + jj] > 0) c++;
+        byCount[c].insert(l);
+        countOf[l] = c;
+        left++;
+      }
+  vector<float> vals(win);
+  while (left > 0) {
+    int mx = (int)byCount.size() - 1;
+    while (mx >= 0 && byCount[mx].empty()) mx--;
+    if (mx <= 0) return; /* reference asserts maxNumNbrs > 0 (an all-invalid region) */
+    const uint32_t l = *byCount[mx].begin();
+    const int i = l / W, j = l % W;
+    byCount[mx].erase(l);
+    countOf[l] = -1;
+    uint32_t idx = 0;
+    for (int ii = std::max(0, i - hw); ii <= std::min(H - 1, i + hw); ii++)
+      for (int jj = std::max(0, j - hw); jj <= std::min(W - 1, j + hw); jj++) {
+        const int ll = ii * W + jj;
+        if (data[ll] > 0) {
+          vals[idx++] = data[ll];
+        } else if (countOf[ll] >= 0) {
+          byCount[countOf[ll]].erase(ll);
+          countOf[ll]++;
+          byCount[countOf[ll]].insert(ll);
+        }
+      }
+    std::sort(vals.begin(), vals.begin() + idx);
+    data[l] = vals[(int)std::floor(idx / 2)];
+    left--;
+  }
+}
+
+/* staticDepthNoiseModeling.cpp:85-113: depth-boundary mask, then the exact L1
+ * distance to the nearest boundary pixel (what cv::distanceTransform(L1)
+ * returns), capped: distances above `cap` are reported as cap+1. */
+void l1_distance_to_boundary(const float* depth, int W, int H, int cap, float* dist) {
+  vector<uint8_t> boundary((size_t)W * H, 0);
+  for (int i = 0; i < H; i++)
+    for (int j = 0; j < W; j++) {
+      const float d = depth[(size_t)i * W + j];
+      bool b = false;
+      if (i < H - 1) {
+        const float dn = depth[(size_t)(i + 1) * W + j];
+        if (dn > 0 && std::fabs(d - dn) > .02 * stereo_error_ratio(std::min(d, dn))) b = true;
+      }
+      if (!b && j < W - 1) {
+        const float dn = depth[(size_t)i * W + j + 1];
+        if (dn > 0 && std::fabs(d - dn) > .02 * stereo_error_ratio(std::min(d, dn))) b = true;
+      }
+      boundary[(size_t)i * W + j] = b;
+    }
+  for (int i = 0; i < H; i++)
+    for (int j = 0; j < W; j++) {
+      int best = cap + 1;
+      for (int di = -cap; di <= cap; di++) {
+        const int ii = i + di;
+        if (ii < 0 || ii >= H) continue;
+        const int rem = cap - std::abs(di);
+        for (int dj = -rem; dj <= rem; dj++) {
+          const int jj = j + dj;
+          if (jj < 0 || jj >= W) continue;
+          if (boundary[(size_t)ii * W + jj]) best = std::min(best, std::abs(di) + std::abs(dj));
+        }
+      }
+      dist[(size_t)i * W + j] = (float)best;
+    }
+}
+
+/* staticDepthNoiseModeling.cpp:225-258 computeDepthMapStdevsPointwise with the
+ * sigma at 1 m passed as float (:227,232), then :115-124 inflation. */
+void depth_stdevs_pointwise(const float* depth, int W, int H, int max_dist, float* st) {
+  const float sigmaAt1m = stereo_error(1);
+  const float invalidSigma = 1;
+  for (size_t l = 0; l < (size_t)W * H; l++)
+    st[l] = depth[l] > 0 ? (float)std::max(1e-4, sigmaAt1m * stereo_error_ratio(depth[l])) : invalidSigma;
+  if (max_dist > 0) {
+    vector<float> dist((size_t)W * H);
+    l1_distance_to_boundary(depth, W, H, max_dist, dist.data());
+    for (size_t l = 0; l < (size_t)W * H; l++)
+      if (dist[l] <= max_dist) {
+        const float a = dist[l] / max_dist;
+        st[l] = std::exp((1 - a) * std::log((double)(10 * st[l])) + a * std::log((double)st[l]));
+      }
+  }
+}
+
+/* ==========================================================================
+ * The level solve.  CT is the storage type of the system and of du, WT the
+ * type of the device weight copies (float in the reference); FSQ selects the
+ * reference's float squares (sqrCU, .cu:44-47).
+ */
+template <class CT, bool REF>
+struct Solver {
+  typedef typename std::conditional<REF, float, double>::type WT;
+
+  static inline double sq(double x) {
+    if (REF) { float f = (float)x; return (double)(float)(f * f); }
+    return x * x;
+  }
+
+  int W, H;
+  size_t N;
+  static const int C = 5;
+  const orc_level_in* in;
+  const orc_params* p;
+
+  vector<double> Iw, Dw;                 /* warped frame 2 */
+  vector<double> u;                      /* [3][N] planes */
+  vector<CT> du;                         /* [3][N] planes */
+  vector<WT> dW, sW, mW, rW;             /* device weight copies */
+  vector<WT> cW;                         /* convergence weights f/D1 */
+  vector<double> Ix, Iy, It, Dx, Dy, Dt; /* derivatives */
+  vector<WT> q;
+  vector<double> phi, psi, pi;
+  vector<CT> coef;                       /* 13 planes */
+
+  Solver(const orc_level_in* in_, const orc_params* p_) : in(in_), p(p_) {
+    W = in->W; H = in->H; N = (size_t)W * H;
+    Iw.assign(in->warpI2, in->warpI2 + N * C);
+    Dw.assign(in->warpD2, in->warpD2 + N);
+    u.assign(3 * N, 0.0);
+    du.assign(3 * N, (CT)0);
+    dW.resize(N); sW.resize(N); mW.resize(N); rW.resize(2 * N); cW.resize(N);
+    for (size_t l = 0; l < N; l++) {
+      dW[l] = (WT)in->delta[l]; sW[l] = (WT)in->s[l]; mW[l] = (WT)in->m[l];
+      rW[2 * l] = (WT)in->r[2 * l]; rW[2 * l + 1] = (WT)in->r[2 * l + 1];
+      cW[l] = (WT)(in->focal_level / in->D1[l]);
+    }
+    Ix.resize(N * C); Iy.resize(N * C); It.resize(N * C);
+    Dx.resize(N); Dy.resize(N); Dt.resize(N); q.resize(N);
+    phi.resize(N); psi.resize(N * C); pi.resize(N);
+    coef.assign(13 * N, (CT)0);
+  }
+
+  /* .cu:84-147 imgDerivativesKernel / ...WithClamp */
+  void derivatives() {
+    static const double g[5] = {1, -8, 0, 8, -1};
+    const double* I1 = in->I1;
+    for (int i = 0; i < H; i++)
+      for (int j = 0; j < W; j++) {
+        const size_t l = (size_t)i * W + j;
+        for (int k = 0; k < C; k++) {
+          double sx = 0, sy = 0;
+          for (int c = j - 2, m = 0; c <= j + 2; c++, m++) {
+            const int cc = std::max(0, std::min(W - 1, c));
+            const size_t o = ((size_t)i * W + cc) * C + k;
+            sx += g[m] * (.4 * I1[o] + .6 * Iw[o]);
+          }
+          for (int r = i - 2, m = 0; r <= i + 2; r++, m++) {
+            const int rr = std::max(0, std::min(H - 1, r));
+            const size_t o = ((size_t)rr * W + j) * C + k;
+            sy += g[m] * (.4 * I1[o] + .6 * Iw[o]);
+          }
+          Ix[l * C + k] = sx / 12;
+          Iy[l * C + k] = sy / 12;
+          It[l * C + k] = Iw[l * C + k] - I1[l * C + k];
+        }
+        const double mg = .04;
+        double sx = 0, sy = 0;
+        for (int c = j - 2, m = 0; c <= j + 2; c++, m++) {
+          const int cc = std::max(0, std::min(W - 1, c));
+          sx += g[m] * (.4 * in->D1[(size_t)i * W + cc] + .6 * Dw[(size_t)i * W + cc]);
+        }
+        for (int r = i - 2, m = 0; r <= i + 2; r++, m++) {
+          const int rr = std::max(0, std::min(H - 1, r));
+          sy += g[m] * (.4 * in->D1[(size_t)rr * W + j] + .6 * Dw[(size_t)rr * W + j]);
+        }
+        Dx[l] = std::min(std::max(sx / 12, -mg), mg);
+        Dy[l] = std::min(std::max(sy / 12, -mg), mg);
+        Dt[l] = std::min(std::max(Dw[l] - in->D1[l], -mg), mg);
+      }
+  }
+
+  /* .cu:640-648 fillOcclusionPenaltyKernel (unwarped D2) */
+  void occlusion() {
+    for (size_t l = 0; l < N; l++) {
+      const double d1 = in->D1[l];
+      q[l] = (d1 + u[2 * N + l] < in->D2[l] + .02 * sq(d1)) ? (WT)1 : (WT).01;
+    }
+  }
+
+  /* .cu:156-182 forward differences of u+du for component c, pixel (i,j) */
+  inline void flow_grad(int c, int i, int j, double& gx, double& gy) const {
+    const size_t l = (size_t)i * W + j;
+    const double* uc = u.data() + c * N;
+    const CT* dc = du.data() + c * N;
+    gy = (i == H - 1) ? 0.0 : (uc[l + W] + (double)dc[l + W] - uc[l] - (double)dc[l]);
+    gx = (j == W - 1) ? 0.0 : (uc[l + 1] + (double)dc[l + 1] - uc[l] - (double)dc[l]);
+  }
+
+  /* .cu:340-371 penaltyDerivativesKernel */
+  void penalties() {
+    const double epsD = 1e-6, epsZ = 1e-10, epsS = 1e-6;
+    const double cw = p->color_data_weight;
+    for (int i = 0; i < H; i++)
+      for (int j = 0; j < W; j++) {
+        const size_t l = (size_t)i * W + j;
+        double ux, uy, vx, vy, wx, wy;
+        flow_grad(0, i, j, ux, uy);
+        flow_grad(1, i, j, vx, vy);
+        flow_grad(2, i, j, wx, wy);
+        double t;
+        if (REF) {
+          const float a = (float)sq(ux) + (float)sq(uy) + (float)sq(vx) + (float)sq(vy) +
+                          (float)sW[l] * ((float)sq(wx) + (float)sq(wy));
+          t = a;
+        } else {
+          t = sq(ux) + sq(uy) + sq(vx) + sq(vy) + (double)sW[l] * (sq(wx) + sq(wy));
+        }
+        phi[l] = 0.5 / std::sqrt(t + epsS);
+        const double d0 = (double)du[l], d1 = (double)du[N + l], d2 = (double)du[2 * N + l];
+        for (int k = 0; k < C; k++) {
+          const size_t o = l * C + k;
+          const double tt = (It[o] + Ix[o] * d0 + Iy[o] * d1) * (double)q[l];
+          psi[o] = .5 / std::sqrt(sq(tt) + epsD) * cw;
+        }
+        const double t2 = Dt[l] + Dx[l] * d0 + Dy[l] * d1 - d2;
+        pi[l] = .5 / std::sqrt(sq(t2) + epsZ) * (double)dW[l];
+      }
+  }
+
+  /* .cu:240-333 uv/w flow Laplacians (on u only, normalised by the mean
+   * regularisation weight of the existing edges) */
+  double laplacian(int c, int i, int j) const {
+    const size_t l = (size_t)i * W + j;
+    const double* uc = u.data() + c * N;
+    double lap = 0;
+    if (j < W - 1) lap -= (uc[l + 1] - uc[l]) * phi[l] * (double)rW[l * 2 + 0];
+    if (j > 0) lap += (uc[l] - uc[l - 1]) * phi[l - 1] * (double)rW[(l - 1) * 2 + 0];
+    if (i < H - 1) lap -= (uc[l + W] - uc[l]) * phi[l] * (double)rW[l * 2 + 1];
+    if (i > 0) lap += (uc[l] - uc[l - W]) * phi[l - W] * (double)rW[(l - W) * 2 + 1];
+    double norm = 0;
+    int count = 0;
+    if (j < W - 1) { norm += rW[l * 2 + 0]; count++; }
+    if (j > 0) { norm += rW[(l - 1) * 2 + 0]; count++; }
+    if (i < H - 1) { norm += rW[l * 2 + 1]; count++; }
+    if (i > 0) { norm += rW[(l - W) * 2 + 1]; count++; }
+    if (std::fabs(norm) < 1e-5) lap = 0;
+    else lap /= (norm / count);
+    return lap;
+  }
+
+  /* .cu:189-230 products, .cu:477-575 fillSystemMtxKernel, written into the
+   * compact form: own right/down edge weights (xy and z), the symmetric 3x3
+   * block and the rhs. */
+  void assemble() {
+    const double alpha = p->alpha, gamma = p->gamma;
+    CT* eRxy = coef.data() + 0 * N; CT* eRz = coef.data() + 1 * N;
+    CT* eDxy = coef.data() + 2 * N; CT* eDz = coef.data() + 3 * N;
+    CT* Buu = coef.data() + 4 * N; CT* Bvv = coef.data() + 5 * N; CT* Bww = coef.data() + 6 * N;
+    CT* Buv = coef.data() + 7 * N; CT* Buw = coef.data() + 8 * N; CT* Bvw = coef.data() + 9 * N;
+    CT* ru = coef.data() + 10 * N; CT* rv = coef.data() + 11 * N; CT* rw = coef.data() + 12 * N;
+    for (int i = 0; i < H; i++)
+      for (int j = 0; j < W; j++) {
+        const size_t l = (size_t)i * W + j;
+        /* products (multiply3CollapseKernel / ...Collapse2Kernel / multiply3Kernel) */
+        double sxy = 0, sxx = 0, syy = 0, stx = 0, sty = 0;
+        for (int k = 0; k < C; k++) {
+          const size_t o = l * C + k;
+          sxy += psi[o] * Ix[o] * Iy[o];
+          sxx += psi[o] * Ix[o] * Ix[o];
+          syy += psi[o] * Iy[o] * Iy[o];
+        }
+        for (int k = 0; k < C; k++) { const size_t o = l * C + k; stx += psi[o] * Ix[o] * It[o]; }
+        for (int k = 0; k < C; k++) { const size_t o = l * C + k; sty += psi[o] * Iy[o] * It[o]; }
+        const double ql = (double)q[l];
+        const double ixy = sxy / C, ix2 = sxx / C, iy2 = syy / C;
+        const double itx = stx * ql * ql / C, ity = sty * ql * ql / C;
+        const double P = pi[l];
+        const double dxy = P * Dx[l] * Dy[l], dx2 = P * Dx[l] * Dx[l], dy2 = P * Dy[l] * Dy[l];
+        const double dtx = P * Dx[l] * Dt[l], dty = P * Dy[l] * Dt[l];
+        const double uL = laplacian(0, i, j), vL = laplacian(1, i, j);
+        const double wL = (double)sW[l] * laplacian(2, i, j);
+
+        double wsXY = 0, wsZ = 0;
+        if (j > 0) { const double a = phi[l - 1] * (double)rW[(l - 1) * 2 + 0], b = a * (double)sW[l - 1]; wsXY += a; wsZ += b; }
+        if (i > 0) { const double a = phi[l - W] * (double)rW[(l - W) * 2 + 1], b = a * (double)sW[l - W]; wsXY += a; wsZ += b; }
+        if (j < W - 1) {
+          const double a = phi[l] * (double)rW[l * 2 + 0], b = a * (double)sW[l];
+          eRxy[l] = (CT)(alpha * a); eRz[l] = (CT)(alpha * b); wsXY += a; wsZ += b;
+        }
+        if (i < H - 1) {
+          const double a = phi[l] * (double)rW[l * 2 + 1], b = a * (double)sW[l];
+          eDxy[l] = (CT)(alpha * a); eDz[l] = (CT)(alpha * b); wsXY += a; wsZ += b;
+        }
+        const double u0 = u[l], u1 = u[N + l], u2 = u[2 * N + l];
+        CT t;
+        t = (CT)(-(ix2 + dx2 + alpha * wsXY)); Buu[l] = (CT)((double)t + -(gamma));
+        Buv[l] = (CT)(-ixy - dxy);
+        Buw[l] = (CT)(P * Dx[l]);
+        t = (CT)(-(-itx - dtx + alpha * -uL)); ru[l] = (CT)((double)t + -(-gamma * u0));
+        t = (CT)(-(iy2 + dy2 + alpha * wsXY)); Bvv[l] = (CT)((double)t + -(gamma));
+        Bvw[l] = (CT)(P * Dy[l]);
+        t = (CT)(-(-ity - dty + alpha * -vL)); rv[l] = (CT)((double)t + -(-gamma * u1));
+        t = (CT)(-(P + alpha * wsZ)); Bww[l] = (CT)((double)t + -(gamma * (double)mW[l]));
+        t = (CT)(-(P * Dt[l] + alpha * -wL)); rw[l] = (CT)((double)t + -(-gamma * (double)mW[l] * u2));
+      }
+  }
+
+  void build_system() {
+    derivatives();
+    occlusion();
+    penalties();
+    assemble();
+  }
+
+  /* .cu:382-470: one colour of the red-black relaxation; returns the largest
+   * |change| over the updated pixels (the check of .cu:914-918 splits exactly
+   * into the two colours' changes). */
+  static CT relax_colour(const CT* coef, int W, int H, int colour, CT* x) {
+    const size_t N = (size_t)W * H;
+    const CT* eRxy = coef + 0 * N; const CT* eRz = coef + 1 * N;
+    const CT* eDxy = coef + 2 * N; const CT* eDz = coef + 3 * N;
+    const CT* Buu = coef + 4 * N; const CT* Bvv = coef + 5 * N; const CT* Bww = coef + 6 * N;
+    const CT* Buv = coef + 7 * N; const CT* Buw = coef + 8 * N; const CT* Bvw = coef + 9 * N;
+    const CT* rhs[3] = {coef + 10 * N, coef + 11 * N, coef + 12 * N};
+    const CT omega = (CT).7;
+    const CT step = (CT).3;
+    CT maxd = 0;
+    for (int i = 0; i < H; i++)
+      for (int j = (i + colour) & 1; j < W; j += 2) {
+        const size_t l = (size_t)i * W + j;
+        CT xo[3] = {x[l], x[N + l], x[2 * N + l]};
+        CT xn[3];
+        for (int k = 0; k < 3; k++) {
+          const CT* xk = x + k * N;
+          CT nv = 0;
+          if (j > 0) nv -= (k < 2 ? eRxy[l - 1] : eRz[l - 1]) * xk[l - 1];
+          if (i > 0) nv -= (k < 2 ? eDxy[l - W] : eDz[l - W]) * xk[l - W];
+          if (j < W - 1) nv -= (k < 2 ? eRxy[l] : eRz[l]) * xk[l + 1];
+          if (i < H - 1) nv -= (k < 2 ? eDxy[l] : eDz[l]) * xk[l + W];
+          nv += rhs[k][l];
+          if (k == 0) { nv -= Buv[l] * xo[1]; nv -= Buw[l] * xo[2]; nv /= Buu[l]; }
+          else if (k == 1) { nv -= Buv[l] * xo[0]; nv -= Bvw[l] * xo[2]; nv /= Bvv[l]; }
+          else { nv -= Buw[l] * xo[0]; nv -= Bvw[l] * xo[1]; nv /= Bww[l]; }
+          CT r = omega * nv + (1 - omega) * xo[k];
+          if (r < xo[k] - step) r = xo[k] - step;
+          else if (r > xo[k] + step) r = xo[k] + step;
+          xn[k] = r;
+        }
+        for (int k = 0; k < 3; k++) {
+          x[k * N + l] = xn[k];
+          const CT d = std::fabs(xn[k] - xo[k]);
+          if (d > maxd) maxd = d;
+        }
+      }
+    return maxd;
+  }
+
+  /* .cu:579-635 GPU warps: floor base, explicit clamps, f32 weights (f64 in the FP64 oracle) */
+  void warp_gpu() {
+    typedef typename std::conditional<REF, float, double>::type AT;
+    const double* I1 = in->I1; const double* I2 = in->I2;
+    for (int i = 0; i < H; i++)
+      for (int j = 0; j < W; j++) {
+        const size_t l = (size_t)i * W + j;
+        const double x = j + u[l], y = i + u[N + l];
+        if (x < 0 || x > W - 1 || y < 0 || y > H - 1) {
+          for (int k = 0; k < C; k++) Iw[l * C + k] = I1[l * C + k];
+          Dw[l] = in->D1[l] - u[2 * N + l];
+        } else {
+          const int x0 = std::max(0, std::min(W - 1, (int)std::floor(x)));
+          const int y0 = std::max(0, std::min(H - 1, (int)std::floor(y)));
+          const int x1 = std::min(W - 1, x0 + 1), y1 = std::min(H - 1, y0 + 1);
+          const AT ax = x - x0, ay = y - y0;
+          const AT w00 = (1 - ax) * (1 - ay), w01 = (1 - ax) * ay, w10 = ax * (1 - ay), w11 = ax * ay;
+          for (int k = 0; k < C; k++)
+            Iw[l * C + k] = w00 * I2[((size_t)y0 * W + x0) * C + k] + w01 * I2[((size_t)y1 * W + x0) * C + k] +
+                            w10 * I2[((size_t)y0 * W + x1) * C + k] + w11 * I2[((size_t)y1 * W + x1) * C + k];
+          const double* D2 = in->D2;
+          Dw[l] = w00 * D2[(size_t)y0 * W + x0] + w01 * D2[(size_t)y1 * W + x0] + w10 * D2[(size_t)y0 * W + x1] +
+                  w11 * D2[(size_t)y1 * W + x1] - u[2 * N + l];
+        }
+      }
+  }
+
+  /* .cu:755-954: the outer/inner loops with their exits */
+  void run(int allow_early, orc_level_stats* st) {
+    const double convThreshold = 4e-4;
+    const int checkEvery = 25;
+    int outer = 0, sweeps = 0;
+    double lastStat = -1;
+    float lastMax = -1;
+    for (int count = 0; count < p->n_outer; count++) {
+      outer++;
+      derivatives();
+      std::fill(du.begin(), du.end(), (CT)0);
+      occlusion();
+      for (int hh = 0; hh < p->n_inner; hh++) {
+        penalties();
+        assemble();
+        for (int it = 0; it < p->n_sor; it++) {
+          const CT m0 = relax_colour(coef.data(), W, H, 0, du.data());
+          const CT m1 = relax_colour(coef.data(), W, H, 1, du.data());
+          sweeps++;
+          if (it % checkEvery == checkEvery - 1) {
+            const CT md = std::max(m0, m1);
+            lastMax = (float)md;
+            if (!p->disable_inner_exit) {
+              if (md < 1e-5) break;
+              if (md >= (CT).3) break;
+            }
+          }
+        }
+      }
+      for (size_t l = 0; l < 3 * N; l++) u[l] += (double)du[l];
+      warp_gpu();
+      if (allow_early) {
+        double acc = 0;
+        for (int c = 0; c < 3; c++)
+          for (size_t l = 0; l < N; l++) {
+            const CT d = du[c * N + l];
+            const CT wgt = (c < 2) ? (CT)1 : (CT)cW[l];
+            acc += (double)(CT)((CT)(d * d) * wgt);
+          }
+        const double stat = acc / (double)(W * H * 3);
+        lastStat = stat;
+        if (!p->disable_outer_exit && stat < convThreshold) break;
+      }
+    }
+    if (st) { st->outer_exec = outer; st->sweeps_exec = sweeps; st->last_stat = lastStat; st->last_maxdiff = lastMax; }
+  }
+};
+
+template <class CT, bool REF>
+void level_solve_t(const orc_level_in* in, const orc_params* p, int allow_early, double* u, double* v,
+                   double* w, orc_level_stats* st) {
+  Solver<CT, REF> S(in, p);
+  const size_t N = S.N;
+  for (size_t l = 0; l < N; l++) { S.u[l] = u[l]; S.u[N + l] = v[l]; S.u[2 * N + l] = w[l]; }
+  S.run(allow_early, st);
+  for (size_t l = 0; l < N; l++) { u[l] = S.u[l]; v[l] = S.u[N + l]; w[l] = S.u[2 * N + l]; }
+}
+
+/* sceneFlowMultiscale.cpp:16-25 getDownsizedCamParams */
+orc_camera downsized(orc_camera c, int width) {
+  const double f = (double)width / c.xRes;
+  c.xRes = std::rint(c.xRes * f);
+  c.yRes = std::rint(c.yRes * f);
+  c.centerX = std::rint(c.centerX * f);
+  c.centerY = std::rint(c.centerY * f);
+  c.focalLength *= f;
+  return c;
+}
+
+/* OpticalFlow.cpp:104-111 with the full-size parameters of
+ * sceneFlowMultiscale.cpp:144: the weight of level pixel l is computed from
+ * D1_level[(l / W_full) * W_level + l % W_full] (only the first N_level
+ * entries reach the solver, .cu:686). */
+void depth_mag(const double* D, int Wl, int Hl, int Wf, int Hf, float f, double* out) {
+  (void)Hf;
+  const size_t Nl = (size_t)Wl * Hl;
+  for (size_t l = 0; l < Nl; l++) {
+    const size_t r = l / (size_t)Wf, c = l % (size_t)Wf;
+    const size_t idx = r * (size_t)Wl + c;
+    const double d = idx < Nl ? D[idx] : 0.0; /* the reference reads past the level buffer here */
+    out[l] = d > 0 ? dsq(f / d) : dsq(f / 2);
+  }
+}
+
+/* OpticalFlow.cpp:113-141 */
+void reg_weights(const float* hpb, const double* depth, int W, int H, int type, double* out) {
+  std::fill(out, out + (size_t)W * H * 2, 0.0);
+  for (int i = 0, l = 0; i < H; i++)
+    for (int j = 0; j < W; j++, l++) {
+      if (j < W - 1) {
+        double v = 1;
+        if (type == ORC_REG_LETOUZEY) v = std::exp(-dsq((depth[l] - depth[l + 1]) / .02));
+        else if (type == ORC_REG_HPB) v = 1 - hpb[((size_t)i * W + j) * 2 + 0] * .99;
+        out[(size_t)l * 2 + 0] = v;
+      }
+      if (i < H - 1) {
+        double v = 1;
+        if (type == ORC_REG_LETOUZEY) v = std::exp(-dsq((depth[l] - depth[l + W]) / .02));
+        else if (type == ORC_REG_HPB) v = 1 - hpb[((size_t)i * W + j) * 2 + 1] * .99;
+        out[(size_t)l * 2 + 1] = v;
+      }
+    }
+}
+
+int level_solve_dispatch(const orc_level_in* in, const orc_params* p, int allow_early, double* u, double* v,
+                         double* w, orc_level_stats* st) {
+  if (p->precision == ORC_PREC_REF) level_solve_t<float, true>(in, p, allow_early, u, v, w, st);
+  else level_solve_t<double, false>(in, p, allow_early, u, v, w, st);
+  return 0;
+}
+
+}  // namespace
+
+/* ========================================================================== */
+extern "C" {
+
+int orc_pyramid_levels(int W, int H, double ratio, int min_width, int* widths, int* heights) {
+  vector<double> one((size_t)W * H, 0.0);
+  /* sizes only: run the size recursion of GaussianPyramid.cpp:23-60 */
+  if (ratio > 0.98 || ratio < 0.4) ratio = 0.75;
+  double w = W * ratio;
+  int n = 1;
+  while (w >= min_width) { n++; w *= ratio; }
+  int nn = std::log(0.25) / std::log(ratio);
+  vector<int> ws(n), hs(n);
+  ws[0] = W; hs[0] = H;
+  for (int i = 1; i < n; i++) {
+    if (i <= nn) { double r = std::pow(ratio, i); ws[i] = (double)W * r; hs[i] = (double)H * r; }
+    else { double rate = (double)std::pow(ratio, i) * W / ws[i - nn]; ws[i] = (double)ws[i - nn] * rate; hs[i] = (double)hs[i - nn] * rate; }
+  }
+  for (int i = 0; i < n && i < 16; i++) { if (widths) widths[i] = ws[i]; if (heights) heights[i] = hs[i]; }
+  return n;
+}
+
+void orc_gaussian_smooth(const double* img, int W, int H, int C, double sigma, int fsize, double* out) {
+  gaussian_smooth(img, W, H, C, sigma, fsize, out);
+}
+void orc_resize_ratio(const double* src, int W, int H, int C, double ratio, double* dst) {
+  resize_ratio(src, W, H, C, ratio, dst);
+}
+void orc_resize_dims(const double* src, int Ws, int Hs, int C, int Wd, int Hd, double* dst) {
+  resize_dims(src, Ws, Hs, C, Wd, Hd, dst);
+}
+void orc_gaussian_pyramid(const double* img, int W, int H, int C, double ratio, int min_width, double* out) {
+  vector<Level> P = gaussian_pyramid(img, W, H, C, ratio, min_width);
+  size_t off = 0;
+  for (auto& L : P) { std::memcpy(out + off, L.d.data(), L.d.size() * sizeof(double)); off += L.d.size(); }
+}
+void orc_depth_map_pyramid(const double* depth, int W, int H, double ratio, int min_width, double* out) {
+  vector<Level> P = depth_map_pyramid(depth, W, H, ratio, min_width);
+  size_t off = 0;
+  for (auto& L : P) { std::memcpy(out + off, L.d.data(), L.d.size() * sizeof(double)); off += L.d.size(); }
+}
+void orc_im2feature(const double* bgr, int W, int H, double* out5) { im2feature(bgr, W, H, out5); }
+void orc_warp_cpu(const double* I1, const double* D1, const double* I2, const double* D2, const double* u,
+                  const double* v, const double* w, int W, int H, int C, double* wI, double* wD) {
+  warp_cpu(I1, D1, I2, D2, u, v, w, W, H, C, wI, wD);
+}
+void orc_depth_stdevs_pointwise(const float* depth, int W, int H, int max_dist, float* st) {
+  depth_stdevs_pointwise(depth, W, H, max_dist, st);
+}
+void orc_l1_distance_to_boundary(const float* depth, int W, int H, int cap, float* dist) {
+  l1_distance_to_boundary(depth, W, H, cap, dist);
+}
+/* OpticalFlow.cpp:71-81 */
+void orc_depth_data_weights(const float* sigma, int W, int H, double dvc, double* out) {
+  for (size_t l = 0; l < (size_t)W * H; l++) out[l] = stereo_error(1) / sigma[l] * dvc;
+}
+void orc_depth_mag_weights(const double* D, int Wl, int Hl, int Wf, int Hf, float f, double* out) {
+  depth_mag(D, Wl, Hl, Wf, Hf, f, out);
+}
+void orc_reg_weights(const float* hpb, const double* depth, int W, int H, int type, double* out) {
+  reg_weights(hpb, depth, W, H, type, out);
+}
+void orc_median_fill(double* depth, int W, int H, int hw) { median_fill(depth, W, H, hw); }
+
+int orc_level_solve(const orc_level_in* in, const orc_params* p, int allow_early, double* u, double* v, double* w,
+                    orc_level_stats* st) {
+  return level_solve_dispatch(in, p, allow_early, u, v, w, st);
+}
+
+void orc_level_system(const orc_level_in* in, const orc_params* p, const double* u, const double* v,
+                      const double* w, double* coef13) {
+  const size_t N = (size_t)in->W * in->H;
+  if (p->precision == ORC_PREC_REF) {
+    Solver<float, true> S(in, p);
+    for (size_t l = 0; l < N; l++) { S.u[l] = u[l]; S.u[N + l] = v[l]; S.u[2 * N + l] = w[l]; }
+    S.build_system();
+    for (size_t k = 0; k < 13 * N; k++) coef13[k] = S.coef[k];
+  } else {
+    Solver<double, false> S(in, p);
+    for (size_t l = 0; l < N; l++) { S.u[l] = u[l]; S.u[N + l] = v[l]; S.u[2 * N + l] = w[l]; }
+    S.build_system();
+    for (size_t k = 0; k < 13 * N; k++) coef13[k] = S.coef[k];
+  }
+}
+
+void orc_relax(const double* coef13, int W, int H, int precision, int n_iter, double* x) {
+  const size_t N = (size_t)W * H;
+  if (precision == ORC_PREC_REF) {
+    vector<float> c(coef13, coef13 + 13 * N), xx(x, x + 3 * N);
+    for (int it = 0; it < n_iter; it++) {
+      Solver<float, true>::relax_colour(c.data(), W, H, 0, xx.data());
+      Solver<float, true>::relax_colour(c.data(), W, H, 1, xx.data());
+    }
+    for (size_t k = 0; k < 3 * N; k++) x[k] = xx[k];
+  } else {
+    vector<double> c(coef13, coef13 + 13 * N), xx(x, x + 3 * N);
+    for (int it = 0; it < n_iter; it++) {
+      Solver<double, false>::relax_colour(c.data(), W, H, 0, xx.data());
+      Solver<double, false>::relax_colour(c.data(), W, H, 1, xx.data());
+    }
+    for (size_t k = 0; k < 3 * N; k++) x[k] = xx[k];
+  }
+}
+
+/* sceneFlowMultiscale.cpp:30-201 */
+int orc_multiscale(const double* im1, const double* dm1, const double* im2, const double* dm2, const float* sigma1,
+                   const float* hpb, int W, int H, const orc_params* p, double* uo, double* vo, double* wo,
+                   orc_stats* st) {
+  if (p->regularization_type == ORC_REG_HPB && !hpb) return -1;
+  const double ratio = p->pyramid_ratio;
+  const size_t N = (size_t)W * H;
+  vector<double> d1(dm1, dm1 + N), d2(dm2, dm2 + N);
+  median_fill(d1.data(), W, H, 1);
+  median_fill(d2.data(), W, H, 1);
+  vector<double> delta(N), sm(N);
+  orc_depth_data_weights(sigma1, W, H, p->depth_vs_color_data_weight, delta.data());
+  for (size_t l = 0; l < N; l++) sm[l] = dsq(p->cam.focalLength);
+  auto P1 = gaussian_pyramid(im1, W, H, 3, ratio, p->min_width);
+  auto P2 = gaussian_pyramid(im2, W, H, 3, ratio, p->min_width);
+  auto PD1 = gaussian_pyramid(d1.data(), W, H, 1, ratio, p->min_width);
+  auto PD2 = gaussian_pyramid(d2.data(), W, H, 1, ratio, p->min_width);
+  auto Pdelta = gaussian_pyramid(delta.data(), W, H, 1, ratio, p->min_width);
+  auto Ps = gaussian_pyramid(sm.data(), W, H, 1, ratio, p->min_width);
+  const int L = (int)P1.size();
+  vector<size_t> hpbOff(L, 0);
+  for (int k = 1; k < L; k++) hpbOff[k] = hpbOff[k - 1] + (size_t)P1[k - 1].W * P1[k - 1].H * 2;
+
+  vector<double> u, v, w;
+  if (st) { std::memset(st, 0, sizeof(*st)); st->n_levels = L; }
+  for (int k = L - 1; k >= 0; k--) {
+    const int Wk = P1[k].W, Hk = P1[k].H;
+    const size_t Nk = (size_t)Wk * Hk;
+    vector<double> I1(Nk * 5), I2(Nk * 5), Iw(Nk * 5), Dw(Nk);
+    im2feature(P1[k].d.data(), Wk, Hk, I1.data());
+    im2feature(P2[k].d.data(), Wk, Hk, I2.data());
+    const vector<double>& D1 = PD1[k].d;
+    const vector<double>& D2 = PD2[k].d;
+    const orc_camera camk = downsized(p->cam, Wk);
+    if (k == L - 1) {
+      u.assign(Nk, 0.0); v.assign(Nk, 0.0); w.assign(Nk, 0.0);
+      Iw = I2; Dw = D2;
+    } else {
+      const int Wp = P1[k + 1].W, Hp = P1[k + 1].H;
+      vector<double> nu(Nk), nv(Nk), nw(Nk);
+      resize_dims(u.data(), Wp, Hp, 1, Wk, Hk, nu.data());
+      resize_dims(v.data(), Wp, Hp, 1, Wk, Hk, nv.data());
+      resize_dims(w.data(), Wp, Hp, 1, Wk, Hk, nw.data());
+      const double s = 1 / ratio;
+      for (size_t l = 0; l < Nk; l++) { nu[l] *= s; nv[l] *= s; }
+      u.swap(nu); v.swap(nv); w.swap(nw);
+      warp_cpu(I1.data(), D1.data(), I2.data(), D2.data(), u.data(), v.data(), w.data(), Wk, Hk, 5, Iw.data(), Dw.data());
+    }
+    vector<double> mag(Nk), reg(Nk * 2);
+    depth_mag(D1.data(), Wk, Hk, (int)p->cam.xRes, (int)p->cam.yRes, p->cam.focalLength, mag.data());
+    reg_weights(hpb ? hpb + hpbOff[k] : nullptr, D1.data(), Wk, Hk, p->regularization_type, reg.data());
+    orc_level_in in;
+    in.W = Wk; in.H = Hk; in.focal_level = camk.focalLength;
+    in.I1 = I1.data(); in.I2 = I2.data(); in.warpI2 = Iw.data();
+    in.D1 = D1.data(); in.D2 = D2.data(); in.warpD2 = Dw.data();
+    in.delta = Pdelta[k].d.data(); in.s = Ps[k].d.data(); in.m = mag.data(); in.r = reg.data();
+    orc_level_stats ls;
+    level_solve_dispatch(&in, p, k < L - 1, u.data(), v.data(), w.data(), &ls);
+    if (st) {
+      st->width[k] = Wk; st->height[k] = Hk;
+      st->outer_exec[k] = ls.outer_exec; st->sweeps_exec[k] = ls.sweeps_exec; st->last_stat[k] = ls.last_stat;
+    }
+  }
+  std::memcpy(uo, u.data(), N * sizeof(double));
+  std::memcpy(vo, v.data(), N * sizeof(double));
+  std::memcpy(wo, w.data(), N * sizeof(double));
+  return 0;
+}
+
+/* sceneFlowWithInit.cpp:7-81 (full resolution only, no early exit) */
+int orc_with_init(const double* im1, const double* dm1, const double* im2, const double* dm2, const float* sigma1,
+                  const float* hpb0, const double* est, int W, int H, const orc_params* p, double* uo, double* vo,
+                  double* wo, orc_stats* st) {
+  if (p->regularization_type == ORC_REG_HPB && !hpb0) return -1;
+  const size_t N = (size_t)W * H;
+  vector<double> u(N), v(N), w(N);
+  for (size_t l = 0; l < N; l++) { u[l] = est[l * 3]; v[l] = est[l * 3 + 1]; w[l] = est[l * 3 + 2]; }
+  vector<double> d1(dm1, dm1 + N), d2(dm2, dm2 + N);
+  median_fill(d1.data(), W, H, 1);
+  median_fill(d2.data(), W, H, 1);
+  vector<double> delta(N), sm(N);
+  orc_depth_data_weights(sigma1, W, H, p->depth_vs_color_data_weight, delta.data());
+  for (size_t l = 0; l < N; l++) sm[l] = dsq(p->cam.focalLength);
+  vector<double> I1(N * 5), I2(N * 5), Iw(N * 5), Dw(N);
+  im2feature(im1, W, H, I1.data());
+  im2feature(im2, W, H, I2.data());
+  warp_cpu(I1.data(), d1.data(), I2.data(), d2.data(), u.data(), v.data(), w.data(), W, H, 5, Iw.data(), Dw.data());
+  vector<double> reg(N * 2), mag(N);
+  reg_weights(hpb0, d1.data(), W, H, p->regularization_type, reg.data());
+  depth_mag(d1.data(), W, H, (int)p->cam.xRes, (int)p->cam.yRes, p->cam.focalLength, mag.data());
+  orc_level_in in;
+  in.W = W; in.H = H; in.focal_level = p->cam.focalLength;
+  in.I1 = I1.data(); in.I2 = I2.data(); in.warpI2 = Iw.data();
+  in.D1 = d1.data(); in.D2 = d2.data(); in.warpD2 = Dw.data();
+  in.delta = delta.data(); in.s = sm.data(); in.m = mag.data(); in.r = reg.data();
+  orc_level_stats ls;
+  level_solve_dispatch(&in, p, 0, u.data(), v.data(), w.data(), &ls);
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->n_levels = 1; st->width[0] = W; st->height[0] = H;
+    st->outer_exec[0] = ls.outer_exec; st->sweeps_exec[0] = ls.sweeps_exec; st->last_stat[0] = ls.last_stat;
+  }
+  std::memcpy(uo, u.data(), N * sizeof(double));
+  std::memcpy(vo, v.data(), N * sizeof(double));
+  std::memcpy(wo, w.data(), N * sizeof(double));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+/*
+ * oracle.h -- C ABI of the CPU restatement of rgbd-flow's scene-flow path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, and only as the checker.
+ *
+ * Layout conventions follow the reference's DImage: row-major, channels
+ * interleaved per pixel ([N][C]).  Every function cites the reference
+ * file:line whose behaviour it restates.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision modes of the restatement. */
+enum {
+  ORC_PREC_REF = 0, /* the reference's storage: f64 images/weights-on-host, f32 A/du/weights-on-device, f32 sqr quirk */
+  ORC_PREC_F64 = 1  /* every buffer and product in f64 (the FP64 oracle) */
+};
+
+enum { ORC_REG_ISOTROPIC = 0, ORC_REG_LETOUZEY = 1, ORC_REG_HPB = 2 };
+
+typedef struct {
+  unsigned int xRes, yRes;
+  float centerX, centerY, focalLength, stereoBaseline, fovX, fovY;
+} orc_camera;
+
+typedef struct {
+  int regularization_type;
+  double color_data_weight, depth_vs_color_data_weight, alpha, gamma;
+  double pyramid_ratio;
+  int min_width, n_outer, n_inner, n_sor;
+  orc_camera cam;
+  int precision;      /* ORC_PREC_* */
+  int disable_inner_exit; /* 1: never leave the relaxation loop early (parity runs) */
+  int disable_outer_exit; /* 1: never leave the outer loop early */
+} orc_params;
+
+typedef struct {
+  int n_levels;
+  int width[16], height[16];
+  int outer_exec[16];      /* outer iterations executed per level */
+  int sweeps_exec[16];     /* red-black iterations executed per level (summed over outers) */
+  double last_stat[16];    /* last outer convergence statistic per level */
+} orc_stats;
+
+/* ---- pyramids (src/GaussianPyramid.cpp:20-63) ---- */
+int orc_pyramid_levels(int W, int H, double ratio, int min_width, int* widths, int* heights);
+void orc_gaussian_smooth(const double* img, int W, int H, int C, double sigma, int fsize, double* out);
+void orc_resize_ratio(const double* src, int W, int H, int C, double ratio, double* dst);
+void orc_resize_dims(const double* src, int Ws, int Hs, int C, int Wd, int Hd, double* dst);
+/* out: all levels concatenated, level 0 first */
+void orc_gaussian_pyramid(const double* img, int W, int H, int C, double ratio, int min_width, double* out);
+/* src/depthMapPyramid.cpp:15-140; out: all levels concatenated */
+void orc_depth_map_pyramid(const double* depth, int W, int H, double ratio, int min_width, double* out);
+
+/* ---- features, warps, weights ---- */
+void orc_im2feature(const double* bgr, int W, int H, double* out5);
+void orc_warp_cpu(const double* I1, const double* D1, const double* I2, const double* D2,
+                  const double* u, const double* v, const double* w, int W, int H, int C,
+                  double* warpI2, double* warpD2);
+void orc_depth_stdevs_pointwise(const float* depth, int W, int H, int max_dist, float* stdevs);
+void orc_l1_distance_to_boundary(const float* depth, int W, int H, int cap, float* dist);
+void orc_depth_data_weights(const float* sigma, int W, int H, double depth_vs_color, double* out);
+void orc_depth_mag_weights(const double* depth_level, int Wl, int Hl, int Wfull, int Hfull,
+                           float f_full, double* out);
+void orc_reg_weights(const float* hpb, const double* depth, int W, int H, int type, double* out);
+void orc_median_fill(double* depth, int W, int H, int halfwidth);
+
+/* ---- one level: sceneFlowSingleScale (src/sceneFlowSingleScaleJacobiCUDA.cu:660-969) ---- */
+typedef struct {
+  int W, H;
+  float focal_level;                 /* camParams.focalLength of the level */
+  const double *I1, *I2, *warpI2;    /* [N][5] */
+  const double *D1, *D2, *warpD2;    /* [N] */
+  const double *delta, *s, *m;       /* [N] depth-data, depth-smoothness, depth-magnitude weights */
+  const double *r;                   /* [N][2] regularisation weights (right, down) */
+} orc_level_in;
+
+typedef struct {
+  int outer_exec, sweeps_exec;
+  double last_stat;
+  float last_maxdiff;
+} orc_level_stats;
+
+int orc_level_solve(const orc_level_in* in, const orc_params* p, int allow_early,
+                    double* u, double* v, double* w, orc_level_stats* st);
+
+/* The system of one outer iteration, in the compact per-pixel form
+ * (eRxy, eRz, eDxy, eDz, Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw) as 13 planes
+ * of N values, computed from (u, du=0, warpI2, warpD2).  Precision REF -> the
+ * values are f32 stored in doubles. */
+void orc_level_system(const orc_level_in* in, const orc_params* p, const double* u,
+                      const double* v, const double* w, double* coef13);
+/* nIter red-black iterations on a compact system; x = [3][N] planes (in/out). */
+void orc_relax(const double* coef13, int W, int H, int precision, int n_iter, double* x);
+
+/* ---- whole pair: OpticalFlow::sceneFlowMultiscale (src/sceneFlowMultiscale.cpp:30-201) ----
+ * im*: [N][3] BGR in [0,1]; dm*: [N] metres; sigma1: [N] depth stdevs of frame 1;
+ * hpb: per level [H_k][W_k][2] concatenated (level 0 first), may be NULL unless HPB. */
+int orc_multiscale(const double* im1, const double* dm1, const double* im2, const double* dm2,
+                   const float* sigma1, const float* hpb, int W, int H, const orc_params* p,
+                   double* u, double* v, double* w, orc_stats* st);
+
+/* OpticalFlow::sceneFlowWithInit (src/sceneFlowWithInit.cpp:7-81); est: [N][3] */
+int orc_with_init(const double* im1, const double* dm1, const double* im2, const double* dm2,
+                  const float* sigma1, const float* hpb0, const double* est, int W, int H,
+                  const orc_params* p, double* u, double* v, double* w, orc_stats* st);
+
+#ifdef __cplusplus
+}
+#endif
