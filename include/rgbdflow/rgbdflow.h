@@ -1,0 +1,191 @@
+/*
+ * rgbdflow.h -- C ABI of librgbdflow, the MI355X (gfx950) scene-flow solver.
+ *
+ * The drop-in C++ layer (include/rgbdflow/OpticalFlow.h) calls HIP only
+ * through these functions.  Plain pointers and sizes; no exceptions cross the
+ * ABI; every function returns RGBDFLOW_OK (0) or a negative status, with a
+ * message available from rgbdflow_last_error().
+ *
+ * Layouts follow the reference's DImage: row-major, channels interleaved per
+ * pixel ([N][C], N = W*H).  Colour is BGR in [0,1] (ceLiuOpticalFlow.cpp:19-34),
+ * depth in metres with <= 0 invalid.
+ *
+ * Threading: one context per host thread (calls on one context are not
+ * re-entrant).  All device work of a context runs on its own HIP stream.
+ */
+#ifndef RGBDFLOW_H
+#define RGBDFLOW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RGBDFLOW_OK 0
+#define RGBDFLOW_ERR_ARG (-1)        /* invalid argument (null pointer, bad size, bad enum) */
+#define RGBDFLOW_ERR_HIP (-2)        /* a HIP runtime call failed */
+#define RGBDFLOW_ERR_BUDGET (-3)     /* the request exceeds the context's HBM budget */
+#define RGBDFLOW_ERR_STATE (-4)      /* call order violated (e.g. run before load) */
+#define RGBDFLOW_ERR_UNSUPPORTED (-5)
+
+/* OpticalFlow::sceneFlowRegularizationType (include/celiu_flow/OpticalFlow.h:99-104) */
+#define RGBDFLOW_REG_ISOTROPIC 0
+#define RGBDFLOW_REG_LETOUZEY 1
+#define RGBDFLOW_REG_HPB 2
+
+/* OpticalFlow::sceneFlowVariant (OpticalFlow.h:93-98); only FULL is solved
+ * (sceneFlowMultiscale.cpp:169-185). */
+#define RGBDFLOW_VARIANT_NONROBUST 0
+#define RGBDFLOW_VARIANT_LINEARIZED 1
+#define RGBDFLOW_VARIANT_FULL 2
+
+/* Relaxation kernels (same iteration, different schedules). */
+#define RGBDFLOW_RELAX_PLAIN 0   /* one launch per colour per iteration */
+#define RGBDFLOW_RELAX_TILED 1   /* several iterations per launch in LDS tiles */
+
+/* rgbd::CameraParams (include/rgbd_util/CameraParams.h:17-28) */
+typedef struct rgbdflow_camera {
+  unsigned int xRes, yRes;
+  float centerX, centerY, focalLength;
+  float stereoBaseline;
+  float fovX, fovY;
+} rgbdflow_camera;
+
+/* OpticalFlow::sceneFlowParams (OpticalFlow.h:105-129) minus its cv::Mat
+ * members (depth maps and sigmas are passed as arguments), plus schedule
+ * options that do not change the iteration. */
+typedef struct rgbdflow_params {
+  int variant;                   /* RGBDFLOW_VARIANT_* (FULL) */
+  int regularization_type;       /* RGBDFLOW_REG_* */
+  rgbdflow_camera cam;
+  double color_data_weight;
+  double depth_vs_color_data_weight;
+  double alpha;                  /* smoothness weight */
+  double gamma;                  /* zero-flow prior weight */
+  double pyramid_ratio;
+  int min_width;
+  int n_outer, n_inner, n_sor;
+  /* extensions */
+  int relax_variant;             /* RGBDFLOW_RELAX_* */
+  int inner_exit;                /* 1: relaxation early exits as the reference (.cu:914-922) */
+  int outer_exit;                /* 1: outer early exit as the reference (.cu:943-951) */
+  int tile_iters;                /* iterations per launch of the tiled relaxation (0 = default) */
+  int reserved[7];
+} rgbdflow_params;
+
+typedef struct rgbdflow_stats {
+  int n_levels;
+  int width[16], height[16];
+  int outer_exec[16];            /* outer iterations run per level */
+  int sweeps_exec[16];           /* red-black iterations run per level (all outers) */
+  double last_stat[16];          /* last outer convergence statistic per level (-1: none) */
+  double device_ms;              /* device time of the call (HIP events) */
+  double relax_ms;               /* summed device time of the relaxation launches, if timed */
+  int relax_launches;            /* relaxation launches timed */
+} rgbdflow_stats;
+
+typedef struct rgbdflow_ctx rgbdflow_ctx;
+
+/* Defaults of OpticalFlow::sceneFlowParams() (OpticalFlow.h:107-108). */
+int rgbdflow_default_params(rgbdflow_params* p);
+
+/* Context: owns a HIP stream and pooled device buffers (reused across calls).
+ * hbm_budget_bytes = 0 means no limit. */
+int rgbdflow_create(rgbdflow_ctx** out, int device, unsigned long long hbm_budget_bytes);
+int rgbdflow_destroy(rgbdflow_ctx* ctx);
+const char* rgbdflow_last_error(const rgbdflow_ctx* ctx);
+unsigned long long rgbdflow_bytes_in_use(const rgbdflow_ctx* ctx);
+int rgbdflow_synchronize(rgbdflow_ctx* ctx);
+/* Time each relaxation launch with HIP events (stats.relax_ms / relax_launches). */
+int rgbdflow_set_relax_timing(rgbdflow_ctx* ctx, int on);
+
+/* Level count and sizes of GaussianPyramid::ConstructPyramid (GaussianPyramid.cpp:20-63);
+ * widths/heights hold 16 entries.  Returns the level count (>= 1) or < 0. */
+int rgbdflow_pyramid_levels(int W, int H, double ratio, int min_width, int* widths, int* heights);
+
+/* sceneFlowSingleScale (OpticalFlow.h:17-22, src/sceneFlowSingleScaleJacobiCUDA.cu:660-969):
+ * one pyramid level.  Im1/Im2/warpIm2: [N][5] features; dm1/dm2/warpDm2,
+ * u/v/w (in/out), depthData/depthSmoothness/depthMag weights: [N];
+ * regularizationWeights: [N][2].  focal_level = camParams.focalLength of the level.
+ * opts may be NULL (plain relaxation, reference exits). */
+int rgbdflow_solve_level(rgbdflow_ctx* ctx, int W, int H, float focal_level, const double* Im1, const double* dm1,
+                         const double* Im2, const double* dm2, const double* warpIm2, const double* warpDm2,
+                         double* u, double* v, double* w, double colorDataTermWeight,
+                         const double* depthDataTermWeights, double alpha, const double* depthSmoothnessTermWeights,
+                         double gamma, const double* depthMagTermWeights, const double* regularizationWeights,
+                         int nOuter, int nInner, int nSOR, int allowEarlyTermination, const rgbdflow_params* opts,
+                         rgbdflow_stats* stats);
+
+/* computeDepthMapStdevsPointwise (staticDepthNoiseModeling.h:40, .cpp:225-258)
+ * with sigma(1 m) = stereoError(1).  depth, stdevs: [N] float. */
+int rgbdflow_depth_stdevs_pointwise(rgbdflow_ctx* ctx, const float* depth, int W, int H, int max_dist,
+                                    float* stdevs);
+
+/* OpticalFlow::sceneFlowMultiscale (OpticalFlow.h:133, sceneFlowMultiscale.cpp:30-201),
+ * split so that inputs can stay resident in HBM between calls:
+ *   pair_load  copies one frame pair to the device: im1/im2 [N][3] BGR,
+ *              dm1/dm2 [N] hole-filled depth (recursiveMedianFilter applied),
+ *              sigma1 [N] depth stdevs of frame 1 (params.depthSigma1), or
+ *              NULL: computed on the device from dm1 by
+ *              computeDepthMapStdevsPointwise(maxDist 4) at every run,
+ *              hpb: boundary probabilities of every level, [H_k][W_k][2]
+ *              concatenated finest first (NULL unless RGBDFLOW_REG_HPB).
+ *              W, H must equal params->cam.xRes, yRes; the pyramid shape
+ *              (pyramid_ratio, min_width) is fixed by the params given here.
+ *   pair_run   solves the loaded pair on the device.  stats != NULL waits for
+ *              completion and fills them; stats == NULL returns after enqueueing.
+ *   pair_read  copies the flow (u, v in pixels, w in metres; [N] each) back. */
+int rgbdflow_pair_load(rgbdflow_ctx* ctx, const rgbdflow_params* params, int W, int H, const double* im1,
+                       const double* dm1, const double* im2, const double* dm2, const float* sigma1,
+                       const float* hpb);
+int rgbdflow_pair_run(rgbdflow_ctx* ctx, const rgbdflow_params* params, rgbdflow_stats* stats);
+int rgbdflow_pair_read(rgbdflow_ctx* ctx, double* u, double* v, double* w);
+/* load + run + read */
+int rgbdflow_multiscale(rgbdflow_ctx* ctx, int W, int H, const double* im1, const double* dm1, const double* im2,
+                        const double* dm2, const float* sigma1, const float* hpb, const rgbdflow_params* params,
+                        double* u, double* v, double* w, rgbdflow_stats* stats);
+
+/* OpticalFlow::sceneFlowWithInit (OpticalFlow.h:145, sceneFlowWithInit.cpp:7-81):
+ * full resolution only, from the estimate est [N][3], no early termination.
+ * hpb0: level-0 map [H][W][2] (NULL unless HPB). */
+int rgbdflow_with_init(rgbdflow_ctx* ctx, int W, int H, const double* im1, const double* dm1, const double* im2,
+                       const double* dm2, const float* sigma1, const float* hpb0, const double* est,
+                       const rgbdflow_params* params, double* u, double* v, double* w, rgbdflow_stats* stats);
+
+/* ---- stage entry points: each runs one kernel of the path on the device
+ * (used by the per-stage parity tests and available to callers that build
+ * their own pipeline) ---- */
+
+/* GaussianPyramid::ConstructPyramid of a [N][C] image; out: levels concatenated
+ * finest first, each [N_k][C]. */
+int rgbdflow_gaussian_pyramid(rgbdflow_ctx* ctx, const double* img, int W, int H, int C, double ratio, int min_width,
+                              double* out);
+/* DepthMapPyramid::ConstructPyramid (depthMapPyramid.cpp:15-140); out concatenated. */
+int rgbdflow_depth_map_pyramid(rgbdflow_ctx* ctx, const double* depth, int W, int H, double ratio, int min_width,
+                               double* out);
+/* OpticalFlow::im2feature RGB_WITH_GRADIENT (OpticalFlow.cpp:148-220): [N][3] -> [N][5] */
+int rgbdflow_im2feature(rgbdflow_ctx* ctx, const double* bgr, int W, int H, double* out5);
+/* warpBy3DFlow (opticalFlowUtils.cpp:76-99): I [N][C], D/u/v/w [N] */
+int rgbdflow_warp_by_3d_flow(rgbdflow_ctx* ctx, const double* I1, const double* D1, const double* I2,
+                             const double* D2, const double* u, const double* v, const double* w, int W, int H, int C,
+                             double* warpI2, double* warpD2);
+/* Image::imresize(w,h) of u, v, w, then u, v *= uv_scale (sceneFlowMultiscale.cpp:130-134) */
+int rgbdflow_upsample_flow(rgbdflow_ctx* ctx, const double* u, const double* v, const double* w, int Ws, int Hs,
+                           int Wd, int Hd, double uv_scale, double* uo, double* vo, double* wo);
+/* The compact system of one outer iteration (13 planes of N values, f32 values
+ * widened to f64): eRxy, eRz, eDxy, eDz, Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw.
+ * Arguments as rgbdflow_solve_level. */
+int rgbdflow_level_system(rgbdflow_ctx* ctx, int W, int H, float focal_level, const double* Im1, const double* dm1,
+                          const double* Im2, const double* dm2, const double* warpIm2, const double* warpDm2,
+                          const double* u, const double* v, const double* w, double colorDataTermWeight,
+                          const double* depthDataTermWeights, double alpha, const double* depthSmoothnessTermWeights,
+                          double gamma, const double* depthMagTermWeights, const double* regularizationWeights,
+                          double* coef13);
+/* n_iter red-black iterations (red then black) on a compact system; x: [3][N]
+ * planes in/out.  relax_variant: RGBDFLOW_RELAX_*; tile_iters as in params. */
+int rgbdflow_relax(rgbdflow_ctx* ctx, const double* coef13, int W, int H, int n_iter, int relax_variant,
+                   int tile_iters, double* x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,297 @@
+"""Python binding of librgbdflow's C ABI (include/rgbdflow/rgbdflow.h).
+
+A harness for tests and benchmarks: every call goes through the C ABI into
+the gfx950 HIP kernels.  There is no Python or CPU fallback -- if the native
+library is missing, or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librgbdflow.so")
+LIB_CXX_PATH = os.path.join(HERE, "librgbdflow_cxx.so")
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_BUDGET, ERR_STATE, ERR_UNSUPPORTED = -1, -2, -3, -4, -5
+REG_ISOTROPIC, REG_LETOUZEY, REG_HPB = 0, 1, 2
+VARIANT_FULL = 2
+RELAX_PLAIN, RELAX_TILED = 0, 1
+
+# Every symbol include/rgbdflow/rgbdflow.h declares.
+EXPORTS = [
+    "rgbdflow_default_params", "rgbdflow_create", "rgbdflow_destroy", "rgbdflow_last_error",
+    "rgbdflow_bytes_in_use", "rgbdflow_synchronize", "rgbdflow_set_relax_timing", "rgbdflow_pyramid_levels",
+    "rgbdflow_solve_level", "rgbdflow_depth_stdevs_pointwise", "rgbdflow_pair_load", "rgbdflow_pair_run",
+    "rgbdflow_pair_read", "rgbdflow_multiscale", "rgbdflow_with_init", "rgbdflow_gaussian_pyramid",
+    "rgbdflow_depth_map_pyramid", "rgbdflow_im2feature", "rgbdflow_warp_by_3d_flow", "rgbdflow_upsample_flow",
+    "rgbdflow_level_system", "rgbdflow_relax",
+]
+
+
+class Camera(C.Structure):
+    _fields_ = [("xRes", C.c_uint), ("yRes", C.c_uint), ("centerX", C.c_float), ("centerY", C.c_float),
+                ("focalLength", C.c_float), ("stereoBaseline", C.c_float), ("fovX", C.c_float),
+                ("fovY", C.c_float)]
+
+
+class Params(C.Structure):
+    _fields_ = [("variant", C.c_int), ("regularization_type", C.c_int), ("cam", Camera),
+                ("color_data_weight", C.c_double), ("depth_vs_color_data_weight", C.c_double),
+                ("alpha", C.c_double), ("gamma", C.c_double), ("pyramid_ratio", C.c_double),
+                ("min_width", C.c_int), ("n_outer", C.c_int), ("n_inner", C.c_int), ("n_sor", C.c_int),
+                ("relax_variant", C.c_int), ("inner_exit", C.c_int), ("outer_exit", C.c_int),
+                ("tile_iters", C.c_int), ("reserved", C.c_int * 7)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_levels", C.c_int), ("width", C.c_int * 16), ("height", C.c_int * 16),
+                ("outer_exec", C.c_int * 16), ("sweeps_exec", C.c_int * 16), ("last_stat", C.c_double * 16),
+                ("device_ms", C.c_double), ("relax_ms", C.c_double), ("relax_launches", C.c_int)]
+
+    def levels(self):
+        return [dict(W=self.width[k], H=self.height[k], outer=self.outer_exec[k], sweeps=self.sweeps_exec[k],
+                     stat=self.last_stat[k]) for k in range(self.n_levels)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"librgbdflow is not built ({LIB_PATH}); run __graft_entry__.build()")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.rgbdflow_last_error.restype = C.c_char_p
+        _lib.rgbdflow_bytes_in_use.restype = C.c_ulonglong
+        _lib.rgbdflow_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_ulonglong]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class RgbdFlowError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rgbdflow error {code}: {msg}")
+        self.code = code
+
+
+def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
+                ratio=0.5, min_width=None, reg=REG_HPB, cw=1.0, dvc=1.0, inner_exit=True, outer_exit=True,
+                relax_variant=RELAX_PLAIN, tile_iters=0) -> Params:
+    p = Params()
+    lib().rgbdflow_default_params(C.byref(p))
+    p.regularization_type = reg
+    p.color_data_weight = cw
+    p.depth_vs_color_data_weight = dvc
+    p.alpha = alpha
+    p.gamma = gamma
+    p.pyramid_ratio = ratio
+    p.min_width = min_width if min_width is not None else W // 16
+    p.n_outer, p.n_inner, p.n_sor = n_outer, n_inner, n_sor
+    p.cam.xRes, p.cam.yRes = W, H
+    p.cam.focalLength = f if f is not None else 525.0 * W / 640.0
+    p.cam.centerX = cx if cx is not None else (W - 1) / 2.0
+    p.cam.centerY = cy if cy is not None else (H - 1) / 2.0
+    p.inner_exit = int(inner_exit)
+    p.outer_exit = int(outer_exit)
+    p.relax_variant = relax_variant
+    p.tile_iters = tile_iters
+    return p
+
+
+def pyramid_levels(W, H, ratio, min_width):
+    ws = (C.c_int * 16)()
+    hs = (C.c_int * 16)()
+    n = lib().rgbdflow_pyramid_levels(W, H, C.c_double(ratio), min_width, ws, hs)
+    if n < 0:
+        raise RgbdFlowError(n, "bad pyramid arguments")
+    return [(ws[i], hs[i]) for i in range(n)]
+
+
+class Context:
+    """One librgbdflow context (HIP stream + pooled HBM buffers) on one device."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        rc = lib().rgbdflow_create(C.byref(h), device, hbm_budget)
+        if rc != OK:
+            raise RgbdFlowError(rc, "rgbdflow_create failed (no HIP device?)")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().rgbdflow_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != OK:
+            raise RgbdFlowError(rc, lib().rgbdflow_last_error(self.h).decode())
+
+    def bytes_in_use(self):
+        return lib().rgbdflow_bytes_in_use(self.h)
+
+    def synchronize(self):
+        self._check(lib().rgbdflow_synchronize(self.h))
+
+    def set_relax_timing(self, on: bool):
+        self._check(lib().rgbdflow_set_relax_timing(self.h, int(on)))
+
+    # ---------------------------------------------------------------- stages
+    def gaussian_pyramid(self, img, ratio, min_width):
+        img = _f64(img)
+        H, W = img.shape[:2]
+        Cn = 1 if img.ndim == 2 else img.shape[2]
+        levels = pyramid_levels(W, H, ratio, min_width)
+        flat = np.empty(sum(w * h for w, h in levels) * Cn, np.float64)
+        self._check(lib().rgbdflow_gaussian_pyramid(self.h, _p(img), W, H, Cn, C.c_double(ratio), min_width,
+                                                    _p(flat)))
+        out, off = [], 0
+        for (w, h) in levels:
+            n = w * h * Cn
+            out.append(flat[off:off + n].reshape((h, w) + ((Cn,) if Cn > 1 else ())))
+            off += n
+        return out
+
+    def depth_map_pyramid(self, depth, ratio, min_width):
+        d = _f64(depth)
+        H, W = d.shape
+        n = len(pyramid_levels(W, H, ratio, min_width))
+        r = ratio if 0.4 <= ratio <= 0.98 else 0.75
+        sizes = [(int(W * r ** m), int(H * r ** m)) for m in range(n)]
+        flat = np.empty(sum(w * h for w, h in sizes), np.float64)
+        self._check(lib().rgbdflow_depth_map_pyramid(self.h, _p(d), W, H, C.c_double(ratio), min_width, _p(flat)))
+        out, off = [], 0
+        for (w, h) in sizes:
+            out.append(flat[off:off + w * h].reshape(h, w))
+            off += w * h
+        return out
+
+    def im2feature(self, bgr):
+        bgr = _f64(bgr)
+        H, W = bgr.shape[:2]
+        out = np.empty((H, W, 5), np.float64)
+        self._check(lib().rgbdflow_im2feature(self.h, _p(bgr), W, H, _p(out)))
+        return out
+
+    def warp_by_3d_flow(self, I1, D1, I2, D2, u, v, w):
+        I1, D1, I2, D2, u, v, w = map(_f64, (I1, D1, I2, D2, u, v, w))
+        H, W = D1.shape
+        wI = np.empty_like(I1)
+        wD = np.empty_like(D1)
+        self._check(lib().rgbdflow_warp_by_3d_flow(self.h, _p(I1), _p(D1), _p(I2), _p(D2), _p(u), _p(v), _p(w), W,
+                                                   H, I1.shape[2], _p(wI), _p(wD)))
+        return wI, wD
+
+    def upsample_flow(self, u, v, w, Wd, Hd, uv_scale):
+        u, v, w = _f64(u), _f64(v), _f64(w)
+        Hs, Ws = u.shape
+        uo = np.empty((Hd, Wd)); vo = np.empty((Hd, Wd)); wo = np.empty((Hd, Wd))
+        self._check(lib().rgbdflow_upsample_flow(self.h, _p(u), _p(v), _p(w), Ws, Hs, Wd, Hd, C.c_double(uv_scale),
+                                                 _p(uo), _p(vo), _p(wo)))
+        return uo, vo, wo
+
+    def depth_stdevs(self, depth_f32, max_dist=4):
+        d = _f32(depth_f32)
+        H, W = d.shape
+        out = np.empty_like(d)
+        self._check(lib().rgbdflow_depth_stdevs_pointwise(self.h, _p(d), W, H, max_dist, _p(out)))
+        return out
+
+    @staticmethod
+    def _level_args(li):
+        return (C.c_float(li.focal_level), _p(li.I1), _p(li.D1), _p(li.I2), _p(li.D2), _p(li.warpI2),
+                _p(li.warpD2))
+
+    def level_system(self, li, params, u, v, w):
+        u, v, w = _f64(u), _f64(v), _f64(w)
+        out = np.empty((13, li.H, li.W), np.float64)
+        self._check(lib().rgbdflow_level_system(
+            self.h, li.W, li.H, *self._level_args(li), _p(u), _p(v), _p(w), C.c_double(params.color_data_weight),
+            _p(li.delta), C.c_double(params.alpha), _p(li.s), C.c_double(params.gamma), _p(li.m), _p(li.r),
+            _p(out)))
+        return out
+
+    def relax(self, coef13, n_iter, x, relax_variant=RELAX_PLAIN, tile_iters=0):
+        c = _f64(coef13)
+        _, H, W = c.shape
+        xx = _f64(x).copy()
+        self._check(lib().rgbdflow_relax(self.h, _p(c), W, H, n_iter, relax_variant, tile_iters, _p(xx)))
+        return xx
+
+    def solve_level(self, li, params, allow_early, u, v, w):
+        u, v, w = _f64(u).copy(), _f64(v).copy(), _f64(w).copy()
+        st = Stats()
+        self._check(lib().rgbdflow_solve_level(
+            self.h, li.W, li.H, *self._level_args(li), _p(u), _p(v), _p(w), C.c_double(params.color_data_weight),
+            _p(li.delta), C.c_double(params.alpha), _p(li.s), C.c_double(params.gamma), _p(li.m), _p(li.r),
+            params.n_outer, params.n_inner, params.n_sor, int(allow_early), C.byref(params), C.byref(st)))
+        return u, v, w, st
+
+    # ------------------------------------------------------------ pair path
+    def pair_load(self, params, im1, dm1, im2, dm2, sigma1=None, hpb=None):
+        im1, dm1, im2, dm2 = _f64(im1), _f64(dm1), _f64(im2), _f64(dm2)
+        H, W = dm1.shape
+        s1 = _f32(sigma1) if sigma1 is not None else None
+        h = _f32(hpb) if hpb is not None else None
+        self._keep = (im1, dm1, im2, dm2, s1, h)
+        self._check(lib().rgbdflow_pair_load(self.h, C.byref(params), W, H, _p(im1), _p(dm1), _p(im2), _p(dm2),
+                                             _p(s1), _p(h)))
+        self._pair_shape = (H, W)
+
+    def pair_run(self, params, stats=True):
+        if stats:
+            st = Stats()
+            self._check(lib().rgbdflow_pair_run(self.h, C.byref(params), C.byref(st)))
+            return st
+        self._check(lib().rgbdflow_pair_run(self.h, C.byref(params), None))
+        return None
+
+    def pair_read(self):
+        H, W = self._pair_shape
+        u = np.empty((H, W)); v = np.empty((H, W)); w = np.empty((H, W))
+        self._check(lib().rgbdflow_pair_read(self.h, _p(u), _p(v), _p(w)))
+        return u, v, w
+
+    def multiscale(self, params, im1, dm1, im2, dm2, sigma1=None, hpb=None):
+        self.pair_load(params, im1, dm1, im2, dm2, sigma1, hpb)
+        st = self.pair_run(params, stats=True)
+        u, v, w = self.pair_read()
+        return u, v, w, st
+
+    def with_init(self, params, im1, dm1, im2, dm2, est, sigma1=None, hpb0=None):
+        im1, dm1, im2, dm2, est = _f64(im1), _f64(dm1), _f64(im2), _f64(dm2), _f64(est)
+        H, W = dm1.shape
+        s1 = _f32(sigma1) if sigma1 is not None else None
+        h = _f32(hpb0) if hpb0 is not None else None
+        u = np.empty((H, W)); v = np.empty((H, W)); w = np.empty((H, W))
+        st = Stats()
+        self._check(lib().rgbdflow_with_init(self.h, W, H, _p(im1), _p(dm1), _p(im2), _p(dm2), _p(s1), _p(h),
+                                             _p(est), C.byref(params), _p(u), _p(v), _p(w), C.byref(st)))
+        return u, v, w, st

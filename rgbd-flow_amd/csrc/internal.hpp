@@ -27,9 +27,11 @@ enum Coef { kERxy = 0, kERz, kEDxy, kEDz, kBuu, kBvv, kBww, kBuv, kBuw, kBvw, kR
 struct SolveCtl {
   int outer_done;                     // outer loop left early (stat < threshold)
   int outer_exit_at;                  // outer iteration that set outer_done
-  unsigned int maxdiff[kMaxChecks];   // float bits of max|du change| of each check iteration
+  int sweeps_exec;                    // red-black iterations executed (all outers)
+  int pad;
+  unsigned int maxdiff[kMaxChecks];   // float bits of max|du change| per check of the
+                                      // current outer, [inner][check]; zeroed per outer
   double stat[kMaxOuter];             // outer convergence statistic per outer iteration
-  unsigned int stat_blocks_done[kMaxOuter];
 };
 
 // Everything one level-solve kernel needs (passed by value).
@@ -72,7 +74,7 @@ struct SolveScalars {
 // ---------------------------------------------------------------- launchers
 // kernels_solve.hip
 void launch_ctl_reset(hipStream_t s, SolveCtl* ctl);
-void launch_derivs(hipStream_t s, const LevelView& v, int outer);
+void launch_derivs(hipStream_t s, const LevelView& v, int n_slots);
 void launch_phi(hipStream_t s, const LevelView& v, int outer);
 void launch_system(hipStream_t s, const LevelView& v, const SolveScalars& k, int outer);
 void launch_relax_plain(hipStream_t s, const LevelView& v, int colour, int outer, int slot_base,
@@ -84,6 +86,7 @@ int stat_blocks(int N);
 void launch_deinterleave(hipStream_t s, const double* src, int N, int C, double* dst);
 void launch_interleave(hipStream_t s, const double* src, int N, int C, double* dst);
 void launch_fill_f64(hipStream_t s, double* dst, size_t n, double v);
+void launch_to_f32(hipStream_t s, const double* src, size_t n, float* dst);
 void launch_pyramid_level(hipStream_t s, const double* src, int Ws, int Hs, int C, const double* taps,
                           int fsize, double ratio, double* dst, int Wd, int Hd);
 void launch_features(hipStream_t s, const double* bgr, int W, int H, double* out5);

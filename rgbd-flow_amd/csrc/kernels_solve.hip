@@ -1,0 +1,385 @@
+// kernels_solve.hip -- one pyramid level of the variational scene-flow solve on gfx950.
+//
+// Restates sceneFlowSingleScale (src/sceneFlowSingleScaleJacobiCUDA.cu:660-969):
+// derivatives + occlusion weights once per outer iteration, robust weights and the
+// compact system per inner iteration, the red-black relaxation, then u += du and
+// the re-warp.  Arithmetic follows the reference's precision exactly: images,
+// derivatives, products and u in f64; the system, du, q and the device weight
+// copies in f32; the squares inside the robust weights in f32 (sqrCU, .cu:44-47).
+// Built with -ffp-contract=off so every rounding happens where the source says.
+//
+// Early exits are decided on the device: a kernel reads the level's SolveCtl at
+// entry and returns when the outer loop or the current relaxation has stopped,
+// so the host enqueues a whole level (or pair) without waiting on it.
+#include "internal.hpp"
+
+namespace rgbdflow {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// sqrCU (.cu:44-47): the argument is rounded to f32 and squared in f32.
+__device__ __forceinline__ float sqr_f32(double x) {
+  const float f = (float)x;
+  return f * f;
+}
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__global__ void k_ctl_reset(SolveCtl* ctl) {
+  unsigned int* p = reinterpret_cast<unsigned int*>(ctl);
+  const size_t n = sizeof(SolveCtl) / sizeof(unsigned int);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// .cu:84-147 (x/y derivatives of 0.4*I1 + 0.6*warped I2 with [1,-8,0,8,-1]/12,
+// clamp-to-edge; depth derivatives clamped to +-0.04), .cu:776 (du = 0),
+// .cu:640-648 (occlusion weight from the UNWARPED D2).
+__global__ void __launch_bounds__(kBlock) k_derivs(LevelView v, int n_slots) {
+  if (v.ctl->outer_done) return;
+  const int l = blockIdx.x * kBlock + threadIdx.x;
+  for (int t = l; t < n_slots; t += gridDim.x * kBlock) v.ctl->maxdiff[t] = 0u;
+  if (l >= v.N) return;
+  const int W = v.W, H = v.H, N = v.N;
+  const int i = l / W, j = l - i * W;
+  const double g[5] = {1, -8, 0, 8, -1};
+  for (int k = 0; k < kFeat; k++) {
+    const double* a = v.I1 + (size_t)k * N;
+    const double* b = v.Iw + (size_t)k * N;
+    double sx = 0, sy = 0;
+    for (int m = 0; m < 5; m++) {
+      const int c = clampi(j - 2 + m, 0, W - 1);
+      const int o = i * W + c;
+      sx += g[m] * (.4 * a[o] + .6 * b[o]);
+    }
+    for (int m = 0; m < 5; m++) {
+      const int r = clampi(i - 2 + m, 0, H - 1);
+      const int o = r * W + j;
+      sy += g[m] * (.4 * a[o] + .6 * b[o]);
+    }
+    v.Ix[(size_t)k * N + l] = sx / 12;
+    v.Iy[(size_t)k * N + l] = sy / 12;
+    v.It[(size_t)k * N + l] = b[l] - a[l];
+  }
+  {
+    const double mg = .04;
+    double sx = 0, sy = 0;
+    for (int m = 0; m < 5; m++) {
+      const int o = i * W + clampi(j - 2 + m, 0, W - 1);
+      sx += g[m] * (.4 * v.D1[o] + .6 * v.Dw[o]);
+    }
+    for (int m = 0; m < 5; m++) {
+      const int o = clampi(i - 2 + m, 0, H - 1) * W + j;
+      sy += g[m] * (.4 * v.D1[o] + .6 * v.Dw[o]);
+    }
+    v.Dx[l] = fmin(fmax(sx / 12, -mg), mg);
+    v.Dy[l] = fmin(fmax(sy / 12, -mg), mg);
+    v.Dt[l] = fmin(fmax(v.Dw[l] - v.D1[l], -mg), mg);
+  }
+  v.du[l] = 0.f;
+  v.du[N + l] = 0.f;
+  v.du[2 * N + l] = 0.f;
+  const double d1 = v.D1[l];
+  v.q[l] = (d1 + v.u[2 * N + l] < v.D2[l] + .02 * (double)sqr_f32(d1)) ? 1.f : (float).01;
+}
+
+// .cu:156-182: forward differences of u + du (zero on the last row/column).
+__device__ __forceinline__ void flow_grad(const LevelView& v, int c, int l, int i, int j, double& gx, double& gy) {
+  const double* uc = v.u + (size_t)c * v.N;
+  const float* dc = v.du + (size_t)c * v.N;
+  gy = (i == v.H - 1) ? 0.0 : (uc[l + v.W] + (double)dc[l + v.W] - uc[l] - (double)dc[l]);
+  gx = (j == v.W - 1) ? 0.0 : (uc[l + 1] + (double)dc[l + 1] - uc[l] - (double)dc[l]);
+}
+
+// .cu:354-355: smoothness weight phi' of the Charbonnier penalty.
+__global__ void __launch_bounds__(kBlock) k_phi(LevelView v) {
+  if (v.ctl->outer_done) return;
+  const int l = blockIdx.x * kBlock + threadIdx.x;
+  if (l >= v.N) return;
+  const int i = l / v.W, j = l - i * v.W;
+  double ux, uy, vx, vy, wx, wy;
+  flow_grad(v, 0, l, i, j, ux, uy);
+  flow_grad(v, 1, l, i, j, vx, vy);
+  flow_grad(v, 2, l, i, j, wx, wy);
+  const float t = sqr_f32(ux) + sqr_f32(uy) + sqr_f32(vx) + sqr_f32(vy) + v.sW[l] * (sqr_f32(wx) + sqr_f32(wy));
+  v.phi[l] = 0.5 / sqrt((double)t + 1e-6);
+}
+
+// .cu:240-333: phi- and r-weighted Laplacian of u (not u+du), normalised by the
+// mean regularisation weight of the existing edges.
+__device__ __forceinline__ double laplacian(const LevelView& v, int c, int l, int i, int j) {
+  const double* uc = v.u + (size_t)c * v.N;
+  const float* rR = v.rW;
+  const float* rD = v.rW + v.N;
+  const int W = v.W;
+  double lap = 0;
+  if (j < W - 1) lap -= (uc[l + 1] - uc[l]) * v.phi[l] * (double)rR[l];
+  if (j > 0) lap += (uc[l] - uc[l - 1]) * v.phi[l - 1] * (double)rR[l - 1];
+  if (i < v.H - 1) lap -= (uc[l + W] - uc[l]) * v.phi[l] * (double)rD[l];
+  if (i > 0) lap += (uc[l] - uc[l - W]) * v.phi[l - W] * (double)rD[l - W];
+  double norm = 0;
+  int count = 0;
+  if (j < W - 1) { norm += rR[l]; count++; }
+  if (j > 0) { norm += rR[l - 1]; count++; }
+  if (i < v.H - 1) { norm += rD[l]; count++; }
+  if (i > 0) { norm += rD[l - W]; count++; }
+  if (fabs(norm) < 1e-5) lap = 0;
+  else lap /= (norm / count);
+  return lap;
+}
+
+// .cu:356-369 (psi', pi'), .cu:189-230 (products), .cu:477-575 (assembly),
+// written as the compact per-pixel system (Coef).
+__global__ void __launch_bounds__(kBlock) k_system(LevelView v, SolveScalars s) {
+  if (v.ctl->outer_done) return;
+  const int l = blockIdx.x * kBlock + threadIdx.x;
+  if (l >= v.N) return;
+  const int W = v.W, H = v.H, N = v.N;
+  const int i = l / W, j = l - i * W;
+  const double alpha = s.alpha, gamma = s.gamma;
+  const double d0 = (double)v.du[l], d1 = (double)v.du[N + l], d2 = (double)v.du[2 * N + l];
+  const double ql = (double)v.q[l];
+
+  double sxy = 0, sxx = 0, syy = 0, stx = 0, sty = 0;
+  double psi[kFeat];
+  for (int k = 0; k < kFeat; k++) {
+    const size_t o = (size_t)k * N + l;
+    const double tt = (v.It[o] + v.Ix[o] * d0 + v.Iy[o] * d1) * ql;
+    psi[k] = .5 / sqrt((double)sqr_f32(tt) + 1e-6) * s.color_weight;
+  }
+  for (int k = 0; k < kFeat; k++) {
+    const size_t o = (size_t)k * N + l;
+    sxy += psi[k] * v.Ix[o] * v.Iy[o];
+    sxx += psi[k] * v.Ix[o] * v.Ix[o];
+    syy += psi[k] * v.Iy[o] * v.Iy[o];
+  }
+  for (int k = 0; k < kFeat; k++) { const size_t o = (size_t)k * N + l; stx += psi[k] * v.Ix[o] * v.It[o]; }
+  for (int k = 0; k < kFeat; k++) { const size_t o = (size_t)k * N + l; sty += psi[k] * v.Iy[o] * v.It[o]; }
+  const double ixy = sxy / kFeat, ix2 = sxx / kFeat, iy2 = syy / kFeat;
+  const double itx = stx * ql * ql / kFeat, ity = sty * ql * ql / kFeat;
+
+  const double Dx = v.Dx[l], Dy = v.Dy[l], Dt = v.Dt[l];
+  const double t2 = Dt + Dx * d0 + Dy * d1 - d2;
+  const double P = .5 / sqrt((double)sqr_f32(t2) + 1e-10) * (double)v.dW[l];
+  const double dxy = P * Dx * Dy, dx2 = P * Dx * Dx, dy2 = P * Dy * Dy;
+  const double dtx = P * Dx * Dt, dty = P * Dy * Dt;
+
+  const double uL = laplacian(v, 0, l, i, j), vL = laplacian(v, 1, l, i, j);
+  const double wL = (double)v.sW[l] * laplacian(v, 2, l, i, j);
+
+  const float* rR = v.rW;
+  const float* rD = v.rW + N;
+  double wsXY = 0, wsZ = 0;
+  float eRxy = 0.f, eRz = 0.f, eDxy = 0.f, eDz = 0.f;
+  if (j > 0) { const double a = v.phi[l - 1] * (double)rR[l - 1], b = a * (double)v.sW[l - 1]; wsXY += a; wsZ += b; }
+  if (i > 0) { const double a = v.phi[l - W] * (double)rD[l - W], b = a * (double)v.sW[l - W]; wsXY += a; wsZ += b; }
+  if (j < W - 1) {
+    const double a = v.phi[l] * (double)rR[l], b = a * (double)v.sW[l];
+    eRxy = (float)(alpha * a); eRz = (float)(alpha * b); wsXY += a; wsZ += b;
+  }
+  if (i < H - 1) {
+    const double a = v.phi[l] * (double)rD[l], b = a * (double)v.sW[l];
+    eDxy = (float)(alpha * a); eDz = (float)(alpha * b); wsXY += a; wsZ += b;
+  }
+  const double u0 = v.u[l], u1 = v.u[N + l], u2 = v.u[2 * N + l];
+  const double m = (double)v.mW[l];
+  float t;
+  float* C = v.coef;
+  C[kERxy * (size_t)N + l] = eRxy;
+  C[kERz * (size_t)N + l] = eRz;
+  C[kEDxy * (size_t)N + l] = eDxy;
+  C[kEDz * (size_t)N + l] = eDz;
+  t = (float)(-(ix2 + dx2 + alpha * wsXY)); C[kBuu * (size_t)N + l] = (float)((double)t + -(gamma));
+  C[kBuv * (size_t)N + l] = (float)(-ixy - dxy);
+  C[kBuw * (size_t)N + l] = (float)(P * Dx);
+  t = (float)(-(-itx - dtx + alpha * -uL)); C[kRu * (size_t)N + l] = (float)((double)t + -(-gamma * u0));
+  t = (float)(-(iy2 + dy2 + alpha * wsXY)); C[kBvv * (size_t)N + l] = (float)((double)t + -(gamma));
+  C[kBvw * (size_t)N + l] = (float)(P * Dy);
+  t = (float)(-(-ity - dty + alpha * -vL)); C[kRv * (size_t)N + l] = (float)((double)t + -(-gamma * u1));
+  t = (float)(-(P + alpha * wsZ)); C[kBww * (size_t)N + l] = (float)((double)t + -(gamma * m));
+  t = (float)(-(P * Dt + alpha * -wL)); C[kRw * (size_t)N + l] = (float)((double)t + -(-gamma * m * u2));
+}
+
+// The relaxation has stopped for this (outer, inner) if an earlier check
+// iteration saw max|change| < 1e-5 or >= 0.3 (.cu:914-922).
+__device__ __forceinline__ bool relax_stopped(const SolveCtl* ctl, int slot_base, int n_prev_checks) {
+  for (int c = 0; c < n_prev_checks; c++) {
+    const float md = __uint_as_float(ctl->maxdiff[slot_base + c]);
+    if ((double)md < 1e-5 || md >= .3f) return true;
+  }
+  return false;
+}
+
+// .cu:382-470: one colour of the red-black relaxation with omega = 0.7 and the
+// +-0.3 step clamp; the three unknowns of a pixel use each other's old values.
+// One thread per pixel of the colour; x = du, coefficients from the compact system.
+__global__ void __launch_bounds__(256) k_relax_plain(LevelView v, int colour, int slot_base, int n_prev_checks,
+                                                     int check_slot) {
+  if (v.ctl->outer_done) return;
+  if (relax_stopped(v.ctl, slot_base, n_prev_checks)) return;
+  const int W = v.W, H = v.H, N = v.N;
+  const int xc = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  const int j = 2 * xc + ((i + colour) & 1);
+  float change = 0.f;
+  if (i < H && j < W) {
+    const int l = i * W + j;
+    const float* C = v.coef;
+    float* x = v.du;
+    const float xo0 = x[l], xo1 = x[N + l], xo2 = x[2 * N + l];
+    const float xo[3] = {xo0, xo1, xo2};
+    float xn[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float* xk = x + (size_t)k * N;
+      const float* eR = C + (size_t)(k < 2 ? kERxy : kERz) * N;
+      const float* eD = C + (size_t)(k < 2 ? kEDxy : kEDz) * N;
+      float nv = 0.f;
+      if (j > 0) nv -= eR[l - 1] * xk[l - 1];
+      if (i > 0) nv -= eD[l - W] * xk[l - W];
+      if (j < W - 1) nv -= eR[l] * xk[l + 1];
+      if (i < H - 1) nv -= eD[l] * xk[l + W];
+      nv += C[(size_t)(kRu + k) * N + l];
+      if (k == 0) {
+        nv -= C[kBuv * (size_t)N + l] * xo[1];
+        nv -= C[kBuw * (size_t)N + l] * xo[2];
+        nv /= C[kBuu * (size_t)N + l];
+      } else if (k == 1) {
+        nv -= C[kBuv * (size_t)N + l] * xo[0];
+        nv -= C[kBvw * (size_t)N + l] * xo[2];
+        nv /= C[kBvv * (size_t)N + l];
+      } else {
+        nv -= C[kBuw * (size_t)N + l] * xo[0];
+        nv -= C[kBvw * (size_t)N + l] * xo[1];
+        nv /= C[kBww * (size_t)N + l];
+      }
+      const float omega = .7f;
+      float r = omega * nv + (1 - omega) * xo[k];
+      if (r < xo[k] - .3f) r = xo[k] - .3f;
+      else if (r > xo[k] + .3f) r = xo[k] + .3f;
+      xn[k] = r;
+    }
+    x[l] = xn[0];
+    x[N + l] = xn[1];
+    x[2 * N + l] = xn[2];
+    change = fmaxf(fmaxf(fabsf(xn[0] - xo0), fabsf(xn[1] - xo1)), fabsf(xn[2] - xo2));
+  }
+  if (check_slot >= 0) {
+    const float m = wave_max(change);
+    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(&v.ctl->maxdiff[check_slot], __float_as_uint(m));
+  }
+  if (colour == 1 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0)
+    v.ctl->sweeps_exec += 1;
+}
+
+// .cu:934-941 (u += du, GPU warps: floor base, explicit clamps, f32 weights,
+// out-of-bounds -> frame 1) and the per-block part of the outer statistic
+// sum(du_k^2 * c_k) (.cu:948), accumulated in f64 in a fixed order.
+__global__ void __launch_bounds__(kBlock) k_update_warp(LevelView v, int want_stat) {
+  __shared__ double red[kBlock];
+  if (v.ctl->outer_done) return;
+  const int l = blockIdx.x * kBlock + threadIdx.x;
+  const int W = v.W, H = v.H, N = v.N;
+  double term = 0;
+  if (l < N) {
+    const int i = l / W, j = l - i * W;
+    const float e0 = v.du[l], e1 = v.du[N + l], e2 = v.du[2 * N + l];
+    const double u0 = v.u[l] + (double)e0;
+    const double u1 = v.u[N + l] + (double)e1;
+    const double u2 = v.u[2 * N + l] + (double)e2;
+    v.u[l] = u0;
+    v.u[N + l] = u1;
+    v.u[2 * N + l] = u2;
+    const double x = j + u0, y = i + u1;
+    if (x < 0 || x > W - 1 || y < 0 || y > H - 1) {
+      for (int k = 0; k < kFeat; k++) v.Iw[(size_t)k * N + l] = v.I1[(size_t)k * N + l];
+      v.Dw[l] = v.D1[l] - u2;
+    } else {
+      const int x0 = clampi((int)floor(x), 0, W - 1), y0 = clampi((int)floor(y), 0, H - 1);
+      const int x1 = min(W - 1, x0 + 1), y1 = min(H - 1, y0 + 1);
+      const float ax = x - x0, ay = y - y0;
+      const float w00 = (1 - ax) * (1 - ay), w01 = (1 - ax) * ay, w10 = ax * (1 - ay), w11 = ax * ay;
+      const int o00 = y0 * W + x0, o01 = y1 * W + x0, o10 = y0 * W + x1, o11 = y1 * W + x1;
+      for (int k = 0; k < kFeat; k++) {
+        const double* b = v.I2 + (size_t)k * N;
+        v.Iw[(size_t)k * N + l] = w00 * b[o00] + w01 * b[o01] + w10 * b[o10] + w11 * b[o11];
+      }
+      v.Dw[l] = w00 * v.D2[o00] + w01 * v.D2[o01] + w10 * v.D2[o10] + w11 * v.D2[o11] - u2;
+    }
+    if (want_stat) {
+      // sqrCU(du_k) * c_k in f32 (thrustSqrMultFunctor), c = (1, 1, f/D1)
+      term = (double)(e0 * e0);
+      term += (double)(e1 * e1);
+      term += (double)((e2 * e2) * v.cW[l]);
+    }
+  }
+  if (!want_stat) return;
+  red[threadIdx.x] = term;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) v.partials[blockIdx.x] = red[0];
+}
+
+// Fixed-order sum of the per-block partials -> outer statistic; sets the
+// level's outer_done flag when early termination applies (.cu:943-951).
+__global__ void __launch_bounds__(kBlock) k_stat_final(LevelView v, int nblocks, int outer, int decide) {
+  __shared__ double red[kBlock];
+  if (v.ctl->outer_done) return;
+  double acc = 0;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += v.partials[b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double stat = red[0] / (double)(v.W * v.H * 3);
+    v.ctl->stat[outer] = stat;
+    if (decide && stat < 4e-4) {
+      v.ctl->outer_done = 1;
+      v.ctl->outer_exit_at = outer;
+    }
+  }
+}
+
+inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+int stat_blocks(int N) { return blocks_for(N); }
+
+void launch_ctl_reset(hipStream_t s, SolveCtl* ctl) { hipLaunchKernelGGL(k_ctl_reset, dim3(64), dim3(256), 0, s, ctl); }
+
+void launch_derivs(hipStream_t s, const LevelView& v, int n_slots) {
+  hipLaunchKernelGGL(k_derivs, dim3(blocks_for(v.N)), dim3(kBlock), 0, s, v, n_slots);
+}
+void launch_phi(hipStream_t s, const LevelView& v, int) {
+  hipLaunchKernelGGL(k_phi, dim3(blocks_for(v.N)), dim3(kBlock), 0, s, v);
+}
+void launch_system(hipStream_t s, const LevelView& v, const SolveScalars& k, int) {
+  hipLaunchKernelGGL(k_system, dim3(blocks_for(v.N)), dim3(kBlock), 0, s, v, k);
+}
+void launch_relax_plain(hipStream_t s, const LevelView& v, int colour, int, int slot_base, int n_prev_checks,
+                        int check_slot) {
+  const dim3 blk(64, 4);
+  const dim3 grid(((v.W + 1) / 2 + 63) / 64, (v.H + 3) / 4);
+  hipLaunchKernelGGL(k_relax_plain, grid, blk, 0, s, v, colour, slot_base, n_prev_checks, check_slot);
+}
+void launch_update_warp(hipStream_t s, const LevelView& v, const SolveScalars& k, int outer) {
+  const int nb = blocks_for(v.N);
+  const int want = k.allow_early ? 1 : 0;
+  hipLaunchKernelGGL(k_update_warp, dim3(nb), dim3(kBlock), 0, s, v, want);
+  if (want) hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(kBlock), 0, s, v, nb, outer, k.outer_exit ? 1 : 0);
+}
+
+}  // namespace rgbdflow
