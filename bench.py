@@ -1,0 +1,181 @@
+"""Benchmark: 640x480 scene-flow pairs/s on one MI355X.
+
+Workload: one synthetic 640x480 RGB-D frame pair (f = 525, c = (319.5, 239.5),
+6 moving planar segments), 5-level pyramid at ratio 0.5, 15 outer x 50 inner
+iterations, boundary-probability regularisation (maps from the generator's
+segment labels), the reference's mixed precision (f64 images, derivatives and
+flow; f32 system and increments).  A step is one full coarse-to-fine solve of
+the pair: depth stdevs, the six pyramids, per-level features / upsample / warp /
+weights, and the level solves, with the inputs resident in HBM.
+
+Prints one JSON line.  --gpus above 1 is refused: the solve runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "rgbd-flow_amd")
+
+METRIC = "640×480 scene-flow pairs/s + p99 latency, batched vs plain; HBM GB/s"
+B_SWEEP = 72.0     # algorithmic bytes per pixel per red-black iteration (f32 system + increments)
+B_STAGE = 128.0    # algorithmic bytes per pixel per outer iteration (warp, gradients, weights, assembly)
+HBM_PEAK_GBS = 8000.0
+
+
+def _package():
+    spec = importlib.util.spec_from_file_location("rgbdflow_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["rgbdflow_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def workload(R, seed=21):
+    import rgbdflow_amd.synth as S
+    W, H = 640, 480
+    cam = S.Camera(W, H, 525.0, 525.0, 319.5, 239.5)
+    pair = S.make_pair(W, H, seed, n_segments=6, cam=cam)
+    im1, d1, im2, d2 = S.to_solver_inputs(pair)
+    levels = R.pyramid_levels(W, H, 0.5, 40)
+    hpb = S.segment_boundary_maps(pair["seg1"], [w for w, h in levels], [h for w, h in levels])
+    return pair, im1, d1.astype(np.float64), im2, d2.astype(np.float64), hpb, levels
+
+
+def params_for(R, variant):
+    return R.make_params(640, 480, f=525.0, cx=319.5, cy=239.5, n_outer=15, n_sor=50, min_width=40,
+                         relax_variant=R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN)
+
+
+def algorithmic_bytes(st):
+    sweep = stage = 0.0
+    for k in range(st.n_levels):
+        n = st.width[k] * st.height[k]
+        sweep += n * st.sweeps_exec[k] * B_SWEEP
+        stage += n * st.outer_exec[k] * B_STAGE
+    return sweep, stage
+
+
+def cpu_baseline(im1, d1, im2, d2, hpb, budget_s):
+    """The oracle (reference-precision restatement, one thread) on the same
+    workload: as many whole pairs as fit in the time budget (at least one)."""
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    import oracle as O
+    po = O.make_params(640, 480, f=525.0, cx=319.5, cy=239.5, n_outer=15, n_sor=50, min_width=40)
+    sigma = O.depth_stdevs(d1.astype(np.float32), 4)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        O.multiscale(im1, d1, im2, d2, sigma, hpb, po)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 8:
+            break
+    return n / el, n, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--variant", choices=["plain", "tiled"], default="plain")
+    ap.add_argument("--latency-steps", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} refused: the scene-flow solve runs on one GPU "
+              "(pairs are independent; see DESIGN.md on sharding)", file=sys.stderr)
+        sys.exit(2)
+
+    R = _package()
+    pair, im1, d1, im2, d2, hpb, levels = workload(R)
+    p = params_for(R, args.variant)
+    ctx = R.Context(0)
+    ctx.pair_load(p, im1, d1, im2, d2, None, hpb)   # stdevs computed on the device every step
+    print(f"bench: 640x480 pair, {len(levels)} levels, 15x50, variant {args.variant}", file=sys.stderr, flush=True)
+    st = None
+    for _ in range(args.warmup):
+        st = ctx.pair_run(p, stats=True)
+    if st is None:
+        st = ctx.pair_run(p, stats=True)
+
+    # timed region: K back-to-back solves of the resident pair
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.pair_run(p, stats=False)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    pairs_per_s = args.steps / elapsed
+    print(f"bench: timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # per-pair device latency (HIP events around each solve)
+    lat = []
+    for _ in range(args.latency_steps):
+        lat.append(ctx.pair_run(p, stats=True).device_ms)
+    lat = np.array(lat)
+
+    # dominant kernel: the relaxation, timed launch by launch with HIP events
+    ctx.set_relax_timing(True)
+    rt = ctx.pair_run(p, stats=True)
+    ctx.set_relax_timing(False)
+    sweep_bytes, stage_bytes = algorithmic_bytes(rt)
+    relax_s = rt.relax_ms / 1e3
+    achieved = sweep_bytes / relax_s / 1e9 if relax_s > 0 else 0.0
+    avg_launch_us = rt.relax_ms * 1e3 / max(rt.relax_launches, 1)
+
+    out = {
+        "metric": METRIC,
+        "value": round(pairs_per_s, 3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, ground-truth flow)",
+        "config": {
+            "workload": "640x480 RGB-D pair, 5-level pyramid ratio 0.5, 15 outer x 50 inner, HPB regularisation",
+            "precision": "reference mixed: f64 images/derivatives/flow, f32 system/increments",
+            "relax": args.variant,
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "sweeps_exec": [rt.sweeps_exec[k] for k in range(rt.n_levels)],
+            "outer_exec": [rt.outer_exec[k] for k in range(rt.n_levels)],
+            "algorithmic_GBps_pair": round((sweep_bytes + stage_bytes) / (np.percentile(lat, 50) / 1e3) / 1e9, 1),
+            "relax_launches_per_pair": rt.relax_launches,
+            "relax_avg_launch_us": round(avg_launch_us, 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        v, n, el = cpu_baseline(im1, d1, im2, d2, hpb, args.cpu_budget)
+        out["cpu_baseline"] = {"value": round(v, 4), "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} whole pairs of the same 640x480 workload through the oracle "
+                                         f"(reference precision, 1 thread, {el:.1f} s)"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
