@@ -68,8 +68,10 @@ typedef struct rgbdflow_params {
   int relax_variant;             /* RGBDFLOW_RELAX_* */
   int inner_exit;                /* 1: relaxation early exits as the reference (.cu:914-922) */
   int outer_exit;                /* 1: outer early exit as the reference (.cu:943-951) */
-  int tile_iters;                /* iterations per launch of the tiled relaxation (0 = default) */
-  int reserved[7];
+  int tile_iters;                /* iterations per launch of the tiled relaxation (0 = planner) */
+  int tile_w, tile_h;            /* interior tile of the tiled relaxation (0 = planner) */
+  int tile_config;               /* workgroup shape of the tiled relaxation (-1 = planner) */
+  int reserved[4];
 } rgbdflow_params;
 
 typedef struct rgbdflow_stats {
@@ -181,9 +183,9 @@ int rgbdflow_level_system(rgbdflow_ctx* ctx, int W, int H, float focal_level, co
                           double gamma, const double* depthMagTermWeights, const double* regularizationWeights,
                           double* coef13);
 /* n_iter red-black iterations (red then black) on a compact system; x: [3][N]
- * planes in/out.  relax_variant: RGBDFLOW_RELAX_*; tile_iters as in params. */
-int rgbdflow_relax(rgbdflow_ctx* ctx, const double* coef13, int W, int H, int n_iter, int relax_variant,
-                   int tile_iters, double* x);
+ * planes in/out.  schedule: relax_variant and tile_* of params (NULL: plain). */
+int rgbdflow_relax(rgbdflow_ctx* ctx, const double* coef13, int W, int H, int n_iter,
+                   const rgbdflow_params* schedule, double* x);
 
 #ifdef __cplusplus
 }

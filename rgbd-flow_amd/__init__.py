@@ -44,7 +44,8 @@ class Params(C.Structure):
                 ("alpha", C.c_double), ("gamma", C.c_double), ("pyramid_ratio", C.c_double),
                 ("min_width", C.c_int), ("n_outer", C.c_int), ("n_inner", C.c_int), ("n_sor", C.c_int),
                 ("relax_variant", C.c_int), ("inner_exit", C.c_int), ("outer_exit", C.c_int),
-                ("tile_iters", C.c_int), ("reserved", C.c_int * 7)]
+                ("tile_iters", C.c_int), ("tile_w", C.c_int), ("tile_h", C.c_int), ("tile_config", C.c_int),
+                ("reserved", C.c_int * 4)]
 
 
 class Stats(C.Structure):
@@ -92,7 +93,7 @@ class RgbdFlowError(RuntimeError):
 
 def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
                 ratio=0.5, min_width=None, reg=REG_HPB, cw=1.0, dvc=1.0, inner_exit=True, outer_exit=True,
-                relax_variant=RELAX_PLAIN, tile_iters=0) -> Params:
+                relax_variant=RELAX_PLAIN, tile_iters=0, tile_w=0, tile_h=0, tile_config=-1) -> Params:
     p = Params()
     lib().rgbdflow_default_params(C.byref(p))
     p.regularization_type = reg
@@ -111,6 +112,7 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.outer_exit = int(outer_exit)
     p.relax_variant = relax_variant
     p.tile_iters = tile_iters
+    p.tile_w, p.tile_h, p.tile_config = tile_w, tile_h, tile_config
     return p
 
 
@@ -238,11 +240,12 @@ class Context:
             _p(out)))
         return out
 
-    def relax(self, coef13, n_iter, x, relax_variant=RELAX_PLAIN, tile_iters=0):
+    def relax(self, coef13, n_iter, x, schedule: Params | None = None):
         c = _f64(coef13)
         _, H, W = c.shape
         xx = _f64(x).copy()
-        self._check(lib().rgbdflow_relax(self.h, _p(c), W, H, n_iter, relax_variant, tile_iters, _p(xx)))
+        self._check(lib().rgbdflow_relax(self.h, _p(c), W, H, n_iter, C.byref(schedule) if schedule else None,
+                                         _p(xx)))
         return xx
 
     def solve_level(self, li, params, allow_early, u, v, w):

@@ -35,6 +35,8 @@ struct LevelShape {
 
 struct rgbdflow_ctx {
   int device = 0;
+  int cu_count = 256;
+  size_t lds_per_block = 65536;
   hipStream_t stream = nullptr;
   unsigned long long budget = 0;
   unsigned long long in_use = 0;
@@ -53,6 +55,7 @@ struct rgbdflow_ctx {
   bool p_sigma_given = false;
   bool p_hpb_given = false;
   std::vector<LevelShape> p_levels;
+  std::map<std::string, TileSched> plans;  // tile schedules by level shape and request
   int p_final_slot = 0;  // which of the two flow buffers holds the finest result
 };
 
@@ -192,7 +195,7 @@ float downsized_focal(const rgbdflow_camera& c, int width) {
 // Level work buffers, sized for N pixels.
 struct Work {
   double *I1, *I2, *Iw, *Dw, *u[2], *Ix, *Iy, *It, *Dx, *Dy, *Dt, *phi, *partials;
-  float *du, *dW, *sW, *mW, *rW, *cW, *q, *coef;
+  float *du, *du_alt, *dW, *sW, *mW, *rW, *cW, *q, *coef;
   SolveCtl* ctl;  // kMaxLevels entries
 };
 
@@ -212,6 +215,7 @@ int get_work(rgbdflow_ctx* c, size_t N, Work& w) {
   TRY(get(c, "phi", N, &w.phi));
   TRY(get(c, "partials", (size_t)stat_blocks((int)N) + 16, &w.partials));
   TRY(get(c, "du", 3 * N, &w.du));
+  TRY(get(c, "du_alt", 3 * N, &w.du_alt));
   TRY(get(c, "dW", N, &w.dW));
   TRY(get(c, "sW", N, &w.sW));
   TRY(get(c, "mW", N, &w.mW));
@@ -237,6 +241,7 @@ LevelView make_view(const Work& w, int W, int H, const double* I1, const double*
   v.Dw = w.Dw;
   v.u = u;
   v.du = w.du;
+  v.du_alt = w.du_alt;
   v.dW = w.dW;
   v.sW = w.sW;
   v.mW = w.mW;
@@ -275,9 +280,119 @@ void relax_event(rgbdflow_ctx* c, bool begin) {
   (void)begin;
 }
 
+struct RelaxPlan {
+  int variant = RGBDFLOW_RELAX_PLAIN;
+  TileSched ts{};
+};
+
+// Tile schedule of the tiled relaxation for a W x H level.  Explicit tile_*
+// parameters win; otherwise a cost model picks the workgroup shape, the tile
+// and the iterations per launch: a tile covering the whole level runs a whole
+// check interval per launch; otherwise cost ~ launches x (launch overhead +
+// waves x per-workgroup work), where the per-workgroup work grows with the
+// region (tile + halo of 2 T) per thread.
+TileSched plan_tiles(const rgbdflow_ctx* c, int W, int H, int req_T, int req_w, int req_h, int req_cfg) {
+  TileSched best{};
+  double best_cost = 1e300;
+  const int n_cfg = tiled_config_count();
+  for (int cfg = 0; cfg < n_cfg; cfg++) {
+    if (req_cfg >= 0 && cfg != req_cfg) continue;
+    if (tiled_config_lds(cfg) > c->lds_per_block) continue;
+    int nt, p;
+    tiled_config_shape(cfg, &nt, &p);
+    const int cap = nt * p;
+    const int lds_occ = (int)std::max<size_t>(1, (160 * 1024) / tiled_config_lds(cfg));
+    const int occ = std::max(1, std::min(lds_occ, 2048 / nt));
+    const long long pairs_img = (long long)((W + 1) / 2) * H;
+    if (pairs_img <= cap && req_w == 0) {
+      const double per_pass = std::ceil((double)pairs_img / nt) * 0.06 + 0.25;
+      const double cost = 2 * (4.0 + std::ceil((double)pairs_img / nt) * 0.6 + 50 * per_pass);
+      if (cost < best_cost) {
+        best_cost = cost;
+        best = TileSched{W + (W & 1), H, 1, 1, 1, cap, cfg, kCheckEvery};
+      }
+      continue;
+    }
+    const int Ts[] = {2, 3, 4, 5, 6, 8};
+    for (int T : Ts) {
+      if (req_T > 0 && T != req_T) continue;
+      const int h = 2 * T;
+      for (int TW = 8; TW <= W + 7; TW += 8) {
+        if (req_w > 0 && TW != req_w + (req_w & 1)) continue;
+        for (int TH = 4; TH <= H + 3; TH += 4) {
+          if (req_h > 0 && TH != req_h) continue;
+          const int rw = std::min(TW + 2 * h, W + (W & 1)), rh = std::min(TH + 2 * h, H);
+          const long long rp = (long long)((rw + 1) / 2) * rh;
+          if (rp > cap) continue;
+          const int tiles = ((W + TW - 1) / TW) * ((H + TH - 1) / TH);
+          const double waves = std::ceil((double)tiles / (c->cu_count * occ));
+          const double per_thread = std::ceil((double)rp / nt);
+          const double per_wg = per_thread * 0.6 + 2 * T * (per_thread * 0.06 + 0.25);
+          const double launches = std::ceil((double)kCheckEvery / T) * 2;
+          const double cost = launches * (4.0 + waves * per_wg * occ);
+          if (cost < best_cost) {
+            best_cost = cost;
+            best = TileSched{TW, TH, (W + TW - 1) / TW, (H + TH - 1) / TH, 0, cap, cfg, T};
+          }
+        }
+      }
+    }
+  }
+  return best;
+}
+
+RelaxPlan make_relax_plan(rgbdflow_ctx* c, int W, int H, const rgbdflow_params* p) {
+  RelaxPlan r;
+  if (!p || p->relax_variant != RGBDFLOW_RELAX_TILED) return r;
+  char key[96];
+  std::snprintf(key, sizeof key, "%d,%d,%d,%d,%d,%d", W, H, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
+  auto it = c->plans.find(key);
+  if (it == c->plans.end())
+    it = c->plans.emplace(key, plan_tiles(c, W, H, p->tile_iters, p->tile_w, p->tile_h, p->tile_config)).first;
+  r.ts = it->second;
+  if (r.ts.cap == 0) return r;  // nothing fits: plain schedule
+  r.variant = RGBDFLOW_RELAX_TILED;
+  return r;
+}
+
+// nSOR red-black iterations of one (outer, inner), with the convergence
+// checks of .cu:905-922 at iterations 24, 49, ...; the tiled schedule splits
+// the iterations into launches that end at every check iteration.
+void enqueue_relax(rgbdflow_ctx* c, const LevelView& v, const SolveScalars& k, const RelaxPlan& plan, int slot_base) {
+  hipStream_t s = c->stream;
+  if (plan.variant == RGBDFLOW_RELAX_TILED) {
+    int it = 0;
+    float* cur = v.du;
+    float* nxt = v.du_alt;
+    while (it < k.n_sor) {
+      const int boundary = (it / kCheckEvery + 1) * kCheckEvery;
+      const int T = std::min(std::min(plan.ts.tmax, boundary - it), k.n_sor - it);
+      const int last = it + T - 1;
+      const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
+      const int slot = (last % kCheckEvery == kCheckEvery - 1) ? slot_base + last / kCheckEvery : -1;
+      relax_event(c, true);
+      launch_relax_tiled(s, v, plan.ts, cur, nxt, T, slot_base, n_prev, slot);
+      relax_event(c, false);
+      std::swap(cur, nxt);
+      it += T;
+    }
+    if (cur != v.du) launch_copy_f32(s, cur, (size_t)3 * v.N, v.du);
+    return;
+  }
+  for (int it = 0; it < k.n_sor; it++) {
+    const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
+    const int slot = (it % kCheckEvery == kCheckEvery - 1) ? slot_base + it / kCheckEvery : -1;
+    relax_event(c, true);
+    launch_relax_plain(s, v, 0, 0, slot_base, n_prev, slot);
+    relax_event(c, false);
+    relax_event(c, true);
+    launch_relax_plain(s, v, 1, 0, slot_base, n_prev, slot);
+    relax_event(c, false);
+  }
+}
+
 // sceneFlowSingleScale's loops (.cu:755-954) enqueued on the device.
-void enqueue_level_solve(rgbdflow_ctx* c, const LevelView& v, const SolveScalars& k, int relax_variant,
-                         int tile_iters) {
+void enqueue_level_solve(rgbdflow_ctx* c, const LevelView& v, const SolveScalars& k, const RelaxPlan& plan) {
   hipStream_t s = c->stream;
   const int checks = k.n_sor / kCheckEvery;
   launch_ctl_reset(s, v.ctl);
@@ -286,19 +401,7 @@ void enqueue_level_solve(rgbdflow_ctx* c, const LevelView& v, const SolveScalars
     for (int h = 0; h < k.n_inner; h++) {
       launch_phi(s, v, o);
       launch_system(s, v, k, o);
-      const int slot_base = h * checks;
-      (void)relax_variant;
-      (void)tile_iters;
-      for (int it = 0; it < k.n_sor; it++) {
-        const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
-        const int slot = (it % kCheckEvery == kCheckEvery - 1) ? slot_base + it / kCheckEvery : -1;
-        relax_event(c, true);
-        launch_relax_plain(s, v, 0, o, slot_base, n_prev, slot);
-        relax_event(c, false);
-        relax_event(c, true);
-        launch_relax_plain(s, v, 1, o, slot_base, n_prev, slot);
-        relax_event(c, false);
-      }
+      enqueue_relax(c, v, k, plan, h * checks);
     }
     launch_update_warp(s, v, k, o);
   }
@@ -427,6 +530,7 @@ int rgbdflow_default_params(rgbdflow_params* p) {
   p->relax_variant = RGBDFLOW_RELAX_PLAIN;
   p->inner_exit = 1;
   p->outer_exit = 1;
+  p->tile_config = -1;
   return RGBDFLOW_OK;
 }
 
@@ -445,6 +549,10 @@ int rgbdflow_create(rgbdflow_ctx** out, int device, unsigned long long budget) {
     delete c;
     return RGBDFLOW_ERR_HIP;
   }
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0) c->cu_count = v;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+    c->lds_per_block = (size_t)v;
   *out = c;
   return RGBDFLOW_OK;
 }
@@ -508,7 +616,7 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
   LevelView view = make_view(wk, W, H, wk.I1, wk.I2, D1, D2, wk.u[0], 0);
   c->relax_ev_used = 0;
   HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-  enqueue_level_solve(c, view, k, opts ? opts->relax_variant : RGBDFLOW_RELAX_PLAIN, opts ? opts->tile_iters : 0);
+  enqueue_level_solve(c, view, k, make_relax_plan(c, W, H, opts));
   HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
   TRY(download(c, u, wk.u[0], N * 8));
   TRY(download(c, v, wk.u[0] + N, N * 8));
@@ -558,7 +666,7 @@ int rgbdflow_level_system(rgbdflow_ctx* c, int W, int H, float focal_level, cons
   return RGBDFLOW_OK;
 }
 
-int rgbdflow_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, int n_iter, int relax_variant, int tile_iters,
+int rgbdflow_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, int n_iter, const rgbdflow_params* sched,
                    double* x) {
   if (!c) return RGBDFLOW_ERR_ARG;
   if (bad_dims(W, H) || !coef13 || !x || n_iter < 0) return fail(c, RGBDFLOW_ERR_ARG, "bad relax arguments");
@@ -573,12 +681,13 @@ int rgbdflow_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, int n_it
   TRY(upload(c, wk.du, hx.data(), hx.size() * 4));
   LevelView view = make_view(wk, W, H, wk.I1, wk.I2, nullptr, nullptr, wk.u[0], 0);
   launch_ctl_reset(c->stream, view.ctl);
-  (void)relax_variant;
-  (void)tile_iters;
-  for (int it = 0; it < n_iter; it++) {
-    launch_relax_plain(c->stream, view, 0, 0, 0, 0, -1);
-    launch_relax_plain(c->stream, view, 1, 0, 0, 0, -1);
-  }
+  const RelaxPlan plan = make_relax_plan(c, W, H, sched);
+  SolveScalars k{};
+  k.n_sor = n_iter;
+  k.inner_exit = 0;
+  // checks are recorded but never stop this call (inner_exit = 0)
+  if (n_iter / kCheckEvery + 1 > kMaxChecks) return fail(c, RGBDFLOW_ERR_ARG, "n_iter too large");
+  enqueue_relax(c, view, k, plan, 0);
   TRY(download(c, hx.data(), wk.du, hx.size() * 4));
   TRY(sync(c));
   for (size_t i = 0; i < hx.size(); i++) x[i] = hx[i];
@@ -865,7 +974,7 @@ int rgbdflow_pair_run(rgbdflow_ctx* c, const rgbdflow_params* p, rgbdflow_stats*
     launch_level_weights(s, PDel[k], PS[k], hpb ? hpb + hpb_off[k] : nullptr, p->regularization_type, PD1[k], Wk, Hk,
                          (int)p->cam.xRes, p->cam.focalLength, fk, wk.dW, wk.sW, wk.mW, wk.rW, wk.cW);
     ks[k] = scalars_from(p, p->color_data_weight, p->alpha, p->gamma, p->n_outer, p->n_inner, p->n_sor, k < nL - 1);
-    enqueue_level_solve(c, view, ks[k], p->relax_variant, p->tile_iters);
+    enqueue_level_solve(c, view, ks[k], make_relax_plan(c, Wk, Hk, p));
   }
   c->p_final_slot = cur;
   HIP_TRY(c, hipEventRecord(c->ev1, s));
@@ -968,7 +1077,7 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   launch_level_weights(s, del0, sm0, h, p->regularization_type, d1, W, H, (int)p->cam.xRes, p->cam.focalLength,
                        p->cam.focalLength, wk.dW, wk.sW, wk.mW, wk.rW, wk.cW);
   const SolveScalars k = scalars_from(p, p->color_data_weight, p->alpha, p->gamma, p->n_outer, p->n_inner, p->n_sor, 0);
-  enqueue_level_solve(c, view, k, p->relax_variant, p->tile_iters);
+  enqueue_level_solve(c, view, k, make_relax_plan(c, W, H, p));
   HIP_TRY(c, hipEventRecord(c->ev1, s));
   TRY(download(c, u, wk.u[0], N * 8));
   TRY(download(c, v, wk.u[0] + N, N * 8));

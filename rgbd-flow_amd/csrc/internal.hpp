@@ -45,6 +45,7 @@ struct LevelView {
   double* Dw;           // [N] frame-2 depth warped by u, minus w
   double* u;            // [3][N] flow (u px, v px, w m)
   float* du;            // [3][N] increment solved by the relaxation
+  float* du_alt;        // [3][N] second buffer of the tiled relaxation (reads one, writes the other)
   const float* dW;      // [N] depth-data weights (delta)
   const float* sW;      // [N] depth-smoothness weights
   const float* mW;      // [N] depth-magnitude weights
@@ -61,6 +62,16 @@ struct LevelView {
   float* coef;          // [13][N]
   double* partials;     // per-block partial sums of the outer statistic
   SolveCtl* ctl;
+};
+
+// Tile schedule of the tiled relaxation (kernels_relax_tiled.hip).
+struct TileSched {
+  int TW, TH;            // interior tile size (TW even)
+  int tiles_x, tiles_y;
+  int single;            // 1: one tile covers the level (no halo, up to 25 iterations per launch)
+  int cap;               // pair capacity of a workgroup (LDS plane length)
+  int config;            // kernel configuration index
+  int tmax;              // iterations per launch
 };
 
 struct SolveScalars {
@@ -81,6 +92,14 @@ void launch_relax_plain(hipStream_t s, const LevelView& v, int colour, int outer
                         int n_prev_checks, int check_slot);
 void launch_update_warp(hipStream_t s, const LevelView& v, const SolveScalars& k, int outer);
 int stat_blocks(int N);
+
+// kernels_relax_tiled.hip
+int tiled_config_count();
+void tiled_config_shape(int idx, int* nt, int* p);
+size_t tiled_config_lds(int idx);
+void launch_relax_tiled(hipStream_t s, const LevelView& v, const TileSched& ts, const float* src, float* dst,
+                        int n_iter, int slot_base, int n_prev_checks, int check_slot);
+void launch_copy_f32(hipStream_t s, const float* src, size_t n, float* dst);
 
 // kernels_pyramid.hip
 void launch_deinterleave(hipStream_t s, const double* src, int N, int C, double* dst);

@@ -12,6 +12,7 @@
 // entry and returns when the outer loop or the current relaxation has stopped,
 // so the host enqueues a whole level (or pair) without waiting on it.
 #include "internal.hpp"
+#include "relax_pixel.hpp"
 
 namespace rgbdflow {
 
@@ -26,11 +27,6 @@ __device__ __forceinline__ float sqr_f32(double x) {
 }
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
-
-__device__ __forceinline__ float wave_max(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
-}
 
 __global__ void k_ctl_reset(SolveCtl* ctl) {
   unsigned int* p = reinterpret_cast<unsigned int*>(ctl);
@@ -206,19 +202,9 @@ __global__ void __launch_bounds__(kBlock) k_system(LevelView v, SolveScalars s) 
   t = (float)(-(P * Dt + alpha * -wL)); C[kRw * (size_t)N + l] = (float)((double)t + -(-gamma * m * u2));
 }
 
-// The relaxation has stopped for this (outer, inner) if an earlier check
-// iteration saw max|change| < 1e-5 or >= 0.3 (.cu:914-922).
-__device__ __forceinline__ bool relax_stopped(const SolveCtl* ctl, int slot_base, int n_prev_checks) {
-  for (int c = 0; c < n_prev_checks; c++) {
-    const float md = __uint_as_float(ctl->maxdiff[slot_base + c]);
-    if ((double)md < 1e-5 || md >= .3f) return true;
-  }
-  return false;
-}
-
-// .cu:382-470: one colour of the red-black relaxation with omega = 0.7 and the
-// +-0.3 step clamp; the three unknowns of a pixel use each other's old values.
-// One thread per pixel of the colour; x = du, coefficients from the compact system.
+// .cu:382-470: one colour of the red-black relaxation, one thread per pixel of
+// the colour, neighbours read from global memory (the plain schedule: one
+// launch per colour per iteration, as the reference launches it).
 __global__ void __launch_bounds__(256) k_relax_plain(LevelView v, int colour, int slot_base, int n_prev_checks,
                                                      int check_slot) {
   if (v.ctl->outer_done) return;
@@ -230,45 +216,24 @@ __global__ void __launch_bounds__(256) k_relax_plain(LevelView v, int colour, in
   float change = 0.f;
   if (i < H && j < W) {
     const int l = i * W + j;
-    const float* C = v.coef;
+    const bool hl = j > 0, hu = i > 0, hr = j < W - 1, hd = i < H - 1;
+    const PixCoef c = load_coef(v.coef, N, W, l, hl, hu);
     float* x = v.du;
-    const float xo0 = x[l], xo1 = x[N + l], xo2 = x[2 * N + l];
-    const float xo[3] = {xo0, xo1, xo2};
-    float xn[3];
+    float xo[3], xl[3], xu[3], xr[3], xd[3], xn[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const float* xk = x + (size_t)k * N;
-      const float* eR = C + (size_t)(k < 2 ? kERxy : kERz) * N;
-      const float* eD = C + (size_t)(k < 2 ? kEDxy : kEDz) * N;
-      float nv = 0.f;
-      if (j > 0) nv -= eR[l - 1] * xk[l - 1];
-      if (i > 0) nv -= eD[l - W] * xk[l - W];
-      if (j < W - 1) nv -= eR[l] * xk[l + 1];
-      if (i < H - 1) nv -= eD[l] * xk[l + W];
-      nv += C[(size_t)(kRu + k) * N + l];
-      if (k == 0) {
-        nv -= C[kBuv * (size_t)N + l] * xo[1];
-        nv -= C[kBuw * (size_t)N + l] * xo[2];
-        nv /= C[kBuu * (size_t)N + l];
-      } else if (k == 1) {
-        nv -= C[kBuv * (size_t)N + l] * xo[0];
-        nv -= C[kBvw * (size_t)N + l] * xo[2];
-        nv /= C[kBvv * (size_t)N + l];
-      } else {
-        nv -= C[kBuw * (size_t)N + l] * xo[0];
-        nv -= C[kBvw * (size_t)N + l] * xo[1];
-        nv /= C[kBww * (size_t)N + l];
-      }
-      const float omega = .7f;
-      float r = omega * nv + (1 - omega) * xo[k];
-      if (r < xo[k] - .3f) r = xo[k] - .3f;
-      else if (r > xo[k] + .3f) r = xo[k] + .3f;
-      xn[k] = r;
+      xo[k] = xk[l];
+      xl[k] = hl ? xk[l - 1] : 0.f;
+      xu[k] = hu ? xk[l - W] : 0.f;
+      xr[k] = hr ? xk[l + 1] : 0.f;
+      xd[k] = hd ? xk[l + W] : 0.f;
     }
+    relax_update(c, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn);
     x[l] = xn[0];
     x[N + l] = xn[1];
     x[2 * N + l] = xn[2];
-    change = fmaxf(fmaxf(fabsf(xn[0] - xo0), fabsf(xn[1] - xo1)), fabsf(xn[2] - xo2));
+    change = fmaxf(fmaxf(fabsf(xn[0] - xo[0]), fabsf(xn[1] - xo[1])), fabsf(xn[2] - xo[2]));
   }
   if (check_slot >= 0) {
     const float m = wave_max(change);
