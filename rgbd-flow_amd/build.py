@@ -58,6 +58,15 @@ def build(verbose: bool = False) -> str:
         _run(["g++", *CXX_FLAGS, *inc, "-shared", "-o", tmp, *cxx, "-L", HERE, "-lrgbdflow",
               f"-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, LIB_CXX)
+        # the drop-in API's test driver (run by tests/test_gpu_dropin.py)
+        drv_src = os.path.join(ROOT, "tests", "cxx", "test_dropin.cpp")
+        if os.path.exists(drv_src):
+            drv_dir = os.path.join(ROOT, "tests", "cxx", "build")
+            os.makedirs(drv_dir, exist_ok=True)
+            drv = os.path.join(drv_dir, "test_dropin")
+            _run(["g++", *CXX_FLAGS, *inc, "-I", os.path.join(ROOT, "include"), "-o", drv + ".tmp", drv_src,
+                  "-L", HERE, "-lrgbdflow_cxx", "-lrgbdflow", "-Wl,-rpath,$ORIGIN/../../../rgbd-flow_amd"])
+            os.replace(drv + ".tmp", drv)
     if verbose:
         print("built", LIB)
     return LIB

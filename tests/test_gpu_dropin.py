@@ -1,0 +1,66 @@
+"""The drop-in C++ API (librgbdflow_cxx.so) end to end, driven like the
+reference's demo: runFlowTwoFramesSceneFlowExperiments -> sceneFlowMultiscale
+-> writeSceneFlow (.flo3), compared with the oracle on the same inputs."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import ROOT, hpb_for, load_package, make_case, oracle
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(ROOT, "tests", "cxx", "build", "test_dropin")
+
+
+def _read_flo3(path):
+    with open(path, "rb") as f:
+        magic = f.read(8)
+        w, h = struct.unpack("<ii", f.read(8))
+        data = np.frombuffer(f.read(), np.float32)
+    assert magic == bytes([237, 159, 43, 26, 202, 94, 218, 68])
+    return data.reshape(h, w, 3)
+
+
+@pytest.mark.parametrize("relax_variant", [0, 1])
+def test_run_two_frames_matches_oracle(tmp_path, relax_variant):
+    load_package()
+    O = oracle()
+    W, H, f, cx, cy = 320, 240, 262.5, 159.5, 119.5
+    pair, im1, d1, im2, d2, sigma = make_case(W, H, 31, 4, f, cx, cy)
+    min_width = 60  # setUpSceneFlowParams (runFlowTwoFrames.cpp:72)
+    hpb, levels = hpb_for(pair, W, H, 0.75, min_width)  # sceneFlowParams' default ratio .75
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as fh:
+        fh.write(struct.pack("<iifff", W, H, f, cx, cy))
+        fh.write(pair["img1"].tobytes())
+        fh.write(d1.astype(np.float32).tobytes())
+        fh.write(pair["img2"].tobytes())
+        fh.write(d2.astype(np.float32).tobytes())
+        fh.write(struct.pack("<i", len(levels)))
+        off = 0
+        for (w, h) in levels:
+            fh.write(struct.pack("<ii", w, h))
+            fh.write(hpb[off:off + w * h * 2].astype(np.float32).tobytes())
+            off += w * h * 2
+    out = tmp_path / "flow.flo3"
+    max_iters = 30
+    r = subprocess.run([DRIVER, str(inp), str(out), str(max_iters), str(relax_variant)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    flow = _read_flo3(out)
+    # the oracle with runFlowTwoFrames' parameters (runFlowTwoFrames.cpp:66-100),
+    # fed the depth stdevs the driver computed on the GPU (those agree with the
+    # oracle's to 1 f32 ulp, test_gpu_parity.py::test_depth_stdevs; the flow is
+    # compared for the plumbing of the C++ layer)
+    R = load_package()
+    with R.Context(0) as ctx:
+        sigma = ctx.depth_stdevs(d1.astype(np.float32), 4)
+    p = O.make_params(W, H, f=f, cx=cx, cy=cy, alpha=.02, gamma=1e-5, n_outer=50, n_inner=1, n_sor=max_iters,
+                      ratio=.75, min_width=min_width)
+    u, v, w, st = O.multiscale(im1, d1, im2, d2, sigma, hpb, p)
+    ref = np.stack([u, v, w], -1).astype(np.float32)
+    assert np.max(np.abs(flow[..., :2] - ref[..., :2])) <= 1e-3
+    assert np.max(np.abs(flow[..., 2] - ref[..., 2])) <= 1e-5
