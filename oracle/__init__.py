@@ -265,6 +265,23 @@ def level_solve(li: LevelInputs, params: Params, allow_early, u, v, w):
     return u, v, w, st
 
 
+def warp_gpu_style(I1, D1, I2, D2, u, v, w, precision=PREC_REF):
+    I1, D1, I2, D2, u, v, w = map(_f64, (I1, D1, I2, D2, u, v, w))
+    H, W = D1.shape
+    wI = np.empty_like(I1)
+    wD = np.empty_like(D1)
+    lib().orc_warp_gpu_style(_p(I1), _p(D1), _p(I2), _p(D2), _p(u), _p(v), _p(w), W, H, precision, _p(wI), _p(wD))
+    return wI, wD
+
+
+def level_weights(li: LevelInputs, params: Params, u, v, w):
+    u, v, w = _f64(u), _f64(v), _f64(w)
+    phi = np.empty((li.H, li.W)); psi = np.empty((li.H, li.W, 5)); pi = np.empty((li.H, li.W))
+    s = li.struct()
+    lib().orc_level_weights(C.byref(s), C.byref(params), _p(u), _p(v), _p(w), _p(phi), _p(psi), _p(pi))
+    return phi, psi, pi
+
+
 def level_system(li: LevelInputs, params: Params, u, v, w):
     u, v, w = _f64(u), _f64(v), _f64(w)
     out = np.empty((13, li.H, li.W), np.float64)

@@ -858,6 +858,41 @@ void orc_level_system(const orc_level_in* in, const orc_params* p, const double*
   }
 }
 
+void orc_warp_gpu_style(const double* I1, const double* D1, const double* I2, const double* D2, const double* u,
+                        const double* v, const double* w, int W, int H, int precision, double* wI, double* wD) {
+  const size_t N = (size_t)W * H;
+  std::vector<double> z(N, 0.0), one(N * 2, 1.0);
+  orc_level_in in;
+  in.W = W; in.H = H; in.focal_level = 1;
+  in.I1 = I1; in.I2 = I2; in.warpI2 = I2; in.D1 = D1; in.D2 = D2; in.warpD2 = D2;
+  in.delta = z.data(); in.s = z.data(); in.m = z.data(); in.r = one.data();
+  orc_params p{};
+  auto run = [&](auto& S) {
+    for (size_t l = 0; l < N; l++) { S.u[l] = u[l]; S.u[N + l] = v[l]; S.u[2 * N + l] = w[l]; }
+    S.warp_gpu();
+    std::memcpy(wI, S.Iw.data(), N * 5 * sizeof(double));
+    std::memcpy(wD, S.Dw.data(), N * sizeof(double));
+  };
+  if (precision == ORC_PREC_REF) { Solver<float, true> S(&in, &p); run(S); }
+  else { Solver<double, false> S(&in, &p); run(S); }
+}
+
+void orc_level_weights(const orc_level_in* in, const orc_params* p, const double* u, const double* v,
+                       const double* w, double* phi, double* psi, double* pi) {
+  const size_t N = (size_t)in->W * in->H;
+  auto run = [&](auto& S) {
+    for (size_t l = 0; l < N; l++) { S.u[l] = u[l]; S.u[N + l] = v[l]; S.u[2 * N + l] = w[l]; }
+    S.derivatives();
+    S.occlusion();
+    S.penalties();
+    std::memcpy(phi, S.phi.data(), N * sizeof(double));
+    std::memcpy(psi, S.psi.data(), N * 5 * sizeof(double));
+    std::memcpy(pi, S.pi.data(), N * sizeof(double));
+  };
+  if (p->precision == ORC_PREC_REF) { Solver<float, true> S(in, p); run(S); }
+  else { Solver<double, false> S(in, p); run(S); }
+}
+
 void orc_relax(const double* coef13, int W, int H, int precision, int n_iter, double* x) {
   const size_t N = (size_t)W * H;
   if (precision == ORC_PREC_REF) {

@@ -96,6 +96,14 @@ int orc_level_solve(const orc_level_in* in, const orc_params* p, int allow_early
  * values are f32 stored in doubles. */
 void orc_level_system(const orc_level_in* in, const orc_params* p, const double* u,
                       const double* v, const double* w, double* coef13);
+/* The per-outer-iteration warp of the level solve (.cu:579-635): floor base,
+ * explicit clamps, f32 weights in ORC_PREC_REF (f64 in ORC_PREC_F64). */
+void orc_warp_gpu_style(const double* I1, const double* D1, const double* I2, const double* D2, const double* u,
+                        const double* v, const double* w, int W, int H, int precision, double* warpI2,
+                        double* warpD2);
+/* The robust weights at du = 0 (.cu:340-371): phi [N], psi [N][5], pi [N]. */
+void orc_level_weights(const orc_level_in* in, const orc_params* p, const double* u, const double* v,
+                       const double* w, double* phi, double* psi, double* pi);
 /* nIter red-black iterations on a compact system; x = [3][N] planes (in/out). */
 void orc_relax(const double* coef13, int W, int H, int precision, int n_iter, double* x);
 

@@ -327,3 +327,119 @@ def test_reference_vs_f64_precision_close():
     c = O.level_solve(li, O.make_params(48, 36, n_outer=2, n_sor=25), True, u, v, w)
     d = O.level_solve(li, O.make_params(48, 36, n_outer=2, n_sor=25, precision=O.PREC_F64), True, u, v, w)
     assert c[3].last_maxdiff >= np.float32(0.3) > d[3].last_maxdiff - 1e-7
+
+
+# ------------------------------------- the level-solve warp and the exits
+def test_warp_gpu_style_identities():
+    rng = np.random.default_rng(11)
+    H, W = 13, 17
+    I1, I2 = rng.uniform(size=(H, W, 5)), rng.uniform(size=(H, W, 5))
+    D1, D2 = rng.uniform(1, 3, (H, W)), rng.uniform(1, 3, (H, W))
+    z = np.zeros((H, W))
+    for prec in (O.PREC_REF, O.PREC_F64):
+        wI, wD = O.warp_gpu_style(I1, D1, I2, D2, z, z, z, prec)
+        assert np.array_equal(wI, I2) and np.array_equal(wD, D2)
+        wI, wD = O.warp_gpu_style(I1, D1, I2, D2, -np.ones((H, W)), z, 0.5 * np.ones((H, W)), prec)
+        assert np.array_equal(wI[:, 1:], I2[:, :-1]) and np.array_equal(wI[:, 0], I1[:, 0])
+        assert np.array_equal(wD[:, 1:], D2[:, :-1] - 0.5) and np.array_equal(wD[:, 0], D1[:, 0] - 0.5)
+    # f32 bilinear weights (REF) vs f64 weights: both reproduce a linear image, to f32 accuracy
+    yy, xx = np.mgrid[0:H, 0:W].astype(float)
+    lin = 0.3 * xx - 0.2 * yy + 1
+    u, v = rng.uniform(-2, 2, (H, W)), rng.uniform(-2, 2, (H, W))
+    x, y = xx + u, yy + v
+    inb = (x >= 0) & (x <= W - 1) & (y >= 0) & (y <= H - 1)
+    I = np.repeat(lin[..., None], 5, -1)
+    _, a = O.warp_gpu_style(I, lin, I, lin, u, v, z, O.PREC_F64)
+    _, b = O.warp_gpu_style(I, lin, I, lin, u, v, z, O.PREC_REF)
+    exact = 0.3 * x - 0.2 * y + 1
+    assert np.allclose(a[inb], exact[inb], rtol=0, atol=1e-12)
+    assert np.allclose(b[inb], exact[inb], rtol=0, atol=1e-5)
+    assert not np.allclose(b[inb], exact[inb], rtol=0, atol=1e-12)   # the weights really are f32
+
+
+def test_exits_on_a_converged_system():
+    """Identical frames give a zero system: the relaxation stops at the first
+    check (iteration 25, max change 0 < 1e-5) and, where allowed, the outer
+    loop after its first iteration (statistic 0 < 4e-4)."""
+    li, (u, v, w), _ = level_inputs(seed=13, W=32, H=24, level_flow=False)
+    li2 = O.LevelInputs(li.I1, li.I1, li.I1, li.D1, li.D1, li.D1, li.delta, li.s, li.m, li.r, li.focal_level)
+    z = np.zeros((24, 32))
+    _, _, _, st = O.level_solve(li2, O.make_params(32, 24, n_outer=5, n_sor=60), True, z, z, z)
+    assert (st.outer_exec, st.sweeps_exec) == (1, 25)
+    _, _, _, st = O.level_solve(li2, O.make_params(32, 24, n_outer=5, n_sor=60), False, z, z, z)
+    assert (st.outer_exec, st.sweeps_exec) == (5, 125)
+    _, _, _, st = O.level_solve(li2, O.make_params(32, 24, n_outer=5, n_sor=60, disable_inner_exit=True,
+                                                    disable_outer_exit=True), True, z, z, z)
+    assert (st.outer_exec, st.sweeps_exec) == (5, 300)
+
+
+# ------------------------------------------- assembly vs the lagged energy
+def _derivs(A, B):
+    """x/y derivatives of 0.4 A + 0.6 B with [1,-8,0,8,-1]/12, clamp-to-edge."""
+    M = 0.4 * A + 0.6 * B
+    H, W = M.shape[:2]
+    g = np.array([1, -8, 0, 8, -1]) / 12.0
+    dx = sum(g[m] * M[:, np.clip(np.arange(W) - 2 + m, 0, W - 1)] for m in range(5))
+    dy = sum(g[m] * M[np.clip(np.arange(H) - 2 + m, 0, H - 1)] for m in range(5))
+    return dx, dy
+
+
+def test_system_is_the_lagged_energy_gradient():
+    """With q = 1 (no occlusion), isotropic regularisation and a constant
+    depth-smoothness weight, the reference's quirks (q^2 on the It products
+    only, the Laplacian normalisation, edge-owner weights) vanish and the
+    assembled system is exactly the linearised Euler-Lagrange system of the
+    quadratic energy with frozen robust weights:
+        b - M du = grad E(du),
+        E(du) = 1/2 sum_p [ <psi (It + Ix du + Iy dv)^2> + pi (Dt + Dx du + Dy dv - dw)^2
+                            + gamma ((u+du)^2 + (v+dv)^2 + m (w+dw)^2) ]
+              + alpha/2 sum_edges phi_o r_o [ d(u+du)^2 + d(v+dv)^2 + s d(w+dw)^2 ].
+    A dropped term, a wrong sign or a transposed neighbour breaks the equality."""
+    rng = np.random.default_rng(14)
+    H, W = 6, 8
+    yy, xx = np.mgrid[0:H, 0:W].astype(float)
+    I1 = np.stack([np.sin(0.5 * xx + k) * np.cos(0.3 * yy - k) + 0.1 * k for k in range(5)], -1)
+    I2 = I1 + 0.05 * rng.normal(size=I1.shape)
+    Iw = I1 + 0.05 * rng.normal(size=I1.shape)
+    D1 = 1.5 + 0.002 * xx + 0.003 * yy
+    D2 = D1 + 1.0                                     # q = 1 everywhere
+    Dw = D1 + 0.002 * rng.normal(size=D1.shape)
+    s_const, alpha, gamma = 3.0, 0.7, 0.05
+    delta = rng.uniform(0.5, 1.5, (H, W))
+    m = rng.uniform(1, 2, (H, W))
+    r = O.reg_weights(None, D1, O.REG_ISOTROPIC)
+    u, v, w = (0.1 * rng.normal(size=(H, W)) for _ in range(3))
+    li = O.LevelInputs(I1, I2, Iw, D1, D2, Dw, delta, np.full((H, W), s_const), m, r, np.float32(1.0))
+    p = O.make_params(W, H, alpha=alpha, gamma=gamma, precision=O.PREC_F64, reg=O.REG_ISOTROPIC)
+    coef = O.level_system(li, p, u, v, w)
+    M, b = _sparse_system(coef)
+    phi, psi, pi = O.level_weights(li, p, u, v, w)
+    Ix, Iy = _derivs(I1, Iw)
+    It = Iw - I1
+    Dx, Dy = _derivs(D1, Dw)
+    Dt = Dw - D1
+    assert max(np.abs(Dx).max(), np.abs(Dy).max(), np.abs(Dt).max()) < 0.04   # clamp inactive
+    N = H * W
+
+    def energy(x):
+        du, dv, dw = x[:N].reshape(H, W), x[N:2 * N].reshape(H, W), x[2 * N:].reshape(H, W)
+        rc = It + Ix * du[..., None] + Iy * dv[..., None]
+        e = 0.5 * np.sum(np.mean(psi * rc ** 2, -1))
+        e += 0.5 * np.sum(pi * (Dt + Dx * du + Dy * dv - dw) ** 2)
+        e += 0.5 * gamma * np.sum((u + du) ** 2 + (v + dv) ** 2 + m * (w + dw) ** 2)
+        U, V, Wz = u + du, v + dv, w + dw
+        wr = (phi * r[..., 0])[:, :-1]
+        wd = (phi * r[..., 1])[:-1, :]
+        for F, sc in ((U, 1.0), (V, 1.0), (Wz, s_const)):
+            e += 0.5 * alpha * sc * (np.sum(wr * np.diff(F, axis=1) ** 2) + np.sum(wd * np.diff(F, axis=0) ** 2))
+        return e
+
+    for trial in range(3):
+        x = 0.05 * rng.normal(size=3 * N)
+        grad = np.empty(3 * N)
+        for i in range(3 * N):
+            e = np.zeros(3 * N)
+            e[i] = 1.0
+            grad[i] = (energy(x + e) - energy(x - e)) / 2.0
+        lhs = b - M @ x
+        assert np.max(np.abs(lhs - grad)) < 1e-9 * max(1.0, np.abs(grad).max()), np.max(np.abs(lhs - grad))
