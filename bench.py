@@ -87,7 +87,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--variant", choices=["plain", "tiled"], default="plain")
+    ap.add_argument("--variant", choices=["plain", "tiled"], default="tiled")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=5, help="steps of the plain schedule for comparison")
     ap.add_argument("--latency-steps", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -101,6 +103,15 @@ def main():
     pair, im1, d1, im2, d2, hpb, levels = workload(R)
     p = params_for(R, args.variant)
     ctx = R.Context(0)
+    tuned = []
+    if args.variant == "tiled" and not args.no_autotune:
+        # one-time schedule search per level shape (outside the timed region)
+        for (w, h) in levels:
+            ch = ctx.autotune_relax(w, h)
+            tuned.append(dict(level=f"{w}x{h}", tile=f"{ch['tile_w']}x{ch['tile_h']}", iters=ch["tile_iters"],
+                              threads=ch["threads"], pairs_per_thread=ch["pairs_per_thread"],
+                              us_per_iter=round(ch["us_per_iter"], 2)))
+        print(f"bench: autotuned {tuned}", file=sys.stderr, flush=True)
     ctx.pair_load(p, im1, d1, im2, d2, None, hpb)   # stdevs computed on the device every step
     print(f"bench: 640x480 pair, {len(levels)} levels, 15x50, variant {args.variant}", file=sys.stderr, flush=True)
     st = None
@@ -118,6 +129,19 @@ def main():
     elapsed = time.perf_counter() - t0
     pairs_per_s = args.steps / elapsed
     print(f"bench: timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # the plain schedule (one launch per colour per iteration, as the reference
+    # launches it) on the same resident pair, for comparison
+    plain_pps = None
+    if args.variant != "plain" and args.plain_steps > 0:
+        pp = params_for(R, "plain")
+        ctx.pair_run(pp, stats=True)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            ctx.pair_run(pp, stats=False)
+        ctx.synchronize()
+        plain_pps = args.plain_steps / (time.perf_counter() - t1)
 
     # per-pair device latency (HIP events around each solve)
     lat = []
@@ -158,6 +182,9 @@ def main():
             "algorithmic_GBps_pair": round((sweep_bytes + stage_bytes) / (np.percentile(lat, 50) / 1e3) / 1e9, 1),
             "relax_launches_per_pair": rt.relax_launches,
             "relax_avg_launch_us": round(avg_launch_us, 3),
+            "plain_pairs_per_s": round(plain_pps, 3) if plain_pps else None,
+            "speedup_vs_plain": round(pairs_per_s / plain_pps, 2) if plain_pps else None,
+            "autotuned": tuned,
         },
         "roofline": {
             "bound": "hbm",

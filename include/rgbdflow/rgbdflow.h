@@ -187,6 +187,23 @@ int rgbdflow_level_system(rgbdflow_ctx* ctx, int W, int H, float focal_level, co
 int rgbdflow_relax(rgbdflow_ctx* ctx, const double* coef13, int W, int H, int n_iter,
                    const rgbdflow_params* schedule, double* x);
 
+/* ---- autotuning ---- */
+typedef struct rgbdflow_tile_choice {
+  int tile_w, tile_h;            /* interior tile */
+  int tile_iters;                /* red-black iterations per launch */
+  int tile_config;               /* workgroup shape index */
+  int threads, pairs_per_thread; /* the workgroup shape */
+  int single;                    /* 1: one tile covers the level */
+  int candidates;                /* schedules timed */
+  double us_per_iter;            /* measured device time per red-black iteration */
+} rgbdflow_tile_choice;
+
+/* Times candidate schedules of the tiled relaxation (workgroup shape, tile
+ * shape, iterations per launch) on a W x H level and makes the fastest the
+ * context's schedule for that shape whenever tile_* are left to the planner
+ * (tile_iters = tile_w = tile_h = 0, tile_config = -1).  out may be NULL. */
+int rgbdflow_autotune_relax(rgbdflow_ctx* ctx, int W, int H, rgbdflow_tile_choice* out);
+
 #ifdef __cplusplus
 }
 #endif

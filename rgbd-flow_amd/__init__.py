@@ -28,7 +28,7 @@ EXPORTS = [
     "rgbdflow_solve_level", "rgbdflow_depth_stdevs_pointwise", "rgbdflow_pair_load", "rgbdflow_pair_run",
     "rgbdflow_pair_read", "rgbdflow_multiscale", "rgbdflow_with_init", "rgbdflow_gaussian_pyramid",
     "rgbdflow_depth_map_pyramid", "rgbdflow_im2feature", "rgbdflow_warp_by_3d_flow", "rgbdflow_upsample_flow",
-    "rgbdflow_level_system", "rgbdflow_relax",
+    "rgbdflow_level_system", "rgbdflow_relax", "rgbdflow_autotune_relax",
 ]
 
 
@@ -56,6 +56,15 @@ class Stats(C.Structure):
     def levels(self):
         return [dict(W=self.width[k], H=self.height[k], outer=self.outer_exec[k], sweeps=self.sweeps_exec[k],
                      stat=self.last_stat[k]) for k in range(self.n_levels)]
+
+
+class TileChoice(C.Structure):
+    _fields_ = [("tile_w", C.c_int), ("tile_h", C.c_int), ("tile_iters", C.c_int), ("tile_config", C.c_int),
+                ("threads", C.c_int), ("pairs_per_thread", C.c_int), ("single", C.c_int), ("candidates", C.c_int),
+                ("us_per_iter", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 _lib = None
@@ -161,6 +170,13 @@ class Context:
 
     def synchronize(self):
         self._check(lib().rgbdflow_synchronize(self.h))
+
+    def autotune_relax(self, W, H):
+        """Time the tiled relaxation's schedules on a W x H level; the fastest
+        becomes this context's default for that shape."""
+        ch = TileChoice()
+        self._check(lib().rgbdflow_autotune_relax(self.h, W, H, C.byref(ch)))
+        return ch.as_dict()
 
     def set_relax_timing(self, on: bool):
         self._check(lib().rgbdflow_set_relax_timing(self.h, int(on)))

@@ -341,11 +341,16 @@ TileSched plan_tiles(const rgbdflow_ctx* c, int W, int H, int req_T, int req_w, 
   return best;
 }
 
+std::string plan_key(int W, int H, int T, int tw, int th, int cfg) {
+  char key[96];
+  std::snprintf(key, sizeof key, "%d,%d,%d,%d,%d,%d", W, H, T, tw, th, cfg);
+  return key;
+}
+
 RelaxPlan make_relax_plan(rgbdflow_ctx* c, int W, int H, const rgbdflow_params* p) {
   RelaxPlan r;
   if (!p || p->relax_variant != RGBDFLOW_RELAX_TILED) return r;
-  char key[96];
-  std::snprintf(key, sizeof key, "%d,%d,%d,%d,%d,%d", W, H, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
+  const std::string key = plan_key(W, H, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
   auto it = c->plans.find(key);
   if (it == c->plans.end())
     it = c->plans.emplace(key, plan_tiles(c, W, H, p->tile_iters, p->tile_w, p->tile_h, p->tile_config)).first;
@@ -1019,6 +1024,97 @@ int rgbdflow_multiscale(rgbdflow_ctx* c, int W, int H, const double* im1, const 
   rgbdflow_stats local;
   TRY(rgbdflow_pair_run(c, p, st ? st : &local));
   return rgbdflow_pair_read(c, u, v, w);
+}
+
+int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, rgbdflow_tile_choice* out) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (bad_dims(W, H)) return fail(c, RGBDFLOW_ERR_ARG, "bad autotune size");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const size_t N = (size_t)W * H;
+  Work wk;
+  TRY(get_work(c, N, wk));
+  LevelView view = make_view(wk, W, H, wk.I1, wk.I2, nullptr, nullptr, wk.u[0], 0);
+  hipStream_t s = c->stream;
+  launch_ctl_reset(s, view.ctl);
+  launch_fill_test_system(s, wk.coef, wk.du, W, H);
+  SolveScalars k{};
+  k.n_sor = kCheckEvery;  // one check interval: the unit the schedule repeats
+  k.inner_exit = 0;
+  // candidate schedules
+  std::vector<TileSched> cand;
+  const long long pairs_img = (long long)((W + 1) / 2) * H;
+  const int Ts[] = {1, 2, 3, 4, 5, 6, 8, 12};
+  const int TWs[] = {8, 16, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320};
+  for (int cfg = 0; cfg < tiled_config_count(); cfg++) {
+    if (tiled_config_lds(cfg) > c->lds_per_block) continue;
+    int nt, pp;
+    tiled_config_shape(cfg, &nt, &pp);
+    const int cap = nt * pp;
+    if (pairs_img <= cap) {
+      cand.push_back(TileSched{W + (W & 1), H, 1, 1, 1, cap, cfg, kCheckEvery});
+      continue;
+    }
+    for (int T : Ts) {
+      const int h = 2 * T;
+      for (int TW : TWs) {
+        if (TW > W + (W & 1)) continue;
+        const int rw = std::min(TW + 2 * h, W + (W & 1));
+        const int rows = cap / ((rw + 1) / 2);
+        const int thmax = std::min(rows - 2 * h, H);
+        if (thmax < 2) continue;
+        for (int TH : {thmax, thmax / 2}) {
+          if (TH < 2) continue;
+          if (((rw + 1) / 2) * (long long)std::min(TH + 2 * h, H) > cap) continue;
+          cand.push_back(TileSched{TW, TH, (W + TW - 1) / TW, (H + TH - 1) / TH, 0, cap, cfg, T});
+        }
+      }
+    }
+  }
+  if (cand.empty()) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "no tiled schedule fits this level");
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  double best_us = 1e300;
+  TileSched best{};
+  const bool timing = c->time_relax;
+  c->time_relax = false;
+  for (const TileSched& ts : cand) {
+    RelaxPlan plan;
+    plan.variant = RGBDFLOW_RELAX_TILED;
+    plan.ts = ts;
+    enqueue_relax(c, view, k, plan, 0);  // warm-up
+    double t_best = 1e300;
+    for (int rep = 0; rep < 3; rep++) {
+      hipEventRecord(e0, s);
+      enqueue_relax(c, view, k, plan, 0);
+      hipEventRecord(e1, s);
+      hipEventSynchronize(e1);
+      float ms = 0;
+      hipEventElapsedTime(&ms, e0, e1);
+      t_best = std::min(t_best, (double)ms);
+    }
+    const double us = t_best * 1e3 / kCheckEvery;
+    if (us < best_us) {
+      best_us = us;
+      best = ts;
+    }
+  }
+  c->time_relax = timing;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  TRY(sync(c));
+  c->plans[plan_key(W, H, 0, 0, 0, -1)] = best;
+  if (out) {
+    out->tile_w = best.TW;
+    out->tile_h = best.TH;
+    out->tile_iters = best.tmax;
+    out->tile_config = best.config;
+    tiled_config_shape(best.config, &out->threads, &out->pairs_per_thread);
+    out->single = best.single;
+    out->candidates = (int)cand.size();
+    out->us_per_iter = best_us;
+  }
+  return RGBDFLOW_OK;
 }
 
 int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const double* dm1, const double* im2,

@@ -100,6 +100,7 @@ size_t tiled_config_lds(int idx);
 void launch_relax_tiled(hipStream_t s, const LevelView& v, const TileSched& ts, const float* src, float* dst,
                         int n_iter, int slot_base, int n_prev_checks, int check_slot);
 void launch_copy_f32(hipStream_t s, const float* src, size_t n, float* dst);
+void launch_fill_test_system(hipStream_t s, float* coef, float* du, int W, int H);
 
 // kernels_pyramid.hip
 void launch_deinterleave(hipStream_t s, const double* src, int N, int C, double* dst);

@@ -180,6 +180,34 @@ __global__ void k_copy_f32(const float* src, size_t n, float* dst) {
     dst[i] = src[i];
 }
 
+// A well-conditioned synthetic system for timing schedules: unit-scale
+// diagonal, weak couplings, small rhs; du = 0.
+__global__ void k_fill_test_system(float* coef, float* du, int W, int H) {
+  const int N = W * H;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < N; l += gridDim.x * blockDim.x) {
+    const int j = l % W, i = l / W;
+    coef[kERxy * (size_t)N + l] = j < W - 1 ? 0.1f : 0.f;
+    coef[kERz * (size_t)N + l] = j < W - 1 ? 0.1f : 0.f;
+    coef[kEDxy * (size_t)N + l] = i < H - 1 ? 0.1f : 0.f;
+    coef[kEDz * (size_t)N + l] = i < H - 1 ? 0.1f : 0.f;
+    coef[kBuu * (size_t)N + l] = -1.f;
+    coef[kBvv * (size_t)N + l] = -1.f;
+    coef[kBww * (size_t)N + l] = -1.f;
+    coef[kBuv * (size_t)N + l] = 0.01f;
+    coef[kBuw * (size_t)N + l] = 0.01f;
+    coef[kBvw * (size_t)N + l] = 0.01f;
+    coef[kRu * (size_t)N + l] = 1e-3f * (l % 7);
+    coef[kRv * (size_t)N + l] = 1e-3f * (l % 5);
+    coef[kRw * (size_t)N + l] = 1e-3f * (l % 3);
+    du[l] = du[N + l] = du[2 * N + l] = 0.f;
+  }
+}
+
+void launch_fill_test_system(hipStream_t s, float* coef, float* du, int W, int H) {
+  int nb = (int)std::min<size_t>(((size_t)W * H + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_test_system, dim3(nb), dim3(256), 0, s, coef, du, W, H);
+}
+
 void launch_copy_f32(hipStream_t s, const float* src, size_t n, float* dst) {
   int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(256), 0, s, src, n, dst);

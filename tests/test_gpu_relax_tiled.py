@@ -107,3 +107,20 @@ def test_tiled_relax_more_tiles_than_resident(ctx, R, tile_iters):
     a = O.relax(coef, 20, x0)
     b = ctx.relax(coef, 20, x0, R.make_params(640, 480, relax_variant=R.RELAX_TILED, tile_iters=tile_iters))
     assert np.array_equal(a, b), float(np.abs(a - b).max())
+
+
+def test_autotuned_schedule_is_exact(R):
+    """Whatever schedule the autotuner picks, the solve is unchanged."""
+    O = oracle()
+    W, H = 320, 240
+    pair, im1, d1, im2, d2, sigma = make_case(W, H, 17, 4, 262.5, 159.5, 119.5)
+    hpb, lv = hpb_for(pair, W, H, 0.5, 40)
+    kw = dict(f=262.5, cx=159.5, cy=119.5, n_outer=6, n_sor=30, min_width=40)
+    ua, va, wa, sa = O.multiscale(im1, d1, im2, d2, sigma, hpb, O.make_params(W, H, **kw))
+    with R.Context(0) as c:
+        for (w, h) in lv:
+            ch = c.autotune_relax(w, h)
+            assert ch["candidates"] > 0 and ch["us_per_iter"] > 0
+        ub, vb, wb, sb = c.multiscale(R.make_params(W, H, relax_variant=R.RELAX_TILED, **kw), im1, d1, im2, d2,
+                                      sigma, hpb)
+    assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
