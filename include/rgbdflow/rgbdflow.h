@@ -146,6 +146,35 @@ int rgbdflow_multiscale(rgbdflow_ctx* ctx, int W, int H, const double* im1, cons
                         const double* dm2, const float* sigma1, const float* hpb, const rgbdflow_params* params,
                         double* u, double* v, double* w, rgbdflow_stats* stats);
 
+/* Batched sceneFlowMultiscale: n independent frame pairs, of any sizes,
+ * solved together.  Every kernel launch of a batch serves all of its pairs
+ * (the same step of each pair's coarse-to-fine schedule; pyramids are aligned
+ * on their finest level), and each pair keeps its own early exits, so the
+ * results equal n separate pair_run calls bit for bit.  The reference solves
+ * one pair per call (runFlowTwoFrames.cpp:114-126); this entry has no
+ * counterpart there and is the throughput path for callers with many pairs.
+ *   batch_load  copies the pairs to the device (replaces any loaded pair or
+ *               batch).  Per pair, as pair_load; cam.xRes == 0 takes
+ *               params->cam, min_width <= 0 takes params->min_width.
+ *   batch_run   solves every loaded pair.  stats: NULL (returns after
+ *               enqueueing) or n entries (waits; device_ms is the batch's).
+ *   batch_read  copies pair i's flow back. */
+typedef struct rgbdflow_pair_input {
+  int W, H;
+  const double* im1;   /* [N][3] BGR in [0,1] */
+  const double* dm1;   /* [N] hole-filled depth, metres */
+  const double* im2;
+  const double* dm2;
+  const float* sigma1; /* [N] depth stdevs of frame 1, or NULL (computed on the device) */
+  const float* hpb;    /* boundary maps of every level, or NULL */
+  rgbdflow_camera cam; /* camera of this pair (xRes == 0: params->cam) */
+  int min_width;       /* pyramid min width (<= 0: params->min_width) */
+  int reserved;
+} rgbdflow_pair_input;
+int rgbdflow_batch_load(rgbdflow_ctx* ctx, const rgbdflow_params* params, int n, const rgbdflow_pair_input* pairs);
+int rgbdflow_batch_run(rgbdflow_ctx* ctx, const rgbdflow_params* params, rgbdflow_stats* stats);
+int rgbdflow_batch_read(rgbdflow_ctx* ctx, int i, double* u, double* v, double* w);
+
 /* OpticalFlow::sceneFlowWithInit (OpticalFlow.h:145, sceneFlowWithInit.cpp:7-81):
  * full resolution only, from the estimate est [N][3], no early termination.
  * hpb0: level-0 map [H][W][2] (NULL unless HPB). */
@@ -199,10 +228,11 @@ typedef struct rgbdflow_tile_choice {
 } rgbdflow_tile_choice;
 
 /* Times candidate schedules of the tiled relaxation (workgroup shape, tile
- * shape, iterations per launch) on a W x H level and makes the fastest the
- * context's schedule for that shape whenever tile_* are left to the planner
- * (tile_iters = tile_w = tile_h = 0, tile_config = -1).  out may be NULL. */
-int rgbdflow_autotune_relax(rgbdflow_ctx* ctx, int W, int H, rgbdflow_tile_choice* out);
+ * shape, iterations per launch) on `batch` W x H levels relaxed by one launch
+ * each, and makes the fastest the context's schedule for that shape whenever
+ * tile_* are left to the planner (tile_iters = tile_w = tile_h = 0,
+ * tile_config = -1).  out may be NULL. */
+int rgbdflow_autotune_relax(rgbdflow_ctx* ctx, int W, int H, int batch, rgbdflow_tile_choice* out);
 
 #ifdef __cplusplus
 }

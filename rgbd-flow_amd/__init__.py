@@ -28,7 +28,8 @@ EXPORTS = [
     "rgbdflow_solve_level", "rgbdflow_depth_stdevs_pointwise", "rgbdflow_pair_load", "rgbdflow_pair_run",
     "rgbdflow_pair_read", "rgbdflow_multiscale", "rgbdflow_with_init", "rgbdflow_gaussian_pyramid",
     "rgbdflow_depth_map_pyramid", "rgbdflow_im2feature", "rgbdflow_warp_by_3d_flow", "rgbdflow_upsample_flow",
-    "rgbdflow_level_system", "rgbdflow_relax", "rgbdflow_autotune_relax",
+    "rgbdflow_level_system", "rgbdflow_relax", "rgbdflow_autotune_relax", "rgbdflow_batch_load",
+    "rgbdflow_batch_run", "rgbdflow_batch_read",
 ]
 
 
@@ -56,6 +57,12 @@ class Stats(C.Structure):
     def levels(self):
         return [dict(W=self.width[k], H=self.height[k], outer=self.outer_exec[k], sweeps=self.sweeps_exec[k],
                      stat=self.last_stat[k]) for k in range(self.n_levels)]
+
+
+class PairInput(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("im1", C.c_void_p), ("dm1", C.c_void_p), ("im2", C.c_void_p),
+                ("dm2", C.c_void_p), ("sigma1", C.c_void_p), ("hpb", C.c_void_p), ("cam", Camera),
+                ("min_width", C.c_int), ("reserved", C.c_int)]
 
 
 class TileChoice(C.Structure):
@@ -143,6 +150,8 @@ class Context:
         if rc != OK:
             raise RgbdFlowError(rc, "rgbdflow_create failed (no HIP device?)")
         self.h = h
+        self._batch_shapes = []
+        self._pair_shape = None
 
     def close(self):
         if self.h:
@@ -171,11 +180,11 @@ class Context:
     def synchronize(self):
         self._check(lib().rgbdflow_synchronize(self.h))
 
-    def autotune_relax(self, W, H):
-        """Time the tiled relaxation's schedules on a W x H level; the fastest
-        becomes this context's default for that shape."""
+    def autotune_relax(self, W, H, batch=1):
+        """Time the tiled relaxation's schedules on `batch` W x H levels; the
+        fastest becomes this context's default for that shape."""
         ch = TileChoice()
-        self._check(lib().rgbdflow_autotune_relax(self.h, W, H, C.byref(ch)))
+        self._check(lib().rgbdflow_autotune_relax(self.h, W, H, int(batch), C.byref(ch)))
         return ch.as_dict()
 
     def set_relax_timing(self, on: bool):
@@ -283,6 +292,7 @@ class Context:
         self._check(lib().rgbdflow_pair_load(self.h, C.byref(params), W, H, _p(im1), _p(dm1), _p(im2), _p(dm2),
                                              _p(s1), _p(h)))
         self._pair_shape = (H, W)
+        self._batch_shapes = [(H, W)]
 
     def pair_run(self, params, stats=True):
         if stats:
@@ -296,6 +306,43 @@ class Context:
         H, W = self._pair_shape
         u = np.empty((H, W)); v = np.empty((H, W)); w = np.empty((H, W))
         self._check(lib().rgbdflow_pair_read(self.h, _p(u), _p(v), _p(w)))
+        return u, v, w
+
+    # ---------------------------------------------------------------- batches
+    def batch_load(self, params, pairs):
+        """pairs: list of dicts with im1, dm1, im2, dm2 and optionally sigma1,
+        hpb, cam (a Camera) and min_width."""
+        keep, arr = [], (PairInput * len(pairs))()
+        for i, pr in enumerate(pairs):
+            im1, dm1, im2, dm2 = _f64(pr["im1"]), _f64(pr["dm1"]), _f64(pr["im2"]), _f64(pr["dm2"])
+            s1 = _f32(pr["sigma1"]) if pr.get("sigma1") is not None else None
+            h = _f32(pr["hpb"]) if pr.get("hpb") is not None else None
+            keep.append((im1, dm1, im2, dm2, s1, h))
+            H, W = dm1.shape
+            a = arr[i]
+            a.W, a.H = W, H
+            a.im1, a.dm1, a.im2, a.dm2 = _p(im1), _p(dm1), _p(im2), _p(dm2)
+            a.sigma1, a.hpb = _p(s1), _p(h)
+            if pr.get("cam") is not None:
+                a.cam = pr["cam"]
+            a.min_width = int(pr.get("min_width", 0))
+        self._check(lib().rgbdflow_batch_load(self.h, C.byref(params), len(pairs), arr))
+        self._batch_shapes = [k[1].shape for k in keep]
+        self._pair_shape = self._batch_shapes[0]
+
+    def batch_run(self, params, stats=True):
+        n = len(self._batch_shapes)
+        if stats:
+            st = (Stats * n)()
+            self._check(lib().rgbdflow_batch_run(self.h, C.byref(params), st))
+            return list(st)
+        self._check(lib().rgbdflow_batch_run(self.h, C.byref(params), None))
+        return None
+
+    def batch_read(self, i):
+        H, W = self._batch_shapes[i]
+        u = np.empty((H, W)); v = np.empty((H, W)); w = np.empty((H, W))
+        self._check(lib().rgbdflow_batch_read(self.h, int(i), _p(u), _p(v), _p(w)))
         return u, v, w
 
     def multiscale(self, params, im1, dm1, im2, dm2, sigma1=None, hpb=None):

@@ -1,10 +1,14 @@
 // capi.cpp -- the C ABI of librgbdflow: context, HBM buffer pool and the
-// device-resident orchestration of the coarse-to-fine solve.
+// device-resident orchestration of batched coarse-to-fine solves.
 //
-// The host enqueues a whole frame pair (every pyramid level, every outer and
-// inner iteration) on the context's stream without waiting: early exits are
-// decided on the device (SolveCtl), and the host reads the control blocks
-// back once, after the pair, to report iteration counts.
+// A batch of frame pairs (any sizes) is solved in lock step: pyramids are
+// aligned on their finest level, and step s of the batch runs, for every pair
+// that has it, the level S-1-s (S = the largest level count).  Each launch
+// serves all pairs of a step through a work-list (internal.hpp), so a batch
+// of B pairs costs the launches of one pair.  The host enqueues the whole
+// batch without waiting: early exits are decided on the device (SolveCtl per
+// pair and level), and the host reads the control blocks back once, after
+// the batch, to report iteration counts.  A single pair is a batch of one.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +35,57 @@ struct LevelShape {
   int W, H;
 };
 
+// Work buffers of one solve slot, sized for its finest level; a level of N
+// pixels uses the first N entries of every plane.
+struct Work {
+  double *I1, *I2, *Iw, *Dw, *u[2], *Ix, *Iy, *It, *Dx, *Dy, *Dt, *phi, *partials;
+  float *du, *du_alt, *dW, *sW, *mW, *rW, *cW, *q, *coef;
+  SolveCtl* ctl;  // kMaxLevels entries, one per level
+};
+
+// One loaded frame pair: inputs, pyramids (levels >= 1 packed finest first)
+// and its work buffers.
+struct PairSlot {
+  int W = 0, H = 0, min_width = 0;
+  bool cam_given = false, sigma_given = false, hpb_given = false;
+  rgbdflow_camera cam{};
+  std::vector<LevelShape> L;
+  std::vector<size_t> off;   // level k >= 1: element offset (per channel) in the pyramid arrays
+  std::vector<size_t> hoff;  // level k: float offset of its boundary map
+  double *im1 = nullptr, *im2 = nullptr, *dm1 = nullptr, *dm2 = nullptr, *del0 = nullptr, *sm0 = nullptr;
+  double *pim1 = nullptr, *pim2 = nullptr, *pd1 = nullptr, *pd2 = nullptr, *pdel = nullptr, *psm = nullptr;
+  float *sigma = nullptr, *hpb = nullptr;
+  Work wk{};
+};
+
+// Device work-lists of one step of a batch.
+struct StepPlan {
+  int n = 0;
+  int pix = 0, plain = 0, tiled = 0;  // total blocks of each kernel family
+  size_t items_off = 0, pix_off = 0, plain_off = 0, tiled_off = 0;
+  Views vpix{}, vplain{}, vtiled{};
+  bool tiled_ok = false;
+  int cfg = 0, cap = 0, T = 0;
+  bool any_early = false;
+};
+
+struct BatchPlan {
+  bool valid = false;
+  unsigned long long gen = 0, plans_gen = 0;
+  std::vector<unsigned char> params;
+  std::vector<char> host;  // packed descriptors, uploaded to "desc"
+  std::vector<StepPlan> steps;
+  size_t prep_items = 0, prep_start = 0;
+  int prep_n = 0, prep_blocks = 0;
+  struct Wave {
+    size_t items, start;
+    int n, blocks;
+    WorkList<PyrJob> wl;
+  };
+  std::vector<Wave> waves;
+  WorkList<PairPrep> prep{};
+};
+
 }  // namespace
 
 struct rgbdflow_ctx {
@@ -46,17 +101,15 @@ struct rgbdflow_ctx {
   bool time_relax = false;
   std::vector<hipEvent_t> relax_ev;
   int relax_ev_used = 0;
-  // loaded pair
-  bool pair_loaded = false;
-  bool pair_ran = false;
-  int pW = 0, pH = 0;
-  double p_ratio = 0;
-  int p_min_width = 0;
-  bool p_sigma_given = false;
-  bool p_hpb_given = false;
-  std::vector<LevelShape> p_levels;
+  // loaded batch
+  bool loaded = false, ran = false;
+  double load_ratio = 0;
+  int load_min_width = 0;
+  std::vector<PairSlot> pairs;
+  unsigned long long gen = 0;  // bumped by every load (buffers may move)
+  BatchPlan plan;
   std::map<std::string, TileSched> plans;  // tile schedules by level shape and request
-  int p_final_slot = 0;  // which of the two flow buffers holds the finest result
+  unsigned long long plans_gen = 0;
 };
 
 namespace {
@@ -74,7 +127,15 @@ int fail(rgbdflow_ctx* c, int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(ctx, RGBDFLOW_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+#define TRY(expr)                       \
+  do {                                  \
+    int rc_ = (expr);                   \
+    if (rc_ != RGBDFLOW_OK) return rc_; \
+  } while (0)
+
 // Grow-only named device buffer, charged against the context's HBM budget.
+// A buffer is freed only after the stream has drained (work in flight may
+// still read it).
 int ensure(rgbdflow_ctx* c, const std::string& name, size_t bytes, void** out) {
   Buffer& b = c->bufs[name];
   if (b.bytes >= bytes && b.p) {
@@ -82,6 +143,7 @@ int ensure(rgbdflow_ctx* c, const std::string& name, size_t bytes, void** out) {
     return RGBDFLOW_OK;
   }
   if (b.p) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     hipFree(b.p);
     c->in_use -= b.bytes;
     b.p = nullptr;
@@ -109,12 +171,6 @@ int get(rgbdflow_ctx* c, const std::string& name, size_t count, T** out) {
   *out = static_cast<T*>(p);
   return rc;
 }
-
-#define TRY(expr)                 \
-  do {                            \
-    int rc_ = (expr);             \
-    if (rc_ != RGBDFLOW_OK) return rc_; \
-  } while (0)
 
 // GaussianPyramid.cpp:20-63 level sizes.
 std::vector<LevelShape> pyramid_shape(int W, int H, double ratio, int min_width, int* n_smooth, double* r_eff) {
@@ -155,27 +211,37 @@ std::vector<double> gaussian_taps(double sigma, int fsize) {
   return g;
 }
 
-// Builds levels 1..L-1 of a Gaussian pyramid on the device.  lvl[k] points at
-// level k's planes ([C][N_k]); lvl[0] is the source image.
-void build_pyramid(hipStream_t s, const std::vector<LevelShape>& L, int ns, double ratio, int C,
-                   std::vector<double*>& lvl) {
+// The job that makes level i (>= 1) of a Gaussian pyramid (GaussianPyramid.cpp:47-61):
+// levels up to n_smooth smooth the finest image with sigma (1/ratio - 1) i and
+// resize it by ratio^i; deeper levels re-smooth level i - n_smooth.  lvl[k]
+// points at level k's planes.  Returns false if the filter is too wide.
+bool pyramid_job(const std::vector<LevelShape>& L, int ns, double ratio, int C, const std::vector<double*>& lvl,
+                 int i, PyrJob* j) {
   const double base = (1 / ratio - 1);
-  const double nSigma = base * ns;
-  for (size_t i = 1; i < L.size(); i++) {
-    if ((int)i <= ns) {
-      const double sigma = base * i;
-      const int fs = (int)(sigma * 3);
-      const std::vector<double> g = gaussian_taps(sigma, fs);
-      launch_pyramid_level(s, lvl[0], L[0].W, L[0].H, C, g.data(), fs, std::pow(ratio, (int)i), lvl[i], L[i].W,
-                           L[i].H);
-    } else {
-      const int fs = (int)(nSigma * 3);
-      const std::vector<double> g = gaussian_taps(nSigma, fs);
-      const LevelShape& src = L[i - ns];
-      const double rate = (double)std::pow(ratio, (int)i) * L[0].W / src.W;
-      launch_pyramid_level(s, lvl[i - ns], src.W, src.H, C, g.data(), fs, rate, lvl[i], L[i].W, L[i].H);
-    }
+  const double sigma = (i <= ns) ? base * i : base * ns;
+  const int fs = (int)(sigma * 3);
+  if (2 * fs + 1 > kMaxTaps) return false;
+  const std::vector<double> g = gaussian_taps(sigma, fs);
+  std::memset(j, 0, sizeof(*j));
+  for (int t = 0; t < 2 * fs + 1; t++) j->taps[t] = g[t];
+  j->fs = fs;
+  j->C = C;
+  j->dst = lvl[i];
+  j->Wd = L[i].W;
+  j->Hd = L[i].H;
+  if (i <= ns) {
+    j->src = lvl[0];
+    j->Ws = L[0].W;
+    j->Hs = L[0].H;
+    j->ratio = std::pow(ratio, i);
+  } else {
+    const LevelShape& s = L[i - ns];
+    j->src = lvl[i - ns];
+    j->Ws = s.W;
+    j->Hs = s.H;
+    j->ratio = (double)std::pow(ratio, i) * L[0].W / s.W;
   }
+  return true;
 }
 
 size_t pyramid_elems(const std::vector<LevelShape>& L, int from) {
@@ -192,49 +258,43 @@ float downsized_focal(const rgbdflow_camera& c, int width) {
   return fl;
 }
 
-// Level work buffers, sized for N pixels.
-struct Work {
-  double *I1, *I2, *Iw, *Dw, *u[2], *Ix, *Iy, *It, *Dx, *Dy, *Dt, *phi, *partials;
-  float *du, *du_alt, *dW, *sW, *mW, *rW, *cW, *q, *coef;
-  SolveCtl* ctl;  // kMaxLevels entries
-};
-
-int get_work(rgbdflow_ctx* c, size_t N, Work& w) {
-  TRY(get(c, "I1", kFeat * N, &w.I1));
-  TRY(get(c, "I2", kFeat * N, &w.I2));
-  TRY(get(c, "Iw", kFeat * N, &w.Iw));
-  TRY(get(c, "Dw", N, &w.Dw));
-  TRY(get(c, "u0", 3 * N, &w.u[0]));
-  TRY(get(c, "u1", 3 * N, &w.u[1]));
-  TRY(get(c, "Ix", kFeat * N, &w.Ix));
-  TRY(get(c, "Iy", kFeat * N, &w.Iy));
-  TRY(get(c, "It", kFeat * N, &w.It));
-  TRY(get(c, "Dx", N, &w.Dx));
-  TRY(get(c, "Dy", N, &w.Dy));
-  TRY(get(c, "Dt", N, &w.Dt));
-  TRY(get(c, "phi", N, &w.phi));
-  TRY(get(c, "partials", (size_t)stat_blocks((int)N) + 16, &w.partials));
-  TRY(get(c, "du", 3 * N, &w.du));
-  TRY(get(c, "du_alt", 3 * N, &w.du_alt));
-  TRY(get(c, "dW", N, &w.dW));
-  TRY(get(c, "sW", N, &w.sW));
-  TRY(get(c, "mW", N, &w.mW));
-  TRY(get(c, "rW", 2 * N, &w.rW));
-  TRY(get(c, "cW", N, &w.cW));
-  TRY(get(c, "q", N, &w.q));
-  TRY(get(c, "coef", (size_t)kCoef * N, &w.coef));
-  TRY(get(c, "ctl", kMaxLevels, &w.ctl));
+int get_work(rgbdflow_ctx* c, const std::string& pre, size_t N, Work& w) {
+  TRY(get(c, pre + "I1", kFeat * N, &w.I1));
+  TRY(get(c, pre + "I2", kFeat * N, &w.I2));
+  TRY(get(c, pre + "Iw", kFeat * N, &w.Iw));
+  TRY(get(c, pre + "Dw", N, &w.Dw));
+  TRY(get(c, pre + "u0", 3 * N, &w.u[0]));
+  TRY(get(c, pre + "u1", 3 * N, &w.u[1]));
+  TRY(get(c, pre + "Ix", kFeat * N, &w.Ix));
+  TRY(get(c, pre + "Iy", kFeat * N, &w.Iy));
+  TRY(get(c, pre + "It", kFeat * N, &w.It));
+  TRY(get(c, pre + "Dx", N, &w.Dx));
+  TRY(get(c, pre + "Dy", N, &w.Dy));
+  TRY(get(c, pre + "Dt", N, &w.Dt));
+  TRY(get(c, pre + "phi", N, &w.phi));
+  TRY(get(c, pre + "partials", (size_t)pix_blocks((int)N) + 16, &w.partials));
+  TRY(get(c, pre + "du", 3 * N, &w.du));
+  TRY(get(c, pre + "du_alt", 3 * N, &w.du_alt));
+  TRY(get(c, pre + "dW", N, &w.dW));
+  TRY(get(c, pre + "sW", N, &w.sW));
+  TRY(get(c, pre + "mW", N, &w.mW));
+  TRY(get(c, pre + "rW", 2 * N, &w.rW));
+  TRY(get(c, pre + "cW", N, &w.cW));
+  TRY(get(c, pre + "q", N, &w.q));
+  TRY(get(c, pre + "coef", (size_t)kCoef * N, &w.coef));
+  TRY(get(c, pre + "ctl", kMaxLevels, &w.ctl));
   return RGBDFLOW_OK;
 }
 
-LevelView make_view(const Work& w, int W, int H, const double* I1, const double* I2, const double* D1,
-                    const double* D2, double* u, int level) {
+// The solve fields of a level view over work buffers (level-entry fields zero).
+LevelView make_view(const Work& w, int W, int H, const double* D1, const double* D2, double* u, int level) {
   LevelView v;
+  std::memset(&v, 0, sizeof(v));
   v.W = W;
   v.H = H;
   v.N = W * H;
-  v.I1 = I1;
-  v.I2 = I2;
+  v.I1 = w.I1;
+  v.I2 = w.I2;
   v.Iw = w.Iw;
   v.D1 = D1;
   v.D2 = D2;
@@ -258,6 +318,7 @@ LevelView make_view(const Work& w, int W, int H, const double* I1, const double*
   v.coef = w.coef;
   v.partials = w.partials;
   v.ctl = w.ctl + level;
+  v.uv_scale = 1;
   return v;
 }
 
@@ -269,7 +330,7 @@ int validate_iters(rgbdflow_ctx* c, int nOuter, int nInner, int nSOR) {
   return RGBDFLOW_OK;
 }
 
-void relax_event(rgbdflow_ctx* c, bool begin) {
+void relax_event(rgbdflow_ctx* c) {
   if (!c->time_relax) return;
   if (c->relax_ev_used >= (int)c->relax_ev.size()) {
     hipEvent_t e;
@@ -277,20 +338,14 @@ void relax_event(rgbdflow_ctx* c, bool begin) {
     c->relax_ev.push_back(e);
   }
   hipEventRecord(c->relax_ev[c->relax_ev_used++], c->stream);
-  (void)begin;
 }
-
-struct RelaxPlan {
-  int variant = RGBDFLOW_RELAX_PLAIN;
-  TileSched ts{};
-};
 
 // Tile schedule of the tiled relaxation for a W x H level.  Explicit tile_*
 // parameters win; otherwise a cost model picks the workgroup shape, the tile
 // and the iterations per launch: a tile covering the whole level runs a whole
 // check interval per launch; otherwise cost ~ launches x (launch overhead +
 // waves x per-workgroup work), where the per-workgroup work grows with the
-// region (tile + halo of 2 T) per thread.
+// region (tile + halo of 2 T) per thread.  cap == 0: nothing fits.
 TileSched plan_tiles(const rgbdflow_ctx* c, int W, int H, int req_T, int req_w, int req_h, int req_cfg) {
   TileSched best{};
   double best_cost = 1e300;
@@ -313,14 +368,16 @@ TileSched plan_tiles(const rgbdflow_ctx* c, int W, int H, int req_T, int req_w, 
       }
       continue;
     }
-    const int Ts[] = {2, 3, 4, 5, 6, 8};
+    const int Ts[] = {1, 2, 3, 4, 5, 6, 8, 12};
     for (int T : Ts) {
       if (req_T > 0 && T != req_T) continue;
+      if (req_T <= 0 && (T == 1 || T == 12)) continue;  // the cost model's range
       const int h = 2 * T;
       for (int TW = 8; TW <= W + 7; TW += 8) {
         if (req_w > 0 && TW != req_w + (req_w & 1)) continue;
-        for (int TH = 4; TH <= H + 3; TH += 4) {
+        for (int TH = 2; TH <= H + 3; TH += 2) {
           if (req_h > 0 && TH != req_h) continue;
+          if (req_h <= 0 && (TH & 3)) continue;
           const int rw = std::min(TW + 2 * h, W + (W & 1)), rh = std::min(TH + 2 * h, H);
           const long long rp = (long long)((rw + 1) / 2) * rh;
           if (rp > cap) continue;
@@ -347,90 +404,80 @@ std::string plan_key(int W, int H, int T, int tw, int th, int cfg) {
   return key;
 }
 
-RelaxPlan make_relax_plan(rgbdflow_ctx* c, int W, int H, const rgbdflow_params* p) {
-  RelaxPlan r;
-  if (!p || p->relax_variant != RGBDFLOW_RELAX_TILED) return r;
-  const std::string key = plan_key(W, H, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
+// Cached plan_tiles.
+TileSched cached_plan(rgbdflow_ctx* c, int W, int H, int T, int tw, int th, int cfg) {
+  const std::string key = plan_key(W, H, T, tw, th, cfg);
   auto it = c->plans.find(key);
-  if (it == c->plans.end())
-    it = c->plans.emplace(key, plan_tiles(c, W, H, p->tile_iters, p->tile_w, p->tile_h, p->tile_config)).first;
-  r.ts = it->second;
-  if (r.ts.cap == 0) return r;  // nothing fits: plain schedule
-  r.variant = RGBDFLOW_RELAX_TILED;
-  return r;
+  if (it == c->plans.end()) it = c->plans.emplace(key, plan_tiles(c, W, H, T, tw, th, cfg)).first;
+  return it->second;
 }
 
-// nSOR red-black iterations of one (outer, inner), with the convergence
-// checks of .cu:905-922 at iterations 24, 49, ...; the tiled schedule splits
-// the iterations into launches that end at every check iteration.
-void enqueue_relax(rgbdflow_ctx* c, const LevelView& v, const SolveScalars& k, const RelaxPlan& plan, int slot_base) {
-  hipStream_t s = c->stream;
-  if (plan.variant == RGBDFLOW_RELAX_TILED) {
-    int it = 0;
-    float* cur = v.du;
-    float* nxt = v.du_alt;
-    while (it < k.n_sor) {
-      const int boundary = (it / kCheckEvery + 1) * kCheckEvery;
-      const int T = std::min(std::min(plan.ts.tmax, boundary - it), k.n_sor - it);
-      const int last = it + T - 1;
-      const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
-      const int slot = (last % kCheckEvery == kCheckEvery - 1) ? slot_base + last / kCheckEvery : -1;
-      relax_event(c, true);
-      launch_relax_tiled(s, v, plan.ts, cur, nxt, T, slot_base, n_prev, slot);
-      relax_event(c, false);
-      std::swap(cur, nxt);
-      it += T;
+// Relaxation schedule of one step.  The tiled kernel takes one workgroup
+// shape and one iteration count per launch, so the largest item's schedule
+// (the caller's tile_* request, else the tuned or modelled one) fixes them
+// and the other items are planned under them.  Falls back to the plain
+// schedule when an item has no tiled schedule.
+void plan_step_relax(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<LevelView>& items, StepPlan& st) {
+  st.tiled_ok = false;
+  if (!p || p->relax_variant != RGBDFLOW_RELAX_TILED || items.empty()) return;
+  size_t lead = 0;
+  for (size_t i = 1; i < items.size(); i++)
+    if (items[i].N > items[lead].N) lead = i;
+  const int LW = items[lead].W, LH = items[lead].H;
+  const TileSched lt = cached_plan(c, LW, LH, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
+  if (lt.cap == 0) return;
+  std::vector<TileSched> ts(items.size());
+  for (size_t i = 0; i < items.size(); i++) {
+    if (items[i].W == LW && items[i].H == LH) {
+      ts[i] = lt;
+      continue;
     }
-    if (cur != v.du) launch_copy_f32(s, cur, (size_t)3 * v.N, v.du);
-    return;
+    // a smaller item: a whole-level tile under the lead's workgroup shape,
+    // else tiles of the lead's iteration count
+    const TileSched t = cached_plan(c, items[i].W, items[i].H, lt.single ? 0 : lt.tmax, 0, 0, lt.config);
+    if (t.cap == 0 || (!t.single && (lt.single || t.tmax != lt.tmax))) return;
+    ts[i] = t;
   }
-  for (int it = 0; it < k.n_sor; it++) {
-    const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
-    const int slot = (it % kCheckEvery == kCheckEvery - 1) ? slot_base + it / kCheckEvery : -1;
-    relax_event(c, true);
-    launch_relax_plain(s, v, 0, 0, slot_base, n_prev, slot);
-    relax_event(c, false);
-    relax_event(c, true);
-    launch_relax_plain(s, v, 1, 0, slot_base, n_prev, slot);
-    relax_event(c, false);
-  }
+  for (size_t i = 0; i < items.size(); i++) items[i].ts = ts[i];
+  st.tiled_ok = true;
+  st.cfg = lt.config;
+  st.cap = lt.cap;
+  st.T = lt.tmax;
 }
 
-// sceneFlowSingleScale's loops (.cu:755-954) enqueued on the device.
-void enqueue_level_solve(rgbdflow_ctx* c, const LevelView& v, const SolveScalars& k, const RelaxPlan& plan) {
-  hipStream_t s = c->stream;
-  const int checks = k.n_sor / kCheckEvery;
-  launch_ctl_reset(s, v.ctl);
-  for (int o = 0; o < k.n_outer; o++) {
-    launch_derivs(s, v, k.n_inner * checks);
-    for (int h = 0; h < k.n_inner; h++) {
-      launch_phi(s, v, o);
-      launch_system(s, v, k, o);
-      enqueue_relax(c, v, k, plan, h * checks);
-    }
-    launch_update_warp(s, v, k, o);
+size_t pack(std::vector<char>& host, const void* p, size_t bytes) {
+  const size_t off = (host.size() + 255) & ~size_t(255);
+  host.resize(off + bytes);
+  std::memcpy(host.data() + off, p, bytes);
+  return off;
+}
+
+// Packs a step's items and the block offsets of each kernel family.
+void pack_step(std::vector<char>& host, const std::vector<LevelView>& items, StepPlan& st) {
+  st.n = (int)items.size();
+  std::vector<int> pix(st.n + 1), plain(st.n + 1), tiled(st.n + 1);
+  pix[0] = plain[0] = tiled[0] = 0;
+  for (int i = 0; i < st.n; i++) {
+    pix[i + 1] = pix[i] + pix_blocks(items[i].N);
+    plain[i + 1] = plain[i] + plain_blocks(items[i].W, items[i].H);
+    tiled[i + 1] = tiled[i] + (st.tiled_ok ? tiled_blocks(items[i].ts) : 0);
   }
+  st.pix = pix[st.n];
+  st.plain = plain[st.n];
+  st.tiled = tiled[st.n];
+  st.any_early = false;
+  for (const LevelView& v : items) st.any_early = st.any_early || v.allow_early;
+  st.items_off = pack(host, items.data(), items.size() * sizeof(LevelView));
+  st.pix_off = pack(host, pix.data(), pix.size() * sizeof(int));
+  st.plain_off = pack(host, plain.data(), plain.size() * sizeof(int));
+  st.tiled_off = pack(host, tiled.data(), tiled.size() * sizeof(int));
 }
 
-void ctl_to_stats(const SolveCtl& ctl, const SolveScalars& k, int& outer_exec, int& sweeps, double& last_stat) {
-  outer_exec = ctl.outer_done ? ctl.outer_exit_at + 1 : k.n_outer;
-  sweeps = ctl.sweeps_exec;
-  last_stat = (k.allow_early && outer_exec > 0) ? ctl.stat[outer_exec - 1] : -1.0;
-}
-
-SolveScalars scalars_from(const rgbdflow_params* p, double cw, double alpha, double gamma, int nOuter, int nInner,
-                          int nSOR, int allow_early) {
-  SolveScalars k;
-  k.alpha = alpha;
-  k.gamma = gamma;
-  k.color_weight = cw;
-  k.n_outer = nOuter;
-  k.n_inner = nInner;
-  k.n_sor = nSOR;
-  k.allow_early = allow_early;
-  k.inner_exit = p ? p->inner_exit : 1;
-  k.outer_exit = p ? p->outer_exit : 1;
-  return k;
+void bind_step(char* dev, StepPlan& st) {
+  const LevelView* items = reinterpret_cast<const LevelView*>(dev + st.items_off);
+  st.vpix = Views{items, reinterpret_cast<const int*>(dev + st.pix_off), st.n};
+  st.vplain = Views{items, reinterpret_cast<const int*>(dev + st.plain_off), st.n};
+  st.vtiled = Views{items, reinterpret_cast<const int*>(dev + st.tiled_off), st.n};
 }
 
 int upload(rgbdflow_ctx* c, void* dst, const void* src, size_t bytes) {
@@ -442,6 +489,106 @@ int download(rgbdflow_ctx* c, void* dst, const void* src, size_t bytes) {
   return RGBDFLOW_OK;
 }
 
+int sync(rgbdflow_ctx* c) {
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipGetLastError());
+  return RGBDFLOW_OK;
+}
+
+// Uploads packed descriptors to the named device buffer once the stream has
+// drained (work in flight may read the buffer).
+int upload_desc(rgbdflow_ctx* c, const char* name, const std::vector<char>& host, char** dev) {
+  TRY(sync(c));
+  TRY(get(c, name, host.size(), dev));
+  HIP_TRY(c, hipMemcpy(*dev, host.data(), host.size(), hipMemcpyHostToDevice));
+  return RGBDFLOW_OK;
+}
+
+// A one-step plan over the given items (the single-level entry points and the
+// autotuner).  keep_ts: the items' tile schedules and st's launch shape are set.
+int one_step(rgbdflow_ctx* c, const char* name, std::vector<LevelView>& items, const rgbdflow_params* sched,
+             StepPlan& st, bool keep_ts = false) {
+  if (!keep_ts) plan_step_relax(c, sched, items, st);
+  std::vector<char> host;
+  pack_step(host, items, st);
+  char* dev;
+  TRY(upload_desc(c, name, host, &dev));
+  bind_step(dev, st);
+  return RGBDFLOW_OK;
+}
+
+// nSOR red-black iterations of one (outer, inner), with the convergence
+// checks of .cu:905-922 at iterations 24, 49, ...; the tiled schedule splits
+// the iterations into launches that end at every check iteration.
+void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, int slot_base) {
+  hipStream_t s = c->stream;
+  if (st.tiled_ok) {
+    int it = 0, flip = 0;
+    while (it < k.n_sor) {
+      const int boundary = (it / kCheckEvery + 1) * kCheckEvery;
+      const int T = std::min(std::min(st.T, boundary - it), k.n_sor - it);
+      const int last = it + T - 1;
+      const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
+      const int slot = (last % kCheckEvery == kCheckEvery - 1) ? slot_base + last / kCheckEvery : -1;
+      relax_event(c);
+      launch_relax_tiled(s, st.vtiled, st.tiled, st.cfg, st.cap, flip, T, slot_base, n_prev, slot);
+      relax_event(c);
+      flip ^= 1;
+      it += T;
+    }
+    if (flip) launch_copy_du(s, st.vpix, st.pix);
+    return;
+  }
+  for (int it = 0; it < k.n_sor; it++) {
+    const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
+    const int slot = (it % kCheckEvery == kCheckEvery - 1) ? slot_base + it / kCheckEvery : -1;
+    relax_event(c);
+    launch_relax_plain(s, st.vplain, st.plain, 0, slot_base, n_prev, slot);
+    relax_event(c);
+    relax_event(c);
+    launch_relax_plain(s, st.vplain, st.plain, 1, slot_base, n_prev, slot);
+    relax_event(c);
+  }
+}
+
+// sceneFlowSingleScale's loops (.cu:755-954) for every item of a step.
+void enqueue_level_solve(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k) {
+  hipStream_t s = c->stream;
+  const int checks = k.n_sor / kCheckEvery;
+  launch_ctl_reset(s, st.vpix);
+  for (int o = 0; o < k.n_outer; o++) {
+    launch_derivs(s, st.vpix, st.pix, k.n_inner * checks);
+    for (int h = 0; h < k.n_inner; h++) {
+      launch_phi(s, st.vpix, st.pix);
+      launch_system(s, st.vpix, st.pix, k);
+      enqueue_relax(c, st, k, h * checks);
+    }
+    launch_update_warp(s, st.vpix, st.pix, st.any_early ? 1 : 0);
+    if (st.any_early) launch_stat_final(s, st.vpix, o, k.outer_exit);
+  }
+}
+
+void ctl_to_stats(const SolveCtl& ctl, int n_outer, bool allow_early, int& outer_exec, int& sweeps,
+                  double& last_stat) {
+  outer_exec = ctl.outer_done ? ctl.outer_exit_at + 1 : n_outer;
+  sweeps = ctl.sweeps_exec;
+  last_stat = (allow_early && outer_exec > 0) ? ctl.stat[outer_exec - 1] : -1.0;
+}
+
+SolveScalars scalars_from(const rgbdflow_params* p, double cw, double alpha, double gamma, int nOuter, int nInner,
+                          int nSOR) {
+  SolveScalars k;
+  k.alpha = alpha;
+  k.gamma = gamma;
+  k.color_weight = cw;
+  k.n_outer = nOuter;
+  k.n_inner = nInner;
+  k.n_sor = nSOR;
+  k.inner_exit = p ? p->inner_exit : 1;
+  k.outer_exit = p ? p->outer_exit : 1;
+  return k;
+}
+
 // Interleaved host [N][C] f64 -> device planes [C][N].
 int upload_planes(rgbdflow_ctx* c, const double* host, size_t N, int C, double* dst) {
   double* stage;
@@ -450,19 +597,13 @@ int upload_planes(rgbdflow_ctx* c, const double* host, size_t N, int C, double* 
   launch_deinterleave(c->stream, stage, (int)N, C, dst);
   return RGBDFLOW_OK;
 }
-// Device planes [C][N] -> interleaved host [N][C] f64.
+// Device planes [C][N] -> interleaved host [N][C] f64 (waits).
 int download_planes(rgbdflow_ctx* c, const double* src, size_t N, int C, double* host) {
   double* stage;
   TRY(get(c, "stage_out", N * C, &stage));
   launch_interleave(c->stream, src, (int)N, C, stage);
   TRY(download(c, host, stage, N * C * sizeof(double)));
-  return RGBDFLOW_OK;
-}
-
-int sync(rgbdflow_ctx* c) {
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipGetLastError());
-  return RGBDFLOW_OK;
+  return sync(c);
 }
 
 void collect_relax_timing(rgbdflow_ctx* c, rgbdflow_stats* st) {
@@ -480,38 +621,342 @@ void collect_relax_timing(rgbdflow_ctx* c, rgbdflow_stats* st) {
   st->relax_launches = n;
 }
 
-// Level-solve inputs supplied by the host (sceneFlowSingleScale's arguments).
+bool bad_dims(int W, int H) { return W < 2 || H < 2 || (long long)W * H > (1LL << 28); }
+
+// Level-solve inputs supplied by the host (sceneFlowSingleScale's arguments)
+// in the single-level work set "d.": features, warp, depths, flow and the f64
+// weights (converted to the solver's f32 copies by the level-weights kernel).
 int stage_level_inputs(rgbdflow_ctx* c, int W, int H, float focal_level, const double* Im1, const double* dm1,
                        const double* Im2, const double* dm2, const double* warpIm2, const double* warpDm2,
                        const double* u, const double* v, const double* w, const double* dW64, const double* sW64,
-                       const double* mW64, const double* r64, Work& wk, double** D1, double** D2) {
+                       const double* mW64, const double* r64, Work& wk, LevelView& view) {
   const size_t N = (size_t)W * H;
-  TRY(get_work(c, N, wk));
-  TRY(get(c, "lvD1", N, D1));
-  TRY(get(c, "lvD2", N, D2));
+  TRY(get_work(c, "d.", N, wk));
+  double *D1, *D2, *d, *s, *m, *r;
+  TRY(get(c, "d.D1", N, &D1));
+  TRY(get(c, "d.D2", N, &D2));
+  TRY(get(c, "d.W0", N, &d));
+  TRY(get(c, "d.W1", N, &s));
+  TRY(get(c, "d.W2", N, &m));
+  TRY(get(c, "d.W3", 2 * N, &r));
   TRY(upload_planes(c, Im1, N, kFeat, wk.I1));
   TRY(upload_planes(c, Im2, N, kFeat, wk.I2));
   TRY(upload_planes(c, warpIm2, N, kFeat, wk.Iw));
-  TRY(upload(c, *D1, dm1, N * 8));
-  TRY(upload(c, *D2, dm2, N * 8));
+  TRY(upload(c, D1, dm1, N * 8));
+  TRY(upload(c, D2, dm2, N * 8));
   TRY(upload(c, wk.Dw, warpDm2, N * 8));
   TRY(upload(c, wk.u[0], u, N * 8));
   TRY(upload(c, wk.u[0] + N, v, N * 8));
   TRY(upload(c, wk.u[0] + 2 * N, w, N * 8));
-  double *d, *s, *m, *r;
-  TRY(get(c, "lvW0", N, &d));
-  TRY(get(c, "lvW1", N, &s));
-  TRY(get(c, "lvW2", N, &m));
-  TRY(get(c, "lvW3", 2 * N, &r));
   TRY(upload(c, d, dW64, N * 8));
   TRY(upload(c, s, sW64, N * 8));
   TRY(upload(c, m, mW64, N * 8));
   TRY(upload(c, r, r64, 2 * N * 8));
-  launch_level_weights_f64(c->stream, d, s, m, r, *D1, W, H, focal_level, wk.dW, wk.sW, wk.mW, wk.rW, wk.cW);
+  view = make_view(wk, W, H, D1, D2, wk.u[0], 0);
+  view.delk = d;
+  view.smk = s;
+  view.mag64 = m;
+  view.reg64 = r;
+  view.f_level = focal_level;
   return RGBDFLOW_OK;
 }
 
-bool bad_dims(int W, int H) { return W < 2 || H < 2 || (long long)W * H > (1LL << 28); }
+// ------------------------------------------------------------------ batches
+int load_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const rgbdflow_pair_input& in, PairSlot& ps) {
+  if (bad_dims(in.W, in.H) || !in.im1 || !in.dm1 || !in.im2 || !in.dm2)
+    return fail(c, RGBDFLOW_ERR_ARG, "bad arguments of pair " + std::to_string(idx));
+  ps.W = in.W;
+  ps.H = in.H;
+  ps.cam_given = in.cam.xRes != 0;
+  ps.cam = in.cam;
+  ps.min_width = in.min_width > 0 ? in.min_width : p->min_width;
+  ps.sigma_given = in.sigma1 != nullptr;
+  ps.hpb_given = in.hpb != nullptr;
+  ps.L = pyramid_shape(in.W, in.H, p->pyramid_ratio, ps.min_width, nullptr, nullptr);
+  const int nL = (int)ps.L.size();
+  if (nL > kMaxLevels) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "too many pyramid levels");
+  ps.off.assign(nL, 0);
+  for (int k = 2; k < nL; k++) ps.off[k] = ps.off[k - 1] + (size_t)ps.L[k - 1].W * ps.L[k - 1].H;
+  ps.hoff.assign(nL, 0);
+  for (int k = 1; k < nL; k++) ps.hoff[k] = ps.hoff[k - 1] + (size_t)ps.L[k - 1].W * ps.L[k - 1].H * 2;
+  const size_t N = (size_t)in.W * in.H, rest = pyramid_elems(ps.L, 1) + 1;
+  const std::string pre = "b" + std::to_string(idx) + ".";
+  TRY(get(c, pre + "im1", 3 * N, &ps.im1));
+  TRY(get(c, pre + "im2", 3 * N, &ps.im2));
+  TRY(get(c, pre + "dm1", N, &ps.dm1));
+  TRY(get(c, pre + "dm2", N, &ps.dm2));
+  TRY(get(c, pre + "del0", N, &ps.del0));
+  TRY(get(c, pre + "sm0", N, &ps.sm0));
+  TRY(get(c, pre + "sig", N, &ps.sigma));
+  TRY(get(c, pre + "pim1", 3 * rest, &ps.pim1));
+  TRY(get(c, pre + "pim2", 3 * rest, &ps.pim2));
+  TRY(get(c, pre + "pd1", rest, &ps.pd1));
+  TRY(get(c, pre + "pd2", rest, &ps.pd2));
+  TRY(get(c, pre + "pdel", rest, &ps.pdel));
+  TRY(get(c, pre + "psm", rest, &ps.psm));
+  ps.hpb = nullptr;
+  if (in.hpb) TRY(get(c, pre + "hpb", pyramid_elems(ps.L, 0) * 2, &ps.hpb));
+  TRY(get_work(c, pre, N, ps.wk));
+  TRY(upload_planes(c, in.im1, N, 3, ps.im1));
+  TRY(upload_planes(c, in.im2, N, 3, ps.im2));
+  TRY(upload(c, ps.dm1, in.dm1, N * 8));
+  TRY(upload(c, ps.dm2, in.dm2, N * 8));
+  if (in.sigma1) TRY(upload(c, ps.sigma, in.sigma1, N * 4));
+  if (in.hpb) TRY(upload(c, ps.hpb, in.hpb, pyramid_elems(ps.L, 0) * 2 * 4));
+  return sync(c);  // the caller may release its arrays after return
+}
+
+// Level k's planes of one of a pair's pyramids (level 0 = the base image).
+double* level_of(const PairSlot& ps, double* base, double* rest, int C, int k) {
+  if (k == 0) return base;
+  return rest + ps.off[k] * C;
+}
+
+// Builds (or reuses) the descriptors of a batch run under params p.
+int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
+  BatchPlan& bp = c->plan;
+  const unsigned char* pb = reinterpret_cast<const unsigned char*>(p);
+  if (bp.valid && bp.gen == c->gen && bp.plans_gen == c->plans_gen && bp.params.size() == sizeof(*p) &&
+      std::memcmp(bp.params.data(), pb, sizeof(*p)) == 0)
+    return RGBDFLOW_OK;
+  bp.valid = false;
+  bp.host.clear();
+  bp.steps.clear();
+  bp.waves.clear();
+  const int B = (int)c->pairs.size();
+  // per-pair preparation
+  {
+    std::vector<PairPrep> items(B);
+    std::vector<int> start(B + 1, 0);
+    for (int i = 0; i < B; i++) {
+      const PairSlot& ps = c->pairs[i];
+      const rgbdflow_camera& cam = ps.cam_given ? ps.cam : p->cam;
+      PairPrep& q = items[i];
+      q.dm1 = ps.dm1;
+      q.sigma = ps.sigma;
+      q.del0 = ps.del0;
+      q.sm0 = ps.sm0;
+      q.W = ps.W;
+      q.H = ps.H;
+      q.compute_sigma = ps.sigma_given ? 0 : 1;
+      q.dvc = p->depth_vs_color_data_weight;
+      const double f0 = cam.focalLength;
+      q.f2 = f0 * f0;
+      start[i + 1] = start[i] + pix_blocks(ps.W * ps.H);
+    }
+    bp.prep_items = pack(bp.host, items.data(), items.size() * sizeof(PairPrep));
+    bp.prep_start = pack(bp.host, start.data(), start.size() * sizeof(int));
+    bp.prep_n = B;
+    bp.prep_blocks = start[B];
+  }
+  // the six pyramids of sceneFlowMultiscale.cpp:40-85 (the raw-depth pyramid
+  // feeds only the boundary probabilities, which the caller supplies), in
+  // waves: wave w makes levels w ns + 1 .. (w + 1) ns of every pyramid, which
+  // read only level 0 or levels of earlier waves
+  {
+    int ns = 0;
+    double ratio = 0;
+    pyramid_shape(16, 16, p->pyramid_ratio, 1 << 30, &ns, &ratio);
+    std::vector<std::vector<PyrJob>> waves;
+    for (int i = 0; i < B; i++) {
+      PairSlot& ps = c->pairs[i];
+      const int nL = (int)ps.L.size();
+      struct Img {
+        double *base, *rest;
+        int C;
+      } imgs[6] = {{ps.im1, ps.pim1, 3}, {ps.im2, ps.pim2, 3},  {ps.dm1, ps.pd1, 1},
+                   {ps.dm2, ps.pd2, 1},  {ps.del0, ps.pdel, 1}, {ps.sm0, ps.psm, 1}};
+      for (const Img& im : imgs) {
+        std::vector<double*> lvl(nL);
+        for (int k = 0; k < nL; k++) lvl[k] = level_of(ps, im.base, im.rest, im.C, k);
+        for (int k = 1; k < nL; k++) {
+          PyrJob j;
+          if (!pyramid_job(ps.L, ns, ratio, im.C, lvl, k, &j))
+            return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "pyramid filter wider than 31 taps");
+          const int w = (k - 1) / ns;
+          if ((int)waves.size() <= w) waves.resize(w + 1);
+          waves[w].push_back(j);
+        }
+      }
+    }
+    for (const std::vector<PyrJob>& jobs : waves) {
+      std::vector<int> start(jobs.size() + 1, 0);
+      for (size_t j = 0; j < jobs.size(); j++) start[j + 1] = start[j] + pyramid_job_blocks(jobs[j]);
+      BatchPlan::Wave wv;
+      wv.items = pack(bp.host, jobs.data(), jobs.size() * sizeof(PyrJob));
+      wv.start = pack(bp.host, start.data(), start.size() * sizeof(int));
+      wv.n = (int)jobs.size();
+      wv.blocks = start.back();
+      wv.wl = WorkList<PyrJob>{};
+      bp.waves.push_back(wv);
+    }
+  }
+  // the level steps (sceneFlowMultiscale.cpp:95-196), aligned on the finest level
+  int S = 0;
+  for (const PairSlot& ps : c->pairs) S = std::max(S, (int)ps.L.size());
+  for (int s = 0; s < S; s++) {
+    const int k = S - 1 - s;
+    std::vector<LevelView> items;
+    for (int i = 0; i < B; i++) {
+      PairSlot& ps = c->pairs[i];
+      const int nL = (int)ps.L.size();
+      if (k >= nL) continue;
+      const rgbdflow_camera& cam = ps.cam_given ? ps.cam : p->cam;
+      const int Wk = ps.L[k].W, Hk = ps.L[k].H;
+      const int slot = (nL - 1 - k) & 1;  // flow buffers alternate level by level
+      LevelView v = make_view(ps.wk, Wk, Hk, level_of(ps, ps.dm1, ps.pd1, 1, k), level_of(ps, ps.dm2, ps.pd2, 1, k),
+                              ps.wk.u[slot], k);
+      v.allow_early = k < nL - 1;
+      v.P1k = level_of(ps, ps.im1, ps.pim1, 3, k);
+      v.P2k = level_of(ps, ps.im2, ps.pim2, 3, k);
+      v.delk = level_of(ps, ps.del0, ps.pdel, 1, k);
+      v.smk = level_of(ps, ps.sm0, ps.psm, 1, k);
+      v.hpbk = ps.hpb ? ps.hpb + ps.hoff[k] : nullptr;
+      if (k < nL - 1) {
+        v.u_prev = ps.wk.u[1 - slot];
+        v.Wp = ps.L[k + 1].W;
+        v.Hp = ps.L[k + 1].H;
+      }
+      v.uv_scale = 1 / p->pyramid_ratio;
+      v.reg_type = p->regularization_type;
+      v.W_full = (int)cam.xRes;
+      v.f_full = cam.focalLength;
+      v.f_level = downsized_focal(cam, Wk);
+      items.push_back(v);
+    }
+    StepPlan st;
+    plan_step_relax(c, p, items, st);
+    pack_step(bp.host, items, st);
+    bp.steps.push_back(st);
+  }
+  char* dev;
+  TRY(upload_desc(c, "desc", bp.host, &dev));
+  for (StepPlan& st : bp.steps) bind_step(dev, st);
+  for (BatchPlan::Wave& wv : bp.waves)
+    wv.wl = WorkList<PyrJob>{reinterpret_cast<const PyrJob*>(dev + wv.items),
+                             reinterpret_cast<const int*>(dev + wv.start), wv.n};
+  bp.prep = WorkList<PairPrep>{reinterpret_cast<const PairPrep*>(dev + bp.prep_items),
+                               reinterpret_cast<const int*>(dev + bp.prep_start), bp.prep_n};
+  bp.params.assign(pb, pb + sizeof(*p));
+  bp.gen = c->gen;
+  bp.plans_gen = c->plans_gen;
+  bp.valid = true;
+  return RGBDFLOW_OK;
+}
+
+int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
+  if (!p) return fail(c, RGBDFLOW_ERR_ARG, "null params");
+  if (p->variant != RGBDFLOW_VARIANT_FULL) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "bad scene flow variant");
+  if (p->regularization_type < 0 || p->regularization_type > 2) return fail(c, RGBDFLOW_ERR_ARG, "bad regularization");
+  return validate_iters(c, p->n_outer, p->n_inner, p->n_sor);
+}
+
+int batch_load(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const rgbdflow_pair_input* in) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (!p || n < 1 || !in) return fail(c, RGBDFLOW_ERR_ARG, "bad batch arguments");
+  if (p->regularization_type == RGBDFLOW_REG_HPB)
+    for (int i = 0; i < n; i++)
+      if (!in[i].hpb) return fail(c, RGBDFLOW_ERR_ARG, "HPB regularisation needs the boundary maps (hpb)");
+  HIP_TRY(c, hipSetDevice(c->device));
+  c->loaded = false;
+  c->ran = false;
+  c->gen++;
+  c->pairs.assign(n, PairSlot());
+  for (int i = 0; i < n; i++) TRY(load_pair(c, p, i, in[i], c->pairs[i]));
+  c->load_ratio = p->pyramid_ratio;
+  c->load_min_width = p->min_width;
+  c->loaded = true;
+  return RGBDFLOW_OK;
+}
+
+int batch_run(rgbdflow_ctx* c, const rgbdflow_params* p, rgbdflow_stats* st) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  TRY(check_run_params(c, p));
+  if (!c->loaded) return fail(c, RGBDFLOW_ERR_STATE, "run before load");
+  if (p->pyramid_ratio != c->load_ratio || p->min_width != c->load_min_width)
+    return fail(c, RGBDFLOW_ERR_STATE, "pyramid shape differs from the load's params");
+  for (size_t i = 0; i < c->pairs.size(); i++) {
+    const PairSlot& ps = c->pairs[i];
+    const rgbdflow_camera& cam = ps.cam_given ? ps.cam : p->cam;
+    if ((int)cam.xRes != ps.W || (int)cam.yRes != ps.H)
+      return fail(c, RGBDFLOW_ERR_ARG, "camera resolution differs from the image size of pair " + std::to_string(i));
+    if (p->regularization_type == RGBDFLOW_REG_HPB && !ps.hpb_given)
+      return fail(c, RGBDFLOW_ERR_ARG, "HPB regularisation needs the boundary maps (hpb)");
+  }
+  HIP_TRY(c, hipSetDevice(c->device));
+  TRY(batch_plan(c, p));
+  const BatchPlan& bp = c->plan;
+  hipStream_t s = c->stream;
+  const SolveScalars k = scalars_from(p, p->color_data_weight, p->alpha, p->gamma, p->n_outer, p->n_inner, p->n_sor);
+  c->relax_ev_used = 0;
+  HIP_TRY(c, hipEventRecord(c->ev0, s));
+  // per-pair weights: depth stdevs (runFlowTwoFrames.cpp:89-91) -> delta
+  // (OpticalFlow.cpp:71-81); s = f^2 (OpticalFlow.cpp:90-97); then the pyramids
+  launch_pair_prep(s, bp.prep, bp.prep_blocks);
+  for (const BatchPlan::Wave& wv : bp.waves) launch_pyramid_jobs(s, wv.wl, wv.blocks);
+  // coarse to fine
+  for (const StepPlan& sp : bp.steps) {
+    launch_features(s, sp.vpix, sp.pix);
+    launch_level_entry(s, sp.vpix, sp.pix);
+    launch_level_weights(s, sp.vpix, sp.pix);
+    enqueue_level_solve(c, sp, k);
+  }
+  HIP_TRY(c, hipEventRecord(c->ev1, s));
+  HIP_TRY(c, hipGetLastError());
+  c->ran = true;
+  if (!st) return RGBDFLOW_OK;
+  std::vector<std::vector<SolveCtl>> ctl(c->pairs.size());
+  for (size_t i = 0; i < c->pairs.size(); i++) {
+    ctl[i].resize(c->pairs[i].L.size());
+    TRY(download(c, ctl[i].data(), c->pairs[i].wk.ctl, ctl[i].size() * sizeof(SolveCtl)));
+  }
+  TRY(sync(c));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  for (size_t i = 0; i < c->pairs.size(); i++) {
+    const PairSlot& ps = c->pairs[i];
+    rgbdflow_stats& o = st[i];
+    std::memset(&o, 0, sizeof(o));
+    const int nL = (int)ps.L.size();
+    o.n_levels = nL;
+    for (int l = 0; l < nL; l++) {
+      o.width[l] = ps.L[l].W;
+      o.height[l] = ps.L[l].H;
+      ctl_to_stats(ctl[i][l], k.n_outer, l < nL - 1, o.outer_exec[l], o.sweeps_exec[l], o.last_stat[l]);
+    }
+    o.device_ms = ms;
+    collect_relax_timing(c, &o);
+  }
+  return RGBDFLOW_OK;
+}
+
+int batch_read(rgbdflow_ctx* c, int i, double* u, double* v, double* w) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (!c->ran) return fail(c, RGBDFLOW_ERR_STATE, "read before run");
+  if (i < 0 || i >= (int)c->pairs.size()) return fail(c, RGBDFLOW_ERR_ARG, "pair index out of range");
+  if (!u || !v || !w) return fail(c, RGBDFLOW_ERR_ARG, "null output");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const PairSlot& ps = c->pairs[i];
+  const size_t N = (size_t)ps.W * ps.H;
+  const double* uu = ps.wk.u[((int)ps.L.size() - 1) & 1];
+  TRY(download(c, u, uu, N * 8));
+  TRY(download(c, v, uu + N, N * 8));
+  TRY(download(c, w, uu + 2 * N, N * 8));
+  return sync(c);
+}
+
+void fill_stats(rgbdflow_ctx* c, const SolveCtl& ctl, const LevelView& view, int n_outer, rgbdflow_stats* st) {
+  if (!st) return;
+  std::memset(st, 0, sizeof(*st));
+  st->n_levels = 1;
+  st->width[0] = view.W;
+  st->height[0] = view.H;
+  ctl_to_stats(ctl, n_outer, view.allow_early != 0, st->outer_exec[0], st->sweeps_exec[0], st->last_stat[0]);
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  st->device_ms = ms;
+  collect_relax_timing(c, st);
+}
 
 }  // namespace
 
@@ -614,32 +1059,27 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
   HIP_TRY(c, hipSetDevice(c->device));
   const size_t N = (size_t)W * H;
   Work wk;
-  double *D1, *D2;
+  LevelView view;
   TRY(stage_level_inputs(c, W, H, focal_level, Im1, dm1, Im2, dm2, warpIm2, warpDm2, u, v, w, dW, sW, mW, rW, wk,
-                         &D1, &D2));
-  const SolveScalars k = scalars_from(opts, cw, alpha, gamma, nOuter, nInner, nSOR, allowEarly);
-  LevelView view = make_view(wk, W, H, wk.I1, wk.I2, D1, D2, wk.u[0], 0);
+                         view));
+  view.allow_early = allowEarly ? 1 : 0;
+  const SolveScalars k = scalars_from(opts, cw, alpha, gamma, nOuter, nInner, nSOR);
+  std::vector<LevelView> items{view};
+  StepPlan sp;
+  TRY(one_step(c, "d.desc", items, opts, sp));
+  hipStream_t s = c->stream;
   c->relax_ev_used = 0;
-  HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-  enqueue_level_solve(c, view, k, make_relax_plan(c, W, H, opts));
-  HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev0, s));
+  launch_level_weights(s, sp.vpix, sp.pix);
+  enqueue_level_solve(c, sp, k);
+  HIP_TRY(c, hipEventRecord(c->ev1, s));
+  SolveCtl ctl;
+  TRY(download(c, &ctl, view.ctl, sizeof(SolveCtl)));
   TRY(download(c, u, wk.u[0], N * 8));
   TRY(download(c, v, wk.u[0] + N, N * 8));
   TRY(download(c, w, wk.u[0] + 2 * N, N * 8));
-  SolveCtl ctl;
-  TRY(download(c, &ctl, view.ctl, sizeof(SolveCtl)));
   TRY(sync(c));
-  if (st) {
-    std::memset(st, 0, sizeof(*st));
-    st->n_levels = 1;
-    st->width[0] = W;
-    st->height[0] = H;
-    ctl_to_stats(ctl, k, st->outer_exec[0], st->sweeps_exec[0], st->last_stat[0]);
-    float ms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    st->device_ms = ms;
-    collect_relax_timing(c, st);
-  }
+  fill_stats(c, ctl, view, nOuter, st);
   return RGBDFLOW_OK;
 }
 
@@ -655,19 +1095,23 @@ int rgbdflow_level_system(rgbdflow_ctx* c, int W, int H, float focal_level, cons
   HIP_TRY(c, hipSetDevice(c->device));
   const size_t N = (size_t)W * H;
   Work wk;
-  double *D1, *D2;
+  LevelView view;
   TRY(stage_level_inputs(c, W, H, focal_level, Im1, dm1, Im2, dm2, warpIm2, warpDm2, u, v, w, dW, sW, mW, rW, wk,
-                         &D1, &D2));
-  const SolveScalars k = scalars_from(nullptr, cw, alpha, gamma, 1, 1, 0, 0);
-  LevelView view = make_view(wk, W, H, wk.I1, wk.I2, D1, D2, wk.u[0], 0);
-  launch_ctl_reset(c->stream, view.ctl);
-  launch_derivs(c->stream, view, 0);
-  launch_phi(c->stream, view, 0);
-  launch_system(c->stream, view, k, 0);
-  std::vector<float> host((size_t)kCoef * N);
-  TRY(download(c, host.data(), wk.coef, host.size() * sizeof(float)));
+                         view));
+  const SolveScalars k = scalars_from(nullptr, cw, alpha, gamma, 1, 1, 0);
+  std::vector<LevelView> items{view};
+  StepPlan sp;
+  TRY(one_step(c, "d.desc", items, nullptr, sp));
+  hipStream_t s = c->stream;
+  launch_level_weights(s, sp.vpix, sp.pix);
+  launch_ctl_reset(s, sp.vpix);
+  launch_derivs(s, sp.vpix, sp.pix, 0);
+  launch_phi(s, sp.vpix, sp.pix);
+  launch_system(s, sp.vpix, sp.pix, k);
+  std::vector<float> host_c((size_t)kCoef * N);
+  TRY(download(c, host_c.data(), wk.coef, host_c.size() * sizeof(float)));
   TRY(sync(c));
-  for (size_t i = 0; i < host.size(); i++) coef13[i] = host[i];
+  for (size_t i = 0; i < host_c.size(); i++) coef13[i] = host_c[i];
   return RGBDFLOW_OK;
 }
 
@@ -675,24 +1119,24 @@ int rgbdflow_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, int n_it
                    double* x) {
   if (!c) return RGBDFLOW_ERR_ARG;
   if (bad_dims(W, H) || !coef13 || !x || n_iter < 0) return fail(c, RGBDFLOW_ERR_ARG, "bad relax arguments");
+  if (n_iter / kCheckEvery + 1 > kMaxChecks) return fail(c, RGBDFLOW_ERR_ARG, "n_iter too large");
   HIP_TRY(c, hipSetDevice(c->device));
   const size_t N = (size_t)W * H;
   Work wk;
-  TRY(get_work(c, N, wk));
+  TRY(get_work(c, "d.", N, wk));
   std::vector<float> hc((size_t)kCoef * N), hx(3 * N);
   for (size_t i = 0; i < hc.size(); i++) hc[i] = (float)coef13[i];
   for (size_t i = 0; i < hx.size(); i++) hx[i] = (float)x[i];
   TRY(upload(c, wk.coef, hc.data(), hc.size() * 4));
   TRY(upload(c, wk.du, hx.data(), hx.size() * 4));
-  LevelView view = make_view(wk, W, H, wk.I1, wk.I2, nullptr, nullptr, wk.u[0], 0);
-  launch_ctl_reset(c->stream, view.ctl);
-  const RelaxPlan plan = make_relax_plan(c, W, H, sched);
+  std::vector<LevelView> items{make_view(wk, W, H, nullptr, nullptr, wk.u[0], 0)};
+  StepPlan sp;
+  TRY(one_step(c, "d.desc", items, sched, sp));
+  launch_ctl_reset(c->stream, sp.vpix);
   SolveScalars k{};
   k.n_sor = n_iter;
-  k.inner_exit = 0;
-  // checks are recorded but never stop this call (inner_exit = 0)
-  if (n_iter / kCheckEvery + 1 > kMaxChecks) return fail(c, RGBDFLOW_ERR_ARG, "n_iter too large");
-  enqueue_relax(c, view, k, plan, 0);
+  k.inner_exit = 0;  // checks are recorded but never stop this call
+  enqueue_relax(c, sp, k, 0);
   TRY(download(c, hx.data(), wk.du, hx.size() * 4));
   TRY(sync(c));
   for (size_t i = 0; i < hx.size(); i++) x[i] = hx[i];
@@ -733,12 +1177,31 @@ int rgbdflow_gaussian_pyramid(rgbdflow_ctx* c, const double* img, int W, int H, 
     lvl[i] = rest + off;
     off += (size_t)L[i].W * L[i].H * C;
   }
-  build_pyramid(c->stream, L, ns, r, C, lvl);
+  // one launch per wave (a wave's levels read only level 0 or earlier waves)
+  std::vector<std::vector<PyrJob>> waves;
+  for (int i = 1; i < (int)L.size(); i++) {
+    PyrJob j;
+    if (!pyramid_job(L, ns, r, C, lvl, i, &j)) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "filter too wide");
+    const int w = (i - 1) / ns;
+    if ((int)waves.size() <= w) waves.resize(w + 1);
+    waves[w].push_back(j);
+  }
+  for (const std::vector<PyrJob>& jobs : waves) {
+    std::vector<char> host;
+    std::vector<int> start(jobs.size() + 1, 0);
+    for (size_t j = 0; j < jobs.size(); j++) start[j + 1] = start[j] + pyramid_job_blocks(jobs[j]);
+    const size_t io = pack(host, jobs.data(), jobs.size() * sizeof(PyrJob));
+    const size_t so = pack(host, start.data(), start.size() * sizeof(int));
+    char* dev;
+    TRY(upload_desc(c, "gp_desc", host, &dev));
+    const WorkList<PyrJob> wl{reinterpret_cast<const PyrJob*>(dev + io), reinterpret_cast<const int*>(dev + so),
+                              (int)jobs.size()};
+    launch_pyramid_jobs(c->stream, wl, start.back());
+  }
   size_t hoff = 0;
   for (size_t i = 0; i < L.size(); i++) {
     const size_t n = (size_t)L[i].W * L[i].H;
     TRY(download_planes(c, lvl[i], n, C, out + hoff));
-    TRY(sync(c));
     hoff += n * C;
   }
   return RGBDFLOW_OK;
@@ -780,9 +1243,8 @@ int rgbdflow_im2feature(rgbdflow_ctx* c, const double* bgr, int W, int H, double
   TRY(get(c, "feat_in", 3 * N, &in));
   TRY(get(c, "feat_out", kFeat * N, &f));
   TRY(upload_planes(c, bgr, N, 3, in));
-  launch_features(c->stream, in, W, H, f);
-  TRY(download_planes(c, f, N, kFeat, out5));
-  return sync(c);
+  launch_features_planes(c->stream, in, W, H, f);
+  return download_planes(c, f, N, kFeat, out5);
 }
 
 int rgbdflow_warp_by_3d_flow(rgbdflow_ctx* c, const double* I1, const double* D1, const double* I2, const double* D2,
@@ -795,19 +1257,27 @@ int rgbdflow_warp_by_3d_flow(rgbdflow_ctx* c, const double* I1, const double* D1
   HIP_TRY(c, hipSetDevice(c->device));
   const size_t N = (size_t)W * H;
   Work wk;
-  TRY(get_work(c, N, wk));
+  TRY(get_work(c, "d.", N, wk));
   double *d1, *d2;
-  TRY(get(c, "lvD1", N, &d1));
-  TRY(get(c, "lvD2", N, &d2));
+  TRY(get(c, "d.D1", N, &d1));
+  TRY(get(c, "d.D2", N, &d2));
   TRY(upload_planes(c, I1, N, kFeat, wk.I1));
   TRY(upload_planes(c, I2, N, kFeat, wk.I2));
   TRY(upload(c, d1, D1, N * 8));
   TRY(upload(c, d2, D2, N * 8));
-  TRY(upload(c, wk.u[0], u, N * 8));
-  TRY(upload(c, wk.u[0] + N, v, N * 8));
-  TRY(upload(c, wk.u[0] + 2 * N, w, N * 8));
-  LevelView view = make_view(wk, W, H, wk.I1, wk.I2, d1, d2, wk.u[0], 0);
-  launch_warp_cpu_style(c->stream, view);
+  TRY(upload(c, wk.u[1], u, N * 8));
+  TRY(upload(c, wk.u[1] + N, v, N * 8));
+  TRY(upload(c, wk.u[1] + 2 * N, w, N * 8));
+  // the level entry with a flow of the level's own size: u taken as it is, then the warp
+  LevelView view = make_view(wk, W, H, d1, d2, wk.u[0], 0);
+  view.u_prev = wk.u[1];
+  view.Wp = W;
+  view.Hp = H;
+  view.uv_scale = 1;
+  std::vector<LevelView> items{view};
+  StepPlan sp;
+  TRY(one_step(c, "d.desc", items, nullptr, sp));
+  launch_level_entry(c->stream, sp.vpix, sp.pix);
   TRY(download_planes(c, wk.Iw, N, kFeat, wI));
   TRY(download(c, wD, wk.Dw, N * 8));
   return sync(c);
@@ -818,204 +1288,72 @@ int rgbdflow_upsample_flow(rgbdflow_ctx* c, const double* u, const double* v, co
   if (!c) return RGBDFLOW_ERR_ARG;
   if (bad_dims(Ws, Hs) || bad_dims(Wd, Hd) || !u || !v || !w || !uo || !vo || !wo)
     return fail(c, RGBDFLOW_ERR_ARG, "bad upsample arguments");
+  if (Ws == Wd && Hs == Hd) return fail(c, RGBDFLOW_ERR_ARG, "upsample_flow: source and target sizes are equal");
   HIP_TRY(c, hipSetDevice(c->device));
   const size_t Ns = (size_t)Ws * Hs, Nd = (size_t)Wd * Hd;
-  double *a, *b;
+  // the level entry of a Wd x Hd level from a Ws x Hs flow; its warp writes
+  // scratch planes (frames of zeros)
+  Work wk;
+  TRY(get_work(c, "d.", Nd, wk));
+  double *a, *d1, *d2;
   TRY(get(c, "ups_in", 3 * Ns, &a));
-  TRY(get(c, "ups_out", 3 * Nd, &b));
+  TRY(get(c, "d.D1", Nd, &d1));
+  TRY(get(c, "d.D2", Nd, &d2));
+  HIP_TRY(c, hipMemsetAsync(wk.I1, 0, kFeat * Nd * 8, c->stream));
+  HIP_TRY(c, hipMemsetAsync(wk.I2, 0, kFeat * Nd * 8, c->stream));
+  HIP_TRY(c, hipMemsetAsync(d1, 0, Nd * 8, c->stream));
+  HIP_TRY(c, hipMemsetAsync(d2, 0, Nd * 8, c->stream));
   TRY(upload(c, a, u, Ns * 8));
   TRY(upload(c, a + Ns, v, Ns * 8));
   TRY(upload(c, a + 2 * Ns, w, Ns * 8));
-  launch_upsample(c->stream, a, Ws, Hs, b, Wd, Hd, uv_scale);
-  TRY(download(c, uo, b, Nd * 8));
-  TRY(download(c, vo, b + Nd, Nd * 8));
-  TRY(download(c, wo, b + 2 * Nd, Nd * 8));
+  LevelView view = make_view(wk, Wd, Hd, d1, d2, wk.u[0], 0);
+  view.u_prev = a;
+  view.Wp = Ws;
+  view.Hp = Hs;
+  view.uv_scale = uv_scale;
+  std::vector<LevelView> items{view};
+  StepPlan sp;
+  TRY(one_step(c, "d.desc", items, nullptr, sp));
+  launch_level_entry(c->stream, sp.vpix, sp.pix);
+  TRY(download(c, uo, wk.u[0], Nd * 8));
+  TRY(download(c, vo, wk.u[0] + Nd, Nd * 8));
+  TRY(download(c, wo, wk.u[0] + 2 * Nd, Nd * 8));
   return sync(c);
 }
+
+// ---------------------------------------------------------------- batches
+int rgbdflow_batch_load(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const rgbdflow_pair_input* pairs) {
+  return batch_load(c, p, n, pairs);
+}
+int rgbdflow_batch_run(rgbdflow_ctx* c, const rgbdflow_params* p, rgbdflow_stats* stats) {
+  return batch_run(c, p, stats);
+}
+int rgbdflow_batch_read(rgbdflow_ctx* c, int i, double* u, double* v, double* w) { return batch_read(c, i, u, v, w); }
 
 // ---------------------------------------------------------------- pair path
 int rgbdflow_pair_load(rgbdflow_ctx* c, const rgbdflow_params* p, int W, int H, const double* im1, const double* dm1,
                        const double* im2, const double* dm2, const float* sigma1, const float* hpb) {
   if (!c) return RGBDFLOW_ERR_ARG;
-  if (!p || bad_dims(W, H) || !im1 || !dm1 || !im2 || !dm2) return fail(c, RGBDFLOW_ERR_ARG, "bad pair arguments");
-  if (p->regularization_type == RGBDFLOW_REG_HPB && !hpb)
-    return fail(c, RGBDFLOW_ERR_ARG, "HPB regularisation needs the boundary maps (hpb)");
-  HIP_TRY(c, hipSetDevice(c->device));
-  c->pair_loaded = false;
-  const size_t N = (size_t)W * H;
-  const std::vector<LevelShape> L = pyramid_shape(W, H, p->pyramid_ratio, p->min_width, nullptr, nullptr);
-  if ((int)L.size() > kMaxLevels) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "too many pyramid levels");
-  double *i1, *i2, *d1, *d2;
-  float* s1;
-  TRY(get(c, "p_im1", 3 * N, &i1));
-  TRY(get(c, "p_im2", 3 * N, &i2));
-  TRY(get(c, "p_dm1", N, &d1));
-  TRY(get(c, "p_dm2", N, &d2));
-  TRY(get(c, "p_sig1", N, &s1));
-  TRY(upload_planes(c, im1, N, 3, i1));
-  TRY(upload_planes(c, im2, N, 3, i2));
-  TRY(upload(c, d1, dm1, N * 8));
-  TRY(upload(c, d2, dm2, N * 8));
-  if (sigma1) TRY(upload(c, s1, sigma1, N * 4));
-  if (hpb) {
-    float* h;
-    const size_t nh = pyramid_elems(L, 0) * 2;
-    TRY(get(c, "p_hpb", nh, &h));
-    TRY(upload(c, h, hpb, nh * 4));
-  }
-  TRY(sync(c));
-  c->pW = W;
-  c->pH = H;
-  c->p_ratio = p->pyramid_ratio;
-  c->p_min_width = p->min_width;
-  c->p_sigma_given = sigma1 != nullptr;
-  c->p_hpb_given = hpb != nullptr;
-  c->p_levels = L;
-  c->pair_loaded = true;
-  c->pair_ran = false;
-  return RGBDFLOW_OK;
+  if (!p) return fail(c, RGBDFLOW_ERR_ARG, "bad pair arguments");
+  rgbdflow_pair_input in;
+  std::memset(&in, 0, sizeof(in));
+  in.W = W;
+  in.H = H;
+  in.im1 = im1;
+  in.dm1 = dm1;
+  in.im2 = im2;
+  in.dm2 = dm2;
+  in.sigma1 = sigma1;
+  in.hpb = hpb;
+  return batch_load(c, p, 1, &in);
 }
 
 int rgbdflow_pair_run(rgbdflow_ctx* c, const rgbdflow_params* p, rgbdflow_stats* st) {
-  if (!c) return RGBDFLOW_ERR_ARG;
-  if (!p) return fail(c, RGBDFLOW_ERR_ARG, "null params");
-  if (!c->pair_loaded) return fail(c, RGBDFLOW_ERR_STATE, "pair_run before pair_load");
-  if (p->pyramid_ratio != c->p_ratio || p->min_width != c->p_min_width)
-    return fail(c, RGBDFLOW_ERR_STATE, "pyramid shape differs from pair_load's params");
-  if ((int)p->cam.xRes != c->pW || (int)p->cam.yRes != c->pH)
-    return fail(c, RGBDFLOW_ERR_ARG, "params.cam resolution differs from the image size");
-  if (p->variant != RGBDFLOW_VARIANT_FULL) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "bad scene flow variant");
-  if (p->regularization_type == RGBDFLOW_REG_HPB && !c->p_hpb_given)
-    return fail(c, RGBDFLOW_ERR_ARG, "HPB regularisation needs the boundary maps (hpb)");
-  if (p->regularization_type < 0 || p->regularization_type > 2) return fail(c, RGBDFLOW_ERR_ARG, "bad regularization");
-  TRY(validate_iters(c, p->n_outer, p->n_inner, p->n_sor));
-  HIP_TRY(c, hipSetDevice(c->device));
-  const int W = c->pW, H = c->pH;
-  const size_t N = (size_t)W * H;
-  int ns;
-  double ratio;
-  const std::vector<LevelShape> L = pyramid_shape(W, H, p->pyramid_ratio, p->min_width, &ns, &ratio);
-  const int nL = (int)L.size();
-  const size_t rest = pyramid_elems(L, 1);
-  double *i1, *i2, *d1, *d2, *pi1, *pi2, *pd1, *pd2, *pdel, *psm, *del0, *sm0;
-  float *s1, *hpb = nullptr, *dm1f;
-  TRY(get(c, "p_im1", 3 * N, &i1));
-  TRY(get(c, "p_im2", 3 * N, &i2));
-  TRY(get(c, "p_dm1", N, &d1));
-  TRY(get(c, "p_dm2", N, &d2));
-  TRY(get(c, "p_sig1", N, &s1));
-  if (c->p_hpb_given) TRY(get(c, "p_hpb", pyramid_elems(L, 0) * 2, &hpb));
-  TRY(get(c, "py_im1", 3 * rest + 1, &pi1));
-  TRY(get(c, "py_im2", 3 * rest + 1, &pi2));
-  TRY(get(c, "py_d1", rest + 1, &pd1));
-  TRY(get(c, "py_d2", rest + 1, &pd2));
-  TRY(get(c, "py_del", rest + 1, &pdel));
-  TRY(get(c, "py_sm", rest + 1, &psm));
-  TRY(get(c, "p_del0", N, &del0));
-  TRY(get(c, "p_sm0", N, &sm0));
-  TRY(get(c, "p_dm1f", N, &dm1f));
-  Work wk;
-  TRY(get_work(c, N, wk));
-  hipStream_t s = c->stream;
-  c->relax_ev_used = 0;
-  HIP_TRY(c, hipEventRecord(c->ev0, s));
-
-  // per-pair weights: depth stdevs (runFlowTwoFrames.cpp:89-91) -> delta
-  // (OpticalFlow.cpp:71-81); s = f^2 (OpticalFlow.cpp:90-97)
-  if (!c->p_sigma_given) {
-    launch_to_f32(s, d1, N, dm1f);
-    launch_depth_sigma(s, dm1f, W, H, 4, s1);
-  }
-  launch_depth_data_weights(s, s1, (int)N, p->depth_vs_color_data_weight, del0);
-  const double f0 = p->cam.focalLength;
-  launch_fill_f64(s, sm0, N, f0 * f0);
-
-  // the six pyramids of sceneFlowMultiscale.cpp:40-85 (the raw-depth pyramid
-  // feeds only the boundary probabilities, which the caller supplies)
-  auto levels_of = [&](double* base, double* restp, int C) {
-    std::vector<double*> lv(nL);
-    lv[0] = base;
-    size_t off = 0;
-    for (int i = 1; i < nL; i++) {
-      lv[i] = restp + off;
-      off += (size_t)L[i].W * L[i].H * C;
-    }
-    return lv;
-  };
-  std::vector<double*> P1 = levels_of(i1, pi1, 3), P2 = levels_of(i2, pi2, 3), PD1 = levels_of(d1, pd1, 1),
-                       PD2 = levels_of(d2, pd2, 1), PDel = levels_of(del0, pdel, 1), PS = levels_of(sm0, psm, 1);
-  build_pyramid(s, L, ns, ratio, 3, P1);
-  build_pyramid(s, L, ns, ratio, 3, P2);
-  build_pyramid(s, L, ns, ratio, 1, PD1);
-  build_pyramid(s, L, ns, ratio, 1, PD2);
-  build_pyramid(s, L, ns, ratio, 1, PDel);
-  build_pyramid(s, L, ns, ratio, 1, PS);
-
-  std::vector<size_t> hpb_off(nL, 0);
-  for (int k = 1; k < nL; k++) hpb_off[k] = hpb_off[k - 1] + (size_t)L[k - 1].W * L[k - 1].H * 2;
-
-  std::vector<SolveScalars> ks(nL);
-  int cur = 0;
-  for (int k = nL - 1; k >= 0; k--) {
-    const int Wk = L[k].W, Hk = L[k].H;
-    const size_t Nk = (size_t)Wk * Hk;
-    launch_features(s, P1[k], Wk, Hk, wk.I1);
-    launch_features(s, P2[k], Wk, Hk, wk.I2);
-    const float fk = downsized_focal(p->cam, Wk);
-    LevelView view;
-    if (k == nL - 1) {
-      cur = 0;
-      view = make_view(wk, Wk, Hk, wk.I1, wk.I2, PD1[k], PD2[k], wk.u[cur], k);
-      launch_fill_f64(s, wk.u[cur], 3 * Nk, 0.0);
-      HIP_TRY(c, hipMemcpyAsync(wk.Iw, wk.I2, kFeat * Nk * 8, hipMemcpyDeviceToDevice, s));
-      HIP_TRY(c, hipMemcpyAsync(wk.Dw, PD2[k], Nk * 8, hipMemcpyDeviceToDevice, s));
-    } else {
-      const int nxt = 1 - cur;
-      launch_upsample(s, wk.u[cur], L[k + 1].W, L[k + 1].H, wk.u[nxt], Wk, Hk, 1 / p->pyramid_ratio);
-      cur = nxt;
-      view = make_view(wk, Wk, Hk, wk.I1, wk.I2, PD1[k], PD2[k], wk.u[cur], k);
-      launch_warp_cpu_style(s, view);
-    }
-    launch_level_weights(s, PDel[k], PS[k], hpb ? hpb + hpb_off[k] : nullptr, p->regularization_type, PD1[k], Wk, Hk,
-                         (int)p->cam.xRes, p->cam.focalLength, fk, wk.dW, wk.sW, wk.mW, wk.rW, wk.cW);
-    ks[k] = scalars_from(p, p->color_data_weight, p->alpha, p->gamma, p->n_outer, p->n_inner, p->n_sor, k < nL - 1);
-    enqueue_level_solve(c, view, ks[k], make_relax_plan(c, Wk, Hk, p));
-  }
-  c->p_final_slot = cur;
-  HIP_TRY(c, hipEventRecord(c->ev1, s));
-  HIP_TRY(c, hipGetLastError());
-  c->pair_ran = true;
-  if (!st) return RGBDFLOW_OK;
-  std::vector<SolveCtl> ctl(nL);
-  TRY(download(c, ctl.data(), wk.ctl, nL * sizeof(SolveCtl)));
-  TRY(sync(c));
-  std::memset(st, 0, sizeof(*st));
-  st->n_levels = nL;
-  for (int k = 0; k < nL; k++) {
-    st->width[k] = L[k].W;
-    st->height[k] = L[k].H;
-    ctl_to_stats(ctl[k], ks[k], st->outer_exec[k], st->sweeps_exec[k], st->last_stat[k]);
-  }
-  float ms = 0;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
-  st->device_ms = ms;
-  collect_relax_timing(c, st);
-  return RGBDFLOW_OK;
+  if (c && c->loaded && c->pairs.size() != 1) return fail(c, RGBDFLOW_ERR_STATE, "a batch is loaded: use batch_run");
+  return batch_run(c, p, st);
 }
 
-int rgbdflow_pair_read(rgbdflow_ctx* c, double* u, double* v, double* w) {
-  if (!c) return RGBDFLOW_ERR_ARG;
-  if (!c->pair_ran) return fail(c, RGBDFLOW_ERR_STATE, "pair_read before pair_run");
-  if (!u || !v || !w) return fail(c, RGBDFLOW_ERR_ARG, "null output");
-  HIP_TRY(c, hipSetDevice(c->device));
-  const size_t N = (size_t)c->pW * c->pH;
-  double* uu;
-  TRY(get(c, c->p_final_slot ? "u1" : "u0", 3 * N, &uu));
-  TRY(download(c, u, uu, N * 8));
-  TRY(download(c, v, uu + N, N * 8));
-  TRY(download(c, w, uu + 2 * N, N * 8));
-  return sync(c);
-}
+int rgbdflow_pair_read(rgbdflow_ctx* c, double* u, double* v, double* w) { return batch_read(c, 0, u, v, w); }
 
 int rgbdflow_multiscale(rgbdflow_ctx* c, int W, int H, const double* im1, const double* dm1, const double* im2,
                         const double* dm2, const float* sigma1, const float* hpb, const rgbdflow_params* p, double* u,
@@ -1026,17 +1364,32 @@ int rgbdflow_multiscale(rgbdflow_ctx* c, int W, int H, const double* im1, const 
   return rgbdflow_pair_read(c, u, v, w);
 }
 
-int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, rgbdflow_tile_choice* out) {
+int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, rgbdflow_tile_choice* out) {
   if (!c) return RGBDFLOW_ERR_ARG;
-  if (bad_dims(W, H)) return fail(c, RGBDFLOW_ERR_ARG, "bad autotune size");
+  if (bad_dims(W, H) || batch < 1 || batch > 256) return fail(c, RGBDFLOW_ERR_ARG, "bad autotune size");
   HIP_TRY(c, hipSetDevice(c->device));
   const size_t N = (size_t)W * H;
-  Work wk;
-  TRY(get_work(c, N, wk));
-  LevelView view = make_view(wk, W, H, wk.I1, wk.I2, nullptr, nullptr, wk.u[0], 0);
+  // `batch` independent test systems (own coefficients and increments)
+  float *coef, *du, *du_alt;
+  SolveCtl* ctl;
+  TRY(get(c, "at.coef", (size_t)batch * kCoef * N, &coef));
+  TRY(get(c, "at.du", (size_t)batch * 3 * N, &du));
+  TRY(get(c, "at.du_alt", (size_t)batch * 3 * N, &du_alt));
+  TRY(get(c, "at.ctl", batch, &ctl));
   hipStream_t s = c->stream;
-  launch_ctl_reset(s, view.ctl);
-  launch_fill_test_system(s, wk.coef, wk.du, W, H);
+  std::vector<LevelView> items(batch);
+  for (int b = 0; b < batch; b++) {
+    LevelView& v = items[b];
+    std::memset(&v, 0, sizeof(v));
+    v.W = W;
+    v.H = H;
+    v.N = (int)N;
+    v.coef = coef + (size_t)b * kCoef * N;
+    v.du = du + (size_t)b * 3 * N;
+    v.du_alt = du_alt + (size_t)b * 3 * N;
+    v.ctl = ctl + b;
+    launch_fill_test_system(s, v.coef, v.du, W, H);
+  }
   SolveScalars k{};
   k.n_sor = kCheckEvery;  // one check interval: the unit the schedule repeats
   k.inner_exit = 0;
@@ -1078,15 +1431,22 @@ int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, rgbdflow_tile_choice*
   TileSched best{};
   const bool timing = c->time_relax;
   c->time_relax = false;
+  int rc = RGBDFLOW_OK;
   for (const TileSched& ts : cand) {
-    RelaxPlan plan;
-    plan.variant = RGBDFLOW_RELAX_TILED;
-    plan.ts = ts;
-    enqueue_relax(c, view, k, plan, 0);  // warm-up
+    for (LevelView& v : items) v.ts = ts;
+    StepPlan sp;
+    sp.tiled_ok = true;
+    sp.cfg = ts.config;
+    sp.cap = ts.cap;
+    sp.T = ts.tmax;
+    rc = one_step(c, "at.desc", items, nullptr, sp, true);
+    if (rc != RGBDFLOW_OK) break;
+    launch_ctl_reset(s, sp.vpix);
+    enqueue_relax(c, sp, k, 0);  // warm-up
     double t_best = 1e300;
     for (int rep = 0; rep < 3; rep++) {
       hipEventRecord(e0, s);
-      enqueue_relax(c, view, k, plan, 0);
+      enqueue_relax(c, sp, k, 0);
       hipEventRecord(e1, s);
       hipEventSynchronize(e1);
       float ms = 0;
@@ -1102,8 +1462,10 @@ int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, rgbdflow_tile_choice*
   c->time_relax = timing;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  if (rc != RGBDFLOW_OK) return rc;
   TRY(sync(c));
   c->plans[plan_key(W, H, 0, 0, 0, -1)] = best;
+  c->plans_gen++;
   if (out) {
     out->tile_w = best.TW;
     out->tile_h = best.TH;
@@ -1127,71 +1489,85 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
     return fail(c, RGBDFLOW_ERR_ARG, "HPB regularisation needs the boundary map (hpb0)");
   if ((int)p->cam.xRes != W || (int)p->cam.yRes != H)
     return fail(c, RGBDFLOW_ERR_ARG, "params.cam resolution differs from the image size");
-  TRY(validate_iters(c, p->n_outer, p->n_inner, p->n_sor));
+  TRY(check_run_params(c, p));
   HIP_TRY(c, hipSetDevice(c->device));
   const size_t N = (size_t)W * H;
   Work wk;
-  TRY(get_work(c, N, wk));
-  double *i1, *i2, *d1, *d2, *del0, *sm0, *est_d;
-  float *s1, *h = nullptr, *dm1f;
-  TRY(get(c, "p_im1", 3 * N, &i1));
-  TRY(get(c, "p_im2", 3 * N, &i2));
-  TRY(get(c, "p_dm1", N, &d1));
-  TRY(get(c, "p_dm2", N, &d2));
-  TRY(get(c, "p_sig1", N, &s1));
-  TRY(get(c, "p_del0", N, &del0));
-  TRY(get(c, "p_sm0", N, &sm0));
-  TRY(get(c, "p_dm1f", N, &dm1f));
-  TRY(get(c, "wi_est", 3 * N, &est_d));
-  c->pair_loaded = false;
-  c->pair_ran = false;
-  hipStream_t s = c->stream;
+  TRY(get_work(c, "d.", N, wk));
+  double *i1, *i2, *d1, *d2, *del0, *sm0;
+  float *s1, *h = nullptr;
+  TRY(get(c, "wi.im1", 3 * N, &i1));
+  TRY(get(c, "wi.im2", 3 * N, &i2));
+  TRY(get(c, "wi.dm1", N, &d1));
+  TRY(get(c, "wi.dm2", N, &d2));
+  TRY(get(c, "wi.sig", N, &s1));
+  TRY(get(c, "wi.del0", N, &del0));
+  TRY(get(c, "wi.sm0", N, &sm0));
+  if (hpb0) TRY(get(c, "wi.hpb", N * 2, &h));
   TRY(upload_planes(c, im1, N, 3, i1));
   TRY(upload_planes(c, im2, N, 3, i2));
   TRY(upload(c, d1, dm1, N * 8));
   TRY(upload(c, d2, dm2, N * 8));
-  TRY(upload_planes(c, est, N, 3, wk.u[0]));
-  if (hpb0) {
-    TRY(get(c, "p_hpb", N * 2, &h));
-    TRY(upload(c, h, hpb0, N * 2 * 4));
-  }
-  if (sigma1) {
-    TRY(upload(c, s1, sigma1, N * 4));
-  } else {
-    launch_to_f32(s, d1, N, dm1f);
-    launch_depth_sigma(s, dm1f, W, H, 4, s1);
-  }
+  TRY(upload_planes(c, est, N, 3, wk.u[1]));
+  if (hpb0) TRY(upload(c, h, hpb0, N * 2 * 4));
+  if (sigma1) TRY(upload(c, s1, sigma1, N * 4));
+  // per-pair weights, as the batch path computes them
+  PairPrep pp;
+  std::memset(&pp, 0, sizeof(pp));
+  pp.dm1 = d1;
+  pp.sigma = s1;
+  pp.del0 = del0;
+  pp.sm0 = sm0;
+  pp.W = W;
+  pp.H = H;
+  pp.compute_sigma = sigma1 ? 0 : 1;
+  pp.dvc = p->depth_vs_color_data_weight;
+  const double f0 = p->cam.focalLength;
+  pp.f2 = f0 * f0;
+  std::vector<char> ph;
+  const int pstart[2] = {0, pix_blocks((int)N)};
+  const size_t pio = pack(ph, &pp, sizeof(pp));
+  const size_t pso = pack(ph, pstart, sizeof(pstart));
+  char* pdev;
+  TRY(upload_desc(c, "wi.desc", ph, &pdev));
+  const WorkList<PairPrep> pwl{reinterpret_cast<const PairPrep*>(pdev + pio), reinterpret_cast<const int*>(pdev + pso),
+                               1};
+  // sceneFlowWithInit.cpp:7-81: one level at full size from est, no early exit
+  LevelView view = make_view(wk, W, H, d1, d2, wk.u[0], 0);
+  view.allow_early = 0;
+  view.P1k = i1;
+  view.P2k = i2;
+  view.delk = del0;
+  view.smk = sm0;
+  view.hpbk = h;
+  view.u_prev = wk.u[1];
+  view.Wp = W;
+  view.Hp = H;
+  view.uv_scale = 1;
+  view.reg_type = p->regularization_type;
+  view.W_full = (int)p->cam.xRes;
+  view.f_full = p->cam.focalLength;
+  view.f_level = p->cam.focalLength;
+  std::vector<LevelView> items{view};
+  StepPlan sp;
+  TRY(one_step(c, "d.desc", items, p, sp));
+  const SolveScalars k = scalars_from(p, p->color_data_weight, p->alpha, p->gamma, p->n_outer, p->n_inner, p->n_sor);
+  hipStream_t s = c->stream;
   c->relax_ev_used = 0;
   HIP_TRY(c, hipEventRecord(c->ev0, s));
-  launch_depth_data_weights(s, s1, (int)N, p->depth_vs_color_data_weight, del0);
-  const double f0 = p->cam.focalLength;
-  launch_fill_f64(s, sm0, N, f0 * f0);
-  launch_features(s, i1, W, H, wk.I1);
-  launch_features(s, i2, W, H, wk.I2);
-  LevelView view = make_view(wk, W, H, wk.I1, wk.I2, d1, d2, wk.u[0], 0);
-  launch_warp_cpu_style(s, view);
-  launch_level_weights(s, del0, sm0, h, p->regularization_type, d1, W, H, (int)p->cam.xRes, p->cam.focalLength,
-                       p->cam.focalLength, wk.dW, wk.sW, wk.mW, wk.rW, wk.cW);
-  const SolveScalars k = scalars_from(p, p->color_data_weight, p->alpha, p->gamma, p->n_outer, p->n_inner, p->n_sor, 0);
-  enqueue_level_solve(c, view, k, make_relax_plan(c, W, H, p));
+  launch_pair_prep(s, pwl, pstart[1]);
+  launch_features(s, sp.vpix, sp.pix);
+  launch_level_entry(s, sp.vpix, sp.pix);
+  launch_level_weights(s, sp.vpix, sp.pix);
+  enqueue_level_solve(c, sp, k);
   HIP_TRY(c, hipEventRecord(c->ev1, s));
+  SolveCtl ctl;
+  TRY(download(c, &ctl, view.ctl, sizeof(SolveCtl)));
   TRY(download(c, u, wk.u[0], N * 8));
   TRY(download(c, v, wk.u[0] + N, N * 8));
   TRY(download(c, w, wk.u[0] + 2 * N, N * 8));
-  SolveCtl ctl;
-  TRY(download(c, &ctl, view.ctl, sizeof(SolveCtl)));
   TRY(sync(c));
-  if (st) {
-    std::memset(st, 0, sizeof(*st));
-    st->n_levels = 1;
-    st->width[0] = W;
-    st->height[0] = H;
-    ctl_to_stats(ctl, k, st->outer_exec[0], st->sweeps_exec[0], st->last_stat[0]);
-    float ms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    st->device_ms = ms;
-    collect_relax_timing(c, st);
-  }
+  fill_stats(c, ctl, view, p->n_outer, st);
   return RGBDFLOW_OK;
 }
 

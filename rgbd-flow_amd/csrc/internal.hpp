@@ -2,7 +2,11 @@
 //
 // Every image lives in HBM as structure-of-arrays planes: channel c of an
 // image with N pixels is the contiguous range [c*N, (c+1)*N), row-major.
-// Level buffers are sized for the finest level and reused by coarser ones.
+//
+// Kernels run over work-lists: one launch serves every item of a list (the
+// same pyramid level of all frame pairs of a batch, say), and each block finds
+// its item by a binary search over the list's block offsets.  A single pair is
+// a list of one.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
@@ -14,8 +18,9 @@ constexpr int kFeat = 5;            // feature channels: gray, d/dx, d/dy, G-B, 
 constexpr int kCoef = 13;           // compact per-pixel system, see Coef below
 constexpr int kMaxLevels = 16;
 constexpr int kCheckEvery = 25;     // relaxation convergence check period (.cu:905)
-constexpr int kMaxChecks = 4096;    // check slots per level solve
+constexpr int kMaxChecks = 4096;    // check slots per outer iteration
 constexpr int kMaxOuter = 1024;
+constexpr int kBlock = 256;         // threads of the per-pixel kernels
 
 // Plane indices of the compact system of one pixel p:
 //   right/down edge weights alpha*phi_p*r_p (xy rows) and alpha*phi_p*r_p*s_p (z row),
@@ -34,7 +39,17 @@ struct SolveCtl {
   double stat[kMaxOuter];             // outer convergence statistic per outer iteration
 };
 
-// Everything one level-solve kernel needs (passed by value).
+// Tile schedule of the tiled relaxation (kernels_relax_tiled.hip).
+struct TileSched {
+  int TW, TH;            // interior tile size (TW even)
+  int tiles_x, tiles_y;
+  int single;            // 1: one tile covers the level (no halo, up to 25 iterations per launch)
+  int cap;               // pair capacity of a workgroup (LDS plane length)
+  int config;            // kernel configuration index
+  int tmax;              // iterations per launch
+};
+
+// Everything a level kernel needs about one pair at one pyramid level.
 struct LevelView {
   int W, H, N;
   const double* I1;     // [5][N] features of frame 1
@@ -46,11 +61,11 @@ struct LevelView {
   double* u;            // [3][N] flow (u px, v px, w m)
   float* du;            // [3][N] increment solved by the relaxation
   float* du_alt;        // [3][N] second buffer of the tiled relaxation (reads one, writes the other)
-  const float* dW;      // [N] depth-data weights (delta)
-  const float* sW;      // [N] depth-smoothness weights
-  const float* mW;      // [N] depth-magnitude weights
-  const float* rW;      // [2][N] regularisation weights (right, down)
-  const float* cW;      // [N] convergence weights f/D1
+  float* dW;            // [N] depth-data weights (delta)
+  float* sW;            // [N] depth-smoothness weights
+  float* mW;            // [N] depth-magnitude weights
+  float* rW;            // [2][N] regularisation weights (right, down)
+  float* cW;            // [N] convergence weights f/D1
   float* q;             // [N] occlusion weights
   double* Ix;           // [5][N]
   double* Iy;           // [5][N]
@@ -62,65 +77,110 @@ struct LevelView {
   float* coef;          // [13][N]
   double* partials;     // per-block partial sums of the outer statistic
   SolveCtl* ctl;
+  int allow_early;      // outer early exit allowed on this level
+  // level entry (features, upsample, warp, weights)
+  const double* P1k;    // [3][N] colour pyramid level of frame 1 (BGR planes)
+  const double* P2k;    // [3][N] frame 2
+  const double* delk;   // [N] depth-data weight pyramid level (f64)
+  const double* smk;    // [N] depth-smoothness weight pyramid level (f64)
+  const float* hpbk;    // [N][2] boundary probabilities, or null
+  const double* mag64;  // [N] given depth-magnitude weights (single-level entry), or null
+  const double* reg64;  // [N][2] given regularisation weights (single-level entry), or null
+  const double* u_prev; // [3][Np] coarser level's flow, or null at the pair's coarsest level
+  int Wp, Hp;
+  double uv_scale;      // 1 / pyramid ratio
+  int reg_type;
+  int W_full;           // camera xRes (full size)
+  float f_full;         // full-size focal length
+  float f_level;        // focal length of the level
+  TileSched ts;
 };
 
-// Tile schedule of the tiled relaxation (kernels_relax_tiled.hip).
-struct TileSched {
-  int TW, TH;            // interior tile size (TW even)
-  int tiles_x, tiles_y;
-  int single;            // 1: one tile covers the level (no halo, up to 25 iterations per launch)
-  int cap;               // pair capacity of a workgroup (LDS plane length)
-  int config;            // kernel configuration index
-  int tmax;              // iterations per launch
+// A launch's work-list: items plus the first block of each item (n + 1 entries).
+template <class T>
+struct WorkList {
+  const T* items;
+  const int* start;
+  int n;
+};
+
+template <class T>
+__device__ __forceinline__ int work_item(const WorkList<T>& w, int blk) {
+  int lo = 0, hi = w.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (w.start[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+typedef WorkList<LevelView> Views;
+
+// One Gaussian-pyramid level of one image (GaussianPyramid.cpp:47-61).
+constexpr int kMaxTaps = 31;
+struct PyrJob {
+  const double* src;
+  double* dst;
+  int Ws, Hs, Wd, Hd, C, fs;
+  double ratio;
+  double taps[kMaxTaps];
+};
+
+// Per-pair preparation at the start of a solve.
+struct PairPrep {
+  const double* dm1;    // [N] hole-filled depth of frame 1
+  float* sigma;         // [N] depth stdevs (computed here when compute_sigma)
+  double* del0;         // [N] depth-data weights, full size
+  double* sm0;          // [N] depth-smoothness weights, full size
+  int W, H, compute_sigma;
+  double dvc;           // depthVsColorDataWeight
+  double f2;            // focal length squared
 };
 
 struct SolveScalars {
   double alpha, gamma, color_weight;
   int n_outer, n_inner, n_sor;
-  int allow_early;          // outer early exit allowed on this level
   int inner_exit;           // relaxation early exits enabled
   int outer_exit;           // outer early exits enabled
 };
 
+// Block counts of one item for each kernel family.
+__host__ __device__ inline int pix_blocks(int N) { return (N + kBlock - 1) / kBlock; }
+__host__ __device__ inline int plain_blocks(int W, int H) { return (((W + 1) / 2 + 63) / 64) * ((H + 3) / 4); }
+__host__ __device__ inline int tiled_blocks(const TileSched& t) { return t.tiles_x * t.tiles_y; }
+
 // ---------------------------------------------------------------- launchers
-// kernels_solve.hip
-void launch_ctl_reset(hipStream_t s, SolveCtl* ctl);
-void launch_derivs(hipStream_t s, const LevelView& v, int n_slots);
-void launch_phi(hipStream_t s, const LevelView& v, int outer);
-void launch_system(hipStream_t s, const LevelView& v, const SolveScalars& k, int outer);
-void launch_relax_plain(hipStream_t s, const LevelView& v, int colour, int outer, int slot_base,
-                        int n_prev_checks, int check_slot);
-void launch_update_warp(hipStream_t s, const LevelView& v, const SolveScalars& k, int outer);
-int stat_blocks(int N);
+// kernels_solve.hip (blocks = total blocks of the list for that family)
+void launch_ctl_reset(hipStream_t s, const Views& v);
+void launch_derivs(hipStream_t s, const Views& v, int blocks, int n_slots);
+void launch_phi(hipStream_t s, const Views& v, int blocks);
+void launch_system(hipStream_t s, const Views& v, int blocks, const SolveScalars& k);
+void launch_relax_plain(hipStream_t s, const Views& v, int blocks, int colour, int slot_base, int n_prev_checks,
+                        int check_slot);
+void launch_update_warp(hipStream_t s, const Views& v, int blocks, int want_stat);
+void launch_stat_final(hipStream_t s, const Views& v, int outer, int decide);
 
 // kernels_relax_tiled.hip
 int tiled_config_count();
 void tiled_config_shape(int idx, int* nt, int* p);
 size_t tiled_config_lds(int idx);
-void launch_relax_tiled(hipStream_t s, const LevelView& v, const TileSched& ts, const float* src, float* dst,
-                        int n_iter, int slot_base, int n_prev_checks, int check_slot);
-void launch_copy_f32(hipStream_t s, const float* src, size_t n, float* dst);
+void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int cap, int flip, int n_iter,
+                        int slot_base, int n_prev_checks, int check_slot);
+void launch_copy_du(hipStream_t s, const Views& v, int blocks);
 void launch_fill_test_system(hipStream_t s, float* coef, float* du, int W, int H);
 
 // kernels_pyramid.hip
 void launch_deinterleave(hipStream_t s, const double* src, int N, int C, double* dst);
 void launch_interleave(hipStream_t s, const double* src, int N, int C, double* dst);
-void launch_fill_f64(hipStream_t s, double* dst, size_t n, double v);
-void launch_to_f32(hipStream_t s, const double* src, size_t n, float* dst);
-void launch_pyramid_level(hipStream_t s, const double* src, int Ws, int Hs, int C, const double* taps,
-                          int fsize, double ratio, double* dst, int Wd, int Hd);
-void launch_features(hipStream_t s, const double* bgr, int W, int H, double* out5);
-void launch_upsample(hipStream_t s, const double* src, int Ws, int Hs, double* dst, int Wd, int Hd,
-                     double uv_scale);
-void launch_warp_cpu_style(hipStream_t s, const LevelView& v);
-void launch_level_weights(hipStream_t s, const double* delta, const double* smooth, const float* hpb,
-                          int reg_type, const double* D1, int W, int H, int W_full, float f_full,
-                          float f_level, float* dW, float* sW, float* mW, float* rW, float* cW);
-void launch_level_weights_f64(hipStream_t s, const double* delta, const double* smooth, const double* mag,
-                              const double* reg2, const double* D1, int W, int H, float f_level, float* dW,
-                              float* sW, float* mW, float* rW, float* cW);
+void launch_pair_prep(hipStream_t s, const WorkList<PairPrep>& w, int blocks);
+void launch_pyramid_jobs(hipStream_t s, const WorkList<PyrJob>& w, int blocks);
+inline int pyramid_job_blocks(const PyrJob& j) { return pix_blocks(j.Wd * j.Hd * j.C); }
+void launch_features(hipStream_t s, const Views& v, int blocks);
+void launch_level_entry(hipStream_t s, const Views& v, int blocks);
+void launch_level_weights(hipStream_t s, const Views& v, int blocks);
 void launch_depth_sigma(hipStream_t s, const float* depth, int W, int H, int max_dist, float* sigma);
-void launch_depth_data_weights(hipStream_t s, const float* sigma, int N, double dvc, double* delta);
 void launch_depth_map_level(hipStream_t s, const double* larger, int LW, int LH, double* dst, int W, int H);
+void launch_features_planes(hipStream_t s, const double* bgr, int W, int H, double* out5);
 
 }  // namespace rgbdflow

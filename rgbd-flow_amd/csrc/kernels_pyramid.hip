@@ -12,10 +12,6 @@ namespace rgbdflow {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kMaxTaps = 64;
-
-struct Taps { double g[kMaxTaps]; };
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 // ImageProcessing.h:25 EnforceRange
@@ -63,32 +59,26 @@ __global__ void k_interleave(const double* src, int N, int C, double* dst) {
   for (int c = 0; c < C; c++) dst[(size_t)l * C + c] = src[(size_t)c * N + l];
 }
 
-__global__ void k_to_f32(const double* src, size_t n, float* dst) {
-  for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) dst[i] = (float)src[i];
-}
-
-__global__ void k_fill(double* dst, size_t n, double v) {
-  for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) dst[i] = v;
-}
-
 // One pyramid level (GaussianPyramid.cpp:47-61): Gaussian smoothing of the
 // source (Image.h:1179-1199 -> imfilter_hv: h-pass then v-pass, clamp-to-edge,
 // ImageProcessing.h:247-269,339-358) followed by ResizeImage(ratio)
 // (ImageProcessing.h:203-222).  Only the smoothed samples the resize reads are
 // evaluated; taps of zero bilinear weight are skipped (adding x*0 leaves the
 // sum unchanged).
-__global__ void __launch_bounds__(kBlock) k_pyramid_level(const double* src, int Ws, int Hs, int C, Taps taps,
-                                                          int fs, double ratio, double* dst, int Wd, int Hd) {
-  const int l = blockIdx.x * kBlock + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_pyramid_jobs(WorkList<PyrJob> w) {
+  const int item = work_item(w, blockIdx.x);
+  const PyrJob& J = w.items[item];
+  const int l = (blockIdx.x - w.start[item]) * kBlock + threadIdx.x;
+  const int Ws = J.Ws, Hs = J.Hs, Wd = J.Wd, Hd = J.Hd, fs = J.fs;
   const int Nd = Wd * Hd;
-  if (l >= Nd * C) return;
+  if (l >= Nd * J.C) return;
   const int c = l / Nd;
   const int p = l - c * Nd;
   const int i = p / Wd, j = p - i * Wd;
-  const double x = (double)(j + 1) / ratio - 1;
-  const double y = (double)(i + 1) / ratio - 1;
+  const double x = (double)(j + 1) / J.ratio - 1;
+  const double y = (double)(i + 1) / J.ratio - 1;
   const Bilerp b = bilerp_taps(x, y, Ws, Hs);
-  const double* s = src + (size_t)c * Ws * Hs;
+  const double* s = J.src + (size_t)c * Ws * Hs;
   double acc = 0;
   for (int t = 0; t < 4; t++) {
     if (b.s[t] == 0.0) continue;
@@ -97,12 +87,12 @@ __global__ void __launch_bounds__(kBlock) k_pyramid_level(const double* src, int
     for (int lv = -fs; lv <= fs; lv++) {
       const int ii = enforce_range(vv + lv, Hs);
       double th = 0;
-      for (int lh = -fs; lh <= fs; lh++) th += s[(size_t)ii * Ws + enforce_range(uu + lh, Ws)] * taps.g[lh + fs];
-      sm += th * taps.g[lv + fs];
+      for (int lh = -fs; lh <= fs; lh++) th += s[(size_t)ii * Ws + enforce_range(uu + lh, Ws)] * J.taps[lh + fs];
+      sm += th * J.taps[lv + fs];
     }
     acc += sm * b.s[t];
   }
-  dst[(size_t)c * Nd + p] = acc;
+  J.dst[(size_t)c * Nd + p] = acc;
 }
 
 // OpticalFlow.cpp:189-214: gray = .114 B + .587 G + .299 R (Image.h:1464),
@@ -110,10 +100,8 @@ __global__ void __launch_bounds__(kBlock) k_pyramid_level(const double* src, int
 __device__ __forceinline__ double gray_at(const double* bgr, int N, int o) {
   return (double)bgr[o] * .114 + bgr[N + o] * .587 + bgr[2 * N + o] * .299;
 }
-__global__ void __launch_bounds__(kBlock) k_features(const double* bgr, int W, int H, double* out) {
-  const int l = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void features_at(const double* bgr, int W, int H, int l, double* out) {
   const int N = W * H;
-  if (l >= N) return;
   const int i = l / W, j = l - i * W;
   double f[5] = {1, -8, 0, 8, -1};
   for (int t = 0; t < 5; t++) f[t] /= 12;
@@ -127,35 +115,67 @@ __global__ void __launch_bounds__(kBlock) k_features(const double* bgr, int W, i
   out[4 * (size_t)N + l] = bgr[N + l] - bgr[2 * N + l];
 }
 
-// Image.h:764-769 imresize(w,h) -> ResizeImage(dst dims) (ImageProcessing.h:224-242)
-// of u, v, w, then u, v scaled by 1/ratio (sceneFlowMultiscale.cpp:130-134).
-__global__ void __launch_bounds__(kBlock) k_upsample(const double* src, int Ws, int Hs, double* dst, int Wd, int Hd,
-                                                     double uv_scale) {
+__global__ void __launch_bounds__(kBlock) k_features_planes(const double* bgr, int W, int H, double* out) {
   const int l = blockIdx.x * kBlock + threadIdx.x;
-  const int Nd = Wd * Hd, Ns = Ws * Hs;
-  if (l >= Nd) return;
-  const int i = l / Wd, j = l - i * Wd;
-  const double xr = (double)Wd / Ws, yr = (double)Hd / Hs;
-  const double x = (double)(j + 1) / xr - 1;
-  const double y = (double)(i + 1) / yr - 1;
-  const Bilerp b = bilerp_taps(x, y, Ws, Hs);
-  for (int c = 0; c < 3; c++) {
-    const double* s = src + (size_t)c * Ns;
-    double acc = 0;
-    for (int t = 0; t < 4; t++) acc += s[b.o[t]] * b.s[t];
-    if (c < 2) acc *= uv_scale;
-    dst[(size_t)c * Nd + l] = acc;
-  }
+  if (l < W * H) features_at(bgr, W, H, l, out);
 }
 
-// warpBy3DFlow (opticalFlowUtils.cpp:76-99) through ImageProcessing::warpImage
-// (ImageProcessing.h:471-491): out of bounds -> frame 1; then depth -= w.
-__global__ void __launch_bounds__(kBlock) k_warp_cpu_style(LevelView v) {
-  const int l = blockIdx.x * kBlock + threadIdx.x;
+// Features of both frames at a level (sceneFlowMultiscale.cpp:102-103).
+__global__ void __launch_bounds__(kBlock) k_features(Views w) {
+  const int item = work_item(w, blockIdx.x);
+  const LevelView& v = w.items[item];
+  const int l = (blockIdx.x - w.start[item]) * kBlock + threadIdx.x;
+  if (l >= v.N) return;
+  features_at(v.P1k, v.W, v.H, l, const_cast<double*>(v.I1));
+  features_at(v.P2k, v.W, v.H, l, const_cast<double*>(v.I2));
+}
+
+// Level entry (sceneFlowMultiscale.cpp:120-137).  At the pair's coarsest
+// level: u = 0 and the warped frame 2 is frame 2.  Otherwise: the coarser
+// flow resized to this level (Image.h:764-769 -> ResizeImage by dimensions,
+// ImageProcessing.h:224-242), u and v scaled by 1/ratio (:133-134), then
+// warpBy3DFlow (opticalFlowUtils.cpp:76-99 -> warpImage, ImageProcessing.h:471-491:
+// out of bounds -> frame 1; then depth -= w).  The warp of pixel p reads only
+// u(p), so resize and warp share one pass.
+__global__ void __launch_bounds__(kBlock) k_level_entry(Views wl) {
+  const int item = work_item(wl, blockIdx.x);
+  const LevelView& v = wl.items[item];
+  const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
   const int W = v.W, H = v.H, N = v.N;
   if (l >= N) return;
+  if (!v.u_prev) {
+    v.u[l] = 0;
+    v.u[N + l] = 0;
+    v.u[2 * N + l] = 0;
+    for (int k = 0; k < kFeat; k++) v.Iw[(size_t)k * N + l] = v.I2[(size_t)k * N + l];
+    v.Dw[l] = v.D2[l];
+    return;
+  }
   const int i = l / W, j = l - i * W;
-  const double y = i + v.u[N + l], x = j + v.u[l];
+  double uu[3];
+  if (v.Wp == W && v.Hp == H) {  // a flow of this level's size (sceneFlowWithInit): taken as it is
+    for (int c = 0; c < 3; c++) {
+      double a = v.u_prev[(size_t)c * N + l];
+      if (c < 2) a *= v.uv_scale;
+      uu[c] = a;
+      v.u[(size_t)c * N + l] = a;
+    }
+  } else {
+    const int Ws = v.Wp, Hs = v.Hp, Ns = Ws * Hs;
+    const double xr = (double)W / Ws, yr = (double)H / Hs;
+    const double x = (double)(j + 1) / xr - 1;
+    const double y = (double)(i + 1) / yr - 1;
+    const Bilerp b = bilerp_taps(x, y, Ws, Hs);
+    for (int c = 0; c < 3; c++) {
+      const double* s = v.u_prev + (size_t)c * Ns;
+      double acc = 0;
+      for (int t = 0; t < 4; t++) acc += s[b.o[t]] * b.s[t];
+      if (c < 2) acc *= v.uv_scale;
+      uu[c] = acc;
+      v.u[(size_t)c * N + l] = acc;
+    }
+  }
+  const double y = i + uu[1], x = j + uu[0];
   double d;
   if (x < 0 || x > W - 1 || y < 0 || y > H - 1) {
     for (int k = 0; k < kFeat; k++) v.Iw[(size_t)k * N + l] = v.I1[(size_t)k * N + l];
@@ -171,7 +191,7 @@ __global__ void __launch_bounds__(kBlock) k_warp_cpu_style(LevelView v) {
     d = 0;
     for (int t = 0; t < 4; t++) d += v.D2[b.o[t]] * b.s[t];
   }
-  v.Dw[l] = d - v.u[2 * N + l];
+  v.Dw[l] = d - uu[2];
 }
 
 // Level weights, converted to f32 as the solver's device copies are (.cu:680-687):
@@ -180,63 +200,53 @@ __global__ void __launch_bounds__(kBlock) k_warp_cpu_style(LevelView v) {
 //      level's depth, as sceneFlowMultiscale.cpp:144 calls it: level pixel l
 //      reads D1[(l / W_full) * W + l % W_full]),
 //   r (OpticalFlow.cpp:113-141), c = f_level / D1 (.cu:693-700).
-__global__ void __launch_bounds__(kBlock) k_level_weights(const double* delta, const double* smooth,
-                                                          const float* hpb, int reg_type, const double* D1, int W,
-                                                          int H, int W_full, float f_full, float f_level, float* dW,
-                                                          float* sW, float* mW, float* rW, float* cW) {
-  const int l = blockIdx.x * kBlock + threadIdx.x;
-  const int N = W * H;
+__global__ void __launch_bounds__(kBlock) k_level_weights(Views wl) {
+  const int item = work_item(wl, blockIdx.x);
+  const LevelView& v = wl.items[item];
+  const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
+  const int W = v.W, H = v.H, N = v.N;
   if (l >= N) return;
   const int i = l / W, j = l - i * W;
-  dW[l] = (float)delta[l];
-  sW[l] = (float)smooth[l];
-  const size_t r = (size_t)l / W_full, c = (size_t)l % W_full;
+  v.dW[l] = (float)v.delk[l];
+  v.sW[l] = (float)v.smk[l];
+  v.cW[l] = (float)(v.f_level / v.D1[l]);
+  if (v.mag64) {  // the single-level entry: m and r given by the caller
+    v.mW[l] = (float)v.mag64[l];
+    v.rW[l] = (float)v.reg64[(size_t)l * 2 + 0];
+    v.rW[N + l] = (float)v.reg64[(size_t)l * 2 + 1];
+    return;
+  }
+  const size_t r = (size_t)l / v.W_full, c = (size_t)l % v.W_full;
   const size_t idx = r * W + c;
-  const double d = idx < (size_t)N ? D1[idx] : 0.0;
-  mW[l] = (float)(d > 0 ? dsq(f_full / d) : dsq(f_full / 2));
+  const double d = idx < (size_t)N ? v.D1[idx] : 0.0;
+  v.mW[l] = (float)(d > 0 ? dsq(v.f_full / d) : dsq(v.f_full / 2));
   double wr = 0, wd = 0;
   if (j < W - 1) {
-    if (reg_type == 0) wr = 1;
-    else if (reg_type == 1) wr = exp(-dsq((D1[l] - D1[l + 1]) / .02));
-    else wr = 1 - hpb[(size_t)l * 2 + 0] * .99;
+    if (v.reg_type == 0) wr = 1;
+    else if (v.reg_type == 1) wr = exp(-dsq((v.D1[l] - v.D1[l + 1]) / .02));
+    else wr = 1 - v.hpbk[(size_t)l * 2 + 0] * .99;
   }
   if (i < H - 1) {
-    if (reg_type == 0) wd = 1;
-    else if (reg_type == 1) wd = exp(-dsq((D1[l] - D1[l + W]) / .02));
-    else wd = 1 - hpb[(size_t)l * 2 + 1] * .99;
+    if (v.reg_type == 0) wd = 1;
+    else if (v.reg_type == 1) wd = exp(-dsq((v.D1[l] - v.D1[l + W]) / .02));
+    else wd = 1 - v.hpbk[(size_t)l * 2 + 1] * .99;
   }
-  rW[l] = (float)wr;
-  rW[N + l] = (float)wd;
-  cW[l] = (float)(f_level / D1[l]);
-}
-
-// The same conversion for the single-level entry point, whose caller supplies
-// m and r directly (sceneFlowSingleScale's arguments).
-__global__ void __launch_bounds__(kBlock) k_level_weights_f64(const double* delta, const double* smooth,
-                                                              const double* mag, const double* reg2, const double* D1,
-                                                              int W, int H, float f_level, float* dW, float* sW,
-                                                              float* mW, float* rW, float* cW) {
-  const int l = blockIdx.x * kBlock + threadIdx.x;
-  const int N = W * H;
-  if (l >= N) return;
-  dW[l] = (float)delta[l];
-  sW[l] = (float)smooth[l];
-  mW[l] = (float)mag[l];
-  rW[l] = (float)reg2[(size_t)l * 2 + 0];
-  rW[N + l] = (float)reg2[(size_t)l * 2 + 1];
-  cW[l] = (float)(f_level / D1[l]);
+  v.rW[l] = (float)wr;
+  v.rW[N + l] = (float)wd;
 }
 
 // staticDepthNoiseModeling.cpp:95-97: depth-boundary pixel (a valid lower or
-// right neighbour more than 0.02 * stereoErrorRatio away).
-__device__ __forceinline__ bool is_depth_boundary(const float* depth, int W, int H, int i, int j) {
-  const float d = depth[(size_t)i * W + j];
+// right neighbour more than 0.02 * stereoErrorRatio away).  Depths are read as
+// float, the type of the reference's depth map (T = double input is cast).
+template <class T>
+__device__ __forceinline__ bool is_depth_boundary(const T* depth, int W, int H, int i, int j) {
+  const float d = (float)depth[(size_t)i * W + j];
   if (i < H - 1) {
-    const float dn = depth[(size_t)(i + 1) * W + j];
+    const float dn = (float)depth[(size_t)(i + 1) * W + j];
     if (dn > 0 && fabsf(d - dn) > .02 * stereo_error_ratio(fminf(d, dn))) return true;
   }
   if (j < W - 1) {
-    const float dn = depth[(size_t)i * W + j + 1];
+    const float dn = (float)depth[(size_t)i * W + j + 1];
     if (dn > 0 && fabsf(d - dn) > .02 * stereo_error_ratio(fminf(d, dn))) return true;
   }
   return false;
@@ -247,13 +257,11 @@ __device__ __forceinline__ bool is_depth_boundary(const float* depth, int W, int
 // exp((1-a) log(10 sigma) + a log(sigma)), a = d/maxDist, within L1 distance
 // d <= maxDist of a depth boundary (:112-124; the exact L1 distance, searched
 // over the diamond of radius maxDist).
-__global__ void __launch_bounds__(kBlock) k_depth_sigma(const float* depth, int W, int H, int max_dist,
-                                                        float* sigma) {
-  const int l = blockIdx.x * kBlock + threadIdx.x;
-  if (l >= W * H) return;
+template <class T>
+__device__ __forceinline__ float depth_sigma_at(const T* depth, int W, int H, int l, int max_dist) {
   const int i = l / W, j = l - i * W;
   const float sigmaAt1m = (float)stereo_error(1);
-  const float z = depth[l];
+  const float z = (float)depth[l];
   float st = z > 0 ? (float)fmax(1e-4, sigmaAt1m * stereo_error_ratio(z)) : 1.0f;
   if (max_dist > 0) {
     int best = max_dist + 1;
@@ -273,14 +281,32 @@ __global__ void __launch_bounds__(kBlock) k_depth_sigma(const float* depth, int 
       st = (float)exp((1 - a) * log((double)(10 * st)) + a * log((double)st));
     }
   }
-  sigma[l] = st;
+  return st;
 }
 
-// OpticalFlow.cpp:71-81: delta = stereoError(1) / sigma * depthVsColorDataWeight
-__global__ void k_depth_data_weights(const float* sigma, int N, double dvc, double* delta) {
+__global__ void __launch_bounds__(kBlock) k_depth_sigma(const float* depth, int W, int H, int max_dist,
+                                                        float* sigma) {
   const int l = blockIdx.x * kBlock + threadIdx.x;
-  if (l >= N) return;
-  delta[l] = stereo_error(1) / sigma[l] * dvc;
+  if (l < W * H) sigma[l] = depth_sigma_at(depth, W, H, l, max_dist);
+}
+
+// Per pair at the start of a solve: depth stdevs of frame 1 if not given
+// (runFlowTwoFrames.cpp:90, from the float depth), delta = stereoError(1) /
+// sigma * depthVsColorDataWeight (OpticalFlow.cpp:71-81), s = f^2 (:90-97).
+__global__ void __launch_bounds__(kBlock) k_pair_prep(WorkList<PairPrep> wl) {
+  const int item = work_item(wl, blockIdx.x);
+  const PairPrep& p = wl.items[item];
+  const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
+  if (l >= p.W * p.H) return;
+  float sg;
+  if (p.compute_sigma) {
+    sg = depth_sigma_at(p.dm1, p.W, p.H, l, 4);
+    p.sigma[l] = sg;
+  } else {
+    sg = p.sigma[l];
+  }
+  p.del0[l] = stereo_error(1) / sg * p.dvc;
+  p.sm0[l] = p.f2;
 }
 
 // depthMapPyramid.cpp:33-138: one level from the next larger one; a valid
@@ -331,51 +357,26 @@ void launch_deinterleave(hipStream_t s, const double* src, int N, int C, double*
 void launch_interleave(hipStream_t s, const double* src, int N, int C, double* dst) {
   hipLaunchKernelGGL(k_interleave, dim3(blocks_for(N)), dim3(kBlock), 0, s, src, N, C, dst);
 }
-void launch_fill_f64(hipStream_t s, double* dst, size_t n, double v) {
-  int nb = blocks_for(n);
-  if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL(k_fill, dim3(nb), dim3(kBlock), 0, s, dst, n, v);
+void launch_pair_prep(hipStream_t s, const WorkList<PairPrep>& w, int blocks) {
+  hipLaunchKernelGGL(k_pair_prep, dim3(blocks), dim3(kBlock), 0, s, w);
 }
-void launch_to_f32(hipStream_t s, const double* src, size_t n, float* dst) {
-  int nb = blocks_for(n);
-  if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(kBlock), 0, s, src, n, dst);
+void launch_pyramid_jobs(hipStream_t s, const WorkList<PyrJob>& w, int blocks) {
+  hipLaunchKernelGGL(k_pyramid_jobs, dim3(blocks), dim3(kBlock), 0, s, w);
 }
-void launch_pyramid_level(hipStream_t s, const double* src, int Ws, int Hs, int C, const double* taps, int fsize,
-                          double ratio, double* dst, int Wd, int Hd) {
-  if (fsize < 0 || 2 * fsize + 1 > kMaxTaps) return;  // callers keep sigma*3 <= 31
-  Taps t{};
-  for (int k = 0; k < 2 * fsize + 1; k++) t.g[k] = taps[k];
-  hipLaunchKernelGGL(k_pyramid_level, dim3(blocks_for((size_t)Wd * Hd * C)), dim3(kBlock), 0, s, src, Ws, Hs, C, t,
-                     fsize, ratio, dst, Wd, Hd);
+void launch_features(hipStream_t s, const Views& v, int blocks) {
+  hipLaunchKernelGGL(k_features, dim3(blocks), dim3(kBlock), 0, s, v);
 }
-void launch_features(hipStream_t s, const double* bgr, int W, int H, double* out5) {
-  hipLaunchKernelGGL(k_features, dim3(blocks_for((size_t)W * H)), dim3(kBlock), 0, s, bgr, W, H, out5);
+void launch_features_planes(hipStream_t s, const double* bgr, int W, int H, double* out5) {
+  hipLaunchKernelGGL(k_features_planes, dim3(blocks_for((size_t)W * H)), dim3(kBlock), 0, s, bgr, W, H, out5);
 }
-void launch_upsample(hipStream_t s, const double* src, int Ws, int Hs, double* dst, int Wd, int Hd, double uv_scale) {
-  hipLaunchKernelGGL(k_upsample, dim3(blocks_for((size_t)Wd * Hd)), dim3(kBlock), 0, s, src, Ws, Hs, dst, Wd, Hd,
-                     uv_scale);
+void launch_level_entry(hipStream_t s, const Views& v, int blocks) {
+  hipLaunchKernelGGL(k_level_entry, dim3(blocks), dim3(kBlock), 0, s, v);
 }
-void launch_warp_cpu_style(hipStream_t s, const LevelView& v) {
-  hipLaunchKernelGGL(k_warp_cpu_style, dim3(blocks_for(v.N)), dim3(kBlock), 0, s, v);
-}
-void launch_level_weights(hipStream_t s, const double* delta, const double* smooth, const float* hpb, int reg_type,
-                          const double* D1, int W, int H, int W_full, float f_full, float f_level, float* dW,
-                          float* sW, float* mW, float* rW, float* cW) {
-  hipLaunchKernelGGL(k_level_weights, dim3(blocks_for((size_t)W * H)), dim3(kBlock), 0, s, delta, smooth, hpb,
-                     reg_type, D1, W, H, W_full, f_full, f_level, dW, sW, mW, rW, cW);
-}
-void launch_level_weights_f64(hipStream_t s, const double* delta, const double* smooth, const double* mag,
-                              const double* reg2, const double* D1, int W, int H, float f_level, float* dW, float* sW,
-                              float* mW, float* rW, float* cW) {
-  hipLaunchKernelGGL(k_level_weights_f64, dim3(blocks_for((size_t)W * H)), dim3(kBlock), 0, s, delta, smooth, mag,
-                     reg2, D1, W, H, f_level, dW, sW, mW, rW, cW);
+void launch_level_weights(hipStream_t s, const Views& v, int blocks) {
+  hipLaunchKernelGGL(k_level_weights, dim3(blocks), dim3(kBlock), 0, s, v);
 }
 void launch_depth_sigma(hipStream_t s, const float* depth, int W, int H, int max_dist, float* sigma) {
   hipLaunchKernelGGL(k_depth_sigma, dim3(blocks_for((size_t)W * H)), dim3(kBlock), 0, s, depth, W, H, max_dist, sigma);
-}
-void launch_depth_data_weights(hipStream_t s, const float* sigma, int N, double dvc, double* delta) {
-  hipLaunchKernelGGL(k_depth_data_weights, dim3(blocks_for(N)), dim3(kBlock), 0, s, sigma, N, dvc, delta);
 }
 void launch_depth_map_level(hipStream_t s, const double* larger, int LW, int LH, double* dst, int W, int H) {
   hipLaunchKernelGGL(k_depth_map_level, dim3(blocks_for((size_t)W * H)), dim3(kBlock), 0, s, larger, LW, LH, dst, W,

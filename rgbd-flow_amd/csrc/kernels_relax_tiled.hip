@@ -65,18 +65,24 @@ __device__ __forceinline__ void tiled_pass(float* X, int cap, int PR, int RH, co
   }
 }
 
-// Reads du from src and writes the tile interiors to dst: a workgroup's halo
-// overlaps other workgroups' interiors, which must still hold the values from
-// before this launch.  A launch after an early exit copies its interior, so
-// the host knows which buffer holds du after every launch.
+// Reads du from one buffer and writes the tile interiors to the other (flip
+// selects which): a workgroup's halo overlaps other workgroups' interiors,
+// which must still hold the values from before this launch.  A launch after an
+// early exit copies its interior, so the host knows which buffer holds du
+// after every launch.  Block b works on tile b - start[item] of its item.
 template <int NT, int P>
-__global__ void __launch_bounds__(NT) k_relax_tiled(LevelView v, TileSched ts, const float* __restrict__ src,
-                                                    float* __restrict__ dst, int n_iter, int slot_base,
+__global__ void __launch_bounds__(NT) k_relax_tiled(Views wl, int flip, int n_iter, int slot_base,
                                                     int n_prev_checks, int check_slot) {
   extern __shared__ float X[];
+  const int item = work_item(wl, blockIdx.x);
+  const LevelView& v = wl.items[item];
   if (v.ctl->outer_done) return;
+  const TileSched ts = v.ts;
+  const float* __restrict__ src = flip ? v.du_alt : v.du;
+  float* __restrict__ dst = flip ? v.du : v.du_alt;
   const int W = v.W, H = v.H, N = v.N;
-  const int bx = blockIdx.x % ts.tiles_x, by = blockIdx.x / ts.tiles_x;
+  const int tb = blockIdx.x - wl.start[item];
+  const int bx = tb % ts.tiles_x, by = tb / ts.tiles_x;
   const int ox = bx * ts.TW, oy = by * ts.TH;
   const int ex = min(ox + ts.TW, W), ey = min(oy + ts.TH, H);
   if (relax_stopped(v.ctl, slot_base, n_prev_checks)) {
@@ -152,12 +158,22 @@ __global__ void __launch_bounds__(NT) k_relax_tiled(LevelView v, TileSched ts, c
     const float m = wave_max(change);
     if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(&v.ctl->maxdiff[check_slot], __float_as_uint(m));
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) v.ctl->sweeps_exec += n_iter;
+  if (tb == 0 && threadIdx.x == 0) v.ctl->sweeps_exec += n_iter;
+}
+
+// Copies du_alt into du for every item (after an odd number of tiled launches).
+__global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
+  const int item = work_item(wl, blockIdx.x);
+  const LevelView& v = wl.items[item];
+  if (v.ctl->outer_done) return;
+  const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
+  if (l >= v.N) return;
+  for (int k = 0; k < 3; k++) v.du[(size_t)k * v.N + l] = v.du_alt[(size_t)k * v.N + l];
 }
 
 struct TiledConfig {
   int nt, p;
-  void (*fn)(LevelView, TileSched, const float*, float*, int, int, int, int);
+  void (*fn)(Views, int, int, int, int, int);
 };
 
 #define RGBDFLOW_TILED(NT, P) TiledConfig{NT, P, k_relax_tiled<NT, P>}
@@ -208,22 +224,20 @@ void launch_fill_test_system(hipStream_t s, float* coef, float* du, int W, int H
   hipLaunchKernelGGL(k_fill_test_system, dim3(nb), dim3(256), 0, s, coef, du, W, H);
 }
 
-void launch_copy_f32(hipStream_t s, const float* src, size_t n, float* dst) {
-  int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(256), 0, s, src, n, dst);
+void launch_copy_du(hipStream_t s, const Views& v, int blocks) {
+  hipLaunchKernelGGL(k_copy_du, dim3(blocks), dim3(kBlock), 0, s, v);
 }
 
-void launch_relax_tiled(hipStream_t s, const LevelView& v, const TileSched& ts, const float* src, float* dst,
-                        int n_iter, int slot_base, int n_prev_checks, int check_slot) {
-  const TiledConfig& c = kConfigs[ts.config];
-  const size_t lds = (size_t)6 * ts.cap * sizeof(float);
+void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int cap, int flip, int n_iter,
+                        int slot_base, int n_prev_checks, int check_slot) {
+  const TiledConfig& c = kConfigs[config];
+  const size_t lds = (size_t)6 * cap * sizeof(float);
   static bool raised[16] = {};
-  if (lds > 65536 && !raised[ts.config]) {
+  if (lds > 65536 && !raised[config]) {
     hipFuncSetAttribute(reinterpret_cast<const void*>(c.fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    raised[ts.config] = true;
+    raised[config] = true;
   }
-  hipLaunchKernelGGL(c.fn, dim3(ts.tiles_x * ts.tiles_y), dim3(c.nt), lds, s, v, ts, src, dst, n_iter, slot_base,
-                     n_prev_checks, check_slot);
+  hipLaunchKernelGGL(c.fn, dim3(blocks), dim3(c.nt), lds, s, v, flip, n_iter, slot_base, n_prev_checks, check_slot);
 }
 
 }  // namespace rgbdflow

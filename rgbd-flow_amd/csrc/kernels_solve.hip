@@ -8,17 +8,17 @@
 // copies in f32; the squares inside the robust weights in f32 (sqrCU, .cu:44-47).
 // Built with -ffp-contract=off so every rounding happens where the source says.
 //
-// Early exits are decided on the device: a kernel reads the level's SolveCtl at
-// entry and returns when the outer loop or the current relaxation has stopped,
-// so the host enqueues a whole level (or pair) without waiting on it.
+// Every kernel runs over a work-list of (pair, level) items: block b belongs
+// to the item whose block range holds b.  Early exits are decided on the
+// device: a kernel reads its item's SolveCtl at entry and returns when that
+// pair's outer loop or current relaxation has stopped, so the host enqueues
+// whole levels of whole batches without waiting.
 #include "internal.hpp"
 #include "relax_pixel.hpp"
 
 namespace rgbdflow {
 
 namespace {
-
-constexpr int kBlock = 256;
 
 // sqrCU (.cu:44-47): the argument is rounded to f32 and squared in f32.
 __device__ __forceinline__ float sqr_f32(double x) {
@@ -28,19 +28,25 @@ __device__ __forceinline__ float sqr_f32(double x) {
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-__global__ void k_ctl_reset(SolveCtl* ctl) {
-  unsigned int* p = reinterpret_cast<unsigned int*>(ctl);
-  const size_t n = sizeof(SolveCtl) / sizeof(unsigned int);
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+// One block per item: zero the item's control block.
+__global__ void k_ctl_reset(Views w) {
+  const LevelView& v = w.items[blockIdx.x];
+  unsigned int* p = reinterpret_cast<unsigned int*>(v.ctl);
+  const int n = (int)(sizeof(SolveCtl) / sizeof(unsigned int));
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
 // .cu:84-147 (x/y derivatives of 0.4*I1 + 0.6*warped I2 with [1,-8,0,8,-1]/12,
 // clamp-to-edge; depth derivatives clamped to +-0.04), .cu:776 (du = 0),
-// .cu:640-648 (occlusion weight from the UNWARPED D2).
-__global__ void __launch_bounds__(kBlock) k_derivs(LevelView v, int n_slots) {
+// .cu:640-648 (occlusion weight from the UNWARPED D2).  Also zeroes the item's
+// check slots of this outer iteration.
+__global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
+  const int it = work_item(w, blockIdx.x);
+  const LevelView& v = w.items[it];
   if (v.ctl->outer_done) return;
-  const int l = blockIdx.x * kBlock + threadIdx.x;
-  for (int t = l; t < n_slots; t += gridDim.x * kBlock) v.ctl->maxdiff[t] = 0u;
+  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
+  const int stride = (w.start[it + 1] - w.start[it]) * kBlock;
+  for (int t = l; t < n_slots; t += stride) v.ctl->maxdiff[t] = 0u;
   if (l >= v.N) return;
   const int W = v.W, H = v.H, N = v.N;
   const int i = l / W, j = l - i * W;
@@ -50,13 +56,11 @@ __global__ void __launch_bounds__(kBlock) k_derivs(LevelView v, int n_slots) {
     const double* b = v.Iw + (size_t)k * N;
     double sx = 0, sy = 0;
     for (int m = 0; m < 5; m++) {
-      const int c = clampi(j - 2 + m, 0, W - 1);
-      const int o = i * W + c;
+      const int o = i * W + clampi(j - 2 + m, 0, W - 1);
       sx += g[m] * (.4 * a[o] + .6 * b[o]);
     }
     for (int m = 0; m < 5; m++) {
-      const int r = clampi(i - 2 + m, 0, H - 1);
-      const int o = r * W + j;
+      const int o = clampi(i - 2 + m, 0, H - 1) * W + j;
       sy += g[m] * (.4 * a[o] + .6 * b[o]);
     }
     v.Ix[(size_t)k * N + l] = sx / 12;
@@ -94,9 +98,11 @@ __device__ __forceinline__ void flow_grad(const LevelView& v, int c, int l, int 
 }
 
 // .cu:354-355: smoothness weight phi' of the Charbonnier penalty.
-__global__ void __launch_bounds__(kBlock) k_phi(LevelView v) {
+__global__ void __launch_bounds__(kBlock) k_phi(Views w) {
+  const int it = work_item(w, blockIdx.x);
+  const LevelView& v = w.items[it];
   if (v.ctl->outer_done) return;
-  const int l = blockIdx.x * kBlock + threadIdx.x;
+  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
   const int i = l / v.W, j = l - i * v.W;
   double ux, uy, vx, vy, wx, wy;
@@ -132,9 +138,11 @@ __device__ __forceinline__ double laplacian(const LevelView& v, int c, int l, in
 
 // .cu:356-369 (psi', pi'), .cu:189-230 (products), .cu:477-575 (assembly),
 // written as the compact per-pixel system (Coef).
-__global__ void __launch_bounds__(kBlock) k_system(LevelView v, SolveScalars s) {
+__global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
+  const int it = work_item(w, blockIdx.x);
+  const LevelView& v = w.items[it];
   if (v.ctl->outer_done) return;
-  const int l = blockIdx.x * kBlock + threadIdx.x;
+  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
   const int W = v.W, H = v.H, N = v.N;
   const int i = l / W, j = l - i * W;
@@ -205,13 +213,18 @@ __global__ void __launch_bounds__(kBlock) k_system(LevelView v, SolveScalars s) 
 // .cu:382-470: one colour of the red-black relaxation, one thread per pixel of
 // the colour, neighbours read from global memory (the plain schedule: one
 // launch per colour per iteration, as the reference launches it).
-__global__ void __launch_bounds__(256) k_relax_plain(LevelView v, int colour, int slot_base, int n_prev_checks,
+__global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int slot_base, int n_prev_checks,
                                                      int check_slot) {
+  const int it = work_item(w, blockIdx.x);
+  const LevelView& v = w.items[it];
   if (v.ctl->outer_done) return;
   if (relax_stopped(v.ctl, slot_base, n_prev_checks)) return;
   const int W = v.W, H = v.H, N = v.N;
-  const int xc = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  const int lb = blockIdx.x - w.start[it];
+  const int gx = ((W + 1) / 2 + 63) / 64;
+  const int bx = lb % gx, by = lb / gx;
+  const int xc = bx * 64 + (threadIdx.x & 63);
+  const int i = by * 4 + (threadIdx.x >> 6);
   const int j = 2 * xc + ((i + colour) & 1);
   float change = 0.f;
   if (i < H && j < W) {
@@ -239,18 +252,21 @@ __global__ void __launch_bounds__(256) k_relax_plain(LevelView v, int colour, in
     const float m = wave_max(change);
     if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(&v.ctl->maxdiff[check_slot], __float_as_uint(m));
   }
-  if (colour == 1 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0)
-    v.ctl->sweeps_exec += 1;
+  if (colour == 1 && lb == 0 && threadIdx.x == 0) v.ctl->sweeps_exec += 1;
 }
 
 // .cu:934-941 (u += du, GPU warps: floor base, explicit clamps, f32 weights,
 // out-of-bounds -> frame 1) and the per-block part of the outer statistic
 // sum(du_k^2 * c_k) (.cu:948), accumulated in f64 in a fixed order.
-__global__ void __launch_bounds__(kBlock) k_update_warp(LevelView v, int want_stat) {
+__global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat) {
   __shared__ double red[kBlock];
+  const int item = work_item(wl, blockIdx.x);
+  const LevelView& v = wl.items[item];
   if (v.ctl->outer_done) return;
-  const int l = blockIdx.x * kBlock + threadIdx.x;
+  const int lb = blockIdx.x - wl.start[item];
+  const int l = lb * kBlock + threadIdx.x;
   const int W = v.W, H = v.H, N = v.N;
+  const bool stat = want_stat && v.allow_early;
   double term = 0;
   if (l < N) {
     const int i = l / W, j = l - i * W;
@@ -277,28 +293,31 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(LevelView v, int want_st
       }
       v.Dw[l] = w00 * v.D2[o00] + w01 * v.D2[o01] + w10 * v.D2[o10] + w11 * v.D2[o11] - u2;
     }
-    if (want_stat) {
+    if (stat) {
       // sqrCU(du_k) * c_k in f32 (thrustSqrMultFunctor), c = (1, 1, f/D1)
       term = (double)(e0 * e0);
       term += (double)(e1 * e1);
       term += (double)((e2 * e2) * v.cW[l]);
     }
   }
-  if (!want_stat) return;
+  if (!stat) return;
   red[threadIdx.x] = term;
   __syncthreads();
   for (int s = kBlock / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) v.partials[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) v.partials[lb] = red[0];
 }
 
-// Fixed-order sum of the per-block partials -> outer statistic; sets the
-// level's outer_done flag when early termination applies (.cu:943-951).
-__global__ void __launch_bounds__(kBlock) k_stat_final(LevelView v, int nblocks, int outer, int decide) {
+// One block per item: fixed-order sum of the item's partials -> outer
+// statistic; sets the item's outer_done when early termination applies
+// (.cu:943-951).
+__global__ void __launch_bounds__(kBlock) k_stat_final(Views w, int outer, int decide) {
   __shared__ double red[kBlock];
-  if (v.ctl->outer_done) return;
+  const LevelView& v = w.items[blockIdx.x];
+  if (v.ctl->outer_done || !v.allow_early) return;
+  const int nblocks = pix_blocks(v.N);
   double acc = 0;
   for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += v.partials[b];
   red[threadIdx.x] = acc;
@@ -317,34 +336,29 @@ __global__ void __launch_bounds__(kBlock) k_stat_final(LevelView v, int nblocks,
   }
 }
 
-inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
-
 }  // namespace
 
-int stat_blocks(int N) { return blocks_for(N); }
-
-void launch_ctl_reset(hipStream_t s, SolveCtl* ctl) { hipLaunchKernelGGL(k_ctl_reset, dim3(64), dim3(256), 0, s, ctl); }
-
-void launch_derivs(hipStream_t s, const LevelView& v, int n_slots) {
-  hipLaunchKernelGGL(k_derivs, dim3(blocks_for(v.N)), dim3(kBlock), 0, s, v, n_slots);
+void launch_ctl_reset(hipStream_t s, const Views& v) {
+  hipLaunchKernelGGL(k_ctl_reset, dim3(v.n), dim3(256), 0, s, v);
 }
-void launch_phi(hipStream_t s, const LevelView& v, int) {
-  hipLaunchKernelGGL(k_phi, dim3(blocks_for(v.N)), dim3(kBlock), 0, s, v);
+void launch_derivs(hipStream_t s, const Views& v, int blocks, int n_slots) {
+  hipLaunchKernelGGL(k_derivs, dim3(blocks), dim3(kBlock), 0, s, v, n_slots);
 }
-void launch_system(hipStream_t s, const LevelView& v, const SolveScalars& k, int) {
-  hipLaunchKernelGGL(k_system, dim3(blocks_for(v.N)), dim3(kBlock), 0, s, v, k);
+void launch_phi(hipStream_t s, const Views& v, int blocks) {
+  hipLaunchKernelGGL(k_phi, dim3(blocks), dim3(kBlock), 0, s, v);
 }
-void launch_relax_plain(hipStream_t s, const LevelView& v, int colour, int, int slot_base, int n_prev_checks,
+void launch_system(hipStream_t s, const Views& v, int blocks, const SolveScalars& k) {
+  hipLaunchKernelGGL(k_system, dim3(blocks), dim3(kBlock), 0, s, v, k);
+}
+void launch_relax_plain(hipStream_t s, const Views& v, int blocks, int colour, int slot_base, int n_prev_checks,
                         int check_slot) {
-  const dim3 blk(64, 4);
-  const dim3 grid(((v.W + 1) / 2 + 63) / 64, (v.H + 3) / 4);
-  hipLaunchKernelGGL(k_relax_plain, grid, blk, 0, s, v, colour, slot_base, n_prev_checks, check_slot);
+  hipLaunchKernelGGL(k_relax_plain, dim3(blocks), dim3(256), 0, s, v, colour, slot_base, n_prev_checks, check_slot);
 }
-void launch_update_warp(hipStream_t s, const LevelView& v, const SolveScalars& k, int outer) {
-  const int nb = blocks_for(v.N);
-  const int want = k.allow_early ? 1 : 0;
-  hipLaunchKernelGGL(k_update_warp, dim3(nb), dim3(kBlock), 0, s, v, want);
-  if (want) hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(kBlock), 0, s, v, nb, outer, k.outer_exit ? 1 : 0);
+void launch_update_warp(hipStream_t s, const Views& v, int blocks, int want_stat) {
+  hipLaunchKernelGGL(k_update_warp, dim3(blocks), dim3(kBlock), 0, s, v, want_stat);
+}
+void launch_stat_final(hipStream_t s, const Views& v, int outer, int decide) {
+  hipLaunchKernelGGL(k_stat_final, dim3(v.n), dim3(kBlock), 0, s, v, outer, decide);
 }
 
 }  // namespace rgbdflow

@@ -1,0 +1,149 @@
+"""GPU parity of the batched solve: every pair of a batch, solved together
+with the others (one launch per step for the whole batch, uneven sizes and
+level counts allowed), equals the oracle's solve of that pair alone, bit for
+bit, with the same per-level iteration counts."""
+import numpy as np
+import pytest
+
+from helpers import hpb_for, load_package, make_case, oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def R():
+    return load_package()
+
+
+@pytest.fixture(scope="module")
+def ctx(R):
+    c = R.Context(0)
+    yield c
+    c.close()
+
+
+def _case(W, H, seed, min_width, segments=4):
+    pair, im1, d1, im2, d2, sigma = make_case(W, H, seed, segments)
+    hpb, lv = hpb_for(pair, W, H, 0.5, min_width)
+    return dict(W=W, H=H, im1=im1, dm1=d1, im2=im2, dm2=d2, sigma1=sigma, hpb=hpb, min_width=min_width)
+
+
+def _oracle_solve(c, n_outer, n_sor, exits=True):
+    O = oracle()
+    po = O.make_params(c["W"], c["H"], n_outer=n_outer, n_sor=n_sor, min_width=c["min_width"],
+                       disable_inner_exit=not exits, disable_outer_exit=not exits)
+    return O.multiscale(c["im1"], c["dm1"], c["im2"], c["dm2"], c["sigma1"], c["hpb"], po)
+
+
+def _check_batch(ctx, R, cases, relax_variant, n_outer=4, n_sor=30, exits=True, drop_sigma=()):
+    W0, H0 = cases[0]["W"], cases[0]["H"]
+    p = R.make_params(W0, H0, n_outer=n_outer, n_sor=n_sor, min_width=cases[0]["min_width"],
+                      relax_variant=relax_variant, inner_exit=exits, outer_exit=exits)
+    pairs = []
+    for i, c in enumerate(cases):
+        q = dict(c)
+        q["cam"] = R.make_params(c["W"], c["H"]).cam
+        if i in drop_sigma:
+            q["sigma1"] = None   # computed on the device
+        pairs.append(q)
+    ctx.batch_load(p, pairs)
+    stats = ctx.batch_run(p, stats=True)
+    assert len(stats) == len(cases)
+    for i, c in enumerate(cases):
+        print(f"pair {i}: {c['W']}x{c['H']} min_width {c['min_width']}", flush=True)
+        ua, va, wa, sa = _oracle_solve(c, n_outer, n_sor, exits)
+        ub, vb, wb = ctx.batch_read(i)
+        sb = stats[i]
+        L = sa.n_levels
+        assert sb.n_levels == L
+        assert [sb.outer_exec[k] for k in range(L)] == [sa.outer_exec[k] for k in range(L)]
+        assert [sb.sweeps_exec[k] for k in range(L)] == [sa.sweeps_exec[k] for k in range(L)]
+        assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb), i
+    return p
+
+
+@pytest.mark.parametrize("variant", ["plain", "tiled"])
+def test_batch_same_size(ctx, R, variant):
+    cases = [_case(160, 120, seed, 20) for seed in (31, 32, 33)]
+    _check_batch(ctx, R, cases, R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN)
+
+
+@pytest.mark.parametrize("variant", ["plain", "tiled"])
+def test_batch_uneven_sizes(ctx, R, variant):
+    """Different sizes and level counts in one batch (pyramids aligned on the
+    finest level; the largest item of a step fixes the tiled launch shape)."""
+    cases = [_case(200, 150, 41, 20), _case(160, 120, 42, 20), _case(96, 72, 43, 20), _case(128, 96, 44, 40),
+             _case(64, 48, 45, 12)]
+    _check_batch(ctx, R, cases, R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN)
+
+
+def test_batch_no_exits_and_device_sigma(ctx, R):
+    cases = [_case(128, 96, 51, 16), _case(128, 96, 52, 16)]
+    O = oracle()
+    # the device stdevs equal the oracle's to 1 f32 ulp; the oracle's are used
+    # for pair 0, the device's for pair 1, whose solve is checked at tolerance
+    W, H = 128, 96
+    p = R.make_params(W, H, n_outer=3, n_sor=30, min_width=16, relax_variant=R.RELAX_TILED, inner_exit=False,
+                      outer_exit=False)
+    pairs = [dict(cases[0]), dict(cases[1], sigma1=None)]
+    ctx.batch_load(p, pairs)
+    st = ctx.batch_run(p, stats=True)
+    ua, va, wa, sa = _oracle_solve(cases[0], 3, 30, exits=False)
+    ub, vb, wb = ctx.batch_read(0)
+    assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
+    ua, va, wa, sa = _oracle_solve(cases[1], 3, 30, exits=False)
+    ub, vb, wb = ctx.batch_read(1)
+    assert np.max(np.abs(ua - ub)) <= 1e-3 and np.max(np.abs(va - vb)) <= 1e-3 and np.max(np.abs(wa - wb)) <= 1e-5
+    assert st[1].sweeps_exec[0] == 3 * 30
+
+
+def test_batch_rerun_and_new_params(ctx, R):
+    """Descriptors are reused across runs and rebuilt when the params change."""
+    cases = [_case(160, 120, 61, 20), _case(160, 120, 62, 20)]
+    p = _check_batch(ctx, R, cases, R.RELAX_TILED, n_outer=3, n_sor=25)
+    ctx.batch_run(p, stats=False)
+    ctx.synchronize()
+    ua, va, wa, _ = _oracle_solve(cases[1], 3, 25)
+    ub, vb, wb = ctx.batch_read(1)
+    assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
+    # new iteration counts on the loaded batch
+    p.n_outer, p.n_sor = 5, 40
+    st = ctx.batch_run(p, stats=True)
+    for i, c in enumerate(cases):
+        ua, va, wa, sa = _oracle_solve(c, 5, 40)
+        ub, vb, wb = ctx.batch_read(i)
+        assert [st[i].sweeps_exec[k] for k in range(sa.n_levels)] == [sa.sweeps_exec[k] for k in range(sa.n_levels)]
+        assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
+
+
+def test_batch_autotuned(R):
+    """The autotuned default schedule, timed on batches, leaves results unchanged."""
+    cases = [_case(200, 150, 71, 25), _case(200, 150, 72, 25), _case(200, 150, 73, 25)]
+    O = oracle()
+    with R.Context(0) as c:
+        for (w, h) in O.pyramid_levels(200, 150, 0.5, 25):
+            ch = c.autotune_relax(w, h, batch=3)
+            assert ch["candidates"] > 0 and ch["us_per_iter"] > 0
+        _check_batch(c, R, cases, R.RELAX_TILED)
+
+
+def test_batch_errors(R):
+    with R.Context(0) as c:
+        cases = [_case(96, 72, 81, 12), _case(96, 72, 82, 12)]
+        p = R.make_params(96, 72, n_outer=2, n_sor=25, min_width=12)
+        with pytest.raises(R.RgbdFlowError) as e:
+            c.batch_run(p)       # nothing loaded
+        assert e.value.code == R.ERR_STATE
+        c.batch_load(p, cases)
+        with pytest.raises(R.RgbdFlowError) as e:
+            c.pair_run(p)        # a batch of two is loaded
+        assert e.value.code == R.ERR_STATE
+        c.batch_run(p, stats=False)
+        with pytest.raises(R.RgbdFlowError) as e:
+            c._batch_shapes.append((72, 96))
+            c.batch_read(2)
+        assert e.value.code == R.ERR_ARG
+        c._batch_shapes.pop()
+        with pytest.raises(R.RgbdFlowError) as e:
+            c.batch_load(p, [dict(cases[0], hpb=None)])   # HPB regularisation without maps
+        assert e.value.code == R.ERR_ARG
