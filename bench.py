@@ -1,12 +1,15 @@
-"""Benchmark: 640x480 scene-flow pairs/s on one MI355X.
+"""Benchmark: 640x480 scene-flow pairs/s on one MI355X, batched vs plain.
 
-Workload: one synthetic 640x480 RGB-D frame pair (f = 525, c = (319.5, 239.5),
-6 moving planar segments), 5-level pyramid at ratio 0.5, 15 outer x 50 inner
-iterations, boundary-probability regularisation (maps from the generator's
-segment labels), the reference's mixed precision (f64 images, derivatives and
-flow; f32 system and increments).  A step is one full coarse-to-fine solve of
-the pair: depth stdevs, the six pyramids, per-level features / upsample / warp /
-weights, and the level solves, with the inputs resident in HBM.
+Workload: 640x480 synthetic RGB-D frame pairs (f = 525, c = (319.5, 239.5),
+6 moving planar segments, seeds 21, 22, ...), 5-level pyramid at ratio 0.5,
+15 outer x 50 inner iterations, boundary-probability regularisation (maps from
+the generator's segment labels), the reference's mixed precision (f64 images,
+derivatives and flow; f32 system and increments).  A step is one batch of
+--batch pairs solved together (depth stdevs, the six pyramids, per-level
+features / upsample / warp / weights and the level solves of every pair),
+with the inputs resident in HBM.  Beside it: the plain single-pair GPU path
+(one pair, one launch per colour per relaxation iteration, as the reference
+schedules it) and the tiled single-pair path.
 
 Prints one JSON line.  --gpus above 1 is refused: the solve runs on one GPU.
 """
@@ -25,9 +28,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "rgbd-flow_amd")
 
 METRIC = "640×480 scene-flow pairs/s + p99 latency, batched vs plain; HBM GB/s"
+W, H = 640, 480
 B_SWEEP = 72.0     # algorithmic bytes per pixel per red-black iteration (f32 system + increments)
 B_STAGE = 128.0    # algorithmic bytes per pixel per outer iteration (warp, gradients, weights, assembly)
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "relax_tiled_traffic.json")
 
 
 def _package():
@@ -39,42 +44,54 @@ def _package():
     return mod
 
 
-def workload(R, seed=21):
+def workload(R, n_distinct):
     import rgbdflow_amd.synth as S
-    W, H = 640, 480
     cam = S.Camera(W, H, 525.0, 525.0, 319.5, 239.5)
-    pair = S.make_pair(W, H, seed, n_segments=6, cam=cam)
-    im1, d1, im2, d2 = S.to_solver_inputs(pair)
     levels = R.pyramid_levels(W, H, 0.5, 40)
-    hpb = S.segment_boundary_maps(pair["seg1"], [w for w, h in levels], [h for w, h in levels])
-    return pair, im1, d1.astype(np.float64), im2, d2.astype(np.float64), hpb, levels
+    pairs = []
+    for i in range(n_distinct):
+        pair = S.make_pair(W, H, 21 + i, n_segments=6, cam=cam)
+        im1, d1, im2, d2 = S.to_solver_inputs(pair)
+        hpb = S.segment_boundary_maps(pair["seg1"], [w for w, h in levels], [h for w, h in levels])
+        pairs.append(dict(im1=im1, dm1=d1.astype(np.float64), im2=im2, dm2=d2.astype(np.float64), hpb=hpb))
+    return pairs, levels
 
 
 def params_for(R, variant):
-    return R.make_params(640, 480, f=525.0, cx=319.5, cy=239.5, n_outer=15, n_sor=50, min_width=40,
+    return R.make_params(W, H, f=525.0, cx=319.5, cy=239.5, n_outer=15, n_sor=50, min_width=40,
                          relax_variant=R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN)
 
 
-def algorithmic_bytes(st):
+def algorithmic_bytes(stats):
     sweep = stage = 0.0
-    for k in range(st.n_levels):
-        n = st.width[k] * st.height[k]
-        sweep += n * st.sweeps_exec[k] * B_SWEEP
-        stage += n * st.outer_exec[k] * B_STAGE
+    for st in stats:
+        for k in range(st.n_levels):
+            n = st.width[k] * st.height[k]
+            sweep += n * st.sweeps_exec[k] * B_SWEEP
+            stage += n * st.outer_exec[k] * B_STAGE
     return sweep, stage
 
 
-def cpu_baseline(im1, d1, im2, d2, hpb, budget_s):
+def timed_runs(ctx, p, steps):
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.batch_run(p, stats=False)
+    ctx.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(pr, budget_s):
     """The oracle (reference-precision restatement, one thread) on the same
     workload: as many whole pairs as fit in the time budget (at least one)."""
     if ROOT not in sys.path:
         sys.path.insert(0, ROOT)
     import oracle as O
-    po = O.make_params(640, 480, f=525.0, cx=319.5, cy=239.5, n_outer=15, n_sor=50, min_width=40)
-    sigma = O.depth_stdevs(d1.astype(np.float32), 4)
+    po = O.make_params(W, H, f=525.0, cx=319.5, cy=239.5, n_outer=15, n_sor=50, min_width=40)
+    sigma = O.depth_stdevs(pr["dm1"].astype(np.float32), 4)
     n, t0 = 0, time.perf_counter()
     while True:
-        O.multiscale(im1, d1, im2, d2, sigma, hpb, po)
+        O.multiscale(pr["im1"], pr["dm1"], pr["im2"], pr["dm2"], sigma, pr["hpb"], po)
         n += 1
         el = time.perf_counter() - t0
         if el > budget_s or n >= 8:
@@ -85,11 +102,16 @@ def cpu_baseline(im1, d1, im2, d2, hpb, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10, help="timed batches")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=16, help="pairs per step")
+    ap.add_argument("--distinct", type=int, default=4, help="distinct synthetic pairs (cycled over the batch)")
     ap.add_argument("--variant", choices=["plain", "tiled"], default="tiled")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--plain-steps", type=int, default=5, help="steps of the plain schedule for comparison")
+    ap.add_argument("--save-schedules", help="write the autotuned schedules (JSON) here")
+    ap.add_argument("--load-schedules", help="install schedules from this JSON instead of autotuning")
+    ap.add_argument("--plain-steps", type=int, default=5, help="steps of the plain single-pair path")
+    ap.add_argument("--single-steps", type=int, default=10, help="steps of the tiled single-pair path")
     ap.add_argument("--latency-steps", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -100,63 +122,88 @@ def main():
         sys.exit(2)
 
     R = _package()
-    pair, im1, d1, im2, d2, hpb, levels = workload(R)
+    pairs, levels = workload(R, max(1, min(args.distinct, args.batch)))
+    B = args.batch
+    batch = [pairs[i % len(pairs)] for i in range(B)]
     p = params_for(R, args.variant)
     ctx = R.Context(0)
-    tuned = []
-    if args.variant == "tiled" and not args.no_autotune:
-        # one-time schedule search per level shape (outside the timed region)
+
+    def autotune(nb):
+        out = []
         for (w, h) in levels:
-            ch = ctx.autotune_relax(w, h)
-            tuned.append(dict(level=f"{w}x{h}", tile=f"{ch['tile_w']}x{ch['tile_h']}", iters=ch["tile_iters"],
-                              threads=ch["threads"], pairs_per_thread=ch["pairs_per_thread"],
-                              us_per_iter=round(ch["us_per_iter"], 2)))
-        print(f"bench: autotuned {tuned}", file=sys.stderr, flush=True)
-    ctx.pair_load(p, im1, d1, im2, d2, None, hpb)   # stdevs computed on the device every step
-    print(f"bench: 640x480 pair, {len(levels)} levels, 15x50, variant {args.variant}", file=sys.stderr, flush=True)
-    st = None
+            ch = ctx.autotune_relax(w, h, batch=nb)
+            raw.setdefault(nb, []).append(dict(W=w, H=h, **ch))
+            out.append(dict(level=f"{w}x{h}", tile=f"{ch['tile_w']}x{ch['tile_h']}", iters=ch["tile_iters"],
+                            threads=ch["threads"], pairs_per_thread=ch["pairs_per_thread"],
+                            us_per_iter=round(ch["us_per_iter"], 2)))
+        return out
+
+    tuned, raw = [], {}
+    if args.variant == "tiled" and args.load_schedules:
+        with open(args.load_schedules) as f:
+            for ch in json.load(f)[str(B)]:
+                ctx.set_relax_schedule(ch["W"], ch["H"], ch)
+                tuned.append(dict(level=f"{ch['W']}x{ch['H']}", tile=f"{ch['tile_w']}x{ch['tile_h']}",
+                                  iters=ch["tile_iters"], threads=ch["threads"], loaded=True))
+    elif args.variant == "tiled" and not args.no_autotune:
+        # one-time schedule search per level shape (outside the timed region)
+        tuned = autotune(B)
+        print(f"bench: autotuned for batch {B}: {tuned}", file=sys.stderr, flush=True)
+    ctx.batch_load(p, batch)   # stdevs computed on the device every step
+    print(f"bench: {B} x 640x480 pairs, {len(levels)} levels, 15x50, variant {args.variant}", file=sys.stderr,
+          flush=True)
+    st = ctx.batch_run(p, stats=True)
     for _ in range(args.warmup):
-        st = ctx.pair_run(p, stats=True)
-    if st is None:
-        st = ctx.pair_run(p, stats=True)
+        ctx.batch_run(p, stats=False)
 
-    # timed region: K back-to-back solves of the resident pair
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.pair_run(p, stats=False)
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    pairs_per_s = args.steps / elapsed
-    print(f"bench: timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    # timed region: K batches of resident pairs
+    elapsed = timed_runs(ctx, p, args.steps)
+    pairs_per_s = B * args.steps / elapsed
+    print(f"bench: timed {args.steps} batches in {elapsed:.3f} s", file=sys.stderr, flush=True)
 
-    # the plain schedule (one launch per colour per iteration, as the reference
-    # launches it) on the same resident pair, for comparison
-    plain_pps = None
-    if args.variant != "plain" and args.plain_steps > 0:
-        pp = params_for(R, "plain")
-        ctx.pair_run(pp, stats=True)
-        ctx.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(args.plain_steps):
-            ctx.pair_run(pp, stats=False)
-        ctx.synchronize()
-        plain_pps = args.plain_steps / (time.perf_counter() - t1)
-
-    # per-pair device latency (HIP events around each solve)
-    lat = []
-    for _ in range(args.latency_steps):
-        lat.append(ctx.pair_run(p, stats=True).device_ms)
-    lat = np.array(lat)
+    # batch latency (HIP events around each batch)
+    lat = np.array([ctx.batch_run(p, stats=True)[0].device_ms for _ in range(args.latency_steps)])
 
     # dominant kernel: the relaxation, timed launch by launch with HIP events
     ctx.set_relax_timing(True)
-    rt = ctx.pair_run(p, stats=True)
+    rst = ctx.batch_run(p, stats=True)
     ctx.set_relax_timing(False)
-    sweep_bytes, stage_bytes = algorithmic_bytes(rt)
-    relax_s = rt.relax_ms / 1e3
-    achieved = sweep_bytes / relax_s / 1e9 if relax_s > 0 else 0.0
-    avg_launch_us = rt.relax_ms * 1e3 / max(rt.relax_launches, 1)
+    sweep_bytes, stage_bytes = algorithmic_bytes(rst)
+    relax_ms = rst[0].relax_ms
+    launches = rst[0].relax_launches
+    achieved = sweep_bytes / (relax_ms / 1e3) / 1e9 if relax_ms > 0 else 0.0
+    avg_launch_us = relax_ms * 1e3 / max(launches, 1)
+
+    # the plain single-pair GPU path and the tiled single-pair path, for comparison
+    plain_pps = single_pps = None
+    single_lat = None
+    if args.plain_steps > 0:
+        pp = params_for(R, "plain")
+        ctx.batch_load(pp, batch[:1])
+        ctx.batch_run(pp, stats=True)
+        plain_pps = args.plain_steps / timed_runs(ctx, pp, args.plain_steps)
+    if args.single_steps > 0 and args.variant == "tiled":
+        if args.load_schedules:
+            with open(args.load_schedules) as f:
+                for ch in json.load(f).get("1", []):
+                    ctx.set_relax_schedule(ch["W"], ch["H"], ch)
+        elif not args.no_autotune:
+            autotune(1)
+        ctx.batch_load(p, batch[:1])
+        ctx.batch_run(p, stats=True)
+        single_pps = args.single_steps / timed_runs(ctx, p, args.single_steps)
+        single_lat = np.array([ctx.batch_run(p, stats=True)[0].device_ms for _ in range(10)])
+
+    if args.save_schedules:
+        with open(args.save_schedules, "w") as f:
+            json.dump({str(k): v for k, v in raw.items()}, f)
+
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        if tr.get("batch") == B and tr.get("variant") == args.variant:
+            traffic = tr.get("bytes_per_launch")
 
     out = {
         "metric": METRIC,
@@ -172,18 +219,22 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generator, ground-truth flow)",
         "config": {
-            "workload": "640x480 RGB-D pair, 5-level pyramid ratio 0.5, 15 outer x 50 inner, HPB regularisation",
+            "workload": f"{B} x 640x480 RGB-D pairs per step ({len(pairs)} distinct), 5-level pyramid ratio 0.5, "
+                        "15 outer x 50 inner, HPB regularisation",
+            "batch": B,
             "precision": "reference mixed: f64 images/derivatives/flow, f32 system/increments",
             "relax": args.variant,
-            "p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "sweeps_exec": [rt.sweeps_exec[k] for k in range(rt.n_levels)],
-            "outer_exec": [rt.outer_exec[k] for k in range(rt.n_levels)],
-            "algorithmic_GBps_pair": round((sweep_bytes + stage_bytes) / (np.percentile(lat, 50) / 1e3) / 1e9, 1),
-            "relax_launches_per_pair": rt.relax_launches,
+            "batch_p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "batch_p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "sweeps_exec_pair0": [rst[0].sweeps_exec[k] for k in range(rst[0].n_levels)],
+            "outer_exec_pair0": [rst[0].outer_exec[k] for k in range(rst[0].n_levels)],
+            "algorithmic_GBps": round((sweep_bytes + stage_bytes) / (np.percentile(lat, 50) / 1e3) / 1e9, 1),
+            "relax_launches_per_batch": launches,
             "relax_avg_launch_us": round(avg_launch_us, 3),
-            "plain_pairs_per_s": round(plain_pps, 3) if plain_pps else None,
-            "speedup_vs_plain": round(pairs_per_s / plain_pps, 2) if plain_pps else None,
+            "plain_single_pairs_per_s": round(plain_pps, 3) if plain_pps else None,
+            "speedup_vs_plain_single": round(pairs_per_s / plain_pps, 2) if plain_pps else None,
+            "tiled_single_pairs_per_s": round(single_pps, 3) if single_pps else None,
+            "tiled_single_p99_ms": round(float(np.percentile(single_lat, 99)), 3) if single_lat is not None else None,
             "autotuned": tuned,
         },
         "roofline": {
@@ -192,11 +243,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
         },
     }
     if not args.no_cpu_baseline:
-        v, n, el = cpu_baseline(im1, d1, im2, d2, hpb, args.cpu_budget)
+        v, n, el = cpu_baseline(pairs[0], args.cpu_budget)
         out["cpu_baseline"] = {"value": round(v, 4), "unit": "pairs/s", "cores": 1, "kind": "port",
                                "sample": f"{n} whole pairs of the same 640x480 workload through the oracle "
                                          f"(reference precision, 1 thread, {el:.1f} s)"}

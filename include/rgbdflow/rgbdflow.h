@@ -233,6 +233,14 @@ typedef struct rgbdflow_tile_choice {
  * tile_* are left to the planner (tile_iters = tile_w = tile_h = 0,
  * tile_config = -1).  out may be NULL. */
 int rgbdflow_autotune_relax(rgbdflow_ctx* ctx, int W, int H, int batch, rgbdflow_tile_choice* out);
+/* Makes a schedule (as autotune_relax reports it: tile_w, tile_h, tile_iters,
+ * tile_config, single) the context's schedule for W x H levels, e.g. one
+ * tuned earlier and stored by the caller. */
+int rgbdflow_set_relax_schedule(rgbdflow_ctx* ctx, int W, int H, const rgbdflow_tile_choice* choice);
+/* Device time per red-black iteration of a schedule on `batch` W x H levels,
+ * measured as autotune_relax measures its candidates. */
+int rgbdflow_time_relax_schedule(rgbdflow_ctx* ctx, int W, int H, int batch, const rgbdflow_tile_choice* choice,
+                                 double* us_per_iter);
 
 #ifdef __cplusplus
 }

@@ -324,3 +324,15 @@ def with_init(im1, dm1, im2, dm2, sigma1, hpb0, est, params: Params):
     if rc != 0:
         raise RuntimeError(f"orc_with_init failed ({rc})")
     return u, v, w, st
+
+
+def check_reciprocal_division(n, seed, mode, recipe=0):
+    """Samples n (a, b) in the reciprocal division's domain (divcheck.cpp) and
+    returns (mismatches against IEEE a / b, first mismatching (a, b))."""
+    L = lib()
+    f = L.orc_check_reciprocal_division
+    f.restype = C.c_uint64
+    f.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    a, b = C.c_float(), C.c_float()
+    bad = f(n, seed, mode, recipe, C.byref(a), C.byref(b))
+    return int(bad), (a.value, b.value)

@@ -1,0 +1,87 @@
+// divcheck.cpp -- checker (test infrastructure) for the reciprocal division
+// of the tiled relaxation kernel.
+//
+// The kernel replaces each IEEE single-precision division a / b of the
+// relaxation update (src/sceneFlowSingleScaleJacobiCUDA.cu:440,449,458) by
+//   y = RN(1 / b)            (an IEEE division, once per pixel and launch)
+//   q0 = RN(a * y)
+//   r  = fma(-b, q0, a)      (exact: the remainder of q0)
+//   q  = fma(r, y, q0)
+// which is RN(a / b) by Markstein's theorem (IBM J. Res. Dev. 34(1), 1990)
+// when no intermediate underflows or overflows; the kernel takes this path
+// only for |b| in [2^-60, 2^60] and a = 0 or |a| in [2^-60, 2^60] and uses the
+// IEEE division otherwise.  This file samples (a, b) in that domain and counts
+// the cases where the two differ in any bit.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+namespace {
+
+uint64_t splitmix(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+float make(uint32_t sign, int e, uint32_t mant) {
+  const uint32_t bits = (sign << 31) | ((uint32_t)(e + 127) << 23) | (mant & 0x7FFFFFu);
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
+
+bool in_domain(float a, float b) {
+  const float lo = std::ldexp(1.0f, -60), hi = std::ldexp(1.0f, 60);
+  const float aa = std::fabs(a), ab = std::fabs(b);
+  return ab >= lo && ab <= hi && (a == 0.f || (aa >= lo && aa <= hi));
+}
+
+float fast_div(float a, float b, int recipe) {
+  const float y = 1.0f / b;
+  const float q0 = a * y;
+  if (recipe == 1) return q0;  // negative control: no correction step
+  const float r = std::fmaf(-b, q0, a);
+  return std::fmaf(r, y, q0);
+}
+
+}  // namespace
+
+extern "C" {
+
+// mode 0: uniform significands and exponents in the domain; mode 1: b's
+// significand near the ends of its binade; mode 2: a's significand near b's
+// (quotients near 1 and near the ends of a binade); mode 3: small integer
+// multiples (quotients exactly representable and halfway-adjacent).
+// recipe 0 is the kernel's; recipe 1 drops the correction step (a negative
+// control the check must catch).  Returns the number of mismatches; the first
+// is stored in *ea, *eb.
+uint64_t orc_check_reciprocal_division(uint64_t n, uint64_t seed, int mode, int recipe, float* ea, float* eb) {
+  uint64_t s = seed * 0x2545F4914F6CDD1Dull + 1, bad = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    const uint64_t r1 = splitmix(s), r2 = splitmix(s);
+    const int eA = (int)(r1 % 121) - 60, eB = (int)((r1 >> 8) % 121) - 60;
+    uint32_t mA = (uint32_t)(r2 & 0x7FFFFF), mB = (uint32_t)((r2 >> 23) & 0x7FFFFF);
+    if (mode == 1) mB = (r1 >> 20) & 1 ? (uint32_t)((r2 >> 23) & 0xFF) : 0x7FFFFFu - (uint32_t)((r2 >> 23) & 0xFF);
+    if (mode == 2) mA = (mB + (uint32_t)((r2 >> 50) & 0x3FF) - 512u) & 0x7FFFFFu;
+    float a = make((uint32_t)(r1 >> 40) & 1, eA, mA), b = make((uint32_t)(r1 >> 41) & 1, eB, mB);
+    if (mode == 3) a = b * (float)((int)((r2 >> 40) % 2001) - 1000);
+    if ((r1 >> 42) % 64 == 0) a = 0.f;
+    if (!in_domain(a, b)) continue;
+    const float q = fast_div(a, b, recipe), ref = a / b;
+    uint32_t x, y;
+    std::memcpy(&x, &q, 4);
+    std::memcpy(&y, &ref, 4);
+    if (x != y) {
+      if (bad == 0 && ea && eb) {
+        *ea = a;
+        *eb = b;
+      }
+      bad++;
+    }
+  }
+  return bad;
+}
+
+}  // extern "C"

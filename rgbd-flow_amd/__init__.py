@@ -29,7 +29,8 @@ EXPORTS = [
     "rgbdflow_pair_read", "rgbdflow_multiscale", "rgbdflow_with_init", "rgbdflow_gaussian_pyramid",
     "rgbdflow_depth_map_pyramid", "rgbdflow_im2feature", "rgbdflow_warp_by_3d_flow", "rgbdflow_upsample_flow",
     "rgbdflow_level_system", "rgbdflow_relax", "rgbdflow_autotune_relax", "rgbdflow_batch_load",
-    "rgbdflow_batch_run", "rgbdflow_batch_read",
+    "rgbdflow_batch_run", "rgbdflow_batch_read", "rgbdflow_set_relax_schedule",
+    "rgbdflow_time_relax_schedule",
 ]
 
 
@@ -186,6 +187,24 @@ class Context:
         ch = TileChoice()
         self._check(lib().rgbdflow_autotune_relax(self.h, W, H, int(batch), C.byref(ch)))
         return ch.as_dict()
+
+    def set_relax_schedule(self, W, H, choice: dict):
+        """Install a schedule (a dict as autotune_relax returns) for W x H levels."""
+        ch = TileChoice()
+        for f, _ in TileChoice._fields_:
+            if f in choice:
+                setattr(ch, f, choice[f])
+        self._check(lib().rgbdflow_set_relax_schedule(self.h, W, H, C.byref(ch)))
+
+    def time_relax_schedule(self, W, H, choice: dict, batch=1):
+        """Device microseconds per red-black iteration of a schedule."""
+        ch = TileChoice()
+        for f, _ in TileChoice._fields_:
+            if f in choice:
+                setattr(ch, f, choice[f])
+        us = C.c_double()
+        self._check(lib().rgbdflow_time_relax_schedule(self.h, W, H, int(batch), C.byref(ch), C.byref(us)))
+        return us.value
 
     def set_relax_timing(self, on: bool):
         self._check(lib().rgbdflow_set_relax_timing(self.h, int(on)))

@@ -65,7 +65,7 @@ struct StepPlan {
   size_t items_off = 0, pix_off = 0, plain_off = 0, tiled_off = 0;
   Views vpix{}, vplain{}, vtiled{};
   bool tiled_ok = false;
-  int cfg = 0, cap = 0, T = 0;
+  int cfg = 0, slots = 0, T = 0;  // tiled launch shape: workgroup shape, LDS slots, iterations
   bool any_early = false;
 };
 
@@ -352,14 +352,14 @@ TileSched plan_tiles(const rgbdflow_ctx* c, int W, int H, int req_T, int req_w, 
   const int n_cfg = tiled_config_count();
   for (int cfg = 0; cfg < n_cfg; cfg++) {
     if (req_cfg >= 0 && cfg != req_cfg) continue;
-    if (tiled_config_lds(cfg) > c->lds_per_block) continue;
     int nt, p;
     tiled_config_shape(cfg, &nt, &p);
     const int cap = nt * p;
-    const int lds_occ = (int)std::max<size_t>(1, (160 * 1024) / tiled_config_lds(cfg));
-    const int occ = std::max(1, std::min(lds_occ, 2048 / nt));
     const long long pairs_img = (long long)((W + 1) / 2) * H;
     if (pairs_img <= cap && req_w == 0) {
+      if (tiled_lds_bytes(tiled_slots(TileSched{W + (W & 1), H, 1, 1, 1, cap, cfg, kCheckEvery}, W, H, 0)) >
+          c->lds_per_block)
+        continue;
       const double per_pass = std::ceil((double)pairs_img / nt) * 0.06 + 0.25;
       const double cost = 2 * (4.0 + std::ceil((double)pairs_img / nt) * 0.6 + 50 * per_pass);
       if (cost < best_cost) {
@@ -381,6 +381,10 @@ TileSched plan_tiles(const rgbdflow_ctx* c, int W, int H, int req_T, int req_w, 
           const int rw = std::min(TW + 2 * h, W + (W & 1)), rh = std::min(TH + 2 * h, H);
           const long long rp = (long long)((rw + 1) / 2) * rh;
           if (rp > cap) continue;
+          const TileSched cand{TW, TH, (W + TW - 1) / TW, (H + TH - 1) / TH, 0, cap, cfg, T};
+          const size_t lds = tiled_lds_bytes(tiled_slots(cand, W, H, T));
+          if (lds > c->lds_per_block) continue;
+          const int occ = std::max(1, std::min((int)((160 * 1024) / lds), 2048 / nt));
           const int tiles = ((W + TW - 1) / TW) * ((H + TH - 1) / TH);
           const double waves = std::ceil((double)tiles / (c->cu_count * occ));
           const double per_thread = std::ceil((double)rp / nt);
@@ -389,7 +393,7 @@ TileSched plan_tiles(const rgbdflow_ctx* c, int W, int H, int req_T, int req_w, 
           const double cost = launches * (4.0 + waves * per_wg * occ);
           if (cost < best_cost) {
             best_cost = cost;
-            best = TileSched{TW, TH, (W + TW - 1) / TW, (H + TH - 1) / TH, 0, cap, cfg, T};
+            best = cand;
           }
         }
       }
@@ -438,10 +442,14 @@ void plan_step_relax(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<Leve
     if (t.cap == 0 || (!t.single && (lt.single || t.tmax != lt.tmax))) return;
     ts[i] = t;
   }
-  for (size_t i = 0; i < items.size(); i++) items[i].ts = ts[i];
+  st.slots = 0;
+  for (size_t i = 0; i < items.size(); i++) {
+    items[i].ts = ts[i];
+    st.slots = std::max(st.slots, tiled_slots(ts[i], items[i].W, items[i].H, lt.tmax));
+  }
+  if (tiled_lds_bytes(st.slots) > c->lds_per_block) return;
   st.tiled_ok = true;
   st.cfg = lt.config;
-  st.cap = lt.cap;
   st.T = lt.tmax;
 }
 
@@ -531,12 +539,12 @@ void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, i
       const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
       const int slot = (last % kCheckEvery == kCheckEvery - 1) ? slot_base + last / kCheckEvery : -1;
       relax_event(c);
-      launch_relax_tiled(s, st.vtiled, st.tiled, st.cfg, st.cap, flip, T, slot_base, n_prev, slot);
+      launch_relax_tiled(s, st.vtiled, st.tiled, st.cfg, st.slots, flip, T, slot_base, n_prev, slot);
       relax_event(c);
       flip ^= 1;
       it += T;
     }
-    if (flip) launch_copy_du(s, st.vpix, st.pix);
+    if (k.n_sor > 0) launch_copy_du(s, st.vpix, st.pix);
     return;
   }
   for (int it = 0; it < k.n_sor; it++) {
@@ -557,10 +565,14 @@ void enqueue_level_solve(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars
   const int checks = k.n_sor / kCheckEvery;
   launch_ctl_reset(s, st.vpix);
   for (int o = 0; o < k.n_outer; o++) {
-    launch_derivs(s, st.vpix, st.pix, k.n_inner * checks);
+    if (k.n_inner == 0) launch_derivs(s, st.vpix, st.pix, 0);  // du = 0 (.cu:776), then the re-warp
     for (int h = 0; h < k.n_inner; h++) {
-      launch_phi(s, st.vpix, st.pix);
-      launch_system(s, st.vpix, st.pix, k);
+      if (h == 0) {
+        launch_outer_first(s, st.vpix, st.pix, k.n_inner * checks, k, k.n_inner > 1 ? 1 : 0);
+      } else {
+        launch_phi(s, st.vpix, st.pix);
+        launch_system(s, st.vpix, st.pix, k);
+      }
       enqueue_relax(c, st, k, h * checks);
     }
     launch_update_warp(s, st.vpix, st.pix, st.any_early ? 1 : 0);
@@ -1364,20 +1376,21 @@ int rgbdflow_multiscale(rgbdflow_ctx* c, int W, int H, const double* im1, const 
   return rgbdflow_pair_read(c, u, v, w);
 }
 
-int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, rgbdflow_tile_choice* out) {
-  if (!c) return RGBDFLOW_ERR_ARG;
-  if (bad_dims(W, H) || batch < 1 || batch > 256) return fail(c, RGBDFLOW_ERR_ARG, "bad autotune size");
-  HIP_TRY(c, hipSetDevice(c->device));
+}  // extern "C"
+
+namespace {
+
+// `batch` independent W x H test systems (own coefficients and increments)
+// for timing relaxation schedules.
+int timing_items(rgbdflow_ctx* c, int W, int H, int batch, std::vector<LevelView>& items) {
   const size_t N = (size_t)W * H;
-  // `batch` independent test systems (own coefficients and increments)
   float *coef, *du, *du_alt;
   SolveCtl* ctl;
   TRY(get(c, "at.coef", (size_t)batch * kCoef * N, &coef));
   TRY(get(c, "at.du", (size_t)batch * 3 * N, &du));
   TRY(get(c, "at.du_alt", (size_t)batch * 3 * N, &du_alt));
   TRY(get(c, "at.ctl", batch, &ctl));
-  hipStream_t s = c->stream;
-  std::vector<LevelView> items(batch);
+  items.assign(batch, LevelView{});
   for (int b = 0; b < batch; b++) {
     LevelView& v = items[b];
     std::memset(&v, 0, sizeof(v));
@@ -1388,23 +1401,104 @@ int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, rgbdflow_t
     v.du = du + (size_t)b * 3 * N;
     v.du_alt = du_alt + (size_t)b * 3 * N;
     v.ctl = ctl + b;
-    launch_fill_test_system(s, v.coef, v.du, W, H);
+    launch_fill_test_system(c->stream, v.coef, v.du, W, H);
   }
+  return RGBDFLOW_OK;
+}
+
+// Device time per red-black iteration of one check interval (25 iterations)
+// of a schedule on the timing items: best of 3 after a warm-up.
+int time_schedule(rgbdflow_ctx* c, std::vector<LevelView>& items, const TileSched& ts, hipEvent_t e0, hipEvent_t e1,
+                  double* us) {
+  const int W = items[0].W, H = items[0].H;
+  for (LevelView& v : items) v.ts = ts;
+  StepPlan sp;
+  sp.tiled_ok = true;
+  sp.cfg = ts.config;
+  sp.slots = tiled_slots(ts, W, H, ts.tmax);
+  sp.T = ts.tmax;
+  TRY(one_step(c, "at.desc", items, nullptr, sp, true));
   SolveScalars k{};
   k.n_sor = kCheckEvery;  // one check interval: the unit the schedule repeats
   k.inner_exit = 0;
+  hipStream_t s = c->stream;
+  const bool timing = c->time_relax;
+  c->time_relax = false;
+  launch_ctl_reset(s, sp.vpix);
+  enqueue_relax(c, sp, k, 0);  // warm-up
+  double t_best = 1e300;
+  for (int rep = 0; rep < 3; rep++) {
+    hipEventRecord(e0, s);
+    enqueue_relax(c, sp, k, 0);
+    hipEventRecord(e1, s);
+    hipEventSynchronize(e1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    t_best = std::min(t_best, (double)ms);
+  }
+  c->time_relax = timing;
+  *us = t_best * 1e3 / kCheckEvery;
+  return sync(c);
+}
+
+// The tile schedule a caller's choice describes, validated for a W x H level.
+int schedule_from_choice(rgbdflow_ctx* c, int W, int H, const rgbdflow_tile_choice* ch, TileSched* out) {
+  if (ch->tile_config < 0 || ch->tile_config >= tiled_config_count())
+    return fail(c, RGBDFLOW_ERR_ARG, "bad workgroup shape index");
+  if (ch->tile_iters < 1 || ch->tile_iters > kCheckEvery) return fail(c, RGBDFLOW_ERR_ARG, "bad iterations per launch");
+  int nt, pp;
+  tiled_config_shape(ch->tile_config, &nt, &pp);
+  const int cap = nt * pp;
+  TileSched t{};
+  if (ch->single) {
+    if ((long long)((W + 1) / 2) * H > cap) return fail(c, RGBDFLOW_ERR_ARG, "level does not fit one workgroup");
+    t = TileSched{W + (W & 1), H, 1, 1, 1, cap, ch->tile_config, ch->tile_iters};
+  } else {
+    const int TW = ch->tile_w, TH = ch->tile_h, h = 2 * ch->tile_iters;
+    if (TW < 2 || (TW & 1) || TH < 1) return fail(c, RGBDFLOW_ERR_ARG, "bad tile shape");
+    const int rw = std::min(TW + 2 * h, W + (W & 1)), rh = std::min(TH + 2 * h, H);
+    if ((long long)((rw + 1) / 2) * rh > cap) return fail(c, RGBDFLOW_ERR_ARG, "tile and halo exceed the workgroup");
+    t = TileSched{TW, TH, (W + TW - 1) / TW, (H + TH - 1) / TH, 0, cap, ch->tile_config, ch->tile_iters};
+  }
+  if (tiled_lds_bytes(tiled_slots(t, W, H, t.tmax)) > c->lds_per_block)
+    return fail(c, RGBDFLOW_ERR_ARG, "tile exceeds the workgroup's LDS");
+  *out = t;
+  return RGBDFLOW_OK;
+}
+
+void choice_from_schedule(const TileSched& t, int candidates, double us, rgbdflow_tile_choice* out) {
+  out->tile_w = t.TW;
+  out->tile_h = t.TH;
+  out->tile_iters = t.tmax;
+  out->tile_config = t.config;
+  tiled_config_shape(t.config, &out->threads, &out->pairs_per_thread);
+  out->single = t.single;
+  out->candidates = candidates;
+  out->us_per_iter = us;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, rgbdflow_tile_choice* out) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (bad_dims(W, H) || batch < 1 || batch > 256) return fail(c, RGBDFLOW_ERR_ARG, "bad autotune size");
+  HIP_TRY(c, hipSetDevice(c->device));
+  std::vector<LevelView> items;
+  TRY(timing_items(c, W, H, batch, items));
   // candidate schedules
   std::vector<TileSched> cand;
   const long long pairs_img = (long long)((W + 1) / 2) * H;
   const int Ts[] = {1, 2, 3, 4, 5, 6, 8, 12};
   const int TWs[] = {8, 16, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320};
   for (int cfg = 0; cfg < tiled_config_count(); cfg++) {
-    if (tiled_config_lds(cfg) > c->lds_per_block) continue;
     int nt, pp;
     tiled_config_shape(cfg, &nt, &pp);
     const int cap = nt * pp;
     if (pairs_img <= cap) {
-      cand.push_back(TileSched{W + (W & 1), H, 1, 1, 1, cap, cfg, kCheckEvery});
+      const TileSched t{W + (W & 1), H, 1, 1, 1, cap, cfg, kCheckEvery};
+      if (tiled_lds_bytes(tiled_slots(t, W, H, 0)) <= c->lds_per_block) cand.push_back(t);
       continue;
     }
     for (int T : Ts) {
@@ -1418,7 +1512,9 @@ int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, rgbdflow_t
         for (int TH : {thmax, thmax / 2}) {
           if (TH < 2) continue;
           if (((rw + 1) / 2) * (long long)std::min(TH + 2 * h, H) > cap) continue;
-          cand.push_back(TileSched{TW, TH, (W + TW - 1) / TW, (H + TH - 1) / TH, 0, cap, cfg, T});
+          const TileSched t{TW, TH, (W + TW - 1) / TW, (H + TH - 1) / TH, 0, cap, cfg, T};
+          if (tiled_lds_bytes(tiled_slots(t, W, H, T)) > c->lds_per_block) continue;
+          cand.push_back(t);
         }
       }
     }
@@ -1429,53 +1525,51 @@ int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, rgbdflow_t
   HIP_TRY(c, hipEventCreate(&e1));
   double best_us = 1e300;
   TileSched best{};
-  const bool timing = c->time_relax;
-  c->time_relax = false;
   int rc = RGBDFLOW_OK;
   for (const TileSched& ts : cand) {
-    for (LevelView& v : items) v.ts = ts;
-    StepPlan sp;
-    sp.tiled_ok = true;
-    sp.cfg = ts.config;
-    sp.cap = ts.cap;
-    sp.T = ts.tmax;
-    rc = one_step(c, "at.desc", items, nullptr, sp, true);
+    double us = 0;
+    rc = time_schedule(c, items, ts, e0, e1, &us);
     if (rc != RGBDFLOW_OK) break;
-    launch_ctl_reset(s, sp.vpix);
-    enqueue_relax(c, sp, k, 0);  // warm-up
-    double t_best = 1e300;
-    for (int rep = 0; rep < 3; rep++) {
-      hipEventRecord(e0, s);
-      enqueue_relax(c, sp, k, 0);
-      hipEventRecord(e1, s);
-      hipEventSynchronize(e1);
-      float ms = 0;
-      hipEventElapsedTime(&ms, e0, e1);
-      t_best = std::min(t_best, (double)ms);
-    }
-    const double us = t_best * 1e3 / kCheckEvery;
     if (us < best_us) {
       best_us = us;
       best = ts;
     }
   }
-  c->time_relax = timing;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (rc != RGBDFLOW_OK) return rc;
-  TRY(sync(c));
   c->plans[plan_key(W, H, 0, 0, 0, -1)] = best;
   c->plans_gen++;
-  if (out) {
-    out->tile_w = best.TW;
-    out->tile_h = best.TH;
-    out->tile_iters = best.tmax;
-    out->tile_config = best.config;
-    tiled_config_shape(best.config, &out->threads, &out->pairs_per_thread);
-    out->single = best.single;
-    out->candidates = (int)cand.size();
-    out->us_per_iter = best_us;
-  }
+  if (out) choice_from_schedule(best, (int)cand.size(), best_us, out);
+  return RGBDFLOW_OK;
+}
+
+int rgbdflow_time_relax_schedule(rgbdflow_ctx* c, int W, int H, int batch, const rgbdflow_tile_choice* ch,
+                                 double* us_per_iter) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (bad_dims(W, H) || batch < 1 || batch > 256 || !ch || !us_per_iter)
+    return fail(c, RGBDFLOW_ERR_ARG, "bad timing arguments");
+  HIP_TRY(c, hipSetDevice(c->device));
+  TileSched t;
+  TRY(schedule_from_choice(c, W, H, ch, &t));
+  std::vector<LevelView> items;
+  TRY(timing_items(c, W, H, batch, items));
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  const int rc = time_schedule(c, items, t, e0, e1, us_per_iter);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return rc;
+}
+
+int rgbdflow_set_relax_schedule(rgbdflow_ctx* c, int W, int H, const rgbdflow_tile_choice* ch) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (bad_dims(W, H) || !ch) return fail(c, RGBDFLOW_ERR_ARG, "bad schedule arguments");
+  TileSched t;
+  TRY(schedule_from_choice(c, W, H, ch, &t));
+  c->plans[plan_key(W, H, 0, 0, 0, -1)] = t;
+  c->plans_gen++;
   return RGBDFLOW_OK;
 }
 

@@ -33,7 +33,7 @@ struct SolveCtl {
   int outer_done;                     // outer loop left early (stat < threshold)
   int outer_exit_at;                  // outer iteration that set outer_done
   int sweeps_exec;                    // red-black iterations executed (all outers)
-  int pad;
+  int du_in_alt;                      // the last tiled launch that ran wrote du_alt
   unsigned int maxdiff[kMaxChecks];   // float bits of max|du change| per check of the
                                       // current outer, [inner][check]; zeroed per outer
   double stat[kMaxOuter];             // outer convergence statistic per outer iteration
@@ -156,6 +156,9 @@ void launch_ctl_reset(hipStream_t s, const Views& v);
 void launch_derivs(hipStream_t s, const Views& v, int blocks, int n_slots);
 void launch_phi(hipStream_t s, const Views& v, int blocks);
 void launch_system(hipStream_t s, const Views& v, int blocks, const SolveScalars& k);
+// derivatives + phi + system of the first inner iteration in one pass
+// (store: keep derivatives, occlusion weights and phi for later inner iterations)
+void launch_outer_first(hipStream_t s, const Views& v, int blocks, int n_slots, const SolveScalars& k, int store);
 void launch_relax_plain(hipStream_t s, const Views& v, int blocks, int colour, int slot_base, int n_prev_checks,
                         int check_slot);
 void launch_update_warp(hipStream_t s, const Views& v, int blocks, int want_stat);
@@ -164,8 +167,23 @@ void launch_stat_final(hipStream_t s, const Views& v, int outer, int decide);
 // kernels_relax_tiled.hip
 int tiled_config_count();
 void tiled_config_shape(int idx, int* nt, int* p);
-size_t tiled_config_lds(int idx);
-void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int cap, int flip, int n_iter,
+size_t tiled_lds_bytes(int slots);
+// LDS slots per colour plane for a tile schedule run T iterations per launch:
+// the largest region (tile + halo of 2T, clipped to the level, in pixel pairs)
+// padded by one pair on every side.
+inline int tiled_slots(const TileSched& t, int W, int H, int T) {
+  int PR, RH;
+  if (t.single) {
+    PR = (W + 1) / 2;
+    RH = H;
+  } else {
+    const int h = 2 * T;
+    PR = ((t.TW + 2 * h < W ? t.TW + 2 * h : W) + 1) / 2;
+    RH = t.TH + 2 * h < H ? t.TH + 2 * h : H;
+  }
+  return (PR + 2) * (RH + 2);
+}
+void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int slots, int flip, int n_iter,
                         int slot_base, int n_prev_checks, int check_slot);
 void launch_copy_du(hipStream_t s, const Views& v, int blocks);
 void launch_fill_test_system(hipStream_t s, float* coef, float* du, int W, int H);

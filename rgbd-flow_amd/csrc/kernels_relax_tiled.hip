@@ -28,87 +28,86 @@ namespace {
 
 // One colour pass over the pairs a thread owns; colour C is a compile-time
 // constant so every per-pixel array is indexed statically (kept in VGPRs).
+// X4: two colour planes of Si float4 slots (x, y, z, unused) over the region
+// padded by one pair on every side; pads and absent pixels hold zeros.
 template <int P, int C>
-__device__ __forceinline__ void tiled_pass(float* X, int cap, int PR, int RH, const PixCoef (&cf)[P][2],
-                                           const int (&qy)[P], const int (&qk)[P], const int (&par)[P],
+__device__ __forceinline__ void tiled_pass(float4* X4, int Si, int PRp, const PixCoef (&cf)[P][2],
+                                           const float (&yr)[P][2][3], const int (&qi)[P], const int (&par)[P],
                                            const unsigned (&fl)[P][2], bool last, float& change) {
   constexpr int O = 1 - C;  // the colour read
+  const float4* Xo = X4 + O * Si;
+  float4* Xc = X4 + C * Si;
 #pragma unroll
   for (int s = 0; s < P; s++) {
     const unsigned f = fl[s][C];
     if (!(f & 16u)) continue;  // no pixel of this colour
-    const int y = qy[s], kk = qk[s], e = par[s];
-    const int q = y * PR + kk;
+    const int e = par[s];
+    const int idx = qi[s];
     // same-row neighbours of the other colour: red reads pairs (k-1+e, k+e),
     // black reads (k-e, k+1-e); the rows above/below read pair k
-    const int kl = C == 0 ? kk - 1 + e : kk - e;
-    const int kr = C == 0 ? kk + e : kk + 1 - e;
-    const int ql = y * PR + min(max(kl, 0), PR - 1);
-    const int qr = y * PR + min(max(kr, 0), PR - 1);
-    const int qu = max(y - 1, 0) * PR + kk;
-    const int qd = min(y + 1, RH - 1) * PR + kk;
-    float xo[3], xl[3], xu[3], xr[3], xd[3], xn[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const float* Po = X + (O * 3 + k) * cap;
-      xo[k] = X[(C * 3 + k) * cap + q];
-      xl[k] = Po[ql];
-      xu[k] = Po[qu];
-      xr[k] = Po[qr];
-      xd[k] = Po[qd];
-    }
-    relax_update(cf[s][C], f & 1u, f & 2u, f & 4u, f & 8u, xl, xu, xr, xd, xo, xn);
-#pragma unroll
-    for (int k = 0; k < 3; k++) X[(C * 3 + k) * cap + q] = xn[k];
+    const int il = C == 0 ? idx - 1 + e : idx - e;
+    const int ir = C == 0 ? idx + e : idx + 1 - e;
+    const float4 o = Xc[idx];
+    const float4 L = Xo[il], R = Xo[ir], U = Xo[idx - PRp], D = Xo[idx + PRp];
+    float xn[3];
+    relax_update_dense(cf[s][C], yr[s][C], f & 64u, L, U, R, D, o, xn);
+    Xc[idx] = make_float4(xn[0], xn[1], xn[2], 0.f);
     if (last && (f & 32u))
-      change = fmaxf(change, fmaxf(fmaxf(fabsf(xn[0] - xo[0]), fabsf(xn[1] - xo[1])), fabsf(xn[2] - xo[2])));
+      change = fmaxf(change, fmaxf(fmaxf(fabsf(xn[0] - o.x), fabsf(xn[1] - o.y)), fabsf(xn[2] - o.z)));
   }
 }
 
-// Reads du from one buffer and writes the tile interiors to the other (flip
-// selects which): a workgroup's halo overlaps other workgroups' interiors,
-// which must still hold the values from before this launch.  A launch after an
-// early exit copies its interior, so the host knows which buffer holds du
-// after every launch.  Block b works on tile b - start[item] of its item.
+// One tile: reads du from one buffer and writes the tile's interior to the
+// other (flip selects which): a workgroup's halo overlaps other tiles'
+// interiors, which must still hold the values from before this launch.  Once
+// the relaxation has stopped for this (outer, inner), every later launch does
+// nothing; the last launch that ran records in SolveCtl::du_in_alt where du
+// is, and k_copy_du moves it to du at the end.  t: the tile's index in the
+// launch's work-list.
 template <int NT, int P>
-__global__ void __launch_bounds__(NT) k_relax_tiled(Views wl, int flip, int n_iter, int slot_base,
-                                                    int n_prev_checks, int check_slot) {
-  extern __shared__ float X[];
-  const int item = work_item(wl, blockIdx.x);
+__device__ __forceinline__ void relax_tile(float4* X4, const Views& wl, int t, int flip, int n_iter, int slot_base,
+                                           int n_prev_checks, int check_slot) {
+  const int item = work_item(wl, t);
   const LevelView& v = wl.items[item];
   if (v.ctl->outer_done) return;
+  if (relax_stopped(v.ctl, slot_base, n_prev_checks)) return;
   const TileSched ts = v.ts;
   const float* __restrict__ src = flip ? v.du_alt : v.du;
   float* __restrict__ dst = flip ? v.du : v.du_alt;
   const int W = v.W, H = v.H, N = v.N;
-  const int tb = blockIdx.x - wl.start[item];
+  const int tb = t - wl.start[item];
   const int bx = tb % ts.tiles_x, by = tb / ts.tiles_x;
   const int ox = bx * ts.TW, oy = by * ts.TH;
   const int ex = min(ox + ts.TW, W), ey = min(oy + ts.TH, H);
-  if (relax_stopped(v.ctl, slot_base, n_prev_checks)) {
-    for (int t = threadIdx.x; t < (ex - ox) * (ey - oy); t += NT) {
-      const int l = (oy + t / (ex - ox)) * W + ox + t % (ex - ox);
-#pragma unroll
-      for (int k = 0; k < 3; k++) dst[(size_t)k * N + l] = src[(size_t)k * N + l];
-    }
-    return;
-  }
   const int h = ts.single ? 0 : 2 * n_iter;
   const int rx0 = max(0, ox - h) & ~1, ry0 = max(0, oy - h);
   const int rx1 = min(W, ex + h), ry1 = min(H, ey + h);
   const int PR = (rx1 - rx0 + 1) >> 1, RH = ry1 - ry0;
   const int np = PR * RH;
-  const int cap = ts.cap;  // plane (colour c, component k) at X + (c * 3 + k) * cap
+  const int PRp = PR + 2;
+  const int Si = PRp * (RH + 2);  // slots per colour plane
+
+  // zero the pad ring of both planes (region slots are written by their owners)
+  for (int t = threadIdx.x; t < 2 * (2 * PRp + 2 * RH); t += NT) {
+    const int c = t >= 2 * PRp + 2 * RH;
+    const int r = t - c * (2 * PRp + 2 * RH);
+    int slot;
+    if (r < PRp) slot = r;                                   // top pad row
+    else if (r < 2 * PRp) slot = (RH + 1) * PRp + r - PRp;   // bottom pad row
+    else if (r < 2 * PRp + RH) slot = (r - 2 * PRp + 1) * PRp;              // left pads
+    else slot = (r - 2 * PRp - RH + 1) * PRp + PRp - 1;      // right pads
+    X4[c * Si + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 
   PixCoef cf[P][2];
-  int qy[P], qk[P], par[P];
-  unsigned fl[P][2];  // bits: 1 left, 2 up, 4 right, 8 down exist; 16 pixel valid; 32 interior
+  float yr[P][2][3];
+  int qi[P], par[P];
+  unsigned fl[P][2];  // bits: 16 pixel valid; 32 interior; 64 diagonals in the reciprocal division's domain
 #pragma unroll
   for (int s = 0; s < P; s++) {
     const int q = threadIdx.x + s * NT;
     const int y = q / max(PR, 1), kk = q - y * PR;
-    qy[s] = y;
-    qk[s] = kk;
+    qi[s] = (y + 1) * PRp + kk + 1;
     const int gy = ry0 + y;
     const int e = (gy + rx0) & 1;  // 0: the pair's first pixel is red
     par[s] = e;
@@ -116,19 +115,28 @@ __global__ void __launch_bounds__(NT) k_relax_tiled(Views wl, int flip, int n_it
     for (int c = 0; c < 2; c++) {
       const int gc = rx0 + 2 * kk + (c == 0 ? e : 1 - e);
       const bool valid = q < np && gc < W;
-      const bool hl = gc > 0, hu = gy > 0;
-      unsigned f = (hl ? 1u : 0u) | (hu ? 2u : 0u) | (gc < W - 1 ? 4u : 0u) | (gy < H - 1 ? 8u : 0u);
+      unsigned f = 0u;
       if (valid) f |= 16u;
       if (valid && gc >= ox && gc < ex && gy >= oy && gy < ey) f |= 32u;
-      fl[s][c] = f;
       const int l = gy * W + gc;
       if (valid) {
-        cf[s][c] = load_coef(v.coef, N, W, l, hl, hu);
-#pragma unroll
-        for (int k = 0; k < 3; k++) X[(c * 3 + k) * cap + q] = src[(size_t)k * N + l];
+        const bool hl = gc > 0, hu = gy > 0, hr = gc < W - 1, hd = gy < H - 1;
+        PixCoef k = load_coef(v.coef, N, W, l, hl, hu);
+        // a missing neighbour's coefficient is exactly zero (its value is a zero pad)
+        if (!hr) k.rxy = k.rz = 0.f;
+        if (!hd) k.dxy = k.dz = 0.f;
+        cf[s][c] = k;
+        yr[s][c][0] = 1.0f / k.Buu;
+        yr[s][c][1] = 1.0f / k.Bvv;
+        yr[s][c][2] = 1.0f / k.Bww;
+        if (recip_den_ok(k.Buu) && recip_den_ok(k.Bvv) && recip_den_ok(k.Bww)) f |= 64u;
+        X4[c * Si + qi[s]] = make_float4(src[l], src[(size_t)N + l], src[2 * (size_t)N + l], 0.f);
       } else {
         cf[s][c] = PixCoef{};
+        yr[s][c][0] = yr[s][c][1] = yr[s][c][2] = 0.f;
+        if (q < np) X4[c * Si + qi[s]] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
+      fl[s][c] = f;
     }
   }
 
@@ -136,36 +144,58 @@ __global__ void __launch_bounds__(NT) k_relax_tiled(Views wl, int flip, int n_it
   for (int it = 0; it < n_iter; it++) {
     const bool last = it == n_iter - 1;
     __syncthreads();
-    tiled_pass<P, 0>(X, cap, PR, RH, cf, qy, qk, par, fl, last, change);
+    tiled_pass<P, 0>(X4, Si, PRp, cf, yr, qi, par, fl, last, change);
     __syncthreads();
-    tiled_pass<P, 1>(X, cap, PR, RH, cf, qy, qk, par, fl, last, change);
+    tiled_pass<P, 1>(X4, Si, PRp, cf, yr, qi, par, fl, last, change);
   }
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < P; s++) {
-    const int q = qy[s] * PR + qk[s];
-    const int gy = ry0 + qy[s];
+    const int y = qi[s] / PRp - 1, kk = qi[s] - (y + 1) * PRp - 1;
+    const int gy = ry0 + y;
 #pragma unroll
     for (int c = 0; c < 2; c++) {
       if (!(fl[s][c] & 32u)) continue;
-      const int gc = rx0 + 2 * qk[s] + (c == 0 ? par[s] : 1 - par[s]);
+      const int gc = rx0 + 2 * kk + (c == 0 ? par[s] : 1 - par[s]);
       const int l = gy * W + gc;
-#pragma unroll
-      for (int k = 0; k < 3; k++) dst[(size_t)k * N + l] = X[(c * 3 + k) * cap + q];
+      const float4 x = X4[c * Si + qi[s]];
+      dst[l] = x.x;
+      dst[(size_t)N + l] = x.y;
+      dst[2 * (size_t)N + l] = x.z;
     }
   }
   if (check_slot >= 0) {
     const float m = wave_max(change);
     if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(&v.ctl->maxdiff[check_slot], __float_as_uint(m));
   }
-  if (tb == 0 && threadIdx.x == 0) v.ctl->sweeps_exec += n_iter;
+  if (tb == 0 && threadIdx.x == 0) {
+    v.ctl->sweeps_exec += n_iter;
+    v.ctl->du_in_alt = flip ? 0 : 1;
+  }
 }
 
-// Copies du_alt into du for every item (after an odd number of tiled launches).
+// A persistent launch: workgroup b loops over tiles.  Workgroups are placed on
+// the 8 XCDs round-robin by id, and the grid is a multiple of 8, so workgroup
+// b always runs on XCD b % 8; each XCD gets a contiguous run of tiles, whose
+// shared halos then hit in that XCD's L2.
+template <int NT, int P>
+__global__ void __launch_bounds__(NT) k_relax_tiled(Views wl, int total, int flip, int n_iter, int slot_base,
+                                                    int n_prev_checks, int check_slot) {
+  extern __shared__ float4 X4[];
+  const int per_xcd = (total + 7) / 8;
+  for (int b = blockIdx.x; b < 8 * per_xcd; b += gridDim.x) {
+    const int t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= total) continue;
+    relax_tile<NT, P>(X4, wl, t, flip, n_iter, slot_base, n_prev_checks, check_slot);
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+// Copies du_alt into du for every item whose last tiled launch wrote du_alt.
 __global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
   const int item = work_item(wl, blockIdx.x);
   const LevelView& v = wl.items[item];
-  if (v.ctl->outer_done) return;
+  if (v.ctl->outer_done || !v.ctl->du_in_alt) return;
   const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
   for (int k = 0; k < 3; k++) v.du[(size_t)k * v.N + l] = v.du_alt[(size_t)k * v.N + l];
@@ -173,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
 
 struct TiledConfig {
   int nt, p;
-  void (*fn)(Views, int, int, int, int, int);
+  void (*fn)(Views, int, int, int, int, int, int);
 };
 
 #define RGBDFLOW_TILED(NT, P) TiledConfig{NT, P, k_relax_tiled<NT, P>}
@@ -189,7 +219,7 @@ void tiled_config_shape(int idx, int* nt, int* p) {
   *p = kConfigs[idx].p;
 }
 
-size_t tiled_config_lds(int idx) { return (size_t)6 * kConfigs[idx].nt * kConfigs[idx].p * sizeof(float); }
+size_t tiled_lds_bytes(int slots) { return (size_t)2 * slots * sizeof(float4); }
 
 __global__ void k_copy_f32(const float* src, size_t n, float* dst) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -228,16 +258,37 @@ void launch_copy_du(hipStream_t s, const Views& v, int blocks) {
   hipLaunchKernelGGL(k_copy_du, dim3(blocks), dim3(kBlock), 0, s, v);
 }
 
-void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int cap, int flip, int n_iter,
+void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int slots, int flip, int n_iter,
                         int slot_base, int n_prev_checks, int check_slot) {
   const TiledConfig& c = kConfigs[config];
-  const size_t lds = (size_t)6 * cap * sizeof(float);
-  static bool raised[16] = {};
-  if (lds > 65536 && !raised[config]) {
+  const size_t lds = tiled_lds_bytes(slots);
+  static size_t raised[16] = {};
+  if (lds > 65536 && lds > raised[config]) {
     hipFuncSetAttribute(reinterpret_cast<const void*>(c.fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    raised[config] = true;
+    raised[config] = lds;
   }
-  hipLaunchKernelGGL(c.fn, dim3(blocks), dim3(c.nt), lds, s, v, flip, n_iter, slot_base, n_prev_checks, check_slot);
+  // resident workgroups per CU for this shape (cached per configuration and LDS size)
+  static int occ_cfg[16] = {}, occ_lds[16] = {}, occ_val[16] = {}, cus = 0;
+  if (occ_lds[config] != (int)lds || occ_cfg[config] == 0) {
+    int n = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(c.fn), c.nt, lds) !=
+            hipSuccess ||
+        n < 1)
+      n = 1;
+    occ_val[config] = n;
+    occ_lds[config] = (int)lds;
+    occ_cfg[config] = 1;
+  }
+  if (cus == 0) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  }
+  const int per_xcd = (blocks + 7) / 8;
+  int grid = std::min(8 * per_xcd, cus * occ_val[config]);
+  grid = std::max(8, grid & ~7);
+  hipLaunchKernelGGL(c.fn, dim3(grid), dim3(c.nt), lds, s, v, blocks, flip, n_iter, slot_base, n_prev_checks,
+                     check_slot);
 }
 
 }  // namespace rgbdflow

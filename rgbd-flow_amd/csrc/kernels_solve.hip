@@ -36,20 +36,16 @@ __global__ void k_ctl_reset(Views w) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
-// .cu:84-147 (x/y derivatives of 0.4*I1 + 0.6*warped I2 with [1,-8,0,8,-1]/12,
-// clamp-to-edge; depth derivatives clamped to +-0.04), .cu:776 (du = 0),
-// .cu:640-648 (occlusion weight from the UNWARPED D2).  Also zeroes the item's
-// check slots of this outer iteration.
-__global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
-  const int it = work_item(w, blockIdx.x);
-  const LevelView& v = w.items[it];
-  if (v.ctl->outer_done) return;
-  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
-  const int stride = (w.start[it + 1] - w.start[it]) * kBlock;
-  for (int t = l; t < n_slots; t += stride) v.ctl->maxdiff[t] = 0u;
-  if (l >= v.N) return;
+// Image and depth derivatives of one pixel (.cu:84-147): x/y derivatives of
+// 0.4*I1 + 0.6*warped I2 with [1,-8,0,8,-1]/12, clamp-to-edge; It = Iw - I1;
+// depth derivatives clamped to +-0.04.
+struct PixDerivs {
+  double Ix[kFeat], Iy[kFeat], It[kFeat];
+  double Dx, Dy, Dt;
+};
+
+__device__ __forceinline__ void derivs_at(const LevelView& v, int l, int i, int j, PixDerivs& d) {
   const int W = v.W, H = v.H, N = v.N;
-  const int i = l / W, j = l - i * W;
   const double g[5] = {1, -8, 0, 8, -1};
   for (int k = 0; k < kFeat; k++) {
     const double* a = v.I1 + (size_t)k * N;
@@ -63,41 +59,84 @@ __global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
       const int o = clampi(i - 2 + m, 0, H - 1) * W + j;
       sy += g[m] * (.4 * a[o] + .6 * b[o]);
     }
-    v.Ix[(size_t)k * N + l] = sx / 12;
-    v.Iy[(size_t)k * N + l] = sy / 12;
-    v.It[(size_t)k * N + l] = b[l] - a[l];
+    d.Ix[k] = sx / 12;
+    d.Iy[k] = sy / 12;
+    d.It[k] = b[l] - a[l];
   }
-  {
-    const double mg = .04;
-    double sx = 0, sy = 0;
-    for (int m = 0; m < 5; m++) {
-      const int o = i * W + clampi(j - 2 + m, 0, W - 1);
-      sx += g[m] * (.4 * v.D1[o] + .6 * v.Dw[o]);
-    }
-    for (int m = 0; m < 5; m++) {
-      const int o = clampi(i - 2 + m, 0, H - 1) * W + j;
-      sy += g[m] * (.4 * v.D1[o] + .6 * v.Dw[o]);
-    }
-    v.Dx[l] = fmin(fmax(sx / 12, -mg), mg);
-    v.Dy[l] = fmin(fmax(sy / 12, -mg), mg);
-    v.Dt[l] = fmin(fmax(v.Dw[l] - v.D1[l], -mg), mg);
+  const double mg = .04;
+  double sx = 0, sy = 0;
+  for (int m = 0; m < 5; m++) {
+    const int o = i * W + clampi(j - 2 + m, 0, W - 1);
+    sx += g[m] * (.4 * v.D1[o] + .6 * v.Dw[o]);
   }
+  for (int m = 0; m < 5; m++) {
+    const int o = clampi(i - 2 + m, 0, H - 1) * W + j;
+    sy += g[m] * (.4 * v.D1[o] + .6 * v.Dw[o]);
+  }
+  d.Dx = fmin(fmax(sx / 12, -mg), mg);
+  d.Dy = fmin(fmax(sy / 12, -mg), mg);
+  d.Dt = fmin(fmax(v.Dw[l] - v.D1[l], -mg), mg);
+}
+
+// .cu:640-648: occlusion weight from the UNWARPED D2.
+__device__ __forceinline__ float occlusion_at(const LevelView& v, int l) {
+  const double d1 = v.D1[l];
+  return (d1 + v.u[2 * v.N + l] < v.D2[l] + .02 * (double)sqr_f32(d1)) ? 1.f : (float).01;
+}
+
+// Derivatives + occlusion stored for the inner iterations, du = 0 (.cu:776);
+// also zeroes the item's check slots of this outer iteration.
+__global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
+  const int it = work_item(w, blockIdx.x);
+  const LevelView& v = w.items[it];
+  if (v.ctl->outer_done) return;
+  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
+  const int stride = (w.start[it + 1] - w.start[it]) * kBlock;
+  for (int t = l; t < n_slots; t += stride) v.ctl->maxdiff[t] = 0u;
+  if (l >= v.N) return;
+  const int N = v.N;
+  const int i = l / v.W, j = l - i * v.W;
+  PixDerivs d;
+  derivs_at(v, l, i, j, d);
+  for (int k = 0; k < kFeat; k++) {
+    v.Ix[(size_t)k * N + l] = d.Ix[k];
+    v.Iy[(size_t)k * N + l] = d.Iy[k];
+    v.It[(size_t)k * N + l] = d.It[k];
+  }
+  v.Dx[l] = d.Dx;
+  v.Dy[l] = d.Dy;
+  v.Dt[l] = d.Dt;
   v.du[l] = 0.f;
   v.du[N + l] = 0.f;
   v.du[2 * N + l] = 0.f;
-  const double d1 = v.D1[l];
-  v.q[l] = (d1 + v.u[2 * N + l] < v.D2[l] + .02 * (double)sqr_f32(d1)) ? 1.f : (float).01;
+  v.q[l] = occlusion_at(v, l);
 }
 
 // .cu:156-182: forward differences of u + du (zero on the last row/column).
+// zero_du: du is known to be zero (first inner iteration), the same sums
+// with du's terms written as 0.
+template <bool zero_du>
 __device__ __forceinline__ void flow_grad(const LevelView& v, int c, int l, int i, int j, double& gx, double& gy) {
   const double* uc = v.u + (size_t)c * v.N;
   const float* dc = v.du + (size_t)c * v.N;
-  gy = (i == v.H - 1) ? 0.0 : (uc[l + v.W] + (double)dc[l + v.W] - uc[l] - (double)dc[l]);
-  gx = (j == v.W - 1) ? 0.0 : (uc[l + 1] + (double)dc[l + 1] - uc[l] - (double)dc[l]);
+  const double dl = zero_du ? 0.0 : (double)dc[l];
+  const double dd = zero_du || i == v.H - 1 ? 0.0 : (double)dc[l + v.W];
+  const double dr = zero_du || j == v.W - 1 ? 0.0 : (double)dc[l + 1];
+  gy = (i == v.H - 1) ? 0.0 : (uc[l + v.W] + dd - uc[l] - dl);
+  gx = (j == v.W - 1) ? 0.0 : (uc[l + 1] + dr - uc[l] - dl);
 }
 
-// .cu:354-355: smoothness weight phi' of the Charbonnier penalty.
+// .cu:354-355: smoothness weight phi' of the Charbonnier penalty at pixel l.
+template <bool zero_du>
+__device__ __forceinline__ double phi_at(const LevelView& v, int l, int i, int j) {
+  double ux, uy, vx, vy, wx, wy;
+  flow_grad<zero_du>(v, 0, l, i, j, ux, uy);
+  flow_grad<zero_du>(v, 1, l, i, j, vx, vy);
+  flow_grad<zero_du>(v, 2, l, i, j, wx, wy);
+  const float t = sqr_f32(ux) + sqr_f32(uy) + sqr_f32(vx) + sqr_f32(vy) + v.sW[l] * (sqr_f32(wx) + sqr_f32(wy));
+  return 0.5 / sqrt((double)t + 1e-6);
+}
+
 __global__ void __launch_bounds__(kBlock) k_phi(Views w) {
   const int it = work_item(w, blockIdx.x);
   const LevelView& v = w.items[it];
@@ -105,26 +144,23 @@ __global__ void __launch_bounds__(kBlock) k_phi(Views w) {
   const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
   const int i = l / v.W, j = l - i * v.W;
-  double ux, uy, vx, vy, wx, wy;
-  flow_grad(v, 0, l, i, j, ux, uy);
-  flow_grad(v, 1, l, i, j, vx, vy);
-  flow_grad(v, 2, l, i, j, wx, wy);
-  const float t = sqr_f32(ux) + sqr_f32(uy) + sqr_f32(vx) + sqr_f32(vy) + v.sW[l] * (sqr_f32(wx) + sqr_f32(wy));
-  v.phi[l] = 0.5 / sqrt((double)t + 1e-6);
+  v.phi[l] = phi_at<false>(v, l, i, j);
 }
 
 // .cu:240-333: phi- and r-weighted Laplacian of u (not u+du), normalised by the
-// mean regularisation weight of the existing edges.
-__device__ __forceinline__ double laplacian(const LevelView& v, int c, int l, int i, int j) {
+// mean regularisation weight of the existing edges.  pC, pL, pU: phi at the
+// pixel, its left and its upper neighbour.
+__device__ __forceinline__ double laplacian(const LevelView& v, int c, int l, int i, int j, double pC, double pL,
+                                            double pU) {
   const double* uc = v.u + (size_t)c * v.N;
   const float* rR = v.rW;
   const float* rD = v.rW + v.N;
   const int W = v.W;
   double lap = 0;
-  if (j < W - 1) lap -= (uc[l + 1] - uc[l]) * v.phi[l] * (double)rR[l];
-  if (j > 0) lap += (uc[l] - uc[l - 1]) * v.phi[l - 1] * (double)rR[l - 1];
-  if (i < v.H - 1) lap -= (uc[l + W] - uc[l]) * v.phi[l] * (double)rD[l];
-  if (i > 0) lap += (uc[l] - uc[l - W]) * v.phi[l - W] * (double)rD[l - W];
+  if (j < W - 1) lap -= (uc[l + 1] - uc[l]) * pC * (double)rR[l];
+  if (j > 0) lap += (uc[l] - uc[l - 1]) * pL * (double)rR[l - 1];
+  if (i < v.H - 1) lap -= (uc[l + W] - uc[l]) * pC * (double)rD[l];
+  if (i > 0) lap += (uc[l] - uc[l - W]) * pU * (double)rD[l - W];
   double norm = 0;
   int count = 0;
   if (j < W - 1) { norm += rR[l]; count++; }
@@ -137,58 +173,51 @@ __device__ __forceinline__ double laplacian(const LevelView& v, int c, int l, in
 }
 
 // .cu:356-369 (psi', pi'), .cu:189-230 (products), .cu:477-575 (assembly),
-// written as the compact per-pixel system (Coef).
-__global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
-  const int it = work_item(w, blockIdx.x);
-  const LevelView& v = w.items[it];
-  if (v.ctl->outer_done) return;
-  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
-  if (l >= v.N) return;
+// written as the compact per-pixel system (Coef).  d0, d1, d2: du at the
+// pixel; pC, pL, pU: phi at the pixel, its left and upper neighbours (pL, pU
+// unused where the neighbour does not exist).
+__device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars& s, int l, int i, int j, double d0,
+                                         double d1, double d2, const PixDerivs& dv, double ql, double pC, double pL,
+                                         double pU) {
   const int W = v.W, H = v.H, N = v.N;
-  const int i = l / W, j = l - i * W;
   const double alpha = s.alpha, gamma = s.gamma;
-  const double d0 = (double)v.du[l], d1 = (double)v.du[N + l], d2 = (double)v.du[2 * N + l];
-  const double ql = (double)v.q[l];
-
   double sxy = 0, sxx = 0, syy = 0, stx = 0, sty = 0;
   double psi[kFeat];
   for (int k = 0; k < kFeat; k++) {
-    const size_t o = (size_t)k * N + l;
-    const double tt = (v.It[o] + v.Ix[o] * d0 + v.Iy[o] * d1) * ql;
+    const double tt = (dv.It[k] + dv.Ix[k] * d0 + dv.Iy[k] * d1) * ql;
     psi[k] = .5 / sqrt((double)sqr_f32(tt) + 1e-6) * s.color_weight;
   }
   for (int k = 0; k < kFeat; k++) {
-    const size_t o = (size_t)k * N + l;
-    sxy += psi[k] * v.Ix[o] * v.Iy[o];
-    sxx += psi[k] * v.Ix[o] * v.Ix[o];
-    syy += psi[k] * v.Iy[o] * v.Iy[o];
+    sxy += psi[k] * dv.Ix[k] * dv.Iy[k];
+    sxx += psi[k] * dv.Ix[k] * dv.Ix[k];
+    syy += psi[k] * dv.Iy[k] * dv.Iy[k];
   }
-  for (int k = 0; k < kFeat; k++) { const size_t o = (size_t)k * N + l; stx += psi[k] * v.Ix[o] * v.It[o]; }
-  for (int k = 0; k < kFeat; k++) { const size_t o = (size_t)k * N + l; sty += psi[k] * v.Iy[o] * v.It[o]; }
+  for (int k = 0; k < kFeat; k++) stx += psi[k] * dv.Ix[k] * dv.It[k];
+  for (int k = 0; k < kFeat; k++) sty += psi[k] * dv.Iy[k] * dv.It[k];
   const double ixy = sxy / kFeat, ix2 = sxx / kFeat, iy2 = syy / kFeat;
   const double itx = stx * ql * ql / kFeat, ity = sty * ql * ql / kFeat;
 
-  const double Dx = v.Dx[l], Dy = v.Dy[l], Dt = v.Dt[l];
+  const double Dx = dv.Dx, Dy = dv.Dy, Dt = dv.Dt;
   const double t2 = Dt + Dx * d0 + Dy * d1 - d2;
   const double P = .5 / sqrt((double)sqr_f32(t2) + 1e-10) * (double)v.dW[l];
   const double dxy = P * Dx * Dy, dx2 = P * Dx * Dx, dy2 = P * Dy * Dy;
   const double dtx = P * Dx * Dt, dty = P * Dy * Dt;
 
-  const double uL = laplacian(v, 0, l, i, j), vL = laplacian(v, 1, l, i, j);
-  const double wL = (double)v.sW[l] * laplacian(v, 2, l, i, j);
+  const double uL = laplacian(v, 0, l, i, j, pC, pL, pU), vL = laplacian(v, 1, l, i, j, pC, pL, pU);
+  const double wL = (double)v.sW[l] * laplacian(v, 2, l, i, j, pC, pL, pU);
 
   const float* rR = v.rW;
   const float* rD = v.rW + N;
   double wsXY = 0, wsZ = 0;
   float eRxy = 0.f, eRz = 0.f, eDxy = 0.f, eDz = 0.f;
-  if (j > 0) { const double a = v.phi[l - 1] * (double)rR[l - 1], b = a * (double)v.sW[l - 1]; wsXY += a; wsZ += b; }
-  if (i > 0) { const double a = v.phi[l - W] * (double)rD[l - W], b = a * (double)v.sW[l - W]; wsXY += a; wsZ += b; }
+  if (j > 0) { const double a = pL * (double)rR[l - 1], b = a * (double)v.sW[l - 1]; wsXY += a; wsZ += b; }
+  if (i > 0) { const double a = pU * (double)rD[l - W], b = a * (double)v.sW[l - W]; wsXY += a; wsZ += b; }
   if (j < W - 1) {
-    const double a = v.phi[l] * (double)rR[l], b = a * (double)v.sW[l];
+    const double a = pC * (double)rR[l], b = a * (double)v.sW[l];
     eRxy = (float)(alpha * a); eRz = (float)(alpha * b); wsXY += a; wsZ += b;
   }
   if (i < H - 1) {
-    const double a = v.phi[l] * (double)rD[l], b = a * (double)v.sW[l];
+    const double a = pC * (double)rD[l], b = a * (double)v.sW[l];
     eDxy = (float)(alpha * a); eDz = (float)(alpha * b); wsXY += a; wsZ += b;
   }
   const double u0 = v.u[l], u1 = v.u[N + l], u2 = v.u[2 * N + l];
@@ -208,6 +237,70 @@ __global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
   t = (float)(-(-ity - dty + alpha * -vL)); C[kRv * (size_t)N + l] = (float)((double)t + -(-gamma * u1));
   t = (float)(-(P + alpha * wsZ)); C[kBww * (size_t)N + l] = (float)((double)t + -(gamma * m));
   t = (float)(-(P * Dt + alpha * -wL)); C[kRw * (size_t)N + l] = (float)((double)t + -(-gamma * m * u2));
+}
+
+// The system of an inner iteration after the first, from the stored
+// derivatives, occlusion weights and phi (k_phi) and the current du.
+__global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
+  const int it = work_item(w, blockIdx.x);
+  const LevelView& v = w.items[it];
+  if (v.ctl->outer_done) return;
+  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
+  if (l >= v.N) return;
+  const int W = v.W, N = v.N;
+  const int i = l / W, j = l - i * W;
+  PixDerivs d;
+  for (int k = 0; k < kFeat; k++) {
+    d.Ix[k] = v.Ix[(size_t)k * N + l];
+    d.Iy[k] = v.Iy[(size_t)k * N + l];
+    d.It[k] = v.It[(size_t)k * N + l];
+  }
+  d.Dx = v.Dx[l];
+  d.Dy = v.Dy[l];
+  d.Dt = v.Dt[l];
+  const double pC = v.phi[l], pL = j > 0 ? v.phi[l - 1] : 0.0, pU = i > 0 ? v.phi[l - W] : 0.0;
+  assemble(v, s, l, i, j, (double)v.du[l], (double)v.du[N + l], (double)v.du[2 * N + l], d, (double)v.q[l], pC, pL,
+           pU);
+}
+
+// The first inner iteration of an outer iteration in one pass: derivatives,
+// occlusion weight, du = 0, phi at the pixel and at its left and upper
+// neighbours (recomputed from u: du is zero), and the system -- the values
+// k_derivs, k_phi and k_system compute, without storing the intermediates.
+// store: also store derivatives and occlusion weights for later inner
+// iterations.  Zeroes the item's check slots of this outer iteration.
+__global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, SolveScalars s, int store) {
+  const int it = work_item(w, blockIdx.x);
+  const LevelView& v = w.items[it];
+  if (v.ctl->outer_done) return;
+  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
+  const int stride = (w.start[it + 1] - w.start[it]) * kBlock;
+  for (int t = l; t < n_slots; t += stride) v.ctl->maxdiff[t] = 0u;
+  if (l >= v.N) return;
+  const int W = v.W, N = v.N;
+  const int i = l / W, j = l - i * W;
+  PixDerivs d;
+  derivs_at(v, l, i, j, d);
+  const float q = occlusion_at(v, l);
+  v.du[l] = 0.f;
+  v.du[N + l] = 0.f;
+  v.du[2 * N + l] = 0.f;
+  if (store) {
+    for (int k = 0; k < kFeat; k++) {
+      v.Ix[(size_t)k * N + l] = d.Ix[k];
+      v.Iy[(size_t)k * N + l] = d.Iy[k];
+      v.It[(size_t)k * N + l] = d.It[k];
+    }
+    v.Dx[l] = d.Dx;
+    v.Dy[l] = d.Dy;
+    v.Dt[l] = d.Dt;
+    v.q[l] = q;
+  }
+  const double pC = phi_at<true>(v, l, i, j);
+  const double pL = j > 0 ? phi_at<true>(v, l - 1, i, j - 1) : 0.0;
+  const double pU = i > 0 ? phi_at<true>(v, l - W, i - 1, j) : 0.0;
+  if (store) v.phi[l] = pC;
+  assemble(v, s, l, i, j, 0.0, 0.0, 0.0, d, (double)q, pC, pL, pU);
 }
 
 // .cu:382-470: one colour of the red-black relaxation, one thread per pixel of
@@ -349,6 +442,9 @@ void launch_phi(hipStream_t s, const Views& v, int blocks) {
 }
 void launch_system(hipStream_t s, const Views& v, int blocks, const SolveScalars& k) {
   hipLaunchKernelGGL(k_system, dim3(blocks), dim3(kBlock), 0, s, v, k);
+}
+void launch_outer_first(hipStream_t s, const Views& v, int blocks, int n_slots, const SolveScalars& k, int store) {
+  hipLaunchKernelGGL(k_outer_first, dim3(blocks), dim3(kBlock), 0, s, v, n_slots, k, store);
 }
 void launch_relax_plain(hipStream_t s, const Views& v, int blocks, int colour, int slot_base, int n_prev_checks,
                         int check_slot) {

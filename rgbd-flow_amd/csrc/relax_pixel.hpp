@@ -81,6 +81,82 @@ __device__ __forceinline__ void relax_update(const PixCoef& c, bool hl, bool hu,
   }
 }
 
+// Markstein's division (IBM J. Res. Dev. 34(1), 1990): with y = RN(1/b),
+// q0 = RN(a y), r = a - b q0 (exact by FMA), RN(q0 + r y) = RN(a/b) whenever
+// no intermediate underflows or overflows, which holds for |b| in
+// [2^-60, 2^60] and a = 0 or |a| in [2^-60, 2^60].
+__device__ __forceinline__ bool recip_div_ok(float a) {
+  const float aa = fabsf(a);
+  return a == 0.f || (aa >= 0x1p-60f && aa <= 0x1p60f);
+}
+__device__ __forceinline__ bool recip_den_ok(float b) {
+  const float ab = fabsf(b);
+  return ab >= 0x1p-60f && ab <= 0x1p60f;
+}
+__device__ __forceinline__ float recip_div(float a, float b, float y) {
+  const float q0 = a * y;
+  const float r = __builtin_fmaf(-b, q0, a);
+  return __builtin_fmaf(r, y, q0);
+}
+
+// relax_update with every neighbour term evaluated: a missing neighbour
+// carries a zero coefficient and a zero value, so its term subtracts +0 from
+// a sum that is never -0 (it starts at +0 and only exact cancellation makes
+// zero, which rounds to +0) and leaves it unchanged; the result is the same
+// bits as relax_update's.  L, U, R, D, o: neighbour and old values (x, y, z
+// in .x .y .z).  y3: 1/Buu, 1/Bvv, 1/Bww (IEEE divisions); den_ok: all three
+// diagonals are in the reciprocal division's domain.
+__device__ __forceinline__ void relax_update_dense(const PixCoef& c, const float (&y3)[3], bool den_ok,
+                                                   const float4& L, const float4& U, const float4& R,
+                                                   const float4& D, const float4& o, float xn[3]) {
+  float nv[3];
+  const float xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z}, xd[3] = {D.x, D.y, D.z};
+  const float xo[3] = {o.x, o.y, o.z};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float el = k < 2 ? c.lxy : c.lz;
+    const float eu = k < 2 ? c.uxy : c.uz;
+    const float er = k < 2 ? c.rxy : c.rz;
+    const float ed = k < 2 ? c.dxy : c.dz;
+    float a = 0.f;
+    a -= el * xl[k];
+    a -= eu * xu[k];
+    a -= er * xr[k];
+    a -= ed * xd[k];
+    if (k == 0) {
+      a += c.ru;
+      a -= c.Buv * xo[1];
+      a -= c.Buw * xo[2];
+    } else if (k == 1) {
+      a += c.rv;
+      a -= c.Buv * xo[0];
+      a -= c.Bvw * xo[2];
+    } else {
+      a += c.rw;
+      a -= c.Buw * xo[0];
+      a -= c.Bvw * xo[1];
+    }
+    nv[k] = a;
+  }
+  if (__builtin_expect(den_ok && recip_div_ok(nv[0]) && recip_div_ok(nv[1]) && recip_div_ok(nv[2]), 1)) {
+    nv[0] = recip_div(nv[0], c.Buu, y3[0]);
+    nv[1] = recip_div(nv[1], c.Bvv, y3[1]);
+    nv[2] = recip_div(nv[2], c.Bww, y3[2]);
+  } else {
+    nv[0] /= c.Buu;
+    nv[1] /= c.Bvv;
+    nv[2] /= c.Bww;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float omega = .7f;
+    float r = omega * nv[k] + (1 - omega) * xo[k];
+    if (r < xo[k] - .3f) r = xo[k] - .3f;
+    else if (r > xo[k] + .3f) r = xo[k] + .3f;
+    xn[k] = r;
+  }
+}
+
 // The relaxation has stopped for this (outer, inner) if an earlier check
 // iteration saw max|change| < 1e-5 or >= 0.3 (.cu:914-922).
 __device__ __forceinline__ bool relax_stopped(const SolveCtl* ctl, int slot_base, int n_prev_checks) {

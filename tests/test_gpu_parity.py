@@ -245,3 +245,21 @@ def test_error_paths(R):
         with pytest.raises(R.RgbdFlowError) as e:
             c.pair_run(p)
         assert e.value.code == R.ERR_ARG
+
+
+@pytest.mark.parametrize("n_inner", [0, 2, 3])
+@pytest.mark.parametrize("variant", ["plain", "tiled"])
+def test_multiscale_inner_iterations(ctx, R, n_inner, variant):
+    """nInner other than 1: the first inner iteration's fused pass stores the
+    derivatives for the later ones (and nInner = 0 only re-warps)."""
+    O = oracle()
+    W, H = 128, 96
+    pair, im1, d1, im2, d2, sigma = make_case(W, H, 16, 4)
+    hpb, lv = hpb_for(pair, W, H, 0.5, 32)
+    kw = dict(n_outer=3, n_inner=n_inner, n_sor=30, min_width=32)
+    ua, va, wa, sa = O.multiscale(im1, d1, im2, d2, sigma, hpb, O.make_params(W, H, **kw))
+    pg = R.make_params(W, H, relax_variant=R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN, **kw)
+    ub, vb, wb, sb = ctx.multiscale(pg, im1, d1, im2, d2, sigma, hpb)
+    L = sa.n_levels
+    assert [sb.sweeps_exec[k] for k in range(L)] == [sa.sweeps_exec[k] for k in range(L)]
+    assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
