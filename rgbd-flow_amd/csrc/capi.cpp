@@ -39,7 +39,9 @@ struct LevelShape {
 // pixels uses the first N entries of every plane.
 struct Work {
   double *I1, *I2, *Iw, *Dw, *u[2], *Ix, *Iy, *It, *Dx, *Dy, *Dt, *phi, *partials;
-  float *du, *du_alt, *dW, *sW, *mW, *rW, *cW, *q, *coef;
+  float *dW, *sW, *mW, *rW, *cW, *q;
+  float4 *du, *du_alt;  // [N] (du, dv, dw, unused)
+  float4* coef;         // [N][kRec] system records
   SolveCtl* ctl;  // kMaxLevels entries, one per level
 };
 
@@ -273,15 +275,15 @@ int get_work(rgbdflow_ctx* c, const std::string& pre, size_t N, Work& w) {
   TRY(get(c, pre + "Dt", N, &w.Dt));
   TRY(get(c, pre + "phi", N, &w.phi));
   TRY(get(c, pre + "partials", (size_t)pix_blocks((int)N) + 16, &w.partials));
-  TRY(get(c, pre + "du", 3 * N, &w.du));
-  TRY(get(c, pre + "du_alt", 3 * N, &w.du_alt));
+  TRY(get(c, pre + "du", N, &w.du));
+  TRY(get(c, pre + "du_alt", N, &w.du_alt));
   TRY(get(c, pre + "dW", N, &w.dW));
   TRY(get(c, pre + "sW", N, &w.sW));
   TRY(get(c, pre + "mW", N, &w.mW));
   TRY(get(c, pre + "rW", 2 * N, &w.rW));
   TRY(get(c, pre + "cW", N, &w.cW));
   TRY(get(c, pre + "q", N, &w.q));
-  TRY(get(c, pre + "coef", (size_t)kCoef * N, &w.coef));
+  TRY(get(c, pre + "coef", (size_t)kRec * N, &w.coef));
   TRY(get(c, pre + "ctl", kMaxLevels, &w.ctl));
   return RGBDFLOW_OK;
 }
@@ -1120,10 +1122,12 @@ int rgbdflow_level_system(rgbdflow_ctx* c, int W, int H, float focal_level, cons
   launch_derivs(s, sp.vpix, sp.pix, 0);
   launch_phi(s, sp.vpix, sp.pix);
   launch_system(s, sp.vpix, sp.pix, k);
-  std::vector<float> host_c((size_t)kCoef * N);
-  TRY(download(c, host_c.data(), wk.coef, host_c.size() * sizeof(float)));
+  std::vector<float> rec((size_t)kRec * 4 * N);
+  TRY(download(c, rec.data(), wk.coef, rec.size() * sizeof(float)));
   TRY(sync(c));
-  for (size_t i = 0; i < host_c.size(); i++) coef13[i] = host_c[i];
+  // record order (eRxy, eRz, eDxy, eDz, Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw, ...) is the plane order
+  for (size_t l = 0; l < N; l++)
+    for (int k = 0; k < kCoef; k++) coef13[(size_t)k * N + l] = rec[l * 16 + k];
   return RGBDFLOW_OK;
 }
 
@@ -1136,10 +1140,17 @@ int rgbdflow_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, int n_it
   const size_t N = (size_t)W * H;
   Work wk;
   TRY(get_work(c, "d.", N, wk));
-  std::vector<float> hc((size_t)kCoef * N), hx(3 * N);
-  for (size_t i = 0; i < hc.size(); i++) hc[i] = (float)coef13[i];
-  for (size_t i = 0; i < hx.size(); i++) hx[i] = (float)x[i];
-  TRY(upload(c, wk.coef, hc.data(), hc.size() * 4));
+  // planes -> records (the reciprocals are IEEE single-precision divisions, as on the device)
+  std::vector<float> rec((size_t)kRec * 4 * N), hx(4 * N, 0.f);
+  for (size_t l = 0; l < N; l++) {
+    float* r = rec.data() + l * 16;
+    for (int k = 0; k < kCoef; k++) r[k] = (float)coef13[(size_t)k * N + l];
+    r[13] = 1.0f / r[kBuu];
+    r[14] = 1.0f / r[kBvv];
+    r[15] = 1.0f / r[kBww];
+    for (int k = 0; k < 3; k++) hx[l * 4 + k] = (float)x[(size_t)k * N + l];
+  }
+  TRY(upload(c, wk.coef, rec.data(), rec.size() * 4));
   TRY(upload(c, wk.du, hx.data(), hx.size() * 4));
   std::vector<LevelView> items{make_view(wk, W, H, nullptr, nullptr, wk.u[0], 0)};
   StepPlan sp;
@@ -1151,7 +1162,8 @@ int rgbdflow_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, int n_it
   enqueue_relax(c, sp, k, 0);
   TRY(download(c, hx.data(), wk.du, hx.size() * 4));
   TRY(sync(c));
-  for (size_t i = 0; i < hx.size(); i++) x[i] = hx[i];
+  for (size_t l = 0; l < N; l++)
+    for (int k = 0; k < 3; k++) x[(size_t)k * N + l] = hx[l * 4 + k];
   return RGBDFLOW_OK;
 }
 
@@ -1384,11 +1396,11 @@ namespace {
 // for timing relaxation schedules.
 int timing_items(rgbdflow_ctx* c, int W, int H, int batch, std::vector<LevelView>& items) {
   const size_t N = (size_t)W * H;
-  float *coef, *du, *du_alt;
+  float4 *coef, *du, *du_alt;
   SolveCtl* ctl;
-  TRY(get(c, "at.coef", (size_t)batch * kCoef * N, &coef));
-  TRY(get(c, "at.du", (size_t)batch * 3 * N, &du));
-  TRY(get(c, "at.du_alt", (size_t)batch * 3 * N, &du_alt));
+  TRY(get(c, "at.coef", (size_t)batch * kRec * N, &coef));
+  TRY(get(c, "at.du", (size_t)batch * N, &du));
+  TRY(get(c, "at.du_alt", (size_t)batch * N, &du_alt));
   TRY(get(c, "at.ctl", batch, &ctl));
   items.assign(batch, LevelView{});
   for (int b = 0; b < batch; b++) {
@@ -1397,9 +1409,9 @@ int timing_items(rgbdflow_ctx* c, int W, int H, int batch, std::vector<LevelView
     v.W = W;
     v.H = H;
     v.N = (int)N;
-    v.coef = coef + (size_t)b * kCoef * N;
-    v.du = du + (size_t)b * 3 * N;
-    v.du_alt = du_alt + (size_t)b * 3 * N;
+    v.coef = coef + (size_t)b * kRec * N;
+    v.du = du + (size_t)b * N;
+    v.du_alt = du_alt + (size_t)b * N;
     v.ctl = ctl + b;
     launch_fill_test_system(c->stream, v.coef, v.du, W, H);
   }

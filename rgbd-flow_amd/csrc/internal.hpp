@@ -22,10 +22,18 @@ constexpr int kMaxChecks = 4096;    // check slots per outer iteration
 constexpr int kMaxOuter = 1024;
 constexpr int kBlock = 256;         // threads of the per-pixel kernels
 
-// Plane indices of the compact system of one pixel p:
-//   right/down edge weights alpha*phi_p*r_p (xy rows) and alpha*phi_p*r_p*s_p (z row),
-//   the symmetric 3x3 block B (diagonal includes gamma terms) and the rhs.
+// The compact system of one pixel p, in the order of the 13-plane export
+// (rgbdflow_level_system): right/down edge weights alpha*phi_p*r_p (xy rows)
+// and alpha*phi_p*r_p*s_p (z row), the symmetric 3x3 block B (diagonal
+// includes gamma terms) and the rhs.
 enum Coef { kERxy = 0, kERz, kEDxy, kEDz, kBuu, kBvv, kBww, kBuv, kBuw, kBvw, kRu, kRv, kRw };
+
+// In HBM the system is one 64-byte record per pixel (kRec float4), so the
+// relaxation loads a pixel with four 16-byte loads:
+//   rec[0] = (eRxy, eRz, eDxy, eDz)   rec[1] = (Buu, Bvv, Bww, Buv)
+//   rec[2] = (Buw, Bvw, ru, rv)       rec[3] = (rw, 1/Buu, 1/Bvv, 1/Bww)
+// The reciprocals are IEEE single-precision divisions, made once per system.
+constexpr int kRec = 4;
 
 // Device control block of one level solve.  Kernels read it at entry to skip
 // work after an early exit, so the host never waits inside a level.
@@ -59,8 +67,8 @@ struct LevelView {
   const double* D2;     // [N]
   double* Dw;           // [N] frame-2 depth warped by u, minus w
   double* u;            // [3][N] flow (u px, v px, w m)
-  float* du;            // [3][N] increment solved by the relaxation
-  float* du_alt;        // [3][N] second buffer of the tiled relaxation (reads one, writes the other)
+  float4* du;           // [N] increment (du, dv, dw, unused) solved by the relaxation
+  float4* du_alt;       // [N] second buffer of the tiled relaxation (reads one, writes the other)
   float* dW;            // [N] depth-data weights (delta)
   float* sW;            // [N] depth-smoothness weights
   float* mW;            // [N] depth-magnitude weights
@@ -74,7 +82,7 @@ struct LevelView {
   double* Dy;           // [N]
   double* Dt;           // [N]
   double* phi;          // [N]
-  float* coef;          // [13][N]
+  float4* coef;         // [N][kRec] system records
   double* partials;     // per-block partial sums of the outer statistic
   SolveCtl* ctl;
   int allow_early;      // outer early exit allowed on this level
@@ -186,7 +194,7 @@ inline int tiled_slots(const TileSched& t, int W, int H, int T) {
 void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int slots, int flip, int n_iter,
                         int slot_base, int n_prev_checks, int check_slot);
 void launch_copy_du(hipStream_t s, const Views& v, int blocks);
-void launch_fill_test_system(hipStream_t s, float* coef, float* du, int W, int H);
+void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H);
 
 // kernels_pyramid.hip
 void launch_deinterleave(hipStream_t s, const double* src, int N, int C, double* dst);

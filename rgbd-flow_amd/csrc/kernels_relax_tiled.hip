@@ -26,17 +26,29 @@ namespace rgbdflow {
 
 namespace {
 
+// The values a thread keeps in registers for one of its pixels: the 3x3
+// block, the rhs, the diagonal's reciprocals and the pixel's own right/down
+// edge weights.  The left/up edge weights are the neighbours' right/down
+// ones, read from LDS.
+struct PixRegs {
+  float Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw;
+  float yu, yv, yw;          // 1/Buu, 1/Bvv, 1/Bww
+  float rxy, rz, dxy, dz;    // own right/down edge weights
+};
+
 // One colour pass over the pairs a thread owns; colour C is a compile-time
 // constant so every per-pixel array is indexed statically (kept in VGPRs).
-// X4: two colour planes of Si float4 slots (x, y, z, unused) over the region
-// padded by one pair on every side; pads and absent pixels hold zeros.
+// LDS: per colour plane of Si slots over the region padded by one pair on
+// every side, XA = (x, y, z, eRxy) and XB = (eRz, eDxy, eDz, 0); pads and
+// absent pixels hold zeros.
 template <int P, int C>
-__device__ __forceinline__ void tiled_pass(float4* X4, int Si, int PRp, const PixCoef (&cf)[P][2],
-                                           const float (&yr)[P][2][3], const int (&qi)[P], const int (&par)[P],
-                                           const unsigned (&fl)[P][2], bool last, float& change) {
+__device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int PRp, const PixRegs (&pr)[P][2],
+                                           const int (&qi)[P], const int (&par)[P], const unsigned (&fl)[P][2],
+                                           bool last, float& change) {
   constexpr int O = 1 - C;  // the colour read
-  const float4* Xo = X4 + O * Si;
-  float4* Xc = X4 + C * Si;
+  const float4* Ao = XA + O * Si;
+  const float4* Bo = XB + O * Si;
+  float4* Ac = XA + C * Si;
 #pragma unroll
   for (int s = 0; s < P; s++) {
     const unsigned f = fl[s][C];
@@ -47,11 +59,31 @@ __device__ __forceinline__ void tiled_pass(float4* X4, int Si, int PRp, const Pi
     // black reads (k-e, k+1-e); the rows above/below read pair k
     const int il = C == 0 ? idx - 1 + e : idx - e;
     const int ir = C == 0 ? idx + e : idx + 1 - e;
-    const float4 o = Xc[idx];
-    const float4 L = Xo[il], R = Xo[ir], U = Xo[idx - PRp], D = Xo[idx + PRp];
+    const float4 o = Ac[idx];
+    const float4 L = Ao[il], LB = Bo[il], U = Ao[idx - PRp], UB = Bo[idx - PRp], R = Ao[ir], D = Ao[idx + PRp];
+    const PixRegs& r = pr[s][C];
+    PixCoef c;
+    c.lxy = L.w;
+    c.lz = LB.x;
+    c.uxy = UB.y;
+    c.uz = UB.z;
+    c.rxy = r.rxy;
+    c.rz = r.rz;
+    c.dxy = r.dxy;
+    c.dz = r.dz;
+    c.Buu = r.Buu;
+    c.Bvv = r.Bvv;
+    c.Bww = r.Bww;
+    c.Buv = r.Buv;
+    c.Buw = r.Buw;
+    c.Bvw = r.Bvw;
+    c.ru = r.ru;
+    c.rv = r.rv;
+    c.rw = r.rw;
+    const float y3[3] = {r.yu, r.yv, r.yw};
     float xn[3];
-    relax_update_dense(cf[s][C], yr[s][C], f & 64u, L, U, R, D, o, xn);
-    Xc[idx] = make_float4(xn[0], xn[1], xn[2], 0.f);
+    relax_update_dense(c, y3, f & 64u, L, U, R, D, o, xn);
+    Ac[idx] = make_float4(xn[0], xn[1], xn[2], r.rxy);
     if (last && (f & 32u))
       change = fmaxf(change, fmaxf(fmaxf(fabsf(xn[0] - o.x), fabsf(xn[1] - o.y)), fabsf(xn[2] - o.z)));
   }
@@ -65,16 +97,16 @@ __device__ __forceinline__ void tiled_pass(float4* X4, int Si, int PRp, const Pi
 // is, and k_copy_du moves it to du at the end.  t: the tile's index in the
 // launch's work-list.
 template <int NT, int P>
-__device__ __forceinline__ void relax_tile(float4* X4, const Views& wl, int t, int flip, int n_iter, int slot_base,
+__device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, int flip, int n_iter, int slot_base,
                                            int n_prev_checks, int check_slot) {
   const int item = work_item(wl, t);
   const LevelView& v = wl.items[item];
   if (v.ctl->outer_done) return;
   if (relax_stopped(v.ctl, slot_base, n_prev_checks)) return;
   const TileSched ts = v.ts;
-  const float* __restrict__ src = flip ? v.du_alt : v.du;
-  float* __restrict__ dst = flip ? v.du : v.du_alt;
-  const int W = v.W, H = v.H, N = v.N;
+  const float4* __restrict__ src = flip ? v.du_alt : v.du;
+  float4* __restrict__ dst = flip ? v.du : v.du_alt;
+  const int W = v.W, H = v.H;
   const int tb = t - wl.start[item];
   const int bx = tb % ts.tiles_x, by = tb / ts.tiles_x;
   const int ox = bx * ts.TW, oy = by * ts.TH;
@@ -86,21 +118,24 @@ __device__ __forceinline__ void relax_tile(float4* X4, const Views& wl, int t, i
   const int np = PR * RH;
   const int PRp = PR + 2;
   const int Si = PRp * (RH + 2);  // slots per colour plane
+  float4* XB = XA + 2 * Si;
 
-  // zero the pad ring of both planes (region slots are written by their owners)
-  for (int t = threadIdx.x; t < 2 * (2 * PRp + 2 * RH); t += NT) {
-    const int c = t >= 2 * PRp + 2 * RH;
-    const int r = t - c * (2 * PRp + 2 * RH);
+  // zero the pad ring of both planes of XA and XB (region slots are written by their owners)
+  const int ring = 2 * PRp + 2 * RH;
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int t2 = threadIdx.x; t2 < 2 * ring; t2 += NT) {
+    const int c = t2 >= ring;
+    const int r = t2 - c * ring;
     int slot;
-    if (r < PRp) slot = r;                                   // top pad row
-    else if (r < 2 * PRp) slot = (RH + 1) * PRp + r - PRp;   // bottom pad row
-    else if (r < 2 * PRp + RH) slot = (r - 2 * PRp + 1) * PRp;              // left pads
-    else slot = (r - 2 * PRp - RH + 1) * PRp + PRp - 1;      // right pads
-    X4[c * Si + slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (r < PRp) slot = r;                                          // top pad row
+    else if (r < 2 * PRp) slot = (RH + 1) * PRp + r - PRp;          // bottom pad row
+    else if (r < 2 * PRp + RH) slot = (r - 2 * PRp + 1) * PRp;      // left pads
+    else slot = (r - 2 * PRp - RH + 1) * PRp + PRp - 1;             // right pads
+    XA[c * Si + slot] = zero;
+    XB[c * Si + slot] = zero;
   }
 
-  PixCoef cf[P][2];
-  float yr[P][2][3];
+  PixRegs pr[P][2];
   int qi[P], par[P];
   unsigned fl[P][2];  // bits: 16 pixel valid; 32 interior; 64 diagonals in the reciprocal division's domain
 #pragma unroll
@@ -119,23 +154,37 @@ __device__ __forceinline__ void relax_tile(float4* X4, const Views& wl, int t, i
       if (valid) f |= 16u;
       if (valid && gc >= ox && gc < ex && gy >= oy && gy < ey) f |= 32u;
       const int l = gy * W + gc;
+      PixRegs r{};
       if (valid) {
-        const bool hl = gc > 0, hu = gy > 0, hr = gc < W - 1, hd = gy < H - 1;
-        PixCoef k = load_coef(v.coef, N, W, l, hl, hu);
+        const float4* rec = v.coef + (size_t)l * kRec;
+        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
         // a missing neighbour's coefficient is exactly zero (its value is a zero pad)
-        if (!hr) k.rxy = k.rz = 0.f;
-        if (!hd) k.dxy = k.dz = 0.f;
-        cf[s][c] = k;
-        yr[s][c][0] = 1.0f / k.Buu;
-        yr[s][c][1] = 1.0f / k.Bvv;
-        yr[s][c][2] = 1.0f / k.Bww;
-        if (recip_den_ok(k.Buu) && recip_den_ok(k.Bvv) && recip_den_ok(k.Bww)) f |= 64u;
-        X4[c * Si + qi[s]] = make_float4(src[l], src[(size_t)N + l], src[2 * (size_t)N + l], 0.f);
-      } else {
-        cf[s][c] = PixCoef{};
-        yr[s][c][0] = yr[s][c][1] = yr[s][c][2] = 0.f;
-        if (q < np) X4[c * Si + qi[s]] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool hr = gc < W - 1, hd = gy < H - 1;
+        r.rxy = hr ? r0.x : 0.f;
+        r.rz = hr ? r0.y : 0.f;
+        r.dxy = hd ? r0.z : 0.f;
+        r.dz = hd ? r0.w : 0.f;
+        r.Buu = r1.x;
+        r.Bvv = r1.y;
+        r.Bww = r1.z;
+        r.Buv = r1.w;
+        r.Buw = r2.x;
+        r.Bvw = r2.y;
+        r.ru = r2.z;
+        r.rv = r2.w;
+        r.rw = r3.x;
+        r.yu = r3.y;
+        r.yv = r3.z;
+        r.yw = r3.w;
+        if (recip_den_ok(r.Buu) && recip_den_ok(r.Bvv) && recip_den_ok(r.Bww)) f |= 64u;
+        const float4 x = src[l];
+        XA[c * Si + qi[s]] = make_float4(x.x, x.y, x.z, r.rxy);
+        XB[c * Si + qi[s]] = make_float4(r.rz, r.dxy, r.dz, 0.f);
+      } else if (q < np) {
+        XA[c * Si + qi[s]] = zero;
+        XB[c * Si + qi[s]] = zero;
       }
+      pr[s][c] = r;
       fl[s][c] = f;
     }
   }
@@ -144,9 +193,9 @@ __device__ __forceinline__ void relax_tile(float4* X4, const Views& wl, int t, i
   for (int it = 0; it < n_iter; it++) {
     const bool last = it == n_iter - 1;
     __syncthreads();
-    tiled_pass<P, 0>(X4, Si, PRp, cf, yr, qi, par, fl, last, change);
+    tiled_pass<P, 0>(XA, XB, Si, PRp, pr, qi, par, fl, last, change);
     __syncthreads();
-    tiled_pass<P, 1>(X4, Si, PRp, cf, yr, qi, par, fl, last, change);
+    tiled_pass<P, 1>(XA, XB, Si, PRp, pr, qi, par, fl, last, change);
   }
   __syncthreads();
 #pragma unroll
@@ -157,11 +206,8 @@ __device__ __forceinline__ void relax_tile(float4* X4, const Views& wl, int t, i
     for (int c = 0; c < 2; c++) {
       if (!(fl[s][c] & 32u)) continue;
       const int gc = rx0 + 2 * kk + (c == 0 ? par[s] : 1 - par[s]);
-      const int l = gy * W + gc;
-      const float4 x = X4[c * Si + qi[s]];
-      dst[l] = x.x;
-      dst[(size_t)N + l] = x.y;
-      dst[2 * (size_t)N + l] = x.z;
+      const float4 x = XA[c * Si + qi[s]];
+      dst[gy * W + gc] = make_float4(x.x, x.y, x.z, 0.f);
     }
   }
   if (check_slot >= 0) {
@@ -178,9 +224,9 @@ __device__ __forceinline__ void relax_tile(float4* X4, const Views& wl, int t, i
 // the 8 XCDs round-robin by id, and the grid is a multiple of 8, so workgroup
 // b always runs on XCD b % 8; each XCD gets a contiguous run of tiles, whose
 // shared halos then hit in that XCD's L2.
-template <int NT, int P>
-__global__ void __launch_bounds__(NT) k_relax_tiled(Views wl, int total, int flip, int n_iter, int slot_base,
-                                                    int n_prev_checks, int check_slot) {
+template <int NT, int P, int MB>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 256 * MB)))
+k_relax_tiled(Views wl, int total, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot) {
   extern __shared__ float4 X4[];
   const int per_xcd = (total + 7) / 8;
   for (int b = blockIdx.x; b < 8 * per_xcd; b += gridDim.x) {
@@ -198,7 +244,7 @@ __global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
   if (v.ctl->outer_done || !v.ctl->du_in_alt) return;
   const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
-  for (int k = 0; k < 3; k++) v.du[(size_t)k * v.N + l] = v.du_alt[(size_t)k * v.N + l];
+  v.du[l] = v.du_alt[l];
 }
 
 struct TiledConfig {
@@ -206,9 +252,12 @@ struct TiledConfig {
   void (*fn)(Views, int, int, int, int, int, int);
 };
 
-#define RGBDFLOW_TILED(NT, P) TiledConfig{NT, P, k_relax_tiled<NT, P>}
-const TiledConfig kConfigs[] = {RGBDFLOW_TILED(256, 1), RGBDFLOW_TILED(512, 1), RGBDFLOW_TILED(1024, 1),
-                                RGBDFLOW_TILED(256, 2), RGBDFLOW_TILED(512, 2)};
+// (threads, pairs per thread, workgroups per CU the registers must allow)
+#define RGBDFLOW_TILED(NT, P, MB) TiledConfig{NT, P, k_relax_tiled<NT, P, MB>}
+const TiledConfig kConfigs[] = {RGBDFLOW_TILED(256, 1, 1), RGBDFLOW_TILED(512, 1, 1), RGBDFLOW_TILED(1024, 1, 1),
+                                RGBDFLOW_TILED(256, 2, 1), RGBDFLOW_TILED(512, 2, 1), RGBDFLOW_TILED(1024, 1, 2),
+                                RGBDFLOW_TILED(512, 1, 4), RGBDFLOW_TILED(512, 2, 2), RGBDFLOW_TILED(1024, 2, 1),
+                                RGBDFLOW_TILED(256, 2, 4)};
 #undef RGBDFLOW_TILED
 
 }  // namespace
@@ -219,37 +268,25 @@ void tiled_config_shape(int idx, int* nt, int* p) {
   *p = kConfigs[idx].p;
 }
 
-size_t tiled_lds_bytes(int slots) { return (size_t)2 * slots * sizeof(float4); }
+size_t tiled_lds_bytes(int slots) { return (size_t)4 * slots * sizeof(float4); }
 
-__global__ void k_copy_f32(const float* src, size_t n, float* dst) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
-}
 
 // A well-conditioned synthetic system for timing schedules: unit-scale
 // diagonal, weak couplings, small rhs; du = 0.
-__global__ void k_fill_test_system(float* coef, float* du, int W, int H) {
+__global__ void k_fill_test_system(float4* coef, float4* du, int W, int H) {
   const int N = W * H;
   for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < N; l += gridDim.x * blockDim.x) {
     const int j = l % W, i = l / W;
-    coef[kERxy * (size_t)N + l] = j < W - 1 ? 0.1f : 0.f;
-    coef[kERz * (size_t)N + l] = j < W - 1 ? 0.1f : 0.f;
-    coef[kEDxy * (size_t)N + l] = i < H - 1 ? 0.1f : 0.f;
-    coef[kEDz * (size_t)N + l] = i < H - 1 ? 0.1f : 0.f;
-    coef[kBuu * (size_t)N + l] = -1.f;
-    coef[kBvv * (size_t)N + l] = -1.f;
-    coef[kBww * (size_t)N + l] = -1.f;
-    coef[kBuv * (size_t)N + l] = 0.01f;
-    coef[kBuw * (size_t)N + l] = 0.01f;
-    coef[kBvw * (size_t)N + l] = 0.01f;
-    coef[kRu * (size_t)N + l] = 1e-3f * (l % 7);
-    coef[kRv * (size_t)N + l] = 1e-3f * (l % 5);
-    coef[kRw * (size_t)N + l] = 1e-3f * (l % 3);
-    du[l] = du[N + l] = du[2 * N + l] = 0.f;
+    const float er = j < W - 1 ? 0.1f : 0.f, ed = i < H - 1 ? 0.1f : 0.f;
+    coef[(size_t)l * kRec + 0] = make_float4(er, er, ed, ed);
+    coef[(size_t)l * kRec + 1] = make_float4(-1.f, -1.f, -1.f, 0.01f);
+    coef[(size_t)l * kRec + 2] = make_float4(0.01f, 0.01f, 1e-3f * (l % 7), 1e-3f * (l % 5));
+    coef[(size_t)l * kRec + 3] = make_float4(1e-3f * (l % 3), -1.f, -1.f, -1.f);
+    du[l] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
-void launch_fill_test_system(hipStream_t s, float* coef, float* du, int W, int H) {
+void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H) {
   int nb = (int)std::min<size_t>(((size_t)W * H + 255) / 256, 4096);
   hipLaunchKernelGGL(k_fill_test_system, dim3(nb), dim3(256), 0, s, coef, du, W, H);
 }

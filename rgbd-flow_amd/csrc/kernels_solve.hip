@@ -106,11 +106,11 @@ __global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
   v.Dx[l] = d.Dx;
   v.Dy[l] = d.Dy;
   v.Dt[l] = d.Dt;
-  v.du[l] = 0.f;
-  v.du[N + l] = 0.f;
-  v.du[2 * N + l] = 0.f;
+  v.du[l] = make_float4(0.f, 0.f, 0.f, 0.f);
   v.q[l] = occlusion_at(v, l);
 }
+
+__device__ __forceinline__ float du_comp(const float4& d, int c) { return c == 0 ? d.x : (c == 1 ? d.y : d.z); }
 
 // .cu:156-182: forward differences of u + du (zero on the last row/column).
 // zero_du: du is known to be zero (first inner iteration), the same sums
@@ -118,10 +118,9 @@ __global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
 template <bool zero_du>
 __device__ __forceinline__ void flow_grad(const LevelView& v, int c, int l, int i, int j, double& gx, double& gy) {
   const double* uc = v.u + (size_t)c * v.N;
-  const float* dc = v.du + (size_t)c * v.N;
-  const double dl = zero_du ? 0.0 : (double)dc[l];
-  const double dd = zero_du || i == v.H - 1 ? 0.0 : (double)dc[l + v.W];
-  const double dr = zero_du || j == v.W - 1 ? 0.0 : (double)dc[l + 1];
+  const double dl = zero_du ? 0.0 : (double)du_comp(v.du[l], c);
+  const double dd = zero_du || i == v.H - 1 ? 0.0 : (double)du_comp(v.du[l + v.W], c);
+  const double dr = zero_du || j == v.W - 1 ? 0.0 : (double)du_comp(v.du[l + 1], c);
   gy = (i == v.H - 1) ? 0.0 : (uc[l + v.W] + dd - uc[l] - dl);
   gx = (j == v.W - 1) ? 0.0 : (uc[l + 1] + dr - uc[l] - dl);
 }
@@ -223,20 +222,26 @@ __device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars&
   const double u0 = v.u[l], u1 = v.u[N + l], u2 = v.u[2 * N + l];
   const double m = (double)v.mW[l];
   float t;
-  float* C = v.coef;
-  C[kERxy * (size_t)N + l] = eRxy;
-  C[kERz * (size_t)N + l] = eRz;
-  C[kEDxy * (size_t)N + l] = eDxy;
-  C[kEDz * (size_t)N + l] = eDz;
-  t = (float)(-(ix2 + dx2 + alpha * wsXY)); C[kBuu * (size_t)N + l] = (float)((double)t + -(gamma));
-  C[kBuv * (size_t)N + l] = (float)(-ixy - dxy);
-  C[kBuw * (size_t)N + l] = (float)(P * Dx);
-  t = (float)(-(-itx - dtx + alpha * -uL)); C[kRu * (size_t)N + l] = (float)((double)t + -(-gamma * u0));
-  t = (float)(-(iy2 + dy2 + alpha * wsXY)); C[kBvv * (size_t)N + l] = (float)((double)t + -(gamma));
-  C[kBvw * (size_t)N + l] = (float)(P * Dy);
-  t = (float)(-(-ity - dty + alpha * -vL)); C[kRv * (size_t)N + l] = (float)((double)t + -(-gamma * u1));
-  t = (float)(-(P + alpha * wsZ)); C[kBww * (size_t)N + l] = (float)((double)t + -(gamma * m));
-  t = (float)(-(P * Dt + alpha * -wL)); C[kRw * (size_t)N + l] = (float)((double)t + -(-gamma * m * u2));
+  t = (float)(-(ix2 + dx2 + alpha * wsXY));
+  const float Buu = (float)((double)t + -(gamma));
+  const float Buv = (float)(-ixy - dxy);
+  const float Buw = (float)(P * Dx);
+  t = (float)(-(-itx - dtx + alpha * -uL));
+  const float ru = (float)((double)t + -(-gamma * u0));
+  t = (float)(-(iy2 + dy2 + alpha * wsXY));
+  const float Bvv = (float)((double)t + -(gamma));
+  const float Bvw = (float)(P * Dy);
+  t = (float)(-(-ity - dty + alpha * -vL));
+  const float rv = (float)((double)t + -(-gamma * u1));
+  t = (float)(-(P + alpha * wsZ));
+  const float Bww = (float)((double)t + -(gamma * m));
+  t = (float)(-(P * Dt + alpha * -wL));
+  const float rw = (float)((double)t + -(-gamma * m * u2));
+  float4* C = v.coef + (size_t)l * kRec;
+  C[0] = make_float4(eRxy, eRz, eDxy, eDz);
+  C[1] = make_float4(Buu, Bvv, Bww, Buv);
+  C[2] = make_float4(Buw, Bvw, ru, rv);
+  C[3] = make_float4(rw, 1.0f / Buu, 1.0f / Bvv, 1.0f / Bww);
 }
 
 // The system of an inner iteration after the first, from the stored
@@ -259,8 +264,8 @@ __global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
   d.Dy = v.Dy[l];
   d.Dt = v.Dt[l];
   const double pC = v.phi[l], pL = j > 0 ? v.phi[l - 1] : 0.0, pU = i > 0 ? v.phi[l - W] : 0.0;
-  assemble(v, s, l, i, j, (double)v.du[l], (double)v.du[N + l], (double)v.du[2 * N + l], d, (double)v.q[l], pC, pL,
-           pU);
+  const float4 du = v.du[l];
+  assemble(v, s, l, i, j, (double)du.x, (double)du.y, (double)du.z, d, (double)v.q[l], pC, pL, pU);
 }
 
 // The first inner iteration of an outer iteration in one pass: derivatives,
@@ -282,9 +287,7 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
   PixDerivs d;
   derivs_at(v, l, i, j, d);
   const float q = occlusion_at(v, l);
-  v.du[l] = 0.f;
-  v.du[N + l] = 0.f;
-  v.du[2 * N + l] = 0.f;
+  v.du[l] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (store) {
     for (int k = 0; k < kFeat; k++) {
       v.Ix[(size_t)k * N + l] = d.Ix[k];
@@ -323,22 +326,16 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
   if (i < H && j < W) {
     const int l = i * W + j;
     const bool hl = j > 0, hu = i > 0, hr = j < W - 1, hd = i < H - 1;
-    const PixCoef c = load_coef(v.coef, N, W, l, hl, hu);
-    float* x = v.du;
-    float xo[3], xl[3], xu[3], xr[3], xd[3], xn[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const float* xk = x + (size_t)k * N;
-      xo[k] = xk[l];
-      xl[k] = hl ? xk[l - 1] : 0.f;
-      xu[k] = hu ? xk[l - W] : 0.f;
-      xr[k] = hr ? xk[l + 1] : 0.f;
-      xd[k] = hd ? xk[l + W] : 0.f;
-    }
+    const PixCoef c = load_coef(v.coef, W, l, hl, hu);
+    float4* x = v.du;
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 o = x[l], L = hl ? x[l - 1] : zero, U = hu ? x[l - W] : zero, R = hr ? x[l + 1] : zero,
+                 D = hd ? x[l + W] : zero;
+    const float xo[3] = {o.x, o.y, o.z}, xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z},
+                xd[3] = {D.x, D.y, D.z};
+    float xn[3];
     relax_update(c, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn);
-    x[l] = xn[0];
-    x[N + l] = xn[1];
-    x[2 * N + l] = xn[2];
+    x[l] = make_float4(xn[0], xn[1], xn[2], 0.f);
     change = fmaxf(fmaxf(fabsf(xn[0] - xo[0]), fabsf(xn[1] - xo[1])), fabsf(xn[2] - xo[2]));
   }
   if (check_slot >= 0) {
@@ -363,7 +360,8 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat)
   double term = 0;
   if (l < N) {
     const int i = l / W, j = l - i * W;
-    const float e0 = v.du[l], e1 = v.du[N + l], e2 = v.du[2 * N + l];
+    const float4 e = v.du[l];
+    const float e0 = e.x, e1 = e.y, e2 = e.z;
     const double u0 = v.u[l] + (double)e0;
     const double u1 = v.u[N + l] + (double)e1;
     const double u2 = v.u[2 * N + l] + (double)e2;

@@ -19,25 +19,32 @@ struct PixCoef {
   float ru, rv, rw;                          // rhs
 };
 
-__device__ __forceinline__ PixCoef load_coef(const float* C, int N, int W, int l, bool hl, bool hu) {
+// A pixel's coefficients from the system records; the left and up edge
+// weights are the left neighbour's right and the upper neighbour's down ones
+// (zero where the neighbour does not exist).
+__device__ __forceinline__ PixCoef load_coef(const float4* C, int W, int l, bool hl, bool hu) {
   PixCoef c;
-  c.lxy = hl ? C[kERxy * (size_t)N + l - 1] : 0.f;
-  c.lz = hl ? C[kERz * (size_t)N + l - 1] : 0.f;
-  c.uxy = hu ? C[kEDxy * (size_t)N + l - W] : 0.f;
-  c.uz = hu ? C[kEDz * (size_t)N + l - W] : 0.f;
-  c.rxy = C[kERxy * (size_t)N + l];
-  c.rz = C[kERz * (size_t)N + l];
-  c.dxy = C[kEDxy * (size_t)N + l];
-  c.dz = C[kEDz * (size_t)N + l];
-  c.Buu = C[kBuu * (size_t)N + l];
-  c.Bvv = C[kBvv * (size_t)N + l];
-  c.Bww = C[kBww * (size_t)N + l];
-  c.Buv = C[kBuv * (size_t)N + l];
-  c.Buw = C[kBuw * (size_t)N + l];
-  c.Bvw = C[kBvw * (size_t)N + l];
-  c.ru = C[kRu * (size_t)N + l];
-  c.rv = C[kRv * (size_t)N + l];
-  c.rw = C[kRw * (size_t)N + l];
+  const float4 r0 = C[(size_t)l * kRec + 0], r1 = C[(size_t)l * kRec + 1], r2 = C[(size_t)l * kRec + 2],
+               r3 = C[(size_t)l * kRec + 3];
+  const float4 L = hl ? C[(size_t)(l - 1) * kRec] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 U = hu ? C[(size_t)(l - W) * kRec] : make_float4(0.f, 0.f, 0.f, 0.f);
+  c.lxy = L.x;
+  c.lz = L.y;
+  c.uxy = U.z;
+  c.uz = U.w;
+  c.rxy = r0.x;
+  c.rz = r0.y;
+  c.dxy = r0.z;
+  c.dz = r0.w;
+  c.Buu = r1.x;
+  c.Bvv = r1.y;
+  c.Bww = r1.z;
+  c.Buv = r1.w;
+  c.Buw = r2.x;
+  c.Bvw = r2.y;
+  c.ru = r2.z;
+  c.rv = r2.w;
+  c.rw = r3.x;
   return c;
 }
 
