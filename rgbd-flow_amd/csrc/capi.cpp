@@ -63,9 +63,9 @@ struct PairSlot {
 // Device work-lists of one step of a batch.
 struct StepPlan {
   int n = 0;
-  int pix = 0, plain = 0, tiled = 0;  // total blocks of each kernel family
-  size_t items_off = 0, pix_off = 0, plain_off = 0, tiled_off = 0;
-  Views vpix{}, vplain{}, vtiled{};
+  int pix = 0, plain = 0, tiled = 0, otile = 0;  // total blocks of each kernel family
+  size_t items_off = 0, pix_off = 0, plain_off = 0, tiled_off = 0, otile_off = 0;
+  Views vpix{}, vplain{}, vtiled{}, votile{};
   bool tiled_ok = false;
   int cfg = 0, slots = 0, T = 0;  // tiled launch shape: workgroup shape, LDS slots, iterations
   bool any_early = false;
@@ -465,9 +465,10 @@ size_t pack(std::vector<char>& host, const void* p, size_t bytes) {
 // Packs a step's items and the block offsets of each kernel family.
 void pack_step(std::vector<char>& host, const std::vector<LevelView>& items, StepPlan& st) {
   st.n = (int)items.size();
-  std::vector<int> pix(st.n + 1), plain(st.n + 1), tiled(st.n + 1);
-  pix[0] = plain[0] = tiled[0] = 0;
+  std::vector<int> pix(st.n + 1), plain(st.n + 1), tiled(st.n + 1), otile(st.n + 1);
+  pix[0] = plain[0] = tiled[0] = otile[0] = 0;
   for (int i = 0; i < st.n; i++) {
+    otile[i + 1] = otile[i] + otile_blocks(items[i].W, items[i].H);
     pix[i + 1] = pix[i] + pix_blocks(items[i].N);
     plain[i + 1] = plain[i] + plain_blocks(items[i].W, items[i].H);
     tiled[i + 1] = tiled[i] + (st.tiled_ok ? tiled_blocks(items[i].ts) : 0);
@@ -475,12 +476,14 @@ void pack_step(std::vector<char>& host, const std::vector<LevelView>& items, Ste
   st.pix = pix[st.n];
   st.plain = plain[st.n];
   st.tiled = tiled[st.n];
+  st.otile = otile[st.n];
   st.any_early = false;
   for (const LevelView& v : items) st.any_early = st.any_early || v.allow_early;
   st.items_off = pack(host, items.data(), items.size() * sizeof(LevelView));
   st.pix_off = pack(host, pix.data(), pix.size() * sizeof(int));
   st.plain_off = pack(host, plain.data(), plain.size() * sizeof(int));
   st.tiled_off = pack(host, tiled.data(), tiled.size() * sizeof(int));
+  st.otile_off = pack(host, otile.data(), otile.size() * sizeof(int));
 }
 
 void bind_step(char* dev, StepPlan& st) {
@@ -488,6 +491,7 @@ void bind_step(char* dev, StepPlan& st) {
   st.vpix = Views{items, reinterpret_cast<const int*>(dev + st.pix_off), st.n};
   st.vplain = Views{items, reinterpret_cast<const int*>(dev + st.plain_off), st.n};
   st.vtiled = Views{items, reinterpret_cast<const int*>(dev + st.tiled_off), st.n};
+  st.votile = Views{items, reinterpret_cast<const int*>(dev + st.otile_off), st.n};
 }
 
 int upload(rgbdflow_ctx* c, void* dst, const void* src, size_t bytes) {
@@ -570,7 +574,7 @@ void enqueue_level_solve(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars
     if (k.n_inner == 0) launch_derivs(s, st.vpix, st.pix, 0);  // du = 0 (.cu:776), then the re-warp
     for (int h = 0; h < k.n_inner; h++) {
       if (h == 0) {
-        launch_outer_first(s, st.vpix, st.pix, k.n_inner * checks, k, k.n_inner > 1 ? 1 : 0);
+        launch_outer_first(s, st.votile, st.otile, k.n_inner * checks, k, k.n_inner > 1 ? 1 : 0);
       } else {
         launch_phi(s, st.vpix, st.pix);
         launch_system(s, st.vpix, st.pix, k);

@@ -157,6 +157,9 @@ struct SolveScalars {
 __host__ __device__ inline int pix_blocks(int N) { return (N + kBlock - 1) / kBlock; }
 __host__ __device__ inline int plain_blocks(int W, int H) { return (((W + 1) / 2 + 63) / 64) * ((H + 3) / 4); }
 __host__ __device__ inline int tiled_blocks(const TileSched& t) { return t.tiles_x * t.tiles_y; }
+// tiles of the fused first-inner kernel (kBlock threads, one pixel each)
+constexpr int kOtW = 64, kOtH = kBlock / kOtW;
+__host__ __device__ inline int otile_blocks(int W, int H) { return ((W + kOtW - 1) / kOtW) * ((H + kOtH - 1) / kOtH); }
 
 // ---------------------------------------------------------------- launchers
 // kernels_solve.hip (blocks = total blocks of the list for that family)

@@ -272,22 +272,69 @@ __global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
 // occlusion weight, du = 0, phi at the pixel and at its left and upper
 // neighbours (recomputed from u: du is zero), and the system -- the values
 // k_derivs, k_phi and k_system compute, without storing the intermediates.
-// store: also store derivatives and occlusion weights for later inner
+// Each block owns a kOtW x kOtH tile: the stencil inputs 0.4*I1 + 0.6*Iw of
+// the tile and a 2-pixel border (clamped to the image, as the stencils clamp)
+// and phi of the tile and its upper row / left column are staged in LDS once.
+// store: also store derivatives, occlusion weights and phi for later inner
 // iterations.  Zeroes the item's check slots of this outer iteration.
+constexpr int kRw = kOtW + 4, kRh = kOtH + 4;  // stencil region
+constexpr int kPw = kOtW + 1, kPh = kOtH + 1;  // phi region
+
 __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, SolveScalars s, int store) {
+  __shared__ double Ms[kFeat + 1][kRh * kRw];  // 0.4*I1 + 0.6*Iw (5 features, depth)
+  __shared__ double Ps[kPh * kPw];            // phi
   const int it = work_item(w, blockIdx.x);
   const LevelView& v = w.items[it];
   if (v.ctl->outer_done) return;
-  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
-  const int stride = (w.start[it + 1] - w.start[it]) * kBlock;
-  for (int t = l; t < n_slots; t += stride) v.ctl->maxdiff[t] = 0u;
-  if (l >= v.N) return;
-  const int W = v.W, N = v.N;
-  const int i = l / W, j = l - i * W;
+  const int lb = blockIdx.x - w.start[it];
+  {
+    const int stride = (w.start[it + 1] - w.start[it]) * kBlock;
+    for (int t = lb * kBlock + threadIdx.x; t < n_slots; t += stride) v.ctl->maxdiff[t] = 0u;
+  }
+  const int W = v.W, H = v.H, N = v.N;
+  const int tx = (W + kOtW - 1) / kOtW;
+  const int x0 = (lb % tx) * kOtW, y0 = (lb / tx) * kOtH;
+  for (int t = threadIdx.x; t < kRh * kRw; t += kBlock) {
+    const int ry = t / kRw, rx = t - ry * kRw;
+    const int o = clampi(y0 - 2 + ry, 0, H - 1) * W + clampi(x0 - 2 + rx, 0, W - 1);
+    for (int k = 0; k < kFeat; k++) Ms[k][t] = .4 * v.I1[(size_t)k * N + o] + .6 * v.Iw[(size_t)k * N + o];
+    Ms[kFeat][t] = .4 * v.D1[o] + .6 * v.Dw[o];
+  }
+  for (int t = threadIdx.x; t < kPh * kPw; t += kBlock) {
+    const int py = t / kPw, px = t - py * kPw;
+    const int i = y0 - 1 + py, j = x0 - 1 + px;
+    Ps[t] = (i >= 0 && j >= 0 && i < H && j < W) ? phi_at<true>(v, i * W + j, i, j) : 0.0;
+  }
+  __syncthreads();
+  const int ty = threadIdx.x / kOtW, txx = threadIdx.x - ty * kOtW;
+  const int i = y0 + ty, j = x0 + txx;
+  if (i >= H || j >= W) return;
+  const int l = i * W + j;
+  // derivs_at, with the stencil inputs from LDS (same values, same order)
   PixDerivs d;
-  derivs_at(v, l, i, j, d);
+  const double g[5] = {1, -8, 0, 8, -1};
+  const int c0 = (ty + 2) * kRw + txx + 2;
+  for (int k = 0; k < kFeat; k++) {
+    double sx = 0, sy = 0;
+    for (int m = 0; m < 5; m++) sx += g[m] * Ms[k][c0 - 2 + m];
+    for (int m = 0; m < 5; m++) sy += g[m] * Ms[k][c0 + (m - 2) * kRw];
+    d.Ix[k] = sx / 12;
+    d.Iy[k] = sy / 12;
+    d.It[k] = v.Iw[(size_t)k * N + l] - v.I1[(size_t)k * N + l];
+  }
+  {
+    const double mg = .04;
+    double sx = 0, sy = 0;
+    for (int m = 0; m < 5; m++) sx += g[m] * Ms[kFeat][c0 - 2 + m];
+    for (int m = 0; m < 5; m++) sy += g[m] * Ms[kFeat][c0 + (m - 2) * kRw];
+    d.Dx = fmin(fmax(sx / 12, -mg), mg);
+    d.Dy = fmin(fmax(sy / 12, -mg), mg);
+    d.Dt = fmin(fmax(v.Dw[l] - v.D1[l], -mg), mg);
+  }
   const float q = occlusion_at(v, l);
   v.du[l] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int p0 = (ty + 1) * kPw + txx + 1;
+  const double pC = Ps[p0], pL = Ps[p0 - 1], pU = Ps[p0 - kPw];
   if (store) {
     for (int k = 0; k < kFeat; k++) {
       v.Ix[(size_t)k * N + l] = d.Ix[k];
@@ -298,11 +345,8 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     v.Dy[l] = d.Dy;
     v.Dt[l] = d.Dt;
     v.q[l] = q;
+    v.phi[l] = pC;
   }
-  const double pC = phi_at<true>(v, l, i, j);
-  const double pL = j > 0 ? phi_at<true>(v, l - 1, i, j - 1) : 0.0;
-  const double pU = i > 0 ? phi_at<true>(v, l - W, i - 1, j) : 0.0;
-  if (store) v.phi[l] = pC;
   assemble(v, s, l, i, j, 0.0, 0.0, 0.0, d, (double)q, pC, pL, pU);
 }
 
