@@ -57,9 +57,38 @@ def workload(R, n_distinct):
     return pairs, levels
 
 
-def params_for(R, variant):
+MIXED_SIZES = [(320, 240), (640, 480), (960, 540), (1280, 720)]
+
+
+def mixed_workload(R, n, n_distinct, seed=100):
+    """n pairs with sizes drawn uniformly from MIXED_SIZES, intrinsics scaled
+    by W/640, 2-8 moving segments; n_distinct synthetic pairs per size are
+    generated and cycled."""
+    import rgbdflow_amd.synth as S
+    rng = np.random.default_rng(seed)
+    sizes = [MIXED_SIZES[k] for k in rng.integers(0, len(MIXED_SIZES), n)]
+    made = {}
+    pairs = []
+    for i, (w, h) in enumerate(sizes):
+        lst = made.setdefault((w, h), [])
+        if len(lst) < n_distinct:
+            f = 525.0 * w / 640
+            cam = S.Camera(w, h, f, f, (w - 1) / 2, (h - 1) / 2)
+            segs = int(rng.integers(2, 9))
+            pr = S.make_pair(w, h, seed + i, n_segments=segs, cam=cam)
+            im1, d1, im2, d2 = S.to_solver_inputs(pr)
+            lv = R.pyramid_levels(w, h, 0.5, w // 16)
+            hpb = S.segment_boundary_maps(pr["seg1"], [a for a, b in lv], [b for a, b in lv])
+            c = R.make_params(w, h, f=f).cam
+            lst.append(dict(im1=im1, dm1=d1.astype(np.float64), im2=im2, dm2=d2.astype(np.float64), hpb=hpb,
+                            cam=c, min_width=w // 16))
+        pairs.append(lst[sum(1 for x in sizes[:i] if x == (w, h)) % len(lst)])
+    return pairs, sizes
+
+
+def params_for(R, variant, graph=True):
     return R.make_params(W, H, f=525.0, cx=319.5, cy=239.5, n_outer=15, n_sor=50, min_width=40,
-                         relax_variant=R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN)
+                         relax_variant=R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN, graph=graph)
 
 
 def algorithmic_bytes(stats):
@@ -115,6 +144,9 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch every kernel instead of replaying a HIP graph")
+    ap.add_argument("--mixed", type=int, default=0,
+                    help="instead: a batch of this many pairs of mixed sizes (320x240 .. 1280x720)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: the scene-flow solve runs on one GPU "
@@ -122,10 +154,12 @@ def main():
         sys.exit(2)
 
     R = _package()
+    if args.mixed:
+        return mixed_main(R, args)
     pairs, levels = workload(R, max(1, min(args.distinct, args.batch)))
     B = args.batch
     batch = [pairs[i % len(pairs)] for i in range(B)]
-    p = params_for(R, args.variant)
+    p = params_for(R, args.variant, not args.no_graph)
     ctx = R.Context(0)
 
     def autotune(nb):
@@ -178,7 +212,7 @@ def main():
     plain_pps = single_pps = None
     single_lat = None
     if args.plain_steps > 0:
-        pp = params_for(R, "plain")
+        pp = params_for(R, "plain", graph=False)
         ctx.batch_load(pp, batch[:1])
         ctx.batch_run(pp, stats=True)
         plain_pps = args.plain_steps / timed_runs(ctx, pp, args.plain_steps)
@@ -251,6 +285,51 @@ def main():
         out["cpu_baseline"] = {"value": round(v, 4), "unit": "pairs/s", "cores": 1, "kind": "port",
                                "sample": f"{n} whole pairs of the same 640x480 workload through the oracle "
                                          f"(reference precision, 1 thread, {el:.1f} s)"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def mixed_main(R, args):
+    """The mixed-size batch: one batch of --mixed pairs, every level step of
+    every pair in one launch per kernel, schedules autotuned per level shape."""
+    pairs, sizes = mixed_workload(R, args.mixed, max(1, args.distinct // 2))
+    print(f"bench: mixed batch of {len(pairs)}: " +
+          ", ".join(f"{w}x{h}: {sizes.count((w, h))}" for (w, h) in MIXED_SIZES), file=sys.stderr, flush=True)
+    ctx = R.Context(0)
+    p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED,
+                      graph=not args.no_graph)
+    if not args.no_autotune:
+        for (w, h) in sorted(set(sizes), key=lambda t: -t[0]):
+            n = sizes.count((w, h))
+            for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
+                ctx.autotune_relax(lw, lh, batch=n)
+    ctx.batch_load(p, pairs)
+    st = ctx.batch_run(p, stats=True)
+    for _ in range(args.warmup):
+        ctx.batch_run(p, stats=False)
+    elapsed = timed_runs(ctx, p, args.steps)
+    lat = np.array([ctx.batch_run(p, stats=True)[0].device_ms for _ in range(max(3, args.latency_steps // 4))])
+    npx = sum(w * h for (w, h) in sizes)
+    out = {
+        "metric": "mixed-size scene-flow pairs/s (one batch)",
+        "value": round(len(pairs) * args.steps / elapsed, 3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"{len(pairs)} pairs, sizes uniform over 320x240/640x480/960x540/1280x720, "
+                               "5-level pyramids, 15 outer x 50 inner, HPB regularisation",
+                   "sizes": {f"{w}x{h}": sizes.count((w, h)) for (w, h) in MIXED_SIZES},
+                   "megapixels_per_s": round(npx * args.steps / elapsed / 1e6, 2),
+                   "batch_p50_ms": round(float(np.percentile(lat, 50)), 3),
+                   "graph": not args.no_graph},
+    }
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -71,7 +71,9 @@ typedef struct rgbdflow_params {
   int tile_iters;                /* iterations per launch of the tiled relaxation (0 = planner) */
   int tile_w, tile_h;            /* interior tile of the tiled relaxation (0 = planner) */
   int tile_config;               /* workgroup shape of the tiled relaxation (-1 = planner) */
-  int reserved[4];
+  int graph;                     /* 1: batch runs replay their schedule as a captured HIP graph
+                                    (captured at the second run with the same batch and params) */
+  int reserved[3];
 } rgbdflow_params;
 
 typedef struct rgbdflow_stats {

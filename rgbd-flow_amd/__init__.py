@@ -47,7 +47,7 @@ class Params(C.Structure):
                 ("min_width", C.c_int), ("n_outer", C.c_int), ("n_inner", C.c_int), ("n_sor", C.c_int),
                 ("relax_variant", C.c_int), ("inner_exit", C.c_int), ("outer_exit", C.c_int),
                 ("tile_iters", C.c_int), ("tile_w", C.c_int), ("tile_h", C.c_int), ("tile_config", C.c_int),
-                ("reserved", C.c_int * 4)]
+                ("graph", C.c_int), ("reserved", C.c_int * 3)]
 
 
 class Stats(C.Structure):
@@ -110,7 +110,7 @@ class RgbdFlowError(RuntimeError):
 
 def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
                 ratio=0.5, min_width=None, reg=REG_HPB, cw=1.0, dvc=1.0, inner_exit=True, outer_exit=True,
-                relax_variant=RELAX_PLAIN, tile_iters=0, tile_w=0, tile_h=0, tile_config=-1) -> Params:
+                relax_variant=RELAX_PLAIN, tile_iters=0, tile_w=0, tile_h=0, tile_config=-1, graph=False) -> Params:
     p = Params()
     lib().rgbdflow_default_params(C.byref(p))
     p.regularization_type = reg
@@ -130,6 +130,7 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.relax_variant = relax_variant
     p.tile_iters = tile_iters
     p.tile_w, p.tile_h, p.tile_config = tile_w, tile_h, tile_config
+    p.graph = int(graph)
     return p
 
 

@@ -86,6 +86,17 @@ struct BatchPlan {
   };
   std::vector<Wave> waves;
   WorkList<PairPrep> prep{};
+  // the schedule captured as a HIP graph (params.graph)
+  int eager_runs = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void drop_graph() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    eager_runs = 0;
+  }
 };
 
 }  // namespace
@@ -738,6 +749,8 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
       std::memcmp(bp.params.data(), pb, sizeof(*p)) == 0)
     return RGBDFLOW_OK;
   bp.valid = false;
+  TRY(sync(c));
+  bp.drop_graph();
   bp.host.clear();
   bp.steps.clear();
   bp.waves.clear();
@@ -869,6 +882,22 @@ int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
   return validate_iters(c, p->n_outer, p->n_inner, p->n_sor);
 }
 
+// Every launch of a batch run, on the context's stream.
+void enqueue_batch(rgbdflow_ctx* c, const BatchPlan& bp, const SolveScalars& k) {
+  hipStream_t s = c->stream;
+  // per-pair weights: depth stdevs (runFlowTwoFrames.cpp:89-91) -> delta
+  // (OpticalFlow.cpp:71-81); s = f^2 (OpticalFlow.cpp:90-97); then the pyramids
+  launch_pair_prep(s, bp.prep, bp.prep_blocks);
+  for (const BatchPlan::Wave& wv : bp.waves) launch_pyramid_jobs(s, wv.wl, wv.blocks);
+  // coarse to fine
+  for (const StepPlan& sp : bp.steps) {
+    launch_features(s, sp.vpix, sp.pix);
+    launch_level_entry(s, sp.vpix, sp.pix);
+    launch_level_weights(s, sp.vpix, sp.pix);
+    enqueue_level_solve(c, sp, k);
+  }
+}
+
 int batch_load(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const rgbdflow_pair_input* in) {
   if (!c) return RGBDFLOW_ERR_ARG;
   if (!p || n < 1 || !in) return fail(c, RGBDFLOW_ERR_ARG, "bad batch arguments");
@@ -907,17 +936,24 @@ int batch_run(rgbdflow_ctx* c, const rgbdflow_params* p, rgbdflow_stats* st) {
   hipStream_t s = c->stream;
   const SolveScalars k = scalars_from(p, p->color_data_weight, p->alpha, p->gamma, p->n_outer, p->n_inner, p->n_sor);
   c->relax_ev_used = 0;
+  BatchPlan& bpm = c->plan;
+  const bool graph = p->graph && !c->time_relax;
+  if (graph && !bpm.exec && bpm.eager_runs > 0) {
+    // capture the schedule (every launch configuration has run once, so no
+    // launcher queries the device inside the capture)
+    HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue_batch(c, bp, k);
+    hipGraph_t g = nullptr;
+    HIP_TRY(c, hipStreamEndCapture(s, &g));
+    bpm.graph = g;
+    HIP_TRY(c, hipGraphInstantiate(&bpm.exec, g, nullptr, nullptr, 0));
+  }
   HIP_TRY(c, hipEventRecord(c->ev0, s));
-  // per-pair weights: depth stdevs (runFlowTwoFrames.cpp:89-91) -> delta
-  // (OpticalFlow.cpp:71-81); s = f^2 (OpticalFlow.cpp:90-97); then the pyramids
-  launch_pair_prep(s, bp.prep, bp.prep_blocks);
-  for (const BatchPlan::Wave& wv : bp.waves) launch_pyramid_jobs(s, wv.wl, wv.blocks);
-  // coarse to fine
-  for (const StepPlan& sp : bp.steps) {
-    launch_features(s, sp.vpix, sp.pix);
-    launch_level_entry(s, sp.vpix, sp.pix);
-    launch_level_weights(s, sp.vpix, sp.pix);
-    enqueue_level_solve(c, sp, k);
+  if (graph && bpm.exec) {
+    HIP_TRY(c, hipGraphLaunch(bpm.exec, s));
+  } else {
+    enqueue_batch(c, bp, k);
+    bpm.eager_runs++;
   }
   HIP_TRY(c, hipEventRecord(c->ev1, s));
   HIP_TRY(c, hipGetLastError());
@@ -1029,6 +1065,7 @@ int rgbdflow_destroy(rgbdflow_ctx* c) {
   if (!c) return RGBDFLOW_ERR_ARG;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  c->plan.drop_graph();
   for (auto& kv : c->bufs)
     if (kv.second.p) hipFree(kv.second.p);
   for (hipEvent_t e : c->relax_ev) hipEventDestroy(e);

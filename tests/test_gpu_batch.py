@@ -147,3 +147,35 @@ def test_batch_errors(R):
         with pytest.raises(R.RgbdFlowError) as e:
             c.batch_load(p, [dict(cases[0], hpb=None)])   # HPB regularisation without maps
         assert e.value.code == R.ERR_ARG
+
+
+@pytest.mark.parametrize("variant", ["plain", "tiled"])
+def test_batch_graph_replays(ctx, R, variant):
+    """params.graph: the second run captures the schedule as a HIP graph and
+    later runs replay it; every replay (with other device work and
+    synchronisations in between) equals the oracle bit for bit."""
+    O = oracle()
+    cases = [_case(160, 120, 91, 20), _case(96, 72, 92, 12)]
+    p = R.make_params(160, 120, n_outer=4, n_sor=30, min_width=20, graph=True,
+                      relax_variant=R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN)
+    pairs = [dict(c, cam=R.make_params(c["W"], c["H"]).cam) for c in cases]
+    ctx.batch_load(p, pairs)
+    ref = [_oracle_solve(c, 4, 30) for c in cases]
+    coef = O.level_system(*_small_system(O))
+    for run in range(4):
+        st = ctx.batch_run(p, stats=True)
+        for i, c in enumerate(cases):
+            ua, va, wa, sa = ref[i]
+            ub, vb, wb = ctx.batch_read(i)
+            assert [st[i].sweeps_exec[k] for k in range(sa.n_levels)] == \
+                [sa.sweeps_exec[k] for k in range(sa.n_levels)], (run, i)
+            assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb), (run, i)
+        # eager work on the same stream between replays
+        ctx.relax(coef, 7, np.zeros((3, 24, 32)))
+        ctx.synchronize()
+
+
+def _small_system(O):
+    from helpers import level_inputs
+    li, (u, v, w), _ = level_inputs(seed=5, W=32, H=24)
+    return li, O.make_params(32, 24), u, v, w
