@@ -244,6 +244,48 @@ int rgbdflow_set_relax_schedule(rgbdflow_ctx* ctx, int W, int H, const rgbdflow_
 int rgbdflow_time_relax_schedule(rgbdflow_ctx* ctx, int W, int H, int batch, const rgbdflow_tile_choice* choice,
                                  double* us_per_iter);
 
+/* ---- serving: continuous batching of RGB-D frame streams ----
+ * No counterpart in the reference, which solves one pair per call
+ * (runFlowTwoFrames.cpp:114-126).  A server owns `lanes` executor contexts
+ * and a worker thread.  Each stream keeps its last frame on the device; every
+ * submitted frame after the first forms the pair (last frame, frame), solved
+ * by sceneFlowMultiscale exactly as rgbdflow_multiscale would (depth stdevs
+ * computed on the device, no hole fill), in whatever batch the worker puts it:
+ * the result does not depend on the batch. */
+typedef struct rgbdflow_server rgbdflow_server;
+typedef struct rgbdflow_server_config {
+  int device;
+  int max_batch;                  /* pairs per batch (<= 0: 16) */
+  int lanes;                      /* batches in flight at once, each on its own HIP stream (<= 0: 2) */
+  unsigned long long hbm_budget;  /* bytes for stream state and in-flight pairs (0: no limit) */
+} rgbdflow_server_config;
+typedef struct rgbdflow_request_info {
+  int has_flow;       /* 0 for a stream's first frame */
+  int status;         /* RGBDFLOW_OK or the error of the request's batch */
+  int batch_pairs;    /* pairs in that batch */
+  int stream;
+  double queue_ms;    /* submit -> batch start */
+  double batch_ms;    /* batch start -> batch done */
+  double latency_ms;  /* submit -> result available */
+} rgbdflow_request_info;
+int rgbdflow_server_create(rgbdflow_server** out, const rgbdflow_server_config* cfg, const rgbdflow_params* params);
+int rgbdflow_server_destroy(rgbdflow_server* srv);
+const char* rgbdflow_server_last_error(const rgbdflow_server* srv);
+/* cam NULL: params->cam; min_width <= 0: params->min_width */
+int rgbdflow_stream_open(rgbdflow_server* srv, int W, int H, const rgbdflow_camera* cam, int min_width,
+                         int* stream_id);
+/* frame: bgr [N][3] u8 (cv2liuBGR: c / 255.0, no channel swap), depth [N] f32 metres
+ * (cv2liuDepth, no fill), hpb: the frame's boundary maps of every level (NULL
+ * unless RGBDFLOW_REG_HPB).  Copies the arrays; returns a ticket. */
+int rgbdflow_stream_submit(rgbdflow_server* srv, int stream_id, const unsigned char* bgr, const float* depth,
+                           const float* hpb, unsigned long long* ticket);
+/* 1: done (u, v, w [N] copied when non-NULL and info->has_flow; the ticket is
+ * released), 0: not done within wait_ms (< 0: wait for it), < 0: error. */
+int rgbdflow_stream_poll(rgbdflow_server* srv, unsigned long long ticket, int wait_ms, double* u, double* v,
+                         double* w, rgbdflow_request_info* info);
+/* waits for the stream's requests, then frees its state */
+int rgbdflow_stream_close(rgbdflow_server* srv, int stream_id);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,7 +30,9 @@ EXPORTS = [
     "rgbdflow_depth_map_pyramid", "rgbdflow_im2feature", "rgbdflow_warp_by_3d_flow", "rgbdflow_upsample_flow",
     "rgbdflow_level_system", "rgbdflow_relax", "rgbdflow_autotune_relax", "rgbdflow_batch_load",
     "rgbdflow_batch_run", "rgbdflow_batch_read", "rgbdflow_set_relax_schedule",
-    "rgbdflow_time_relax_schedule",
+    "rgbdflow_time_relax_schedule", "rgbdflow_server_create", "rgbdflow_server_destroy",
+    "rgbdflow_server_last_error", "rgbdflow_stream_open", "rgbdflow_stream_submit", "rgbdflow_stream_poll",
+    "rgbdflow_stream_close",
 ]
 
 
@@ -66,6 +68,18 @@ class PairInput(C.Structure):
                 ("min_width", C.c_int), ("reserved", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("max_batch", C.c_int), ("lanes", C.c_int), ("hbm_budget", C.c_ulonglong)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("has_flow", C.c_int), ("status", C.c_int), ("batch_pairs", C.c_int), ("stream", C.c_int),
+                ("queue_ms", C.c_double), ("batch_ms", C.c_double), ("latency_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class TileChoice(C.Structure):
     _fields_ = [("tile_w", C.c_int), ("tile_h", C.c_int), ("tile_iters", C.c_int), ("tile_config", C.c_int),
                 ("threads", C.c_int), ("pairs_per_thread", C.c_int), ("single", C.c_int), ("candidates", C.c_int),
@@ -87,6 +101,7 @@ def lib():
         _lib.rgbdflow_last_error.restype = C.c_char_p
         _lib.rgbdflow_bytes_in_use.restype = C.c_ulonglong
         _lib.rgbdflow_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_ulonglong]
+        _lib.rgbdflow_server_last_error.restype = C.c_char_p
     return _lib
 
 
@@ -381,3 +396,62 @@ class Context:
         self._check(lib().rgbdflow_with_init(self.h, W, H, _p(im1), _p(dm1), _p(im2), _p(dm2), _p(s1), _p(h),
                                              _p(est), C.byref(params), _p(u), _p(v), _p(w), C.byref(st)))
         return u, v, w, st
+
+
+class Server:
+    """A serving runtime (rgbdflow_server_*): streams of frames, continuous batching."""
+
+    def __init__(self, params, device=0, max_batch=16, lanes=2, hbm_budget=0):
+        cfg = ServerConfig(device, max_batch, lanes, hbm_budget)
+        h = C.c_void_p()
+        rc = lib().rgbdflow_server_create(C.byref(h), C.byref(cfg), C.byref(params))
+        if rc != OK:
+            raise RgbdFlowError(rc, "rgbdflow_server_create failed")
+        self.h = h
+        self._sizes = {}
+
+    def _check(self, rc):
+        if rc < 0:
+            raise RgbdFlowError(rc, lib().rgbdflow_server_last_error(self.h).decode())
+        return rc
+
+    def open(self, W, H, cam=None, min_width=0):
+        sid = C.c_int()
+        self._check(lib().rgbdflow_stream_open(self.h, W, H, C.byref(cam) if cam is not None else None, min_width,
+                                               C.byref(sid)))
+        self._sizes[sid.value] = (H, W)
+        return sid.value
+
+    def submit(self, stream, bgr_u8, depth_f32, hpb=None):
+        b = np.ascontiguousarray(bgr_u8, dtype=np.uint8)
+        d = _f32(depth_f32)
+        hh = _f32(hpb) if hpb is not None else None
+        t = C.c_ulonglong()
+        self._check(lib().rgbdflow_stream_submit(self.h, stream, _p(b), _p(d), _p(hh), C.byref(t)))
+        return (t.value, stream)
+
+    def poll(self, ticket, wait_ms=-1):
+        """(info dict, (u, v, w) or None) when done; None if not done in wait_ms."""
+        t, stream = ticket
+        H, W = self._sizes[stream]
+        u = np.empty((H, W)); v = np.empty((H, W)); w = np.empty((H, W))
+        info = RequestInfo()
+        rc = self._check(lib().rgbdflow_stream_poll(self.h, C.c_ulonglong(t), int(wait_ms), _p(u), _p(v), _p(w),
+                                                    C.byref(info)))
+        if rc == 0:
+            return None
+        return info.as_dict(), ((u, v, w) if info.has_flow else None)
+
+    def close_stream(self, stream):
+        self._check(lib().rgbdflow_stream_close(self.h, stream))
+
+    def close(self):
+        if self.h:
+            lib().rgbdflow_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

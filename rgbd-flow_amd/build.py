@@ -16,7 +16,7 @@ LIB_CXX = os.path.join(HERE, "librgbdflow_cxx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RGBDFLOW_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels_solve.hip", "kernels_pyramid.hip", "kernels_relax_tiled.hip", "capi.cpp"]
+HIP_SOURCES = ["kernels_solve.hip", "kernels_pyramid.hip", "kernels_relax_tiled.hip", "capi.cpp", "server.cpp"]
 CXX_SOURCES = ["dropin.cpp", "median_fill.cpp"]
 
 # -ffp-contract=off: products and sums round where the source says, which the
@@ -50,7 +50,7 @@ def build(verbose: bool = False) -> str:
         list(ex.map(_run, jobs))
     objs = [j[-1] for j in jobs]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
     cxx = [os.path.join(CSRC, s) for s in CXX_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     if cxx:

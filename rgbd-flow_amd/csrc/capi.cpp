@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/rgbdflow/rgbdflow.h"
+#include "capi_internal.hpp"
 #include "internal.hpp"
 
 using namespace rgbdflow;
@@ -50,6 +51,7 @@ struct Work {
 struct PairSlot {
   int W = 0, H = 0, min_width = 0;
   bool cam_given = false, sigma_given = false, hpb_given = false;
+  bool pyr1_given = false;  // frame 1's colour and depth pyramids were loaded (not built)
   rgbdflow_camera cam{};
   std::vector<LevelShape> L;
   std::vector<size_t> off;   // level k >= 1: element offset (per channel) in the pyramid arrays
@@ -691,24 +693,25 @@ int stage_level_inputs(rgbdflow_ctx* c, int W, int H, float focal_level, const d
 }
 
 // ------------------------------------------------------------------ batches
-int load_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const rgbdflow_pair_input& in, PairSlot& ps) {
-  if (bad_dims(in.W, in.H) || !in.im1 || !in.dm1 || !in.im2 || !in.dm2)
-    return fail(c, RGBDFLOW_ERR_ARG, "bad arguments of pair " + std::to_string(idx));
-  ps.W = in.W;
-  ps.H = in.H;
-  ps.cam_given = in.cam.xRes != 0;
-  ps.cam = in.cam;
-  ps.min_width = in.min_width > 0 ? in.min_width : p->min_width;
-  ps.sigma_given = in.sigma1 != nullptr;
-  ps.hpb_given = in.hpb != nullptr;
-  ps.L = pyramid_shape(in.W, in.H, p->pyramid_ratio, ps.min_width, nullptr, nullptr);
+// Shape and buffers of pair slot idx (W x H, camera, pyramid min width).
+int alloc_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, int W, int H, const rgbdflow_camera& cam,
+               int min_width, bool hpb, PairSlot& ps) {
+  if (bad_dims(W, H)) return fail(c, RGBDFLOW_ERR_ARG, "bad size of pair " + std::to_string(idx));
+  ps.W = W;
+  ps.H = H;
+  ps.cam_given = cam.xRes != 0;
+  ps.cam = cam;
+  ps.min_width = min_width > 0 ? min_width : p->min_width;
+  ps.hpb_given = hpb;
+  ps.pyr1_given = false;
+  ps.L = pyramid_shape(W, H, p->pyramid_ratio, ps.min_width, nullptr, nullptr);
   const int nL = (int)ps.L.size();
   if (nL > kMaxLevels) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "too many pyramid levels");
   ps.off.assign(nL, 0);
   for (int k = 2; k < nL; k++) ps.off[k] = ps.off[k - 1] + (size_t)ps.L[k - 1].W * ps.L[k - 1].H;
   ps.hoff.assign(nL, 0);
   for (int k = 1; k < nL; k++) ps.hoff[k] = ps.hoff[k - 1] + (size_t)ps.L[k - 1].W * ps.L[k - 1].H * 2;
-  const size_t N = (size_t)in.W * in.H, rest = pyramid_elems(ps.L, 1) + 1;
+  const size_t N = (size_t)W * H, rest = pyramid_elems(ps.L, 1) + 1;
   const std::string pre = "b" + std::to_string(idx) + ".";
   TRY(get(c, pre + "im1", 3 * N, &ps.im1));
   TRY(get(c, pre + "im2", 3 * N, &ps.im2));
@@ -724,8 +727,16 @@ int load_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const rgbdflow
   TRY(get(c, pre + "pdel", rest, &ps.pdel));
   TRY(get(c, pre + "psm", rest, &ps.psm));
   ps.hpb = nullptr;
-  if (in.hpb) TRY(get(c, pre + "hpb", pyramid_elems(ps.L, 0) * 2, &ps.hpb));
-  TRY(get_work(c, pre, N, ps.wk));
+  if (hpb) TRY(get(c, pre + "hpb", pyramid_elems(ps.L, 0) * 2, &ps.hpb));
+  return get_work(c, pre, N, ps.wk);
+}
+
+int load_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const rgbdflow_pair_input& in, PairSlot& ps) {
+  if (!in.im1 || !in.dm1 || !in.im2 || !in.dm2)
+    return fail(c, RGBDFLOW_ERR_ARG, "bad arguments of pair " + std::to_string(idx));
+  TRY(alloc_pair(c, p, idx, in.W, in.H, in.cam, in.min_width, in.hpb != nullptr, ps));
+  ps.sigma_given = in.sigma1 != nullptr;
+  const size_t N = (size_t)in.W * in.H;
   TRY(upload_planes(c, in.im1, N, 3, ps.im1));
   TRY(upload_planes(c, in.im2, N, 3, ps.im2));
   TRY(upload(c, ps.dm1, in.dm1, N * 8));
@@ -798,6 +809,7 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
       } imgs[6] = {{ps.im1, ps.pim1, 3}, {ps.im2, ps.pim2, 3},  {ps.dm1, ps.pd1, 1},
                    {ps.dm2, ps.pd2, 1},  {ps.del0, ps.pdel, 1}, {ps.sm0, ps.psm, 1}};
       for (const Img& im : imgs) {
+        if (ps.pyr1_given && (im.base == ps.im1 || im.base == ps.dm1)) continue;  // loaded with frame 1
         std::vector<double*> lvl(nL);
         for (int k = 0; k < nL; k++) lvl[k] = level_of(ps, im.base, im.rest, im.C, k);
         for (int k = 1; k < nL; k++) {
@@ -880,6 +892,28 @@ int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
   if (p->variant != RGBDFLOW_VARIANT_FULL) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "bad scene flow variant");
   if (p->regularization_type < 0 || p->regularization_type > 2) return fail(c, RGBDFLOW_ERR_ARG, "bad regularization");
   return validate_iters(c, p->n_outer, p->n_inner, p->n_sor);
+}
+
+// A pair whose frames are already on the device (the serving path).
+int load_pair_device(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const PairDev& in, PairSlot& ps) {
+  if (!in.f1.im || !in.f1.dm || !in.f2.im || !in.f2.dm)
+    return fail(c, RGBDFLOW_ERR_ARG, "bad device frames of pair " + std::to_string(idx));
+  TRY(alloc_pair(c, p, idx, in.W, in.H, in.cam, in.min_width, in.f1.hpb != nullptr, ps));
+  ps.sigma_given = false;
+  const size_t N = (size_t)in.W * in.H, rest = pyramid_elems(ps.L, 1);
+  hipStream_t s = c->stream;
+  HIP_TRY(c, hipMemcpyAsync(ps.im1, in.f1.im, 3 * N * 8, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(ps.dm1, in.f1.dm, N * 8, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(ps.im2, in.f2.im, 3 * N * 8, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(ps.dm2, in.f2.dm, N * 8, hipMemcpyDeviceToDevice, s));
+  if (in.f1.hpb)
+    HIP_TRY(c, hipMemcpyAsync(ps.hpb, in.f1.hpb, pyramid_elems(ps.L, 0) * 2 * 4, hipMemcpyDeviceToDevice, s));
+  if (in.f1.pim && in.f1.pd && rest > 0) {
+    HIP_TRY(c, hipMemcpyAsync(ps.pim1, in.f1.pim, 3 * rest * 8, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(ps.pd1, in.f1.pd, rest * 8, hipMemcpyDeviceToDevice, s));
+    ps.pyr1_given = true;
+  }
+  return RGBDFLOW_OK;
 }
 
 // Every launch of a batch run, on the context's stream.
@@ -1719,3 +1753,44 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
 }
 
 }  // extern "C"
+
+namespace rgbdflow {
+
+int batch_load_device(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const PairDev* in) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (!p || n < 1 || !in) return fail(c, RGBDFLOW_ERR_ARG, "bad batch arguments");
+  HIP_TRY(c, hipSetDevice(c->device));
+  c->loaded = false;
+  c->ran = false;
+  c->gen++;
+  c->pairs.assign(n, PairSlot());
+  for (int i = 0; i < n; i++) TRY(load_pair_device(c, p, i, in[i], c->pairs[i]));
+  c->load_ratio = p->pyramid_ratio;
+  c->load_min_width = p->min_width;
+  c->loaded = true;
+  return RGBDFLOW_OK;
+}
+
+int pair_outputs(rgbdflow_ctx* c, int i, FrameDev* f2, const double** flow, size_t* pyr_elems) {
+  if (!c || !c->ran || i < 0 || i >= (int)c->pairs.size()) return RGBDFLOW_ERR_STATE;
+  const PairSlot& ps = c->pairs[i];
+  f2->im = ps.im2;
+  f2->dm = ps.dm2;
+  f2->pim = ps.pim2;
+  f2->pd = ps.pd2;
+  f2->hpb = nullptr;
+  *flow = ps.wk.u[((int)ps.L.size() - 1) & 1];
+  *pyr_elems = pyramid_elems(ps.L, 1);
+  return RGBDFLOW_OK;
+}
+
+hipStream_t ctx_stream(rgbdflow_ctx* c) { return c->stream; }
+
+size_t pyramid_rest_elems(int W, int H, double ratio, int min_width) {
+  return pyramid_elems(pyramid_shape(W, H, ratio, min_width, nullptr, nullptr), 1);
+}
+size_t pyramid_all_elems(int W, int H, double ratio, int min_width) {
+  return pyramid_elems(pyramid_shape(W, H, ratio, min_width, nullptr, nullptr), 0);
+}
+
+}  // namespace rgbdflow

@@ -211,5 +211,6 @@ void launch_level_weights(hipStream_t s, const Views& v, int blocks);
 void launch_depth_sigma(hipStream_t s, const float* depth, int W, int H, int max_dist, float* sigma);
 void launch_depth_map_level(hipStream_t s, const double* larger, int LW, int LH, double* dst, int W, int H);
 void launch_features_planes(hipStream_t s, const double* bgr, int W, int H, double* out5);
+void launch_ingest(hipStream_t s, const unsigned char* bgr, const float* depth, int N, double* im, double* dm);
 
 }  // namespace rgbdflow

@@ -347,6 +347,16 @@ __global__ void __launch_bounds__(kBlock) k_depth_map_level(const double* L, int
   dst[l] = n2 > 1 ? (double)(sum2 / (float)n2) : c;
 }
 
+// Frame ingestion (ceLiuOpticalFlow.cpp:19-47 cv2liuBGR / cv2liuDepth): u8
+// BGR [N][3] -> planes of c / 255.0 (no channel swap), f32 depth -> f64.
+__global__ void __launch_bounds__(kBlock) k_ingest(const unsigned char* bgr, const float* depth, int N, double* im,
+                                                   double* dm) {
+  const int l = blockIdx.x * kBlock + threadIdx.x;
+  if (l >= N) return;
+  for (int c = 0; c < 3; c++) im[(size_t)c * N + l] = bgr[(size_t)l * 3 + c] / 255.0;
+  dm[l] = depth[l];
+}
+
 inline int blocks_for(size_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -356,6 +366,9 @@ void launch_deinterleave(hipStream_t s, const double* src, int N, int C, double*
 }
 void launch_interleave(hipStream_t s, const double* src, int N, int C, double* dst) {
   hipLaunchKernelGGL(k_interleave, dim3(blocks_for(N)), dim3(kBlock), 0, s, src, N, C, dst);
+}
+void launch_ingest(hipStream_t s, const unsigned char* bgr, const float* depth, int N, double* im, double* dm) {
+  hipLaunchKernelGGL(k_ingest, dim3(blocks_for(N)), dim3(kBlock), 0, s, bgr, depth, N, im, dm);
 }
 void launch_pair_prep(hipStream_t s, const WorkList<PairPrep>& w, int blocks) {
   hipLaunchKernelGGL(k_pair_prep, dim3(blocks), dim3(kBlock), 0, s, w);

@@ -299,13 +299,14 @@ void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, i
                         int slot_base, int n_prev_checks, int check_slot) {
   const TiledConfig& c = kConfigs[config];
   const size_t lds = tiled_lds_bytes(slots);
-  static size_t raised[16] = {};
+  // per-thread caches: launches may come from several host threads (one per context)
+  thread_local size_t raised[16] = {};
   if (lds > 65536 && lds > raised[config]) {
     hipFuncSetAttribute(reinterpret_cast<const void*>(c.fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     raised[config] = lds;
   }
   // resident workgroups per CU for this shape (cached per configuration and LDS size)
-  static int occ_cfg[16] = {}, occ_lds[16] = {}, occ_val[16] = {}, cus = 0;
+  thread_local int occ_cfg[16] = {}, occ_lds[16] = {}, occ_val[16] = {}, cus = 0;
   if (occ_lds[config] != (int)lds || occ_cfg[config] == 0) {
     int n = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(c.fn), c.nt, lds) !=

@@ -1,0 +1,465 @@
+// server.cpp -- serving runtime: continuous batching of RGB-D frame streams.
+//
+// Each stream keeps its last frame on the device (colour and depth planes,
+// boundary maps, and the colour and depth pyramids it got as a pair's frame
+// 2), so a new frame is uploaded once and the pair (last frame, new frame)
+// reuses frame 1's pyramids.  A worker thread forms batches from pending
+// requests -- at most one pair per stream in flight, at most max_batch pairs,
+// admitted while the in-flight pixels fit the HBM budget -- and runs them on
+// `lanes` executor contexts with their own HIP streams, so a batch formed
+// while another is on the GPU runs beside it.  Results do not depend on how
+// requests are batched: every pair's solve is the same whatever else is in
+// its batch.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/rgbdflow/rgbdflow.h"
+#include "capi_internal.hpp"
+#include "internal.hpp"
+
+using namespace rgbdflow;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// Device bytes an in-flight pair uses per pixel in an executor context
+// (inputs, pyramids, work buffers; see capi.cpp's pair slots).
+constexpr double kPairBytesPerPixel = 640.0;
+
+struct Frame {
+  double *im = nullptr, *dm = nullptr, *pim = nullptr, *pd = nullptr;
+  float* hpb = nullptr;
+  bool pyr = false;  // pim / pd hold this frame's pyramids
+};
+
+struct StreamState {
+  int W = 0, H = 0, min_width = 0;
+  rgbdflow_camera cam{};
+  size_t N = 0, rest = 0, hpb_n = 0;
+  Frame f[2];
+  int cur = -1;          // index of the last frame, -1 before the first
+  bool busy = false;     // a request of this stream is in a batch
+  int queued = 0;        // requests of this stream waiting
+  bool closing = false;
+  unsigned char* stage_bgr = nullptr;
+  float* stage_depth = nullptr;
+  void* mem = nullptr;
+  size_t bytes = 0;
+};
+
+struct Request {
+  unsigned long long ticket = 0;
+  int stream = -1;
+  StreamState* st = nullptr;  // set when picked (the stream stays open while busy)
+  std::vector<unsigned char> bgr;
+  std::vector<float> depth, hpb;
+  Clock::time_point t_submit;
+  int next = 0;  // frame slot the new frame goes to
+};
+
+struct Result {
+  bool done = false;
+  std::vector<double> u, v, w;
+  rgbdflow_request_info info{};
+};
+
+struct Lane {
+  rgbdflow_ctx* ctx = nullptr;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  std::vector<Request> reqs;     // in dispatch order
+  std::vector<int> pair_of;      // request -> pair index in the batch, -1 for a first frame
+  int status = RGBDFLOW_OK;
+  size_t px = 0;
+  Clock::time_point t_start;
+};
+
+}  // namespace
+
+struct rgbdflow_server {
+  int device = 0;
+  int max_batch = 16;
+  rgbdflow_params params{};
+  unsigned long long budget = 0;
+  unsigned long long state_bytes = 0;
+  size_t inflight_px = 0;
+  std::vector<Lane> lanes;
+  std::map<int, std::unique_ptr<StreamState>> streams;
+  std::deque<Request> pending;
+  std::map<unsigned long long, Result> results;
+  unsigned long long next_ticket = 1;
+  int next_stream = 0;
+  std::mutex m;
+  std::condition_variable cv_work, cv_done;
+  std::thread worker;
+  bool stop = false;
+  std::string err;
+};
+
+namespace {
+
+int sfail(rgbdflow_server* s, int code, const std::string& msg) {
+  if (s) s->err = msg;
+  return code;
+}
+
+size_t budget_px(const rgbdflow_server* s) {
+  if (!s->budget) return (size_t)-1;
+  if (s->state_bytes >= s->budget) return 0;
+  return (size_t)((double)(s->budget - s->state_bytes) / kPairBytesPerPixel);
+}
+
+// Uploads and converts the new frames, loads the pairs and enqueues the batch
+// on the lane's stream (worker thread, no lock held; the streams involved are
+// busy, so nothing else touches their state).
+void dispatch(rgbdflow_server* S, Lane& ln) {
+  hipStream_t hs = ctx_stream(ln.ctx);
+  std::vector<PairDev> pairs;
+  ln.pair_of.assign(ln.reqs.size(), -1);
+  ln.status = RGBDFLOW_OK;
+  for (size_t r = 0; r < ln.reqs.size(); r++) {
+    Request& q = ln.reqs[r];
+    StreamState& st = *q.st;
+    Frame& F = st.f[q.next];
+    hipMemcpyAsync(st.stage_bgr, q.bgr.data(), q.bgr.size(), hipMemcpyHostToDevice, hs);
+    hipMemcpyAsync(st.stage_depth, q.depth.data(), q.depth.size() * 4, hipMemcpyHostToDevice, hs);
+    launch_ingest(hs, st.stage_bgr, st.stage_depth, (int)st.N, F.im, F.dm);
+    if (!q.hpb.empty()) hipMemcpyAsync(F.hpb, q.hpb.data(), q.hpb.size() * 4, hipMemcpyHostToDevice, hs);
+    F.pyr = false;
+    if (st.cur >= 0) {
+      const Frame& P = st.f[st.cur];
+      PairDev pd{};
+      pd.W = st.W;
+      pd.H = st.H;
+      pd.min_width = st.min_width;
+      pd.cam = st.cam;
+      pd.f1 = FrameDev{P.im, P.dm, P.pyr ? P.pim : nullptr, P.pyr ? P.pd : nullptr, q.hpb.empty() ? nullptr : P.hpb};
+      pd.f2 = FrameDev{F.im, F.dm, nullptr, nullptr, nullptr};
+      ln.pair_of[r] = (int)pairs.size();
+      pairs.push_back(pd);
+    }
+  }
+  if (!pairs.empty()) {
+    int rc = batch_load_device(ln.ctx, &S->params, (int)pairs.size(), pairs.data());
+    if (rc == RGBDFLOW_OK) rc = rgbdflow_batch_run(ln.ctx, &S->params, nullptr);
+    ln.status = rc;
+  }
+  hipEventRecord(ln.done, hs);
+  ln.t_start = Clock::now();
+}
+
+// Collects a finished lane: keeps each stream's new frame with its pyramids,
+// downloads the flows, and publishes the results.
+void finish(rgbdflow_server* S, Lane& ln) {
+  hipStream_t hs = ctx_stream(ln.ctx);
+  const Clock::time_point t_end = Clock::now();
+  std::vector<Result> out(ln.reqs.size());
+  for (size_t r = 0; r < ln.reqs.size(); r++) {
+    const Request& q = ln.reqs[r];
+    StreamState& st = *q.st;
+    Result& res = out[r];
+    res.info.stream = q.stream;
+    res.info.status = ln.status;
+    res.info.batch_pairs = 0;
+    if (ln.pair_of[r] < 0 || ln.status != RGBDFLOW_OK) continue;
+    FrameDev f2;
+    const double* flow;
+    size_t rest;
+    if (pair_outputs(ln.ctx, ln.pair_of[r], &f2, &flow, &rest) != RGBDFLOW_OK) {
+      res.info.status = RGBDFLOW_ERR_STATE;
+      continue;
+    }
+    Frame& F = st.f[q.next];
+    if (rest > 0) {
+      hipMemcpyAsync(F.pim, f2.pim, 3 * rest * 8, hipMemcpyDeviceToDevice, hs);
+      hipMemcpyAsync(F.pd, f2.pd, rest * 8, hipMemcpyDeviceToDevice, hs);
+    }
+    F.pyr = true;
+    res.u.resize(st.N);
+    res.v.resize(st.N);
+    res.w.resize(st.N);
+    hipMemcpyAsync(res.u.data(), flow, st.N * 8, hipMemcpyDeviceToHost, hs);
+    hipMemcpyAsync(res.v.data(), flow + st.N, st.N * 8, hipMemcpyDeviceToHost, hs);
+    hipMemcpyAsync(res.w.data(), flow + 2 * st.N, st.N * 8, hipMemcpyDeviceToHost, hs);
+    res.info.has_flow = 1;
+  }
+  const hipError_t e = hipStreamSynchronize(hs);
+  const Clock::time_point t_done = Clock::now();
+  int n_pairs = 0;
+  for (int p : ln.pair_of) n_pairs += p >= 0;
+  std::lock_guard<std::mutex> lk(S->m);
+  for (size_t r = 0; r < ln.reqs.size(); r++) {
+    const Request& q = ln.reqs[r];
+    StreamState& st = *q.st;
+    st.cur = q.next;
+    st.busy = false;
+    Result& res = S->results[q.ticket];
+    res.u.swap(out[r].u);
+    res.v.swap(out[r].v);
+    res.w.swap(out[r].w);
+    res.info = out[r].info;
+    if (e != hipSuccess) res.info.status = RGBDFLOW_ERR_HIP;
+    res.info.batch_pairs = n_pairs;
+    res.info.queue_ms = ms_between(q.t_submit, ln.t_start);
+    res.info.batch_ms = ms_between(ln.t_start, t_end);
+    res.info.latency_ms = ms_between(q.t_submit, t_done);
+    res.done = true;
+  }
+  S->inflight_px -= ln.px;
+  ln.busy = false;
+  ln.reqs.clear();
+  S->cv_done.notify_all();
+}
+
+// Picks the requests of the next batch (lock held): in submission order, one
+// per stream, while the pixels fit the budget.
+void pick(rgbdflow_server* S, Lane& ln) {
+  const size_t cap = budget_px(S);
+  size_t px = 0;
+  int n_pairs = 0;
+  for (auto it = S->pending.begin(); it != S->pending.end() && n_pairs < S->max_batch;) {
+    StreamState& st = *S->streams[it->stream];
+    if (st.busy) {
+      ++it;
+      continue;
+    }
+    const bool pair = st.cur >= 0;
+    if (pair && S->inflight_px + px + st.N > cap && (n_pairs > 0 || S->inflight_px > 0)) break;
+    it->next = st.cur < 0 ? 0 : 1 - st.cur;
+    it->st = &st;
+    st.busy = true;
+    st.queued--;
+    if (pair) {
+      px += st.N;
+      n_pairs++;
+    }
+    ln.reqs.push_back(std::move(*it));
+    it = S->pending.erase(it);
+  }
+  ln.px = px;
+  S->inflight_px += px;
+}
+
+void worker_main(rgbdflow_server* S) {
+  hipSetDevice(S->device);
+  std::unique_lock<std::mutex> lk(S->m);
+  for (;;) {
+    bool idle = true, progressed = false;
+    for (Lane& ln : S->lanes) {
+      if (!ln.busy) continue;
+      idle = false;
+      if (hipEventQuery(ln.done) == hipSuccess) {
+        lk.unlock();
+        finish(S, ln);
+        lk.lock();
+        progressed = true;
+      }
+    }
+    for (Lane& ln : S->lanes) {
+      if (ln.busy || S->pending.empty()) continue;
+      pick(S, ln);
+      if (ln.reqs.empty()) continue;
+      ln.busy = true;
+      idle = false;
+      lk.unlock();
+      dispatch(S, ln);
+      lk.lock();
+      progressed = true;
+    }
+    if (S->stop && idle && S->pending.empty()) break;
+    if (!progressed) S->cv_work.wait_for(lk, std::chrono::microseconds(100));
+  }
+}
+
+void free_stream(StreamState& st) {
+  if (st.mem) hipFree(st.mem);
+  st.mem = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rgbdflow_server_create(rgbdflow_server** out, const rgbdflow_server_config* cfg, const rgbdflow_params* params) {
+  if (!out || !cfg || !params) return RGBDFLOW_ERR_ARG;
+  *out = nullptr;
+  if (params->variant != RGBDFLOW_VARIANT_FULL) return RGBDFLOW_ERR_UNSUPPORTED;
+  std::unique_ptr<rgbdflow_server> S(new rgbdflow_server());
+  S->device = cfg->device;
+  S->max_batch = cfg->max_batch > 0 ? cfg->max_batch : 16;
+  S->budget = cfg->hbm_budget;
+  S->params = *params;
+  const int n_lanes = cfg->lanes > 0 ? cfg->lanes : 2;
+  S->lanes.resize(n_lanes);
+  for (Lane& ln : S->lanes) {
+    int rc = rgbdflow_create(&ln.ctx, cfg->device, 0);
+    if (rc != RGBDFLOW_OK) {
+      for (Lane& l2 : S->lanes)
+        if (l2.ctx) rgbdflow_destroy(l2.ctx);
+      return rc;
+    }
+    if (hipEventCreateWithFlags(&ln.done, hipEventDisableTiming) != hipSuccess) return RGBDFLOW_ERR_HIP;
+  }
+  rgbdflow_server* raw = S.release();
+  raw->worker = std::thread(worker_main, raw);
+  *out = raw;
+  return RGBDFLOW_OK;
+}
+
+int rgbdflow_server_destroy(rgbdflow_server* S) {
+  if (!S) return RGBDFLOW_ERR_ARG;
+  {
+    std::lock_guard<std::mutex> lk(S->m);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  hipSetDevice(S->device);
+  for (Lane& ln : S->lanes) {
+    if (ln.done) hipEventDestroy(ln.done);
+    if (ln.ctx) rgbdflow_destroy(ln.ctx);
+  }
+  for (auto& kv : S->streams) free_stream(*kv.second);
+  delete S;
+  return RGBDFLOW_OK;
+}
+
+const char* rgbdflow_server_last_error(const rgbdflow_server* S) { return S ? S->err.c_str() : "null server"; }
+
+int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera* cam, int min_width, int* stream_id) {
+  if (!S) return RGBDFLOW_ERR_ARG;
+  if (W < 2 || H < 2 || (long long)W * H > (1LL << 28) || !stream_id)
+    return sfail(S, RGBDFLOW_ERR_ARG, "bad stream arguments");
+  std::unique_ptr<StreamState> st(new StreamState());
+  st->W = W;
+  st->H = H;
+  st->min_width = min_width > 0 ? min_width : S->params.min_width;
+  st->cam = cam ? *cam : S->params.cam;
+  if ((int)st->cam.xRes != W || (int)st->cam.yRes != H)
+    return sfail(S, RGBDFLOW_ERR_ARG, "camera resolution differs from the stream size");
+  st->N = (size_t)W * H;
+  st->rest = pyramid_rest_elems(W, H, S->params.pyramid_ratio, st->min_width);
+  st->hpb_n = pyramid_all_elems(W, H, S->params.pyramid_ratio, st->min_width) * 2;
+  // one allocation: two frames (planes, pyramids, boundary maps) and the upload staging
+  auto r256 = [](size_t n) { return (n + 255) & ~size_t(255); };
+  const size_t frame_bytes = r256(3 * st->N * 8) + r256(st->N * 8) + r256(3 * st->rest * 8 + 8) +
+                             r256(st->rest * 8 + 8) + r256(st->hpb_n * 4);
+  const size_t bytes = 2 * frame_bytes + r256(3 * st->N) + r256(st->N * 4);
+  {
+    std::lock_guard<std::mutex> lk(S->m);
+    if (S->budget && S->state_bytes + bytes > S->budget)
+      return sfail(S, RGBDFLOW_ERR_BUDGET, "stream state exceeds the HBM budget");
+  }
+  hipSetDevice(S->device);
+  if (hipMalloc(&st->mem, bytes) != hipSuccess) return sfail(S, RGBDFLOW_ERR_HIP, "hipMalloc of the stream state");
+  st->bytes = bytes;
+  char* p = static_cast<char*>(st->mem);
+  auto take = [&](size_t n) {
+    char* q = p;
+    p += r256(n);
+    return q;
+  };
+  for (Frame& F : st->f) {
+    F.im = reinterpret_cast<double*>(take(3 * st->N * 8));
+    F.dm = reinterpret_cast<double*>(take(st->N * 8));
+    F.pim = reinterpret_cast<double*>(take(3 * st->rest * 8 + 8));
+    F.pd = reinterpret_cast<double*>(take(st->rest * 8 + 8));
+    F.hpb = reinterpret_cast<float*>(take(st->hpb_n * 4));
+  }
+  st->stage_bgr = reinterpret_cast<unsigned char*>(take(3 * st->N));
+  st->stage_depth = reinterpret_cast<float*>(take(st->N * 4));
+  std::lock_guard<std::mutex> lk(S->m);
+  S->state_bytes += st->bytes;
+  *stream_id = S->next_stream++;
+  S->streams[*stream_id] = std::move(st);
+  return RGBDFLOW_OK;
+}
+
+int rgbdflow_stream_submit(rgbdflow_server* S, int stream_id, const unsigned char* bgr, const float* depth,
+                           const float* hpb, unsigned long long* ticket) {
+  if (!S) return RGBDFLOW_ERR_ARG;
+  if (!bgr || !depth || !ticket) return sfail(S, RGBDFLOW_ERR_ARG, "null frame");
+  if (S->params.regularization_type == RGBDFLOW_REG_HPB && !hpb)
+    return sfail(S, RGBDFLOW_ERR_ARG, "HPB regularisation needs the frame's boundary maps");
+  Request q;
+  size_t N, hn;
+  {
+    std::lock_guard<std::mutex> lk(S->m);
+    auto it = S->streams.find(stream_id);
+    if (it == S->streams.end() || it->second->closing) return sfail(S, RGBDFLOW_ERR_ARG, "no such stream");
+    N = it->second->N;
+    hn = it->second->hpb_n;
+  }
+  q.bgr.assign(bgr, bgr + 3 * N);
+  q.depth.assign(depth, depth + N);
+  if (hpb) q.hpb.assign(hpb, hpb + hn);
+  q.stream = stream_id;
+  q.t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(S->m);
+    q.ticket = S->next_ticket++;
+    *ticket = q.ticket;
+    S->results[q.ticket] = Result();
+    S->streams[stream_id]->queued++;
+    S->pending.push_back(std::move(q));
+  }
+  S->cv_work.notify_all();
+  return RGBDFLOW_OK;
+}
+
+int rgbdflow_stream_poll(rgbdflow_server* S, unsigned long long ticket, int wait_ms, double* u, double* v, double* w,
+                         rgbdflow_request_info* info) {
+  if (!S) return RGBDFLOW_ERR_ARG;
+  std::unique_lock<std::mutex> lk(S->m);
+  auto it = S->results.find(ticket);
+  if (it == S->results.end()) return sfail(S, RGBDFLOW_ERR_ARG, "unknown ticket");
+  if (!it->second.done && wait_ms != 0) {
+    auto ready = [&] { return S->results[ticket].done; };
+    if (wait_ms < 0) S->cv_done.wait(lk, ready);
+    else S->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), ready);
+    it = S->results.find(ticket);
+  }
+  if (!it->second.done) return 0;
+  Result& r = it->second;
+  if (info) *info = r.info;
+  if (r.info.has_flow) {
+    if (u) std::memcpy(u, r.u.data(), r.u.size() * 8);
+    if (v) std::memcpy(v, r.v.data(), r.v.size() * 8);
+    if (w) std::memcpy(w, r.w.data(), r.w.size() * 8);
+  }
+  const int status = r.info.status;
+  S->results.erase(it);
+  if (status != RGBDFLOW_OK) return sfail(S, status, "the batch of this request failed");
+  return 1;
+}
+
+int rgbdflow_stream_close(rgbdflow_server* S, int stream_id) {
+  if (!S) return RGBDFLOW_ERR_ARG;
+  std::unique_lock<std::mutex> lk(S->m);
+  auto it = S->streams.find(stream_id);
+  if (it == S->streams.end()) return sfail(S, RGBDFLOW_ERR_ARG, "no such stream");
+  StreamState* st = it->second.get();
+  st->closing = true;
+  S->cv_done.wait(lk, [&] { return !st->busy && st->queued == 0; });
+  S->state_bytes -= st->bytes;
+  free_stream(*st);
+  S->streams.erase(it);
+  return RGBDFLOW_OK;
+}
+
+}  // extern "C"

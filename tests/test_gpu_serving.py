@@ -1,0 +1,89 @@
+"""The serving runtime (rgbdflow_server_*): frame streams of different sizes
+submitted together, solved in continuously formed batches on two lanes; each
+pair's flow equals the same pair solved alone (rgbdflow_multiscale), bit for
+bit, however the requests were batched."""
+import numpy as np
+import pytest
+
+from helpers import load_package, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def R():
+    return load_package()
+
+
+def _stream_frames(S, W, H, seed):
+    """Two frames A, B of a synthetic scene, in the submission format (u8 BGR,
+    f32 depth), with boundary maps for each as a frame 1."""
+    pair = S.make_pair(W, H, seed, 4)
+    im1, d1, im2, d2 = S.to_solver_inputs(pair)
+    frames = [(pair["img1"], d1), (pair["img2"], d2)]
+    return frames, pair
+
+
+def _maps(R, S, pair, W, H, mw, shift):
+    lv = R.pyramid_levels(W, H, 0.5, mw)
+    seg = np.roll(pair["seg1"], shift, axis=1)  # a different (still valid) map per frame
+    return S.segment_boundary_maps(seg, [a for a, b in lv], [b for a, b in lv])
+
+
+def _alone(R, ctx, p, W, H, mw, f1, f2, hpb1):
+    q = R.make_params(W, H, n_outer=p.n_outer, n_sor=p.n_sor, min_width=mw, relax_variant=p.relax_variant)
+    im1 = f1[0].astype(np.float64) / 255.0
+    im2 = f2[0].astype(np.float64) / 255.0
+    u, v, w, _ = ctx.multiscale(q, im1, f1[1].astype(np.float64), im2, f2[1].astype(np.float64), None, hpb1)
+    return u, v, w
+
+
+@pytest.mark.parametrize("max_batch,lanes", [(8, 2), (2, 2), (1, 1)])
+def test_serving_results_equal_alone(R, max_batch, lanes):
+    S = synth()
+    sizes = [(128, 96, 16), (160, 120, 20), (96, 72, 12)]
+    p = R.make_params(128, 96, n_outer=3, n_sor=25, min_width=16, relax_variant=R.RELAX_TILED)
+    streams = []
+    with R.Server(p, max_batch=max_batch, lanes=lanes) as srv:
+        for k, (W, H, mw) in enumerate(sizes):
+            frames, pair = _stream_frames(S, W, H, 200 + k)
+            maps = [_maps(R, S, pair, W, H, mw, sh) for sh in (0, 3)]
+            sid = srv.open(W, H, R.make_params(W, H).cam, mw)
+            streams.append((sid, W, H, mw, frames, maps))
+        # frames A, B, A per stream, interleaved across streams
+        tickets = []
+        for t, fi in enumerate([0, 1, 0]):
+            for (sid, W, H, mw, frames, maps) in streams:
+                tickets.append((sid, t, fi, srv.submit(sid, frames[fi][0], frames[fi][1], maps[fi])))
+        results = {}
+        for (sid, t, fi, tk) in tickets:
+            print(f"poll stream {sid} frame {t}", flush=True)
+            info, flow = srv.poll(tk, wait_ms=60000)
+            assert info["status"] == 0
+            assert info["has_flow"] == (t > 0)
+            results[(sid, t)] = flow
+    with R.Context(0) as ctx:
+        for (sid, W, H, mw, frames, maps) in streams:
+            for t, (a, b) in [(1, (0, 1)), (2, (1, 0))]:
+                u, v, w = _alone(R, ctx, p, W, H, mw, frames[a], frames[b], maps[a])
+                ub, vb, wb = results[(sid, t)]
+                assert np.array_equal(u, ub) and np.array_equal(v, vb) and np.array_equal(w, wb), (sid, t)
+
+
+def test_serving_budget_and_errors(R):
+    S = synth()
+    p = R.make_params(96, 72, n_outer=2, n_sor=25, min_width=12, reg=R.REG_LETOUZEY)
+    # a budget that holds the stream state and about one pair in flight at a time
+    with R.Server(p, max_batch=8, lanes=2, hbm_budget=8 << 20) as srv:
+        frames, pair = _stream_frames(S, 96, 72, 300)
+        sids = [srv.open(96, 72, None, 12) for _ in range(3)]
+        tk = [srv.submit(s, frames[i % 2][0], frames[i % 2][1]) for i in range(2) for s in sids]
+        for t in tk:
+            info, flow = srv.poll(t, wait_ms=60000)
+            assert info["status"] == 0
+        with pytest.raises(R.RgbdFlowError):
+            srv.submit(12345, frames[0][0], frames[0][1])   # no such stream
+        with pytest.raises(R.RgbdFlowError):
+            srv.open(96, 71, None, 12)                      # camera size differs (params.cam is 96x72)
+        for s in sids:
+            srv.close_stream(s)
