@@ -147,6 +147,11 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel instead of replaying a HIP graph")
     ap.add_argument("--mixed", type=int, default=0,
                     help="instead: a batch of this many pairs of mixed sizes (320x240 .. 1280x720)")
+    ap.add_argument("--serve", default="",
+                    help="instead: serving, streams as 'WxH:count,...' (e.g. 320x240:4,640x480:4,960x540:4,1280x720:4)")
+    ap.add_argument("--fps", type=float, default=30.0, help="serving: frames per second per stream (Poisson)")
+    ap.add_argument("--seconds", type=float, default=3.0, help="serving: arrival window")
+    ap.add_argument("--max-batch", type=int, default=16, help="serving: pairs per batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: the scene-flow solve runs on one GPU "
@@ -156,6 +161,8 @@ def main():
     R = _package()
     if args.mixed:
         return mixed_main(R, args)
+    if args.serve:
+        return serve_main(R, args)
     pairs, levels = workload(R, max(1, min(args.distinct, args.batch)))
     B = args.batch
     batch = [pairs[i % len(pairs)] for i in range(B)]
@@ -331,6 +338,89 @@ def mixed_main(R, args):
                    "graph": not args.no_graph},
     }
     ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def serve_main(R, args):
+    """Serving: streams submit frames at Poisson arrival times; every frame after
+    a stream's first is a pair solved by the server's continuous batching.
+    Reports offered and achieved pairs/s and per-request latency."""
+    import threading
+    import rgbdflow_amd.synth as S
+    spec = []
+    for tok in args.serve.split(","):
+        size, cnt = tok.split(":")
+        w, h = (int(x) for x in size.split("x"))
+        spec += [(w, h)] * int(cnt)
+    sizes = sorted(set(spec), key=lambda t: -t[0])
+    frames = {}
+    for (w, h) in sizes:   # two frames per size, alternated by every stream of that size
+        f = 525.0 * w / 640
+        pr = S.make_pair(w, h, 500 + w, n_segments=6, cam=S.Camera(w, h, f, f, (w - 1) / 2, (h - 1) / 2))
+        _, d1, _, d2 = S.to_solver_inputs(pr)
+        lv = R.pyramid_levels(w, h, 0.5, w // 16)
+        hpb = S.segment_boundary_maps(pr["seg1"], [a for a, b in lv], [b for a, b in lv])
+        frames[(w, h)] = [(pr["img1"], d1, hpb), (pr["img2"], d2, hpb)]
+    p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED)
+    srv = R.Server(p, max_batch=args.max_batch, lanes=2)
+    if not args.no_autotune:
+        with R.Context(0) as tctx:
+            for (w, h) in sizes:
+                for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
+                    srv.set_relax_schedule(lw, lh, tctx.autotune_relax(lw, lh, batch=max(1, args.max_batch // 2)))
+    sids = [srv.open(w, h, R.make_params(w, h).cam, w // 16) for (w, h) in spec]
+    rng = np.random.default_rng(7)
+    events = []
+    for k, (w, h) in enumerate(spec):
+        t = rng.exponential(1 / args.fps)
+        n = 0
+        while t < args.seconds:
+            events.append((t, k, n))
+            t += rng.exponential(1 / args.fps)
+            n += 1
+    events.sort()
+    tickets = []
+    print(f"bench: serving {len(spec)} streams, {len(events)} frames over {args.seconds} s", file=sys.stderr,
+          flush=True)
+
+    def submitter():
+        t0 = time.perf_counter()
+        for (t, k, n) in events:
+            dt = t - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            img, dep, hpb = frames[spec[k]][n % 2]
+            tickets.append((k, n, srv.submit(sids[k], img, dep, hpb)))
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    lat, qms, bsz = [], [], []
+    for (k, n, tk) in tickets:
+        info, _ = srv.poll(tk, wait_ms=-1)
+        if info["has_flow"]:
+            lat.append(info["latency_ms"])
+            qms.append(info["queue_ms"])
+            bsz.append(info["batch_pairs"])
+    wall = time.perf_counter() - t0
+    srv.close()
+    pairs = len(lat)
+    out = {
+        "metric": "served scene-flow pairs/s and request latency",
+        "value": round(pairs / wall, 3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {"streams": args.serve, "fps_per_stream": args.fps, "arrival_seconds": args.seconds,
+                   "offered_pairs_per_s": round(pairs / args.seconds, 2), "max_batch": args.max_batch,
+                   "lanes": 2, "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                   "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                   "queue_p50_ms": round(float(np.percentile(qms, 50)), 2),
+                   "mean_batch_pairs": round(float(np.mean(bsz)), 2), "wall_s": round(wall, 2)},
+    }
     print(json.dumps(out), flush=True)
 
 

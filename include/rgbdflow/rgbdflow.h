@@ -285,6 +285,8 @@ int rgbdflow_stream_poll(rgbdflow_server* srv, unsigned long long ticket, int wa
                          double* w, rgbdflow_request_info* info);
 /* waits for the stream's requests, then frees its state */
 int rgbdflow_stream_close(rgbdflow_server* srv, int stream_id);
+/* rgbdflow_set_relax_schedule on every lane (waits until no batch is in flight) */
+int rgbdflow_server_set_relax_schedule(rgbdflow_server* srv, int W, int H, const rgbdflow_tile_choice* choice);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,7 @@ EXPORTS = [
     "rgbdflow_batch_run", "rgbdflow_batch_read", "rgbdflow_set_relax_schedule",
     "rgbdflow_time_relax_schedule", "rgbdflow_server_create", "rgbdflow_server_destroy",
     "rgbdflow_server_last_error", "rgbdflow_stream_open", "rgbdflow_stream_submit", "rgbdflow_stream_poll",
-    "rgbdflow_stream_close",
+    "rgbdflow_stream_close", "rgbdflow_server_set_relax_schedule",
 ]
 
 
@@ -441,6 +441,13 @@ class Server:
         if rc == 0:
             return None
         return info.as_dict(), ((u, v, w) if info.has_flow else None)
+
+    def set_relax_schedule(self, W, H, choice: dict):
+        ch = TileChoice()
+        for f, _ in TileChoice._fields_:
+            if f in choice:
+                setattr(ch, f, choice[f])
+        self._check(lib().rgbdflow_server_set_relax_schedule(self.h, W, H, C.byref(ch)))
 
     def close_stream(self, stream):
         self._check(lib().rgbdflow_stream_close(self.h, stream))

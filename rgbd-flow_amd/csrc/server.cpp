@@ -448,6 +448,24 @@ int rgbdflow_stream_poll(rgbdflow_server* S, unsigned long long ticket, int wait
   return 1;
 }
 
+int rgbdflow_server_set_relax_schedule(rgbdflow_server* S, int W, int H, const rgbdflow_tile_choice* choice) {
+  if (!S) return RGBDFLOW_ERR_ARG;
+  std::unique_lock<std::mutex> lk(S->m);
+  // the lanes' contexts are touched only by the worker, and only while it has
+  // work: wait until it has none (it cannot get any while the lock is held)
+  S->cv_done.wait(lk, [&] {
+    if (!S->pending.empty()) return false;
+    for (const Lane& ln : S->lanes)
+      if (ln.busy) return false;
+    return true;
+  });
+  for (Lane& ln : S->lanes) {
+    const int rc = rgbdflow_set_relax_schedule(ln.ctx, W, H, choice);
+    if (rc != RGBDFLOW_OK) return sfail(S, rc, rgbdflow_last_error(ln.ctx));
+  }
+  return RGBDFLOW_OK;
+}
+
 int rgbdflow_stream_close(rgbdflow_server* S, int stream_id) {
   if (!S) return RGBDFLOW_ERR_ARG;
   std::unique_lock<std::mutex> lk(S->m);
