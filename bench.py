@@ -30,6 +30,7 @@ PKG_DIR = os.path.join(ROOT, "rgbd-flow_amd")
 METRIC = "640×480 scene-flow pairs/s + p99 latency, batched vs plain; HBM GB/s"
 W, H = 640, 480
 B_SWEEP = 72.0     # algorithmic bytes per pixel per red-black iteration (f32 system + increments)
+B_SWEEP_16 = 48.0  # the same with the 12 system values stored in 16 bits
 B_STAGE = 128.0    # algorithmic bytes per pixel per outer iteration (warp, gradients, weights, assembly)
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "relax_tiled_traffic.json")
@@ -86,17 +87,19 @@ def mixed_workload(R, n, n_distinct, seed=100):
     return pairs, sizes
 
 
-def params_for(R, variant, graph=True):
+def params_for(R, variant, graph=True, storage="f32"):
+    st = {"f32": R.STORE_F32, "f16": R.STORE_F16, "bf16": R.STORE_BF16}[storage]
     return R.make_params(W, H, f=525.0, cx=319.5, cy=239.5, n_outer=15, n_sor=50, min_width=40,
-                         relax_variant=R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN, graph=graph)
+                         relax_variant=R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN, graph=graph,
+                         storage=st)
 
 
-def algorithmic_bytes(stats):
+def algorithmic_bytes(stats, b_sweep=B_SWEEP):
     sweep = stage = 0.0
     for st in stats:
         for k in range(st.n_levels):
             n = st.width[k] * st.height[k]
-            sweep += n * st.sweeps_exec[k] * B_SWEEP
+            sweep += n * st.sweeps_exec[k] * b_sweep
             stage += n * st.outer_exec[k] * B_STAGE
     return sweep, stage
 
@@ -145,6 +148,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel instead of replaying a HIP graph")
+    ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
+                    help="storage of the per-pixel systems (f32: the reference's precision)")
     ap.add_argument("--mixed", type=int, default=0,
                     help="instead: a batch of this many pairs of mixed sizes (320x240 .. 1280x720)")
     ap.add_argument("--serve", default="",
@@ -166,13 +171,13 @@ def main():
     pairs, levels = workload(R, max(1, min(args.distinct, args.batch)))
     B = args.batch
     batch = [pairs[i % len(pairs)] for i in range(B)]
-    p = params_for(R, args.variant, not args.no_graph)
+    p = params_for(R, args.variant, not args.no_graph, args.storage)
     ctx = R.Context(0)
 
     def autotune(nb):
         out = []
         for (w, h) in levels:
-            ch = ctx.autotune_relax(w, h, batch=nb)
+            ch = ctx.autotune_relax(w, h, batch=nb, params=p)
             raw.setdefault(nb, []).append(dict(W=w, H=h, **ch))
             out.append(dict(level=f"{w}x{h}", tile=f"{ch['tile_w']}x{ch['tile_h']}", iters=ch["tile_iters"],
                             threads=ch["threads"], pairs_per_thread=ch["pairs_per_thread"],
@@ -209,7 +214,7 @@ def main():
     ctx.set_relax_timing(True)
     rst = ctx.batch_run(p, stats=True)
     ctx.set_relax_timing(False)
-    sweep_bytes, stage_bytes = algorithmic_bytes(rst)
+    sweep_bytes, stage_bytes = algorithmic_bytes(rst, B_SWEEP if args.storage == "f32" else B_SWEEP_16)
     relax_ms = rst[0].relax_ms
     launches = rst[0].relax_launches
     achieved = sweep_bytes / (relax_ms / 1e3) / 1e9 if relax_ms > 0 else 0.0
@@ -243,7 +248,7 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if tr.get("batch") == B and tr.get("variant") == args.variant:
+        if tr.get("batch") == B and tr.get("variant") == args.variant and tr.get("storage", "f32") == args.storage:
             traffic = tr.get("bytes_per_launch")
 
     out = {
@@ -257,7 +262,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.storage == "f32" else f"f32 compute, {args.storage} system storage",
         "data": "synthetic (seeded generator, ground-truth flow)",
         "config": {
             "workload": f"{B} x 640x480 RGB-D pairs per step ({len(pairs)} distinct), 5-level pyramid ratio 0.5, "
@@ -367,7 +372,8 @@ def serve_main(R, args):
         with R.Context(0) as tctx:
             for (w, h) in sizes:
                 for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
-                    srv.set_relax_schedule(lw, lh, tctx.autotune_relax(lw, lh, batch=max(1, args.max_batch // 2)))
+                    srv.set_relax_schedule(lw, lh, tctx.autotune_relax(lw, lh, batch=max(1, args.max_batch // 2),
+                                                                       params=p))
     sids = [srv.open(w, h, R.make_params(w, h).cam, w // 16) for (w, h) in spec]
     rng = np.random.default_rng(7)
     events = []

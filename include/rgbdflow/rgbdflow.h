@@ -42,6 +42,14 @@ extern "C" {
 #define RGBDFLOW_RELAX_PLAIN 0   /* one launch per colour per iteration */
 #define RGBDFLOW_RELAX_TILED 1   /* several iterations per launch in LDS tiles */
 
+/* Storage of the per-pixel systems the relaxation reads.  F32 is the
+ * reference's precision; F16 (with a per-pixel power-of-two scale) and BF16
+ * halve the bytes the relaxation loads and are precision variants: the
+ * arithmetic stays f32, the results differ from the reference's. */
+#define RGBDFLOW_STORE_F32 0
+#define RGBDFLOW_STORE_F16 1
+#define RGBDFLOW_STORE_BF16 2
+
 /* rgbd::CameraParams (include/rgbd_util/CameraParams.h:17-28) */
 typedef struct rgbdflow_camera {
   unsigned int xRes, yRes;
@@ -73,7 +81,9 @@ typedef struct rgbdflow_params {
   int tile_config;               /* workgroup shape of the tiled relaxation (-1 = planner) */
   int graph;                     /* 1: batch runs replay their schedule as a captured HIP graph
                                     (captured at the second run with the same batch and params) */
-  int reserved[3];
+  int storage;                   /* RGBDFLOW_STORE_*: storage of the per-pixel systems */
+  float omega;                   /* relaxation factor (<= 0: the reference's 0.7, .cu:461) */
+  int reserved[1];
 } rgbdflow_params;
 
 typedef struct rgbdflow_stats {
@@ -231,18 +241,20 @@ typedef struct rgbdflow_tile_choice {
 
 /* Times candidate schedules of the tiled relaxation (workgroup shape, tile
  * shape, iterations per launch) on `batch` W x H levels relaxed by one launch
- * each, and makes the fastest the context's schedule for that shape whenever
- * tile_* are left to the planner (tile_iters = tile_w = tile_h = 0,
- * tile_config = -1).  out may be NULL. */
-int rgbdflow_autotune_relax(rgbdflow_ctx* ctx, int W, int H, int batch, rgbdflow_tile_choice* out);
+ * each, with systems stored as params->storage (params may be NULL: f32), and
+ * makes the fastest the context's schedule for that shape whenever tile_* are
+ * left to the planner (tile_iters = tile_w = tile_h = 0, tile_config = -1).
+ * out may be NULL. */
+int rgbdflow_autotune_relax(rgbdflow_ctx* ctx, int W, int H, int batch, const rgbdflow_params* params,
+                            rgbdflow_tile_choice* out);
 /* Makes a schedule (as autotune_relax reports it: tile_w, tile_h, tile_iters,
  * tile_config, single) the context's schedule for W x H levels, e.g. one
  * tuned earlier and stored by the caller. */
 int rgbdflow_set_relax_schedule(rgbdflow_ctx* ctx, int W, int H, const rgbdflow_tile_choice* choice);
 /* Device time per red-black iteration of a schedule on `batch` W x H levels,
  * measured as autotune_relax measures its candidates. */
-int rgbdflow_time_relax_schedule(rgbdflow_ctx* ctx, int W, int H, int batch, const rgbdflow_tile_choice* choice,
-                                 double* us_per_iter);
+int rgbdflow_time_relax_schedule(rgbdflow_ctx* ctx, int W, int H, int batch, const rgbdflow_params* params,
+                                 const rgbdflow_tile_choice* choice, double* us_per_iter);
 
 /* ---- serving: continuous batching of RGB-D frame streams ----
  * No counterpart in the reference, which solves one pair per call

@@ -30,7 +30,7 @@ class Params(C.Structure):
                 ("depth_vs_color_data_weight", C.c_double), ("alpha", C.c_double), ("gamma", C.c_double),
                 ("pyramid_ratio", C.c_double), ("min_width", C.c_int), ("n_outer", C.c_int),
                 ("n_inner", C.c_int), ("n_sor", C.c_int), ("cam", Camera), ("precision", C.c_int),
-                ("disable_inner_exit", C.c_int), ("disable_outer_exit", C.c_int)]
+                ("disable_inner_exit", C.c_int), ("disable_outer_exit", C.c_int), ("omega", C.c_float)]
 
 
 class Stats(C.Structure):
@@ -85,7 +85,7 @@ def _f32(a):
 
 def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
                 ratio=0.5, min_width=None, reg=REG_HPB, precision=PREC_REF, cw=1.0, dvc=1.0,
-                disable_inner_exit=False, disable_outer_exit=False) -> Params:
+                disable_inner_exit=False, disable_outer_exit=False, omega=0.7) -> Params:
     p = Params()
     p.regularization_type = reg
     p.color_data_weight = cw
@@ -102,6 +102,7 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.precision = precision
     p.disable_inner_exit = int(disable_inner_exit)
     p.disable_outer_exit = int(disable_outer_exit)
+    p.omega = omega
     return p
 
 

@@ -594,14 +594,15 @@ struct Solver {
   /* .cu:382-470: one colour of the red-black relaxation; returns the largest
    * |change| over the updated pixels (the check of .cu:914-918 splits exactly
    * into the two colours' changes). */
-  static CT relax_colour(const CT* coef, int W, int H, int colour, CT* x) {
+  static CT relax_colour(const CT* coef, int W, int H, int colour, CT* x, float omega_f = .7f) {
     const size_t N = (size_t)W * H;
     const CT* eRxy = coef + 0 * N; const CT* eRz = coef + 1 * N;
     const CT* eDxy = coef + 2 * N; const CT* eDz = coef + 3 * N;
     const CT* Buu = coef + 4 * N; const CT* Bvv = coef + 5 * N; const CT* Bww = coef + 6 * N;
     const CT* Buv = coef + 7 * N; const CT* Buw = coef + 8 * N; const CT* Bvw = coef + 9 * N;
     const CT* rhs[3] = {coef + 10 * N, coef + 11 * N, coef + 12 * N};
-    const CT omega = (CT).7;
+    // .cu:461 `const float omega = .7`; a variant factor is given as a float too
+    const CT omega = (CT)omega_f;
     const CT step = (CT).3;
     CT maxd = 0;
     for (int i = 0; i < H; i++)
@@ -677,8 +678,9 @@ struct Solver {
         penalties();
         assemble();
         for (int it = 0; it < p->n_sor; it++) {
-          const CT m0 = relax_colour(coef.data(), W, H, 0, du.data());
-          const CT m1 = relax_colour(coef.data(), W, H, 1, du.data());
+          const float om = p->omega > 0 ? p->omega : .7f;
+          const CT m0 = relax_colour(coef.data(), W, H, 0, du.data(), om);
+          const CT m1 = relax_colour(coef.data(), W, H, 1, du.data(), om);
           sweeps++;
           if (it % checkEvery == checkEvery - 1) {
             const CT md = std::max(m0, m1);

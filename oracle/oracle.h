@@ -38,6 +38,7 @@ typedef struct {
   int precision;      /* ORC_PREC_* */
   int disable_inner_exit; /* 1: never leave the relaxation loop early (parity runs) */
   int disable_outer_exit; /* 1: never leave the outer loop early */
+  float omega;            /* relaxation factor (.cu:461: 0.7; <= 0 means 0.7) */
 } orc_params;
 
 typedef struct {

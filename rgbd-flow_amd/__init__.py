@@ -20,6 +20,7 @@ ERR_ARG, ERR_HIP, ERR_BUDGET, ERR_STATE, ERR_UNSUPPORTED = -1, -2, -3, -4, -5
 REG_ISOTROPIC, REG_LETOUZEY, REG_HPB = 0, 1, 2
 VARIANT_FULL = 2
 RELAX_PLAIN, RELAX_TILED = 0, 1
+STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 
 # Every symbol include/rgbdflow/rgbdflow.h declares.
 EXPORTS = [
@@ -49,7 +50,7 @@ class Params(C.Structure):
                 ("min_width", C.c_int), ("n_outer", C.c_int), ("n_inner", C.c_int), ("n_sor", C.c_int),
                 ("relax_variant", C.c_int), ("inner_exit", C.c_int), ("outer_exit", C.c_int),
                 ("tile_iters", C.c_int), ("tile_w", C.c_int), ("tile_h", C.c_int), ("tile_config", C.c_int),
-                ("graph", C.c_int), ("reserved", C.c_int * 3)]
+                ("graph", C.c_int), ("storage", C.c_int), ("omega", C.c_float), ("reserved", C.c_int * 1)]
 
 
 class Stats(C.Structure):
@@ -125,7 +126,8 @@ class RgbdFlowError(RuntimeError):
 
 def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
                 ratio=0.5, min_width=None, reg=REG_HPB, cw=1.0, dvc=1.0, inner_exit=True, outer_exit=True,
-                relax_variant=RELAX_PLAIN, tile_iters=0, tile_w=0, tile_h=0, tile_config=-1, graph=False) -> Params:
+                relax_variant=RELAX_PLAIN, tile_iters=0, tile_w=0, tile_h=0, tile_config=-1, graph=False,
+                storage=STORE_F32, omega=0.7) -> Params:
     p = Params()
     lib().rgbdflow_default_params(C.byref(p))
     p.regularization_type = reg
@@ -146,6 +148,8 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.tile_iters = tile_iters
     p.tile_w, p.tile_h, p.tile_config = tile_w, tile_h, tile_config
     p.graph = int(graph)
+    p.storage = storage
+    p.omega = omega
     return p
 
 
@@ -197,11 +201,13 @@ class Context:
     def synchronize(self):
         self._check(lib().rgbdflow_synchronize(self.h))
 
-    def autotune_relax(self, W, H, batch=1):
-        """Time the tiled relaxation's schedules on `batch` W x H levels; the
-        fastest becomes this context's default for that shape."""
+    def autotune_relax(self, W, H, batch=1, params: Params | None = None):
+        """Time the tiled relaxation's schedules on `batch` W x H levels (systems
+        stored as params.storage); the fastest becomes this context's default
+        for that shape."""
         ch = TileChoice()
-        self._check(lib().rgbdflow_autotune_relax(self.h, W, H, int(batch), C.byref(ch)))
+        self._check(lib().rgbdflow_autotune_relax(self.h, W, H, int(batch),
+                                                  C.byref(params) if params is not None else None, C.byref(ch)))
         return ch.as_dict()
 
     def set_relax_schedule(self, W, H, choice: dict):
@@ -212,14 +218,16 @@ class Context:
                 setattr(ch, f, choice[f])
         self._check(lib().rgbdflow_set_relax_schedule(self.h, W, H, C.byref(ch)))
 
-    def time_relax_schedule(self, W, H, choice: dict, batch=1):
+    def time_relax_schedule(self, W, H, choice: dict, batch=1, params: Params | None = None):
         """Device microseconds per red-black iteration of a schedule."""
         ch = TileChoice()
         for f, _ in TileChoice._fields_:
             if f in choice:
                 setattr(ch, f, choice[f])
         us = C.c_double()
-        self._check(lib().rgbdflow_time_relax_schedule(self.h, W, H, int(batch), C.byref(ch), C.byref(us)))
+        self._check(lib().rgbdflow_time_relax_schedule(self.h, W, H, int(batch),
+                                                       C.byref(params) if params is not None else None,
+                                                       C.byref(ch), C.byref(us)))
         return us.value
 
     def set_relax_timing(self, on: bool):

@@ -334,6 +334,8 @@ LevelView make_view(const Work& w, int W, int H, const double* D1, const double*
   v.partials = w.partials;
   v.ctl = w.ctl + level;
   v.uv_scale = 1;
+  v.storage = RGBDFLOW_STORE_F32;
+  v.omega = .7f;
   return v;
 }
 
@@ -861,6 +863,8 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
         v.Hp = ps.L[k + 1].H;
       }
       v.uv_scale = 1 / p->pyramid_ratio;
+      v.storage = p->storage;
+      v.omega = p->omega > 0 ? p->omega : .7f;
       v.reg_type = p->regularization_type;
       v.W_full = (int)cam.xRes;
       v.f_full = cam.focalLength;
@@ -891,6 +895,8 @@ int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
   if (!p) return fail(c, RGBDFLOW_ERR_ARG, "null params");
   if (p->variant != RGBDFLOW_VARIANT_FULL) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "bad scene flow variant");
   if (p->regularization_type < 0 || p->regularization_type > 2) return fail(c, RGBDFLOW_ERR_ARG, "bad regularization");
+  if (p->storage < RGBDFLOW_STORE_F32 || p->storage > RGBDFLOW_STORE_BF16)
+    return fail(c, RGBDFLOW_ERR_ARG, "bad system storage");
   return validate_iters(c, p->n_outer, p->n_inner, p->n_sor);
 }
 
@@ -1069,6 +1075,8 @@ int rgbdflow_default_params(rgbdflow_params* p) {
   p->inner_exit = 1;
   p->outer_exit = 1;
   p->tile_config = -1;
+  p->storage = RGBDFLOW_STORE_F32;
+  p->omega = .7f;
   return RGBDFLOW_OK;
 }
 
@@ -1152,6 +1160,10 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
   TRY(stage_level_inputs(c, W, H, focal_level, Im1, dm1, Im2, dm2, warpIm2, warpDm2, u, v, w, dW, sW, mW, rW, wk,
                          view));
   view.allow_early = allowEarly ? 1 : 0;
+  if (opts) {
+    view.storage = opts->storage;
+    view.omega = opts->omega > 0 ? opts->omega : .7f;
+  }
   const SolveScalars k = scalars_from(opts, cw, alpha, gamma, nOuter, nInner, nSOR);
   std::vector<LevelView> items{view};
   StepPlan sp;
@@ -1469,7 +1481,7 @@ namespace {
 
 // `batch` independent W x H test systems (own coefficients and increments)
 // for timing relaxation schedules.
-int timing_items(rgbdflow_ctx* c, int W, int H, int batch, std::vector<LevelView>& items) {
+int timing_items(rgbdflow_ctx* c, int W, int H, int batch, int storage, std::vector<LevelView>& items) {
   const size_t N = (size_t)W * H;
   float4 *coef, *du, *du_alt;
   SolveCtl* ctl;
@@ -1488,7 +1500,9 @@ int timing_items(rgbdflow_ctx* c, int W, int H, int batch, std::vector<LevelView
     v.du = du + (size_t)b * N;
     v.du_alt = du_alt + (size_t)b * N;
     v.ctl = ctl + b;
-    launch_fill_test_system(c->stream, v.coef, v.du, W, H);
+    v.omega = .7f;
+    v.storage = storage;
+    launch_fill_test_system(c->stream, v.coef, v.du, W, H, storage);
   }
   return RGBDFLOW_OK;
 }
@@ -1568,12 +1582,13 @@ void choice_from_schedule(const TileSched& t, int candidates, double us, rgbdflo
 
 extern "C" {
 
-int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, rgbdflow_tile_choice* out) {
+int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, const rgbdflow_params* p,
+                            rgbdflow_tile_choice* out) {
   if (!c) return RGBDFLOW_ERR_ARG;
   if (bad_dims(W, H) || batch < 1 || batch > 256) return fail(c, RGBDFLOW_ERR_ARG, "bad autotune size");
   HIP_TRY(c, hipSetDevice(c->device));
   std::vector<LevelView> items;
-  TRY(timing_items(c, W, H, batch, items));
+  TRY(timing_items(c, W, H, batch, p ? p->storage : RGBDFLOW_STORE_F32, items));
   // candidate schedules
   std::vector<TileSched> cand;
   const long long pairs_img = (long long)((W + 1) / 2) * H;
@@ -1631,8 +1646,8 @@ int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, rgbdflow_t
   return RGBDFLOW_OK;
 }
 
-int rgbdflow_time_relax_schedule(rgbdflow_ctx* c, int W, int H, int batch, const rgbdflow_tile_choice* ch,
-                                 double* us_per_iter) {
+int rgbdflow_time_relax_schedule(rgbdflow_ctx* c, int W, int H, int batch, const rgbdflow_params* p,
+                                 const rgbdflow_tile_choice* ch, double* us_per_iter) {
   if (!c) return RGBDFLOW_ERR_ARG;
   if (bad_dims(W, H) || batch < 1 || batch > 256 || !ch || !us_per_iter)
     return fail(c, RGBDFLOW_ERR_ARG, "bad timing arguments");
@@ -1640,7 +1655,7 @@ int rgbdflow_time_relax_schedule(rgbdflow_ctx* c, int W, int H, int batch, const
   TileSched t;
   TRY(schedule_from_choice(c, W, H, ch, &t));
   std::vector<LevelView> items;
-  TRY(timing_items(c, W, H, batch, items));
+  TRY(timing_items(c, W, H, batch, p ? p->storage : RGBDFLOW_STORE_F32, items));
   hipEvent_t e0, e1;
   HIP_TRY(c, hipEventCreate(&e0));
   HIP_TRY(c, hipEventCreate(&e1));
@@ -1726,6 +1741,8 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   view.Hp = H;
   view.uv_scale = 1;
   view.reg_type = p->regularization_type;
+  view.storage = p->storage;
+  view.omega = p->omega > 0 ? p->omega : .7f;
   view.W_full = (int)p->cam.xRes;
   view.f_full = p->cam.focalLength;
   view.f_level = p->cam.focalLength;

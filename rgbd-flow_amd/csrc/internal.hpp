@@ -98,6 +98,8 @@ struct LevelView {
   int Wp, Hp;
   double uv_scale;      // 1 / pyramid ratio
   int reg_type;
+  int storage;          // system record storage (records.hpp: f32, f16, bf16)
+  float omega;          // relaxation factor (.cu:461: 0.7)
   int W_full;           // camera xRes (full size)
   float f_full;         // full-size focal length
   float f_level;        // focal length of the level
@@ -197,7 +199,7 @@ inline int tiled_slots(const TileSched& t, int W, int H, int T) {
 void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int slots, int flip, int n_iter,
                         int slot_base, int n_prev_checks, int check_slot);
 void launch_copy_du(hipStream_t s, const Views& v, int blocks);
-void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H);
+void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H, int storage);
 
 // kernels_pyramid.hip
 void launch_deinterleave(hipStream_t s, const double* src, int N, int C, double* dst);

@@ -18,6 +18,7 @@
 // consecutive lanes read consecutive LDS words.  Each thread keeps the
 // coefficients of the pixels it owns in registers for all T iterations.
 #include "internal.hpp"
+#include "records.hpp"
 #include "relax_pixel.hpp"
 
 #include <algorithm>
@@ -42,7 +43,8 @@ struct PixRegs {
 // every side, XA = (x, y, z, eRxy) and XB = (eRz, eDxy, eDz, 0); pads and
 // absent pixels hold zeros.
 template <int P, int C>
-__device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int PRp, const PixRegs (&pr)[P][2],
+__device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int PRp, float omega,
+                                           const PixRegs (&pr)[P][2],
                                            const int (&qi)[P], const int (&par)[P], const unsigned (&fl)[P][2],
                                            bool last, float& change) {
   constexpr int O = 1 - C;  // the colour read
@@ -82,7 +84,7 @@ __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int P
     c.rw = r.rw;
     const float y3[3] = {r.yu, r.yv, r.yw};
     float xn[3];
-    relax_update_dense(c, y3, f & 64u, L, U, R, D, o, xn);
+    relax_update_dense(c, omega, y3, f & 64u, L, U, R, D, o, xn);
     Ac[idx] = make_float4(xn[0], xn[1], xn[2], r.rxy);
     if (last && (f & 32u))
       change = fmaxf(change, fmaxf(fmaxf(fabsf(xn[0] - o.x), fabsf(xn[1] - o.y)), fabsf(xn[2] - o.z)));
@@ -156,26 +158,26 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
       const int l = gy * W + gc;
       PixRegs r{};
       if (valid) {
-        const float4* rec = v.coef + (size_t)l * kRec;
-        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+        float cv[kCoef], yv[3];
+        load_record(v.coef, v.storage, (size_t)l, cv, yv);
         // a missing neighbour's coefficient is exactly zero (its value is a zero pad)
         const bool hr = gc < W - 1, hd = gy < H - 1;
-        r.rxy = hr ? r0.x : 0.f;
-        r.rz = hr ? r0.y : 0.f;
-        r.dxy = hd ? r0.z : 0.f;
-        r.dz = hd ? r0.w : 0.f;
-        r.Buu = r1.x;
-        r.Bvv = r1.y;
-        r.Bww = r1.z;
-        r.Buv = r1.w;
-        r.Buw = r2.x;
-        r.Bvw = r2.y;
-        r.ru = r2.z;
-        r.rv = r2.w;
-        r.rw = r3.x;
-        r.yu = r3.y;
-        r.yv = r3.z;
-        r.yw = r3.w;
+        r.rxy = hr ? cv[kERxy] : 0.f;
+        r.rz = hr ? cv[kERz] : 0.f;
+        r.dxy = hd ? cv[kEDxy] : 0.f;
+        r.dz = hd ? cv[kEDz] : 0.f;
+        r.Buu = cv[kBuu];
+        r.Bvv = cv[kBvv];
+        r.Bww = cv[kBww];
+        r.Buv = cv[kBuv];
+        r.Buw = cv[kBuw];
+        r.Bvw = cv[kBvw];
+        r.ru = cv[kRu];
+        r.rv = cv[kRv];
+        r.rw = cv[kRw];
+        r.yu = yv[0];
+        r.yv = yv[1];
+        r.yw = yv[2];
         if (recip_den_ok(r.Buu) && recip_den_ok(r.Bvv) && recip_den_ok(r.Bww)) f |= 64u;
         const float4 x = src[l];
         XA[c * Si + qi[s]] = make_float4(x.x, x.y, x.z, r.rxy);
@@ -193,9 +195,9 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
   for (int it = 0; it < n_iter; it++) {
     const bool last = it == n_iter - 1;
     __syncthreads();
-    tiled_pass<P, 0>(XA, XB, Si, PRp, pr, qi, par, fl, last, change);
+    tiled_pass<P, 0>(XA, XB, Si, PRp, v.omega, pr, qi, par, fl, last, change);
     __syncthreads();
-    tiled_pass<P, 1>(XA, XB, Si, PRp, pr, qi, par, fl, last, change);
+    tiled_pass<P, 1>(XA, XB, Si, PRp, v.omega, pr, qi, par, fl, last, change);
   }
   __syncthreads();
 #pragma unroll
@@ -273,22 +275,21 @@ size_t tiled_lds_bytes(int slots) { return (size_t)4 * slots * sizeof(float4); }
 
 // A well-conditioned synthetic system for timing schedules: unit-scale
 // diagonal, weak couplings, small rhs; du = 0.
-__global__ void k_fill_test_system(float4* coef, float4* du, int W, int H) {
+__global__ void k_fill_test_system(float4* coef, float4* du, int W, int H, int storage) {
   const int N = W * H;
   for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < N; l += gridDim.x * blockDim.x) {
     const int j = l % W, i = l / W;
     const float er = j < W - 1 ? 0.1f : 0.f, ed = i < H - 1 ? 0.1f : 0.f;
-    coef[(size_t)l * kRec + 0] = make_float4(er, er, ed, ed);
-    coef[(size_t)l * kRec + 1] = make_float4(-1.f, -1.f, -1.f, 0.01f);
-    coef[(size_t)l * kRec + 2] = make_float4(0.01f, 0.01f, 1e-3f * (l % 7), 1e-3f * (l % 5));
-    coef[(size_t)l * kRec + 3] = make_float4(1e-3f * (l % 3), -1.f, -1.f, -1.f);
+    const float c[kCoef] = {er, er, ed, ed, -1.f, -1.f, -1.f, 0.01f, 0.01f, 0.01f,
+                            1e-3f * (l % 7), 1e-3f * (l % 5), 1e-3f * (l % 3)};
+    store_record(coef, storage, (size_t)l, c);
     du[l] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
-void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H) {
+void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H, int storage) {
   int nb = (int)std::min<size_t>(((size_t)W * H + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_fill_test_system, dim3(nb), dim3(256), 0, s, coef, du, W, H);
+  hipLaunchKernelGGL(k_fill_test_system, dim3(nb), dim3(256), 0, s, coef, du, W, H, storage);
 }
 
 void launch_copy_du(hipStream_t s, const Views& v, int blocks) {

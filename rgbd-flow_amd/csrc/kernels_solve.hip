@@ -14,6 +14,7 @@
 // pair's outer loop or current relaxation has stopped, so the host enqueues
 // whole levels of whole batches without waiting.
 #include "internal.hpp"
+#include "records.hpp"
 #include "relax_pixel.hpp"
 
 namespace rgbdflow {
@@ -237,11 +238,8 @@ __device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars&
   const float Bww = (float)((double)t + -(gamma * m));
   t = (float)(-(P * Dt + alpha * -wL));
   const float rw = (float)((double)t + -(-gamma * m * u2));
-  float4* C = v.coef + (size_t)l * kRec;
-  C[0] = make_float4(eRxy, eRz, eDxy, eDz);
-  C[1] = make_float4(Buu, Bvv, Bww, Buv);
-  C[2] = make_float4(Buw, Bvw, ru, rv);
-  C[3] = make_float4(rw, 1.0f / Buu, 1.0f / Bvv, 1.0f / Bww);
+  const float rec[kCoef] = {eRxy, eRz, eDxy, eDz, Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw};
+  store_record(v.coef, v.storage, (size_t)l, rec);
 }
 
 // The system of an inner iteration after the first, from the stored
@@ -370,7 +368,7 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
   if (i < H && j < W) {
     const int l = i * W + j;
     const bool hl = j > 0, hu = i > 0, hr = j < W - 1, hd = i < H - 1;
-    const PixCoef c = load_coef(v.coef, W, l, hl, hu);
+    const PixCoef c = load_coef(v.coef, v.storage, W, l, hl, hu);
     float4* x = v.du;
     const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 o = x[l], L = hl ? x[l - 1] : zero, U = hu ? x[l - W] : zero, R = hr ? x[l + 1] : zero,
@@ -378,7 +376,7 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
     const float xo[3] = {o.x, o.y, o.z}, xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z},
                 xd[3] = {D.x, D.y, D.z};
     float xn[3];
-    relax_update(c, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn);
+    relax_update(c, v.omega, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn);
     x[l] = make_float4(xn[0], xn[1], xn[2], 0.f);
     change = fmaxf(fmaxf(fabsf(xn[0] - xo[0]), fabsf(xn[1] - xo[1])), fabsf(xn[2] - xo[2]));
   }

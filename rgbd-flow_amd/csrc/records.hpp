@@ -1,0 +1,94 @@
+// records.hpp -- storage of the per-pixel system records.
+//
+// F32 (the reference's precision): four float4 per pixel,
+//   (eRxy, eRz, eDxy, eDz) (Buu, Bvv, Bww, Buv) (Buw, Bvw, ru, rv) (rw, 1/Buu, 1/Bvv, 1/Bww).
+// F16 / BF16 (precision variants): sixteen 16-bit values per pixel (32 bytes),
+//   the 13 coefficients in the same order, then for F16 a per-pixel power-of-
+//   two exponent e (values are stored as c * 2^-e so the largest is in
+//   [2^14, 2^15), inside FP16's range); the reciprocals are recomputed from the
+//   decoded diagonal when a record is loaded.  Arithmetic stays f32.
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace rgbdflow {
+
+enum Storage { kStoreF32 = 0, kStoreF16 = 1, kStoreBF16 = 2 };
+
+__device__ __forceinline__ unsigned short f32_to_bf16(float x) {
+  const unsigned u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // NaN stays NaN
+  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);                  // round to nearest even
+}
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+__device__ __forceinline__ unsigned short f32_to_f16_bits(float x) {
+  return __half_as_ushort(__float2half_rn(x));
+}
+__device__ __forceinline__ float f16_bits_to_f32(unsigned short h) { return __half2float(__ushort_as_half(h)); }
+
+// Power-of-two scale 2^k as a float (k within the normal range).
+__device__ __forceinline__ float pow2f(int k) { return __uint_as_float((unsigned)(k + 127) << 23); }
+
+// Writes pixel l's record; c: the 13 values in Coef order.
+__device__ __forceinline__ void store_record(float4* base, int storage, size_t l, const float (&c)[kCoef]) {
+  if (storage == kStoreF32) {
+    float4* C = base + l * kRec;
+    C[0] = make_float4(c[0], c[1], c[2], c[3]);
+    C[1] = make_float4(c[4], c[5], c[6], c[7]);
+    C[2] = make_float4(c[8], c[9], c[10], c[11]);
+    C[3] = make_float4(c[12], 1.0f / c[kBuu], 1.0f / c[kBvv], 1.0f / c[kBww]);
+    return;
+  }
+  unsigned short h[16];
+  if (storage == kStoreF16) {
+    float m = 0.f;
+    for (int k = 0; k < kCoef; k++) m = fmaxf(m, fabsf(c[k]));
+    int e = 0;
+    if (m > 0.f && m <= 3.4e38f) e = min(max(ilogbf(m) - 14, -100), 100);
+    const float s = pow2f(-e);
+    for (int k = 0; k < kCoef; k++) h[k] = f32_to_f16_bits(c[k] * s);
+    h[13] = (unsigned short)(short)e;
+  } else {
+    for (int k = 0; k < kCoef; k++) h[k] = f32_to_bf16(c[k]);
+    h[13] = 0;
+  }
+  h[14] = h[15] = 0;
+  uint4* R = reinterpret_cast<uint4*>(base) + l * 2;
+  R[0] = make_uint4(h[0] | (unsigned)h[1] << 16, h[2] | (unsigned)h[3] << 16, h[4] | (unsigned)h[5] << 16,
+                    h[6] | (unsigned)h[7] << 16);
+  R[1] = make_uint4(h[8] | (unsigned)h[9] << 16, h[10] | (unsigned)h[11] << 16, h[12] | (unsigned)h[13] << 16,
+                    h[14] | (unsigned)h[15] << 16);
+}
+
+// Reads pixel l's record: the 13 values and the reciprocals of the diagonal.
+__device__ __forceinline__ void load_record(const float4* base, int storage, size_t l, float (&c)[kCoef],
+                                            float (&y)[3]) {
+  if (storage == kStoreF32) {
+    const float4* C = base + l * kRec;
+    const float4 r0 = C[0], r1 = C[1], r2 = C[2], r3 = C[3];
+    c[0] = r0.x; c[1] = r0.y; c[2] = r0.z; c[3] = r0.w;
+    c[4] = r1.x; c[5] = r1.y; c[6] = r1.z; c[7] = r1.w;
+    c[8] = r2.x; c[9] = r2.y; c[10] = r2.z; c[11] = r2.w;
+    c[12] = r3.x;
+    y[0] = r3.y; y[1] = r3.z; y[2] = r3.w;
+    return;
+  }
+  const uint4* R = reinterpret_cast<const uint4*>(base) + l * 2;
+  const uint4 a = R[0], b = R[1];
+  const unsigned w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  if (storage == kStoreF16) {
+    const int e = (short)(unsigned short)(w[6] >> 16);
+    const float s = pow2f(e);
+    for (int k = 0; k < kCoef; k++) c[k] = f16_bits_to_f32((unsigned short)(w[k >> 1] >> ((k & 1) * 16))) * s;
+  } else {
+    for (int k = 0; k < kCoef; k++) c[k] = bf16_to_f32((unsigned short)(w[k >> 1] >> ((k & 1) * 16)));
+  }
+  y[0] = 1.0f / c[kBuu];
+  y[1] = 1.0f / c[kBvv];
+  y[2] = 1.0f / c[kBww];
+}
+
+}  // namespace rgbdflow

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
+#include "records.hpp"
 
 namespace rgbdflow {
 
@@ -22,35 +23,37 @@ struct PixCoef {
 // A pixel's coefficients from the system records; the left and up edge
 // weights are the left neighbour's right and the upper neighbour's down ones
 // (zero where the neighbour does not exist).
-__device__ __forceinline__ PixCoef load_coef(const float4* C, int W, int l, bool hl, bool hu) {
+__device__ __forceinline__ PixCoef load_coef(const float4* C, int storage, int W, int l, bool hl, bool hu) {
   PixCoef c;
-  const float4 r0 = C[(size_t)l * kRec + 0], r1 = C[(size_t)l * kRec + 1], r2 = C[(size_t)l * kRec + 2],
-               r3 = C[(size_t)l * kRec + 3];
-  const float4 L = hl ? C[(size_t)(l - 1) * kRec] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const float4 U = hu ? C[(size_t)(l - W) * kRec] : make_float4(0.f, 0.f, 0.f, 0.f);
-  c.lxy = L.x;
-  c.lz = L.y;
-  c.uxy = U.z;
-  c.uz = U.w;
-  c.rxy = r0.x;
-  c.rz = r0.y;
-  c.dxy = r0.z;
-  c.dz = r0.w;
-  c.Buu = r1.x;
-  c.Bvv = r1.y;
-  c.Bww = r1.z;
-  c.Buv = r1.w;
-  c.Buw = r2.x;
-  c.Bvw = r2.y;
-  c.ru = r2.z;
-  c.rv = r2.w;
-  c.rw = r3.x;
+  float r[kCoef], y[3];
+  load_record(C, storage, (size_t)l, r, y);
+  float L[kCoef] = {}, U[kCoef] = {};
+  if (hl) load_record(C, storage, (size_t)(l - 1), L, y);
+  if (hu) load_record(C, storage, (size_t)(l - W), U, y);
+  c.lxy = hl ? L[kERxy] : 0.f;
+  c.lz = hl ? L[kERz] : 0.f;
+  c.uxy = hu ? U[kEDxy] : 0.f;
+  c.uz = hu ? U[kEDz] : 0.f;
+  c.rxy = r[kERxy];
+  c.rz = r[kERz];
+  c.dxy = r[kEDxy];
+  c.dz = r[kEDz];
+  c.Buu = r[kBuu];
+  c.Bvv = r[kBvv];
+  c.Bww = r[kBww];
+  c.Buv = r[kBuv];
+  c.Buw = r[kBuw];
+  c.Bvw = r[kBvw];
+  c.ru = r[kRu];
+  c.rv = r[kRv];
+  c.rw = r[kRw];
   return c;
 }
 
 // One pixel's update.  h* say which neighbours exist in the image; x* hold the
 // neighbours' (other colour) values, xo the pixel's old values.
-__device__ __forceinline__ void relax_update(const PixCoef& c, bool hl, bool hu, bool hr, bool hd, const float xl[3],
+__device__ __forceinline__ void relax_update(const PixCoef& c, float omega, bool hl, bool hu, bool hr, bool hd,
+                                             const float xl[3],
                                              const float xu[3], const float xr[3], const float xd[3],
                                              const float xo[3], float xn[3]) {
 #pragma unroll
@@ -80,7 +83,6 @@ __device__ __forceinline__ void relax_update(const PixCoef& c, bool hl, bool hu,
       nv -= c.Bvw * xo[1];
       nv /= c.Bww;
     }
-    const float omega = .7f;
     float r = omega * nv + (1 - omega) * xo[k];
     if (r < xo[k] - .3f) r = xo[k] - .3f;
     else if (r > xo[k] + .3f) r = xo[k] + .3f;
@@ -113,7 +115,7 @@ __device__ __forceinline__ float recip_div(float a, float b, float y) {
 // bits as relax_update's.  L, U, R, D, o: neighbour and old values (x, y, z
 // in .x .y .z).  y3: 1/Buu, 1/Bvv, 1/Bww (IEEE divisions); den_ok: all three
 // diagonals are in the reciprocal division's domain.
-__device__ __forceinline__ void relax_update_dense(const PixCoef& c, const float (&y3)[3], bool den_ok,
+__device__ __forceinline__ void relax_update_dense(const PixCoef& c, float omega, const float (&y3)[3], bool den_ok,
                                                    const float4& L, const float4& U, const float4& R,
                                                    const float4& D, const float4& o, float xn[3]) {
   float nv[3];
@@ -156,7 +158,6 @@ __device__ __forceinline__ void relax_update_dense(const PixCoef& c, const float
   }
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    const float omega = .7f;
     float r = omega * nv[k] + (1 - omega) * xo[k];
     if (r < xo[k] - .3f) r = xo[k] - .3f;
     else if (r > xo[k] + .3f) r = xo[k] + .3f;
