@@ -337,3 +337,15 @@ def check_reciprocal_division(n, seed, mode, recipe=0):
     a, b = C.c_float(), C.c_float()
     bad = f(n, seed, mode, recipe, C.byref(a), C.byref(b))
     return int(bad), (a.value, b.value)
+
+
+def check_div12(n, seed, recipe=0):
+    """Samples n f64 x in the domain of the kernels' division by 12
+    (divcheck.cpp) and returns (mismatches against IEEE x / 12, first x)."""
+    L = lib()
+    f = L.orc_check_div12
+    f.restype = C.c_uint64
+    f.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_double)]
+    x = C.c_double()
+    bad = f(n, seed, recipe, C.byref(x))
+    return int(bad), x.value

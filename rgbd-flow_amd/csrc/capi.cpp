@@ -70,6 +70,8 @@ struct StepPlan {
   Views vpix{}, vplain{}, vtiled{}, votile{};
   bool tiled_ok = false;
   int cfg = 0, slots = 0, T = 0;  // tiled launch shape: workgroup shape, LDS slots, iterations
+  int storage = 0;                 // system record storage of the step's items
+  bool omega_fixed = true;         // every item relaxes with the factor 0.7
   bool any_early = false;
 };
 
@@ -335,7 +337,7 @@ LevelView make_view(const Work& w, int W, int H, const double* D1, const double*
   v.ctl = w.ctl + level;
   v.uv_scale = 1;
   v.storage = RGBDFLOW_STORE_F32;
-  v.omega = .7f;
+  set_omega(v, .7f);
   return v;
 }
 
@@ -480,6 +482,9 @@ size_t pack(std::vector<char>& host, const void* p, size_t bytes) {
 // Packs a step's items and the block offsets of each kernel family.
 void pack_step(std::vector<char>& host, const std::vector<LevelView>& items, StepPlan& st) {
   st.n = (int)items.size();
+  st.storage = items.empty() ? 0 : items[0].storage;
+  st.omega_fixed = true;
+  for (const LevelView& v : items) st.omega_fixed = st.omega_fixed && v.omega == .7f;
   std::vector<int> pix(st.n + 1), plain(st.n + 1), tiled(st.n + 1), otile(st.n + 1);
   pix[0] = plain[0] = tiled[0] = otile[0] = 0;
   for (int i = 0; i < st.n; i++) {
@@ -560,7 +565,8 @@ void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, i
       const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
       const int slot = (last % kCheckEvery == kCheckEvery - 1) ? slot_base + last / kCheckEvery : -1;
       relax_event(c);
-      launch_relax_tiled(s, st.vtiled, st.tiled, st.cfg, st.slots, flip, T, slot_base, n_prev, slot);
+      launch_relax_tiled(s, st.vtiled, st.tiled, st.cfg, st.storage, st.omega_fixed, st.slots, flip, T, slot_base,
+                         n_prev, slot);
       relax_event(c);
       flip ^= 1;
       it += T;
@@ -864,7 +870,7 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
       }
       v.uv_scale = 1 / p->pyramid_ratio;
       v.storage = p->storage;
-      v.omega = p->omega > 0 ? p->omega : .7f;
+      set_omega(v, p->omega);
       v.reg_type = p->regularization_type;
       v.W_full = (int)cam.xRes;
       v.f_full = cam.focalLength;
@@ -1162,7 +1168,7 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
   view.allow_early = allowEarly ? 1 : 0;
   if (opts) {
     view.storage = opts->storage;
-    view.omega = opts->omega > 0 ? opts->omega : .7f;
+    set_omega(view, opts->omega);
   }
   const SolveScalars k = scalars_from(opts, cw, alpha, gamma, nOuter, nInner, nSOR);
   std::vector<LevelView> items{view};
@@ -1500,7 +1506,7 @@ int timing_items(rgbdflow_ctx* c, int W, int H, int batch, int storage, std::vec
     v.du = du + (size_t)b * N;
     v.du_alt = du_alt + (size_t)b * N;
     v.ctl = ctl + b;
-    v.omega = .7f;
+    set_omega(v, .7f);
     v.storage = storage;
     launch_fill_test_system(c->stream, v.coef, v.du, W, H, storage);
   }
@@ -1742,7 +1748,7 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   view.uv_scale = 1;
   view.reg_type = p->regularization_type;
   view.storage = p->storage;
-  view.omega = p->omega > 0 ? p->omega : .7f;
+  set_omega(view, p->omega);
   view.W_full = (int)p->cam.xRes;
   view.f_full = p->cam.focalLength;
   view.f_level = p->cam.focalLength;

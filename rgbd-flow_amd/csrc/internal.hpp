@@ -100,11 +100,18 @@ struct LevelView {
   int reg_type;
   int storage;          // system record storage (records.hpp: f32, f16, bf16)
   float omega;          // relaxation factor (.cu:461: 0.7)
+  float omega_c;        // 1 - omega, rounded to float as the update computes it
   int W_full;           // camera xRes (full size)
   float f_full;         // full-size focal length
   float f_level;        // focal length of the level
   TileSched ts;
 };
+
+// Sets a view's relaxation factor; a factor <= 0 selects the reference's 0.7.
+inline void set_omega(LevelView& v, float omega) {
+  v.omega = omega > 0 ? omega : .7f;
+  v.omega_c = 1.f - v.omega;
+}
 
 // A launch's work-list: items plus the first block of each item (n + 1 entries).
 template <class T>
@@ -196,8 +203,9 @@ inline int tiled_slots(const TileSched& t, int W, int H, int T) {
   }
   return (PR + 2) * (RH + 2);
 }
-void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int slots, int flip, int n_iter,
-                        int slot_base, int n_prev_checks, int check_slot);
+// omega_fixed: every item of the launch relaxes with the factor 0.7.
+void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int storage, bool omega_fixed,
+                        int slots, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot);
 void launch_copy_du(hipStream_t s, const Views& v, int blocks);
 void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H, int storage);
 

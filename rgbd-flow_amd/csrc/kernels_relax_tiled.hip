@@ -43,7 +43,7 @@ struct PixRegs {
 // every side, XA = (x, y, z, eRxy) and XB = (eRz, eDxy, eDz, 0); pads and
 // absent pixels hold zeros.
 template <int P, int C>
-__device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int PRp, float omega,
+__device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int PRp, float omega, float omega_c,
                                            const PixRegs (&pr)[P][2],
                                            const int (&qi)[P], const int (&par)[P], const unsigned (&fl)[P][2],
                                            bool last, float& change) {
@@ -84,7 +84,7 @@ __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int P
     c.rw = r.rw;
     const float y3[3] = {r.yu, r.yv, r.yw};
     float xn[3];
-    relax_update_dense(c, omega, y3, f & 64u, L, U, R, D, o, xn);
+    relax_update_dense(c, omega, omega_c, y3, f & 64u, L, U, R, D, o, xn);
     Ac[idx] = make_float4(xn[0], xn[1], xn[2], r.rxy);
     if (last && (f & 32u))
       change = fmaxf(change, fmaxf(fmaxf(fabsf(xn[0] - o.x), fabsf(xn[1] - o.y)), fabsf(xn[2] - o.z)));
@@ -98,7 +98,7 @@ __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int P
 // nothing; the last launch that ran records in SolveCtl::du_in_alt where du
 // is, and k_copy_du moves it to du at the end.  t: the tile's index in the
 // launch's work-list.
-template <int NT, int P>
+template <int NT, int P, int ST, bool FIXED>
 __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, int flip, int n_iter, int slot_base,
                                            int n_prev_checks, int check_slot) {
   const int item = work_item(wl, t);
@@ -159,7 +159,7 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
       PixRegs r{};
       if (valid) {
         float cv[kCoef], yv[3];
-        load_record(v.coef, v.storage, (size_t)l, cv, yv);
+        load_record(v.coef, ST, (size_t)l, cv, yv);
         // a missing neighbour's coefficient is exactly zero (its value is a zero pad)
         const bool hr = gc < W - 1, hd = gy < H - 1;
         r.rxy = hr ? cv[kERxy] : 0.f;
@@ -192,12 +192,15 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
   }
 
   float change = 0.f;
+  // FIXED: the reference's factor 0.7 as constants (a factor held in registers
+  // costs the widest configurations spills in the iteration loop)
+  const float om = FIXED ? .7f : v.omega, om_c = FIXED ? 1.f - .7f : v.omega_c;
   for (int it = 0; it < n_iter; it++) {
     const bool last = it == n_iter - 1;
     __syncthreads();
-    tiled_pass<P, 0>(XA, XB, Si, PRp, v.omega, pr, qi, par, fl, last, change);
+    tiled_pass<P, 0>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, last, change);
     __syncthreads();
-    tiled_pass<P, 1>(XA, XB, Si, PRp, v.omega, pr, qi, par, fl, last, change);
+    tiled_pass<P, 1>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, last, change);
   }
   __syncthreads();
 #pragma unroll
@@ -226,7 +229,7 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
 // the 8 XCDs round-robin by id, and the grid is a multiple of 8, so workgroup
 // b always runs on XCD b % 8; each XCD gets a contiguous run of tiles, whose
 // shared halos then hit in that XCD's L2.
-template <int NT, int P, int MB>
+template <int NT, int P, int MB, int ST, bool FIXED>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 256 * MB)))
 k_relax_tiled(Views wl, int total, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot) {
   extern __shared__ float4 X4[];
@@ -234,7 +237,7 @@ k_relax_tiled(Views wl, int total, int flip, int n_iter, int slot_base, int n_pr
   for (int b = blockIdx.x; b < 8 * per_xcd; b += gridDim.x) {
     const int t = (b & 7) * per_xcd + (b >> 3);
     if (t >= total) continue;
-    relax_tile<NT, P>(X4, wl, t, flip, n_iter, slot_base, n_prev_checks, check_slot);
+    relax_tile<NT, P, ST, FIXED>(X4, wl, t, flip, n_iter, slot_base, n_prev_checks, check_slot);
     __syncthreads();  // the next tile reuses the LDS
   }
 }
@@ -249,18 +252,29 @@ __global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
   v.du[l] = v.du_alt[l];
 }
 
+typedef void (*TiledFn)(Views, int, int, int, int, int, int);
 struct TiledConfig {
   int nt, p;
-  void (*fn)(Views, int, int, int, int, int, int);
+  TiledFn fn[3][2];  // per system storage (records.hpp), [1]: relaxation factor 0.7
 };
 
-// (threads, pairs per thread, workgroups per CU the registers must allow)
-#define RGBDFLOW_TILED(NT, P, MB) TiledConfig{NT, P, k_relax_tiled<NT, P, MB>}
+// (threads, pairs per thread, workgroups per CU the registers must allow);
+// the record storage is a template parameter so each variant's load is
+// compiled on its own
+#define RGBDFLOW_TILED_ST(NT, P, MB, ST) {k_relax_tiled<NT, P, MB, ST, false>, k_relax_tiled<NT, P, MB, ST, true>}
+#define RGBDFLOW_TILED(NT, P, MB)                                                                    \
+  TiledConfig {                                                                                      \
+    NT, P, {                                                                                         \
+      RGBDFLOW_TILED_ST(NT, P, MB, kStoreF32), RGBDFLOW_TILED_ST(NT, P, MB, kStoreF16),              \
+          RGBDFLOW_TILED_ST(NT, P, MB, kStoreBF16)                                                   \
+    }                                                                                                \
+  }
 const TiledConfig kConfigs[] = {RGBDFLOW_TILED(256, 1, 1), RGBDFLOW_TILED(512, 1, 1), RGBDFLOW_TILED(1024, 1, 1),
                                 RGBDFLOW_TILED(256, 2, 1), RGBDFLOW_TILED(512, 2, 1), RGBDFLOW_TILED(1024, 1, 2),
                                 RGBDFLOW_TILED(512, 1, 4), RGBDFLOW_TILED(512, 2, 2), RGBDFLOW_TILED(1024, 2, 1),
                                 RGBDFLOW_TILED(256, 2, 4)};
 #undef RGBDFLOW_TILED
+#undef RGBDFLOW_TILED_ST
 
 }  // namespace
 
@@ -296,27 +310,29 @@ void launch_copy_du(hipStream_t s, const Views& v, int blocks) {
   hipLaunchKernelGGL(k_copy_du, dim3(blocks), dim3(kBlock), 0, s, v);
 }
 
-void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int slots, int flip, int n_iter,
-                        int slot_base, int n_prev_checks, int check_slot) {
+void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int storage, bool omega_fixed,
+                        int slots, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot) {
   const TiledConfig& c = kConfigs[config];
+  const TiledFn fn = c.fn[storage][omega_fixed ? 1 : 0];
+  const int key = (config * 3 + storage) * 2 + (omega_fixed ? 1 : 0);
   const size_t lds = tiled_lds_bytes(slots);
   // per-thread caches: launches may come from several host threads (one per context)
-  thread_local size_t raised[16] = {};
-  if (lds > 65536 && lds > raised[config]) {
-    hipFuncSetAttribute(reinterpret_cast<const void*>(c.fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    raised[config] = lds;
+  thread_local size_t raised[128] = {};
+  if (lds > 65536 && lds > raised[key]) {
+    hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    raised[key] = lds;
   }
-  // resident workgroups per CU for this shape (cached per configuration and LDS size)
-  thread_local int occ_cfg[16] = {}, occ_lds[16] = {}, occ_val[16] = {}, cus = 0;
-  if (occ_lds[config] != (int)lds || occ_cfg[config] == 0) {
+  // resident workgroups per CU for this shape (cached per kernel and LDS size)
+  thread_local int occ_cfg[128] = {}, occ_lds[128] = {}, occ_val[128] = {}, cus = 0;
+  if (occ_lds[key] != (int)lds || occ_cfg[key] == 0) {
     int n = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(c.fn), c.nt, lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(fn), c.nt, lds) !=
             hipSuccess ||
         n < 1)
       n = 1;
-    occ_val[config] = n;
-    occ_lds[config] = (int)lds;
-    occ_cfg[config] = 1;
+    occ_val[key] = n;
+    occ_lds[key] = (int)lds;
+    occ_cfg[key] = 1;
   }
   if (cus == 0) {
     int dev = 0;
@@ -324,9 +340,9 @@ void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, i
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
   }
   const int per_xcd = (blocks + 7) / 8;
-  int grid = std::min(8 * per_xcd, cus * occ_val[config]);
+  int grid = std::min(8 * per_xcd, cus * occ_val[key]);
   grid = std::max(8, grid & ~7);
-  hipLaunchKernelGGL(c.fn, dim3(grid), dim3(c.nt), lds, s, v, blocks, flip, n_iter, slot_base, n_prev_checks,
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(c.nt), lds, s, v, blocks, flip, n_iter, slot_base, n_prev_checks,
                      check_slot);
 }
 

@@ -37,6 +37,19 @@ __global__ void k_ctl_reset(Views w) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
+// x / 12 in f64, bit-equal to the IEEE division: Markstein's correction with
+// the correctly rounded reciprocal of 12 (IBM J. Res. Dev. 34(1), 1990) away
+// from zero, underflow and overflow; the division itself elsewhere.
+__device__ __forceinline__ double div12(double x) {
+  const double ax = fabs(x);
+  if (__builtin_expect(ax >= 0x1p-960 && ax <= 0x1p960, 1)) {
+    const double y = 1.0 / 12.0;
+    const double q0 = x * y;
+    return __builtin_fma(__builtin_fma(-12.0, q0, x), y, q0);
+  }
+  return x / 12;
+}
+
 // Image and depth derivatives of one pixel (.cu:84-147): x/y derivatives of
 // 0.4*I1 + 0.6*warped I2 with [1,-8,0,8,-1]/12, clamp-to-edge; It = Iw - I1;
 // depth derivatives clamped to +-0.04.
@@ -60,8 +73,8 @@ __device__ __forceinline__ void derivs_at(const LevelView& v, int l, int i, int 
       const int o = clampi(i - 2 + m, 0, H - 1) * W + j;
       sy += g[m] * (.4 * a[o] + .6 * b[o]);
     }
-    d.Ix[k] = sx / 12;
-    d.Iy[k] = sy / 12;
+    d.Ix[k] = div12(sx);
+    d.Iy[k] = div12(sy);
     d.It[k] = b[l] - a[l];
   }
   const double mg = .04;
@@ -74,8 +87,8 @@ __device__ __forceinline__ void derivs_at(const LevelView& v, int l, int i, int 
     const int o = clampi(i - 2 + m, 0, H - 1) * W + j;
     sy += g[m] * (.4 * v.D1[o] + .6 * v.Dw[o]);
   }
-  d.Dx = fmin(fmax(sx / 12, -mg), mg);
-  d.Dy = fmin(fmax(sy / 12, -mg), mg);
+  d.Dx = fmin(fmax(div12(sx), -mg), mg);
+  d.Dy = fmin(fmax(div12(sy), -mg), mg);
   d.Dt = fmin(fmax(v.Dw[l] - v.D1[l], -mg), mg);
 }
 
@@ -316,8 +329,8 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     double sx = 0, sy = 0;
     for (int m = 0; m < 5; m++) sx += g[m] * Ms[k][c0 - 2 + m];
     for (int m = 0; m < 5; m++) sy += g[m] * Ms[k][c0 + (m - 2) * kRw];
-    d.Ix[k] = sx / 12;
-    d.Iy[k] = sy / 12;
+    d.Ix[k] = div12(sx);
+    d.Iy[k] = div12(sy);
     d.It[k] = v.Iw[(size_t)k * N + l] - v.I1[(size_t)k * N + l];
   }
   {
@@ -325,8 +338,8 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     double sx = 0, sy = 0;
     for (int m = 0; m < 5; m++) sx += g[m] * Ms[kFeat][c0 - 2 + m];
     for (int m = 0; m < 5; m++) sy += g[m] * Ms[kFeat][c0 + (m - 2) * kRw];
-    d.Dx = fmin(fmax(sx / 12, -mg), mg);
-    d.Dy = fmin(fmax(sy / 12, -mg), mg);
+    d.Dx = fmin(fmax(div12(sx), -mg), mg);
+    d.Dy = fmin(fmax(div12(sy), -mg), mg);
     d.Dt = fmin(fmax(v.Dw[l] - v.D1[l], -mg), mg);
   }
   const float q = occlusion_at(v, l);
@@ -376,7 +389,7 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
     const float xo[3] = {o.x, o.y, o.z}, xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z},
                 xd[3] = {D.x, D.y, D.z};
     float xn[3];
-    relax_update(c, v.omega, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn);
+    relax_update(c, v.omega, v.omega_c, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn);
     x[l] = make_float4(xn[0], xn[1], xn[2], 0.f);
     change = fmaxf(fmaxf(fabsf(xn[0] - xo[0]), fabsf(xn[1] - xo[1])), fabsf(xn[2] - xo[2]));
   }

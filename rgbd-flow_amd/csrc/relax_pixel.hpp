@@ -52,10 +52,10 @@ __device__ __forceinline__ PixCoef load_coef(const float4* C, int storage, int W
 
 // One pixel's update.  h* say which neighbours exist in the image; x* hold the
 // neighbours' (other colour) values, xo the pixel's old values.
-__device__ __forceinline__ void relax_update(const PixCoef& c, float omega, bool hl, bool hu, bool hr, bool hd,
-                                             const float xl[3],
-                                             const float xu[3], const float xr[3], const float xd[3],
-                                             const float xo[3], float xn[3]) {
+// omega_c = 1 - omega in float.
+__device__ __forceinline__ void relax_update(const PixCoef& c, float omega, float omega_c, bool hl, bool hu, bool hr,
+                                             bool hd, const float xl[3], const float xu[3], const float xr[3],
+                                             const float xd[3], const float xo[3], float xn[3]) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     const float el = k < 2 ? c.lxy : c.lz;
@@ -83,7 +83,7 @@ __device__ __forceinline__ void relax_update(const PixCoef& c, float omega, bool
       nv -= c.Bvw * xo[1];
       nv /= c.Bww;
     }
-    float r = omega * nv + (1 - omega) * xo[k];
+    float r = omega * nv + omega_c * xo[k];
     if (r < xo[k] - .3f) r = xo[k] - .3f;
     else if (r > xo[k] + .3f) r = xo[k] + .3f;
     xn[k] = r;
@@ -115,7 +115,8 @@ __device__ __forceinline__ float recip_div(float a, float b, float y) {
 // bits as relax_update's.  L, U, R, D, o: neighbour and old values (x, y, z
 // in .x .y .z).  y3: 1/Buu, 1/Bvv, 1/Bww (IEEE divisions); den_ok: all three
 // diagonals are in the reciprocal division's domain.
-__device__ __forceinline__ void relax_update_dense(const PixCoef& c, float omega, const float (&y3)[3], bool den_ok,
+__device__ __forceinline__ void relax_update_dense(const PixCoef& c, float omega, float omega_c,
+                                                   const float (&y3)[3], bool den_ok,
                                                    const float4& L, const float4& U, const float4& R,
                                                    const float4& D, const float4& o, float xn[3]) {
   float nv[3];
@@ -158,7 +159,7 @@ __device__ __forceinline__ void relax_update_dense(const PixCoef& c, float omega
   }
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    float r = omega * nv[k] + (1 - omega) * xo[k];
+    float r = omega * nv[k] + omega_c * xo[k];
     if (r < xo[k] - .3f) r = xo[k] - .3f;
     else if (r > xo[k] + .3f) r = xo[k] + .3f;
     xn[k] = r;

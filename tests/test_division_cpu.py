@@ -19,3 +19,11 @@ def test_reciprocal_division_is_ieee(mode):
 def test_negative_control_is_caught(mode):
     bad, _ = oracle().check_reciprocal_division(200_000, 7, mode, recipe=1)
     assert bad > 1000
+
+
+def test_div12_is_ieee():
+    """The derivative stencils' f64 division by 12 (kernels_solve.hip div12)."""
+    bad, x = oracle().check_div12(5_000_000, 3)
+    assert bad == 0, x
+    bad, _ = oracle().check_div12(200_000, 4, recipe=1)
+    assert bad > 1000
