@@ -107,6 +107,38 @@ struct LevelView {
   TileSched ts;
 };
 
+// Device code reaches the level buffers through global-address-space
+// pointers: the compiler then emits global loads and stores.  Generic
+// pointers read from a view give flat accesses, whose completion shares the
+// LDS counter, so every LDS wait would also wait for outstanding HBM loads.
+#define RGBDFLOW_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ RGBDFLOW_GLOBAL T* gp(T* p) {
+  return (RGBDFLOW_GLOBAL T*)p;
+}
+
+// 16-byte loads and stores through global pointers (float4 and uint4 are
+// classes whose copies do not take address-space-qualified operands; the
+// native vector types do).
+typedef float gvec4f __attribute__((ext_vector_type(4)));
+typedef unsigned gvec4u __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gload(const float4* p) {
+  const gvec4f x = *(const RGBDFLOW_GLOBAL gvec4f*)p;
+  return make_float4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ void gstore(float4* p, const float4& v) {
+  const gvec4f x = {v.x, v.y, v.z, v.w};
+  *(RGBDFLOW_GLOBAL gvec4f*)p = x;
+}
+__device__ __forceinline__ uint4 gload(const uint4* p) {
+  const gvec4u x = *(const RGBDFLOW_GLOBAL gvec4u*)p;
+  return make_uint4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ void gstore(uint4* p, const uint4& v) {
+  const gvec4u x = {v.x, v.y, v.z, v.w};
+  *(RGBDFLOW_GLOBAL gvec4u*)p = x;
+}
+
 // Sets a view's relaxation factor; a factor <= 0 selects the reference's 0.7.
 inline void set_omega(LevelView& v, float omega) {
   v.omega = omega > 0 ? omega : .7f;

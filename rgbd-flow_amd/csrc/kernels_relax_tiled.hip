@@ -179,7 +179,7 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
         r.yv = yv[1];
         r.yw = yv[2];
         if (recip_den_ok(r.Buu) && recip_den_ok(r.Bvv) && recip_den_ok(r.Bww)) f |= 64u;
-        const float4 x = src[l];
+        const float4 x = gload(src + l);
         XA[c * Si + qi[s]] = make_float4(x.x, x.y, x.z, r.rxy);
         XB[c * Si + qi[s]] = make_float4(r.rz, r.dxy, r.dz, 0.f);
       } else if (q < np) {
@@ -212,7 +212,7 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
       if (!(fl[s][c] & 32u)) continue;
       const int gc = rx0 + 2 * kk + (c == 0 ? par[s] : 1 - par[s]);
       const float4 x = XA[c * Si + qi[s]];
-      dst[gy * W + gc] = make_float4(x.x, x.y, x.z, 0.f);
+      gstore(dst + gy * W + gc, make_float4(x.x, x.y, x.z, 0.f));
     }
   }
   if (check_slot >= 0) {
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
   if (v.ctl->outer_done || !v.ctl->du_in_alt) return;
   const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
-  v.du[l] = v.du_alt[l];
+  gstore(v.du + l, gload(v.du_alt + l));
 }
 
 typedef void (*TiledFn)(Views, int, int, int, int, int, int);

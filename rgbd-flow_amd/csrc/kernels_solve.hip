@@ -62,8 +62,8 @@ __device__ __forceinline__ void derivs_at(const LevelView& v, int l, int i, int 
   const int W = v.W, H = v.H, N = v.N;
   const double g[5] = {1, -8, 0, 8, -1};
   for (int k = 0; k < kFeat; k++) {
-    const double* a = v.I1 + (size_t)k * N;
-    const double* b = v.Iw + (size_t)k * N;
+    const RGBDFLOW_GLOBAL double* a = gp(v.I1) + (size_t)k * N;
+    const RGBDFLOW_GLOBAL double* b = gp(v.Iw) + (size_t)k * N;
     double sx = 0, sy = 0;
     for (int m = 0; m < 5; m++) {
       const int o = i * W + clampi(j - 2 + m, 0, W - 1);
@@ -81,21 +81,21 @@ __device__ __forceinline__ void derivs_at(const LevelView& v, int l, int i, int 
   double sx = 0, sy = 0;
   for (int m = 0; m < 5; m++) {
     const int o = i * W + clampi(j - 2 + m, 0, W - 1);
-    sx += g[m] * (.4 * v.D1[o] + .6 * v.Dw[o]);
+    sx += g[m] * (.4 * gp(v.D1)[o] + .6 * gp(v.Dw)[o]);
   }
   for (int m = 0; m < 5; m++) {
     const int o = clampi(i - 2 + m, 0, H - 1) * W + j;
-    sy += g[m] * (.4 * v.D1[o] + .6 * v.Dw[o]);
+    sy += g[m] * (.4 * gp(v.D1)[o] + .6 * gp(v.Dw)[o]);
   }
   d.Dx = fmin(fmax(div12(sx), -mg), mg);
   d.Dy = fmin(fmax(div12(sy), -mg), mg);
-  d.Dt = fmin(fmax(v.Dw[l] - v.D1[l], -mg), mg);
+  d.Dt = fmin(fmax(gp(v.Dw)[l] - gp(v.D1)[l], -mg), mg);
 }
 
 // .cu:640-648: occlusion weight from the UNWARPED D2.
 __device__ __forceinline__ float occlusion_at(const LevelView& v, int l) {
-  const double d1 = v.D1[l];
-  return (d1 + v.u[2 * v.N + l] < v.D2[l] + .02 * (double)sqr_f32(d1)) ? 1.f : (float).01;
+  const double d1 = gp(v.D1)[l];
+  return (d1 + gp(v.u)[2 * v.N + l] < gp(v.D2)[l] + .02 * (double)sqr_f32(d1)) ? 1.f : (float).01;
 }
 
 // Derivatives + occlusion stored for the inner iterations, du = 0 (.cu:776);
@@ -113,15 +113,15 @@ __global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
   PixDerivs d;
   derivs_at(v, l, i, j, d);
   for (int k = 0; k < kFeat; k++) {
-    v.Ix[(size_t)k * N + l] = d.Ix[k];
-    v.Iy[(size_t)k * N + l] = d.Iy[k];
-    v.It[(size_t)k * N + l] = d.It[k];
+    gp(v.Ix)[(size_t)k * N + l] = d.Ix[k];
+    gp(v.Iy)[(size_t)k * N + l] = d.Iy[k];
+    gp(v.It)[(size_t)k * N + l] = d.It[k];
   }
-  v.Dx[l] = d.Dx;
-  v.Dy[l] = d.Dy;
-  v.Dt[l] = d.Dt;
-  v.du[l] = make_float4(0.f, 0.f, 0.f, 0.f);
-  v.q[l] = occlusion_at(v, l);
+  gp(v.Dx)[l] = d.Dx;
+  gp(v.Dy)[l] = d.Dy;
+  gp(v.Dt)[l] = d.Dt;
+  gstore(v.du + l, make_float4(0.f, 0.f, 0.f, 0.f));
+  gp(v.q)[l] = occlusion_at(v, l);
 }
 
 __device__ __forceinline__ float du_comp(const float4& d, int c) { return c == 0 ? d.x : (c == 1 ? d.y : d.z); }
@@ -131,10 +131,10 @@ __device__ __forceinline__ float du_comp(const float4& d, int c) { return c == 0
 // with du's terms written as 0.
 template <bool zero_du>
 __device__ __forceinline__ void flow_grad(const LevelView& v, int c, int l, int i, int j, double& gx, double& gy) {
-  const double* uc = v.u + (size_t)c * v.N;
-  const double dl = zero_du ? 0.0 : (double)du_comp(v.du[l], c);
-  const double dd = zero_du || i == v.H - 1 ? 0.0 : (double)du_comp(v.du[l + v.W], c);
-  const double dr = zero_du || j == v.W - 1 ? 0.0 : (double)du_comp(v.du[l + 1], c);
+  const RGBDFLOW_GLOBAL double* uc = gp(v.u) + (size_t)c * v.N;
+  const double dl = zero_du ? 0.0 : (double)du_comp(gload(v.du + l), c);
+  const double dd = zero_du || i == v.H - 1 ? 0.0 : (double)du_comp(gload(v.du + l + v.W), c);
+  const double dr = zero_du || j == v.W - 1 ? 0.0 : (double)du_comp(gload(v.du + l + 1), c);
   gy = (i == v.H - 1) ? 0.0 : (uc[l + v.W] + dd - uc[l] - dl);
   gx = (j == v.W - 1) ? 0.0 : (uc[l + 1] + dr - uc[l] - dl);
 }
@@ -146,7 +146,7 @@ __device__ __forceinline__ double phi_at(const LevelView& v, int l, int i, int j
   flow_grad<zero_du>(v, 0, l, i, j, ux, uy);
   flow_grad<zero_du>(v, 1, l, i, j, vx, vy);
   flow_grad<zero_du>(v, 2, l, i, j, wx, wy);
-  const float t = sqr_f32(ux) + sqr_f32(uy) + sqr_f32(vx) + sqr_f32(vy) + v.sW[l] * (sqr_f32(wx) + sqr_f32(wy));
+  const float t = sqr_f32(ux) + sqr_f32(uy) + sqr_f32(vx) + sqr_f32(vy) + gp(v.sW)[l] * (sqr_f32(wx) + sqr_f32(wy));
   return 0.5 / sqrt((double)t + 1e-6);
 }
 
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kBlock) k_phi(Views w) {
   const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
   const int i = l / v.W, j = l - i * v.W;
-  v.phi[l] = phi_at<false>(v, l, i, j);
+  gp(v.phi)[l] = phi_at<false>(v, l, i, j);
 }
 
 // .cu:240-333: phi- and r-weighted Laplacian of u (not u+du), normalised by the
@@ -165,9 +165,9 @@ __global__ void __launch_bounds__(kBlock) k_phi(Views w) {
 // pixel, its left and its upper neighbour.
 __device__ __forceinline__ double laplacian(const LevelView& v, int c, int l, int i, int j, double pC, double pL,
                                             double pU) {
-  const double* uc = v.u + (size_t)c * v.N;
-  const float* rR = v.rW;
-  const float* rD = v.rW + v.N;
+  const RGBDFLOW_GLOBAL double* uc = gp(v.u) + (size_t)c * v.N;
+  const RGBDFLOW_GLOBAL float* rR = gp(v.rW);
+  const RGBDFLOW_GLOBAL float* rD = gp(v.rW) + v.N;
   const int W = v.W;
   double lap = 0;
   if (j < W - 1) lap -= (uc[l + 1] - uc[l]) * pC * (double)rR[l];
@@ -212,29 +212,29 @@ __device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars&
 
   const double Dx = dv.Dx, Dy = dv.Dy, Dt = dv.Dt;
   const double t2 = Dt + Dx * d0 + Dy * d1 - d2;
-  const double P = .5 / sqrt((double)sqr_f32(t2) + 1e-10) * (double)v.dW[l];
+  const double P = .5 / sqrt((double)sqr_f32(t2) + 1e-10) * (double)gp(v.dW)[l];
   const double dxy = P * Dx * Dy, dx2 = P * Dx * Dx, dy2 = P * Dy * Dy;
   const double dtx = P * Dx * Dt, dty = P * Dy * Dt;
 
   const double uL = laplacian(v, 0, l, i, j, pC, pL, pU), vL = laplacian(v, 1, l, i, j, pC, pL, pU);
-  const double wL = (double)v.sW[l] * laplacian(v, 2, l, i, j, pC, pL, pU);
+  const double wL = (double)gp(v.sW)[l] * laplacian(v, 2, l, i, j, pC, pL, pU);
 
-  const float* rR = v.rW;
-  const float* rD = v.rW + N;
+  const RGBDFLOW_GLOBAL float* rR = gp(v.rW);
+  const RGBDFLOW_GLOBAL float* rD = gp(v.rW) + N;
   double wsXY = 0, wsZ = 0;
   float eRxy = 0.f, eRz = 0.f, eDxy = 0.f, eDz = 0.f;
-  if (j > 0) { const double a = pL * (double)rR[l - 1], b = a * (double)v.sW[l - 1]; wsXY += a; wsZ += b; }
-  if (i > 0) { const double a = pU * (double)rD[l - W], b = a * (double)v.sW[l - W]; wsXY += a; wsZ += b; }
+  if (j > 0) { const double a = pL * (double)rR[l - 1], b = a * (double)gp(v.sW)[l - 1]; wsXY += a; wsZ += b; }
+  if (i > 0) { const double a = pU * (double)rD[l - W], b = a * (double)gp(v.sW)[l - W]; wsXY += a; wsZ += b; }
   if (j < W - 1) {
-    const double a = pC * (double)rR[l], b = a * (double)v.sW[l];
+    const double a = pC * (double)rR[l], b = a * (double)gp(v.sW)[l];
     eRxy = (float)(alpha * a); eRz = (float)(alpha * b); wsXY += a; wsZ += b;
   }
   if (i < H - 1) {
-    const double a = pC * (double)rD[l], b = a * (double)v.sW[l];
+    const double a = pC * (double)rD[l], b = a * (double)gp(v.sW)[l];
     eDxy = (float)(alpha * a); eDz = (float)(alpha * b); wsXY += a; wsZ += b;
   }
-  const double u0 = v.u[l], u1 = v.u[N + l], u2 = v.u[2 * N + l];
-  const double m = (double)v.mW[l];
+  const double u0 = gp(v.u)[l], u1 = gp(v.u)[N + l], u2 = gp(v.u)[2 * N + l];
+  const double m = (double)gp(v.mW)[l];
   float t;
   t = (float)(-(ix2 + dx2 + alpha * wsXY));
   const float Buu = (float)((double)t + -(gamma));
@@ -267,16 +267,16 @@ __global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
   const int i = l / W, j = l - i * W;
   PixDerivs d;
   for (int k = 0; k < kFeat; k++) {
-    d.Ix[k] = v.Ix[(size_t)k * N + l];
-    d.Iy[k] = v.Iy[(size_t)k * N + l];
-    d.It[k] = v.It[(size_t)k * N + l];
+    d.Ix[k] = gp(v.Ix)[(size_t)k * N + l];
+    d.Iy[k] = gp(v.Iy)[(size_t)k * N + l];
+    d.It[k] = gp(v.It)[(size_t)k * N + l];
   }
-  d.Dx = v.Dx[l];
-  d.Dy = v.Dy[l];
-  d.Dt = v.Dt[l];
-  const double pC = v.phi[l], pL = j > 0 ? v.phi[l - 1] : 0.0, pU = i > 0 ? v.phi[l - W] : 0.0;
-  const float4 du = v.du[l];
-  assemble(v, s, l, i, j, (double)du.x, (double)du.y, (double)du.z, d, (double)v.q[l], pC, pL, pU);
+  d.Dx = gp(v.Dx)[l];
+  d.Dy = gp(v.Dy)[l];
+  d.Dt = gp(v.Dt)[l];
+  const double pC = gp(v.phi)[l], pL = j > 0 ? gp(v.phi)[l - 1] : 0.0, pU = i > 0 ? gp(v.phi)[l - W] : 0.0;
+  const float4 du = gload(v.du + l);
+  assemble(v, s, l, i, j, (double)du.x, (double)du.y, (double)du.z, d, (double)gp(v.q)[l], pC, pL, pU);
 }
 
 // The first inner iteration of an outer iteration in one pass: derivatives,
@@ -308,8 +308,8 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
   for (int t = threadIdx.x; t < kRh * kRw; t += kBlock) {
     const int ry = t / kRw, rx = t - ry * kRw;
     const int o = clampi(y0 - 2 + ry, 0, H - 1) * W + clampi(x0 - 2 + rx, 0, W - 1);
-    for (int k = 0; k < kFeat; k++) Ms[k][t] = .4 * v.I1[(size_t)k * N + o] + .6 * v.Iw[(size_t)k * N + o];
-    Ms[kFeat][t] = .4 * v.D1[o] + .6 * v.Dw[o];
+    for (int k = 0; k < kFeat; k++) Ms[k][t] = .4 * gp(v.I1)[(size_t)k * N + o] + .6 * gp(v.Iw)[(size_t)k * N + o];
+    Ms[kFeat][t] = .4 * gp(v.D1)[o] + .6 * gp(v.Dw)[o];
   }
   for (int t = threadIdx.x; t < kPh * kPw; t += kBlock) {
     const int py = t / kPw, px = t - py * kPw;
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     for (int m = 0; m < 5; m++) sy += g[m] * Ms[k][c0 + (m - 2) * kRw];
     d.Ix[k] = div12(sx);
     d.Iy[k] = div12(sy);
-    d.It[k] = v.Iw[(size_t)k * N + l] - v.I1[(size_t)k * N + l];
+    d.It[k] = gp(v.Iw)[(size_t)k * N + l] - gp(v.I1)[(size_t)k * N + l];
   }
   {
     const double mg = .04;
@@ -340,23 +340,23 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     for (int m = 0; m < 5; m++) sy += g[m] * Ms[kFeat][c0 + (m - 2) * kRw];
     d.Dx = fmin(fmax(div12(sx), -mg), mg);
     d.Dy = fmin(fmax(div12(sy), -mg), mg);
-    d.Dt = fmin(fmax(v.Dw[l] - v.D1[l], -mg), mg);
+    d.Dt = fmin(fmax(gp(v.Dw)[l] - gp(v.D1)[l], -mg), mg);
   }
   const float q = occlusion_at(v, l);
-  v.du[l] = make_float4(0.f, 0.f, 0.f, 0.f);
+  gstore(v.du + l, make_float4(0.f, 0.f, 0.f, 0.f));
   const int p0 = (ty + 1) * kPw + txx + 1;
   const double pC = Ps[p0], pL = Ps[p0 - 1], pU = Ps[p0 - kPw];
   if (store) {
     for (int k = 0; k < kFeat; k++) {
-      v.Ix[(size_t)k * N + l] = d.Ix[k];
-      v.Iy[(size_t)k * N + l] = d.Iy[k];
-      v.It[(size_t)k * N + l] = d.It[k];
+      gp(v.Ix)[(size_t)k * N + l] = d.Ix[k];
+      gp(v.Iy)[(size_t)k * N + l] = d.Iy[k];
+      gp(v.It)[(size_t)k * N + l] = d.It[k];
     }
-    v.Dx[l] = d.Dx;
-    v.Dy[l] = d.Dy;
-    v.Dt[l] = d.Dt;
-    v.q[l] = q;
-    v.phi[l] = pC;
+    gp(v.Dx)[l] = d.Dx;
+    gp(v.Dy)[l] = d.Dy;
+    gp(v.Dt)[l] = d.Dt;
+    gp(v.q)[l] = q;
+    gp(v.phi)[l] = pC;
   }
   assemble(v, s, l, i, j, 0.0, 0.0, 0.0, d, (double)q, pC, pL, pU);
 }
@@ -384,13 +384,13 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
     const PixCoef c = load_coef(v.coef, v.storage, W, l, hl, hu);
     float4* x = v.du;
     const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 o = x[l], L = hl ? x[l - 1] : zero, U = hu ? x[l - W] : zero, R = hr ? x[l + 1] : zero,
-                 D = hd ? x[l + W] : zero;
+    const float4 o = gload(x + l), L = hl ? gload(x + l - 1) : zero, U = hu ? gload(x + l - W) : zero,
+                 R = hr ? gload(x + l + 1) : zero, D = hd ? gload(x + l + W) : zero;
     const float xo[3] = {o.x, o.y, o.z}, xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z},
                 xd[3] = {D.x, D.y, D.z};
     float xn[3];
     relax_update(c, v.omega, v.omega_c, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn);
-    x[l] = make_float4(xn[0], xn[1], xn[2], 0.f);
+    gstore(x + l, make_float4(xn[0], xn[1], xn[2], 0.f));
     change = fmaxf(fmaxf(fabsf(xn[0] - xo[0]), fabsf(xn[1] - xo[1])), fabsf(xn[2] - xo[2]));
   }
   if (check_slot >= 0) {
@@ -415,18 +415,18 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat)
   double term = 0;
   if (l < N) {
     const int i = l / W, j = l - i * W;
-    const float4 e = v.du[l];
+    const float4 e = gload(v.du + l);
     const float e0 = e.x, e1 = e.y, e2 = e.z;
-    const double u0 = v.u[l] + (double)e0;
-    const double u1 = v.u[N + l] + (double)e1;
-    const double u2 = v.u[2 * N + l] + (double)e2;
-    v.u[l] = u0;
-    v.u[N + l] = u1;
-    v.u[2 * N + l] = u2;
+    const double u0 = gp(v.u)[l] + (double)e0;
+    const double u1 = gp(v.u)[N + l] + (double)e1;
+    const double u2 = gp(v.u)[2 * N + l] + (double)e2;
+    gp(v.u)[l] = u0;
+    gp(v.u)[N + l] = u1;
+    gp(v.u)[2 * N + l] = u2;
     const double x = j + u0, y = i + u1;
     if (x < 0 || x > W - 1 || y < 0 || y > H - 1) {
-      for (int k = 0; k < kFeat; k++) v.Iw[(size_t)k * N + l] = v.I1[(size_t)k * N + l];
-      v.Dw[l] = v.D1[l] - u2;
+      for (int k = 0; k < kFeat; k++) gp(v.Iw)[(size_t)k * N + l] = gp(v.I1)[(size_t)k * N + l];
+      gp(v.Dw)[l] = gp(v.D1)[l] - u2;
     } else {
       const int x0 = clampi((int)floor(x), 0, W - 1), y0 = clampi((int)floor(y), 0, H - 1);
       const int x1 = min(W - 1, x0 + 1), y1 = min(H - 1, y0 + 1);
@@ -434,16 +434,16 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat)
       const float w00 = (1 - ax) * (1 - ay), w01 = (1 - ax) * ay, w10 = ax * (1 - ay), w11 = ax * ay;
       const int o00 = y0 * W + x0, o01 = y1 * W + x0, o10 = y0 * W + x1, o11 = y1 * W + x1;
       for (int k = 0; k < kFeat; k++) {
-        const double* b = v.I2 + (size_t)k * N;
-        v.Iw[(size_t)k * N + l] = w00 * b[o00] + w01 * b[o01] + w10 * b[o10] + w11 * b[o11];
+        const RGBDFLOW_GLOBAL double* b = gp(v.I2) + (size_t)k * N;
+        gp(v.Iw)[(size_t)k * N + l] = w00 * b[o00] + w01 * b[o01] + w10 * b[o10] + w11 * b[o11];
       }
-      v.Dw[l] = w00 * v.D2[o00] + w01 * v.D2[o01] + w10 * v.D2[o10] + w11 * v.D2[o11] - u2;
+      gp(v.Dw)[l] = w00 * gp(v.D2)[o00] + w01 * gp(v.D2)[o01] + w10 * gp(v.D2)[o10] + w11 * gp(v.D2)[o11] - u2;
     }
     if (stat) {
       // sqrCU(du_k) * c_k in f32 (thrustSqrMultFunctor), c = (1, 1, f/D1)
       term = (double)(e0 * e0);
       term += (double)(e1 * e1);
-      term += (double)((e2 * e2) * v.cW[l]);
+      term += (double)((e2 * e2) * gp(v.cW)[l]);
     }
   }
   if (!stat) return;
@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat)
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) v.partials[lb] = red[0];
+  if (threadIdx.x == 0) gp(v.partials)[lb] = red[0];
 }
 
 // One block per item: fixed-order sum of the item's partials -> outer
@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(kBlock) k_stat_final(Views w, int outer, int d
   if (v.ctl->outer_done || !v.allow_early) return;
   const int nblocks = pix_blocks(v.N);
   double acc = 0;
-  for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += v.partials[b];
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += gp(v.partials)[b];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = kBlock / 2; s > 0; s >>= 1) {
