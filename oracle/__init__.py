@@ -339,13 +339,14 @@ def check_reciprocal_division(n, seed, mode, recipe=0):
     return int(bad), (a.value, b.value)
 
 
-def check_div12(n, seed, recipe=0):
-    """Samples n f64 x in the domain of the kernels' division by 12
-    (divcheck.cpp) and returns (mismatches against IEEE x / 12, first x)."""
+def check_div_const(n, seed, d, recipe=0):
+    """Samples n f64 x in the domain of the kernels' division by the integer
+    constant d (divcheck.cpp) and returns (mismatches against IEEE x / d,
+    first x)."""
     L = lib()
-    f = L.orc_check_div12
+    f = L.orc_check_div_const
     f.restype = C.c_uint64
-    f.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_double)]
+    f.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_double)]
     x = C.c_double()
-    bad = f(n, seed, recipe, C.byref(x))
+    bad = f(n, seed, d, recipe, C.byref(x))
     return int(bad), x.value

@@ -88,21 +88,25 @@ uint64_t orc_check_reciprocal_division(uint64_t n, uint64_t seed, int mode, int 
 
 extern "C" {
 
-// The derivative stencils (src/sceneFlowSingleScaleJacobiCUDA.cu:84-147) divide
-// f64 sums by 12; the kernels take q0 = RN(x * RN(1/12)), r = fma(-12, q0, x),
-// q = fma(r, RN(1/12), q0) -- Markstein's correction with the constant's
-// correctly rounded reciprocal -- for x != 0 with |x| in [2^-960, 2^960], and
-// the IEEE division otherwise.  Samples x in that domain and counts the cases
-// that differ from x / 12 in any bit (recipe 1: the uncorrected product, a
-// negative control).
-uint64_t orc_check_div12(uint64_t n, uint64_t seed, int recipe, double* ex) {
+// The kernels divide f64 values by small integer constants d: the derivative
+// stencils' sums by 12 (src/sceneFlowSingleScaleJacobiCUDA.cu:84-147), the
+// channel means of the data-term products by 5 (.cu:211) and the sums of the
+// Laplacians' existing regularisation weights by their count 3 (.cu:274-282).
+// They take q0 = RN(x * RN(1/d)), r = fma(-d, q0, x), q = fma(r, RN(1/d), q0)
+// -- Markstein's correction with the constant's correctly rounded reciprocal
+// -- for x != 0 with |x| in [2^-960, 2^960], and the IEEE division otherwise.
+// Samples x in that domain (random bit patterns, and multiples of d, whose
+// quotients are exact) and counts the cases that differ from x / d in any bit
+// (recipe 1: the uncorrected product, a negative control).
+uint64_t orc_check_div_const(uint64_t n, uint64_t seed, int d, int recipe, double* ex) {
   uint64_t s = seed * 0x9E3779B97F4A7C15ull + 7, bad = 0;
-  const double y = 1.0 / 12.0;
+  const double dd = (double)d;
+  const double y = 1.0 / dd;
   for (uint64_t i = 0; i < n; i++) {
     const uint64_t r1 = splitmix(s), r2 = splitmix(s);
     double x;
     if ((r1 & 3) == 0) {
-      x = (double)(int64_t)(r2 % 2000001 - 1000000) * 12.0 * std::ldexp(1.0, (int)((r1 >> 8) % 200) - 100);
+      x = (double)(int64_t)(r2 % 2000001 - 1000000) * dd * std::ldexp(1.0, (int)((r1 >> 8) % 200) - 100);
     } else {
       const int e = (int)((r1 >> 8) % 1921) - 960;
       const uint64_t bits = ((uint64_t)((r1 >> 40) & 1) << 63) | ((uint64_t)(e + 1023) << 52) | (r2 & 0xFFFFFFFFFFFFFull);
@@ -110,8 +114,8 @@ uint64_t orc_check_div12(uint64_t n, uint64_t seed, int recipe, double* ex) {
     }
     if (x == 0.0 || std::fabs(x) < std::ldexp(1.0, -960) || std::fabs(x) > std::ldexp(1.0, 960)) continue;
     const double q0 = x * y;
-    const double q = recipe == 1 ? q0 : std::fma(std::fma(-12.0, q0, x), y, q0);
-    const double ref = x / 12.0;
+    const double q = recipe == 1 ? q0 : std::fma(std::fma(-dd, q0, x), y, q0);
+    const double ref = x / dd;
     if (std::memcmp(&q, &ref, 8) != 0) {
       if (bad == 0 && ex) *ex = x;
       bad++;

@@ -166,6 +166,16 @@ __device__ __forceinline__ int work_item(const WorkList<T>& w, int blk) {
 
 typedef WorkList<LevelView> Views;
 
+// The dispatcher hands workgroup b of a launch to XCD b % 8.  xcd_block(b, nb)
+// renumbers the nb workgroups so that each XCD takes one contiguous range of
+// the launch's blocks, in order: neighbouring tiles and rows then meet in
+// the same L2 instead of being fetched from HBM once per XCD.  A bijection on
+// [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 // One Gaussian-pyramid level of one image (GaussianPyramid.cpp:47-61).
 constexpr int kMaxTaps = 31;
 struct PyrJob {
@@ -199,7 +209,7 @@ __host__ __device__ inline int pix_blocks(int N) { return (N + kBlock - 1) / kBl
 __host__ __device__ inline int plain_blocks(int W, int H) { return (((W + 1) / 2 + 63) / 64) * ((H + 3) / 4); }
 __host__ __device__ inline int tiled_blocks(const TileSched& t) { return t.tiles_x * t.tiles_y; }
 // tiles of the fused first-inner kernel (kBlock threads, one pixel each)
-constexpr int kOtW = 64, kOtH = kBlock / kOtW;
+constexpr int kOtW = 32, kOtH = kBlock / kOtW;
 __host__ __device__ inline int otile_blocks(int W, int H) { return ((W + kOtW - 1) / kOtW) * ((H + kOtH - 1) / kOtH); }
 
 // ---------------------------------------------------------------- launchers

@@ -66,9 +66,10 @@ __global__ void k_interleave(const double* src, int N, int C, double* dst) {
 // evaluated; taps of zero bilinear weight are skipped (adding x*0 leaves the
 // sum unchanged).
 __global__ void __launch_bounds__(kBlock) k_pyramid_jobs(WorkList<PyrJob> w) {
-  const int item = work_item(w, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int item = work_item(w, blk);
   const PyrJob& J = w.items[item];
-  const int l = (blockIdx.x - w.start[item]) * kBlock + threadIdx.x;
+  const int l = (blk - w.start[item]) * kBlock + threadIdx.x;
   const int Ws = J.Ws, Hs = J.Hs, Wd = J.Wd, Hd = J.Hd, fs = J.fs;
   const int Nd = Wd * Hd;
   if (l >= Nd * J.C) return;
@@ -122,9 +123,10 @@ __global__ void __launch_bounds__(kBlock) k_features_planes(const double* bgr, i
 
 // Features of both frames at a level (sceneFlowMultiscale.cpp:102-103).
 __global__ void __launch_bounds__(kBlock) k_features(Views w) {
-  const int item = work_item(w, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int item = work_item(w, blk);
   const LevelView& v = w.items[item];
-  const int l = (blockIdx.x - w.start[item]) * kBlock + threadIdx.x;
+  const int l = (blk - w.start[item]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
   features_at(v.P1k, v.W, v.H, l, const_cast<double*>(v.I1));
   features_at(v.P2k, v.W, v.H, l, const_cast<double*>(v.I2));
@@ -138,9 +140,10 @@ __global__ void __launch_bounds__(kBlock) k_features(Views w) {
 // out of bounds -> frame 1; then depth -= w).  The warp of pixel p reads only
 // u(p), so resize and warp share one pass.
 __global__ void __launch_bounds__(kBlock) k_level_entry(Views wl) {
-  const int item = work_item(wl, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int item = work_item(wl, blk);
   const LevelView& v = wl.items[item];
-  const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
+  const int l = (blk - wl.start[item]) * kBlock + threadIdx.x;
   const int W = v.W, H = v.H, N = v.N;
   if (l >= N) return;
   if (!v.u_prev) {
@@ -201,9 +204,10 @@ __global__ void __launch_bounds__(kBlock) k_level_entry(Views wl) {
 //      reads D1[(l / W_full) * W + l % W_full]),
 //   r (OpticalFlow.cpp:113-141), c = f_level / D1 (.cu:693-700).
 __global__ void __launch_bounds__(kBlock) k_level_weights(Views wl) {
-  const int item = work_item(wl, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int item = work_item(wl, blk);
   const LevelView& v = wl.items[item];
-  const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
+  const int l = (blk - wl.start[item]) * kBlock + threadIdx.x;
   const int W = v.W, H = v.H, N = v.N;
   if (l >= N) return;
   const int i = l / W, j = l - i * W;
@@ -294,9 +298,10 @@ __global__ void __launch_bounds__(kBlock) k_depth_sigma(const float* depth, int 
 // (runFlowTwoFrames.cpp:90, from the float depth), delta = stereoError(1) /
 // sigma * depthVsColorDataWeight (OpticalFlow.cpp:71-81), s = f^2 (:90-97).
 __global__ void __launch_bounds__(kBlock) k_pair_prep(WorkList<PairPrep> wl) {
-  const int item = work_item(wl, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int item = work_item(wl, blk);
   const PairPrep& p = wl.items[item];
-  const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
+  const int l = (blk - wl.start[item]) * kBlock + threadIdx.x;
   if (l >= p.W * p.H) return;
   float sg;
   if (p.compute_sigma) {

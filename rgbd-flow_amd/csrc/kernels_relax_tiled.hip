@@ -244,10 +244,11 @@ k_relax_tiled(Views wl, int total, int flip, int n_iter, int slot_base, int n_pr
 
 // Copies du_alt into du for every item whose last tiled launch wrote du_alt.
 __global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
-  const int item = work_item(wl, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int item = work_item(wl, blk);
   const LevelView& v = wl.items[item];
   if (v.ctl->outer_done || !v.ctl->du_in_alt) return;
-  const int l = (blockIdx.x - wl.start[item]) * kBlock + threadIdx.x;
+  const int l = (blk - wl.start[item]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
   gstore(v.du + l, gload(v.du_alt + l));
 }

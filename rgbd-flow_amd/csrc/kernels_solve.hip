@@ -101,10 +101,11 @@ __device__ __forceinline__ float occlusion_at(const LevelView& v, int l) {
 // Derivatives + occlusion stored for the inner iterations, du = 0 (.cu:776);
 // also zeroes the item's check slots of this outer iteration.
 __global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
-  const int it = work_item(w, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int it = work_item(w, blk);
   const LevelView& v = w.items[it];
   if (v.ctl->outer_done) return;
-  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
+  const int l = (blk - w.start[it]) * kBlock + threadIdx.x;
   const int stride = (w.start[it + 1] - w.start[it]) * kBlock;
   for (int t = l; t < n_slots; t += stride) v.ctl->maxdiff[t] = 0u;
   if (l >= v.N) return;
@@ -139,47 +140,88 @@ __device__ __forceinline__ void flow_grad(const LevelView& v, int c, int l, int 
   gx = (j == v.W - 1) ? 0.0 : (uc[l + 1] + dr - uc[l] - dl);
 }
 
-// .cu:354-355: smoothness weight phi' of the Charbonnier penalty at pixel l.
+// .cu:354-355: smoothness weight phi' of the Charbonnier penalty from the
+// forward differences (x, y) of the three flow components and the pixel's
+// depth-smoothness weight.
+__device__ __forceinline__ double phi_from_grads(double ux, double uy, double vx, double vy, double wx, double wy,
+                                                 float sw) {
+  const float t = sqr_f32(ux) + sqr_f32(uy) + sqr_f32(vx) + sqr_f32(vy) + sw * (sqr_f32(wx) + sqr_f32(wy));
+  return 0.5 / sqrt((double)t + 1e-6);
+}
+
+// phi' at pixel l.
 template <bool zero_du>
 __device__ __forceinline__ double phi_at(const LevelView& v, int l, int i, int j) {
   double ux, uy, vx, vy, wx, wy;
   flow_grad<zero_du>(v, 0, l, i, j, ux, uy);
   flow_grad<zero_du>(v, 1, l, i, j, vx, vy);
   flow_grad<zero_du>(v, 2, l, i, j, wx, wy);
-  const float t = sqr_f32(ux) + sqr_f32(uy) + sqr_f32(vx) + sqr_f32(vy) + gp(v.sW)[l] * (sqr_f32(wx) + sqr_f32(wy));
-  return 0.5 / sqrt((double)t + 1e-6);
+  return phi_from_grads(ux, uy, vx, vy, wx, wy, gp(v.sW)[l]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_phi(Views w) {
-  const int it = work_item(w, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int it = work_item(w, blk);
   const LevelView& v = w.items[it];
   if (v.ctl->outer_done) return;
-  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
+  const int l = (blk - w.start[it]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
   const int i = l / v.W, j = l - i * v.W;
   gp(v.phi)[l] = phi_at<false>(v, l, i, j);
 }
 
-// .cu:240-333: phi- and r-weighted Laplacian of u (not u+du), normalised by the
-// mean regularisation weight of the existing edges.  pC, pL, pU: phi at the
-// pixel, its left and its upper neighbour.
-__device__ __forceinline__ double laplacian(const LevelView& v, int c, int l, int i, int j, double pC, double pL,
-                                            double pU) {
-  const RGBDFLOW_GLOBAL double* uc = gp(v.u) + (size_t)c * v.N;
+// The values around one pixel that the Laplacians and the assembly read
+// (zeros for neighbours outside the image, which are never used).
+struct PixNbr {
+  double uC[3], uL[3], uR[3], uU[3], uD[3];  // u at the pixel and its left, right, upper, lower neighbours
+  float rR, rRl, rD, rDu;  // right weight at the pixel and its left neighbour; down weight at the pixel and above
+  float sC, sL, sU;        // depth-smoothness weights at the pixel, its left and its upper neighbour
+  float dW, mW;            // depth-data and depth-magnitude weights at the pixel
+};
+
+__device__ __forceinline__ void load_nbr(const LevelView& v, int l, int i, int j, PixNbr& nb) {
+  const int W = v.W, H = v.H, N = v.N;
+  const bool hl = j > 0, hr = j < W - 1, hu = i > 0, hd = i < H - 1;
+  const RGBDFLOW_GLOBAL double* u = gp(v.u);
+  for (int c = 0; c < 3; c++) {
+    const size_t o = (size_t)c * N + l;
+    nb.uC[c] = u[o];
+    nb.uL[c] = hl ? u[o - 1] : 0.0;
+    nb.uR[c] = hr ? u[o + 1] : 0.0;
+    nb.uU[c] = hu ? u[o - W] : 0.0;
+    nb.uD[c] = hd ? u[o + W] : 0.0;
+  }
   const RGBDFLOW_GLOBAL float* rR = gp(v.rW);
-  const RGBDFLOW_GLOBAL float* rD = gp(v.rW) + v.N;
-  const int W = v.W;
+  const RGBDFLOW_GLOBAL float* rD = gp(v.rW) + N;
+  const RGBDFLOW_GLOBAL float* sW = gp(v.sW);
+  nb.rR = rR[l];
+  nb.rRl = hl ? rR[l - 1] : 0.f;
+  nb.rD = rD[l];
+  nb.rDu = hu ? rD[l - W] : 0.f;
+  nb.sC = sW[l];
+  nb.sL = hl ? sW[l - 1] : 0.f;
+  nb.sU = hu ? sW[l - W] : 0.f;
+  nb.dW = gp(v.dW)[l];
+  nb.mW = gp(v.mW)[l];
+}
+
+// .cu:240-333: phi- and r-weighted Laplacian of component c of u (not u+du),
+// normalised by the mean regularisation weight of the existing edges.  pC,
+// pL, pU: phi at the pixel, its left and its upper neighbour.
+__device__ __forceinline__ double laplacian(const PixNbr& nb, int c, int W, int H, int i, int j, double pC, double pL,
+                                            double pU) {
+  const double uc = nb.uC[c];
   double lap = 0;
-  if (j < W - 1) lap -= (uc[l + 1] - uc[l]) * pC * (double)rR[l];
-  if (j > 0) lap += (uc[l] - uc[l - 1]) * pL * (double)rR[l - 1];
-  if (i < v.H - 1) lap -= (uc[l + W] - uc[l]) * pC * (double)rD[l];
-  if (i > 0) lap += (uc[l] - uc[l - W]) * pU * (double)rD[l - W];
+  if (j < W - 1) lap -= (nb.uR[c] - uc) * pC * (double)nb.rR;
+  if (j > 0) lap += (uc - nb.uL[c]) * pL * (double)nb.rRl;
+  if (i < H - 1) lap -= (nb.uD[c] - uc) * pC * (double)nb.rD;
+  if (i > 0) lap += (uc - nb.uU[c]) * pU * (double)nb.rDu;
   double norm = 0;
   int count = 0;
-  if (j < W - 1) { norm += rR[l]; count++; }
-  if (j > 0) { norm += rR[l - 1]; count++; }
-  if (i < v.H - 1) { norm += rD[l]; count++; }
-  if (i > 0) { norm += rD[l - W]; count++; }
+  if (j < W - 1) { norm += nb.rR; count++; }
+  if (j > 0) { norm += nb.rRl; count++; }
+  if (i < H - 1) { norm += nb.rD; count++; }
+  if (i > 0) { norm += nb.rDu; count++; }
   if (fabs(norm) < 1e-5) lap = 0;
   else lap /= (norm / count);
   return lap;
@@ -191,8 +233,8 @@ __device__ __forceinline__ double laplacian(const LevelView& v, int c, int l, in
 // unused where the neighbour does not exist).
 __device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars& s, int l, int i, int j, double d0,
                                          double d1, double d2, const PixDerivs& dv, double ql, double pC, double pL,
-                                         double pU) {
-  const int W = v.W, H = v.H, N = v.N;
+                                         double pU, const PixNbr& nb) {
+  const int W = v.W, H = v.H;
   const double alpha = s.alpha, gamma = s.gamma;
   double sxy = 0, sxx = 0, syy = 0, stx = 0, sty = 0;
   double psi[kFeat];
@@ -212,29 +254,27 @@ __device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars&
 
   const double Dx = dv.Dx, Dy = dv.Dy, Dt = dv.Dt;
   const double t2 = Dt + Dx * d0 + Dy * d1 - d2;
-  const double P = .5 / sqrt((double)sqr_f32(t2) + 1e-10) * (double)gp(v.dW)[l];
+  const double P = .5 / sqrt((double)sqr_f32(t2) + 1e-10) * (double)nb.dW;
   const double dxy = P * Dx * Dy, dx2 = P * Dx * Dx, dy2 = P * Dy * Dy;
   const double dtx = P * Dx * Dt, dty = P * Dy * Dt;
 
-  const double uL = laplacian(v, 0, l, i, j, pC, pL, pU), vL = laplacian(v, 1, l, i, j, pC, pL, pU);
-  const double wL = (double)gp(v.sW)[l] * laplacian(v, 2, l, i, j, pC, pL, pU);
+  const double uL = laplacian(nb, 0, W, H, i, j, pC, pL, pU), vL = laplacian(nb, 1, W, H, i, j, pC, pL, pU);
+  const double wL = (double)nb.sC * laplacian(nb, 2, W, H, i, j, pC, pL, pU);
 
-  const RGBDFLOW_GLOBAL float* rR = gp(v.rW);
-  const RGBDFLOW_GLOBAL float* rD = gp(v.rW) + N;
   double wsXY = 0, wsZ = 0;
   float eRxy = 0.f, eRz = 0.f, eDxy = 0.f, eDz = 0.f;
-  if (j > 0) { const double a = pL * (double)rR[l - 1], b = a * (double)gp(v.sW)[l - 1]; wsXY += a; wsZ += b; }
-  if (i > 0) { const double a = pU * (double)rD[l - W], b = a * (double)gp(v.sW)[l - W]; wsXY += a; wsZ += b; }
+  if (j > 0) { const double a = pL * (double)nb.rRl, b = a * (double)nb.sL; wsXY += a; wsZ += b; }
+  if (i > 0) { const double a = pU * (double)nb.rDu, b = a * (double)nb.sU; wsXY += a; wsZ += b; }
   if (j < W - 1) {
-    const double a = pC * (double)rR[l], b = a * (double)gp(v.sW)[l];
+    const double a = pC * (double)nb.rR, b = a * (double)nb.sC;
     eRxy = (float)(alpha * a); eRz = (float)(alpha * b); wsXY += a; wsZ += b;
   }
   if (i < H - 1) {
-    const double a = pC * (double)rD[l], b = a * (double)gp(v.sW)[l];
+    const double a = pC * (double)nb.rD, b = a * (double)nb.sC;
     eDxy = (float)(alpha * a); eDz = (float)(alpha * b); wsXY += a; wsZ += b;
   }
-  const double u0 = gp(v.u)[l], u1 = gp(v.u)[N + l], u2 = gp(v.u)[2 * N + l];
-  const double m = (double)gp(v.mW)[l];
+  const double u0 = nb.uC[0], u1 = nb.uC[1], u2 = nb.uC[2];
+  const double m = (double)nb.mW;
   float t;
   t = (float)(-(ix2 + dx2 + alpha * wsXY));
   const float Buu = (float)((double)t + -(gamma));
@@ -258,10 +298,11 @@ __device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars&
 // The system of an inner iteration after the first, from the stored
 // derivatives, occlusion weights and phi (k_phi) and the current du.
 __global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
-  const int it = work_item(w, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int it = work_item(w, blk);
   const LevelView& v = w.items[it];
   if (v.ctl->outer_done) return;
-  const int l = (blockIdx.x - w.start[it]) * kBlock + threadIdx.x;
+  const int l = (blk - w.start[it]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
   const int W = v.W, N = v.N;
   const int i = l / W, j = l - i * W;
@@ -276,28 +317,40 @@ __global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
   d.Dt = gp(v.Dt)[l];
   const double pC = gp(v.phi)[l], pL = j > 0 ? gp(v.phi)[l - 1] : 0.0, pU = i > 0 ? gp(v.phi)[l - W] : 0.0;
   const float4 du = gload(v.du + l);
-  assemble(v, s, l, i, j, (double)du.x, (double)du.y, (double)du.z, d, (double)gp(v.q)[l], pC, pL, pU);
+  PixNbr nb;
+  load_nbr(v, l, i, j, nb);
+  assemble(v, s, l, i, j, (double)du.x, (double)du.y, (double)du.z, d, (double)gp(v.q)[l], pC, pL, pU, nb);
 }
 
 // The first inner iteration of an outer iteration in one pass: derivatives,
 // occlusion weight, du = 0, phi at the pixel and at its left and upper
-// neighbours (recomputed from u: du is zero), and the system -- the values
-// k_derivs, k_phi and k_system compute, without storing the intermediates.
-// Each block owns a kOtW x kOtH tile: the stencil inputs 0.4*I1 + 0.6*Iw of
-// the tile and a 2-pixel border (clamped to the image, as the stencils clamp)
-// and phi of the tile and its upper row / left column are staged in LDS once.
-// store: also store derivatives, occlusion weights and phi for later inner
-// iterations.  Zeroes the item's check slots of this outer iteration.
-constexpr int kRw = kOtW + 4, kRh = kOtH + 4;  // stencil region
-constexpr int kPw = kOtW + 1, kPh = kOtH + 1;  // phi region
+// neighbours (du is zero), and the system -- the values k_derivs, k_phi and
+// k_system compute, without storing the intermediates.  Each block owns a
+// kOtW x kOtH tile and reads everything it needs in one load phase, all
+// loads in flight together: its own pixels' centre values into registers,
+// and into LDS the stencil inputs 0.4*I1 + 0.6*Iw of the tile and a 2-pixel
+// border (clamped to the image, as the stencils clamp), and u, the
+// regularisation and depth-smoothness weights of the tile and a 1-pixel
+// border (zeros outside the image).  phi of the tile, its left column and
+// upper row then comes from LDS, and so does every neighbour value of the
+// Laplacians and the assembly: after the load phase a pixel touches HBM only
+// to store its record.  store: also store derivatives, occlusion weights and
+// phi for later inner iterations.  Zeroes the item's check slots of this
+// outer iteration.
+constexpr int kRw = kOtW + 4, kRh = kOtH + 4;  // stencil region (2-pixel border)
+constexpr int kUw = kOtW + 2, kUh = kOtH + 2;  // neighbourhood region (1-pixel border)
+constexpr int kPw = kOtW + 1, kPh = kOtH + 1;  // phi region (tile, left column, upper row)
 
 __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, SolveScalars s, int store) {
   __shared__ double Ms[kFeat + 1][kRh * kRw];  // 0.4*I1 + 0.6*Iw (5 features, depth)
-  __shared__ double Ps[kPh * kPw];            // phi
-  const int it = work_item(w, blockIdx.x);
+  __shared__ double Us[3][kUh * kUw];          // u
+  __shared__ float Ss[kUh * kUw], RRs[kUh * kUw], RDs[kUh * kUw];  // sW, rW right, rW down
+  __shared__ double Ps[kPh * kPw];             // phi
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int it = work_item(w, blk);
   const LevelView& v = w.items[it];
   if (v.ctl->outer_done) return;
-  const int lb = blockIdx.x - w.start[it];
+  const int lb = blk - w.start[it];
   {
     const int stride = (w.start[it + 1] - w.start[it]) * kBlock;
     for (int t = lb * kBlock + threadIdx.x; t < n_slots; t += stride) v.ctl->maxdiff[t] = 0u;
@@ -305,22 +358,68 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
   const int W = v.W, H = v.H, N = v.N;
   const int tx = (W + kOtW - 1) / kOtW;
   const int x0 = (lb % tx) * kOtW, y0 = (lb / tx) * kOtH;
+  const int ty = threadIdx.x / kOtW, txx = threadIdx.x - ty * kOtW;
+  const int i = y0 + ty, j = x0 + txx;
+  const bool inside = i < H && j < W;
+  const int l = inside ? i * W + j : 0;
+  const RGBDFLOW_GLOBAL double* I1 = gp(v.I1);
+  const RGBDFLOW_GLOBAL double* Iw = gp(v.Iw);
+  const RGBDFLOW_GLOBAL double* D1 = gp(v.D1);
+  const RGBDFLOW_GLOBAL double* Dw = gp(v.Dw);
+  // the pixel's own values
+  double i1c[kFeat], iwc[kFeat], d1c = 0, dwc = 0, d2c = 0;
+  float dwt = 0.f, mwt = 0.f;
+  if (inside) {
+    for (int k = 0; k < kFeat; k++) {
+      i1c[k] = I1[(size_t)k * N + l];
+      iwc[k] = Iw[(size_t)k * N + l];
+    }
+    d1c = D1[l];
+    dwc = Dw[l];
+    d2c = gp(v.D2)[l];
+    dwt = gp(v.dW)[l];
+    mwt = gp(v.mW)[l];
+  }
   for (int t = threadIdx.x; t < kRh * kRw; t += kBlock) {
     const int ry = t / kRw, rx = t - ry * kRw;
     const int o = clampi(y0 - 2 + ry, 0, H - 1) * W + clampi(x0 - 2 + rx, 0, W - 1);
-    for (int k = 0; k < kFeat; k++) Ms[k][t] = .4 * gp(v.I1)[(size_t)k * N + o] + .6 * gp(v.Iw)[(size_t)k * N + o];
-    Ms[kFeat][t] = .4 * gp(v.D1)[o] + .6 * gp(v.Dw)[o];
+    for (int k = 0; k < kFeat; k++) Ms[k][t] = .4 * I1[(size_t)k * N + o] + .6 * Iw[(size_t)k * N + o];
+    Ms[kFeat][t] = .4 * D1[o] + .6 * Dw[o];
   }
-  for (int t = threadIdx.x; t < kPh * kPw; t += kBlock) {
-    const int py = t / kPw, px = t - py * kPw;
-    const int i = y0 - 1 + py, j = x0 - 1 + px;
-    Ps[t] = (i >= 0 && j >= 0 && i < H && j < W) ? phi_at<true>(v, i * W + j, i, j) : 0.0;
+  for (int t = threadIdx.x; t < kUh * kUw; t += kBlock) {
+    const int ry = t / kUw, rx = t - ry * kUw;
+    const int gy = y0 - 1 + ry, gx = x0 - 1 + rx;
+    if (gy >= 0 && gx >= 0 && gy < H && gx < W) {
+      const int o = gy * W + gx;
+      for (int c = 0; c < 3; c++) Us[c][t] = gp(v.u)[(size_t)c * N + o];
+      Ss[t] = gp(v.sW)[o];
+      RRs[t] = gp(v.rW)[o];
+      RDs[t] = gp(v.rW)[(size_t)N + o];
+    } else {
+      for (int c = 0; c < 3; c++) Us[c][t] = 0.0;
+      Ss[t] = RRs[t] = RDs[t] = 0.f;
+    }
   }
   __syncthreads();
-  const int ty = threadIdx.x / kOtW, txx = threadIdx.x - ty * kOtW;
-  const int i = y0 + ty, j = x0 + txx;
-  if (i >= H || j >= W) return;
-  const int l = i * W + j;
+  // phi with du = 0 (flow_grad<true>, phi_at<true>) from the staged u and sW
+  for (int t = threadIdx.x; t < kPh * kPw; t += kBlock) {
+    const int py = t / kPw, px = t - py * kPw;
+    const int pi = y0 - 1 + py, pj = x0 - 1 + px;
+    double r = 0.0;
+    if (pi >= 0 && pj >= 0 && pi < H && pj < W) {
+      const int c = py * kUw + px;  // (pi, pj) in the neighbourhood region
+      const bool last_row = pi == H - 1, last_col = pj == W - 1;
+      double gx[3], gy[3];
+      for (int k = 0; k < 3; k++) {
+        gy[k] = last_row ? 0.0 : (Us[k][c + kUw] + 0.0 - Us[k][c] - 0.0);
+        gx[k] = last_col ? 0.0 : (Us[k][c + 1] + 0.0 - Us[k][c] - 0.0);
+      }
+      r = phi_from_grads(gx[0], gy[0], gx[1], gy[1], gx[2], gy[2], Ss[c]);
+    }
+    Ps[t] = r;
+  }
+  __syncthreads();
+  if (!inside) return;
   // derivs_at, with the stencil inputs from LDS (same values, same order)
   PixDerivs d;
   const double g[5] = {1, -8, 0, 8, -1};
@@ -331,7 +430,7 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     for (int m = 0; m < 5; m++) sy += g[m] * Ms[k][c0 + (m - 2) * kRw];
     d.Ix[k] = div12(sx);
     d.Iy[k] = div12(sy);
-    d.It[k] = gp(v.Iw)[(size_t)k * N + l] - gp(v.I1)[(size_t)k * N + l];
+    d.It[k] = iwc[k] - i1c[k];
   }
   {
     const double mg = .04;
@@ -340,9 +439,28 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     for (int m = 0; m < 5; m++) sy += g[m] * Ms[kFeat][c0 + (m - 2) * kRw];
     d.Dx = fmin(fmax(div12(sx), -mg), mg);
     d.Dy = fmin(fmax(div12(sy), -mg), mg);
-    d.Dt = fmin(fmax(gp(v.Dw)[l] - gp(v.D1)[l], -mg), mg);
+    d.Dt = fmin(fmax(dwc - d1c, -mg), mg);
   }
-  const float q = occlusion_at(v, l);
+  PixNbr nb;
+  const int cu = (ty + 1) * kUw + txx + 1;
+  for (int k = 0; k < 3; k++) {
+    nb.uC[k] = Us[k][cu];
+    nb.uL[k] = Us[k][cu - 1];
+    nb.uR[k] = Us[k][cu + 1];
+    nb.uU[k] = Us[k][cu - kUw];
+    nb.uD[k] = Us[k][cu + kUw];
+  }
+  nb.rR = RRs[cu];
+  nb.rRl = RRs[cu - 1];
+  nb.rD = RDs[cu];
+  nb.rDu = RDs[cu - kUw];
+  nb.sC = Ss[cu];
+  nb.sL = Ss[cu - 1];
+  nb.sU = Ss[cu - kUw];
+  nb.dW = dwt;
+  nb.mW = mwt;
+  // occlusion_at (.cu:640-648): the unwarped D2
+  const float q = (d1c + nb.uC[2] < d2c + .02 * (double)sqr_f32(d1c)) ? 1.f : (float).01;
   gstore(v.du + l, make_float4(0.f, 0.f, 0.f, 0.f));
   const int p0 = (ty + 1) * kPw + txx + 1;
   const double pC = Ps[p0], pL = Ps[p0 - 1], pU = Ps[p0 - kPw];
@@ -358,7 +476,7 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     gp(v.q)[l] = q;
     gp(v.phi)[l] = pC;
   }
-  assemble(v, s, l, i, j, 0.0, 0.0, 0.0, d, (double)q, pC, pL, pU);
+  assemble(v, s, l, i, j, 0.0, 0.0, 0.0, d, (double)q, pC, pL, pU, nb);
 }
 
 // .cu:382-470: one colour of the red-black relaxation, one thread per pixel of
@@ -366,12 +484,13 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
 // launch per colour per iteration, as the reference launches it).
 __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int slot_base, int n_prev_checks,
                                                      int check_slot) {
-  const int it = work_item(w, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int it = work_item(w, blk);
   const LevelView& v = w.items[it];
   if (v.ctl->outer_done) return;
   if (relax_stopped(v.ctl, slot_base, n_prev_checks)) return;
   const int W = v.W, H = v.H, N = v.N;
-  const int lb = blockIdx.x - w.start[it];
+  const int lb = blk - w.start[it];
   const int gx = ((W + 1) / 2 + 63) / 64;
   const int bx = lb % gx, by = lb / gx;
   const int xc = bx * 64 + (threadIdx.x & 63);
@@ -405,10 +524,11 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
 // sum(du_k^2 * c_k) (.cu:948), accumulated in f64 in a fixed order.
 __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat) {
   __shared__ double red[kBlock];
-  const int item = work_item(wl, blockIdx.x);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int item = work_item(wl, blk);
   const LevelView& v = wl.items[item];
   if (v.ctl->outer_done) return;
-  const int lb = blockIdx.x - wl.start[item];
+  const int lb = blk - wl.start[item];
   const int l = lb * kBlock + threadIdx.x;
   const int W = v.W, H = v.H, N = v.N;
   const bool stat = want_stat && v.allow_early;

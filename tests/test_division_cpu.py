@@ -21,9 +21,12 @@ def test_negative_control_is_caught(mode):
     assert bad > 1000
 
 
-def test_div12_is_ieee():
-    """The derivative stencils' f64 division by 12 (kernels_solve.hip div12)."""
-    bad, x = oracle().check_div12(5_000_000, 3)
+@pytest.mark.parametrize("d", [12, 5, 3])
+def test_division_by_constant_is_ieee(d):
+    """The kernels' f64 divisions by 12 (derivative stencils), 5 (channel
+    means) and 3 (Laplacian weight means): Markstein's correction is bit-equal
+    to IEEE x / d on the sampled domain; the bare product is not."""
+    bad, x = oracle().check_div_const(5_000_000, 3 + d, d)
     assert bad == 0, x
-    bad, _ = oracle().check_div12(200_000, 4, recipe=1)
+    bad, _ = oracle().check_div_const(200_000, 4 + d, d, recipe=1)
     assert bad > 1000
