@@ -327,6 +327,20 @@ def with_init(im1, dm1, im2, dm2, sigma1, hpb0, est, params: Params):
     return u, v, w, st
 
 
+def set_threads(n):
+    """Threads of the level solve's row partitions (1 = scalar, the default);
+    results do not depend on the count."""
+    L = lib()
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_threads(int(n))
+
+
+def threads():
+    L = lib()
+    L.orc_threads.restype = C.c_int
+    return int(L.orc_threads())
+
+
 def check_reciprocal_division(n, seed, mode, recipe=0):
     """Samples n (a, b) in the reciprocal division's domain (divcheck.cpp) and
     returns (mismatches against IEEE a / b, first mismatching (a, b))."""

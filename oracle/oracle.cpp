@@ -20,15 +20,85 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <thread>
 #include <unordered_set>
 #include <vector>
 
 namespace {
 
 using std::vector;
+
+/* Row partitions on a persistent pool of std::threads, for the CPU baseline
+ * on all cores.  Ranges follow src/parallelism.cpp:24-33 partitionEvenly
+ * (n rows into k contiguous parts, the first n mod k one row longer).  The
+ * level solve's loops write disjoint rows and its reductions are maxima, so
+ * results do not depend on the thread count.  Workers spin (yielding) on a
+ * generation counter between jobs: the solve dispatches thousands of short
+ * loops per pair.  One caller at a time. */
+class RowPool {
+ public:
+  ~RowPool() { set_threads(1); }
+  int threads() const { return nt_; }
+  void set_threads(int n) {
+    n = std::max(1, std::min(n, 256));
+    if (n == nt_) return;
+    quit_.store(true);
+    for (auto& t : th_) t.join();
+    th_.clear();
+    quit_.store(false);
+    gen_.store(0);
+    nt_ = n;
+    for (int k = 1; k < nt_; k++) th_.emplace_back([this, k] { worker(k); });
+  }
+  /* f(r0, r1, part) over the partitions of [0, n) */
+  void rows(int n, const std::function<void(int, int, int)>& f) {
+    if (nt_ == 1 || n < 2 * nt_) {
+      f(0, n, 0);
+      return;
+    }
+    n_ = n;
+    job_ = &f;
+    left_.store(nt_ - 1);
+    gen_.fetch_add(1, std::memory_order_release);
+    run_part(0);
+    while (left_.load(std::memory_order_acquire) != 0) std::this_thread::yield();
+  }
+
+ private:
+  void run_part(int k) const {
+    const int q = n_ / nt_, r = n_ % nt_;
+    const int r0 = k * q + std::min(k, r), r1 = r0 + q + (k < r ? 1 : 0);
+    (*job_)(r0, r1, k);
+  }
+  void worker(int k) {
+    int seen = 0;
+    for (;;) {
+      int g;
+      while ((g = gen_.load(std::memory_order_acquire)) == seen) {
+        if (quit_.load()) return;
+        std::this_thread::yield();
+      }
+      seen = g;
+      run_part(k);
+      left_.fetch_sub(1, std::memory_order_acq_rel);
+    }
+  }
+  int nt_ = 1, n_ = 0;
+  const std::function<void(int, int, int)>* job_ = nullptr;
+  std::vector<std::thread> th_;
+  std::atomic<int> gen_{0}, left_{0};
+  std::atomic<bool> quit_{false};
+};
+
+RowPool& pool() {
+  static RowPool p;
+  return p;
+}
 
 /* project.h:29-39 -- __min/__max return the FIRST argument's type */
 template <class A, class B> A liu_min(A a, B b) { return (a > b) ? (A)b : a; }
@@ -423,7 +493,8 @@ struct Solver {
   void derivatives() {
     static const double g[5] = {1, -8, 0, 8, -1};
     const double* I1 = in->I1;
-    for (int i = 0; i < H; i++)
+    pool().rows(H, [&](int i0, int i1, int) {
+    for (int i = i0; i < i1; i++)
       for (int j = 0; j < W; j++) {
         const size_t l = (size_t)i * W + j;
         for (int k = 0; k < C; k++) {
@@ -456,14 +527,17 @@ struct Solver {
         Dy[l] = std::min(std::max(sy / 12, -mg), mg);
         Dt[l] = std::min(std::max(Dw[l] - in->D1[l], -mg), mg);
       }
+    });
   }
 
   /* .cu:640-648 fillOcclusionPenaltyKernel (unwarped D2) */
   void occlusion() {
-    for (size_t l = 0; l < N; l++) {
-      const double d1 = in->D1[l];
-      q[l] = (d1 + u[2 * N + l] < in->D2[l] + .02 * sq(d1)) ? (WT)1 : (WT).01;
-    }
+    pool().rows(H, [&](int i0, int i1, int) {
+      for (size_t l = (size_t)i0 * W; l < (size_t)i1 * W; l++) {
+        const double d1 = in->D1[l];
+        q[l] = (d1 + u[2 * N + l] < in->D2[l] + .02 * sq(d1)) ? (WT)1 : (WT).01;
+      }
+    });
   }
 
   /* .cu:156-182 forward differences of u+du for component c, pixel (i,j) */
@@ -479,7 +553,8 @@ struct Solver {
   void penalties() {
     const double epsD = 1e-6, epsZ = 1e-10, epsS = 1e-6;
     const double cw = p->color_data_weight;
-    for (int i = 0; i < H; i++)
+    pool().rows(H, [&](int i0, int i1, int) {
+    for (int i = i0; i < i1; i++)
       for (int j = 0; j < W; j++) {
         const size_t l = (size_t)i * W + j;
         double ux, uy, vx, vy, wx, wy;
@@ -504,6 +579,7 @@ struct Solver {
         const double t2 = Dt[l] + Dx[l] * d0 + Dy[l] * d1 - d2;
         pi[l] = .5 / std::sqrt(sq(t2) + epsZ) * (double)dW[l];
       }
+    });
   }
 
   /* .cu:240-333 uv/w flow Laplacians (on u only, normalised by the mean
@@ -537,7 +613,8 @@ struct Solver {
     CT* Buu = coef.data() + 4 * N; CT* Bvv = coef.data() + 5 * N; CT* Bww = coef.data() + 6 * N;
     CT* Buv = coef.data() + 7 * N; CT* Buw = coef.data() + 8 * N; CT* Bvw = coef.data() + 9 * N;
     CT* ru = coef.data() + 10 * N; CT* rv = coef.data() + 11 * N; CT* rw = coef.data() + 12 * N;
-    for (int i = 0; i < H; i++)
+    pool().rows(H, [&](int i0, int i1, int) {
+    for (int i = i0; i < i1; i++)
       for (int j = 0; j < W; j++) {
         const size_t l = (size_t)i * W + j;
         /* products (multiply3CollapseKernel / ...Collapse2Kernel / multiply3Kernel) */
@@ -582,6 +659,7 @@ struct Solver {
         t = (CT)(-(P + alpha * wsZ)); Bww[l] = (CT)((double)t + -(gamma * (double)mW[l]));
         t = (CT)(-(P * Dt[l] + alpha * -wL)); rw[l] = (CT)((double)t + -(-gamma * (double)mW[l] * u2));
       }
+    });
   }
 
   void build_system() {
@@ -604,8 +682,10 @@ struct Solver {
     // .cu:461 `const float omega = .7`; a variant factor is given as a float too
     const CT omega = (CT)omega_f;
     const CT step = (CT).3;
+    CT part_max[256] = {};
+    pool().rows(H, [&](int i0, int i1, int part) {
     CT maxd = 0;
-    for (int i = 0; i < H; i++)
+    for (int i = i0; i < i1; i++)
       for (int j = (i + colour) & 1; j < W; j += 2) {
         const size_t l = (size_t)i * W + j;
         CT xo[3] = {x[l], x[N + l], x[2 * N + l]};
@@ -632,6 +712,11 @@ struct Solver {
           if (d > maxd) maxd = d;
         }
       }
+    part_max[part] = maxd;
+    });
+    CT maxd = 0;
+    for (int k = 0; k < pool().threads(); k++)
+      if (part_max[k] > maxd) maxd = part_max[k];
     return maxd;
   }
 
@@ -639,7 +724,8 @@ struct Solver {
   void warp_gpu() {
     typedef typename std::conditional<REF, float, double>::type AT;
     const double* I1 = in->I1; const double* I2 = in->I2;
-    for (int i = 0; i < H; i++)
+    pool().rows(H, [&](int i0, int i1, int) {
+    for (int i = i0; i < i1; i++)
       for (int j = 0; j < W; j++) {
         const size_t l = (size_t)i * W + j;
         const double x = j + u[l], y = i + u[N + l];
@@ -660,6 +746,7 @@ struct Solver {
                   w11 * D2[(size_t)y1 * W + x1] - u[2 * N + l];
         }
       }
+    });
   }
 
   /* .cu:755-954: the outer/inner loops with their exits */
@@ -927,12 +1014,23 @@ int orc_multiscale(const double* im1, const double* dm1, const double* im2, cons
   vector<double> delta(N), sm(N);
   orc_depth_data_weights(sigma1, W, H, p->depth_vs_color_data_weight, delta.data());
   for (size_t l = 0; l < N; l++) sm[l] = dsq(p->cam.focalLength);
-  auto P1 = gaussian_pyramid(im1, W, H, 3, ratio, p->min_width);
-  auto P2 = gaussian_pyramid(im2, W, H, 3, ratio, p->min_width);
-  auto PD1 = gaussian_pyramid(d1.data(), W, H, 1, ratio, p->min_width);
-  auto PD2 = gaussian_pyramid(d2.data(), W, H, 1, ratio, p->min_width);
-  auto Pdelta = gaussian_pyramid(delta.data(), W, H, 1, ratio, p->min_width);
-  auto Ps = gaussian_pyramid(sm.data(), W, H, 1, ratio, p->min_width);
+  // the six pyramids, as concurrent tasks when threads are enabled
+  // (sceneFlowMultiscale.cpp:40-77 builds them in a thread group)
+  vector<Level> P1, P2, PD1, PD2, Pdelta, Ps;
+  {
+    const double* src[6] = {im1, im2, d1.data(), d2.data(), delta.data(), sm.data()};
+    const int nc[6] = {3, 3, 1, 1, 1, 1};
+    vector<Level>* dst[6] = {&P1, &P2, &PD1, &PD2, &Pdelta, &Ps};
+    auto task = [&](int t) { *dst[t] = gaussian_pyramid(src[t], W, H, nc[t], ratio, p->min_width); };
+    if (pool().threads() > 1) {
+      vector<std::thread> th;
+      for (int t = 1; t < 6; t++) th.emplace_back(task, t);
+      task(0);
+      for (auto& x : th) x.join();
+    } else {
+      for (int t = 0; t < 6; t++) task(t);
+    }
+  }
   const int L = (int)P1.size();
   vector<size_t> hpbOff(L, 0);
   for (int k = 1; k < L; k++) hpbOff[k] = hpbOff[k - 1] + (size_t)P1[k - 1].W * P1[k - 1].H * 2;
@@ -1021,5 +1119,8 @@ int orc_with_init(const double* im1, const double* dm1, const double* im2, const
   std::memcpy(wo, w.data(), N * sizeof(double));
   return 0;
 }
+
+void orc_set_threads(int n) { pool().set_threads(n); }
+int orc_threads(void) { return pool().threads(); }
 
 }  // extern "C"

@@ -120,6 +120,11 @@ int orc_with_init(const double* im1, const double* dm1, const double* im2, const
                   const float* sigma1, const float* hpb0, const double* est, int W, int H,
                   const orc_params* p, double* u, double* v, double* w, orc_stats* st);
 
+/* Threads of the level solve's row partitions (1: the scalar restatement,
+ * the default).  Results are identical for every count. */
+void orc_set_threads(int n);
+int orc_threads(void);
+
 #ifdef __cplusplus
 }
 #endif
