@@ -42,11 +42,14 @@ struct PixRegs {
 // LDS: per colour plane of Si slots over the region padded by one pair on
 // every side, XA = (x, y, z, eRxy) and XB = (eRz, eDxy, eDz, 0); pads and
 // absent pixels hold zeros.
+// dmax: pixels farther than dmax (Chebyshev distance, in pixels) from the
+// tile's interior are left as they are -- no value the interior ends with
+// depends on them any more.
 template <int P, int C>
 __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int PRp, float omega, float omega_c,
                                            const PixRegs (&pr)[P][2],
                                            const int (&qi)[P], const int (&par)[P], const unsigned (&fl)[P][2],
-                                           bool last, float& change) {
+                                           int dmax, bool last, float& change) {
   constexpr int O = 1 - C;  // the colour read
   const float4* Ao = XA + O * Si;
   const float4* Bo = XB + O * Si;
@@ -54,7 +57,7 @@ __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int P
 #pragma unroll
   for (int s = 0; s < P; s++) {
     const unsigned f = fl[s][C];
-    if (!(f & 16u)) continue;  // no pixel of this colour
+    if (!(f & 16u) || (int)((f >> 8) & 255u) > dmax) continue;  // no pixel of this colour, or outside the cone
     const int e = par[s];
     const int idx = qi[s];
     // same-row neighbours of the other colour: red reads pairs (k-1+e, k+e),
@@ -117,7 +120,6 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
   const int rx0 = max(0, ox - h) & ~1, ry0 = max(0, oy - h);
   const int rx1 = min(W, ex + h), ry1 = min(H, ey + h);
   const int PR = (rx1 - rx0 + 1) >> 1, RH = ry1 - ry0;
-  const int np = PR * RH;
   const int PRp = PR + 2;
   const int Si = PRp * (RH + 2);  // slots per colour plane
   float4* XB = XA + 2 * Si;
@@ -139,7 +141,10 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
 
   PixRegs pr[P][2];
   int qi[P], par[P];
-  unsigned fl[P][2];  // bits: 16 pixel valid; 32 interior; 64 diagonals in the reciprocal division's domain
+  const int np = PR * RH;
+  // bits: 16 pixel valid; 32 interior; 64 diagonals in the reciprocal
+  // division's domain; 8-15 distance to the interior (Chebyshev, capped)
+  unsigned fl[P][2];
 #pragma unroll
   for (int s = 0; s < P; s++) {
     const int q = threadIdx.x + s * NT;
@@ -153,7 +158,10 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
       const int gc = rx0 + 2 * kk + (c == 0 ? e : 1 - e);
       const bool valid = q < np && gc < W;
       unsigned f = 0u;
-      if (valid) f |= 16u;
+      if (valid) {
+        const int dy = max(max(oy - gy, gy - (ey - 1)), 0), dx = max(max(ox - gc, gc - (ex - 1)), 0);
+        f |= 16u | ((unsigned)min(max(dx, dy), 255) << 8);
+      }
       if (valid && gc >= ox && gc < ex && gy >= oy && gy < ey) f |= 32u;
       const int l = gy * W + gc;
       PixRegs r{};
@@ -195,12 +203,19 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
   // FIXED: the reference's factor 0.7 as constants (a factor held in registers
   // costs the widest configurations spills in the iteration loop)
   const float om = FIXED ? .7f : v.omega, om_c = FIXED ? 1.f - .7f : v.omega_c;
+  // After iteration it only pixels within 2 * (n_iter - 1 - it) of the
+  // interior still matter: the second colour's pass there reads the first
+  // colour within one more pixel, which reads the previous iteration's
+  // second colour within one more again.  Each pass therefore updates the
+  // cone it must (whole waves outside it skip), and the interior ends with
+  // the same values as when every region pixel is updated.
   for (int it = 0; it < n_iter; it++) {
     const bool last = it == n_iter - 1;
+    const int d = 2 * (n_iter - 1 - it);
     __syncthreads();
-    tiled_pass<P, 0>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, last, change);
+    tiled_pass<P, 0>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, d + 1, last, change);
     __syncthreads();
-    tiled_pass<P, 1>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, last, change);
+    tiled_pass<P, 1>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, d, last, change);
   }
   __syncthreads();
 #pragma unroll
