@@ -173,6 +173,10 @@ def main():
     ap.add_argument("--fps", type=float, default=30.0, help="serving: frames per second per stream (Poisson)")
     ap.add_argument("--seconds", type=float, default=3.0, help="serving: arrival window")
     ap.add_argument("--max-batch", type=int, default=16, help="serving: pairs per batch")
+    ap.add_argument("--frames", type=int, default=8, help="serving: frames of each size's synthetic sequence")
+    ap.add_argument("--warm", action="store_true",
+                    help="serving: solve each stream's later pairs as sceneFlowWithInit from its last flow")
+    ap.add_argument("--warm-outer", type=int, default=0, help="serving: outer iterations of warm starts (0: 15)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: the scene-flow solve runs on one GPU "
@@ -366,48 +370,55 @@ def mixed_main(R, args):
     print(json.dumps(out), flush=True)
 
 
-def serve_main(R, args):
-    """Serving: streams submit frames at Poisson arrival times; every frame after
-    a stream's first is a pair solved by the server's continuous batching.
-    Reports offered and achieved pairs/s and per-request latency."""
-    import threading
+def serve_frames(R, spec, n_frames):
+    """Per stream size: a synthetic frame sequence (every segment on a smooth
+    constant-velocity SE(3) trajectory), with each frame's boundary maps."""
     import rgbdflow_amd.synth as S
-    spec = []
-    for tok in args.serve.split(","):
-        size, cnt = tok.split(":")
-        w, h = (int(x) for x in size.split("x"))
-        spec += [(w, h)] * int(cnt)
-    sizes = sorted(set(spec), key=lambda t: -t[0])
     frames = {}
-    for (w, h) in sizes:   # two frames per size, alternated by every stream of that size
+    for (w, h) in sorted(set(spec), key=lambda t: -t[0]):
         f = 525.0 * w / 640
-        pr = S.make_pair(w, h, 500 + w, n_segments=6, cam=S.Camera(w, h, f, f, (w - 1) / 2, (h - 1) / 2))
-        _, d1, _, d2 = S.to_solver_inputs(pr)
+        seq = S.make_sequence(w, h, 500 + w, n_frames, n_segments=6, cam=S.Camera(w, h, f, f, (w - 1) / 2, (h - 1) / 2))
         lv = R.pyramid_levels(w, h, 0.5, w // 16)
-        hpb = S.segment_boundary_maps(pr["seg1"], [a for a, b in lv], [b for a, b in lv])
-        frames[(w, h)] = [(pr["img1"], d1, hpb), (pr["img2"], d2, hpb)]
-    p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED)
-    srv = R.Server(p, max_batch=args.max_batch, lanes=2)
-    if not args.no_autotune:
+        frames[(w, h)] = [(fr["img"], (fr["depth"].astype(np.float64) * .001).astype(np.float32),
+                           S.segment_boundary_maps(fr["seg"], [a for a, b in lv], [b for a, b in lv]))
+                          for fr in seq["frames"]]
+    return frames
+
+
+def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_outer, autotune=True, lanes=2, seed=7):
+    """Streams submit frames at Poisson arrival times (rate fps each); stream k
+    walks its size's sequence back and forth from its own starting frame.
+    Returns the served-request latencies and the summary."""
+    import threading
+    sizes = sorted(set(spec), key=lambda t: -t[0])
+    st = {"f32": R.STORE_F32, "f16": R.STORE_F16, "bf16": R.STORE_BF16}[storage]
+    p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED, graph=True,
+                      storage=st)
+    srv = R.Server(p, max_batch=max_batch, lanes=lanes, warm_start=warm, warm_n_outer=warm_outer)
+    if autotune:
         with R.Context(0) as tctx:
             for (w, h) in sizes:
                 for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
-                    srv.set_relax_schedule(lw, lh, tctx.autotune_relax(lw, lh, batch=max(1, args.max_batch // 2),
+                    srv.set_relax_schedule(lw, lh, tctx.autotune_relax(lw, lh, batch=max(1, min(4, max_batch)),
                                                                        params=p))
     sids = [srv.open(w, h, R.make_params(w, h).cam, w // 16) for (w, h) in spec]
-    rng = np.random.default_rng(7)
+    rng = np.random.default_rng(seed)
     events = []
     for k, (w, h) in enumerate(spec):
-        t = rng.exponential(1 / args.fps)
+        t = rng.exponential(1 / fps)
         n = 0
-        while t < args.seconds:
+        while t < seconds:
             events.append((t, k, n))
-            t += rng.exponential(1 / args.fps)
+            t += rng.exponential(1 / fps)
             n += 1
     events.sort()
     tickets = []
-    print(f"bench: serving {len(spec)} streams, {len(events)} frames over {args.seconds} s", file=sys.stderr,
-          flush=True)
+
+    def frame_of(k, n):
+        seq = frames[spec[k]]
+        period = 2 * (len(seq) - 1)
+        i = (n + 2 * k) % period
+        return seq[i if i < len(seq) else period - i]
 
     def submitter():
         t0 = time.perf_counter()
@@ -415,7 +426,7 @@ def serve_main(R, args):
             dt = t - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-            img, dep, hpb = frames[spec[k]][n % 2]
+            img, dep, hpb = frame_of(k, n)
             tickets.append((k, n, srv.submit(sids[k], img, dep, hpb)))
 
     t0 = time.perf_counter()
@@ -431,21 +442,36 @@ def serve_main(R, args):
             bsz.append(info["batch_pairs"])
     wall = time.perf_counter() - t0
     srv.close()
-    pairs = len(lat)
+    return dict(pairs_per_s=round(len(lat) / wall, 3), offered_pairs_per_s=round(len(lat) / seconds, 2),
+                p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
+                queue_p50_ms=round(float(np.percentile(qms, 50)), 2), mean_batch_pairs=round(float(np.mean(bsz)), 2),
+                requests=len(lat), wall_s=round(wall, 2))
+
+
+def serve_main(R, args):
+    """Serving: streams submit frames at Poisson arrival times; every frame after
+    a stream's first is a pair solved by the server's continuous batching.
+    Reports offered and achieved pairs/s and per-request latency."""
+    spec = []
+    for tok in args.serve.split(","):
+        size, cnt = tok.split(":")
+        w, h = (int(x) for x in size.split("x"))
+        spec += [(w, h)] * int(cnt)
+    frames = serve_frames(R, spec, args.frames)
+    print(f"bench: serving {len(spec)} streams over {args.seconds} s", file=sys.stderr, flush=True)
+    r = serve_run(R, spec, frames, args.fps, args.seconds, args.max_batch, args.storage, args.warm,
+                  args.warm_outer, autotune=not args.no_autotune)
     out = {
         "metric": "served scene-flow pairs/s and request latency",
-        "value": round(pairs / wall, 3),
+        "value": r["pairs_per_s"],
         "unit": "pairs/s",
         "n_gpus": 1,
         "higher_is_better": True,
-        "dtype": "f32",
-        "data": "synthetic (seeded generator)",
-        "config": {"streams": args.serve, "fps_per_stream": args.fps, "arrival_seconds": args.seconds,
-                   "offered_pairs_per_s": round(pairs / args.seconds, 2), "max_batch": args.max_batch,
-                   "lanes": 2, "p50_ms": round(float(np.percentile(lat, 50)), 2),
-                   "p99_ms": round(float(np.percentile(lat, 99)), 2),
-                   "queue_p50_ms": round(float(np.percentile(qms, 50)), 2),
-                   "mean_batch_pairs": round(float(np.mean(bsz)), 2), "wall_s": round(wall, 2)},
+        "dtype": "f32" if args.storage == "f32" else f"f32 compute, {args.storage} system storage",
+        "data": "synthetic (seeded frame sequences, smooth per-segment SE(3) motion)",
+        "config": dict(streams=args.serve, fps_per_stream=args.fps, arrival_seconds=args.seconds,
+                       max_batch=args.max_batch, lanes=2, warm_start=args.warm, warm_n_outer=args.warm_outer,
+                       storage=args.storage, **r),
     }
     print(json.dumps(out), flush=True)
 

@@ -42,6 +42,25 @@ extern "C" {
 #define RGBDFLOW_RELAX_PLAIN 0   /* one launch per colour per iteration */
 #define RGBDFLOW_RELAX_TILED 1   /* several iterations per launch in LDS tiles */
 
+/* Relaxation exits (params.inner_exit), tested at iterations 24, 49, ... of
+ * every (outer, inner) as the reference checks (.cu:905-922). */
+#define RGBDFLOW_EXIT_NONE 0      /* run all nSOR iterations */
+#define RGBDFLOW_EXIT_REFERENCE 1 /* max |change| over the iteration < 1e-5 or >= 0.3 (.cu:914-921) */
+#define RGBDFLOW_EXIT_RESIDUAL 2  /* relative residual ||r - M du||_2 / ||r||_2 <= residual_tol, reduced
+                                     in f64 on the device (not in the reference: a precision-study
+                                     stopping test that is the same for every relaxation factor and
+                                     storage) */
+
+/* Order of a pixel's unknowns within a colour pass (params.pixel_update).
+ * JACOBI is the reference's (.cu:431-459): u, v and w are each computed from
+ * the others' old values; with a factor above 1 that iteration diverges on
+ * these systems.  GS updates u, then v from the new u, then w from the new u
+ * and v: with the red-black pixel order this is point SOR, which converges
+ * for any factor in (0, 2) (the systems are symmetric definite) to the same
+ * fixed point. */
+#define RGBDFLOW_PIXEL_JACOBI 0
+#define RGBDFLOW_PIXEL_GS 1
+
 /* Storage of the per-pixel systems the relaxation reads.  F32 is the
  * reference's precision; F16 (with a per-pixel power-of-two scale) and BF16
  * halve the bytes the relaxation loads and are precision variants: the
@@ -74,7 +93,7 @@ typedef struct rgbdflow_params {
   int n_outer, n_inner, n_sor;
   /* extensions */
   int relax_variant;             /* RGBDFLOW_RELAX_* */
-  int inner_exit;                /* 1: relaxation early exits as the reference (.cu:914-922) */
+  int inner_exit;                /* RGBDFLOW_EXIT_*: relaxation exits at every 25th iteration */
   int outer_exit;                /* 1: outer early exit as the reference (.cu:943-951) */
   int tile_iters;                /* iterations per launch of the tiled relaxation (0 = planner) */
   int tile_w, tile_h;            /* interior tile of the tiled relaxation (0 = planner) */
@@ -83,7 +102,10 @@ typedef struct rgbdflow_params {
                                     (captured at the second run with the same batch and params) */
   int storage;                   /* RGBDFLOW_STORE_*: storage of the per-pixel systems */
   float omega;                   /* relaxation factor (<= 0: the reference's 0.7, .cu:461) */
-  int reserved[1];
+  float residual_tol;            /* RGBDFLOW_EXIT_RESIDUAL: stop once the relative residual is <= this
+                                    (<= 0: 1e-4) */
+  int pixel_update;              /* RGBDFLOW_PIXEL_*: order of a pixel's three unknowns */
+  int reserved;
 } rgbdflow_params;
 
 typedef struct rgbdflow_stats {
@@ -95,6 +117,9 @@ typedef struct rgbdflow_stats {
   double device_ms;              /* device time of the call (HIP events) */
   double relax_ms;               /* summed device time of the relaxation launches, if timed */
   int relax_launches;            /* relaxation launches timed */
+  int residual_checks[16];       /* FP64 residual evaluations per level (RGBDFLOW_EXIT_RESIDUAL) */
+  double residual[16];           /* last relative residual per level (-1: none evaluated) */
+  double energy[16];             /* energy r'du - 1/2 du'M du of the last system at that evaluation */
 } rgbdflow_stats;
 
 typedef struct rgbdflow_ctx rgbdflow_ctx;
@@ -182,6 +207,10 @@ typedef struct rgbdflow_pair_input {
   rgbdflow_camera cam; /* camera of this pair (xRes == 0: params->cam) */
   int min_width;       /* pyramid min width (<= 0: params->min_width) */
   int reserved;
+  const double* est;   /* [N][3] flow estimate, or NULL.  Given: the pair is solved as
+                          OpticalFlow::sceneFlowWithInit (sceneFlowWithInit.cpp:7-81, as
+                          rgbdflow_with_init) -- full resolution only, from est, no outer
+                          early exit; hpb then needs only its level-0 map */
 } rgbdflow_pair_input;
 int rgbdflow_batch_load(rgbdflow_ctx* ctx, const rgbdflow_params* params, int n, const rgbdflow_pair_input* pairs);
 int rgbdflow_batch_run(rgbdflow_ctx* ctx, const rgbdflow_params* params, rgbdflow_stats* stats);
@@ -224,9 +253,21 @@ int rgbdflow_level_system(rgbdflow_ctx* ctx, int W, int H, float focal_level, co
                           double gamma, const double* depthMagTermWeights, const double* regularizationWeights,
                           double* coef13);
 /* n_iter red-black iterations (red then black) on a compact system; x: [3][N]
- * planes in/out.  schedule: relax_variant and tile_* of params (NULL: plain). */
+ * planes in/out.  schedule: relax_variant, tile_* and omega of params (NULL:
+ * plain, 0.7). */
 int rgbdflow_relax(rgbdflow_ctx* ctx, const double* coef13, int W, int H, int n_iter,
                    const rgbdflow_params* schedule, double* x);
+/* Relaxes a compact system from x until the relative residual
+ * ||r - M x||_2 / ||r||_2, reduced in f64 on the device at iterations 24, 49,
+ * ... (RGBDFLOW_EXIT_RESIDUAL), is <= tol, or for max_iter iterations.  Out:
+ * x, the iterations run, the last residual evaluated (-1: none) and the
+ * system's energy r'x - 1/2 x'Mx at that evaluation (convex: -M is positive
+ * definite, and the solution minimises it).  No counterpart in the
+ * reference (whose check is the max change, .cu:914-921): the precision
+ * study's measure of convergence, the same for every relaxation factor. */
+int rgbdflow_relax_residual(rgbdflow_ctx* ctx, const double* coef13, int W, int H, int max_iter, double tol,
+                            const rgbdflow_params* schedule, double* x, int* iters, double* residual,
+                            double* energy);
 
 /* ---- autotuning ---- */
 typedef struct rgbdflow_tile_choice {
@@ -263,13 +304,21 @@ int rgbdflow_time_relax_schedule(rgbdflow_ctx* ctx, int W, int H, int batch, con
  * submitted frame after the first forms the pair (last frame, frame), solved
  * by sceneFlowMultiscale exactly as rgbdflow_multiscale would (depth stdevs
  * computed on the device, no hole fill), in whatever batch the worker puts it:
- * the result does not depend on the batch. */
+ * the result does not depend on the batch.  With warm_start, every pair after
+ * a stream's first is instead solved as rgbdflow_with_init would solve it
+ * with the stream's previous result as the estimate. */
 typedef struct rgbdflow_server rgbdflow_server;
 typedef struct rgbdflow_server_config {
   int device;
   int max_batch;                  /* pairs per batch (<= 0: 16) */
   int lanes;                      /* batches in flight at once, each on its own HIP stream (<= 0: 2) */
+  int warm_start;                 /* 1: a stream's pairs after its first are solved as
+                                     sceneFlowWithInit from the stream's last flow (kept on the
+                                     device), as ceLiuSceneFlow does when given an estimate
+                                     (ceLiuOpticalFlow.cpp:200-204) */
   unsigned long long hbm_budget;  /* bytes for stream state and in-flight pairs (0: no limit) */
+  int warm_n_outer;               /* outer iterations of warm-started pairs (<= 0: params' n_outer) */
+  int reserved;
 } rgbdflow_server_config;
 typedef struct rgbdflow_request_info {
   int has_flow;       /* 0 for a stream's first frame */

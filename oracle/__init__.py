@@ -30,12 +30,14 @@ class Params(C.Structure):
                 ("depth_vs_color_data_weight", C.c_double), ("alpha", C.c_double), ("gamma", C.c_double),
                 ("pyramid_ratio", C.c_double), ("min_width", C.c_int), ("n_outer", C.c_int),
                 ("n_inner", C.c_int), ("n_sor", C.c_int), ("cam", Camera), ("precision", C.c_int),
-                ("disable_inner_exit", C.c_int), ("disable_outer_exit", C.c_int), ("omega", C.c_float)]
+                ("disable_inner_exit", C.c_int), ("disable_outer_exit", C.c_int), ("omega", C.c_float),
+                ("residual_exit", C.c_int), ("residual_tol", C.c_double), ("pixel_gs", C.c_int)]
 
 
 class Stats(C.Structure):
     _fields_ = [("n_levels", C.c_int), ("width", C.c_int * 16), ("height", C.c_int * 16),
-                ("outer_exec", C.c_int * 16), ("sweeps_exec", C.c_int * 16), ("last_stat", C.c_double * 16)]
+                ("outer_exec", C.c_int * 16), ("sweeps_exec", C.c_int * 16), ("last_stat", C.c_double * 16),
+                ("residual_checks", C.c_int * 16), ("residual", C.c_double * 16), ("energy", C.c_double * 16)]
 
 
 class LevelIn(C.Structure):
@@ -47,7 +49,8 @@ class LevelIn(C.Structure):
 
 class LevelStats(C.Structure):
     _fields_ = [("outer_exec", C.c_int), ("sweeps_exec", C.c_int), ("last_stat", C.c_double),
-                ("last_maxdiff", C.c_float)]
+                ("last_maxdiff", C.c_float), ("residual_checks", C.c_int), ("last_residual", C.c_double),
+                ("last_energy", C.c_double)]
 
 
 _lib = None
@@ -85,7 +88,8 @@ def _f32(a):
 
 def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
                 ratio=0.5, min_width=None, reg=REG_HPB, precision=PREC_REF, cw=1.0, dvc=1.0,
-                disable_inner_exit=False, disable_outer_exit=False, omega=0.7) -> Params:
+                disable_inner_exit=False, disable_outer_exit=False, omega=0.7, residual_exit=False,
+                residual_tol=1e-4, pixel_gs=False) -> Params:
     p = Params()
     p.regularization_type = reg
     p.color_data_weight = cw
@@ -103,6 +107,9 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.disable_inner_exit = int(disable_inner_exit)
     p.disable_outer_exit = int(disable_outer_exit)
     p.omega = omega
+    p.residual_exit = int(residual_exit)
+    p.residual_tol = residual_tol
+    p.pixel_gs = int(pixel_gs)
     return p
 
 
@@ -297,6 +304,28 @@ def relax(coef13, n_iter, x, precision=PREC_REF):
     xx = _f64(x).copy()
     lib().orc_relax(_p(c), W, H, precision, n_iter, _p(xx))
     return xx
+
+
+def system_residual(coef13, x):
+    """(relative residual ||r - M x|| / ||r||, energy r'x - 1/2 x'Mx) in f64."""
+    c = _f64(coef13)
+    _, H, W = c.shape
+    xx = _f64(x)
+    out = np.zeros(2)
+    lib().orc_system_residual(_p(c), W, H, _p(xx), _p(out))
+    return float(out[0]), float(out[1])
+
+
+def relax_residual(coef13, max_iter, tol, x, omega=0.7, precision=PREC_REF, pixel_gs=False):
+    """Relax until the residual test passes at a check iteration; returns
+    (x, iterations, residual, energy)."""
+    c = _f64(coef13)
+    _, H, W = c.shape
+    xx = _f64(x).copy()
+    it, res, en = C.c_int(), C.c_double(), C.c_double()
+    lib().orc_relax_residual(_p(c), W, H, precision, C.c_float(omega), int(pixel_gs), int(max_iter),
+                             C.c_double(tol), _p(xx), C.byref(it), C.byref(res), C.byref(en))
+    return xx, it.value, res.value, en.value
 
 
 def multiscale(im1, dm1, im2, dm2, sigma1, hpb, params: Params):

@@ -672,7 +672,7 @@ struct Solver {
   /* .cu:382-470: one colour of the red-black relaxation; returns the largest
    * |change| over the updated pixels (the check of .cu:914-918 splits exactly
    * into the two colours' changes). */
-  static CT relax_colour(const CT* coef, int W, int H, int colour, CT* x, float omega_f = .7f) {
+  static CT relax_colour(const CT* coef, int W, int H, int colour, CT* x, float omega_f = .7f, bool gs = false) {
     const size_t N = (size_t)W * H;
     const CT* eRxy = coef + 0 * N; const CT* eRz = coef + 1 * N;
     const CT* eDxy = coef + 2 * N; const CT* eDz = coef + 3 * N;
@@ -698,9 +698,12 @@ struct Solver {
           if (j < W - 1) nv -= (k < 2 ? eRxy[l] : eRz[l]) * xk[l + 1];
           if (i < H - 1) nv -= (k < 2 ? eDxy[l] : eDz[l]) * xk[l + W];
           nv += rhs[k][l];
+          /* the pixel's other unknowns: their old values (.cu:431-459), or
+           * with gs (point SOR order) the values this pass already gave them */
+          const CT x0 = (gs && k > 0) ? xn[0] : xo[0], x1 = (gs && k > 1) ? xn[1] : xo[1];
           if (k == 0) { nv -= Buv[l] * xo[1]; nv -= Buw[l] * xo[2]; nv /= Buu[l]; }
-          else if (k == 1) { nv -= Buv[l] * xo[0]; nv -= Bvw[l] * xo[2]; nv /= Bvv[l]; }
-          else { nv -= Buw[l] * xo[0]; nv -= Bvw[l] * xo[1]; nv /= Bww[l]; }
+          else if (k == 1) { nv -= Buv[l] * x0; nv -= Bvw[l] * xo[2]; nv /= Bvv[l]; }
+          else { nv -= Buw[l] * x0; nv -= Bvw[l] * x1; nv /= Bww[l]; }
           CT r = omega * nv + (1 - omega) * xo[k];
           if (r < xo[k] - step) r = xo[k] - step;
           else if (r > xo[k] + step) r = xo[k] + step;
@@ -718,6 +721,52 @@ struct Solver {
     for (int k = 0; k < pool().threads(); k++)
       if (part_max[k] > maxd) maxd = part_max[k];
     return maxd;
+  }
+
+  /* The f64 residual of the system at x (orc_system_residual): per pixel and
+   * unknown, res = r - (sum_n e x_n + sum_k' B x_k'), terms summed in f64 in
+   * the order written; sums of res^2, r^2 and x (r + res) per row, then the
+   * rows in order (the same value for any thread count). */
+  static void residual(const CT* coef, int W, int H, const CT* x, double* rel, double* energy) {
+    const size_t N = (size_t)W * H;
+    const CT* eRxy = coef + 0 * N; const CT* eRz = coef + 1 * N;
+    const CT* eDxy = coef + 2 * N; const CT* eDz = coef + 3 * N;
+    const CT* Buu = coef + 4 * N; const CT* Bvv = coef + 5 * N; const CT* Bww = coef + 6 * N;
+    const CT* Buv = coef + 7 * N; const CT* Buw = coef + 8 * N; const CT* Bvw = coef + 9 * N;
+    vector<double> row(3 * (size_t)H);
+    pool().rows(H, [&](int i0, int i1, int) {
+      for (int i = i0; i < i1; i++) {
+        double rr = 0, bb = 0, ex = 0;
+        for (int j = 0; j < W; j++) {
+          const size_t l = (size_t)i * W + j;
+          const double xo[3] = {(double)x[l], (double)x[N + l], (double)x[2 * N + l]};
+          const double B[3][3] = {{(double)Buu[l], (double)Buv[l], (double)Buw[l]},
+                                  {(double)Buv[l], (double)Bvv[l], (double)Bvw[l]},
+                                  {(double)Buw[l], (double)Bvw[l], (double)Bww[l]}};
+          for (int k = 0; k < 3; k++) {
+            const CT* xk = x + k * N;
+            double mx = 0;
+            if (j > 0) mx += (double)(k < 2 ? eRxy[l - 1] : eRz[l - 1]) * (double)xk[l - 1];
+            if (i > 0) mx += (double)(k < 2 ? eDxy[l - W] : eDz[l - W]) * (double)xk[l - W];
+            if (j < W - 1) mx += (double)(k < 2 ? eRxy[l] : eRz[l]) * (double)xk[l + 1];
+            if (i < H - 1) mx += (double)(k < 2 ? eDxy[l] : eDz[l]) * (double)xk[l + W];
+            mx += B[k][0] * xo[0];
+            mx += B[k][1] * xo[1];
+            mx += B[k][2] * xo[2];
+            const double rk = (double)coef[(10 + k) * N + l];
+            const double res = rk - mx;
+            rr += res * res;
+            bb += rk * rk;
+            ex += xo[k] * (rk + res);
+          }
+        }
+        row[3 * i] = rr; row[3 * i + 1] = bb; row[3 * i + 2] = ex;
+      }
+    });
+    double rr = 0, bb = 0, ex = 0;
+    for (int i = 0; i < H; i++) { rr += row[3 * i]; bb += row[3 * i + 1]; ex += row[3 * i + 2]; }
+    *rel = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    *energy = 0.5 * ex;
   }
 
   /* .cu:579-635 GPU warps: floor base, explicit clamps, f32 weights (f64 in the FP64 oracle) */
@@ -753,8 +802,8 @@ struct Solver {
   void run(int allow_early, orc_level_stats* st) {
     const double convThreshold = 4e-4;
     const int checkEvery = 25;
-    int outer = 0, sweeps = 0;
-    double lastStat = -1;
+    int outer = 0, sweeps = 0, rchecks = 0;
+    double lastStat = -1, lastRes = -1, lastEnergy = 0;
     float lastMax = -1;
     for (int count = 0; count < p->n_outer; count++) {
       outer++;
@@ -766,13 +815,18 @@ struct Solver {
         assemble();
         for (int it = 0; it < p->n_sor; it++) {
           const float om = p->omega > 0 ? p->omega : .7f;
-          const CT m0 = relax_colour(coef.data(), W, H, 0, du.data(), om);
-          const CT m1 = relax_colour(coef.data(), W, H, 1, du.data(), om);
+          const CT m0 = relax_colour(coef.data(), W, H, 0, du.data(), om, p->pixel_gs != 0);
+          const CT m1 = relax_colour(coef.data(), W, H, 1, du.data(), om, p->pixel_gs != 0);
           sweeps++;
           if (it % checkEvery == checkEvery - 1) {
             const CT md = std::max(m0, m1);
             lastMax = (float)md;
-            if (!p->disable_inner_exit) {
+            if (p->residual_exit) {
+              /* not in the reference: the FP64 residual test (rgbdflow.h RGBDFLOW_EXIT_RESIDUAL) */
+              residual(coef.data(), W, H, du.data(), &lastRes, &lastEnergy);
+              rchecks++;
+              if (lastRes <= p->residual_tol) break;
+            } else if (!p->disable_inner_exit) {
               if (md < 1e-5) break;
               if (md >= (CT).3) break;
             }
@@ -794,7 +848,10 @@ struct Solver {
         if (!p->disable_outer_exit && stat < convThreshold) break;
       }
     }
-    if (st) { st->outer_exec = outer; st->sweeps_exec = sweeps; st->last_stat = lastStat; st->last_maxdiff = lastMax; }
+    if (st) {
+      st->outer_exec = outer; st->sweeps_exec = sweeps; st->last_stat = lastStat; st->last_maxdiff = lastMax;
+      st->residual_checks = rchecks; st->last_residual = lastRes; st->last_energy = lastEnergy;
+    }
   }
 };
 
@@ -864,6 +921,28 @@ int level_solve_dispatch(const orc_level_in* in, const orc_params* p, int allow_
 }  // namespace
 
 /* ========================================================================== */
+template <class CT, bool REF>
+static void relax_residual_t(const double* coef13, int W, int H, float omega, bool gs, int max_iter, double tol,
+                             double* x, int* iters, double* residual, double* energy) {
+  const size_t N = (size_t)W * H;
+  vector<CT> c(coef13, coef13 + 13 * N), xx(x, x + 3 * N);
+  int it = 0;
+  double res = -1, en = 0;
+  while (it < max_iter) {
+    Solver<CT, REF>::relax_colour(c.data(), W, H, 0, xx.data(), omega, gs);
+    Solver<CT, REF>::relax_colour(c.data(), W, H, 1, xx.data(), omega, gs);
+    it++;
+    if ((it - 1) % 25 == 24) {
+      Solver<CT, REF>::residual(c.data(), W, H, xx.data(), &res, &en);
+      if (res <= tol) break;
+    }
+  }
+  for (size_t k = 0; k < 3 * N; k++) x[k] = xx[k];
+  *iters = it;
+  *residual = res;
+  *energy = en;
+}
+
 extern "C" {
 
 int orc_pyramid_levels(int W, int H, double ratio, int min_width, int* widths, int* heights) {
@@ -1001,6 +1080,19 @@ void orc_relax(const double* coef13, int W, int H, int precision, int n_iter, do
   }
 }
 
+void orc_system_residual(const double* coef13, int W, int H, const double* x, double* out) {
+  Solver<double, false>::residual(coef13, W, H, x, &out[0], &out[1]);
+}
+
+void orc_relax_residual(const double* coef13, int W, int H, int precision, float omega, int pixel_gs, int max_iter,
+                        double tol, double* x, int* iters, double* residual, double* energy) {
+  if (omega <= 0) omega = .7f;
+  if (precision == ORC_PREC_REF)
+    relax_residual_t<float, true>(coef13, W, H, omega, pixel_gs != 0, max_iter, tol, x, iters, residual, energy);
+  else
+    relax_residual_t<double, false>(coef13, W, H, omega, pixel_gs != 0, max_iter, tol, x, iters, residual, energy);
+}
+
 /* sceneFlowMultiscale.cpp:30-201 */
 int orc_multiscale(const double* im1, const double* dm1, const double* im2, const double* dm2, const float* sigma1,
                    const float* hpb, int W, int H, const orc_params* p, double* uo, double* vo, double* wo,
@@ -1073,6 +1165,8 @@ int orc_multiscale(const double* im1, const double* dm1, const double* im2, cons
     if (st) {
       st->width[k] = Wk; st->height[k] = Hk;
       st->outer_exec[k] = ls.outer_exec; st->sweeps_exec[k] = ls.sweeps_exec; st->last_stat[k] = ls.last_stat;
+      st->residual_checks[k] = ls.residual_checks; st->residual[k] = ls.last_residual;
+      st->energy[k] = ls.last_energy;
     }
   }
   std::memcpy(uo, u.data(), N * sizeof(double));
@@ -1113,6 +1207,8 @@ int orc_with_init(const double* im1, const double* dm1, const double* im2, const
     std::memset(st, 0, sizeof(*st));
     st->n_levels = 1; st->width[0] = W; st->height[0] = H;
     st->outer_exec[0] = ls.outer_exec; st->sweeps_exec[0] = ls.sweeps_exec; st->last_stat[0] = ls.last_stat;
+    st->residual_checks[0] = ls.residual_checks; st->residual[0] = ls.last_residual;
+    st->energy[0] = ls.last_energy;
   }
   std::memcpy(uo, u.data(), N * sizeof(double));
   std::memcpy(vo, v.data(), N * sizeof(double));

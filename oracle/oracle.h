@@ -39,6 +39,12 @@ typedef struct {
   int disable_inner_exit; /* 1: never leave the relaxation loop early (parity runs) */
   int disable_outer_exit; /* 1: never leave the outer loop early */
   float omega;            /* relaxation factor (.cu:461: 0.7; <= 0 means 0.7) */
+  int residual_exit;      /* 1: the relaxation's checks (every 25th iteration) stop on the f64
+                             relative residual <= residual_tol instead of the max change */
+  double residual_tol;
+  int pixel_gs;           /* 0: a pixel's unknowns are updated from each other's old values
+                             (.cu:431-459); 1: u, then v from the new u, then w from both
+                             (point SOR order; convergent for any factor in (0, 2)) */
 } orc_params;
 
 typedef struct {
@@ -47,6 +53,9 @@ typedef struct {
   int outer_exec[16];      /* outer iterations executed per level */
   int sweeps_exec[16];     /* red-black iterations executed per level (summed over outers) */
   double last_stat[16];    /* last outer convergence statistic per level */
+  int residual_checks[16]; /* residual evaluations per level (residual_exit) */
+  double residual[16];     /* last relative residual per level (-1: none) */
+  double energy[16];       /* energy r'x - 1/2 x'Mx of the last system at that evaluation */
 } orc_stats;
 
 /* ---- pyramids (src/GaussianPyramid.cpp:20-63) ---- */
@@ -86,6 +95,8 @@ typedef struct {
   int outer_exec, sweeps_exec;
   double last_stat;
   float last_maxdiff;
+  int residual_checks;
+  double last_residual, last_energy;
 } orc_level_stats;
 
 int orc_level_solve(const orc_level_in* in, const orc_params* p, int allow_early,
@@ -107,6 +118,17 @@ void orc_level_weights(const orc_level_in* in, const orc_params* p, const double
                        const double* w, double* phi, double* psi, double* pi);
 /* nIter red-black iterations on a compact system; x = [3][N] planes (in/out). */
 void orc_relax(const double* coef13, int W, int H, int precision, int n_iter, double* x);
+/* The residual of a compact system at x in f64: out[0] = ||r - M x||_2 / ||r||_2
+ * (||r - M x||_2 when r = 0), out[1] = r'x - 1/2 x'Mx, where M is the operator
+ * the relaxation inverts (.cu:404-459): (M x)_k = sum_n e_nk x_nk + sum_k' B_kk' x_k'. */
+void orc_system_residual(const double* coef13, int W, int H, const double* x, double* out);
+/* Relaxation (factor omega, pixel order as orc_params.pixel_gs, precision as
+ * orc_relax) from x with the residual
+ * test at iterations 24, 49, ...: stops once out[0] of orc_system_residual is
+ * <= tol, or after max_iter iterations.  Out: x, iterations run, last
+ * residual (-1: none) and energy. */
+void orc_relax_residual(const double* coef13, int W, int H, int precision, float omega, int pixel_gs, int max_iter,
+                        double tol, double* x, int* iters, double* residual, double* energy);
 
 /* ---- whole pair: OpticalFlow::sceneFlowMultiscale (src/sceneFlowMultiscale.cpp:30-201) ----
  * im*: [N][3] BGR in [0,1]; dm*: [N] metres; sigma1: [N] depth stdevs of frame 1;

@@ -21,6 +21,8 @@ REG_ISOTROPIC, REG_LETOUZEY, REG_HPB = 0, 1, 2
 VARIANT_FULL = 2
 RELAX_PLAIN, RELAX_TILED = 0, 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+EXIT_NONE, EXIT_REFERENCE, EXIT_RESIDUAL = 0, 1, 2
+PIXEL_JACOBI, PIXEL_GS = 0, 1
 
 # Every symbol include/rgbdflow/rgbdflow.h declares.
 EXPORTS = [
@@ -29,7 +31,7 @@ EXPORTS = [
     "rgbdflow_solve_level", "rgbdflow_depth_stdevs_pointwise", "rgbdflow_pair_load", "rgbdflow_pair_run",
     "rgbdflow_pair_read", "rgbdflow_multiscale", "rgbdflow_with_init", "rgbdflow_gaussian_pyramid",
     "rgbdflow_depth_map_pyramid", "rgbdflow_im2feature", "rgbdflow_warp_by_3d_flow", "rgbdflow_upsample_flow",
-    "rgbdflow_level_system", "rgbdflow_relax", "rgbdflow_autotune_relax", "rgbdflow_batch_load",
+    "rgbdflow_level_system", "rgbdflow_relax", "rgbdflow_relax_residual", "rgbdflow_autotune_relax", "rgbdflow_batch_load",
     "rgbdflow_batch_run", "rgbdflow_batch_read", "rgbdflow_set_relax_schedule",
     "rgbdflow_time_relax_schedule", "rgbdflow_server_create", "rgbdflow_server_destroy",
     "rgbdflow_server_last_error", "rgbdflow_stream_open", "rgbdflow_stream_submit", "rgbdflow_stream_poll",
@@ -50,27 +52,31 @@ class Params(C.Structure):
                 ("min_width", C.c_int), ("n_outer", C.c_int), ("n_inner", C.c_int), ("n_sor", C.c_int),
                 ("relax_variant", C.c_int), ("inner_exit", C.c_int), ("outer_exit", C.c_int),
                 ("tile_iters", C.c_int), ("tile_w", C.c_int), ("tile_h", C.c_int), ("tile_config", C.c_int),
-                ("graph", C.c_int), ("storage", C.c_int), ("omega", C.c_float), ("reserved", C.c_int * 1)]
+                ("graph", C.c_int), ("storage", C.c_int), ("omega", C.c_float), ("residual_tol", C.c_float),
+                ("pixel_update", C.c_int), ("reserved", C.c_int)]
 
 
 class Stats(C.Structure):
     _fields_ = [("n_levels", C.c_int), ("width", C.c_int * 16), ("height", C.c_int * 16),
                 ("outer_exec", C.c_int * 16), ("sweeps_exec", C.c_int * 16), ("last_stat", C.c_double * 16),
-                ("device_ms", C.c_double), ("relax_ms", C.c_double), ("relax_launches", C.c_int)]
+                ("device_ms", C.c_double), ("relax_ms", C.c_double), ("relax_launches", C.c_int),
+                ("residual_checks", C.c_int * 16), ("residual", C.c_double * 16), ("energy", C.c_double * 16)]
 
     def levels(self):
         return [dict(W=self.width[k], H=self.height[k], outer=self.outer_exec[k], sweeps=self.sweeps_exec[k],
-                     stat=self.last_stat[k]) for k in range(self.n_levels)]
+                     stat=self.last_stat[k], residual_checks=self.residual_checks[k], residual=self.residual[k],
+                     energy=self.energy[k]) for k in range(self.n_levels)]
 
 
 class PairInput(C.Structure):
     _fields_ = [("W", C.c_int), ("H", C.c_int), ("im1", C.c_void_p), ("dm1", C.c_void_p), ("im2", C.c_void_p),
                 ("dm2", C.c_void_p), ("sigma1", C.c_void_p), ("hpb", C.c_void_p), ("cam", Camera),
-                ("min_width", C.c_int), ("reserved", C.c_int)]
+                ("min_width", C.c_int), ("reserved", C.c_int), ("est", C.c_void_p)]
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("device", C.c_int), ("max_batch", C.c_int), ("lanes", C.c_int), ("hbm_budget", C.c_ulonglong)]
+    _fields_ = [("device", C.c_int), ("max_batch", C.c_int), ("lanes", C.c_int), ("warm_start", C.c_int),
+                ("hbm_budget", C.c_ulonglong), ("warm_n_outer", C.c_int), ("reserved", C.c_int)]
 
 
 class RequestInfo(C.Structure):
@@ -127,7 +133,9 @@ class RgbdFlowError(RuntimeError):
 def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
                 ratio=0.5, min_width=None, reg=REG_HPB, cw=1.0, dvc=1.0, inner_exit=True, outer_exit=True,
                 relax_variant=RELAX_PLAIN, tile_iters=0, tile_w=0, tile_h=0, tile_config=-1, graph=False,
-                storage=STORE_F32, omega=0.7) -> Params:
+                storage=STORE_F32, omega=0.7, residual_tol=1e-4, pixel_update=PIXEL_JACOBI) -> Params:
+    """inner_exit: True / EXIT_REFERENCE (the reference's max-change test),
+    False / EXIT_NONE, or EXIT_RESIDUAL (FP64 relative residual <= residual_tol)."""
     p = Params()
     lib().rgbdflow_default_params(C.byref(p))
     p.regularization_type = reg
@@ -150,6 +158,8 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.graph = int(graph)
     p.storage = storage
     p.omega = omega
+    p.residual_tol = residual_tol
+    p.pixel_update = pixel_update
     return p
 
 
@@ -316,6 +326,19 @@ class Context:
                                          _p(xx)))
         return xx
 
+    def relax_residual(self, coef13, max_iter, tol, x, schedule: Params | None = None):
+        """Relax from x until the FP64 relative residual (checked every 25
+        iterations) is <= tol or max_iter iterations ran; returns
+        (x, iterations, residual, energy)."""
+        c = _f64(coef13)
+        _, H, W = c.shape
+        xx = _f64(x).copy()
+        it, res, en = C.c_int(), C.c_double(), C.c_double()
+        self._check(lib().rgbdflow_relax_residual(self.h, _p(c), W, H, int(max_iter), C.c_double(tol),
+                                                  C.byref(schedule) if schedule else None, _p(xx), C.byref(it),
+                                                  C.byref(res), C.byref(en)))
+        return xx, it.value, res.value, en.value
+
     def solve_level(self, li, params, allow_early, u, v, w):
         u, v, w = _f64(u).copy(), _f64(v).copy(), _f64(w).copy()
         st = Stats()
@@ -354,18 +377,20 @@ class Context:
     # ---------------------------------------------------------------- batches
     def batch_load(self, params, pairs):
         """pairs: list of dicts with im1, dm1, im2, dm2 and optionally sigma1,
-        hpb, cam (a Camera) and min_width."""
+        hpb, cam (a Camera), min_width and est (an (H, W, 3) flow estimate: the
+        pair is then solved as sceneFlowWithInit)."""
         keep, arr = [], (PairInput * len(pairs))()
         for i, pr in enumerate(pairs):
             im1, dm1, im2, dm2 = _f64(pr["im1"]), _f64(pr["dm1"]), _f64(pr["im2"]), _f64(pr["dm2"])
             s1 = _f32(pr["sigma1"]) if pr.get("sigma1") is not None else None
             h = _f32(pr["hpb"]) if pr.get("hpb") is not None else None
-            keep.append((im1, dm1, im2, dm2, s1, h))
+            est = _f64(pr["est"]) if pr.get("est") is not None else None
+            keep.append((im1, dm1, im2, dm2, s1, h, est))
             H, W = dm1.shape
             a = arr[i]
             a.W, a.H = W, H
             a.im1, a.dm1, a.im2, a.dm2 = _p(im1), _p(dm1), _p(im2), _p(dm2)
-            a.sigma1, a.hpb = _p(s1), _p(h)
+            a.sigma1, a.hpb, a.est = _p(s1), _p(h), _p(est)
             if pr.get("cam") is not None:
                 a.cam = pr["cam"]
             a.min_width = int(pr.get("min_width", 0))
@@ -409,8 +434,11 @@ class Context:
 class Server:
     """A serving runtime (rgbdflow_server_*): streams of frames, continuous batching."""
 
-    def __init__(self, params, device=0, max_batch=16, lanes=2, hbm_budget=0):
-        cfg = ServerConfig(device, max_batch, lanes, hbm_budget)
+    def __init__(self, params, device=0, max_batch=16, lanes=2, hbm_budget=0, warm_start=False, warm_n_outer=0):
+        """warm_start: every pair after a stream's first is solved as
+        sceneFlowWithInit from the stream's previous result (warm_n_outer > 0:
+        with that many outer iterations)."""
+        cfg = ServerConfig(device, max_batch, lanes, int(warm_start), hbm_budget, warm_n_outer, 0)
         h = C.c_void_p()
         rc = lib().rgbdflow_server_create(C.byref(h), C.byref(cfg), C.byref(params))
         if rc != OK:

@@ -52,6 +52,7 @@ struct PairSlot {
   int W = 0, H = 0, min_width = 0;
   bool cam_given = false, sigma_given = false, hpb_given = false;
   bool pyr1_given = false;  // frame 1's colour and depth pyramids were loaded (not built)
+  bool init = false;        // solved as sceneFlowWithInit: finest level only, from the estimate in wk.u[1]
   rgbdflow_camera cam{};
   std::vector<LevelShape> L;
   std::vector<size_t> off;   // level k >= 1: element offset (per channel) in the pyramid arrays
@@ -71,7 +72,7 @@ struct StepPlan {
   bool tiled_ok = false;
   int cfg = 0, slots = 0, T = 0;  // tiled launch shape: workgroup shape, LDS slots, iterations
   int storage = 0;                 // system record storage of the step's items
-  bool omega_fixed = true;         // every item relaxes with the factor 0.7
+  int relax_mode = kRelaxFixed;    // RelaxMode of the step's items
   bool any_early = false;
 };
 
@@ -123,7 +124,8 @@ struct rgbdflow_ctx {
   double load_ratio = 0;
   int load_min_width = 0;
   std::vector<PairSlot> pairs;
-  unsigned long long gen = 0;  // bumped by every load (buffers may move)
+  unsigned long long gen = 0;  // bumped by every load that may move buffers
+  std::string dev_sig;         // shapes of the loaded device batch (batch_load_device), else empty
   BatchPlan plan;
   std::map<std::string, TileSched> plans;  // tile schedules by level shape and request
   unsigned long long plans_gen = 0;
@@ -289,7 +291,8 @@ int get_work(rgbdflow_ctx* c, const std::string& pre, size_t N, Work& w) {
   TRY(get(c, pre + "Dy", N, &w.Dy));
   TRY(get(c, pre + "Dt", N, &w.Dt));
   TRY(get(c, pre + "phi", N, &w.phi));
-  TRY(get(c, pre + "partials", (size_t)pix_blocks((int)N) + 16, &w.partials));
+  // per-block partial sums: 1 per block for the outer statistic, 3 for the residual
+  TRY(get(c, pre + "partials", 3 * (size_t)pix_blocks((int)N) + 16, &w.partials));
   TRY(get(c, pre + "du", N, &w.du));
   TRY(get(c, pre + "du_alt", N, &w.du_alt));
   TRY(get(c, pre + "dW", N, &w.dW));
@@ -483,8 +486,12 @@ size_t pack(std::vector<char>& host, const void* p, size_t bytes) {
 void pack_step(std::vector<char>& host, const std::vector<LevelView>& items, StepPlan& st) {
   st.n = (int)items.size();
   st.storage = items.empty() ? 0 : items[0].storage;
-  st.omega_fixed = true;
-  for (const LevelView& v : items) st.omega_fixed = st.omega_fixed && v.omega == .7f;
+  bool fixed = true, gs = !items.empty();
+  for (const LevelView& v : items) {
+    fixed = fixed && v.omega == .7f && !v.pixel_gs;
+    gs = gs && v.pixel_gs;
+  }
+  st.relax_mode = fixed ? kRelaxFixed : gs ? kRelaxGS : kRelaxFree;
   std::vector<int> pix(st.n + 1), plain(st.n + 1), tiled(st.n + 1), otile(st.n + 1);
   pix[0] = plain[0] = tiled[0] = otile[0] = 0;
   for (int i = 0; i < st.n; i++) {
@@ -553,9 +560,14 @@ int one_step(rgbdflow_ctx* c, const char* name, std::vector<LevelView>& items, c
 
 // nSOR red-black iterations of one (outer, inner), with the convergence
 // checks of .cu:905-922 at iterations 24, 49, ...; the tiled schedule splits
-// the iterations into launches that end at every check iteration.
+// the iterations into launches that end at every check iteration.  With
+// RGBDFLOW_EXIT_RESIDUAL the check is the FP64 relative residual, evaluated
+// after the launch that ends at the check iteration (the relaxation launches
+// then record no change maxima).
 void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, int slot_base) {
   hipStream_t s = c->stream;
+  const bool resid = k.inner_exit == RGBDFLOW_EXIT_RESIDUAL;
+  auto check_of = [&](int last) { return (last % kCheckEvery == kCheckEvery - 1) ? last / kCheckEvery : -1; };
   if (st.tiled_ok) {
     int it = 0, flip = 0;
     while (it < k.n_sor) {
@@ -563,11 +575,14 @@ void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, i
       const int T = std::min(std::min(st.T, boundary - it), k.n_sor - it);
       const int last = it + T - 1;
       const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
-      const int slot = (last % kCheckEvery == kCheckEvery - 1) ? slot_base + last / kCheckEvery : -1;
+      const int chk = check_of(last);
+      const int slot = chk >= 0 ? slot_base + chk : -1;
       relax_event(c);
-      launch_relax_tiled(s, st.vtiled, st.tiled, st.cfg, st.storage, st.omega_fixed, st.slots, flip, T, slot_base,
-                         n_prev, slot);
+      launch_relax_tiled(s, st.vtiled, st.tiled, st.cfg, st.storage, st.relax_mode, st.slots, flip, T, slot_base,
+                         n_prev, resid ? -1 : slot);
       relax_event(c);
+      // the launch read du (flip 0) or du_alt (flip 1) and wrote the other
+      if (resid && chk >= 0) launch_residual(s, st.vpix, st.pix, flip ? 0 : 1, slot_base, chk, slot, k.residual_tol);
       flip ^= 1;
       it += T;
     }
@@ -576,13 +591,15 @@ void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, i
   }
   for (int it = 0; it < k.n_sor; it++) {
     const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
-    const int slot = (it % kCheckEvery == kCheckEvery - 1) ? slot_base + it / kCheckEvery : -1;
+    const int chk = check_of(it);
+    const int slot = chk >= 0 ? slot_base + chk : -1;
     relax_event(c);
-    launch_relax_plain(s, st.vplain, st.plain, 0, slot_base, n_prev, slot);
+    launch_relax_plain(s, st.vplain, st.plain, 0, slot_base, n_prev, resid ? -1 : slot);
     relax_event(c);
     relax_event(c);
-    launch_relax_plain(s, st.vplain, st.plain, 1, slot_base, n_prev, slot);
+    launch_relax_plain(s, st.vplain, st.plain, 1, slot_base, n_prev, resid ? -1 : slot);
     relax_event(c);
+    if (resid && chk >= 0) launch_residual(s, st.vpix, st.pix, 0, slot_base, chk, slot, k.residual_tol);
   }
 }
 
@@ -614,6 +631,12 @@ void ctl_to_stats(const SolveCtl& ctl, int n_outer, bool allow_early, int& outer
   last_stat = (allow_early && outer_exec > 0) ? ctl.stat[outer_exec - 1] : -1.0;
 }
 
+void ctl_residual_stats(const SolveCtl& ctl, rgbdflow_stats* st, int k) {
+  st->residual_checks[k] = ctl.resid_checks;
+  st->residual[k] = ctl.resid_checks ? ctl.resid : -1.0;
+  st->energy[k] = ctl.resid_checks ? ctl.energy : 0.0;
+}
+
 SolveScalars scalars_from(const rgbdflow_params* p, double cw, double alpha, double gamma, int nOuter, int nInner,
                           int nSOR) {
   SolveScalars k;
@@ -625,6 +648,7 @@ SolveScalars scalars_from(const rgbdflow_params* p, double cw, double alpha, dou
   k.n_sor = nSOR;
   k.inner_exit = p ? p->inner_exit : 1;
   k.outer_exit = p ? p->outer_exit : 1;
+  k.residual_tol = (p && p->residual_tol > 0) ? p->residual_tol : 1e-4;
   return k;
 }
 
@@ -703,7 +727,7 @@ int stage_level_inputs(rgbdflow_ctx* c, int W, int H, float focal_level, const d
 // ------------------------------------------------------------------ batches
 // Shape and buffers of pair slot idx (W x H, camera, pyramid min width).
 int alloc_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, int W, int H, const rgbdflow_camera& cam,
-               int min_width, bool hpb, PairSlot& ps) {
+               int min_width, bool hpb, bool init, PairSlot& ps) {
   if (bad_dims(W, H)) return fail(c, RGBDFLOW_ERR_ARG, "bad size of pair " + std::to_string(idx));
   ps.W = W;
   ps.H = H;
@@ -713,6 +737,9 @@ int alloc_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, int W, int H,
   ps.hpb_given = hpb;
   ps.pyr1_given = false;
   ps.L = pyramid_shape(W, H, p->pyramid_ratio, ps.min_width, nullptr, nullptr);
+  // sceneFlowWithInit.cpp:7-81 refines at full resolution only
+  ps.init = init;
+  if (init) ps.L.resize(1);
   const int nL = (int)ps.L.size();
   if (nL > kMaxLevels) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "too many pyramid levels");
   ps.off.assign(nL, 0);
@@ -742,9 +769,10 @@ int alloc_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, int W, int H,
 int load_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const rgbdflow_pair_input& in, PairSlot& ps) {
   if (!in.im1 || !in.dm1 || !in.im2 || !in.dm2)
     return fail(c, RGBDFLOW_ERR_ARG, "bad arguments of pair " + std::to_string(idx));
-  TRY(alloc_pair(c, p, idx, in.W, in.H, in.cam, in.min_width, in.hpb != nullptr, ps));
+  TRY(alloc_pair(c, p, idx, in.W, in.H, in.cam, in.min_width, in.hpb != nullptr, in.est != nullptr, ps));
   ps.sigma_given = in.sigma1 != nullptr;
   const size_t N = (size_t)in.W * in.H;
+  if (in.est) TRY(upload_planes(c, in.est, N, 3, ps.wk.u[1]));
   TRY(upload_planes(c, in.im1, N, 3, ps.im1));
   TRY(upload_planes(c, in.im2, N, 3, ps.im2));
   TRY(upload(c, ps.dm1, in.dm1, N * 8));
@@ -863,14 +891,20 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
       v.delk = level_of(ps, ps.del0, ps.pdel, 1, k);
       v.smk = level_of(ps, ps.sm0, ps.psm, 1, k);
       v.hpbk = ps.hpb ? ps.hpb + ps.hoff[k] : nullptr;
+      v.uv_scale = 1 / p->pyramid_ratio;
       if (k < nL - 1) {
         v.u_prev = ps.wk.u[1 - slot];
         v.Wp = ps.L[k + 1].W;
         v.Hp = ps.L[k + 1].H;
+      } else if (ps.init) {
+        // the estimate enters the level as a same-size flow (sceneFlowWithInit.cpp:13-21)
+        v.u_prev = ps.wk.u[1];
+        v.Wp = Wk;
+        v.Hp = Hk;
+        v.uv_scale = 1;
       }
-      v.uv_scale = 1 / p->pyramid_ratio;
       v.storage = p->storage;
-      set_omega(v, p->omega);
+      set_omega(v, p->omega, p->pixel_update);
       v.reg_type = p->regularization_type;
       v.W_full = (int)cam.xRes;
       v.f_full = cam.focalLength;
@@ -903,6 +937,10 @@ int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
   if (p->regularization_type < 0 || p->regularization_type > 2) return fail(c, RGBDFLOW_ERR_ARG, "bad regularization");
   if (p->storage < RGBDFLOW_STORE_F32 || p->storage > RGBDFLOW_STORE_BF16)
     return fail(c, RGBDFLOW_ERR_ARG, "bad system storage");
+  if (p->inner_exit < RGBDFLOW_EXIT_NONE || p->inner_exit > RGBDFLOW_EXIT_RESIDUAL)
+    return fail(c, RGBDFLOW_ERR_ARG, "bad inner_exit");
+  if (p->pixel_update != RGBDFLOW_PIXEL_JACOBI && p->pixel_update != RGBDFLOW_PIXEL_GS)
+    return fail(c, RGBDFLOW_ERR_ARG, "bad pixel_update");
   return validate_iters(c, p->n_outer, p->n_inner, p->n_sor);
 }
 
@@ -910,7 +948,7 @@ int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
 int load_pair_device(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const PairDev& in, PairSlot& ps) {
   if (!in.f1.im || !in.f1.dm || !in.f2.im || !in.f2.dm)
     return fail(c, RGBDFLOW_ERR_ARG, "bad device frames of pair " + std::to_string(idx));
-  TRY(alloc_pair(c, p, idx, in.W, in.H, in.cam, in.min_width, in.f1.hpb != nullptr, ps));
+  TRY(alloc_pair(c, p, idx, in.W, in.H, in.cam, in.min_width, in.f1.hpb != nullptr, in.init != nullptr, ps));
   ps.sigma_given = false;
   const size_t N = (size_t)in.W * in.H, rest = pyramid_elems(ps.L, 1);
   hipStream_t s = c->stream;
@@ -920,6 +958,7 @@ int load_pair_device(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const P
   HIP_TRY(c, hipMemcpyAsync(ps.dm2, in.f2.dm, N * 8, hipMemcpyDeviceToDevice, s));
   if (in.f1.hpb)
     HIP_TRY(c, hipMemcpyAsync(ps.hpb, in.f1.hpb, pyramid_elems(ps.L, 0) * 2 * 4, hipMemcpyDeviceToDevice, s));
+  if (in.init) HIP_TRY(c, hipMemcpyAsync(ps.wk.u[1], in.init, 3 * N * 8, hipMemcpyDeviceToDevice, s));
   if (in.f1.pim && in.f1.pd && rest > 0) {
     HIP_TRY(c, hipMemcpyAsync(ps.pim1, in.f1.pim, 3 * rest * 8, hipMemcpyDeviceToDevice, s));
     HIP_TRY(c, hipMemcpyAsync(ps.pd1, in.f1.pd, rest * 8, hipMemcpyDeviceToDevice, s));
@@ -954,6 +993,7 @@ int batch_load(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const rgbdflow_
   c->loaded = false;
   c->ran = false;
   c->gen++;
+  c->dev_sig.clear();
   c->pairs.assign(n, PairSlot());
   for (int i = 0; i < n; i++) TRY(load_pair(c, p, i, in[i], c->pairs[i]));
   c->load_ratio = p->pyramid_ratio;
@@ -1023,6 +1063,7 @@ int batch_run(rgbdflow_ctx* c, const rgbdflow_params* p, rgbdflow_stats* st) {
       o.width[l] = ps.L[l].W;
       o.height[l] = ps.L[l].H;
       ctl_to_stats(ctl[i][l], k.n_outer, l < nL - 1, o.outer_exec[l], o.sweeps_exec[l], o.last_stat[l]);
+      ctl_residual_stats(ctl[i][l], &o, l);
     }
     o.device_ms = ms;
     collect_relax_timing(c, &o);
@@ -1052,6 +1093,7 @@ void fill_stats(rgbdflow_ctx* c, const SolveCtl& ctl, const LevelView& view, int
   st->width[0] = view.W;
   st->height[0] = view.H;
   ctl_to_stats(ctl, n_outer, view.allow_early != 0, st->outer_exec[0], st->sweeps_exec[0], st->last_stat[0]);
+  ctl_residual_stats(ctl, st, 0);
   float ms = 0;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
   st->device_ms = ms;
@@ -1159,6 +1201,10 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
   if (!Im1 || !dm1 || !Im2 || !dm2 || !warpIm2 || !warpDm2 || !u || !v || !w || !dW || !sW || !mW || !rW)
     return fail(c, RGBDFLOW_ERR_ARG, "null input");
   TRY(validate_iters(c, nOuter, nInner, nSOR));
+  if (opts && (opts->inner_exit < RGBDFLOW_EXIT_NONE || opts->inner_exit > RGBDFLOW_EXIT_RESIDUAL))
+    return fail(c, RGBDFLOW_ERR_ARG, "bad inner_exit");
+  if (opts && opts->pixel_update != RGBDFLOW_PIXEL_JACOBI && opts->pixel_update != RGBDFLOW_PIXEL_GS)
+    return fail(c, RGBDFLOW_ERR_ARG, "bad pixel_update");
   HIP_TRY(c, hipSetDevice(c->device));
   const size_t N = (size_t)W * H;
   Work wk;
@@ -1168,7 +1214,7 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
   view.allow_early = allowEarly ? 1 : 0;
   if (opts) {
     view.storage = opts->storage;
-    set_omega(view, opts->omega);
+    set_omega(view, opts->omega, opts->pixel_update);
   }
   const SolveScalars k = scalars_from(opts, cw, alpha, gamma, nOuter, nInner, nSOR);
   std::vector<LevelView> items{view};
@@ -1224,14 +1270,14 @@ int rgbdflow_level_system(rgbdflow_ctx* c, int W, int H, float focal_level, cons
   return RGBDFLOW_OK;
 }
 
-int rgbdflow_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, int n_iter, const rgbdflow_params* sched,
-                   double* x) {
-  if (!c) return RGBDFLOW_ERR_ARG;
-  if (bad_dims(W, H) || !coef13 || !x || n_iter < 0) return fail(c, RGBDFLOW_ERR_ARG, "bad relax arguments");
-  if (n_iter / kCheckEvery + 1 > kMaxChecks) return fail(c, RGBDFLOW_ERR_ARG, "n_iter too large");
-  HIP_TRY(c, hipSetDevice(c->device));
+namespace {
+
+// Uploads a compact system (13 planes, f32 values widened to f64) as f32
+// records and x ([3][N] planes) as the increment, and plans one relaxation
+// step over it.
+int stage_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, const double* x, const rgbdflow_params* sched,
+                Work& wk, StepPlan& sp, LevelView& view) {
   const size_t N = (size_t)W * H;
-  Work wk;
   TRY(get_work(c, "d.", N, wk));
   // planes -> records (the reciprocals are IEEE single-precision divisions, as on the device)
   std::vector<float> rec((size_t)kRec * 4 * N), hx(4 * N, 0.f);
@@ -1245,18 +1291,65 @@ int rgbdflow_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, int n_it
   }
   TRY(upload(c, wk.coef, rec.data(), rec.size() * 4));
   TRY(upload(c, wk.du, hx.data(), hx.size() * 4));
-  std::vector<LevelView> items{make_view(wk, W, H, nullptr, nullptr, wk.u[0], 0)};
-  StepPlan sp;
+  view = make_view(wk, W, H, nullptr, nullptr, wk.u[0], 0);
+  if (sched) set_omega(view, sched->omega, sched->pixel_update);
+  std::vector<LevelView> items{view};
   TRY(one_step(c, "d.desc", items, sched, sp));
   launch_ctl_reset(c->stream, sp.vpix);
-  SolveScalars k{};
-  k.n_sor = n_iter;
-  k.inner_exit = 0;  // checks are recorded but never stop this call
-  enqueue_relax(c, sp, k, 0);
+  return RGBDFLOW_OK;
+}
+
+int unstage_relax(rgbdflow_ctx* c, const Work& wk, int W, int H, double* x) {
+  const size_t N = (size_t)W * H;
+  std::vector<float> hx(4 * N);
   TRY(download(c, hx.data(), wk.du, hx.size() * 4));
   TRY(sync(c));
   for (size_t l = 0; l < N; l++)
     for (int k = 0; k < 3; k++) x[(size_t)k * N + l] = hx[l * 4 + k];
+  return RGBDFLOW_OK;
+}
+
+}  // namespace
+
+int rgbdflow_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, int n_iter, const rgbdflow_params* sched,
+                   double* x) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (bad_dims(W, H) || !coef13 || !x || n_iter < 0) return fail(c, RGBDFLOW_ERR_ARG, "bad relax arguments");
+  if (n_iter / kCheckEvery + 1 > kMaxChecks) return fail(c, RGBDFLOW_ERR_ARG, "n_iter too large");
+  HIP_TRY(c, hipSetDevice(c->device));
+  Work wk;
+  StepPlan sp;
+  LevelView view;
+  TRY(stage_relax(c, coef13, W, H, x, sched, wk, sp, view));
+  SolveScalars k{};
+  k.n_sor = n_iter;
+  k.inner_exit = 0;  // checks are recorded but never stop this call
+  enqueue_relax(c, sp, k, 0);
+  return unstage_relax(c, wk, W, H, x);
+}
+
+int rgbdflow_relax_residual(rgbdflow_ctx* c, const double* coef13, int W, int H, int max_iter, double tol,
+                            const rgbdflow_params* sched, double* x, int* iters, double* residual, double* energy) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (bad_dims(W, H) || !coef13 || !x || max_iter < 0 || !(tol >= 0))
+    return fail(c, RGBDFLOW_ERR_ARG, "bad relax_residual arguments");
+  if (max_iter / kCheckEvery + 1 > kMaxChecks) return fail(c, RGBDFLOW_ERR_ARG, "max_iter too large");
+  HIP_TRY(c, hipSetDevice(c->device));
+  Work wk;
+  StepPlan sp;
+  LevelView view;
+  TRY(stage_relax(c, coef13, W, H, x, sched, wk, sp, view));
+  SolveScalars k{};
+  k.n_sor = max_iter;
+  k.inner_exit = RGBDFLOW_EXIT_RESIDUAL;
+  k.residual_tol = tol;
+  enqueue_relax(c, sp, k, 0);
+  SolveCtl ctl;
+  TRY(download(c, &ctl, view.ctl, sizeof(SolveCtl)));
+  TRY(unstage_relax(c, wk, W, H, x));
+  if (iters) *iters = ctl.sweeps_exec;
+  if (residual) *residual = ctl.resid_checks ? ctl.resid : -1.0;
+  if (energy) *energy = ctl.energy;
   return RGBDFLOW_OK;
 }
 
@@ -1748,7 +1841,7 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   view.uv_scale = 1;
   view.reg_type = p->regularization_type;
   view.storage = p->storage;
-  set_omega(view, p->omega);
+  set_omega(view, p->omega, p->pixel_update);
   view.W_full = (int)p->cam.xRes;
   view.f_full = p->cam.focalLength;
   view.f_level = p->cam.focalLength;
@@ -1783,13 +1876,29 @@ int batch_load_device(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const Pa
   if (!c) return RGBDFLOW_ERR_ARG;
   if (!p || n < 1 || !in) return fail(c, RGBDFLOW_ERR_ARG, "bad batch arguments");
   HIP_TRY(c, hipSetDevice(c->device));
+  // The shapes of every slot; a batch with the same shapes as the loaded one
+  // gets the same slot buffers back from the pool, so its batch plan (and a
+  // captured graph of it) stays valid: only the frames are copied in.
+  std::string sig;
+  for (int i = 0; i < n; i++) {
+    const int f[6] = {in[i].W, in[i].H, in[i].min_width, in[i].f1.hpb != nullptr, in[i].init != nullptr,
+                      in[i].f1.pim != nullptr && in[i].f1.pd != nullptr};
+    sig.append(reinterpret_cast<const char*>(f), sizeof(f));
+    sig.append(reinterpret_cast<const char*>(&in[i].cam), sizeof(in[i].cam));
+  }
+  const bool same = c->loaded && !c->dev_sig.empty() && sig == c->dev_sig && (int)c->pairs.size() == n &&
+                    p->pyramid_ratio == c->load_ratio && p->min_width == c->load_min_width;
   c->loaded = false;
   c->ran = false;
-  c->gen++;
-  c->pairs.assign(n, PairSlot());
+  c->dev_sig.clear();
+  if (!same) {
+    c->gen++;
+    c->pairs.assign(n, PairSlot());
+  }
   for (int i = 0; i < n; i++) TRY(load_pair_device(c, p, i, in[i], c->pairs[i]));
   c->load_ratio = p->pyramid_ratio;
   c->load_min_width = p->min_width;
+  c->dev_sig = sig;
   c->loaded = true;
   return RGBDFLOW_OK;
 }

@@ -20,10 +20,13 @@ struct FrameDev {
 
 // One pair whose frames are on the device.  Frame 1's pyramids, when given,
 // are used as they are (they are the pyramids frame 1 had as a frame 2).
+// init: a flow estimate ([3][N] planes) -- the pair is then solved as
+// sceneFlowWithInit (finest level only), else null.
 struct PairDev {
   int W, H, min_width;
   rgbdflow_camera cam;
   FrameDev f1, f2;
+  const double* init;
 };
 
 int batch_load_device(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const PairDev* pairs);

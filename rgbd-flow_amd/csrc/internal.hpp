@@ -42,6 +42,9 @@ struct SolveCtl {
   int outer_exit_at;                  // outer iteration that set outer_done
   int sweeps_exec;                    // red-black iterations executed (all outers)
   int du_in_alt;                      // the last tiled launch that ran wrote du_alt
+  int resid_checks;                   // residual evaluations of this level solve
+  double resid;                       // last relative residual ||r - M du|| / ||r|| (f64)
+  double energy;                      // last energy r'du - 1/2 du'M du of the system (f64)
   unsigned int maxdiff[kMaxChecks];   // float bits of max|du change| per check of the
                                       // current outer, [inner][check]; zeroed per outer
   double stat[kMaxOuter];             // outer convergence statistic per outer iteration
@@ -101,6 +104,8 @@ struct LevelView {
   int storage;          // system record storage (records.hpp: f32, f16, bf16)
   float omega;          // relaxation factor (.cu:461: 0.7)
   float omega_c;        // 1 - omega, rounded to float as the update computes it
+  int pixel_gs;         // 0: a pixel's unknowns update from each other's old values (.cu:431-459);
+                        // 1: u, then v from the new u, then w from both (point SOR order)
   int W_full;           // camera xRes (full size)
   float f_full;         // full-size focal length
   float f_level;        // focal length of the level
@@ -139,11 +144,16 @@ __device__ __forceinline__ void gstore(uint4* p, const uint4& v) {
   *(RGBDFLOW_GLOBAL gvec4u*)p = x;
 }
 
-// Sets a view's relaxation factor; a factor <= 0 selects the reference's 0.7.
-inline void set_omega(LevelView& v, float omega) {
+// Sets a view's relaxation factor (<= 0 selects the reference's 0.7) and the
+// order of a pixel's unknowns (pixel_gs, see LevelView).
+inline void set_omega(LevelView& v, float omega, int pixel_gs = 0) {
   v.omega = omega > 0 ? omega : .7f;
   v.omega_c = 1.f - v.omega;
+  v.pixel_gs = pixel_gs ? 1 : 0;
 }
+
+// Tiled relaxation kernel variants by update rule.
+enum RelaxMode { kRelaxFree = 0, kRelaxFixed = 1, kRelaxGS = 2 };
 
 // A launch's work-list: items plus the first block of each item (n + 1 entries).
 template <class T>
@@ -200,7 +210,8 @@ struct PairPrep {
 struct SolveScalars {
   double alpha, gamma, color_weight;
   int n_outer, n_inner, n_sor;
-  int inner_exit;           // relaxation early exits enabled
+  int inner_exit;           // relaxation exits: 0 none, 1 the reference's (max change), 2 FP64 residual
+  double residual_tol;      // inner_exit 2: stop once the relative residual is <= this
   int outer_exit;           // outer early exits enabled
 };
 
@@ -225,6 +236,11 @@ void launch_relax_plain(hipStream_t s, const Views& v, int blocks, int colour, i
                         int check_slot);
 void launch_update_warp(hipStream_t s, const Views& v, int blocks, int want_stat);
 void launch_stat_final(hipStream_t s, const Views& v, int outer, int decide);
+// FP64 residual and energy of every item's system at its current increment
+// (alt: the increment is in du_alt), recorded in SolveCtl; check_slot >= 0
+// also records the stopping decision (residual <= tol) in that check slot.
+void launch_residual(hipStream_t s, const Views& v, int blocks, int alt, int slot_base, int n_prev_checks,
+                     int check_slot, double tol);
 
 // kernels_relax_tiled.hip
 int tiled_config_count();
@@ -245,8 +261,9 @@ inline int tiled_slots(const TileSched& t, int W, int H, int T) {
   }
   return (PR + 2) * (RH + 2);
 }
-// omega_fixed: every item of the launch relaxes with the factor 0.7.
-void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int storage, bool omega_fixed,
+// mode (RelaxMode): kRelaxFixed when every item relaxes as the reference (factor 0.7, old
+// values within the pixel), kRelaxGS when every item uses the point SOR order, else kRelaxFree.
+void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int storage, int mode,
                         int slots, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot);
 void launch_copy_du(hipStream_t s, const Views& v, int blocks);
 void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H, int storage);

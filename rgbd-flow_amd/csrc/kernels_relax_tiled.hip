@@ -45,7 +45,7 @@ struct PixRegs {
 // dmax: pixels farther than dmax (Chebyshev distance, in pixels) from the
 // tile's interior are left as they are -- no value the interior ends with
 // depends on them any more.
-template <int P, int C>
+template <int P, int C, bool GS>
 __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int PRp, float omega, float omega_c,
                                            const PixRegs (&pr)[P][2],
                                            const int (&qi)[P], const int (&par)[P], const unsigned (&fl)[P][2],
@@ -87,7 +87,8 @@ __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int P
     c.rw = r.rw;
     const float y3[3] = {r.yu, r.yv, r.yw};
     float xn[3];
-    relax_update_dense(c, omega, omega_c, y3, f & 64u, L, U, R, D, o, xn);
+    if (GS) relax_update_dense_gs(c, omega, omega_c, y3, f & 64u, L, U, R, D, o, xn);
+    else relax_update_dense(c, omega, omega_c, y3, f & 64u, L, U, R, D, o, xn);
     Ac[idx] = make_float4(xn[0], xn[1], xn[2], r.rxy);
     if (last && (f & 32u))
       change = fmaxf(change, fmaxf(fmaxf(fabsf(xn[0] - o.x), fabsf(xn[1] - o.y)), fabsf(xn[2] - o.z)));
@@ -101,7 +102,7 @@ __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int P
 // nothing; the last launch that ran records in SolveCtl::du_in_alt where du
 // is, and k_copy_du moves it to du at the end.  t: the tile's index in the
 // launch's work-list.
-template <int NT, int P, int ST, bool FIXED>
+template <int NT, int P, int ST, int MODE>
 __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, int flip, int n_iter, int slot_base,
                                            int n_prev_checks, int check_slot) {
   const int item = work_item(wl, t);
@@ -200,8 +201,9 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
   }
 
   float change = 0.f;
-  // FIXED: the reference's factor 0.7 as constants (a factor held in registers
-  // costs the widest configurations spills in the iteration loop)
+  // kRelaxFixed: the reference's factor 0.7 as constants (a factor held in
+  // registers costs the widest configurations spills in the iteration loop)
+  constexpr bool FIXED = MODE == kRelaxFixed, GS = MODE == kRelaxGS;
   const float om = FIXED ? .7f : v.omega, om_c = FIXED ? 1.f - .7f : v.omega_c;
   // After iteration it only pixels within 2 * (n_iter - 1 - it) of the
   // interior still matter: the second colour's pass there reads the first
@@ -213,9 +215,9 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
     const bool last = it == n_iter - 1;
     const int d = 2 * (n_iter - 1 - it);
     __syncthreads();
-    tiled_pass<P, 0>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, d + 1, last, change);
+    tiled_pass<P, 0, GS>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, d + 1, last, change);
     __syncthreads();
-    tiled_pass<P, 1>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, d, last, change);
+    tiled_pass<P, 1, GS>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, d, last, change);
   }
   __syncthreads();
 #pragma unroll
@@ -244,7 +246,7 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
 // the 8 XCDs round-robin by id, and the grid is a multiple of 8, so workgroup
 // b always runs on XCD b % 8; each XCD gets a contiguous run of tiles, whose
 // shared halos then hit in that XCD's L2.
-template <int NT, int P, int MB, int ST, bool FIXED>
+template <int NT, int P, int MB, int ST, int MODE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 256 * MB)))
 k_relax_tiled(Views wl, int total, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot) {
   extern __shared__ float4 X4[];
@@ -252,7 +254,7 @@ k_relax_tiled(Views wl, int total, int flip, int n_iter, int slot_base, int n_pr
   for (int b = blockIdx.x; b < 8 * per_xcd; b += gridDim.x) {
     const int t = (b & 7) * per_xcd + (b >> 3);
     if (t >= total) continue;
-    relax_tile<NT, P, ST, FIXED>(X4, wl, t, flip, n_iter, slot_base, n_prev_checks, check_slot);
+    relax_tile<NT, P, ST, MODE>(X4, wl, t, flip, n_iter, slot_base, n_prev_checks, check_slot);
     __syncthreads();  // the next tile reuses the LDS
   }
 }
@@ -271,13 +273,15 @@ __global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
 typedef void (*TiledFn)(Views, int, int, int, int, int, int);
 struct TiledConfig {
   int nt, p;
-  TiledFn fn[3][2];  // per system storage (records.hpp), [1]: relaxation factor 0.7
+  TiledFn fn[3][3];  // per system storage (records.hpp) and RelaxMode
 };
 
 // (threads, pairs per thread, workgroups per CU the registers must allow);
 // the record storage is a template parameter so each variant's load is
 // compiled on its own
-#define RGBDFLOW_TILED_ST(NT, P, MB, ST) {k_relax_tiled<NT, P, MB, ST, false>, k_relax_tiled<NT, P, MB, ST, true>}
+#define RGBDFLOW_TILED_ST(NT, P, MB, ST) \
+  {k_relax_tiled<NT, P, MB, ST, kRelaxFree>, k_relax_tiled<NT, P, MB, ST, kRelaxFixed>, \
+   k_relax_tiled<NT, P, MB, ST, kRelaxGS>}
 #define RGBDFLOW_TILED(NT, P, MB)                                                                    \
   TiledConfig {                                                                                      \
     NT, P, {                                                                                         \
@@ -326,20 +330,20 @@ void launch_copy_du(hipStream_t s, const Views& v, int blocks) {
   hipLaunchKernelGGL(k_copy_du, dim3(blocks), dim3(kBlock), 0, s, v);
 }
 
-void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int storage, bool omega_fixed,
+void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int storage, int mode,
                         int slots, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot) {
   const TiledConfig& c = kConfigs[config];
-  const TiledFn fn = c.fn[storage][omega_fixed ? 1 : 0];
-  const int key = (config * 3 + storage) * 2 + (omega_fixed ? 1 : 0);
+  const TiledFn fn = c.fn[storage][mode];
+  const int key = (config * 3 + storage) * 3 + mode;
   const size_t lds = tiled_lds_bytes(slots);
   // per-thread caches: launches may come from several host threads (one per context)
-  thread_local size_t raised[128] = {};
+  thread_local size_t raised[160] = {};
   if (lds > 65536 && lds > raised[key]) {
     hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     raised[key] = lds;
   }
   // resident workgroups per CU for this shape (cached per kernel and LDS size)
-  thread_local int occ_cfg[128] = {}, occ_lds[128] = {}, occ_val[128] = {}, cus = 0;
+  thread_local int occ_cfg[160] = {}, occ_lds[160] = {}, occ_val[160] = {}, cus = 0;
   if (occ_lds[key] != (int)lds || occ_cfg[key] == 0) {
     int n = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(fn), c.nt, lds) !=

@@ -508,7 +508,7 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
     const float xo[3] = {o.x, o.y, o.z}, xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z},
                 xd[3] = {D.x, D.y, D.z};
     float xn[3];
-    relax_update(c, v.omega, v.omega_c, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn);
+    relax_update(c, v.omega, v.omega_c, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn, v.pixel_gs != 0);
     gstore(x + l, make_float4(xn[0], xn[1], xn[2], 0.f));
     change = fmaxf(fmaxf(fabsf(xn[0] - xo[0]), fabsf(xn[1] - xo[1])), fabsf(xn[2] - xo[2]));
   }
@@ -576,6 +576,106 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat)
   if (threadIdx.x == 0) gp(v.partials)[lb] = red[0];
 }
 
+// FP64 residual of the relaxation's system at the current increment x, the
+// stopping test of RGBDFLOW_EXIT_RESIDUAL.  Per pixel and unknown k:
+//   res_k = r_k - (sum_n e_nk x_nk + sum_k' B_kk' x_k')
+// (the operator relax_update inverts: left/up edge weights are the
+// neighbours' right/down ones, missing neighbours contribute nothing).  Each
+// product of two f32 values is exact in f64; the terms are summed in f64 in
+// the order written.  Per block: sum res^2, sum r^2 and sum x.(r + res) (the
+// energy r'x - 1/2 x'Mx -- convex, -M being positive definite, and
+// minimised by the solution -- equals 1/2 sum x.(r + res)), reduced with wave
+// shuffles and then across the block's waves in a fixed order.
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_residual(Views wl, int alt, int slot_base, int n_prev_checks) {
+  __shared__ double red[3][kBlock / 64];
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int item = work_item(wl, blk);
+  const LevelView& v = wl.items[item];
+  if (v.ctl->outer_done || relax_stopped(v.ctl, slot_base, n_prev_checks)) return;
+  const int lb = blk - wl.start[item];
+  const int l = lb * kBlock + threadIdx.x;
+  const int W = v.W, H = v.H, N = v.N;
+  const float4* x = alt ? v.du_alt : v.du;
+  double rr = 0, bb = 0, ex = 0;
+  if (l < N) {
+    const int i = l / W, j = l - i * W;
+    const bool hl = j > 0, hu = i > 0, hr = j < W - 1, hd = i < H - 1;
+    const PixCoef c = load_coef(v.coef, v.storage, W, l, hl, hu);
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 o = gload(x + l), L = hl ? gload(x + l - 1) : zero, U = hu ? gload(x + l - W) : zero,
+                 R = hr ? gload(x + l + 1) : zero, D = hd ? gload(x + l + W) : zero;
+    const double xo[3] = {o.x, o.y, o.z}, xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z},
+                 xr[3] = {R.x, R.y, R.z}, xd[3] = {D.x, D.y, D.z};
+    const double B[3][3] = {{c.Buu, c.Buv, c.Buw}, {c.Buv, c.Bvv, c.Bvw}, {c.Buw, c.Bvw, c.Bww}};
+    const double rhs[3] = {c.ru, c.rv, c.rw};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double mx = 0;
+      if (hl) mx += (double)(k < 2 ? c.lxy : c.lz) * xl[k];
+      if (hu) mx += (double)(k < 2 ? c.uxy : c.uz) * xu[k];
+      if (hr) mx += (double)(k < 2 ? c.rxy : c.rz) * xr[k];
+      if (hd) mx += (double)(k < 2 ? c.dxy : c.dz) * xd[k];
+      mx += B[k][0] * xo[0];
+      mx += B[k][1] * xo[1];
+      mx += B[k][2] * xo[2];
+      const double res = rhs[k] - mx;
+      rr += res * res;
+      bb += rhs[k] * rhs[k];
+      ex += xo[k] * (rhs[k] + res);
+    }
+  }
+  rr = wave_sum(rr);
+  bb = wave_sum(bb);
+  ex = wave_sum(ex);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][wv] = rr;
+    red[1][wv] = bb;
+    red[2][wv] = ex;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double a = 0;
+    for (int k = 0; k < kBlock / 64; k++) a += red[threadIdx.x][k];
+    gp(v.partials)[3 * lb + threadIdx.x] = a;
+  }
+}
+
+// One block per item: fixed-order sums of the item's residual partials ->
+// SolveCtl::resid / energy; with a check slot, the slot records whether the
+// relaxation stops here (0 reads as "stop" and 0.01 as "go on" to every later
+// relaxation launch's relax_stopped test).
+__global__ void __launch_bounds__(kBlock) k_residual_final(Views w, int slot_base, int n_prev_checks, int check_slot,
+                                                           double tol) {
+  __shared__ double red[3][kBlock];
+  const LevelView& v = w.items[blockIdx.x];
+  if (v.ctl->outer_done || relax_stopped(v.ctl, slot_base, n_prev_checks)) return;
+  const int nblocks = pix_blocks(v.N);
+  double a[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += kBlock)
+    for (int k = 0; k < 3; k++) a[k] += gp(v.partials)[3 * b + k];
+  for (int k = 0; k < 3; k++) red[k][threadIdx.x] = a[k];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double rr = red[0][0], bb = red[1][0];
+    const double rel = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+    v.ctl->resid = rel;
+    v.ctl->energy = 0.5 * red[2][0];
+    v.ctl->resid_checks += 1;
+    if (check_slot >= 0) v.ctl->maxdiff[check_slot] = rel <= tol ? 0u : __float_as_uint(.01f);
+  }
+}
+
 // One block per item: fixed-order sum of the item's partials -> outer
 // statistic; sets the item's outer_done when early termination applies
 // (.cu:943-951).
@@ -628,6 +728,11 @@ void launch_update_warp(hipStream_t s, const Views& v, int blocks, int want_stat
 }
 void launch_stat_final(hipStream_t s, const Views& v, int outer, int decide) {
   hipLaunchKernelGGL(k_stat_final, dim3(v.n), dim3(kBlock), 0, s, v, outer, decide);
+}
+void launch_residual(hipStream_t s, const Views& v, int blocks, int alt, int slot_base, int n_prev_checks,
+                     int check_slot, double tol) {
+  hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(kBlock), 0, s, v, alt, slot_base, n_prev_checks);
+  hipLaunchKernelGGL(k_residual_final, dim3(v.n), dim3(kBlock), 0, s, v, slot_base, n_prev_checks, check_slot, tol);
 }
 
 }  // namespace rgbdflow

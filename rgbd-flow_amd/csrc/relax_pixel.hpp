@@ -52,12 +52,15 @@ __device__ __forceinline__ PixCoef load_coef(const float4* C, int storage, int W
 
 // One pixel's update.  h* say which neighbours exist in the image; x* hold the
 // neighbours' (other colour) values, xo the pixel's old values.
-// omega_c = 1 - omega in float.
+// omega_c = 1 - omega in float.  gs: the pixel's other unknowns enter with
+// the values this update already gave them (u, then v, then w: point SOR
+// order) instead of their old values (.cu:431-459).
 __device__ __forceinline__ void relax_update(const PixCoef& c, float omega, float omega_c, bool hl, bool hu, bool hr,
                                              bool hd, const float xl[3], const float xu[3], const float xr[3],
-                                             const float xd[3], const float xo[3], float xn[3]) {
+                                             const float xd[3], const float xo[3], float xn[3], bool gs = false) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
+    const float x0 = (gs && k > 0) ? xn[0] : xo[0], x1 = (gs && k > 1) ? xn[1] : xo[1];
     const float el = k < 2 ? c.lxy : c.lz;
     const float eu = k < 2 ? c.uxy : c.uz;
     const float er = k < 2 ? c.rxy : c.rz;
@@ -74,13 +77,13 @@ __device__ __forceinline__ void relax_update(const PixCoef& c, float omega, floa
       nv /= c.Buu;
     } else if (k == 1) {
       nv += c.rv;
-      nv -= c.Buv * xo[0];
+      nv -= c.Buv * x0;
       nv -= c.Bvw * xo[2];
       nv /= c.Bvv;
     } else {
       nv += c.rw;
-      nv -= c.Buw * xo[0];
-      nv -= c.Bvw * xo[1];
+      nv -= c.Buw * x0;
+      nv -= c.Bvw * x1;
       nv /= c.Bww;
     }
     float r = omega * nv + omega_c * xo[k];
@@ -160,6 +163,49 @@ __device__ __forceinline__ void relax_update_dense(const PixCoef& c, float omega
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     float r = omega * nv[k] + omega_c * xo[k];
+    if (r < xo[k] - .3f) r = xo[k] - .3f;
+    else if (r > xo[k] + .3f) r = xo[k] + .3f;
+    xn[k] = r;
+  }
+}
+
+// relax_update_dense in the point SOR order (relax_update with gs): each
+// unknown is divided, relaxed and clamped before the next one reads it.  The
+// reciprocal division is chosen per unknown; in its domain it equals the IEEE
+// quotient, so the choice does not change the value.
+__device__ __forceinline__ void relax_update_dense_gs(const PixCoef& c, float omega, float omega_c,
+                                                      const float (&y3)[3], bool den_ok, const float4& L,
+                                                      const float4& U, const float4& R, const float4& D,
+                                                      const float4& o, float xn[3]) {
+  const float xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z}, xd[3] = {D.x, D.y, D.z};
+  const float xo[3] = {o.x, o.y, o.z};
+  const float den[3] = {c.Buu, c.Bvv, c.Bww};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float el = k < 2 ? c.lxy : c.lz;
+    const float eu = k < 2 ? c.uxy : c.uz;
+    const float er = k < 2 ? c.rxy : c.rz;
+    const float ed = k < 2 ? c.dxy : c.dz;
+    float a = 0.f;
+    a -= el * xl[k];
+    a -= eu * xu[k];
+    a -= er * xr[k];
+    a -= ed * xd[k];
+    if (k == 0) {
+      a += c.ru;
+      a -= c.Buv * xo[1];
+      a -= c.Buw * xo[2];
+    } else if (k == 1) {
+      a += c.rv;
+      a -= c.Buv * xn[0];
+      a -= c.Bvw * xo[2];
+    } else {
+      a += c.rw;
+      a -= c.Buw * xn[0];
+      a -= c.Bvw * xn[1];
+    }
+    const float q = (den_ok && recip_div_ok(a)) ? recip_div(a, den[k], y3[k]) : a / den[k];
+    float r = omega * q + omega_c * xo[k];
     if (r < xo[k] - .3f) r = xo[k] - .3f;
     else if (r > xo[k] + .3f) r = xo[k] + .3f;
     xn[k] = r;

@@ -3,7 +3,8 @@
 // Each stream keeps its last frame on the device (colour and depth planes,
 // boundary maps, and the colour and depth pyramids it got as a pair's frame
 // 2), so a new frame is uploaded once and the pair (last frame, new frame)
-// reuses frame 1's pyramids.  A worker thread forms batches from pending
+// reuses frame 1's pyramids.  With warm starts the stream also keeps its last
+// flow, and its next pair is solved as sceneFlowWithInit from it.  A worker thread forms batches from pending
 // requests -- at most one pair per stream in flight, at most max_batch pairs,
 // admitted while the in-flight pixels fit the HBM budget -- and runs them on
 // `lanes` executor contexts with their own HIP streams, so a batch formed
@@ -52,6 +53,8 @@ struct StreamState {
   rgbdflow_camera cam{};
   size_t N = 0, rest = 0, hpb_n = 0;
   Frame f[2];
+  double* flow = nullptr;  // [3][N] the stream's last result (warm starts)
+  bool has_flow = false;
   int cur = -1;          // index of the last frame, -1 before the first
   bool busy = false;     // a request of this stream is in a batch
   int queued = 0;        // requests of this stream waiting
@@ -70,6 +73,7 @@ struct Request {
   std::vector<float> depth, hpb;
   Clock::time_point t_submit;
   int next = 0;  // frame slot the new frame goes to
+  bool warm = false;  // solved from the stream's last flow
 };
 
 struct Result {
@@ -82,6 +86,7 @@ struct Lane {
   rgbdflow_ctx* ctx = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
+  bool warm = false;             // the batch's pairs are warm starts
   std::vector<Request> reqs;     // in dispatch order
   std::vector<int> pair_of;      // request -> pair index in the batch, -1 for a first frame
   int status = RGBDFLOW_OK;
@@ -95,6 +100,8 @@ struct rgbdflow_server {
   int device = 0;
   int max_batch = 16;
   rgbdflow_params params{};
+  bool warm = false;
+  rgbdflow_params warm_params{};  // params of warm-started pairs
   unsigned long long budget = 0;
   unsigned long long state_bytes = 0;
   size_t inflight_px = 0;
@@ -150,13 +157,15 @@ void dispatch(rgbdflow_server* S, Lane& ln) {
       pd.cam = st.cam;
       pd.f1 = FrameDev{P.im, P.dm, P.pyr ? P.pim : nullptr, P.pyr ? P.pd : nullptr, q.hpb.empty() ? nullptr : P.hpb};
       pd.f2 = FrameDev{F.im, F.dm, nullptr, nullptr, nullptr};
+      pd.init = q.warm ? st.flow : nullptr;
       ln.pair_of[r] = (int)pairs.size();
       pairs.push_back(pd);
     }
   }
   if (!pairs.empty()) {
-    int rc = batch_load_device(ln.ctx, &S->params, (int)pairs.size(), pairs.data());
-    if (rc == RGBDFLOW_OK) rc = rgbdflow_batch_run(ln.ctx, &S->params, nullptr);
+    const rgbdflow_params* prm = ln.warm ? &S->warm_params : &S->params;
+    int rc = batch_load_device(ln.ctx, prm, (int)pairs.size(), pairs.data());
+    if (rc == RGBDFLOW_OK) rc = rgbdflow_batch_run(ln.ctx, prm, nullptr);
     ln.status = rc;
   }
   hipEventRecord(ln.done, hs);
@@ -185,11 +194,12 @@ void finish(rgbdflow_server* S, Lane& ln) {
       continue;
     }
     Frame& F = st.f[q.next];
-    if (rest > 0) {
+    if (rest > 0 && !q.warm) {
       hipMemcpyAsync(F.pim, f2.pim, 3 * rest * 8, hipMemcpyDeviceToDevice, hs);
       hipMemcpyAsync(F.pd, f2.pd, rest * 8, hipMemcpyDeviceToDevice, hs);
     }
-    F.pyr = true;
+    F.pyr = !q.warm;  // a warm start builds no pyramids
+    if (S->warm) hipMemcpyAsync(st.flow, flow, 3 * st.N * 8, hipMemcpyDeviceToDevice, hs);
     res.u.resize(st.N);
     res.v.resize(st.N);
     res.w.resize(st.N);
@@ -208,6 +218,7 @@ void finish(rgbdflow_server* S, Lane& ln) {
     StreamState& st = *q.st;
     st.cur = q.next;
     st.busy = false;
+    if (S->warm && out[r].info.has_flow && e == hipSuccess) st.has_flow = true;
     Result& res = S->results[q.ticket];
     res.u.swap(out[r].u);
     res.v.swap(out[r].v);
@@ -227,11 +238,13 @@ void finish(rgbdflow_server* S, Lane& ln) {
 }
 
 // Picks the requests of the next batch (lock held): in submission order, one
-// per stream, while the pixels fit the budget.
+// per stream, while the pixels fit the budget; warm starts and coarse-to-fine
+// pairs go in separate batches (their params differ), the first pending pair
+// deciding which.
 void pick(rgbdflow_server* S, Lane& ln) {
   const size_t cap = budget_px(S);
   size_t px = 0;
-  int n_pairs = 0;
+  int n_pairs = 0, kind = -1;
   for (auto it = S->pending.begin(); it != S->pending.end() && n_pairs < S->max_batch;) {
     StreamState& st = *S->streams[it->stream];
     if (st.busy) {
@@ -239,6 +252,13 @@ void pick(rgbdflow_server* S, Lane& ln) {
       continue;
     }
     const bool pair = st.cur >= 0;
+    const int warm = (S->warm && st.has_flow) ? 1 : 0;
+    if (pair && kind >= 0 && warm != kind) {
+      ++it;
+      continue;
+    }
+    if (pair) kind = warm;
+    it->warm = warm != 0;
     if (pair && S->inflight_px + px + st.N > cap && (n_pairs > 0 || S->inflight_px > 0)) break;
     it->next = st.cur < 0 ? 0 : 1 - st.cur;
     it->st = &st;
@@ -252,6 +272,7 @@ void pick(rgbdflow_server* S, Lane& ln) {
     it = S->pending.erase(it);
   }
   ln.px = px;
+  ln.warm = kind == 1;
   S->inflight_px += px;
 }
 
@@ -304,6 +325,9 @@ int rgbdflow_server_create(rgbdflow_server** out, const rgbdflow_server_config* 
   S->max_batch = cfg->max_batch > 0 ? cfg->max_batch : 16;
   S->budget = cfg->hbm_budget;
   S->params = *params;
+  S->warm = cfg->warm_start != 0;
+  S->warm_params = *params;
+  if (cfg->warm_n_outer > 0) S->warm_params.n_outer = cfg->warm_n_outer;
   const int n_lanes = cfg->lanes > 0 ? cfg->lanes : 2;
   S->lanes.resize(n_lanes);
   for (Lane& ln : S->lanes) {
@@ -359,7 +383,7 @@ int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera
   auto r256 = [](size_t n) { return (n + 255) & ~size_t(255); };
   const size_t frame_bytes = r256(3 * st->N * 8) + r256(st->N * 8) + r256(3 * st->rest * 8 + 8) +
                              r256(st->rest * 8 + 8) + r256(st->hpb_n * 4);
-  const size_t bytes = 2 * frame_bytes + r256(3 * st->N) + r256(st->N * 4);
+  const size_t bytes = 2 * frame_bytes + r256(3 * st->N) + r256(st->N * 4) + (S->warm ? r256(3 * st->N * 8) : 0);
   {
     std::lock_guard<std::mutex> lk(S->m);
     if (S->budget && S->state_bytes + bytes > S->budget)
@@ -383,6 +407,7 @@ int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera
   }
   st->stage_bgr = reinterpret_cast<unsigned char*>(take(3 * st->N));
   st->stage_depth = reinterpret_cast<float*>(take(st->N * 4));
+  if (S->warm) st->flow = reinterpret_cast<double*>(take(3 * st->N * 8));
   std::lock_guard<std::mutex> lk(S->m);
   S->state_bytes += st->bytes;
   *stream_id = S->next_stream++;

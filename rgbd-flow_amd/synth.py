@@ -109,19 +109,10 @@ def _intersect(plane: _Plane, R: np.ndarray, t: np.ndarray, rays: np.ndarray):
     return np.where(ok, z, np.inf)
 
 
-def make_pair(width: int, height: int, seed: int, n_segments: int = 6,
-              cam: Camera | None = None, max_translation: float = 0.05,
-              max_rotation_deg: float = 3.0, dropout: float = 0.0) -> dict:
-    """One frame pair.  Returns dict(img1, depth1, img2, depth2 (uint16 mm),
-    flow (H,W,3 f32 ground truth), seg1 (H,W int32), cam)."""
-    rng = np.random.default_rng(seed)
-    cam = cam or camera_for(width, height)
-    H, W = height, width
-    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
-    rays = np.stack([(xs - cam.cx) / cam.fx, (ys - cam.cy) / cam.fy, np.ones_like(xs)], -1)
-
+def _scene_planes(rng: np.random.Generator, W: int, H: int, cam: Camera, n_segments: int):
+    """A background plane (nearly fronto-parallel, 3.0-3.8 m) and n_segments - 1
+    textured rectangles at 0.8-2.8 m."""
     planes = []
-    # background: nearly fronto-parallel, 3.0-3.8 m
     tilt = _rotation(rng.normal(size=3) * np.array([1, 1, 0]) + 1e-9, math.radians(rng.uniform(0, 15)))
     planes.append(_Plane(tilt @ np.array([0, 0, -1.0]), np.array([0, 0, rng.uniform(3.0, 3.8)])))
     for _ in range(n_segments - 1):
@@ -135,13 +126,34 @@ def make_pair(width: int, height: int, seed: int, n_segments: int = 6,
         ang = rng.uniform(0, math.pi)
         e1, e2 = math.cos(ang) * e1 + math.sin(ang) * e2, -math.sin(ang) * e1 + math.cos(ang) * e2
         planes.append(_Plane(n, c, (rng.uniform(0.15, 0.5), rng.uniform(0.15, 0.5)), (e1, e2)))
+    return planes
 
+
+def _motion(rng: np.random.Generator, max_translation: float, max_rotation_deg: float):
+    """A random rigid motion: (rotation axis, angle in radians, translation)."""
+    d = rng.normal(size=3)
+    tr = d / np.linalg.norm(d) * rng.uniform(0, max_translation)
+    axis = rng.normal(size=3)
+    angle = math.radians(rng.uniform(0, max_rotation_deg))
+    return axis, angle, tr
+
+
+def make_pair(width: int, height: int, seed: int, n_segments: int = 6,
+              cam: Camera | None = None, max_translation: float = 0.05,
+              max_rotation_deg: float = 3.0, dropout: float = 0.0) -> dict:
+    """One frame pair.  Returns dict(img1, depth1, img2, depth2 (uint16 mm),
+    flow (H,W,3 f32 ground truth), seg1 (H,W int32), cam)."""
+    rng = np.random.default_rng(seed)
+    cam = cam or camera_for(width, height)
+    H, W = height, width
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    rays = np.stack([(xs - cam.cx) / cam.fx, (ys - cam.cy) / cam.fy, np.ones_like(xs)], -1)
+
+    planes = _scene_planes(rng, W, H, cam, n_segments)
     motions = []
     for _ in planes:
-        d = rng.normal(size=3)
-        tr = d / np.linalg.norm(d) * rng.uniform(0, max_translation)
-        Rm = _rotation(rng.normal(size=3), math.radians(rng.uniform(0, max_rotation_deg)))
-        motions.append((Rm, tr))
+        axis, angle, tr = _motion(rng, max_translation, max_rotation_deg)
+        motions.append((_rotation(axis, angle), tr))
 
     I3 = np.eye(3)
     z1 = np.stack([_intersect(p, I3, np.zeros(3), rays) for p in planes])
@@ -187,6 +199,41 @@ def make_pair(width: int, height: int, seed: int, n_segments: int = 6,
                 img2=np.clip(np.round(img2), 0, 255).astype(np.uint8),
                 depth1=d1, depth2=d2, flow=flow.astype(np.float32),
                 seg1=seg1.astype(np.int32), cam=cam)
+
+
+def make_sequence(width: int, height: int, seed: int, n_frames: int, n_segments: int = 6,
+                  cam: Camera | None = None, max_translation: float = 0.01,
+                  max_rotation_deg: float = 0.6) -> dict:
+    """A frame sequence: the scene of make_pair with every segment moving at a
+    constant random velocity (per frame: translation <= max_translation m,
+    rotation <= max_rotation_deg about a fixed axis), so frame k shows segment
+    s at pose (R_s^k, k t_s) -- a smooth SE(3) trajectory.  Returns
+    dict(frames=[dict(img (u8 BGR), depth (uint16 mm), seg (int32))], cam)."""
+    rng = np.random.default_rng(seed)
+    cam = cam or camera_for(width, height)
+    H, W = height, width
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    rays = np.stack([(xs - cam.cx) / cam.fx, (ys - cam.cy) / cam.fy, np.ones_like(xs)], -1)
+    planes = _scene_planes(rng, W, H, cam, n_segments)
+    velocity = [_motion(rng, max_translation, max_rotation_deg) for _ in planes]
+    textures = [_texture(rng, H, W) for _ in planes]
+    frames = []
+    for k in range(n_frames):
+        poses = [(_rotation(axis, k * angle), k * tr) for axis, angle, tr in velocity]
+        z = np.stack([_intersect(p, R, t, rays) for p, (R, t) in zip(planes, poses)])
+        seg = np.argmin(z, axis=0)
+        zt = np.clip(np.take_along_axis(z, seg[None], 0)[0], 0.5, 4.0)
+        img = np.zeros((H, W, 3))
+        X = rays * zt[..., None]
+        for i, (p, (R, t)) in enumerate(zip(planes, poses)):
+            m = seg == i
+            if m.any():
+                Xl = (X[m] - p.c - t) @ R + p.c           # R^T (X - c - t) + c
+                img[m] = _bilinear(textures[i], Xl[:, 0] / Xl[:, 2] * cam.fx + cam.cx,
+                                   Xl[:, 1] / Xl[:, 2] * cam.fy + cam.cy)
+        frames.append(dict(img=np.clip(np.round(img), 0, 255).astype(np.uint8), depth=_quantise_depth(zt),
+                           seg=seg.astype(np.int32)))
+    return dict(frames=frames, cam=cam)
 
 
 def to_solver_inputs(pair: dict):

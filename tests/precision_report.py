@@ -1,0 +1,155 @@
+"""Precision variants of the scene-flow solve, measured on one GPU.
+
+Usage: python tests/precision_report.py [--out FILE] [--pairs N]
+
+Workload: a batch of mixed-size synthetic pairs (sizes drawn from 320x240,
+640x480, 960x540 and 1280x720, intrinsics scaled by W/640, 2-8 moving
+segments), 5-level pyramids at ratio 0.5, 15 outer x 50 inner iterations,
+boundary-probability regularisation.  Variants: the storage of the per-pixel
+systems (f32 as the reference, f16 with a per-pixel power-of-two scale, bf16;
+the arithmetic is f32 in every variant) times the relaxation's update rule
+(the reference's: factor 0.7, a pixel's unknowns from each other's old
+values; point SOR: factor 1.8, u then v then w from the newest values).
+
+Reported per variant: batch pairs/s; flow distance to the f32 reference-rule
+result (EPE of u, v in px; |dw| in m); flow distance to the FP64 oracle of
+the same rule (every buffer f64, on the host's cores); iterations per outer
+iteration to bring the relative residual ||r - M du|| / ||r|| to 1e-4 (the
+FP64 residual exit, at most 500).  Beside them: the converged energies of the
+reference rule and point SOR on one 640x480 level system.
+
+This script is the checker's view of the variants: it uses the oracle, so
+it lives with the tests, not with the product.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+from helpers import ROOT, level_inputs, load_package, oracle  # noqa: E402
+
+SIZES = [(320, 240), (640, 480), (960, 540), (1280, 720)]
+
+
+def workload(R, n, seed=300):
+    import rgbdflow_amd.synth as S
+    O = oracle()
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for i in range(n):
+        w, h = SIZES[i % 4] if i < 4 else SIZES[int(rng.integers(0, 4))]
+        f = 525.0 * w / 640
+        cam = S.Camera(w, h, f, f, (w - 1) / 2, (h - 1) / 2)
+        pr = S.make_pair(w, h, seed + i, n_segments=int(rng.integers(2, 9)), cam=cam)
+        im1, d1, im2, d2 = S.to_solver_inputs(pr)
+        lv = R.pyramid_levels(w, h, 0.5, w // 16)
+        hpb = S.segment_boundary_maps(pr["seg1"], [a for a, b in lv], [b for a, b in lv])
+        pairs.append(dict(im1=im1, dm1=d1.astype(np.float64), im2=im2, dm2=d2.astype(np.float64), hpb=hpb,
+                          sigma1=O.depth_stdevs(d1, 4), cam=R.make_params(w, h, f=f).cam, min_width=w // 16,
+                          W=w, H=h, f=f))
+    return pairs
+
+
+def epe_stats(flows, refs):
+    e = np.concatenate([np.sqrt((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2).ravel() for a, b in zip(flows, refs)])
+    dw = np.concatenate([np.abs(a[2] - b[2]).ravel() for a, b in zip(flows, refs)])
+    return dict(epe_median=float(np.median(e)), epe_mean=float(e.mean()), epe_p99=float(np.percentile(e, 99)),
+                dw_median=float(np.median(dw)), dw_p99=float(np.percentile(dw, 99)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pairs", type=int, default=8)
+    ap.add_argument("--out", default="")
+    ap.add_argument("--threads", type=int, default=0, help="oracle threads (0: the cores this process may use)")
+    args = ap.parse_args()
+    R = load_package()
+    O = oracle()
+    nt = args.threads or min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "64") or 64))
+    O.set_threads(max(1, nt))
+    pairs = workload(R, args.pairs)
+    print(f"precision: {len(pairs)} pairs " + ", ".join(f"{p['W']}x{p['H']}" for p in pairs), flush=True)
+    base = dict(n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED, graph=True)
+    rules = {"reference": dict(omega=0.7, pixel_update=R.PIXEL_JACOBI),
+             "point_sor_1.8": dict(omega=1.8, pixel_update=R.PIXEL_GS)}
+    stores = {"f32": R.STORE_F32, "f16": R.STORE_F16, "bf16": R.STORE_BF16}
+    ctx = R.Context(0)
+    for p in pairs[:4]:
+        for (lw, lh) in R.pyramid_levels(p["W"], p["H"], 0.5, p["min_width"]):
+            ctx.autotune_relax(lw, lh, batch=max(1, len(pairs) // 4))
+    results, flows = [], {}
+    for rname, rule in rules.items():
+        for sname, st in stores.items():
+            p = R.make_params(640, 480, storage=st, **rule, **base)
+            ctx.batch_load(p, pairs)
+            ctx.batch_run(p, stats=True)
+            ctx.batch_run(p, stats=False)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                ctx.batch_run(p, stats=False)
+            ctx.synchronize()
+            pps = 3 * len(pairs) / (time.perf_counter() - t0)
+            flows[(rname, sname)] = [ctx.batch_read(i) for i in range(len(pairs))]
+            # iterations to residual <= 1e-4 (FP64 residual exit, at most 500 per outer)
+            pr = R.make_params(640, 480, storage=st, inner_exit=R.EXIT_RESIDUAL, residual_tol=1e-4,
+                               **rule, **dict(base, n_sor=500, graph=False))
+            ctx.batch_load(pr, pairs)
+            sts = ctx.batch_run(pr, stats=True)
+            its = [s.sweeps_exec[0] / max(1, s.outer_exec[0]) for s in sts]
+            res = [s.residual[0] for s in sts]
+            results.append({"rule": rname, "storage": sname, "pairs_per_s": round(pps, 2),
+                            "finest_iters_per_outer_to_1e-4": round(float(np.mean(its)), 1),
+                            "finest_last_residual_max": float(np.max(res)),
+                            "finest_pairs_reaching_1e-4": int(sum(r <= 1e-4 for r in res))})
+            print(f"precision: {rname} {sname}: {pps:.1f} pairs/s, {np.mean(its):.1f} iterations/outer to 1e-4",
+                  flush=True)
+    ref = flows[("reference", "f32")]
+    # the FP64 oracle of each rule (reference exits), on the host
+    f64 = {}
+    for rname, rule in rules.items():
+        out = []
+        t0 = time.perf_counter()
+        for p in pairs:
+            po = O.make_params(p["W"], p["H"], f=p["f"], n_outer=15, n_sor=50, min_width=p["min_width"],
+                               precision=O.PREC_F64, omega=rule["omega"],
+                               pixel_gs=rule["pixel_update"] == R.PIXEL_GS)
+            u, v, w, _ = O.multiscale(p["im1"], p["dm1"], p["im2"], p["dm2"], p["sigma1"], p["hpb"], po)
+            out.append((u, v, w))
+        f64[rname] = out
+        print(f"precision: FP64 oracle, {rname}: {time.perf_counter() - t0:.1f} s on {nt} threads", flush=True)
+    for r in results:
+        fl = flows[(r["rule"], r["storage"])]
+        r["vs_f32_reference"] = epe_stats(fl, ref)
+        r["vs_fp64_oracle"] = epe_stats(fl, f64[r["rule"]])
+    # converged energies on one 640x480 level system (the level's first outer iteration)
+    li, (u, v, w), _ = level_inputs(seed=8, W=640, H=480)
+    coef = O.level_system(li, O.make_params(640, 480), u, v, w)
+    x0 = np.zeros((3, 480, 640))
+    en = {}
+    for rname, rule in rules.items():
+        _, it, rs, e = ctx.relax_residual(coef, 40000, 1e-6, x0, R.make_params(640, 480, relax_variant=R.RELAX_TILED,
+                                                                               **rule))
+        en[rname] = dict(iterations=it, residual=rs, energy=e)
+    en["relative_energy_difference"] = abs(en["point_sor_1.8"]["energy"] - en["reference"]["energy"]) / abs(
+        en["reference"]["energy"])
+    ctx.close()
+    report = dict(workload=f"{len(pairs)} mixed-size pairs (" + ", ".join(f"{p['W']}x{p['H']}" for p in pairs) +
+                  "), 5-level pyramids ratio 0.5, 15 outer x 50 inner, HPB regularisation",
+                  oracle_threads=nt, variants=results, converged_energy_640x480_level=en)
+    txt = json.dumps(report, indent=1)
+    print(txt)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(txt + "\n")
+
+
+if __name__ == "__main__":
+    main()

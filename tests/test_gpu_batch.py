@@ -179,3 +179,47 @@ def _small_system(O):
     from helpers import level_inputs
     li, (u, v, w), _ = level_inputs(seed=5, W=32, H=24)
     return li, O.make_params(32, 24), u, v, w
+
+
+@pytest.mark.parametrize("relax_variant", [0, 1])
+def test_batch_with_init_pairs(ctx, R, relax_variant):
+    """Pairs given an estimate are solved as sceneFlowWithInit (finest level
+    only, from the estimate, no outer early exit) inside a batch that also
+    holds ordinary coarse-to-fine pairs of other sizes: each equals the
+    oracle's with_init / multiscale of that pair alone, bit for bit."""
+    O = oracle()
+    a = _case(128, 96, 41, 16)
+    b = _case(160, 120, 42, 20)
+    c = _case(96, 72, 43, 12)
+    for k, case in enumerate((a, c)):
+        pair, *_ = make_case(case["W"], case["H"], 41 if k == 0 else 43, 4)
+        case["est"] = pair["flow"].astype(np.float64) * (0.9 if k == 0 else 0.5)
+    for case in (a, b, c):
+        case["cam"] = R.make_params(case["W"], case["H"]).cam
+    p = R.make_params(128, 96, n_outer=3, n_sor=40, min_width=16, relax_variant=relax_variant)
+    ctx.batch_load(p, [a, b, c])
+    sts = ctx.batch_run(p, stats=True)
+    for i, case in enumerate((a, b, c)):
+        ub, vb, wb = ctx.batch_read(i)
+        po = O.make_params(case["W"], case["H"], n_outer=3, n_sor=40, min_width=case["min_width"])
+        if "est" in case:
+            hpb0 = case["hpb"][: case["W"] * case["H"] * 2]
+            ua, va, wa, sa = O.with_init(case["im1"], case["dm1"], case["im2"], case["dm2"], case["sigma1"], hpb0,
+                                         case["est"], po)
+            assert sts[i].n_levels == 1 and sts[i].outer_exec[0] == 3
+            # the same as the single-pair entry point, bit for bit
+            pw = R.make_params(case["W"], case["H"], n_outer=3, n_sor=40, relax_variant=relax_variant)
+            with R.Context(0) as c2:
+                uc, vc, wc, sc = c2.with_init(pw, case["im1"], case["dm1"], case["im2"], case["dm2"], case["est"],
+                                              case["sigma1"], hpb0)
+            assert np.array_equal(uc, ub) and np.array_equal(vc, vb) and np.array_equal(wc, wb)
+            assert sc.sweeps_exec[0] == sts[i].sweeps_exec[0]
+        else:
+            ua, va, wa, sa = O.multiscale(case["im1"], case["dm1"], case["im2"], case["dm2"], case["sigma1"],
+                                          case["hpb"], po)
+        L = sa.n_levels
+        assert [sts[i].sweeps_exec[k] for k in range(L)] == [sa.sweeps_exec[k] for k in range(L)]
+        du, dv, dw = (float(np.abs(x - y).max()) for x, y in ((ua, ub), (va, vb), (wa, wb)))
+        assert du <= 1e-3 and dv <= 1e-3 and dw <= 1e-5, (i, du, dv, dw)
+        if "est" not in case:
+            assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)

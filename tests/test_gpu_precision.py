@@ -68,3 +68,78 @@ def test_storage_variants(ctx, R, storage):
           f"vs FP64 median {np.median(epe64):.4g} (FP32 vs FP64 median {np.median(epe32):.4g}); "
           f"|dw| vs FP32 median {np.median(np.abs(tiled[2] - ref[2])):.3g} m")
     assert np.median(epe) < 0.05 and np.isfinite(tiled[0]).all()
+
+
+# ----------------------------------------- point SOR order and the FP64 residual
+@pytest.fixture(scope="module")
+def system():
+    O = oracle()
+    from helpers import level_inputs
+    li, (u, v, w), _ = level_inputs(seed=8, W=96, H=72)
+    return O.level_system(li, O.make_params(96, 72), u, v, w)
+
+
+SCHEDULES = {
+    "plain": dict(relax_variant=0),
+    "tiled": dict(relax_variant=1, tile_iters=5, tile_w=24, tile_h=16),
+    "tiled_single": dict(relax_variant=1),
+}
+
+
+@pytest.mark.parametrize("sched", list(SCHEDULES))
+@pytest.mark.parametrize("gs,omega", [(False, 0.7), (True, 1.8), (True, 1.0), (False, 1.3)])
+def test_relax_residual_matches_oracle(ctx, R, system, sched, gs, omega):
+    """The FP64 residual exit on the device: the same iterations, the same
+    increments bit for bit, and residual / energy equal to the oracle's f64
+    reduction up to summation order."""
+    O = oracle()
+    x0 = np.zeros((3, 72, 96))
+    xa, ia, ra, ea = O.relax_residual(system, 600, 1e-4, x0, omega=omega, pixel_gs=gs)
+    p = R.make_params(96, 72, omega=omega, pixel_update=R.PIXEL_GS if gs else R.PIXEL_JACOBI, **SCHEDULES[sched])
+    xb, ib, rb, eb = ctx.relax_residual(system, 600, 1e-4, x0, p)
+    print(f"{sched} gs={gs} omega={omega}: {ib} iterations, residual {rb:.3e}, energy {eb:.9g}")
+    assert ia == ib
+    assert np.array_equal(xa, xb), float(np.abs(xa - xb).max())
+    assert abs(ra - rb) <= 1e-10 * ra and abs(ea - eb) <= 1e-10 * abs(ea)
+
+
+def test_sor_converged_energy_agrees(ctx, R, system):
+    """Red-black point SOR with omega = 1.8 reaches the reference iteration's
+    converged energy (omega = 0.7, old values within the pixel) within 1e-4
+    relative, in far fewer iterations (device residuals and energies).  The
+    f32 increments floor the relative residual near 5e-6 here, so the target
+    is 2e-5."""
+    x0 = np.zeros((3, 72, 96))
+    tiled = dict(relax_variant=R.RELAX_TILED)
+    _, i_ref, r_ref, e_ref = ctx.relax_residual(system, 40000, 2e-5, x0, R.make_params(96, 72, **tiled))
+    _, i_sor, r_sor, e_sor = ctx.relax_residual(system, 40000, 2e-5, x0,
+                                                R.make_params(96, 72, omega=1.8, pixel_update=R.PIXEL_GS, **tiled))
+    print(f"reference order omega 0.7: {i_ref} its, energy {e_ref:.12g}; point SOR 1.8: {i_sor} its, {e_sor:.12g}")
+    assert r_ref <= 2e-5 and r_sor <= 2e-5
+    assert abs(e_sor - e_ref) <= 1e-4 * abs(e_ref)
+    assert i_sor * 3 < i_ref
+
+
+@pytest.mark.parametrize("variant", ["plain", "tiled"])
+def test_point_sor_pair_matches_oracle(ctx, R, variant):
+    """Whole pairs with the point SOR order (omega 1.8): bit-exact with the
+    oracle, and with the FP64 residual exit the same iterations per level."""
+    O = oracle()
+    W, H = 160, 120
+    pair, im1, d1, im2, d2, sigma, hpb = _case(W, H)
+    kw = dict(n_outer=4, n_sor=60, min_width=20, omega=1.8)
+    rv = R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN
+    for resid in (False, True):
+        po = O.make_params(W, H, pixel_gs=True, residual_exit=resid, residual_tol=1e-3, **kw)
+        pg = R.make_params(W, H, pixel_update=R.PIXEL_GS, relax_variant=rv, residual_tol=1e-3,
+                           inner_exit=R.EXIT_RESIDUAL if resid else R.EXIT_REFERENCE, **kw)
+        ua, va, wa, sa = O.multiscale(im1, d1, im2, d2, sigma, hpb, po)
+        ub, vb, wb, sb = ctx.multiscale(pg, im1, d1, im2, d2, sigma, hpb)
+        L = sa.n_levels
+        assert [sb.sweeps_exec[k] for k in range(L)] == [sa.sweeps_exec[k] for k in range(L)]
+        assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
+        if resid:
+            assert [sb.residual_checks[k] for k in range(L)] == [sa.residual_checks[k] for k in range(L)]
+            for k in range(L):
+                assert abs(sb.residual[k] - sa.residual[k]) <= 1e-10 * sa.residual[k]
+                assert abs(sb.energy[k] - sa.energy[k]) <= 1e-10 * abs(sa.energy[k])

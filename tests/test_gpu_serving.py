@@ -87,3 +87,48 @@ def test_serving_budget_and_errors(R):
             srv.open(96, 71, None, 12)                      # camera size differs (params.cam is 96x72)
         for s in sids:
             srv.close_stream(s)
+
+
+@pytest.mark.parametrize("max_batch,lanes", [(8, 2), (1, 1)])
+def test_serving_warm_start_equals_with_init(R, max_batch, lanes):
+    """With warm starts a stream's first pair is solved coarse to fine and
+    every later pair as sceneFlowWithInit from the stream's previous result
+    (here with 2 outer iterations): each result equals rgbdflow_with_init of
+    that pair alone with that estimate, bit for bit.  Batches repeat their
+    composition, so later ones replay a captured HIP graph of the plan."""
+    S = synth()
+    sizes = [(128, 96, 16), (96, 72, 12)]
+    p = R.make_params(128, 96, n_outer=3, n_sor=25, min_width=16, relax_variant=R.RELAX_TILED, graph=True)
+    order = [0, 1, 0, 1, 0, 1]
+    streams = []
+    with R.Server(p, max_batch=max_batch, lanes=lanes, warm_start=True, warm_n_outer=2) as srv:
+        for k, (W, H, mw) in enumerate(sizes):
+            frames, pair = _stream_frames(S, W, H, 400 + k)
+            maps = [_maps(R, S, pair, W, H, mw, sh) for sh in (0, 3)]
+            sid = srv.open(W, H, R.make_params(W, H).cam, mw)
+            streams.append((sid, W, H, mw, frames, maps))
+        tickets = []
+        for t, fi in enumerate(order):
+            for (sid, W, H, mw, frames, maps) in streams:
+                tickets.append((sid, t, srv.submit(sid, frames[fi][0], frames[fi][1], maps[fi])))
+        results = {}
+        for (sid, t, tk) in tickets:
+            print(f"poll stream {sid} frame {t}", flush=True)
+            info, flow = srv.poll(tk, wait_ms=60000)
+            assert info["status"] == 0 and info["has_flow"] == (t > 0)
+            results[(sid, t)] = flow
+    with R.Context(0) as ctx:
+        for (sid, W, H, mw, frames, maps) in streams:
+            u, v, w = _alone(R, ctx, p, W, H, mw, frames[order[0]], frames[order[1]], maps[order[0]])
+            ub, vb, wb = results[(sid, 1)]
+            assert np.array_equal(u, ub) and np.array_equal(v, vb) and np.array_equal(w, wb), (sid, 1)
+            q = R.make_params(W, H, n_outer=2, n_sor=25, min_width=mw, relax_variant=R.RELAX_TILED)
+            for t in range(2, len(order)):
+                a, b = order[t - 1], order[t]
+                est = np.stack(results[(sid, t - 1)], axis=-1)
+                im1 = frames[a][0].astype(np.float64) / 255.0
+                im2 = frames[b][0].astype(np.float64) / 255.0
+                u, v, w, st = ctx.with_init(q, im1, frames[a][1].astype(np.float64), im2,
+                                            frames[b][1].astype(np.float64), est, None, maps[a][: W * H * 2])
+                ub, vb, wb = results[(sid, t)]
+                assert np.array_equal(u, ub) and np.array_equal(v, vb) and np.array_equal(w, wb), (sid, t)

@@ -443,3 +443,76 @@ def test_system_is_the_lagged_energy_gradient():
             grad[i] = (energy(x + e) - energy(x - e)) / 2.0
         lhs = b - M @ x
         assert np.max(np.abs(lhs - grad)) < 1e-9 * max(1.0, np.abs(grad).max()), np.max(np.abs(lhs - grad))
+
+
+# ------------------------------------------------- FP64 residual and energy
+def test_system_residual_matches_sparse_operator():
+    """The residual test's f64 reduction equals ||b - M x|| / ||b|| and the
+    energy b'x - x'Mx/2 with M assembled independently (scipy); a transposed
+    neighbour, a dropped coupling or a sign error changes both."""
+    li, (u, v, w), _ = level_inputs(seed=21, W=18, H=14)
+    coef = O.level_system(li, O.make_params(18, 14), u, v, w)
+    M, b = _sparse_system(coef)
+    x = np.random.default_rng(3).normal(scale=0.05, size=(3, 14, 18)).astype(np.float32).astype(np.float64)
+    rel, en = O.system_residual(coef, x)
+    xv = x.reshape(-1)
+    r = b - M @ xv
+    assert np.isclose(rel, np.linalg.norm(r) / np.linalg.norm(b), rtol=1e-12, atol=0)
+    assert np.isclose(en, b @ xv - 0.5 * xv @ (M @ xv), rtol=1e-12, atol=0)
+    # at the direct solution the residual vanishes; the same x against the
+    # system with its right and down couplings swapped (a planted indexing
+    # mistake) or with the rhs negated (a sign error) does not
+    xs = spla.spsolve(M.tocsc(), b).reshape(3, 14, 18)
+    assert O.system_residual(coef, xs)[0] < 1e-10
+    bad = coef.copy()
+    bad[0], bad[2] = coef[2], coef[0]
+    assert O.system_residual(bad, xs)[0] > 1e-3
+    bad = coef.copy()
+    bad[10:13] *= -1
+    assert O.system_residual(bad, xs)[0] > 1.0
+    # zero increment: residual 1, energy 0
+    assert O.system_residual(coef, np.zeros_like(x)) == (1.0, 0.0)
+
+
+def test_relax_residual_reaches_the_minimiser_for_any_factor():
+    """Relaxing until the relative residual is <= tol reaches the energy
+    minimum of the direct solve.  In the point SOR order (u, then v from the
+    new u, then w) every factor in (0, 2) converges, so omega = 0.7 and
+    omega = 1.8 agree on the converged energy; the test runs only at
+    iterations 25, 50, ... and stops at the first that passes."""
+    li, (u, v, w), _ = level_inputs(seed=22, W=20, H=16)
+    coef = O.level_system(li, O.make_params(20, 16, precision=O.PREC_F64), u, v, w)
+    coef[10:13] *= 1e-2                                  # every step inside the 0.3 clamp
+    M, b = _sparse_system(coef)
+    x_direct = spla.spsolve(M.tocsc(), b)
+    e_min = b @ x_direct - 0.5 * x_direct @ (M @ x_direct)
+    x0 = np.zeros((3, 16, 20))
+    energies = {}
+    for gs, om in ((False, 0.7), (True, 0.7), (True, 1.8)):
+        x, it, res, en = O.relax_residual(coef, 20000, 1e-8, x0, omega=om, precision=O.PREC_F64, pixel_gs=gs)
+        assert it % 25 == 0 and res <= 1e-8
+        _, _, res_prev, _ = O.relax_residual(coef, it - 25, 0.0, x0, omega=om, precision=O.PREC_F64, pixel_gs=gs)
+        assert res_prev > 1e-8                           # the previous check had not passed
+        assert abs(en - e_min) <= 1e-6 * abs(e_min)
+        assert np.max(np.abs(x.reshape(-1) - x_direct)) < 1e-6 * np.abs(x_direct).max()
+        energies[(gs, om)] = (it, en)
+    assert energies[(True, 1.8)][0] < energies[(True, 0.7)][0] / 3   # over-relaxation pays
+    assert abs(energies[(True, 1.8)][1] - energies[(False, 0.7)][1]) <= 1e-4 * abs(e_min)
+    # the reference's order (each unknown from the others' old values) does
+    # not converge with omega = 1.8 on this system: it stalls far away
+    _, _, res_j, _ = O.relax_residual(coef, 2000, 0.0, x0, omega=1.8, precision=O.PREC_F64, pixel_gs=False)
+    assert res_j > 1.0
+
+
+def test_level_solve_residual_exit():
+    """With the residual exit each relaxation stops at the first check whose
+    residual is <= tol; the stats report it.  A tolerance of 0 never stops it."""
+    li, (u, v, w), _ = level_inputs(seed=23, W=32, H=24)
+    kw = dict(n_outer=3, n_sor=200)
+    _, _, _, st = O.level_solve(li, O.make_params(32, 24, residual_exit=True, residual_tol=1e-3, omega=1.8,
+                                                  pixel_gs=True, **kw), False, u, v, w)
+    assert st.residual_checks >= 3 and 0 <= st.last_residual <= 1e-3
+    assert st.sweeps_exec % 25 == 0 and st.sweeps_exec < 3 * 200
+    _, _, _, st0 = O.level_solve(li, O.make_params(32, 24, residual_exit=True, residual_tol=0.0, **kw), False,
+                                 u, v, w)
+    assert st0.sweeps_exec == 3 * 200 and st0.residual_checks == 3 * 8 and st0.last_residual > 0
