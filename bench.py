@@ -177,6 +177,11 @@ def main():
     ap.add_argument("--warm", action="store_true",
                     help="serving: solve each stream's later pairs as sceneFlowWithInit from its last flow")
     ap.add_argument("--warm-outer", type=int, default=0, help="serving: outer iterations of warm starts (0: 15)")
+    ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage run of the batch")
+    ap.add_argument("--served-streams", type=int, default=4,
+                    help="640x480 streams of the served measurement beside the batch (0: skip)")
+    ap.add_argument("--served-seconds", type=float, default=3.0)
+    ap.add_argument("--served-max-batch", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: the scene-flow solve runs on one GPU "
@@ -260,6 +265,44 @@ def main():
         single_pps = args.single_steps / timed_runs(ctx, p, args.single_steps)
         single_lat = np.array([ctx.batch_run(p, stats=True)[0].device_ms for _ in range(10)])
 
+    # the same batch with the systems stored in FP16 (f32 arithmetic)
+    f16 = None
+    if args.storage == "f32" and not args.no_f16 and args.variant == "tiled":
+        p16 = params_for(R, args.variant, not args.no_graph, "f16")
+        if not args.no_autotune and not args.load_schedules:
+            for (w, h) in levels:
+                ctx.autotune_relax(w, h, batch=B, params=p16)
+        ctx.batch_load(p16, batch)
+        ctx.batch_run(p16, stats=True)
+        for _ in range(args.warmup):
+            ctx.batch_run(p16, stats=False)
+        el16 = timed_runs(ctx, p16, args.steps)
+        lat16 = np.array([ctx.batch_run(p16, stats=True)[0].device_ms for _ in range(max(5, args.latency_steps // 2))])
+        ctx.set_relax_timing(True)
+        r16 = ctx.batch_run(p16, stats=True)
+        ctx.set_relax_timing(False)
+        sb16, _ = algorithmic_bytes(r16, B_SWEEP_16)
+        a16 = sb16 / (r16[0].relax_ms / 1e3) / 1e9 if r16[0].relax_ms > 0 else 0.0
+        f16 = {"pairs_per_s": round(B * args.steps / el16, 3),
+               "speedup_vs_plain_single": round(B * args.steps / el16 / plain_pps, 2) if plain_pps else None,
+               "batch_p99_ms": round(float(np.percentile(lat16, 99)), 3),
+               "relax_roofline": {"achieved": round(a16, 1), "frac": round(a16 / HBM_PEAK_GBS, 4),
+                                  "bytes_per_px_iter": B_SWEEP_16},
+               "relax_avg_launch_us": round(r16[0].relax_ms * 1e3 / max(r16[0].relax_launches, 1), 3)}
+        print(f"bench: fp16 storage {f16}", file=sys.stderr, flush=True)
+
+    # served: 640x480 streams at 30 fps, FP16 storage, warm starts, continuous batching
+    served = None
+    if args.served_streams > 0:
+        spec = [(W, H)] * args.served_streams
+        frames = serve_frames(R, spec, args.frames)
+        served = serve_run(R, spec, frames, args.fps, args.served_seconds, args.served_max_batch, "f16", True,
+                           args.warm_outer, autotune=not args.no_autotune)
+        served.update(streams=args.served_streams, fps_per_stream=args.fps, max_batch=args.served_max_batch,
+                      storage="f16", warm_start=True, warm_n_outer=args.warm_outer or 15)
+        served["speedup_vs_plain_single"] = round(served["pairs_per_s"] / plain_pps, 2) if plain_pps else None
+        print(f"bench: served {served}", file=sys.stderr, flush=True)
+
     if args.save_schedules:
         with open(args.save_schedules, "w") as f:
             json.dump({str(k): v for k, v in raw.items()}, f)
@@ -302,6 +345,8 @@ def main():
             "tiled_single_pairs_per_s": round(single_pps, 3) if single_pps else None,
             "tiled_single_p99_ms": round(float(np.percentile(single_lat, 99)), 3) if single_lat is not None else None,
             "autotuned": tuned,
+            "fp16_storage": f16,
+            "served_640x480": served,
         },
         "roofline": {
             "bound": "hbm",

@@ -253,8 +253,9 @@ int rgbdflow_level_system(rgbdflow_ctx* ctx, int W, int H, float focal_level, co
                           double gamma, const double* depthMagTermWeights, const double* regularizationWeights,
                           double* coef13);
 /* n_iter red-black iterations (red then black) on a compact system; x: [3][N]
- * planes in/out.  schedule: relax_variant, tile_* and omega of params (NULL:
- * plain, 0.7). */
+ * planes in/out.  schedule: relax_variant, tile_*, omega, pixel_update and
+ * storage (the records re-stored as the level solve stores them) of params
+ * (NULL: plain, 0.7, the reference's pixel order, f32). */
 int rgbdflow_relax(rgbdflow_ctx* ctx, const double* coef13, int W, int H, int n_iter,
                    const rgbdflow_params* schedule, double* x);
 /* Relaxes a compact system from x until the relative residual

@@ -1289,9 +1289,21 @@ int stage_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, const doubl
     r[15] = 1.0f / r[kBww];
     for (int k = 0; k < 3; k++) hx[l * 4 + k] = (float)x[(size_t)k * N + l];
   }
-  TRY(upload(c, wk.coef, rec.data(), rec.size() * 4));
+  const int storage = sched ? sched->storage : RGBDFLOW_STORE_F32;
+  if (storage < RGBDFLOW_STORE_F32 || storage > RGBDFLOW_STORE_BF16)
+    return fail(c, RGBDFLOW_ERR_ARG, "bad system storage");
+  if (storage == RGBDFLOW_STORE_F32) {
+    TRY(upload(c, wk.coef, rec.data(), rec.size() * 4));
+  } else {
+    // the records as the level solve stores them in that storage
+    float4* rec32;
+    TRY(get(c, "d.rec32", (size_t)kRec * N, &rec32));
+    TRY(upload(c, rec32, rec.data(), rec.size() * 4));
+    launch_encode_records(c->stream, rec32, wk.coef, (int)N, storage);
+  }
   TRY(upload(c, wk.du, hx.data(), hx.size() * 4));
   view = make_view(wk, W, H, nullptr, nullptr, wk.u[0], 0);
+  view.storage = storage;
   if (sched) set_omega(view, sched->omega, sched->pixel_update);
   std::vector<LevelView> items{view};
   TRY(one_step(c, "d.desc", items, sched, sp));

@@ -267,6 +267,8 @@ void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, i
                         int slots, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot);
 void launch_copy_du(hipStream_t s, const Views& v, int blocks);
 void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H, int storage);
+// f32 records -> records in `storage` (src and dst distinct)
+void launch_encode_records(hipStream_t s, const float4* src, float4* dst, int N, int storage);
 
 // kernels_pyramid.hip
 void launch_deinterleave(hipStream_t s, const double* src, int N, int C, double* dst);

@@ -321,6 +321,20 @@ __global__ void k_fill_test_system(float4* coef, float4* du, int W, int H, int s
   }
 }
 
+// Re-stores f32 records (kRec float4 per pixel) in another record storage.
+__global__ void k_encode_records(const float4* src, float4* dst, int N, int storage) {
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < N; l += gridDim.x * blockDim.x) {
+    float c[kCoef], y[3];
+    load_record(src, kStoreF32, (size_t)l, c, y);
+    store_record(dst, storage, (size_t)l, c);
+  }
+}
+
+void launch_encode_records(hipStream_t s, const float4* src, float4* dst, int N, int storage) {
+  int nb = (int)std::min<size_t>(((size_t)N + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_encode_records, dim3(nb), dim3(256), 0, s, src, dst, N, storage);
+}
+
 void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H, int storage) {
   int nb = (int)std::min<size_t>(((size_t)W * H + 255) / 256, 4096);
   hipLaunchKernelGGL(k_fill_test_system, dim3(nb), dim3(256), 0, s, coef, du, W, H, storage);

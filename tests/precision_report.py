@@ -13,10 +13,10 @@ values; point SOR: factor 1.8, u then v then w from the newest values).
 
 Reported per variant: batch pairs/s; flow distance to the f32 reference-rule
 result (EPE of u, v in px; |dw| in m); flow distance to the FP64 oracle of
-the same rule (every buffer f64, on the host's cores); iterations per outer
-iteration to bring the relative residual ||r - M du|| / ||r|| to 1e-4 (the
-FP64 residual exit, at most 500).  Beside them: the converged energies of the
-reference rule and point SOR on one 640x480 level system.
+the same rule (every buffer f64, on the host's cores).  Then, on fixed level
+systems of each size, the iterations each variant needs to bring the
+relative residual ||r - M du|| / ||r|| to 1e-4 (the FP64 residual exit, at
+most 20000) and the energy it reaches.
 
 This script is the checker's view of the variants: it uses the oracle, so
 it lives with the tests, not with the product.
@@ -33,7 +33,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-from helpers import ROOT, level_inputs, load_package, oracle  # noqa: E402
+from helpers import level_inputs, load_package, oracle  # noqa: E402
 
 SIZES = [(320, 240), (640, 480), (960, 540), (1280, 720)]
 
@@ -98,19 +98,8 @@ def main():
             ctx.synchronize()
             pps = 3 * len(pairs) / (time.perf_counter() - t0)
             flows[(rname, sname)] = [ctx.batch_read(i) for i in range(len(pairs))]
-            # iterations to residual <= 1e-4 (FP64 residual exit, at most 500 per outer)
-            pr = R.make_params(640, 480, storage=st, inner_exit=R.EXIT_RESIDUAL, residual_tol=1e-4,
-                               **rule, **dict(base, n_sor=500, graph=False))
-            ctx.batch_load(pr, pairs)
-            sts = ctx.batch_run(pr, stats=True)
-            its = [s.sweeps_exec[0] / max(1, s.outer_exec[0]) for s in sts]
-            res = [s.residual[0] for s in sts]
-            results.append({"rule": rname, "storage": sname, "pairs_per_s": round(pps, 2),
-                            "finest_iters_per_outer_to_1e-4": round(float(np.mean(its)), 1),
-                            "finest_last_residual_max": float(np.max(res)),
-                            "finest_pairs_reaching_1e-4": int(sum(r <= 1e-4 for r in res))})
-            print(f"precision: {rname} {sname}: {pps:.1f} pairs/s, {np.mean(its):.1f} iterations/outer to 1e-4",
-                  flush=True)
+            results.append({"rule": rname, "storage": sname, "pairs_per_s": round(pps, 2)})
+            print(f"precision: {rname} {sname}: {pps:.1f} pairs/s", flush=True)
     ref = flows[("reference", "f32")]
     # the FP64 oracle of each rule (reference exits), on the host
     f64 = {}
@@ -129,21 +118,29 @@ def main():
         fl = flows[(r["rule"], r["storage"])]
         r["vs_f32_reference"] = epe_stats(fl, ref)
         r["vs_fp64_oracle"] = epe_stats(fl, f64[r["rule"]])
-    # converged energies on one 640x480 level system (the level's first outer iteration)
-    li, (u, v, w), _ = level_inputs(seed=8, W=640, H=480)
-    coef = O.level_system(li, O.make_params(640, 480), u, v, w)
-    x0 = np.zeros((3, 480, 640))
-    en = {}
-    for rname, rule in rules.items():
-        _, it, rs, e = ctx.relax_residual(coef, 40000, 1e-6, x0, R.make_params(640, 480, relax_variant=R.RELAX_TILED,
-                                                                               **rule))
-        en[rname] = dict(iterations=it, residual=rs, energy=e)
-    en["relative_energy_difference"] = abs(en["point_sor_1.8"]["energy"] - en["reference"]["energy"]) / abs(
-        en["reference"]["energy"])
+    # iterations to a relative residual of 1e-4 on fixed level systems (the
+    # system of a level's first outer iteration at 0.8 x the true flow), per
+    # size, storage and rule; the converged energies beside them
+    conv = []
+    for (w, h) in SIZES:
+        li, (u, v, w_), _ = level_inputs(seed=8, W=w, H=h)
+        coef = O.level_system(li, O.make_params(w, h), u, v, w_)
+        x0 = np.zeros((3, h, w))
+        for rname, rule in rules.items():
+            for sname, st in stores.items():
+                sched = R.make_params(w, h, relax_variant=R.RELAX_TILED, storage=st, **rule)
+                _, it, rs, e = ctx.relax_residual(coef, 20000, 1e-4, x0, sched)
+                conv.append(dict(level=f"{w}x{h}", rule=rname, storage=sname, iterations=it,
+                                 reached=bool(0 <= rs <= 1e-4), residual=rs, energy=e))
+        print(f"precision: convergence on the {w}x{h} system done", flush=True)
+    for c in conv:
+        best = [d["energy"] for d in conv if d["level"] == c["level"] and d["rule"] == "point_sor_1.8" and
+                d["storage"] == "f32"][0]
+        c["energy_rel_to_sor_f32"] = abs(c["energy"] - best) / abs(best)
     ctx.close()
     report = dict(workload=f"{len(pairs)} mixed-size pairs (" + ", ".join(f"{p['W']}x{p['H']}" for p in pairs) +
                   "), 5-level pyramids ratio 0.5, 15 outer x 50 inner, HPB regularisation",
-                  oracle_threads=nt, variants=results, converged_energy_640x480_level=en)
+                  oracle_threads=nt, variants=results, convergence_to_residual_1e_4=conv)
     txt = json.dumps(report, indent=1)
     print(txt)
     if args.out:

@@ -143,3 +143,21 @@ def test_point_sor_pair_matches_oracle(ctx, R, variant):
             for k in range(L):
                 assert abs(sb.residual[k] - sa.residual[k]) <= 1e-10 * sa.residual[k]
                 assert abs(sb.energy[k] - sa.energy[k]) <= 1e-10 * abs(sa.energy[k])
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_relax_storage_schedules_agree(ctx, R, system, storage):
+    """The relaxation stage entry re-stores the records as the level solve
+    does: both schedules give the same bits on the rounded system, and the
+    result stays near the f32 one."""
+    st = R.STORE_F16 if storage == "f16" else R.STORE_BF16
+    x0 = np.zeros((3, 72, 96))
+    a = ctx.relax(system, 30, x0, R.make_params(96, 72, relax_variant=R.RELAX_PLAIN, storage=st))
+    b = ctx.relax(system, 30, x0, R.make_params(96, 72, relax_variant=R.RELAX_TILED, tile_iters=5, tile_w=24,
+                                                tile_h=16, storage=st))
+    f = ctx.relax(system, 30, x0, R.make_params(96, 72, relax_variant=R.RELAX_PLAIN))
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, f)
+    # bf16 keeps 8 significand bits (relative rounding 2^-9), f16 with its scale 11
+    rel = 1e-3 if storage == "f16" else 1e-2
+    assert np.median(np.abs(a - f)) < rel * np.median(np.abs(f)) + 1e-7
