@@ -178,10 +178,12 @@ def main():
                     help="serving: solve each stream's later pairs as sceneFlowWithInit from its last flow")
     ap.add_argument("--warm-outer", type=int, default=0, help="serving: outer iterations of warm starts (0: 15)")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage run of the batch")
-    ap.add_argument("--served-streams", type=int, default=4,
+    ap.add_argument("--served-streams", type=int, default=8,
                     help="640x480 streams of the served measurement beside the batch (0: skip)")
-    ap.add_argument("--served-seconds", type=float, default=3.0)
+    ap.add_argument("--served-seconds", type=float, default=4.0)
     ap.add_argument("--served-max-batch", type=int, default=4)
+    ap.add_argument("--served-warm-outer", type=int, default=5,
+                    help="outer iterations of the served measurement's warm starts")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: the scene-flow solve runs on one GPU "
@@ -297,9 +299,9 @@ def main():
         spec = [(W, H)] * args.served_streams
         frames = serve_frames(R, spec, args.frames)
         served = serve_run(R, spec, frames, args.fps, args.served_seconds, args.served_max_batch, "f16", True,
-                           args.warm_outer, autotune=not args.no_autotune)
+                           args.served_warm_outer, autotune=not args.no_autotune)
         served.update(streams=args.served_streams, fps_per_stream=args.fps, max_batch=args.served_max_batch,
-                      storage="f16", warm_start=True, warm_n_outer=args.warm_outer or 15)
+                      storage="f16", warm_start=True, warm_n_outer=args.served_warm_outer or 15)
         served["speedup_vs_plain_single"] = round(served["pairs_per_s"] / plain_pps, 2) if plain_pps else None
         print(f"bench: served {served}", file=sys.stderr, flush=True)
 
@@ -424,17 +426,20 @@ def serve_frames(R, spec, n_frames):
         f = 525.0 * w / 640
         seq = S.make_sequence(w, h, 500 + w, n_frames, n_segments=6, cam=S.Camera(w, h, f, f, (w - 1) / 2, (h - 1) / 2))
         lv = R.pyramid_levels(w, h, 0.5, w // 16)
-        frames[(w, h)] = [(fr["img"], (fr["depth"].astype(np.float64) * .001).astype(np.float32),
-                           S.segment_boundary_maps(fr["seg"], [a for a, b in lv], [b for a, b in lv]))
-                          for fr in seq["frames"]]
+        frames[(w, h)] = ([(fr["img"], (fr["depth"].astype(np.float64) * .001).astype(np.float32),
+                            S.segment_boundary_maps(fr["seg"], [a for a, b in lv], [b for a, b in lv]))
+                           for fr in seq["frames"]], seq)
     return frames
 
 
-def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_outer, autotune=True, lanes=2, seed=7):
+def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_outer, autotune=True, lanes=2, seed=7,
+              score_every=8):
     """Streams submit frames at Poisson arrival times (rate fps each); stream k
     walks its size's sequence back and forth from its own starting frame.
-    Returns the served-request latencies and the summary."""
+    Returns the summary: throughput, request latencies, and the end-point
+    error against the true flow of every score_every-th served pair."""
     import threading
+    import rgbdflow_amd.synth as S
     sizes = sorted(set(spec), key=lambda t: -t[0])
     st = {"f32": R.STORE_F32, "f16": R.STORE_F16, "bf16": R.STORE_BF16}[storage]
     p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED, graph=True,
@@ -459,11 +464,13 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
     events.sort()
     tickets = []
 
-    def frame_of(k, n):
-        seq = frames[spec[k]]
-        period = 2 * (len(seq) - 1)
+    def index_of(k, n):
+        period = 2 * (len(frames[spec[k]][0]) - 1)
         i = (n + 2 * k) % period
-        return seq[i if i < len(seq) else period - i]
+        return i if i < period // 2 + 1 else period - i
+
+    def frame_of(k, n):
+        return frames[spec[k]][0][index_of(k, n)]
 
     def submitter():
         t0 = time.perf_counter()
@@ -478,19 +485,26 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
     th = threading.Thread(target=submitter)
     th.start()
     th.join()
-    lat, qms, bsz = [], [], []
+    lat, qms, bsz, scored = [], [], [], []
     for (k, n, tk) in tickets:
-        info, _ = srv.poll(tk, wait_ms=-1)
+        info, flow = srv.poll(tk, wait_ms=-1)
         if info["has_flow"]:
             lat.append(info["latency_ms"])
             qms.append(info["queue_ms"])
             bsz.append(info["batch_pairs"])
+            if len(lat) % score_every == 0:
+                scored.append((k, n, flow))
     wall = time.perf_counter() - t0
     srv.close()
+    epe = []
+    for (k, n, flow) in scored:   # the pair is (frame n-1, frame n) of stream k
+        gt = S.sequence_flow(frames[spec[k]][1], index_of(k, n - 1), index_of(k, n))
+        epe.append(float(np.median(np.hypot(flow[0] - gt[..., 0], flow[1] - gt[..., 1]))))
     return dict(pairs_per_s=round(len(lat) / wall, 3), offered_pairs_per_s=round(len(lat) / seconds, 2),
                 p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
                 queue_p50_ms=round(float(np.percentile(qms, 50)), 2), mean_batch_pairs=round(float(np.mean(bsz)), 2),
-                requests=len(lat), wall_s=round(wall, 2))
+                requests=len(lat), wall_s=round(wall, 2),
+                epe_vs_truth_median_px=round(float(np.median(epe)), 4) if epe else None, scored_pairs=len(epe))
 
 
 def serve_main(R, args):

@@ -232,8 +232,33 @@ def make_sequence(width: int, height: int, seed: int, n_frames: int, n_segments:
                 img[m] = _bilinear(textures[i], Xl[:, 0] / Xl[:, 2] * cam.fx + cam.cx,
                                    Xl[:, 1] / Xl[:, 2] * cam.fy + cam.cy)
         frames.append(dict(img=np.clip(np.round(img), 0, 255).astype(np.uint8), depth=_quantise_depth(zt),
-                           seg=seg.astype(np.int32)))
-    return dict(frames=frames, cam=cam)
+                           seg=seg.astype(np.int32), z=zt))
+    return dict(frames=frames, cam=cam, planes=planes, velocity=velocity)
+
+
+def sequence_flow(seq: dict, i: int, j: int) -> np.ndarray:
+    """Ground-truth scene flow (u, v px; w m) of frame i's pixels to frame j of
+    a make_sequence sequence: each pixel's point on its segment, moved from
+    the segment's pose at frame i to its pose at frame j."""
+    cam = seq["cam"]
+    fr = seq["frames"][i]
+    H, W = fr["seg"].shape
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    rays = np.stack([(xs - cam.cx) / cam.fx, (ys - cam.cy) / cam.fy, np.ones_like(xs)], -1)
+    X = rays * fr["z"][..., None]
+    flow = np.zeros((H, W, 3))
+    for s_, (p, (axis, angle, tr)) in enumerate(zip(seq["planes"], seq["velocity"])):
+        m = fr["seg"] == s_
+        if not m.any():
+            continue
+        Ri, ti = _rotation(axis, i * angle), i * tr
+        Rj, tj = _rotation(axis, j * angle), j * tr
+        X0 = (X[m] - p.c - ti) @ Ri + p.c            # the point at pose 0: R_i^T (X - c - t_i) + c
+        Xj = (X0 - p.c) @ Rj.T + p.c + tj
+        flow[m, 0] = Xj[:, 0] / Xj[:, 2] * cam.fx + cam.cx - xs[m]
+        flow[m, 1] = Xj[:, 1] / Xj[:, 2] * cam.fy + cam.cy - ys[m]
+        flow[m, 2] = Xj[:, 2] - X[m][:, 2]
+    return flow
 
 
 def to_solver_inputs(pair: dict):

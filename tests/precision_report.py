@@ -13,7 +13,8 @@ values; point SOR: factor 1.8, u then v then w from the newest values).
 
 Reported per variant: batch pairs/s; flow distance to the f32 reference-rule
 result (EPE of u, v in px; |dw| in m); flow distance to the FP64 oracle of
-the same rule (every buffer f64, on the host's cores).  Then, on fixed level
+the same rule (every buffer f64, on the host's cores); distance to the
+generator's true flow.  Then, on fixed level
 systems of each size, the iterations each variant needs to bring the
 relative residual ||r - M du|| / ||r|| to 1e-4 (the FP64 residual exit, at
 most 20000) and the energy it reaches.
@@ -53,7 +54,7 @@ def workload(R, n, seed=300):
         hpb = S.segment_boundary_maps(pr["seg1"], [a for a, b in lv], [b for a, b in lv])
         pairs.append(dict(im1=im1, dm1=d1.astype(np.float64), im2=im2, dm2=d2.astype(np.float64), hpb=hpb,
                           sigma1=O.depth_stdevs(d1, 4), cam=R.make_params(w, h, f=f).cam, min_width=w // 16,
-                          W=w, H=h, f=f))
+                          W=w, H=h, f=f, truth=pr["flow"].astype(np.float64)))
     return pairs
 
 
@@ -118,6 +119,7 @@ def main():
         fl = flows[(r["rule"], r["storage"])]
         r["vs_f32_reference"] = epe_stats(fl, ref)
         r["vs_fp64_oracle"] = epe_stats(fl, f64[r["rule"]])
+        r["vs_truth"] = epe_stats(fl, [(p["truth"][..., 0], p["truth"][..., 1], p["truth"][..., 2]) for p in pairs])
     # iterations to a relative residual of 1e-4 on fixed level systems (the
     # system of a level's first outer iteration at 0.8 x the true flow), per
     # size, storage and rule; the converged energies beside them
