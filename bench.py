@@ -178,6 +178,7 @@ def main():
                     help="serving: solve each stream's later pairs as sceneFlowWithInit from its last flow")
     ap.add_argument("--warm-outer", type=int, default=0, help="serving: outer iterations of warm starts (0: 15)")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage run of the batch")
+    ap.add_argument("--lanes2", action="store_true", help="also time the batch as two halves on two streams")
     ap.add_argument("--served-streams", type=int, default=8,
                     help="640x480 streams of the served measurement beside the batch (0: skip)")
     ap.add_argument("--served-seconds", type=float, default=4.0)
@@ -267,6 +268,37 @@ def main():
         single_pps = args.single_steps / timed_runs(ctx, p, args.single_steps)
         single_lat = np.array([ctx.batch_run(p, stats=True)[0].device_ms for _ in range(10)])
 
+    # the same pairs as two half batches on two contexts (two HIP streams), so
+    # one half's coarse, latency-bound levels overlap the other's fine levels
+    two_lanes = None
+    if args.lanes2 and args.variant == "tiled" and B >= 2:
+        half = B // 2
+        ctxs = [R.Context(0), R.Context(0)]
+        for (w, h) in levels:
+            ch = ctxs[0].autotune_relax(w, h, batch=half, params=p)
+            ctxs[1].set_relax_schedule(w, h, ch)
+        ctxs[0].batch_load(p, batch[:half])
+        ctxs[1].batch_load(p, batch[half:])
+        for c in ctxs:
+            c.batch_run(p, stats=True)
+        for _ in range(max(1, args.warmup)):
+            for c in ctxs:
+                c.batch_run(p, stats=False)
+        for c in ctxs:
+            c.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for c in ctxs:
+                c.batch_run(p, stats=False)
+        for c in ctxs:
+            c.synchronize()
+        el2 = time.perf_counter() - t0
+        two_lanes = {"pairs_per_s": round(B * args.steps / el2, 3), "ms_per_step": round(el2 / args.steps * 1e3, 3),
+                     "lanes": 2, "pairs_per_lane": half}
+        for c in ctxs:
+            c.close()
+        print(f"bench: two lanes {two_lanes}", file=sys.stderr, flush=True)
+
     # the same batch with the systems stored in FP16 (f32 arithmetic)
     f16 = None
     if args.storage == "f32" and not args.no_f16 and args.variant == "tiled":
@@ -347,6 +379,7 @@ def main():
             "tiled_single_pairs_per_s": round(single_pps, 3) if single_pps else None,
             "tiled_single_p99_ms": round(float(np.percentile(single_lat, 99)), 3) if single_lat is not None else None,
             "autotuned": tuned,
+            "two_lanes": two_lanes,
             "fp16_storage": f16,
             "served_640x480": served,
         },
@@ -433,11 +466,14 @@ def serve_frames(R, spec, n_frames):
 
 
 def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_outer, autotune=True, lanes=2, seed=7,
-              score_every=8):
+              score_every=8, warmup=1.0):
     """Streams submit frames at Poisson arrival times (rate fps each); stream k
     walks its size's sequence back and forth from its own starting frame.
-    Returns the summary: throughput, request latencies, and the end-point
-    error against the true flow of every score_every-th served pair."""
+    Arrivals run for warmup + seconds; requests that arrive in the first
+    `warmup` seconds (while the server's buffers and plans are first made)
+    are served but not counted.  Returns the summary of the counted window:
+    throughput, request latencies, and the end-point error against the true
+    flow of every score_every-th served pair."""
     import threading
     import rgbdflow_amd.synth as S
     sizes = sorted(set(spec), key=lambda t: -t[0])
@@ -457,7 +493,7 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
     for k, (w, h) in enumerate(spec):
         t = rng.exponential(1 / fps)
         n = 0
-        while t < seconds:
+        while t < warmup + seconds:
             events.append((t, k, n))
             t += rng.exponential(1 / fps)
             n += 1
@@ -479,28 +515,38 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
             if dt > 0:
                 time.sleep(dt)
             img, dep, hpb = frame_of(k, n)
-            tickets.append((k, n, srv.submit(sids[k], img, dep, hpb)))
+            tickets.append((k, n, t >= warmup, srv.submit(sids[k], img, dep, hpb)))
 
     t0 = time.perf_counter()
     th = threading.Thread(target=submitter)
     th.start()
-    th.join()
+    # results are collected while frames still arrive, in submission order
     lat, qms, bsz, scored = [], [], [], []
-    for (k, n, tk) in tickets:
+    polled = 0
+    t_last = t0 + warmup + 1e-9
+    while polled < len(tickets) or th.is_alive():
+        if polled == len(tickets):
+            time.sleep(0.0005)
+            continue
+        k, n, counted, tk = tickets[polled]
+        polled += 1
         info, flow = srv.poll(tk, wait_ms=-1)
-        if info["has_flow"]:
+        if info["has_flow"] and counted:
+            t_last = time.perf_counter()
             lat.append(info["latency_ms"])
             qms.append(info["queue_ms"])
             bsz.append(info["batch_pairs"])
             if len(lat) % score_every == 0:
                 scored.append((k, n, flow))
-    wall = time.perf_counter() - t0
+    th.join()
+    wall = t_last - (t0 + warmup)
     srv.close()
     epe = []
     for (k, n, flow) in scored:   # the pair is (frame n-1, frame n) of stream k
         gt = S.sequence_flow(frames[spec[k]][1], index_of(k, n - 1), index_of(k, n))
         epe.append(float(np.median(np.hypot(flow[0] - gt[..., 0], flow[1] - gt[..., 1]))))
     return dict(pairs_per_s=round(len(lat) / wall, 3), offered_pairs_per_s=round(len(lat) / seconds, 2),
+                warmup_s=warmup,
                 p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
                 queue_p50_ms=round(float(np.percentile(qms, 50)), 2), mean_batch_pairs=round(float(np.mean(bsz)), 2),
                 requests=len(lat), wall_s=round(wall, 2),

@@ -8,7 +8,11 @@
 // requests -- at most one pair per stream in flight, at most max_batch pairs,
 // admitted while the in-flight pixels fit the HBM budget -- and runs them on
 // `lanes` executor contexts with their own HIP streams, so a batch formed
-// while another is on the GPU runs beside it.  Results do not depend on how
+// while another is on the GPU runs beside it.  Host copies happen in the
+// client's threads, never in the worker's: stream_submit copies the frame to
+// a device staging buffer, and a finished pair's flow is kept in a device
+// result buffer until stream_poll copies it to the caller's arrays, so the
+// worker only launches work and device-to-device copies.  Results do not depend on how
 // requests are batched: every pair's solve is the same whatever else is in
 // its batch.
 #include <hip/hip_runtime.h>
@@ -59,18 +63,23 @@ struct StreamState {
   bool busy = false;     // a request of this stream is in a batch
   int queued = 0;        // requests of this stream waiting
   bool closing = false;
-  unsigned char* stage_bgr = nullptr;
-  float* stage_depth = nullptr;
   void* mem = nullptr;
   size_t bytes = 0;
+};
+
+// Device staging of one submitted frame: u8 BGR [N][3], then f32 depth [N],
+// then (HPB) the boundary maps, each 256-byte aligned.
+inline size_t r256(size_t n) { return (n + 255) & ~size_t(255); }
+struct Staged {
+  unsigned char* p = nullptr;
+  size_t bytes = 0, depth_off = 0, hpb_off = 0, hpb_n = 0;
 };
 
 struct Request {
   unsigned long long ticket = 0;
   int stream = -1;
   StreamState* st = nullptr;  // set when picked (the stream stays open while busy)
-  std::vector<unsigned char> bgr;
-  std::vector<float> depth, hpb;
+  Staged in;
   Clock::time_point t_submit;
   int next = 0;  // frame slot the new frame goes to
   bool warm = false;  // solved from the stream's last flow
@@ -78,7 +87,8 @@ struct Request {
 
 struct Result {
   bool done = false;
-  std::vector<double> u, v, w;
+  double* flow = nullptr;  // [3][N] device result buffer (has_flow), freed by poll
+  size_t bytes = 0;
   rgbdflow_request_info info{};
 };
 
@@ -116,6 +126,8 @@ struct rgbdflow_server {
   std::thread worker;
   bool stop = false;
   std::string err;
+  hipStream_t copy_stream = nullptr;       // host<->device copies of submit and poll
+  std::multimap<size_t, void*> free_bufs;  // device staging and result buffers by size (lock m)
 };
 
 namespace {
@@ -123,6 +135,35 @@ namespace {
 int sfail(rgbdflow_server* s, int code, const std::string& msg) {
   if (s) s->err = msg;
   return code;
+}
+
+// A device buffer of exactly `bytes` from the server's free list, else a new
+// one (lock not held).
+void* buf_get(rgbdflow_server* S, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(S->m);
+    auto it = S->free_bufs.find(bytes);
+    if (it != S->free_bufs.end()) {
+      void* p = it->second;
+      S->free_bufs.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipSetDevice(S->device);
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  return p;
+}
+
+void buf_put(rgbdflow_server* S, void* p, size_t bytes) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(S->m);
+  S->free_bufs.emplace(bytes, p);
+}
+
+// buf_put with the lock already held
+void buf_put_locked(rgbdflow_server* S, void* p, size_t bytes) {
+  if (p) S->free_bufs.emplace(bytes, p);
 }
 
 size_t budget_px(const rgbdflow_server* s) {
@@ -143,10 +184,8 @@ void dispatch(rgbdflow_server* S, Lane& ln) {
     Request& q = ln.reqs[r];
     StreamState& st = *q.st;
     Frame& F = st.f[q.next];
-    hipMemcpyAsync(st.stage_bgr, q.bgr.data(), q.bgr.size(), hipMemcpyHostToDevice, hs);
-    hipMemcpyAsync(st.stage_depth, q.depth.data(), q.depth.size() * 4, hipMemcpyHostToDevice, hs);
-    launch_ingest(hs, st.stage_bgr, st.stage_depth, (int)st.N, F.im, F.dm);
-    if (!q.hpb.empty()) hipMemcpyAsync(F.hpb, q.hpb.data(), q.hpb.size() * 4, hipMemcpyHostToDevice, hs);
+    launch_ingest(hs, q.in.p, reinterpret_cast<const float*>(q.in.p + q.in.depth_off), (int)st.N, F.im, F.dm);
+    if (q.in.hpb_n) hipMemcpyAsync(F.hpb, q.in.p + q.in.hpb_off, q.in.hpb_n * 4, hipMemcpyDeviceToDevice, hs);
     F.pyr = false;
     if (st.cur >= 0) {
       const Frame& P = st.f[st.cur];
@@ -155,7 +194,7 @@ void dispatch(rgbdflow_server* S, Lane& ln) {
       pd.H = st.H;
       pd.min_width = st.min_width;
       pd.cam = st.cam;
-      pd.f1 = FrameDev{P.im, P.dm, P.pyr ? P.pim : nullptr, P.pyr ? P.pd : nullptr, q.hpb.empty() ? nullptr : P.hpb};
+      pd.f1 = FrameDev{P.im, P.dm, P.pyr ? P.pim : nullptr, P.pyr ? P.pd : nullptr, q.in.hpb_n ? P.hpb : nullptr};
       pd.f2 = FrameDev{F.im, F.dm, nullptr, nullptr, nullptr};
       pd.init = q.warm ? st.flow : nullptr;
       ln.pair_of[r] = (int)pairs.size();
@@ -200,12 +239,13 @@ void finish(rgbdflow_server* S, Lane& ln) {
     }
     F.pyr = !q.warm;  // a warm start builds no pyramids
     if (S->warm) hipMemcpyAsync(st.flow, flow, 3 * st.N * 8, hipMemcpyDeviceToDevice, hs);
-    res.u.resize(st.N);
-    res.v.resize(st.N);
-    res.w.resize(st.N);
-    hipMemcpyAsync(res.u.data(), flow, st.N * 8, hipMemcpyDeviceToHost, hs);
-    hipMemcpyAsync(res.v.data(), flow + st.N, st.N * 8, hipMemcpyDeviceToHost, hs);
-    hipMemcpyAsync(res.w.data(), flow + 2 * st.N, st.N * 8, hipMemcpyDeviceToHost, hs);
+    res.bytes = 3 * st.N * 8;
+    res.flow = static_cast<double*>(buf_get(S, res.bytes));
+    if (!res.flow) {
+      res.info.status = RGBDFLOW_ERR_HIP;
+      continue;
+    }
+    hipMemcpyAsync(res.flow, flow, res.bytes, hipMemcpyDeviceToDevice, hs);
     res.info.has_flow = 1;
   }
   const hipError_t e = hipStreamSynchronize(hs);
@@ -220,10 +260,10 @@ void finish(rgbdflow_server* S, Lane& ln) {
     st.busy = false;
     if (S->warm && out[r].info.has_flow && e == hipSuccess) st.has_flow = true;
     Result& res = S->results[q.ticket];
-    res.u.swap(out[r].u);
-    res.v.swap(out[r].v);
-    res.w.swap(out[r].w);
+    res.flow = out[r].flow;
+    res.bytes = out[r].bytes;
     res.info = out[r].info;
+    buf_put_locked(S, q.in.p, q.in.bytes);  // the batch has read the staged frame
     if (e != hipSuccess) res.info.status = RGBDFLOW_ERR_HIP;
     res.info.batch_pairs = n_pairs;
     res.info.queue_ms = ms_between(q.t_submit, ln.t_start);
@@ -330,15 +370,21 @@ int rgbdflow_server_create(rgbdflow_server** out, const rgbdflow_server_config* 
   if (cfg->warm_n_outer > 0) S->warm_params.n_outer = cfg->warm_n_outer;
   const int n_lanes = cfg->lanes > 0 ? cfg->lanes : 2;
   S->lanes.resize(n_lanes);
-  for (Lane& ln : S->lanes) {
-    int rc = rgbdflow_create(&ln.ctx, cfg->device, 0);
-    if (rc != RGBDFLOW_OK) {
-      for (Lane& l2 : S->lanes)
-        if (l2.ctx) rgbdflow_destroy(l2.ctx);
-      return rc;
+  auto undo = [&](int rc) {
+    for (Lane& ln : S->lanes) {
+      if (ln.done) hipEventDestroy(ln.done);
+      if (ln.ctx) rgbdflow_destroy(ln.ctx);
     }
-    if (hipEventCreateWithFlags(&ln.done, hipEventDisableTiming) != hipSuccess) return RGBDFLOW_ERR_HIP;
+    if (S->copy_stream) hipStreamDestroy(S->copy_stream);
+    return rc;
+  };
+  for (Lane& ln : S->lanes) {
+    const int rc = rgbdflow_create(&ln.ctx, cfg->device, 0);
+    if (rc != RGBDFLOW_OK) return undo(rc);
+    if (hipEventCreateWithFlags(&ln.done, hipEventDisableTiming) != hipSuccess) return undo(RGBDFLOW_ERR_HIP);
   }
+  hipSetDevice(cfg->device);
+  if (hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking) != hipSuccess) return undo(RGBDFLOW_ERR_HIP);
   rgbdflow_server* raw = S.release();
   raw->worker = std::thread(worker_main, raw);
   *out = raw;
@@ -359,6 +405,12 @@ int rgbdflow_server_destroy(rgbdflow_server* S) {
     if (ln.ctx) rgbdflow_destroy(ln.ctx);
   }
   for (auto& kv : S->streams) free_stream(*kv.second);
+  for (auto& kv : S->results)
+    if (kv.second.flow) hipFree(kv.second.flow);
+  for (const Request& q : S->pending)
+    if (q.in.p) hipFree(q.in.p);
+  for (auto& kv : S->free_bufs) hipFree(kv.second);
+  if (S->copy_stream) hipStreamDestroy(S->copy_stream);
   delete S;
   return RGBDFLOW_OK;
 }
@@ -379,11 +431,10 @@ int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera
   st->N = (size_t)W * H;
   st->rest = pyramid_rest_elems(W, H, S->params.pyramid_ratio, st->min_width);
   st->hpb_n = pyramid_all_elems(W, H, S->params.pyramid_ratio, st->min_width) * 2;
-  // one allocation: two frames (planes, pyramids, boundary maps) and the upload staging
-  auto r256 = [](size_t n) { return (n + 255) & ~size_t(255); };
+  // one allocation: two frames (planes, pyramids, boundary maps) and the last flow
   const size_t frame_bytes = r256(3 * st->N * 8) + r256(st->N * 8) + r256(3 * st->rest * 8 + 8) +
                              r256(st->rest * 8 + 8) + r256(st->hpb_n * 4);
-  const size_t bytes = 2 * frame_bytes + r256(3 * st->N) + r256(st->N * 4) + (S->warm ? r256(3 * st->N * 8) : 0);
+  const size_t bytes = 2 * frame_bytes + (S->warm ? r256(3 * st->N * 8) : 0);
   {
     std::lock_guard<std::mutex> lk(S->m);
     if (S->budget && S->state_bytes + bytes > S->budget)
@@ -405,8 +456,6 @@ int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera
     F.pd = reinterpret_cast<double*>(take(st->rest * 8 + 8));
     F.hpb = reinterpret_cast<float*>(take(st->hpb_n * 4));
   }
-  st->stage_bgr = reinterpret_cast<unsigned char*>(take(3 * st->N));
-  st->stage_depth = reinterpret_cast<float*>(take(st->N * 4));
   if (S->warm) st->flow = reinterpret_cast<double*>(take(3 * st->N * 8));
   std::lock_guard<std::mutex> lk(S->m);
   S->state_bytes += st->bytes;
@@ -430,9 +479,23 @@ int rgbdflow_stream_submit(rgbdflow_server* S, int stream_id, const unsigned cha
     N = it->second->N;
     hn = it->second->hpb_n;
   }
-  q.bgr.assign(bgr, bgr + 3 * N);
-  q.depth.assign(depth, depth + N);
-  if (hpb) q.hpb.assign(hpb, hpb + hn);
+  // the frame goes to a device staging buffer here, in the caller's thread
+  Staged& in = q.in;
+  in.depth_off = r256(3 * N);
+  in.hpb_off = in.depth_off + r256(N * 4);
+  in.hpb_n = hpb ? hn : 0;
+  in.bytes = in.hpb_off + r256(in.hpb_n * 4);
+  in.p = static_cast<unsigned char*>(buf_get(S, in.bytes));
+  if (!in.p) return sfail(S, RGBDFLOW_ERR_HIP, "hipMalloc of a frame staging buffer");
+  hipSetDevice(S->device);
+  hipError_t e = hipMemcpyAsync(in.p, bgr, 3 * N, hipMemcpyHostToDevice, S->copy_stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(in.p + in.depth_off, depth, N * 4, hipMemcpyHostToDevice, S->copy_stream);
+  if (e == hipSuccess && hpb) e = hipMemcpyAsync(in.p + in.hpb_off, hpb, hn * 4, hipMemcpyHostToDevice, S->copy_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(S->copy_stream);
+  if (e != hipSuccess) {
+    buf_put(S, in.p, in.bytes);
+    return sfail(S, RGBDFLOW_ERR_HIP, std::string("frame upload: ") + hipGetErrorString(e));
+  }
   q.stream = stream_id;
   q.t_submit = Clock::now();
   {
@@ -460,16 +523,27 @@ int rgbdflow_stream_poll(rgbdflow_server* S, unsigned long long ticket, int wait
     it = S->results.find(ticket);
   }
   if (!it->second.done) return 0;
-  Result& r = it->second;
-  if (info) *info = r.info;
-  if (r.info.has_flow) {
-    if (u) std::memcpy(u, r.u.data(), r.u.size() * 8);
-    if (v) std::memcpy(v, r.v.data(), r.v.size() * 8);
-    if (w) std::memcpy(w, r.w.data(), r.w.size() * 8);
-  }
-  const int status = r.info.status;
+  const Result r = it->second;
   S->results.erase(it);
-  if (status != RGBDFLOW_OK) return sfail(S, status, "the batch of this request failed");
+  lk.unlock();
+  if (info) *info = r.info;
+  int status = r.info.status;
+  if (r.flow && status == RGBDFLOW_OK && r.info.has_flow) {
+    // the result leaves the device here, in the caller's thread
+    const size_t n = r.bytes / 24;
+    hipSetDevice(S->device);
+    hipError_t e = hipSuccess;
+    double* out[3] = {u, v, w};
+    for (int c = 0; c < 3 && e == hipSuccess; c++)
+      if (out[c]) e = hipMemcpyAsync(out[c], r.flow + c * n, n * 8, hipMemcpyDeviceToHost, S->copy_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(S->copy_stream);
+    if (e != hipSuccess) status = RGBDFLOW_ERR_HIP;
+  }
+  buf_put(S, r.flow, r.bytes);
+  if (status != RGBDFLOW_OK) {
+    std::lock_guard<std::mutex> lk2(S->m);
+    return sfail(S, status, "the batch of this request failed");
+  }
   return 1;
 }
 
