@@ -178,6 +178,8 @@ def main():
                     help="serving: solve each stream's later pairs as sceneFlowWithInit from its last flow")
     ap.add_argument("--warm-outer", type=int, default=0, help="serving: outer iterations of warm starts (0: 15)")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage run of the batch")
+    ap.add_argument("--device-hpb", action="store_true",
+                    help="serving: submit no boundary maps; the server makes them on the device")
     ap.add_argument("--lanes2", action="store_true", help="also time the batch as two halves on two streams")
     ap.add_argument("--served-streams", type=int, default=8,
                     help="640x480 streams of the served measurement beside the batch (0: skip)")
@@ -466,7 +468,7 @@ def serve_frames(R, spec, n_frames):
 
 
 def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_outer, autotune=True, lanes=2, seed=7,
-              score_every=8, warmup=1.0):
+              score_every=8, warmup=1.0, device_hpb=False):
     """Streams submit frames at Poisson arrival times (rate fps each); stream k
     walks its size's sequence back and forth from its own starting frame.
     Arrivals run for warmup + seconds; requests that arrive in the first
@@ -515,7 +517,7 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
             if dt > 0:
                 time.sleep(dt)
             img, dep, hpb = frame_of(k, n)
-            tickets.append((k, n, t >= warmup, srv.submit(sids[k], img, dep, hpb)))
+            tickets.append((k, n, t >= warmup, srv.submit(sids[k], img, dep, None if device_hpb else hpb)))
 
     t0 = time.perf_counter()
     th = threading.Thread(target=submitter)
@@ -549,7 +551,7 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
                 warmup_s=warmup,
                 p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
                 queue_p50_ms=round(float(np.percentile(qms, 50)), 2), mean_batch_pairs=round(float(np.mean(bsz)), 2),
-                requests=len(lat), wall_s=round(wall, 2),
+                requests=len(lat), wall_s=round(wall, 2), boundary_maps="device" if device_hpb else "generator labels",
                 epe_vs_truth_median_px=round(float(np.median(epe)), 4) if epe else None, scored_pairs=len(epe))
 
 
@@ -565,7 +567,7 @@ def serve_main(R, args):
     frames = serve_frames(R, spec, args.frames)
     print(f"bench: serving {len(spec)} streams over {args.seconds} s", file=sys.stderr, flush=True)
     r = serve_run(R, spec, frames, args.fps, args.seconds, args.max_batch, args.storage, args.warm,
-                  args.warm_outer, autotune=not args.no_autotune)
+                  args.warm_outer, autotune=not args.no_autotune, device_hpb=args.device_hpb)
     out = {
         "metric": "served scene-flow pairs/s and request latency",
         "value": r["pairs_per_s"],

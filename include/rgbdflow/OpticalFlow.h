@@ -61,8 +61,10 @@ class OpticalFlow {
 
   // The boundary-probability map of one pyramid level ([H][W][2], right and
   // down neighbours) from that level's colour image (BGR in [0,1]) and raw
-  // depth; the role of computeHeuristicPb (src/heuristicProbBoundary.cpp:21-41),
-  // which this build does not provide: HPB regularisation needs it installed.
+  // depth: the role of computeHeuristicPb (src/heuristicProbBoundary.cpp:21-41).
+  // Without an installed function the device's restatement
+  // (rgbdflow_heuristic_pb) is used; installing one (e.g. the reference's own)
+  // replaces it.
   typedef std::function<std::vector<float>(const DImage& img, const DImage& depth)> HeuristicPbFunction;
   static void setHeuristicPbFunction(HeuristicPbFunction f);
 

@@ -167,7 +167,10 @@ int rgbdflow_depth_stdevs_pointwise(rgbdflow_ctx* ctx, const float* depth, int W
  *              NULL: computed on the device from dm1 by
  *              computeDepthMapStdevsPointwise(maxDist 4) at every run,
  *              hpb: boundary probabilities of every level, [H_k][W_k][2]
- *              concatenated finest first (NULL unless RGBDFLOW_REG_HPB).
+ *              concatenated finest first; NULL with RGBDFLOW_REG_HPB: made on
+ *              the device by rgbdflow_heuristic_pb of each level's colour and
+ *              depth-map pyramid levels (sceneFlowMultiscale.cpp:116; dm1 as
+ *              given stands for the reference's colour-filled raw depth, :78-83).
  *              W, H must equal params->cam.xRes, yRes; the pyramid shape
  *              (pyramid_ratio, min_width) is fixed by the params given here.
  *   pair_run   solves the loaded pair on the device.  stats != NULL waits for
@@ -203,7 +206,7 @@ typedef struct rgbdflow_pair_input {
   const double* im2;
   const double* dm2;
   const float* sigma1; /* [N] depth stdevs of frame 1, or NULL (computed on the device) */
-  const float* hpb;    /* boundary maps of every level, or NULL */
+  const float* hpb;    /* boundary maps of every level, or NULL (HPB: made on the device) */
   rgbdflow_camera cam; /* camera of this pair (xRes == 0: params->cam) */
   int min_width;       /* pyramid min width (<= 0: params->min_width) */
   int reserved;
@@ -218,7 +221,8 @@ int rgbdflow_batch_read(rgbdflow_ctx* ctx, int i, double* u, double* v, double* 
 
 /* OpticalFlow::sceneFlowWithInit (OpticalFlow.h:145, sceneFlowWithInit.cpp:7-81):
  * full resolution only, from the estimate est [N][3], no early termination.
- * hpb0: level-0 map [H][W][2] (NULL unless HPB). */
+ * hpb0: level-0 map [H][W][2] (NULL: made on the device from im1, dm1 as
+ * sceneFlowWithInit.cpp:51 does). */
 int rgbdflow_with_init(rgbdflow_ctx* ctx, int W, int H, const double* im1, const double* dm1, const double* im2,
                        const double* dm2, const float* sigma1, const float* hpb0, const double* est,
                        const rgbdflow_params* params, double* u, double* v, double* w, rgbdflow_stats* stats);
@@ -234,6 +238,12 @@ int rgbdflow_gaussian_pyramid(rgbdflow_ctx* ctx, const double* img, int W, int H
 /* DepthMapPyramid::ConstructPyramid (depthMapPyramid.cpp:15-140); out concatenated. */
 int rgbdflow_depth_map_pyramid(rgbdflow_ctx* ctx, const double* depth, int W, int H, double ratio, int min_width,
                                double* out);
+/* computeHeuristicPb (src/heuristicProbBoundary.cpp:21-41 -> src/heuristicBoundaries.cpp:40-358)
+ * of one image: bgr [N][3] in [0,1], depth [N] metres (<= 0 invalid) -> out
+ * [N][2] f32 boundary probabilities (right, down neighbour), the map
+ * OpticalFlow::calculateRegularizationWeights turns into HPB weights.
+ * Normals use an f64 Jacobi eigen-solver, not PCL's: see the kernels. */
+int rgbdflow_heuristic_pb(rgbdflow_ctx* ctx, const double* bgr, const double* depth, int W, int H, float* out);
 /* OpticalFlow::im2feature RGB_WITH_GRADIENT (OpticalFlow.cpp:148-220): [N][3] -> [N][5] */
 int rgbdflow_im2feature(rgbdflow_ctx* ctx, const double* bgr, int W, int H, double* out5);
 /* warpBy3DFlow (opticalFlowUtils.cpp:76-99): I [N][C], D/u/v/w [N] */
@@ -337,8 +347,9 @@ const char* rgbdflow_server_last_error(const rgbdflow_server* srv);
 int rgbdflow_stream_open(rgbdflow_server* srv, int W, int H, const rgbdflow_camera* cam, int min_width,
                          int* stream_id);
 /* frame: bgr [N][3] u8 (cv2liuBGR: c / 255.0, no channel swap), depth [N] f32 metres
- * (cv2liuDepth, no fill), hpb: the frame's boundary maps of every level (NULL
- * unless RGBDFLOW_REG_HPB).  Copies the arrays; returns a ticket. */
+ * (cv2liuDepth, no fill), hpb: the frame's boundary maps of every level, or
+ * NULL (with RGBDFLOW_REG_HPB they are then made on the device when the frame
+ * is a pair's frame 1).  Copies the arrays; returns a ticket. */
 int rgbdflow_stream_submit(rgbdflow_server* srv, int stream_id, const unsigned char* bgr, const float* depth,
                            const float* hpb, unsigned long long* ticket);
 /* 1: done (u, v, w [N] copied when non-NULL and info->has_flow; the ticket is

@@ -306,6 +306,33 @@ def relax(coef13, n_iter, x, precision=PREC_REF):
     return xx
 
 
+def heuristic_pb(bgr, depth):
+    """computeHeuristicPb of one level: (H, W, 3) BGR in [0,1], (H, W) metres
+    -> (H, W, 2) f32 boundary probabilities (right, down)."""
+    b = _f64(bgr)
+    d = _f64(depth)
+    H, W = d.shape
+    out = np.empty((H, W, 2), np.float32)
+    lib().orc_heuristic_pb(_p(b), _p(d), W, H, _p(out))
+    return out
+
+
+def hpb_normals(depth):
+    d = _f64(depth)
+    H, W = d.shape
+    n = np.empty((H, W, 3), np.float32)
+    v = np.empty((H, W), np.uint32)
+    lib().orc_hpb_normals(_p(d), W, H, _p(n), _p(v))
+    return n, v
+
+
+def hpb_exp(x):
+    f = lib().orc_hpb_exp
+    f.restype = C.c_float
+    f.argtypes = [C.c_float]
+    return np.array([f(float(t)) for t in np.asarray(x, np.float32).ravel()], np.float32)
+
+
 def system_residual(coef13, x):
     """(relative residual ||r - M x|| / ||r||, energy r'x - 1/2 x'Mx) in f64."""
     c = _f64(coef13)

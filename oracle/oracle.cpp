@@ -1097,7 +1097,6 @@ void orc_relax_residual(const double* coef13, int W, int H, int precision, float
 int orc_multiscale(const double* im1, const double* dm1, const double* im2, const double* dm2, const float* sigma1,
                    const float* hpb, int W, int H, const orc_params* p, double* uo, double* vo, double* wo,
                    orc_stats* st) {
-  if (p->regularization_type == ORC_REG_HPB && !hpb) return -1;
   const double ratio = p->pyramid_ratio;
   const size_t N = (size_t)W * H;
   vector<double> d1(dm1, dm1 + N), d2(dm2, dm2 + N);
@@ -1126,6 +1125,20 @@ int orc_multiscale(const double* im1, const double* dm1, const double* im2, cons
   const int L = (int)P1.size();
   vector<size_t> hpbOff(L, 0);
   for (int k = 1; k < L; k++) hpbOff[k] = hpbOff[k - 1] + (size_t)P1[k - 1].W * P1[k - 1].H * 2;
+  // no maps given: computeHeuristicPb of each level's colour and raw-depth
+  // pyramid levels, as sceneFlowMultiscale.cpp:116 computes them (the raw
+  // depth is dm1 as given: its colour-guided fill, :78-83, is the caller's)
+  vector<float> hpbOwn;
+  if (p->regularization_type == ORC_REG_HPB && !hpb) {
+    const vector<Level> DM = depth_map_pyramid(dm1, W, H, ratio, p->min_width);
+    if (DM.size() != P1.size()) return -2;
+    hpbOwn.resize(hpbOff[L - 1] + (size_t)P1[L - 1].W * P1[L - 1].H * 2);
+    for (int k = 0; k < L; k++) {
+      if (DM[k].W != P1[k].W || DM[k].H != P1[k].H) return -2;
+      orc_heuristic_pb(P1[k].d.data(), DM[k].d.data(), P1[k].W, P1[k].H, hpbOwn.data() + hpbOff[k]);
+    }
+    hpb = hpbOwn.data();
+  }
 
   vector<double> u, v, w;
   if (st) { std::memset(st, 0, sizeof(*st)); st->n_levels = L; }
@@ -1179,8 +1192,14 @@ int orc_multiscale(const double* im1, const double* dm1, const double* im2, cons
 int orc_with_init(const double* im1, const double* dm1, const double* im2, const double* dm2, const float* sigma1,
                   const float* hpb0, const double* est, int W, int H, const orc_params* p, double* uo, double* vo,
                   double* wo, orc_stats* st) {
-  if (p->regularization_type == ORC_REG_HPB && !hpb0) return -1;
   const size_t N = (size_t)W * H;
+  // no map given: computeHeuristicPb(Im1, dm1) (sceneFlowWithInit.cpp:51)
+  vector<float> hpbOwn;
+  if (p->regularization_type == ORC_REG_HPB && !hpb0) {
+    hpbOwn.resize(2 * N);
+    orc_heuristic_pb(im1, dm1, W, H, hpbOwn.data());
+    hpb0 = hpbOwn.data();
+  }
   vector<double> u(N), v(N), w(N);
   for (size_t l = 0; l < N; l++) { u[l] = est[l * 3]; v[l] = est[l * 3 + 1]; w[l] = est[l * 3 + 2]; }
   vector<double> d1(dm1, dm1 + N), d2(dm2, dm2 + N);

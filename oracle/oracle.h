@@ -132,15 +132,27 @@ void orc_relax_residual(const double* coef13, int W, int H, int precision, float
 
 /* ---- whole pair: OpticalFlow::sceneFlowMultiscale (src/sceneFlowMultiscale.cpp:30-201) ----
  * im*: [N][3] BGR in [0,1]; dm*: [N] metres; sigma1: [N] depth stdevs of frame 1;
- * hpb: per level [H_k][W_k][2] concatenated (level 0 first), may be NULL unless HPB. */
+ * hpb: per level [H_k][W_k][2] concatenated (level 0 first); NULL with HPB
+ * regularisation computes them (orc_heuristic_pb of each level's colour and
+ * depth-map pyramid levels); returns -2 if those pyramids' sizes differ. */
 int orc_multiscale(const double* im1, const double* dm1, const double* im2, const double* dm2,
                    const float* sigma1, const float* hpb, int W, int H, const orc_params* p,
                    double* u, double* v, double* w, orc_stats* st);
 
-/* OpticalFlow::sceneFlowWithInit (src/sceneFlowWithInit.cpp:7-81); est: [N][3] */
+/* OpticalFlow::sceneFlowWithInit (src/sceneFlowWithInit.cpp:7-81); est: [N][3];
+ * hpb0 NULL with HPB: orc_heuristic_pb(im1, dm1) */
 int orc_with_init(const double* im1, const double* dm1, const double* im2, const double* dm2,
                   const float* sigma1, const float* hpb0, const double* est, int W, int H,
                   const orc_params* p, double* u, double* v, double* w, orc_stats* st);
+
+/* ---- boundary probabilities: computeHeuristicPb (src/heuristicProbBoundary.cpp:21-41,
+ * src/heuristicBoundaries.cpp:40-358), hpb.cpp ----
+ * bgr: [N][3] level colour in [0,1]; depth: [N] metres (<= 0 invalid);
+ * out: [N][2] f32 (right, down). */
+void orc_heuristic_pb(const double* bgr, const double* depth, int W, int H, float* out);
+/* the normals it uses: [N][3] (0 where none) and their validity flags [N] */
+void orc_hpb_normals(const double* depth, int W, int H, float* normals, unsigned* valid);
+float orc_hpb_exp(float x);
 
 /* Threads of the level solve's row partitions (1: the scalar restatement,
  * the default).  Results are identical for every count. */

@@ -31,7 +31,7 @@ EXPORTS = [
     "rgbdflow_solve_level", "rgbdflow_depth_stdevs_pointwise", "rgbdflow_pair_load", "rgbdflow_pair_run",
     "rgbdflow_pair_read", "rgbdflow_multiscale", "rgbdflow_with_init", "rgbdflow_gaussian_pyramid",
     "rgbdflow_depth_map_pyramid", "rgbdflow_im2feature", "rgbdflow_warp_by_3d_flow", "rgbdflow_upsample_flow",
-    "rgbdflow_level_system", "rgbdflow_relax", "rgbdflow_relax_residual", "rgbdflow_autotune_relax", "rgbdflow_batch_load",
+    "rgbdflow_level_system", "rgbdflow_relax", "rgbdflow_relax_residual", "rgbdflow_heuristic_pb", "rgbdflow_autotune_relax", "rgbdflow_batch_load",
     "rgbdflow_batch_run", "rgbdflow_batch_read", "rgbdflow_set_relax_schedule",
     "rgbdflow_time_relax_schedule", "rgbdflow_server_create", "rgbdflow_server_destroy",
     "rgbdflow_server_last_error", "rgbdflow_stream_open", "rgbdflow_stream_submit", "rgbdflow_stream_poll",
@@ -271,6 +271,15 @@ class Context:
         for (w, h) in sizes:
             out.append(flat[off:off + w * h].reshape(h, w))
             off += w * h
+        return out
+
+    def heuristic_pb(self, bgr, depth):
+        """computeHeuristicPb of one image on the device: (H, W, 3) BGR in [0,1],
+        (H, W) metres -> (H, W, 2) f32."""
+        b, d = _f64(bgr), _f64(depth)
+        H, W = d.shape
+        out = np.empty((H, W, 2), np.float32)
+        self._check(lib().rgbdflow_heuristic_pb(self.h, _p(b), _p(d), W, H, _p(out)))
         return out
 
     def im2feature(self, bgr):

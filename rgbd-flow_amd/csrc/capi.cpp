@@ -51,6 +51,7 @@ struct Work {
 struct PairSlot {
   int W = 0, H = 0, min_width = 0;
   bool cam_given = false, sigma_given = false, hpb_given = false;
+  bool hpb_device = false;  // boundary maps computed on the device (HPB and none given)
   bool pyr1_given = false;  // frame 1's colour and depth pyramids were loaded (not built)
   bool init = false;        // solved as sceneFlowWithInit: finest level only, from the estimate in wk.u[1]
   rgbdflow_camera cam{};
@@ -60,6 +61,7 @@ struct PairSlot {
   double *im1 = nullptr, *im2 = nullptr, *dm1 = nullptr, *dm2 = nullptr, *del0 = nullptr, *sm0 = nullptr;
   double *pim1 = nullptr, *pim2 = nullptr, *pd1 = nullptr, *pd2 = nullptr, *pdel = nullptr, *psm = nullptr;
   float *sigma = nullptr, *hpb = nullptr;
+  double* pdm = nullptr;  // depth-map pyramid of dm1, levels >= 1 (hpb_device)
   Work wk{};
 };
 
@@ -91,6 +93,16 @@ struct BatchPlan {
   };
   std::vector<Wave> waves;
   WorkList<PairPrep> prep{};
+  // boundary maps computed on the device, level by level: the depth-map
+  // pyramid level (from the next larger one; level 0 is dm1) and the level's
+  // computeHeuristicPb
+  struct HpbStep {
+    const double* larger;  // null at level 0
+    int LW, LH;
+    double* dm;            // the level's depth-map pyramid image
+    HpbLevel lvl;
+  };
+  std::vector<HpbStep> hpb_steps;
   // the schedule captured as a HIP graph (params.graph)
   int eager_runs = 0;
   hipGraph_t graph = nullptr;
@@ -762,7 +774,22 @@ int alloc_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, int W, int H,
   TRY(get(c, pre + "pdel", rest, &ps.pdel));
   TRY(get(c, pre + "psm", rest, &ps.psm));
   ps.hpb = nullptr;
-  if (hpb) TRY(get(c, pre + "hpb", pyramid_elems(ps.L, 0) * 2, &ps.hpb));
+  ps.pdm = nullptr;
+  ps.hpb_device = !hpb && p->regularization_type == RGBDFLOW_REG_HPB;
+  if (hpb || ps.hpb_device) TRY(get(c, pre + "hpb", pyramid_elems(ps.L, 0) * 2, &ps.hpb));
+  if (ps.hpb_device) {
+    // DepthMapPyramid level sizes are (int)(W r^m) (depthMapPyramid.cpp:15-30); the
+    // boundary maps pair them with the colour pyramid's levels, which must match
+    double r = 0;
+    pyramid_shape(W, H, p->pyramid_ratio, ps.min_width, nullptr, &r);
+    for (size_t m = 0; m < ps.L.size(); m++) {
+      const double lr = std::pow(r, (int)m);
+      if ((int)(W * lr) != ps.L[m].W || (int)(H * lr) != ps.L[m].H)
+        return fail(c, RGBDFLOW_ERR_UNSUPPORTED,
+                    "boundary maps on the device need depth-map and colour pyramid levels of equal sizes");
+    }
+    TRY(get(c, pre + "pdm", pyramid_elems(ps.L, 1) + 1, &ps.pdm));
+  }
   return get_work(c, pre, N, ps.wk);
 }
 
@@ -801,6 +828,7 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
   bp.host.clear();
   bp.steps.clear();
   bp.waves.clear();
+  bp.hpb_steps.clear();
   const int B = (int)c->pairs.size();
   // per-pair preparation
   {
@@ -868,6 +896,26 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
       wv.blocks = start.back();
       wv.wl = WorkList<PyrJob>{};
       bp.waves.push_back(wv);
+    }
+  }
+  // boundary maps of pairs loaded without them (sceneFlowMultiscale.cpp:116)
+  {
+    size_t scratch = 0;
+    for (const PairSlot& ps : c->pairs)
+      if (ps.hpb_device) scratch = std::max(scratch, hpb_scratch_floats(ps.W, ps.H));
+    float* hs = nullptr;
+    if (scratch) TRY(get(c, "hpb.scratch", scratch, &hs));
+    for (PairSlot& ps : c->pairs) {
+      if (!ps.hpb_device) continue;
+      for (int k = 0; k < (int)ps.L.size(); k++) {
+        BatchPlan::HpbStep h;
+        h.larger = k == 0 ? nullptr : level_of(ps, ps.dm1, ps.pdm, 1, k - 1);
+        h.LW = k == 0 ? 0 : ps.L[k - 1].W;
+        h.LH = k == 0 ? 0 : ps.L[k - 1].H;
+        h.dm = level_of(ps, ps.dm1, ps.pdm, 1, k);
+        h.lvl = hpb_level(ps.L[k].W, ps.L[k].H, level_of(ps, ps.im1, ps.pim1, 3, k), h.dm, ps.hpb + ps.hoff[k], hs);
+        bp.hpb_steps.push_back(h);
+      }
     }
   }
   // the level steps (sceneFlowMultiscale.cpp:95-196), aligned on the finest level
@@ -974,6 +1022,11 @@ void enqueue_batch(rgbdflow_ctx* c, const BatchPlan& bp, const SolveScalars& k) 
   // (OpticalFlow.cpp:71-81); s = f^2 (OpticalFlow.cpp:90-97); then the pyramids
   launch_pair_prep(s, bp.prep, bp.prep_blocks);
   for (const BatchPlan::Wave& wv : bp.waves) launch_pyramid_jobs(s, wv.wl, wv.blocks);
+  // boundary maps made on the device, from the colour pyramid just built
+  for (const BatchPlan::HpbStep& h : bp.hpb_steps) {
+    if (h.larger) launch_depth_map_level(s, h.larger, h.LW, h.LH, h.dm, h.lvl.W, h.lvl.H);
+    launch_hpb(s, h.lvl);
+  }
   // coarse to fine
   for (const StepPlan& sp : bp.steps) {
     launch_features(s, sp.vpix, sp.pix);
@@ -986,9 +1039,6 @@ void enqueue_batch(rgbdflow_ctx* c, const BatchPlan& bp, const SolveScalars& k) 
 int batch_load(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const rgbdflow_pair_input* in) {
   if (!c) return RGBDFLOW_ERR_ARG;
   if (!p || n < 1 || !in) return fail(c, RGBDFLOW_ERR_ARG, "bad batch arguments");
-  if (p->regularization_type == RGBDFLOW_REG_HPB)
-    for (int i = 0; i < n; i++)
-      if (!in[i].hpb) return fail(c, RGBDFLOW_ERR_ARG, "HPB regularisation needs the boundary maps (hpb)");
   HIP_TRY(c, hipSetDevice(c->device));
   c->loaded = false;
   c->ran = false;
@@ -1013,8 +1063,9 @@ int batch_run(rgbdflow_ctx* c, const rgbdflow_params* p, rgbdflow_stats* st) {
     const rgbdflow_camera& cam = ps.cam_given ? ps.cam : p->cam;
     if ((int)cam.xRes != ps.W || (int)cam.yRes != ps.H)
       return fail(c, RGBDFLOW_ERR_ARG, "camera resolution differs from the image size of pair " + std::to_string(i));
-    if (p->regularization_type == RGBDFLOW_REG_HPB && !ps.hpb_given)
-      return fail(c, RGBDFLOW_ERR_ARG, "HPB regularisation needs the boundary maps (hpb)");
+    if (p->regularization_type == RGBDFLOW_REG_HPB && !ps.hpb_given && !ps.hpb_device)
+      return fail(c, RGBDFLOW_ERR_ARG, "HPB regularisation needs boundary maps: none loaded, none made (the pair "
+                                       "was loaded under other params)");
   }
   HIP_TRY(c, hipSetDevice(c->device));
   TRY(batch_plan(c, p));
@@ -1456,6 +1507,25 @@ int rgbdflow_depth_map_pyramid(rgbdflow_ctx* c, const double* depth, int W, int 
   return sync(c);
 }
 
+int rgbdflow_heuristic_pb(rgbdflow_ctx* c, const double* bgr, const double* depth, int W, int H, float* out) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (bad_dims(W, H) || !bgr || !depth || !out) return fail(c, RGBDFLOW_ERR_ARG, "bad heuristic_pb arguments");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const size_t N = (size_t)W * H;
+  double *b, *d;
+  float *o, *scratch;
+  TRY(get(c, "h.bgr", 3 * N, &b));
+  TRY(get(c, "h.depth", N, &d));
+  TRY(get(c, "h.out", 2 * N, &o));
+  TRY(get(c, "h.scratch", hpb_scratch_floats(W, H), &scratch));
+  TRY(upload_planes(c, bgr, N, 3, b));
+  TRY(upload(c, d, depth, N * 8));
+  launch_hpb(c->stream, hpb_level(W, H, b, d, o, scratch));
+  HIP_TRY(c, hipGetLastError());
+  TRY(download(c, out, o, 2 * N * 4));
+  return sync(c);
+}
+
 int rgbdflow_im2feature(rgbdflow_ctx* c, const double* bgr, int W, int H, double* out5) {
   if (!c) return RGBDFLOW_ERR_ARG;
   if (bad_dims(W, H) || !bgr || !out5) return fail(c, RGBDFLOW_ERR_ARG, "bad feature arguments");
@@ -1792,8 +1862,6 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   if (!c) return RGBDFLOW_ERR_ARG;
   if (!p || bad_dims(W, H) || !im1 || !dm1 || !im2 || !dm2 || !est || !u || !v || !w)
     return fail(c, RGBDFLOW_ERR_ARG, "bad with_init arguments");
-  if (p->regularization_type == RGBDFLOW_REG_HPB && !hpb0)
-    return fail(c, RGBDFLOW_ERR_ARG, "HPB regularisation needs the boundary map (hpb0)");
   if ((int)p->cam.xRes != W || (int)p->cam.yRes != H)
     return fail(c, RGBDFLOW_ERR_ARG, "params.cam resolution differs from the image size");
   TRY(check_run_params(c, p));
@@ -1810,7 +1878,10 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   TRY(get(c, "wi.sig", N, &s1));
   TRY(get(c, "wi.del0", N, &del0));
   TRY(get(c, "wi.sm0", N, &sm0));
-  if (hpb0) TRY(get(c, "wi.hpb", N * 2, &h));
+  const bool hpb_dev = !hpb0 && p->regularization_type == RGBDFLOW_REG_HPB;
+  if (hpb0 || hpb_dev) TRY(get(c, "wi.hpb", N * 2, &h));
+  float* hs = nullptr;
+  if (hpb_dev) TRY(get(c, "hpb.scratch", hpb_scratch_floats(W, H), &hs));
   TRY(upload_planes(c, im1, N, 3, i1));
   TRY(upload_planes(c, im2, N, 3, i2));
   TRY(upload(c, d1, dm1, N * 8));
@@ -1865,6 +1936,8 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   c->relax_ev_used = 0;
   HIP_TRY(c, hipEventRecord(c->ev0, s));
   launch_pair_prep(s, pwl, pstart[1]);
+  // sceneFlowWithInit.cpp:51: computeHeuristicPb(Im1, dm1) when no map is given
+  if (hpb_dev) launch_hpb(s, hpb_level(W, H, i1, d1, h, hs));
   launch_features(s, sp.vpix, sp.pix);
   launch_level_entry(s, sp.vpix, sp.pix);
   launch_level_weights(s, sp.vpix, sp.pix);

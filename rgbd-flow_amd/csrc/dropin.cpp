@@ -87,20 +87,32 @@ rgbdflow_params to_c(const OpticalFlow::sceneFlowParams& p) {
   return c;
 }
 
-// The boundary maps of every level, from the installed HPB function applied to
-// each level of the colour pyramid and of the raw-depth pyramid, as
-// sceneFlowMultiscale.cpp:78-83,116 feeds computeHeuristicPb.
+// computeHeuristicPb of one image: the installed function, else the device's
+// (rgbdflow_heuristic_pb).
+std::vector<float> heuristic_pb(const DImage& img, const DImage& dep) {
+  if (hpb_function()) return hpb_function()(img, dep);
+  std::vector<float> m((size_t)img.width() * img.height() * 2);
+  check(rgbdflow_heuristic_pb(context(), img.data(), dep.data(), img.width(), img.height(), m.data()),
+        "rgbdflow_heuristic_pb");
+  return m;
+}
+
+// The boundary maps of every level: computeHeuristicPb of each level of the
+// colour pyramid and of the raw-depth pyramid, as sceneFlowMultiscale.cpp:78-83,
+// 116 feeds it.
 std::vector<float> boundary_maps(const DImage& Im1, const DImage& dm1, const OpticalFlow::sceneFlowParams& params) {
-  if (!hpb_function())
-    throw std::runtime_error(
-        "HPB regularisation needs a boundary-probability function: call OpticalFlow::setHeuristicPbFunction "
-        "(computeHeuristicPb is not part of this build) or use ISOTROPIC/LETOUZEY");
   const int W = Im1.width(), H = Im1.height();
   int ws[16], hs[16];
   const int L = rgbdflow_pyramid_levels(W, H, params.pyramidRatio, params.minWidth, ws, hs);
   if (L < 1) throw std::runtime_error("bad pyramid shape");
   size_t tot = 0;
   for (int k = 0; k < L; k++) tot += (size_t)ws[k] * hs[k];
+  // DepthMapPyramid's level sizes are (int)(W r^k) (depthMapPyramid.cpp:15-30);
+  // computeHeuristicPb pairs them with the colour levels, which must match
+  const double r = (params.pyramidRatio > 0.98 || params.pyramidRatio < 0.4) ? 0.75 : params.pyramidRatio;
+  for (int k = 0; k < L; k++)
+    if ((int)(W * std::pow(r, k)) != ws[k] || (int)(H * std::pow(r, k)) != hs[k])
+      throw std::runtime_error("HPB: the depth-map and colour pyramids' level sizes differ at this ratio");
   std::vector<double> colour(tot * 3);
   check(rgbdflow_gaussian_pyramid(context(), Im1.data(), W, H, 3, params.pyramidRatio, params.minWidth,
                                   colour.data()),
@@ -117,7 +129,7 @@ std::vector<float> boundary_maps(const DImage& Im1, const DImage& dm1, const Opt
     DImage img(ws[k], hs[k], 3), dep(ws[k], hs[k], 1);
     std::memcpy(img.data(), colour.data() + off * 3, sizeof(double) * ws[k] * hs[k] * 3);
     std::memcpy(dep.data(), raw.data() + off, sizeof(double) * ws[k] * hs[k]);
-    const std::vector<float> m = hpb_function()(img, dep);
+    const std::vector<float> m = heuristic_pb(img, dep);
     if (m.size() != (size_t)ws[k] * hs[k] * 2) throw std::runtime_error("HPB function returned a map of the wrong size");
     out.insert(out.end(), m.begin(), m.end());
     off += (size_t)ws[k] * hs[k];
@@ -194,10 +206,7 @@ void OpticalFlow::sceneFlowWithInit(const DImage& Im1, const DImage& dm1, const 
   recursiveMedianFilter(depth1);
   recursiveMedianFilter(depth2);
   std::vector<float> hpb0;
-  if (params.regularizationType == sceneFlowRegularizationType::HPB) {
-    if (!hpb_function()) boundary_maps(Im1, dm1, params);  // throws the explanatory error
-    hpb0 = hpb_function()(Im1, dm1);
-  }
+  if (params.regularizationType == sceneFlowRegularizationType::HPB) hpb0 = heuristic_pb(Im1, dm1);
   const std::vector<float> sigma = sigma_of(params, W, H);
   const rgbdflow_params p = to_c(params);
   u.allocate(W, H);

@@ -207,6 +207,27 @@ struct PairPrep {
   double f2;            // focal length squared
 };
 
+// One level of the heuristic boundary probabilities (kernels_hpb.hip):
+// inputs, output and the scratch arrays of its stages.
+struct HpbLevel {
+  int W, H;
+  const double* bgr;    // [3][N] colour planes (B, G, R) in [0, 1]
+  const double* depth;  // [N] metres, <= 0 invalid
+  float* out;           // [N][2] boundary probabilities (right, down)
+  float *pts, *col, *nrm;  // [3][N] points, colour, normals
+  unsigned* nval;          // [N] normal validity
+  int* rad;                // [N] half-disk radius
+  float* rowsum;           // [6][N] running row sums before each pixel
+  unsigned* vrowsum;       // [N]
+  float* ii;               // [6][(W+1)(H+1)] integral images (colour, normals)
+  unsigned* iv;            // [(W+1)(H+1)] integral image of the normal validity
+  float* feat;             // [4][N] colour / normal half-disk features (horizontal, vertical)
+  float* bp;               // [N][2] probabilities before non-maximum suppression
+};
+size_t hpb_scratch_floats(int W, int H);
+HpbLevel hpb_level(int W, int H, const double* bgr, const double* depth, float* out, float* scratch);
+void launch_hpb(hipStream_t s, const HpbLevel& L);
+
 struct SolveScalars {
   double alpha, gamma, color_weight;
   int n_outer, n_inner, n_sor;

@@ -468,8 +468,6 @@ int rgbdflow_stream_submit(rgbdflow_server* S, int stream_id, const unsigned cha
                            const float* hpb, unsigned long long* ticket) {
   if (!S) return RGBDFLOW_ERR_ARG;
   if (!bgr || !depth || !ticket) return sfail(S, RGBDFLOW_ERR_ARG, "null frame");
-  if (S->params.regularization_type == RGBDFLOW_REG_HPB && !hpb)
-    return sfail(S, RGBDFLOW_ERR_ARG, "HPB regularisation needs the frame's boundary maps");
   Request q;
   size_t N, hn;
   {

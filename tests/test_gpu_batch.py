@@ -145,7 +145,11 @@ def test_batch_errors(R):
         assert e.value.code == R.ERR_ARG
         c._batch_shapes.pop()
         with pytest.raises(R.RgbdFlowError) as e:
-            c.batch_load(p, [dict(cases[0], hpb=None)])   # HPB regularisation without maps
+            # maps made on the device for a pair loaded without them, but not for a
+            # pair loaded under other params
+            c.batch_load(R.make_params(96, 72, n_outer=2, n_sor=25, min_width=12, reg=R.REG_ISOTROPIC),
+                         [dict(cases[0], hpb=None)])
+            c.batch_run(p)
         assert e.value.code == R.ERR_ARG
 
 
@@ -223,3 +227,44 @@ def test_batch_with_init_pairs(ctx, R, relax_variant):
         assert du <= 1e-3 and dv <= 1e-3 and dw <= 1e-5, (i, du, dv, dw)
         if "est" not in case:
             assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
+
+
+@pytest.mark.parametrize("relax_variant", [0, 1])
+def test_batch_boundary_maps_on_device(ctx, R, relax_variant):
+    """Pairs loaded without boundary maps under HPB regularisation get them
+    made on the device from their colour and depth-map pyramids
+    (sceneFlowMultiscale.cpp:116): each pair equals the oracle run the same
+    way, bit for bit; a with-init pair gets its level-0 map the same way."""
+    O = oracle()
+    a = _case(128, 96, 51, 16)
+    b = _case(160, 120, 52, 20)
+    c = _case(96, 72, 53, 12)
+    pair, *_ = make_case(96, 72, 53, 4)
+    c["est"] = pair["flow"].astype(np.float64) * 0.7
+    for case in (a, b, c):
+        case["cam"] = R.make_params(case["W"], case["H"]).cam
+        case["hpb"] = None
+    p = R.make_params(128, 96, n_outer=3, n_sor=30, min_width=16, relax_variant=relax_variant, graph=True)
+    print(f"device maps, relax variant {relax_variant}", flush=True)
+    ctx.batch_load(p, [a, b, c])
+    for run in range(3):    # eager, capture, replay
+        sts = ctx.batch_run(p, stats=True)
+        for i, case in enumerate((a, b, c)):
+            ub, vb, wb = ctx.batch_read(i)
+            po = O.make_params(case["W"], case["H"], n_outer=3, n_sor=30, min_width=case["min_width"])
+            if "est" in case:
+                ua, va, wa, sa = O.with_init(case["im1"], case["dm1"], case["im2"], case["dm2"], case["sigma1"], None,
+                                             case["est"], po)
+                with R.Context(0) as c2:
+                    pw = R.make_params(case["W"], case["H"], n_outer=3, n_sor=30, relax_variant=relax_variant)
+                    uc, vc, wc, _ = c2.with_init(pw, case["im1"], case["dm1"], case["im2"], case["dm2"], case["est"],
+                                                 case["sigma1"], None)
+                assert np.array_equal(uc, ub) and np.array_equal(vc, vb) and np.array_equal(wc, wb)
+                du, dv, dw = (float(np.abs(x - y).max()) for x, y in ((ua, ub), (va, vb), (wa, wb)))
+                assert du <= 1e-3 and dv <= 1e-3 and dw <= 1e-5, (run, i, du, dv, dw)
+            else:
+                ua, va, wa, sa = O.multiscale(case["im1"], case["dm1"], case["im2"], case["dm2"], case["sigma1"],
+                                              None, po)
+                L = sa.n_levels
+                assert [sts[i].sweeps_exec[k] for k in range(L)] == [sa.sweeps_exec[k] for k in range(L)]
+                assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb), (run, i)

@@ -64,3 +64,9 @@ def test_run_two_frames_matches_oracle(tmp_path, relax_variant):
     ref = np.stack([u, v, w], -1).astype(np.float32)
     assert np.max(np.abs(flow[..., :2] - ref[..., :2])) <= 1e-3
     assert np.max(np.abs(flow[..., 2] - ref[..., 2])) <= 1e-5
+    # the same run with the boundary maps made on the device (no function installed)
+    dev = _read_flo3(str(out) + ".device.flo3")
+    u, v, w, st = O.multiscale(im1, d1, im2, d2, sigma, None, p)
+    ref = np.stack([u, v, w], -1).astype(np.float32)
+    assert np.max(np.abs(dev[..., :2] - ref[..., :2])) <= 1e-3
+    assert np.max(np.abs(dev[..., 2] - ref[..., 2])) <= 1e-5

@@ -234,12 +234,17 @@ def test_error_paths(R):
             c.pair_load(p, z, d, z, d)
         assert e.value.code == R.ERR_BUDGET
     with R.Context(0) as c:
-        p = R.make_params(64, 48)       # HPB without maps
-        z = np.zeros((48, 64, 3))
-        d = np.ones((48, 64))
+        # HPB without maps: made on the device, which needs the depth-map and
+        # colour pyramids to have equal level sizes (47 x 33 at ratio 0.7: 11 x 7
+        # against 11 x 8 at the coarsest level)
+        p = R.make_params(47, 33, ratio=0.7, min_width=11)
+        z = np.zeros((33, 47, 3))
+        d = np.ones((33, 47))
         with pytest.raises(R.RgbdFlowError) as e:
             c.pair_load(p, z, d, z, d)
-        assert e.value.code == R.ERR_ARG
+        assert e.value.code == R.ERR_UNSUPPORTED
+        z = np.zeros((48, 64, 3))
+        d = np.ones((48, 64))
         p = R.make_params(64, 48, reg=R.REG_ISOTROPIC, n_outer=5000)
         c.pair_load(p, z, d, z, d)
         with pytest.raises(R.RgbdFlowError) as e:
