@@ -181,6 +181,8 @@ def main():
     ap.add_argument("--device-hpb", action="store_true",
                     help="serving: submit no boundary maps; the server makes them on the device")
     ap.add_argument("--lanes2", action="store_true", help="also time the batch as two halves on two streams")
+    ap.add_argument("--hpb", default="",
+                    help="instead: time computeHeuristicPb on the device for frames of these sizes ('WxH,...')")
     ap.add_argument("--served-streams", type=int, default=8,
                     help="640x480 streams of the served measurement beside the batch (0: skip)")
     ap.add_argument("--served-seconds", type=float, default=4.0)
@@ -198,6 +200,8 @@ def main():
         return mixed_main(R, args)
     if args.serve:
         return serve_main(R, args)
+    if args.hpb:
+        return hpb_main(R, args)
     pairs, levels = workload(R, max(1, min(args.distinct, args.batch)))
     B = args.batch
     batch = [pairs[i % len(pairs)] for i in range(B)]
@@ -449,6 +453,42 @@ def mixed_main(R, args):
                    "graph": not args.no_graph},
     }
     ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def hpb_main(R, args):
+    """computeHeuristicPb of one frame on the device (seven kernels), per size:
+    wall time of the C-ABI call (host frame in, maps out, PCIe included) over
+    --steps calls after --warmup, and the oracle's time for the same frame on
+    one host core."""
+    import rgbdflow_amd.synth as S
+    ctx = R.Context(0)
+    rows = {}
+    for tok in args.hpb.split(","):
+        w, h = (int(x) for x in tok.split("x"))
+        cam = S.Camera(w, h, 525.0 * w / 640, 525.0 * w / 640, (w - 1) / 2, (h - 1) / 2)
+        pair = S.make_pair(w, h, 21, n_segments=6, cam=cam)
+        im1, d1, _, _ = S.to_solver_inputs(pair)
+        dm = d1.astype(np.float64)
+        for _ in range(args.warmup):
+            ctx.heuristic_pb(im1, dm)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.heuristic_pb(im1, dm)
+        el = (time.perf_counter() - t0) / args.steps
+        row = {"ms_per_frame": round(el * 1e3, 3), "megapixels_per_s": round(w * h / el / 1e6, 1)}
+        if not args.no_cpu_baseline:
+            import oracle as O
+            t0 = time.perf_counter()
+            O.heuristic_pb(im1, dm)
+            row["oracle_ms_1_thread"] = round((time.perf_counter() - t0) * 1e3, 1)
+        rows[tok] = row
+        print(f"bench: hpb {tok}: {row}", file=sys.stderr, flush=True)
+    ctx.close()
+    out = {"metric": "computeHeuristicPb frames/s on the device", "unit": "ms per frame", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "dtype": "f32 (f64 normals)",
+           "data": "synthetic (seeded generator)", "value": rows[args.hpb.split(",")[0]]["ms_per_frame"],
+           "config": {"workload": "one frame per call, host frame in and maps out", "sizes": rows}}
     print(json.dumps(out), flush=True)
 
 

@@ -87,12 +87,18 @@ F3 normalized(F3 v) {  // Eigen's normalize(): v / sqrt(|v|^2) when |v|^2 > 0
   return v;
 }
 
-// Cyclic Jacobi on a symmetric 3x3 (f64), a fixed 12 sweeps; returns the
-// eigenvector of the smallest eigenvalue in n.
+// Cyclic Jacobi on a symmetric 3x3 (f64), at most 12 sweeps, ending before
+// a sweep once the off-diagonal mass is below 1e-15 of the diagonal's;
+// returns the eigenvector of the smallest eigenvalue in n.
 void smallest_eigvec(double a[3][3], double n[3]) {
   double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
-  for (int sweep = 0; sweep < 12; sweep++)
+  for (int sweep = 0; sweep < 12; sweep++) {
+    // converged: the off-diagonal mass is below 1e-15 of the diagonal's
+    // (the eigenvector then moves by far less than an f32 ulp)
+    const double off = (std::fabs(a[0][1]) + std::fabs(a[0][2])) + std::fabs(a[1][2]);
+    const double dia = (std::fabs(a[0][0]) + std::fabs(a[1][1])) + std::fabs(a[2][2]);
+    if (off <= 1e-15 * dia) break;
     for (int r = 0; r < 3; r++) {
       const int p = P[r], q = Q[r];
       const double apq = a[p][q];
@@ -117,6 +123,7 @@ void smallest_eigvec(double a[3][3], double n[3]) {
         v[k][q] = vkq + s * (vkp - tau * vkq);
       }
     }
+  }
   int m = 0;
   if (a[1][1] < a[m][m]) m = 1;
   if (a[2][2] < a[m][m]) m = 2;

@@ -96,13 +96,15 @@ struct BatchPlan {
   // boundary maps computed on the device, level by level: the depth-map
   // pyramid level (from the next larger one; level 0 is dm1) and the level's
   // computeHeuristicPb
-  struct HpbStep {
-    const double* larger;  // null at level 0
+  struct DmStep {
+    const double* larger;  // the next larger level
     int LW, LH;
     double* dm;            // the level's depth-map pyramid image
-    HpbLevel lvl;
+    int W, H;
   };
-  std::vector<HpbStep> hpb_steps;
+  std::vector<DmStep> dm_steps;
+  // then the levels' computeHeuristicPb, several levels per launch
+  std::vector<HpbBatch> hpb_groups;
   // the schedule captured as a HIP graph (params.graph)
   int eager_runs = 0;
   hipGraph_t graph = nullptr;
@@ -828,7 +830,8 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
   bp.host.clear();
   bp.steps.clear();
   bp.waves.clear();
-  bp.hpb_steps.clear();
+  bp.dm_steps.clear();
+  bp.hpb_groups.clear();
   const int B = (int)c->pairs.size();
   // per-pair preparation
   {
@@ -898,24 +901,47 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
       bp.waves.push_back(wv);
     }
   }
-  // boundary maps of pairs loaded without them (sceneFlowMultiscale.cpp:116)
+  // boundary maps of pairs loaded without them (sceneFlowMultiscale.cpp:116):
+  // the depth-map pyramids, then the levels in groups of up to kHpbMaxItems
+  // (and about kHpbGroupFloats of scratch), each level with its own scratch;
+  // groups run one after the other and share the scratch
   {
-    size_t scratch = 0;
-    for (const PairSlot& ps : c->pairs)
-      if (ps.hpb_device) scratch = std::max(scratch, hpb_scratch_floats(ps.W, ps.H));
-    float* hs = nullptr;
-    if (scratch) TRY(get(c, "hpb.scratch", scratch, &hs));
-    for (PairSlot& ps : c->pairs) {
+    constexpr size_t kHpbGroupFloats = size_t(64) << 20;
+    struct Lv {
+      const PairSlot* ps;
+      int k;
+      size_t off;
+    };
+    std::vector<std::vector<Lv>> groups;
+    size_t scratch = 0, used = 0;
+    for (const PairSlot& ps : c->pairs) {
       if (!ps.hpb_device) continue;
       for (int k = 0; k < (int)ps.L.size(); k++) {
-        BatchPlan::HpbStep h;
-        h.larger = k == 0 ? nullptr : level_of(ps, ps.dm1, ps.pdm, 1, k - 1);
-        h.LW = k == 0 ? 0 : ps.L[k - 1].W;
-        h.LH = k == 0 ? 0 : ps.L[k - 1].H;
-        h.dm = level_of(ps, ps.dm1, ps.pdm, 1, k);
-        h.lvl = hpb_level(ps.L[k].W, ps.L[k].H, level_of(ps, ps.im1, ps.pim1, 3, k), h.dm, ps.hpb + ps.hoff[k], hs);
-        bp.hpb_steps.push_back(h);
+        if (k > 0)
+          bp.dm_steps.push_back({level_of(ps, ps.dm1, ps.pdm, 1, k - 1), ps.L[k - 1].W, ps.L[k - 1].H,
+                                 level_of(ps, ps.dm1, ps.pdm, 1, k), ps.L[k].W, ps.L[k].H});
+        const size_t need = (hpb_scratch_floats(ps.L[k].W, ps.L[k].H) + 63) & ~size_t(63);
+        if (groups.empty() || groups.back().size() == (size_t)kHpbMaxItems ||
+            (used > 0 && used + need > kHpbGroupFloats)) {
+          groups.emplace_back();
+          used = 0;
+        }
+        groups.back().push_back({&ps, k, used});
+        used += need;
+        scratch = std::max(scratch, used);
       }
+    }
+    float* hs = nullptr;
+    if (scratch) TRY(get(c, "hpb.scratch", scratch, &hs));
+    for (const std::vector<Lv>& g : groups) {
+      HpbBatch hb;
+      hb.n = 0;
+      for (const Lv& e : g) {
+        const PairSlot& ps = *e.ps;
+        hpb_batch_add(hb, hpb_level(ps.L[e.k].W, ps.L[e.k].H, level_of(ps, ps.im1, ps.pim1, 3, e.k),
+                                    level_of(ps, ps.dm1, ps.pdm, 1, e.k), ps.hpb + ps.hoff[e.k], hs + e.off));
+      }
+      bp.hpb_groups.push_back(hb);
     }
   }
   // the level steps (sceneFlowMultiscale.cpp:95-196), aligned on the finest level
@@ -1023,10 +1049,8 @@ void enqueue_batch(rgbdflow_ctx* c, const BatchPlan& bp, const SolveScalars& k) 
   launch_pair_prep(s, bp.prep, bp.prep_blocks);
   for (const BatchPlan::Wave& wv : bp.waves) launch_pyramid_jobs(s, wv.wl, wv.blocks);
   // boundary maps made on the device, from the colour pyramid just built
-  for (const BatchPlan::HpbStep& h : bp.hpb_steps) {
-    if (h.larger) launch_depth_map_level(s, h.larger, h.LW, h.LH, h.dm, h.lvl.W, h.lvl.H);
-    launch_hpb(s, h.lvl);
-  }
+  for (const BatchPlan::DmStep& d : bp.dm_steps) launch_depth_map_level(s, d.larger, d.LW, d.LH, d.dm, d.W, d.H);
+  for (const HpbBatch& hb : bp.hpb_groups) launch_hpb_batch(s, hb);
   // coarse to fine
   for (const StepPlan& sp : bp.steps) {
     launch_features(s, sp.vpix, sp.pix);

@@ -227,6 +227,20 @@ struct HpbLevel {
 size_t hpb_scratch_floats(int W, int H);
 HpbLevel hpb_level(int W, int H, const double* bgr, const double* depth, float* out, float* scratch);
 void launch_hpb(hipStream_t s, const HpbLevel& L);
+// Several levels (of any pairs, each with its own scratch) in one launch per
+// stage, passed by value as the kernels' argument (3.5 KB, under the 4 KB
+// kernel-argument limit): per stage the first workgroup of each item.
+constexpr int kHpbMaxItems = 24;
+struct HpbBatch {
+  int n;
+  int s_pix[kHpbMaxItems + 1];  // per-pixel stages, 256 pixels a workgroup
+  int s_row[kHpbMaxItems + 1];  // row sums, 64 rows of one channel a workgroup
+  int s_col[kHpbMaxItems + 1];  // integral images, 256 columns x channels a workgroup
+  int s_tile[kHpbMaxItems + 1]; // normals, 16x16 pixels a workgroup
+  HpbLevel items[kHpbMaxItems];
+};
+bool hpb_batch_add(HpbBatch& b, const HpbLevel& L);  // false when full
+void launch_hpb_batch(hipStream_t s, const HpbBatch& b);
 
 struct SolveScalars {
   double alpha, gamma, color_weight;

@@ -9,12 +9,14 @@
 //                   organised point cloud (f = 525 W / 640, c = (W/2, H/2)),
 //                   the per-pixel half-disk radius round(f 0.02 / z) in [1, 5]
 //   k_hpb_normals   normal of each valid point: the points of its 9x9 window
+//                   (staged in LDS per 16x16 pixels)
 //                   within 0.05 x stereoErrorRatio(z) in depth (at least 27),
 //                   f64 two-pass covariance, cyclic Jacobi eigen-solver
 //                   (Numerical Recipes 3rd ed. 11.1; PCL's solver is not
 //                   restated), flipped towards the camera
 //   k_hpb_rows      running row sums of the 7 channels (colour, normal,
-//                   normal validity), left to right in f32 / u32
+//                   normal validity), left to right in f32 / u32, through
+//                   64x64 LDS tiles
 //   k_hpb_cols      the integral images, top to bottom per column with the
 //                   reference's recurrence ii = (up + row sum) + pixel
 //                   (halfDiskFeatures.ipp:139-150), so every sum rounds as
@@ -34,6 +36,18 @@ namespace rgbdflow {
 namespace {
 
 constexpr int kHB = 256;
+
+// The item of a batch that workgroup blk works on: the last item whose first
+// block is <= blk (start holds n + 1 ascending entries).
+__device__ __forceinline__ int hpb_item(const int* start, int n, int blk) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 __device__ __forceinline__ double stereo_error(double z) { return z * z / (1 / .00285 + z); }
 __device__ __forceinline__ double stereo_error_ratio(double z) { return stereo_error(z) / stereo_error(1.0); }
@@ -75,7 +89,12 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
 
 __device__ __forceinline__ void smallest_eigvec(double a[3][3], double n[3]) {
   double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  for (int sweep = 0; sweep < 12; sweep++)
+  for (int sweep = 0; sweep < 12; sweep++) {
+    // converged: the off-diagonal mass is below 1e-15 of the diagonal's
+    // (the eigenvector then moves by far less than an f32 ulp)
+    const double off = (fabs(a[0][1]) + fabs(a[0][2])) + fabs(a[1][2]);
+    const double dia = (fabs(a[0][0]) + fabs(a[1][1])) + fabs(a[2][2]);
+    if (off <= 1e-15 * dia) break;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
       const int p = r < 2 ? 0 : 1, q = r == 0 ? 1 : 2;
@@ -102,6 +121,7 @@ __device__ __forceinline__ void smallest_eigvec(double a[3][3], double n[3]) {
         v[k][q] = vkq + s * (vkp - tau * vkq);
       }
     }
+  }
   int m = 0;
   if (a[1][1] < a[m][m]) m = 1;
   if (a[2][2] < a[m][m]) m = 2;
@@ -119,8 +139,10 @@ __device__ __forceinline__ Cam hpb_cam(int W, int H) {
   return c;
 }
 
-__global__ void __launch_bounds__(kHB) k_hpb_prep(HpbLevel L) {
-  const int l = blockIdx.x * kHB + threadIdx.x;
+__global__ void __launch_bounds__(kHB) k_hpb_prep(const HpbBatch B) {
+  const int it = hpb_item(B.s_pix, B.n, blockIdx.x);
+  const HpbLevel& L = B.items[it];
+  const int l = (blockIdx.x - B.s_pix[it]) * kHB + threadIdx.x;
   const int W = L.W, H = L.H, N = W * H;
   if (l >= N) return;
   const int i = l / W, j = l - i * W;
@@ -137,15 +159,33 @@ __global__ void __launch_bounds__(kHB) k_hpb_prep(HpbLevel L) {
   gp(L.rad)[l] = d > 0 ? max(1, min(5, (int)rintf(c.f * .02f / d))) : 5;
 }
 
-__global__ void __launch_bounds__(kHB) k_hpb_normals(HpbLevel L) {
-  const int l = blockIdx.x * kHB + threadIdx.x;
+// A workgroup takes 16x16 pixels and stages the points of their windows
+// (24x24, the image's outside never read) in LDS once.
+constexpr int kNT = 16, kNR = kNT + 8, kNP = kNR * kNR;
+__global__ void __launch_bounds__(kHB) k_hpb_normals(const HpbBatch B) {
+  __shared__ float PX[kNP], PY[kNP], PZ[kNP];
+  const int it = hpb_item(B.s_tile, B.n, blockIdx.x);
+  const HpbLevel& L = B.items[it];
   const int W = L.W, H = L.H, N = W * H;
-  if (l >= N) return;
-  const int i = l / W, j = l - i * W;
+  const int ntx = (W + kNT - 1) / kNT, b = blockIdx.x - B.s_tile[it];
+  const int oy = (b / ntx) * kNT, ox = (b - (b / ntx) * ntx) * kNT;
+  const int ry0 = oy - 4, rx0 = ox - 4;
   const RGBDFLOW_GLOBAL float* X = gp(L.pts);
+  for (int e = threadIdx.x; e < kNP; e += kHB) {
+    const int gy = ry0 + e / kNR, gx = rx0 + e % kNR;
+    const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
+    const int q = gy * W + gx;
+    PX[e] = in ? X[q] : 0.f;
+    PY[e] = in ? X[N + q] : 0.f;
+    PZ[e] = in ? X[2 * N + q] : 0.f;
+  }
+  __syncthreads();
+  const int i = oy + threadIdx.x / kNT, j = ox + threadIdx.x % kNT;
+  if (i >= H || j >= W) return;
+  const int l = i * W + j;
   float n0 = 0.f, n1 = 0.f, n2 = 0.f;
   unsigned valid = 0;
-  const float z = X[2 * N + l];
+  const float z = PZ[(i - ry0) * kNR + (j - rx0)];
   if (z > 0.f) {
     const double thr = (double).05f * stereo_error_ratio((double)z);
     const int y0 = max(0, i - 4), y1 = min(H - 1, i + 4), x0 = max(0, j - 4), x1 = min(W - 1, j + 4);
@@ -153,12 +193,12 @@ __global__ void __launch_bounds__(kHB) k_hpb_normals(HpbLevel L) {
     double m[3] = {0, 0, 0};
     for (int y = y0; y <= y1; y++)
       for (int x = x0; x <= x1; x++) {
-        const int q = y * W + x;
-        if ((double)fabsf(z - X[2 * N + q]) < thr) {
+        const int q = (y - ry0) * kNR + (x - rx0);
+        if ((double)fabsf(z - PZ[q]) < thr) {
           cnt++;
-          m[0] += X[q];
-          m[1] += X[N + q];
-          m[2] += X[2 * N + q];
+          m[0] += PX[q];
+          m[1] += PY[q];
+          m[2] += PZ[q];
         }
       }
     if (cnt >= 27) {  // max(6, 9 * 9 / 3) neighbours
@@ -166,9 +206,9 @@ __global__ void __launch_bounds__(kHB) k_hpb_normals(HpbLevel L) {
       double a[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
       for (int y = y0; y <= y1; y++)
         for (int x = x0; x <= x1; x++) {
-          const int q = y * W + x;
-          if ((double)fabsf(z - X[2 * N + q]) < thr) {
-            const double d0 = X[q] - m[0], d1 = X[N + q] - m[1], d2 = X[2 * N + q] - m[2];
+          const int q = (y - ry0) * kNR + (x - rx0);
+          if ((double)fabsf(z - PZ[q]) < thr) {
+            const double d0 = PX[q] - m[0], d1 = PY[q] - m[1], d2 = PZ[q] - m[2];
             a[0][0] += d0 * d0;
             a[0][1] += d0 * d1;
             a[0][2] += d0 * d2;
@@ -201,70 +241,133 @@ __global__ void __launch_bounds__(kHB) k_hpb_normals(HpbLevel L) {
   gp(L.nval)[l] = valid;
 }
 
-// channel c of the 7 integral-image inputs at pixel l
-__device__ __forceinline__ float chan_f(const HpbLevel& L, int c, int l, int N) {
-  return c < 3 ? gp(L.col)[(size_t)c * N + l] : gp(L.nrm)[(size_t)(c - 3) * N + l];
-}
-
-// running row sums before each pixel: rs[c][l] = sum of row values left of l
-__global__ void __launch_bounds__(kHB) k_hpb_rows(HpbLevel L) {
-  const int t = blockIdx.x * kHB + threadIdx.x;
-  const int W = L.W, H = L.H, N = W * H;
-  if (t >= 7 * H) return;
-  const int c = t / H, i = t - c * H;
-  if (c < 6) {
-    float s = 0.f;
-    for (int j = 0; j < W; j++) {
-      const int l = i * W + j;
-      gp(L.rowsum)[(size_t)c * N + l] = s;
-      s += chan_f(L, c, l, N);
+// Running row sums before each pixel: rs[c][l] = sum of the row's values
+// left of l, added one by one from the left (every f32 partial sum rounds as
+// a sequential loop rounds it).  One wave per 64 rows of a channel: 64x64
+// tiles are loaded coalesced (lane = column) into LDS, lane r then scans row
+// r of the tile in order, and the tile goes back out coalesced; the next
+// tile's loads are issued before the scan.
+constexpr int kRowTile = 64, kRowPitch = 65;  // odd pitch: the lanes scanning rows hit distinct banks
+template <class V>
+__device__ __forceinline__ void row_scan(const RGBDFLOW_GLOBAL V* src, RGBDFLOW_GLOBAL V* dst, int W, int nr,
+                                         V* T) {
+  const int lane = threadIdx.x;
+  const int nt = (W + kRowTile - 1) / kRowTile;
+  V x[kRowTile];
+  auto load = [&](int t) {
+    const int j = t * kRowTile + lane;
+    const RGBDFLOW_GLOBAL V* q = src + j;
+#pragma unroll
+    for (int r = 0; r < kRowTile; r++) {
+      x[r] = (r < nr && j < W) ? *q : V(0);
+      q += W;
     }
-  } else {
-    unsigned s = 0;
-    for (int j = 0; j < W; j++) {
-      const int l = i * W + j;
-      gp(L.vrowsum)[l] = s;
-      s += gp(L.nval)[l];
+  };
+  load(0);
+  V s = V(0);
+  for (int t = 0; t < nt; t++) {
+    V* B = T + (t & 1) * kRowTile * kRowPitch;
+#pragma unroll
+    for (int r = 0; r < kRowTile; r++) B[r * kRowPitch + lane] = x[r];
+    __syncthreads();
+    if (t + 1 < nt) load(t + 1);
+    // columns past W hold zeros: adding them leaves s as it is, and they are not stored
+    V* row = B + lane * kRowPitch;
+    V v[kRowTile];
+#pragma unroll
+    for (int k = 0; k < kRowTile; k++) v[k] = row[k];
+#pragma unroll
+    for (int k = 0; k < kRowTile; k++) {
+      row[k] = s;
+      s += v[k];
     }
+    __syncthreads();
+    const int j = t * kRowTile + lane;
+    if (j < W)
+      for (int r = 0; r < nr; r++) dst[(size_t)r * W + j] = B[r * kRowPitch + lane];
   }
 }
 
-// integral images with a zero row and column in front: ii[(i+1)(W+1) + j+1]
-__global__ void __launch_bounds__(kHB) k_hpb_cols(HpbLevel L) {
-  const int t = blockIdx.x * kHB + threadIdx.x;
-  const int W = L.W, H = L.H, N = W * H, IW = W + 1;
-  const size_t IN = (size_t)IW * (H + 1);
+__global__ void __launch_bounds__(kRowTile) k_hpb_rows(const HpbBatch B) {
+  __shared__ float T[2 * kRowTile * kRowPitch];
+  const int it = hpb_item(B.s_row, B.n, blockIdx.x);
+  const HpbLevel& L = B.items[it];
+  const int W = L.W, H = L.H, N = W * H;
+  const int ng = (H + kRowTile - 1) / kRowTile, b = blockIdx.x - B.s_row[it];
+  const int c = b / ng, i0 = (b - c * ng) * kRowTile;
+  const int nr = min(kRowTile, H - i0);
+  const size_t o = (size_t)i0 * W;
+  if (c < 6) {
+    const RGBDFLOW_GLOBAL float* src = c < 3 ? gp(L.col) + (size_t)c * N : gp(L.nrm) + (size_t)(c - 3) * N;
+    row_scan<float>(src + o, gp(L.rowsum) + (size_t)c * N + o, W, nr, T);
+  } else {
+    row_scan<unsigned>(gp(L.nval) + o, gp(L.vrowsum) + o, W, nr, reinterpret_cast<unsigned*>(T));
+  }
+}
+
+// The integral images with a zero row and column in front,
+// ii[(i+1)(W+1) + j+1] = (ii above + row sum) + pixel, top to bottom per
+// column (one lane per column and channel, consecutive lanes on consecutive
+// columns); the next kColChunk rows are loaded before the current ones are
+// accumulated and stored.
+constexpr int kColChunk = 32;
+template <class V>
+__device__ __forceinline__ void col_scan(const RGBDFLOW_GLOBAL V* rs, const RGBDFLOW_GLOBAL V* px,
+                                         RGBDFLOW_GLOBAL V* ii, int W, int H, int j) {
+  const int IW = W + 1;
+  ii[j + 1] = V(0);
+  if (j == 0)
+    for (int i = 0; i <= H; i++) ii[(size_t)i * IW] = V(0);
+  V a[kColChunk], b[kColChunk];
+  auto load = [&](int i0) {
+#pragma unroll
+    for (int k = 0; k < kColChunk; k++) {
+      const bool in = i0 + k < H;
+      a[k] = in ? rs[(size_t)(i0 + k) * W + j] : V(0);
+      b[k] = in ? px[(size_t)(i0 + k) * W + j] : V(0);
+    }
+  };
+  load(0);
+  V up = V(0);
+  for (int i0 = 0; i0 < H; i0 += kColChunk) {
+    V ca[kColChunk], cb[kColChunk];
+#pragma unroll
+    for (int k = 0; k < kColChunk; k++) {
+      ca[k] = a[k];
+      cb[k] = b[k];
+    }
+    if (i0 + kColChunk < H) load(i0 + kColChunk);
+    const int n = min(kColChunk, H - i0);
+#pragma unroll
+    for (int k = 0; k < kColChunk; k++)
+      if (k < n) {
+        const V v = (up + ca[k]) + cb[k];
+        ii[(size_t)(i0 + k + 1) * IW + j + 1] = v;
+        up = v;
+      }
+  }
+}
+
+__global__ void __launch_bounds__(kHB) k_hpb_cols(const HpbBatch B) {
+  const int it = hpb_item(B.s_col, B.n, blockIdx.x);
+  const HpbLevel& L = B.items[it];
+  const int t = (blockIdx.x - B.s_col[it]) * kHB + threadIdx.x;
+  const int W = L.W, H = L.H, N = W * H;
+  const size_t IN = (size_t)(W + 1) * (H + 1);
   if (t >= 7 * W) return;
   const int c = t / W, j = t - c * W;
   if (c < 6) {
-    RGBDFLOW_GLOBAL float* ii = gp(L.ii) + (size_t)c * IN;
-    ii[j + 1] = 0.f;
-    if (j == 0)
-      for (int i = 0; i <= H; i++) ii[(size_t)i * IW] = 0.f;
-    float up = 0.f;
-    for (int i = 0; i < H; i++) {
-      const int l = i * W + j;
-      const float v = (up + gp(L.rowsum)[(size_t)c * N + l]) + chan_f(L, c, l, N);
-      ii[(size_t)(i + 1) * IW + j + 1] = v;
-      up = v;
-    }
+    const RGBDFLOW_GLOBAL float* px = c < 3 ? gp(L.col) + (size_t)c * N : gp(L.nrm) + (size_t)(c - 3) * N;
+    col_scan<float>(gp(L.rowsum) + (size_t)c * N, px, gp(L.ii) + (size_t)c * IN, W, H, j);
   } else {
-    RGBDFLOW_GLOBAL unsigned* ii = gp(L.iv);
-    ii[j + 1] = 0u;
-    if (j == 0)
-      for (int i = 0; i <= H; i++) ii[(size_t)i * IW] = 0u;
-    unsigned up = 0u;
-    for (int i = 0; i < H; i++) {
-      const int l = i * W + j;
-      const unsigned v = (up + gp(L.vrowsum)[l]) + gp(L.nval)[l];
-      ii[(size_t)(i + 1) * IW + j + 1] = v;
-      up = v;
-    }
+    col_scan<unsigned>(gp(L.vrowsum), gp(L.nval), gp(L.iv), W, H, j);
   }
 }
 
-__global__ void __launch_bounds__(kHB) k_hpb_halfdisk(HpbLevel L) {
-  const int l = blockIdx.x * kHB + threadIdx.x;
+__global__ void __launch_bounds__(kHB) k_hpb_halfdisk(const HpbBatch B) {
+  const int it = hpb_item(B.s_pix, B.n, blockIdx.x);
+  const HpbLevel& L = B.items[it];
+  const int l = (blockIdx.x - B.s_pix[it]) * kHB + threadIdx.x;
   const int W = L.W, H = L.H, N = W * H, IW = W + 1;
   if (l >= N) return;
   const size_t IN = (size_t)IW * (H + 1);
@@ -368,8 +471,10 @@ __device__ float hpb_calc(const HpbLevel& L, int i, int j, int k, float z1, floa
   return p;
 }
 
-__global__ void __launch_bounds__(kHB) k_hpb_calc(HpbLevel L) {
-  const int l = blockIdx.x * kHB + threadIdx.x;
+__global__ void __launch_bounds__(kHB) k_hpb_calc(const HpbBatch B) {
+  const int it = hpb_item(B.s_pix, B.n, blockIdx.x);
+  const HpbLevel& L = B.items[it];
+  const int l = (blockIdx.x - B.s_pix[it]) * kHB + threadIdx.x;
   const int W = L.W, H = L.H, N = W * H;
   if (l >= N) return;
   const int i = l / W, j = l - i * W;
@@ -382,8 +487,10 @@ __global__ void __launch_bounds__(kHB) k_hpb_calc(HpbLevel L) {
   gp(L.bp)[2 * l + 1] = p1;
 }
 
-__global__ void __launch_bounds__(kHB) k_hpb_nms(HpbLevel L) {
-  const int l = blockIdx.x * kHB + threadIdx.x;
+__global__ void __launch_bounds__(kHB) k_hpb_nms(const HpbBatch B) {
+  const int it = hpb_item(B.s_pix, B.n, blockIdx.x);
+  const HpbLevel& L = B.items[it];
+  const int l = (blockIdx.x - B.s_pix[it]) * kHB + threadIdx.x;
   const int W = L.W, H = L.H, N = W * H;
   if (l >= N) return;
   const int i = l / W, j = l - i * W;
@@ -438,16 +545,42 @@ HpbLevel hpb_level(int W, int H, const double* bgr, const double* depth, float* 
   return L;
 }
 
+namespace {
+int row_blocks(const HpbLevel& L) { return 7 * ((L.H + kRowTile - 1) / kRowTile); }
+int col_blocks(const HpbLevel& L) { return (7 * L.W + kHB - 1) / kHB; }
+int pix_blocks(const HpbLevel& L) { return (L.W * L.H + kHB - 1) / kHB; }
+int tile_blocks(const HpbLevel& L) { return ((L.W + kNT - 1) / kNT) * ((L.H + kNT - 1) / kNT); }
+}  // namespace
+
+bool hpb_batch_add(HpbBatch& b, const HpbLevel& L) {
+  if (b.n >= kHpbMaxItems) return false;
+  if (b.n == 0) b.s_pix[0] = b.s_row[0] = b.s_col[0] = b.s_tile[0] = 0;
+  b.items[b.n] = L;
+  b.s_pix[b.n + 1] = b.s_pix[b.n] + pix_blocks(L);
+  b.s_row[b.n + 1] = b.s_row[b.n] + row_blocks(L);
+  b.s_col[b.n + 1] = b.s_col[b.n] + col_blocks(L);
+  b.s_tile[b.n + 1] = b.s_tile[b.n] + tile_blocks(L);
+  b.n++;
+  return true;
+}
+
+void launch_hpb_batch(hipStream_t s, const HpbBatch& b) {
+  if (b.n == 0) return;
+  const dim3 pix(b.s_pix[b.n]);
+  hipLaunchKernelGGL(k_hpb_prep, pix, dim3(kHB), 0, s, b);
+  hipLaunchKernelGGL(k_hpb_normals, dim3(b.s_tile[b.n]), dim3(kHB), 0, s, b);
+  hipLaunchKernelGGL(k_hpb_rows, dim3(b.s_row[b.n]), dim3(kRowTile), 0, s, b);
+  hipLaunchKernelGGL(k_hpb_cols, dim3(b.s_col[b.n]), dim3(kHB), 0, s, b);
+  hipLaunchKernelGGL(k_hpb_halfdisk, pix, dim3(kHB), 0, s, b);
+  hipLaunchKernelGGL(k_hpb_calc, pix, dim3(kHB), 0, s, b);
+  hipLaunchKernelGGL(k_hpb_nms, pix, dim3(kHB), 0, s, b);
+}
+
 void launch_hpb(hipStream_t s, const HpbLevel& L) {
-  const int N = L.W * L.H;
-  const int nb = (N + kHB - 1) / kHB;
-  hipLaunchKernelGGL(k_hpb_prep, dim3(nb), dim3(kHB), 0, s, L);
-  hipLaunchKernelGGL(k_hpb_normals, dim3(nb), dim3(kHB), 0, s, L);
-  hipLaunchKernelGGL(k_hpb_rows, dim3((7 * L.H + kHB - 1) / kHB), dim3(kHB), 0, s, L);
-  hipLaunchKernelGGL(k_hpb_cols, dim3((7 * L.W + kHB - 1) / kHB), dim3(kHB), 0, s, L);
-  hipLaunchKernelGGL(k_hpb_halfdisk, dim3(nb), dim3(kHB), 0, s, L);
-  hipLaunchKernelGGL(k_hpb_calc, dim3(nb), dim3(kHB), 0, s, L);
-  hipLaunchKernelGGL(k_hpb_nms, dim3(nb), dim3(kHB), 0, s, L);
+  HpbBatch b;
+  b.n = 0;
+  hpb_batch_add(b, L);
+  launch_hpb_batch(s, b);
 }
 
 }  // namespace rgbdflow

@@ -268,3 +268,26 @@ def test_batch_boundary_maps_on_device(ctx, R, relax_variant):
                 L = sa.n_levels
                 assert [sts[i].sweeps_exec[k] for k in range(L)] == [sa.sweeps_exec[k] for k in range(L)]
                 assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb), (run, i)
+
+
+def test_batch_boundary_maps_two_groups(ctx, R):
+    """More pair levels than one launch of the boundary-map kernels takes
+    (24): 7 pairs x 4 levels run as two groups one after the other, sharing
+    their scratch; every pair equals the oracle run with maps made the same
+    way, bit for bit."""
+    O = oracle()
+    cases = [_case(64 + 8 * (i % 3), 48 + 6 * (i % 3), 60 + i, 8) for i in range(7)]
+    for case in cases:
+        case["cam"] = R.make_params(case["W"], case["H"]).cam
+        case["hpb"] = None
+    p = R.make_params(64, 48, n_outer=3, n_sor=20, min_width=8, relax_variant=1, graph=False)
+    ctx.batch_load(p, cases)
+    sts = ctx.batch_run(p, stats=True)
+    for i, case in enumerate(cases):
+        print(f"pair {i}: {case['W']}x{case['H']}", flush=True)
+        po = O.make_params(case["W"], case["H"], n_outer=3, n_sor=20, min_width=8)
+        ua, va, wa, sa = O.multiscale(case["im1"], case["dm1"], case["im2"], case["dm2"], case["sigma1"], None, po)
+        assert sa.n_levels == 4
+        ub, vb, wb = ctx.batch_read(i)
+        assert [sts[i].sweeps_exec[k] for k in range(4)] == [sa.sweeps_exec[k] for k in range(4)]
+        assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb), i
