@@ -178,6 +178,8 @@ def main():
                     help="serving: solve each stream's later pairs as sceneFlowWithInit from its last flow")
     ap.add_argument("--warm-outer", type=int, default=0, help="serving: outer iterations of warm starts (0: 15)")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage run of the batch")
+    ap.add_argument("--images", choices=["f64", "f16", "bf16"], default="f64",
+                    help="serving: storage of the feature images (f64: the reference's precision)")
     ap.add_argument("--device-hpb", action="store_true",
                     help="serving: submit no boundary maps; the server makes them on the device")
     ap.add_argument("--lanes2", action="store_true", help="also time the batch as two halves on two streams")
@@ -330,6 +332,20 @@ def main():
                                   "bytes_per_px_iter": B_SWEEP_16},
                "relax_avg_launch_us": round(r16[0].relax_ms * 1e3 / max(r16[0].relax_launches, 1), 3)}
         print(f"bench: fp16 storage {f16}", file=sys.stderr, flush=True)
+        # and the feature images (frame 1, frame 2, warped frame 2) in FP16 as well
+        p16.image_storage = R.IMAGES_F16
+        ctx.batch_load(p16, batch)
+        ctx.batch_run(p16, stats=True)
+        for _ in range(args.warmup):
+            ctx.batch_run(p16, stats=False)
+        el16i = timed_runs(ctx, p16, args.steps)
+        lat16i = np.array([ctx.batch_run(p16, stats=True)[0].device_ms
+                           for _ in range(max(5, args.latency_steps // 2))])
+        f16["with_fp16_images"] = {
+            "pairs_per_s": round(B * args.steps / el16i, 3),
+            "speedup_vs_plain_single": round(B * args.steps / el16i / plain_pps, 2) if plain_pps else None,
+            "batch_p99_ms": round(float(np.percentile(lat16i, 99)), 3)}
+        print(f"bench: fp16 storage and images {f16['with_fp16_images']}", file=sys.stderr, flush=True)
 
     # served: 640x480 streams at 30 fps, FP16 storage, warm starts, continuous batching
     served = None
@@ -508,7 +524,7 @@ def serve_frames(R, spec, n_frames):
 
 
 def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_outer, autotune=True, lanes=2, seed=7,
-              score_every=8, warmup=1.0, device_hpb=False):
+              score_every=8, warmup=1.0, device_hpb=False, images="f64"):
     """Streams submit frames at Poisson arrival times (rate fps each); stream k
     walks its size's sequence back and forth from its own starting frame.
     Arrivals run for warmup + seconds; requests that arrive in the first
@@ -521,7 +537,7 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
     sizes = sorted(set(spec), key=lambda t: -t[0])
     st = {"f32": R.STORE_F32, "f16": R.STORE_F16, "bf16": R.STORE_BF16}[storage]
     p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED, graph=True,
-                      storage=st)
+                      storage=st, image_storage={"f64": R.IMAGES_F64, "f16": R.IMAGES_F16, "bf16": R.IMAGES_BF16}[images])
     srv = R.Server(p, max_batch=max_batch, lanes=lanes, warm_start=warm, warm_n_outer=warm_outer)
     if autotune:
         with R.Context(0) as tctx:
@@ -607,7 +623,7 @@ def serve_main(R, args):
     frames = serve_frames(R, spec, args.frames)
     print(f"bench: serving {len(spec)} streams over {args.seconds} s", file=sys.stderr, flush=True)
     r = serve_run(R, spec, frames, args.fps, args.seconds, args.max_batch, args.storage, args.warm,
-                  args.warm_outer, autotune=not args.no_autotune, device_hpb=args.device_hpb)
+                  args.warm_outer, autotune=not args.no_autotune, device_hpb=args.device_hpb, images=args.images)
     out = {
         "metric": "served scene-flow pairs/s and request latency",
         "value": r["pairs_per_s"],
@@ -618,7 +634,7 @@ def serve_main(R, args):
         "data": "synthetic (seeded frame sequences, smooth per-segment SE(3) motion)",
         "config": dict(streams=args.serve, fps_per_stream=args.fps, arrival_seconds=args.seconds,
                        max_batch=args.max_batch, lanes=2, warm_start=args.warm, warm_n_outer=args.warm_outer,
-                       storage=args.storage, **r),
+                       storage=args.storage, images=args.images, **r),
     }
     print(json.dumps(out), flush=True)
 

@@ -69,6 +69,16 @@ extern "C" {
 #define RGBDFLOW_STORE_F16 1
 #define RGBDFLOW_STORE_BF16 2
 
+/* Storage of the feature images of a level (frame 1, frame 2 and the warped
+ * frame 2; 5 channels each: gray, its x/y derivatives, G-B, G-R), which the
+ * per-outer stage reads.  F64 is the reference's precision; F16 and BF16
+ * (the value rounded to f32, then to 16 bits, nearest-even) are precision
+ * variants that quarter the bytes: the stage arithmetic stays f64 on the
+ * loaded values.  Depth, flow and weights keep their precision. */
+#define RGBDFLOW_IMAGES_F64 0
+#define RGBDFLOW_IMAGES_F16 1
+#define RGBDFLOW_IMAGES_BF16 2
+
 /* rgbd::CameraParams (include/rgbd_util/CameraParams.h:17-28) */
 typedef struct rgbdflow_camera {
   unsigned int xRes, yRes;
@@ -105,7 +115,9 @@ typedef struct rgbdflow_params {
   float residual_tol;            /* RGBDFLOW_EXIT_RESIDUAL: stop once the relative residual is <= this
                                     (<= 0: 1e-4) */
   int pixel_update;              /* RGBDFLOW_PIXEL_*: order of a pixel's three unknowns */
-  int reserved;
+  int image_storage;             /* RGBDFLOW_IMAGES_*: storage of the feature images (pairs solved
+                                    from frames; the single-level entry points, which take f64
+                                    images, accept F64 only) */
 } rgbdflow_params;
 
 typedef struct rgbdflow_stats {

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
 
 PREC_REF, PREC_F64 = 0, 1
 REG_ISOTROPIC, REG_LETOUZEY, REG_HPB = 0, 1, 2
+IMG_F64, IMG_F16, IMG_BF16 = 0, 1, 2
 
 
 class Camera(C.Structure):
@@ -31,7 +32,8 @@ class Params(C.Structure):
                 ("pyramid_ratio", C.c_double), ("min_width", C.c_int), ("n_outer", C.c_int),
                 ("n_inner", C.c_int), ("n_sor", C.c_int), ("cam", Camera), ("precision", C.c_int),
                 ("disable_inner_exit", C.c_int), ("disable_outer_exit", C.c_int), ("omega", C.c_float),
-                ("residual_exit", C.c_int), ("residual_tol", C.c_double), ("pixel_gs", C.c_int)]
+                ("residual_exit", C.c_int), ("residual_tol", C.c_double), ("pixel_gs", C.c_int),
+                ("image_storage", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -89,7 +91,7 @@ def _f32(a):
 def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
                 ratio=0.5, min_width=None, reg=REG_HPB, precision=PREC_REF, cw=1.0, dvc=1.0,
                 disable_inner_exit=False, disable_outer_exit=False, omega=0.7, residual_exit=False,
-                residual_tol=1e-4, pixel_gs=False) -> Params:
+                residual_tol=1e-4, pixel_gs=False, image_storage=IMG_F64) -> Params:
     p = Params()
     p.regularization_type = reg
     p.color_data_weight = cw
@@ -110,7 +112,16 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.residual_exit = int(residual_exit)
     p.residual_tol = residual_tol
     p.pixel_gs = int(pixel_gs)
+    p.image_storage = image_storage
     return p
+
+
+def img_round(x, storage):
+    """x rounded as a feature image of that storage holds it (f32, then FP16 / BF16)."""
+    f = lib().orc_img_round
+    f.restype = C.c_double
+    f.argtypes = [C.c_double, C.c_int]
+    return np.array([f(float(v), storage) for v in np.ravel(x)]).reshape(np.shape(x))
 
 
 # ---------------------------------------------------------------- pyramids

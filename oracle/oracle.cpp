@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -295,6 +296,36 @@ vector<Level> depth_map_pyramid(const double* img, int W, int H, double ratio, i
     P[m].W = FW; P[m].H = FH; P[m].d.swap(foo);
   }
   return P;
+}
+
+/* The feature-image storage variants: the value rounded to f32, then to FP16
+ * (1 sign, 5 exponent, 10 fraction bits; subnormals below 2^-14, quantum
+ * 2^-24; overflow to infinity) or BF16 (the f32's top 16 bits), each step
+ * to nearest, ties to even.  Scaling by a power of two is exact, so
+ * nearbyint (round-half-even in the default mode) on x / quantum rounds the
+ * f32 value to the grid of FP16 numbers around it. */
+static double img_round(double xd, int storage) {
+  if (storage == ORC_IMG_F64) return xd;
+  const float x = (float)xd;
+  if (storage == ORC_IMG_BF16) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return xd;  // NaN
+    u = ((u + 0x7fffu + ((u >> 16) & 1u)) >> 16) << 16;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+  }
+  if (!(std::fabs(x) <= 3.4e38f) || x == 0.f) return x;  // zero, infinity, NaN as they are
+  int e;
+  std::frexp((double)x, &e);                               // |x| in [2^(e-1), 2^e)
+  const double q = std::ldexp(1.0, std::max(e - 1, -14) - 10);
+  const double r = std::nearbyint((double)x / q) * q;
+  return std::fabs(r) > 65504.0 ? std::copysign(INFINITY, r) : r;
+}
+static void img_round_all(double* a, size_t n, int storage) {
+  if (storage != ORC_IMG_F64)
+    for (size_t i = 0; i < n; i++) a[i] = img_round(a[i], storage);
 }
 
 /* OpticalFlow.cpp:148-220 im2feature(RGB_WITH_GRADIENT); Image.h:955-1016 dx/dy
@@ -788,8 +819,9 @@ struct Solver {
           const AT ax = x - x0, ay = y - y0;
           const AT w00 = (1 - ax) * (1 - ay), w01 = (1 - ax) * ay, w10 = ax * (1 - ay), w11 = ax * ay;
           for (int k = 0; k < C; k++)
-            Iw[l * C + k] = w00 * I2[((size_t)y0 * W + x0) * C + k] + w01 * I2[((size_t)y1 * W + x0) * C + k] +
-                            w10 * I2[((size_t)y0 * W + x1) * C + k] + w11 * I2[((size_t)y1 * W + x1) * C + k];
+            Iw[l * C + k] = img_round(w00 * I2[((size_t)y0 * W + x0) * C + k] + w01 * I2[((size_t)y1 * W + x0) * C + k] +
+                                      w10 * I2[((size_t)y0 * W + x1) * C + k] + w11 * I2[((size_t)y1 * W + x1) * C + k],
+                                      p->image_storage);
           const double* D2 = in->D2;
           Dw[l] = w00 * D2[(size_t)y0 * W + x0] + w01 * D2[(size_t)y1 * W + x0] + w10 * D2[(size_t)y0 * W + x1] +
                   w11 * D2[(size_t)y1 * W + x1] - u[2 * N + l];
@@ -983,6 +1015,7 @@ void orc_depth_map_pyramid(const double* depth, int W, int H, double ratio, int 
   for (auto& L : P) { std::memcpy(out + off, L.d.data(), L.d.size() * sizeof(double)); off += L.d.size(); }
 }
 void orc_im2feature(const double* bgr, int W, int H, double* out5) { im2feature(bgr, W, H, out5); }
+double orc_img_round(double x, int storage) { return img_round(x, storage); }
 void orc_warp_cpu(const double* I1, const double* D1, const double* I2, const double* D2, const double* u,
                   const double* v, const double* w, int W, int H, int C, double* wI, double* wD) {
   warp_cpu(I1, D1, I2, D2, u, v, w, W, H, C, wI, wD);
@@ -1148,6 +1181,8 @@ int orc_multiscale(const double* im1, const double* dm1, const double* im2, cons
     vector<double> I1(Nk * 5), I2(Nk * 5), Iw(Nk * 5), Dw(Nk);
     im2feature(P1[k].d.data(), Wk, Hk, I1.data());
     im2feature(P2[k].d.data(), Wk, Hk, I2.data());
+    img_round_all(I1.data(), Nk * 5, p->image_storage);
+    img_round_all(I2.data(), Nk * 5, p->image_storage);
     const vector<double>& D1 = PD1[k].d;
     const vector<double>& D2 = PD2[k].d;
     const orc_camera camk = downsized(p->cam, Wk);
@@ -1164,6 +1199,7 @@ int orc_multiscale(const double* im1, const double* dm1, const double* im2, cons
       for (size_t l = 0; l < Nk; l++) { nu[l] *= s; nv[l] *= s; }
       u.swap(nu); v.swap(nv); w.swap(nw);
       warp_cpu(I1.data(), D1.data(), I2.data(), D2.data(), u.data(), v.data(), w.data(), Wk, Hk, 5, Iw.data(), Dw.data());
+      img_round_all(Iw.data(), Nk * 5, p->image_storage);
     }
     vector<double> mag(Nk), reg(Nk * 2);
     depth_mag(D1.data(), Wk, Hk, (int)p->cam.xRes, (int)p->cam.yRes, p->cam.focalLength, mag.data());
@@ -1211,7 +1247,10 @@ int orc_with_init(const double* im1, const double* dm1, const double* im2, const
   vector<double> I1(N * 5), I2(N * 5), Iw(N * 5), Dw(N);
   im2feature(im1, W, H, I1.data());
   im2feature(im2, W, H, I2.data());
+  img_round_all(I1.data(), N * 5, p->image_storage);
+  img_round_all(I2.data(), N * 5, p->image_storage);
   warp_cpu(I1.data(), d1.data(), I2.data(), d2.data(), u.data(), v.data(), w.data(), W, H, 5, Iw.data(), Dw.data());
+  img_round_all(Iw.data(), N * 5, p->image_storage);
   vector<double> reg(N * 2), mag(N);
   reg_weights(hpb0, d1.data(), W, H, p->regularization_type, reg.data());
   depth_mag(d1.data(), W, H, (int)p->cam.xRes, (int)p->cam.yRes, p->cam.focalLength, mag.data());

@@ -45,7 +45,13 @@ typedef struct {
   int pixel_gs;           /* 0: a pixel's unknowns are updated from each other's old values
                              (.cu:431-459); 1: u, then v from the new u, then w from both
                              (point SOR order; convergent for any factor in (0, 2)) */
+  int image_storage;      /* ORC_IMG_*: the feature images (frame 1, frame 2, warped frame 2)
+                             rounded, wherever they are produced, to f32 and then to 16 bits */
 } orc_params;
+
+enum { ORC_IMG_F64 = 0, ORC_IMG_F16 = 1, ORC_IMG_BF16 = 2 };
+/* x rounded as a feature image of that storage holds it (nearest-even at each step) */
+double orc_img_round(double x, int storage);
 
 typedef struct {
   int n_levels;

@@ -23,6 +23,7 @@ RELAX_PLAIN, RELAX_TILED = 0, 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 EXIT_NONE, EXIT_REFERENCE, EXIT_RESIDUAL = 0, 1, 2
 PIXEL_JACOBI, PIXEL_GS = 0, 1
+IMAGES_F64, IMAGES_F16, IMAGES_BF16 = 0, 1, 2
 
 # Every symbol include/rgbdflow/rgbdflow.h declares.
 EXPORTS = [
@@ -53,7 +54,7 @@ class Params(C.Structure):
                 ("relax_variant", C.c_int), ("inner_exit", C.c_int), ("outer_exit", C.c_int),
                 ("tile_iters", C.c_int), ("tile_w", C.c_int), ("tile_h", C.c_int), ("tile_config", C.c_int),
                 ("graph", C.c_int), ("storage", C.c_int), ("omega", C.c_float), ("residual_tol", C.c_float),
-                ("pixel_update", C.c_int), ("reserved", C.c_int)]
+                ("pixel_update", C.c_int), ("image_storage", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -133,7 +134,8 @@ class RgbdFlowError(RuntimeError):
 def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
                 ratio=0.5, min_width=None, reg=REG_HPB, cw=1.0, dvc=1.0, inner_exit=True, outer_exit=True,
                 relax_variant=RELAX_PLAIN, tile_iters=0, tile_w=0, tile_h=0, tile_config=-1, graph=False,
-                storage=STORE_F32, omega=0.7, residual_tol=1e-4, pixel_update=PIXEL_JACOBI) -> Params:
+                storage=STORE_F32, omega=0.7, residual_tol=1e-4, pixel_update=PIXEL_JACOBI,
+                image_storage=IMAGES_F64) -> Params:
     """inner_exit: True / EXIT_REFERENCE (the reference's max-change test),
     False / EXIT_NONE, or EXIT_RESIDUAL (FP64 relative residual <= residual_tol)."""
     p = Params()
@@ -160,6 +162,7 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.omega = omega
     p.residual_tol = residual_tol
     p.pixel_update = pixel_update
+    p.image_storage = image_storage
     return p
 
 

@@ -74,6 +74,7 @@ struct StepPlan {
   bool tiled_ok = false;
   int cfg = 0, slots = 0, T = 0;  // tiled launch shape: workgroup shape, LDS slots, iterations
   int storage = 0;                 // system record storage of the step's items
+  int img = 0;                     // feature-image storage of the step's items
   int relax_mode = kRelaxFixed;    // RelaxMode of the step's items
   bool any_early = false;
 };
@@ -500,6 +501,7 @@ size_t pack(std::vector<char>& host, const void* p, size_t bytes) {
 void pack_step(std::vector<char>& host, const std::vector<LevelView>& items, StepPlan& st) {
   st.n = (int)items.size();
   st.storage = items.empty() ? 0 : items[0].storage;
+  st.img = items.empty() ? 0 : items[0].img;
   bool fixed = true, gs = !items.empty();
   for (const LevelView& v : items) {
     fixed = fixed && v.omega == .7f && !v.pixel_gs;
@@ -623,17 +625,17 @@ void enqueue_level_solve(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars
   const int checks = k.n_sor / kCheckEvery;
   launch_ctl_reset(s, st.vpix);
   for (int o = 0; o < k.n_outer; o++) {
-    if (k.n_inner == 0) launch_derivs(s, st.vpix, st.pix, 0);  // du = 0 (.cu:776), then the re-warp
+    if (k.n_inner == 0) launch_derivs(s, st.vpix, st.pix, 0, st.img);  // du = 0 (.cu:776), then the re-warp
     for (int h = 0; h < k.n_inner; h++) {
       if (h == 0) {
-        launch_outer_first(s, st.votile, st.otile, k.n_inner * checks, k, k.n_inner > 1 ? 1 : 0);
+        launch_outer_first(s, st.votile, st.otile, k.n_inner * checks, k, k.n_inner > 1 ? 1 : 0, st.img);
       } else {
         launch_phi(s, st.vpix, st.pix);
         launch_system(s, st.vpix, st.pix, k);
       }
       enqueue_relax(c, st, k, h * checks);
     }
-    launch_update_warp(s, st.vpix, st.pix, st.any_early ? 1 : 0);
+    launch_update_warp(s, st.vpix, st.pix, st.any_early ? 1 : 0, st.img);
     if (st.any_early) launch_stat_final(s, st.vpix, o, k.outer_exit);
   }
 }
@@ -978,6 +980,7 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
         v.uv_scale = 1;
       }
       v.storage = p->storage;
+      v.img = p->image_storage;
       set_omega(v, p->omega, p->pixel_update);
       v.reg_type = p->regularization_type;
       v.W_full = (int)cam.xRes;
@@ -1015,6 +1018,8 @@ int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
     return fail(c, RGBDFLOW_ERR_ARG, "bad inner_exit");
   if (p->pixel_update != RGBDFLOW_PIXEL_JACOBI && p->pixel_update != RGBDFLOW_PIXEL_GS)
     return fail(c, RGBDFLOW_ERR_ARG, "bad pixel_update");
+  if (p->image_storage < RGBDFLOW_IMAGES_F64 || p->image_storage > RGBDFLOW_IMAGES_BF16)
+    return fail(c, RGBDFLOW_ERR_ARG, "bad image_storage");
   return validate_iters(c, p->n_outer, p->n_inner, p->n_sor);
 }
 
@@ -1053,8 +1058,8 @@ void enqueue_batch(rgbdflow_ctx* c, const BatchPlan& bp, const SolveScalars& k) 
   for (const HpbBatch& hb : bp.hpb_groups) launch_hpb_batch(s, hb);
   // coarse to fine
   for (const StepPlan& sp : bp.steps) {
-    launch_features(s, sp.vpix, sp.pix);
-    launch_level_entry(s, sp.vpix, sp.pix);
+    launch_features(s, sp.vpix, sp.pix, sp.img);
+    launch_level_entry(s, sp.vpix, sp.pix, sp.img);
     launch_level_weights(s, sp.vpix, sp.pix);
     enqueue_level_solve(c, sp, k);
   }
@@ -1280,6 +1285,8 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
     return fail(c, RGBDFLOW_ERR_ARG, "bad inner_exit");
   if (opts && opts->pixel_update != RGBDFLOW_PIXEL_JACOBI && opts->pixel_update != RGBDFLOW_PIXEL_GS)
     return fail(c, RGBDFLOW_ERR_ARG, "bad pixel_update");
+  if (opts && opts->image_storage != RGBDFLOW_IMAGES_F64)
+    return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "image_storage: the single-level entry points take f64 images");
   HIP_TRY(c, hipSetDevice(c->device));
   const size_t N = (size_t)W * H;
   Work wk;
@@ -1333,7 +1340,7 @@ int rgbdflow_level_system(rgbdflow_ctx* c, int W, int H, float focal_level, cons
   hipStream_t s = c->stream;
   launch_level_weights(s, sp.vpix, sp.pix);
   launch_ctl_reset(s, sp.vpix);
-  launch_derivs(s, sp.vpix, sp.pix, 0);
+  launch_derivs(s, sp.vpix, sp.pix, 0, sp.img);
   launch_phi(s, sp.vpix, sp.pix);
   launch_system(s, sp.vpix, sp.pix, k);
   std::vector<float> rec((size_t)kRec * 4 * N);
@@ -1593,7 +1600,7 @@ int rgbdflow_warp_by_3d_flow(rgbdflow_ctx* c, const double* I1, const double* D1
   std::vector<LevelView> items{view};
   StepPlan sp;
   TRY(one_step(c, "d.desc", items, nullptr, sp));
-  launch_level_entry(c->stream, sp.vpix, sp.pix);
+  launch_level_entry(c->stream, sp.vpix, sp.pix, sp.img);
   TRY(download_planes(c, wk.Iw, N, kFeat, wI));
   TRY(download(c, wD, wk.Dw, N * 8));
   return sync(c);
@@ -1630,7 +1637,7 @@ int rgbdflow_upsample_flow(rgbdflow_ctx* c, const double* u, const double* v, co
   std::vector<LevelView> items{view};
   StepPlan sp;
   TRY(one_step(c, "d.desc", items, nullptr, sp));
-  launch_level_entry(c->stream, sp.vpix, sp.pix);
+  launch_level_entry(c->stream, sp.vpix, sp.pix, sp.img);
   TRY(download(c, uo, wk.u[0], Nd * 8));
   TRY(download(c, vo, wk.u[0] + Nd, Nd * 8));
   TRY(download(c, wo, wk.u[0] + 2 * Nd, Nd * 8));
@@ -1948,6 +1955,7 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   view.uv_scale = 1;
   view.reg_type = p->regularization_type;
   view.storage = p->storage;
+  view.img = p->image_storage;
   set_omega(view, p->omega, p->pixel_update);
   view.W_full = (int)p->cam.xRes;
   view.f_full = p->cam.focalLength;
@@ -1962,8 +1970,8 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   launch_pair_prep(s, pwl, pstart[1]);
   // sceneFlowWithInit.cpp:51: computeHeuristicPb(Im1, dm1) when no map is given
   if (hpb_dev) launch_hpb(s, hpb_level(W, H, i1, d1, h, hs));
-  launch_features(s, sp.vpix, sp.pix);
-  launch_level_entry(s, sp.vpix, sp.pix);
+  launch_features(s, sp.vpix, sp.pix, sp.img);
+  launch_level_entry(s, sp.vpix, sp.pix, sp.img);
   launch_level_weights(s, sp.vpix, sp.pix);
   enqueue_level_solve(c, sp, k);
   HIP_TRY(c, hipEventRecord(c->ev1, s));

@@ -104,6 +104,7 @@ struct LevelView {
   int storage;          // system record storage (records.hpp: f32, f16, bf16)
   float omega;          // relaxation factor (.cu:461: 0.7)
   float omega_c;        // 1 - omega, rounded to float as the update computes it
+  int img;              // feature-image storage of I1, I2, Iw (ImgStorage; uniform over a launch)
   int pixel_gs;         // 0: a pixel's unknowns update from each other's old values (.cu:431-459);
                         // 1: u, then v from the new u, then w from both (point SOR order)
   int W_full;           // camera xRes (full size)
@@ -151,6 +152,9 @@ inline void set_omega(LevelView& v, float omega, int pixel_gs = 0) {
   v.omega_c = 1.f - v.omega;
   v.pixel_gs = pixel_gs ? 1 : 0;
 }
+
+// Feature-image storage (records.hpp: img_ld / img_st).
+enum ImgStorage { kImgF64 = 0, kImgF16 = 1, kImgBF16 = 2 };
 
 // Tiled relaxation kernel variants by update rule.
 enum RelaxMode { kRelaxFree = 0, kRelaxFixed = 1, kRelaxGS = 2 };
@@ -261,15 +265,16 @@ __host__ __device__ inline int otile_blocks(int W, int H) { return ((W + kOtW - 
 // ---------------------------------------------------------------- launchers
 // kernels_solve.hip (blocks = total blocks of the list for that family)
 void launch_ctl_reset(hipStream_t s, const Views& v);
-void launch_derivs(hipStream_t s, const Views& v, int blocks, int n_slots);
+void launch_derivs(hipStream_t s, const Views& v, int blocks, int n_slots, int img);
 void launch_phi(hipStream_t s, const Views& v, int blocks);
 void launch_system(hipStream_t s, const Views& v, int blocks, const SolveScalars& k);
 // derivatives + phi + system of the first inner iteration in one pass
 // (store: keep derivatives, occlusion weights and phi for later inner iterations)
-void launch_outer_first(hipStream_t s, const Views& v, int blocks, int n_slots, const SolveScalars& k, int store);
+void launch_outer_first(hipStream_t s, const Views& v, int blocks, int n_slots, const SolveScalars& k, int store,
+                        int img);
 void launch_relax_plain(hipStream_t s, const Views& v, int blocks, int colour, int slot_base, int n_prev_checks,
                         int check_slot);
-void launch_update_warp(hipStream_t s, const Views& v, int blocks, int want_stat);
+void launch_update_warp(hipStream_t s, const Views& v, int blocks, int want_stat, int img);
 void launch_stat_final(hipStream_t s, const Views& v, int outer, int decide);
 // FP64 residual and energy of every item's system at its current increment
 // (alt: the increment is in du_alt), recorded in SolveCtl; check_slot >= 0
@@ -311,8 +316,8 @@ void launch_interleave(hipStream_t s, const double* src, int N, int C, double* d
 void launch_pair_prep(hipStream_t s, const WorkList<PairPrep>& w, int blocks);
 void launch_pyramid_jobs(hipStream_t s, const WorkList<PyrJob>& w, int blocks);
 inline int pyramid_job_blocks(const PyrJob& j) { return pix_blocks(j.Wd * j.Hd * j.C); }
-void launch_features(hipStream_t s, const Views& v, int blocks);
-void launch_level_entry(hipStream_t s, const Views& v, int blocks);
+void launch_features(hipStream_t s, const Views& v, int blocks, int img);
+void launch_level_entry(hipStream_t s, const Views& v, int blocks, int img);
 void launch_level_weights(hipStream_t s, const Views& v, int blocks);
 void launch_depth_sigma(hipStream_t s, const float* depth, int W, int H, int max_dist, float* sigma);
 void launch_depth_map_level(hipStream_t s, const double* larger, int LW, int LH, double* dst, int W, int H);

@@ -7,6 +7,7 @@
 // Each kernel evaluates the reference's sums in the reference's order, so its
 // f64 results are the same numbers the reference's host code computes.
 #include "internal.hpp"
+#include "records.hpp"
 
 namespace rgbdflow {
 
@@ -101,6 +102,7 @@ __global__ void __launch_bounds__(kBlock) k_pyramid_jobs(WorkList<PyrJob> w) {
 __device__ __forceinline__ double gray_at(const double* bgr, int N, int o) {
   return (double)bgr[o] * .114 + bgr[N + o] * .587 + bgr[2 * N + o] * .299;
 }
+template <int S>
 __device__ __forceinline__ void features_at(const double* bgr, int W, int H, int l, double* out) {
   const int N = W * H;
   const int i = l / W, j = l - i * W;
@@ -109,27 +111,28 @@ __device__ __forceinline__ void features_at(const double* bgr, int W, int H, int
   double gx = 0, gy = 0;
   for (int t = -2; t <= 2; t++) gx += gray_at(bgr, N, i * W + enforce_range(j + t, W)) * f[t + 2];
   for (int t = -2; t <= 2; t++) gy += gray_at(bgr, N, enforce_range(i + t, H) * W + j) * f[t + 2];
-  out[l] = gray_at(bgr, N, l);
-  out[N + l] = gx;
-  out[2 * N + l] = gy;
-  out[3 * (size_t)N + l] = bgr[N + l] - bgr[l];
-  out[4 * (size_t)N + l] = bgr[N + l] - bgr[2 * N + l];
+  img_st<S>(out, l, gray_at(bgr, N, l));
+  img_st<S>(out, N + l, gx);
+  img_st<S>(out, 2 * (size_t)N + l, gy);
+  img_st<S>(out, 3 * (size_t)N + l, bgr[N + l] - bgr[l]);
+  img_st<S>(out, 4 * (size_t)N + l, bgr[N + l] - bgr[2 * N + l]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_features_planes(const double* bgr, int W, int H, double* out) {
   const int l = blockIdx.x * kBlock + threadIdx.x;
-  if (l < W * H) features_at(bgr, W, H, l, out);
+  if (l < W * H) features_at<kImgF64>(bgr, W, H, l, out);
 }
 
 // Features of both frames at a level (sceneFlowMultiscale.cpp:102-103).
+template <int S>
 __global__ void __launch_bounds__(kBlock) k_features(Views w) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int item = work_item(w, blk);
   const LevelView& v = w.items[item];
   const int l = (blk - w.start[item]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
-  features_at(v.P1k, v.W, v.H, l, const_cast<double*>(v.I1));
-  features_at(v.P2k, v.W, v.H, l, const_cast<double*>(v.I2));
+  features_at<S>(v.P1k, v.W, v.H, l, const_cast<double*>(v.I1));
+  features_at<S>(v.P2k, v.W, v.H, l, const_cast<double*>(v.I2));
 }
 
 // Level entry (sceneFlowMultiscale.cpp:120-137).  At the pair's coarsest
@@ -139,6 +142,7 @@ __global__ void __launch_bounds__(kBlock) k_features(Views w) {
 // warpBy3DFlow (opticalFlowUtils.cpp:76-99 -> warpImage, ImageProcessing.h:471-491:
 // out of bounds -> frame 1; then depth -= w).  The warp of pixel p reads only
 // u(p), so resize and warp share one pass.
+template <int S>
 __global__ void __launch_bounds__(kBlock) k_level_entry(Views wl) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int item = work_item(wl, blk);
@@ -150,7 +154,7 @@ __global__ void __launch_bounds__(kBlock) k_level_entry(Views wl) {
     v.u[l] = 0;
     v.u[N + l] = 0;
     v.u[2 * N + l] = 0;
-    for (int k = 0; k < kFeat; k++) v.Iw[(size_t)k * N + l] = v.I2[(size_t)k * N + l];
+    for (int k = 0; k < kFeat; k++) img_st<S>(v.Iw, (size_t)k * N + l, img_ld<S>(v.I2, (size_t)k * N + l));
     v.Dw[l] = v.D2[l];
     return;
   }
@@ -181,15 +185,14 @@ __global__ void __launch_bounds__(kBlock) k_level_entry(Views wl) {
   const double y = i + uu[1], x = j + uu[0];
   double d;
   if (x < 0 || x > W - 1 || y < 0 || y > H - 1) {
-    for (int k = 0; k < kFeat; k++) v.Iw[(size_t)k * N + l] = v.I1[(size_t)k * N + l];
+    for (int k = 0; k < kFeat; k++) img_st<S>(v.Iw, (size_t)k * N + l, img_ld<S>(v.I1, (size_t)k * N + l));
     d = v.D1[l];
   } else {
     const Bilerp b = bilerp_taps(x, y, W, H);
     for (int k = 0; k < kFeat; k++) {
-      const double* s = v.I2 + (size_t)k * N;
       double acc = 0;
-      for (int t = 0; t < 4; t++) acc += s[b.o[t]] * b.s[t];
-      v.Iw[(size_t)k * N + l] = acc;
+      for (int t = 0; t < 4; t++) acc += img_ld<S>(v.I2, (size_t)k * N + b.o[t]) * b.s[t];
+      img_st<S>(v.Iw, (size_t)k * N + l, acc);
     }
     d = 0;
     for (int t = 0; t < 4; t++) d += v.D2[b.o[t]] * b.s[t];
@@ -381,14 +384,16 @@ void launch_pair_prep(hipStream_t s, const WorkList<PairPrep>& w, int blocks) {
 void launch_pyramid_jobs(hipStream_t s, const WorkList<PyrJob>& w, int blocks) {
   hipLaunchKernelGGL(k_pyramid_jobs, dim3(blocks), dim3(kBlock), 0, s, w);
 }
-void launch_features(hipStream_t s, const Views& v, int blocks) {
-  hipLaunchKernelGGL(k_features, dim3(blocks), dim3(kBlock), 0, s, v);
+void launch_features(hipStream_t s, const Views& v, int blocks, int img) {
+  auto fn = img == kImgF16 ? k_features<kImgF16> : img == kImgBF16 ? k_features<kImgBF16> : k_features<kImgF64>;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, s, v);
 }
 void launch_features_planes(hipStream_t s, const double* bgr, int W, int H, double* out5) {
   hipLaunchKernelGGL(k_features_planes, dim3(blocks_for((size_t)W * H)), dim3(kBlock), 0, s, bgr, W, H, out5);
 }
-void launch_level_entry(hipStream_t s, const Views& v, int blocks) {
-  hipLaunchKernelGGL(k_level_entry, dim3(blocks), dim3(kBlock), 0, s, v);
+void launch_level_entry(hipStream_t s, const Views& v, int blocks, int img) {
+  auto fn = img == kImgF16 ? k_level_entry<kImgF16> : img == kImgBF16 ? k_level_entry<kImgBF16> : k_level_entry<kImgF64>;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, s, v);
 }
 void launch_level_weights(hipStream_t s, const Views& v, int blocks) {
   hipLaunchKernelGGL(k_level_weights, dim3(blocks), dim3(kBlock), 0, s, v);

@@ -58,24 +58,24 @@ struct PixDerivs {
   double Dx, Dy, Dt;
 };
 
+template <int S>
 __device__ __forceinline__ void derivs_at(const LevelView& v, int l, int i, int j, PixDerivs& d) {
   const int W = v.W, H = v.H, N = v.N;
   const double g[5] = {1, -8, 0, 8, -1};
   for (int k = 0; k < kFeat; k++) {
-    const RGBDFLOW_GLOBAL double* a = gp(v.I1) + (size_t)k * N;
-    const RGBDFLOW_GLOBAL double* b = gp(v.Iw) + (size_t)k * N;
+    const size_t kN = (size_t)k * N;
     double sx = 0, sy = 0;
     for (int m = 0; m < 5; m++) {
-      const int o = i * W + clampi(j - 2 + m, 0, W - 1);
-      sx += g[m] * (.4 * a[o] + .6 * b[o]);
+      const size_t o = kN + i * W + clampi(j - 2 + m, 0, W - 1);
+      sx += g[m] * (.4 * img_ld<S>(v.I1, o) + .6 * img_ld<S>(v.Iw, o));
     }
     for (int m = 0; m < 5; m++) {
-      const int o = clampi(i - 2 + m, 0, H - 1) * W + j;
-      sy += g[m] * (.4 * a[o] + .6 * b[o]);
+      const size_t o = kN + clampi(i - 2 + m, 0, H - 1) * W + j;
+      sy += g[m] * (.4 * img_ld<S>(v.I1, o) + .6 * img_ld<S>(v.Iw, o));
     }
     d.Ix[k] = div12(sx);
     d.Iy[k] = div12(sy);
-    d.It[k] = b[l] - a[l];
+    d.It[k] = img_ld<S>(v.Iw, kN + l) - img_ld<S>(v.I1, kN + l);
   }
   const double mg = .04;
   double sx = 0, sy = 0;
@@ -100,6 +100,7 @@ __device__ __forceinline__ float occlusion_at(const LevelView& v, int l) {
 
 // Derivatives + occlusion stored for the inner iterations, du = 0 (.cu:776);
 // also zeroes the item's check slots of this outer iteration.
+template <int S>
 __global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int it = work_item(w, blk);
@@ -112,7 +113,7 @@ __global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
   const int N = v.N;
   const int i = l / v.W, j = l - i * v.W;
   PixDerivs d;
-  derivs_at(v, l, i, j, d);
+  derivs_at<S>(v, l, i, j, d);
   for (int k = 0; k < kFeat; k++) {
     gp(v.Ix)[(size_t)k * N + l] = d.Ix[k];
     gp(v.Iy)[(size_t)k * N + l] = d.Iy[k];
@@ -341,6 +342,7 @@ constexpr int kRw = kOtW + 4, kRh = kOtH + 4;  // stencil region (2-pixel border
 constexpr int kUw = kOtW + 2, kUh = kOtH + 2;  // neighbourhood region (1-pixel border)
 constexpr int kPw = kOtW + 1, kPh = kOtH + 1;  // phi region (tile, left column, upper row)
 
+template <int S>
 __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, SolveScalars s, int store) {
   __shared__ double Ms[kFeat + 1][kRh * kRw];  // 0.4*I1 + 0.6*Iw (5 features, depth)
   __shared__ double Us[3][kUh * kUw];          // u
@@ -362,8 +364,6 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
   const int i = y0 + ty, j = x0 + txx;
   const bool inside = i < H && j < W;
   const int l = inside ? i * W + j : 0;
-  const RGBDFLOW_GLOBAL double* I1 = gp(v.I1);
-  const RGBDFLOW_GLOBAL double* Iw = gp(v.Iw);
   const RGBDFLOW_GLOBAL double* D1 = gp(v.D1);
   const RGBDFLOW_GLOBAL double* Dw = gp(v.Dw);
   // the pixel's own values
@@ -371,8 +371,8 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
   float dwt = 0.f, mwt = 0.f;
   if (inside) {
     for (int k = 0; k < kFeat; k++) {
-      i1c[k] = I1[(size_t)k * N + l];
-      iwc[k] = Iw[(size_t)k * N + l];
+      i1c[k] = img_ld<S>(v.I1, (size_t)k * N + l);
+      iwc[k] = img_ld<S>(v.Iw, (size_t)k * N + l);
     }
     d1c = D1[l];
     dwc = Dw[l];
@@ -383,7 +383,8 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
   for (int t = threadIdx.x; t < kRh * kRw; t += kBlock) {
     const int ry = t / kRw, rx = t - ry * kRw;
     const int o = clampi(y0 - 2 + ry, 0, H - 1) * W + clampi(x0 - 2 + rx, 0, W - 1);
-    for (int k = 0; k < kFeat; k++) Ms[k][t] = .4 * I1[(size_t)k * N + o] + .6 * Iw[(size_t)k * N + o];
+    for (int k = 0; k < kFeat; k++)
+      Ms[k][t] = .4 * img_ld<S>(v.I1, (size_t)k * N + o) + .6 * img_ld<S>(v.Iw, (size_t)k * N + o);
     Ms[kFeat][t] = .4 * D1[o] + .6 * Dw[o];
   }
   for (int t = threadIdx.x; t < kUh * kUw; t += kBlock) {
@@ -522,6 +523,7 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
 // .cu:934-941 (u += du, GPU warps: floor base, explicit clamps, f32 weights,
 // out-of-bounds -> frame 1) and the per-block part of the outer statistic
 // sum(du_k^2 * c_k) (.cu:948), accumulated in f64 in a fixed order.
+template <int S>
 __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat) {
   __shared__ double red[kBlock];
   const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -545,7 +547,7 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat)
     gp(v.u)[2 * N + l] = u2;
     const double x = j + u0, y = i + u1;
     if (x < 0 || x > W - 1 || y < 0 || y > H - 1) {
-      for (int k = 0; k < kFeat; k++) gp(v.Iw)[(size_t)k * N + l] = gp(v.I1)[(size_t)k * N + l];
+      for (int k = 0; k < kFeat; k++) img_st<S>(v.Iw, (size_t)k * N + l, img_ld<S>(v.I1, (size_t)k * N + l));
       gp(v.Dw)[l] = gp(v.D1)[l] - u2;
     } else {
       const int x0 = clampi((int)floor(x), 0, W - 1), y0 = clampi((int)floor(y), 0, H - 1);
@@ -554,8 +556,10 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat)
       const float w00 = (1 - ax) * (1 - ay), w01 = (1 - ax) * ay, w10 = ax * (1 - ay), w11 = ax * ay;
       const int o00 = y0 * W + x0, o01 = y1 * W + x0, o10 = y0 * W + x1, o11 = y1 * W + x1;
       for (int k = 0; k < kFeat; k++) {
-        const RGBDFLOW_GLOBAL double* b = gp(v.I2) + (size_t)k * N;
-        gp(v.Iw)[(size_t)k * N + l] = w00 * b[o00] + w01 * b[o01] + w10 * b[o10] + w11 * b[o11];
+        const size_t kN = (size_t)k * N;
+        img_st<S>(v.Iw, kN + l,
+                  w00 * img_ld<S>(v.I2, kN + o00) + w01 * img_ld<S>(v.I2, kN + o01) +
+                      w10 * img_ld<S>(v.I2, kN + o10) + w11 * img_ld<S>(v.I2, kN + o11));
       }
       gp(v.Dw)[l] = w00 * gp(v.D2)[o00] + w01 * gp(v.D2)[o01] + w10 * gp(v.D2)[o10] + w11 * gp(v.D2)[o11] - u2;
     }
@@ -707,8 +711,9 @@ __global__ void __launch_bounds__(kBlock) k_stat_final(Views w, int outer, int d
 void launch_ctl_reset(hipStream_t s, const Views& v) {
   hipLaunchKernelGGL(k_ctl_reset, dim3(v.n), dim3(256), 0, s, v);
 }
-void launch_derivs(hipStream_t s, const Views& v, int blocks, int n_slots) {
-  hipLaunchKernelGGL(k_derivs, dim3(blocks), dim3(kBlock), 0, s, v, n_slots);
+void launch_derivs(hipStream_t s, const Views& v, int blocks, int n_slots, int img) {
+  auto fn = img == kImgF16 ? k_derivs<kImgF16> : img == kImgBF16 ? k_derivs<kImgBF16> : k_derivs<kImgF64>;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, s, v, n_slots);
 }
 void launch_phi(hipStream_t s, const Views& v, int blocks) {
   hipLaunchKernelGGL(k_phi, dim3(blocks), dim3(kBlock), 0, s, v);
@@ -716,15 +721,20 @@ void launch_phi(hipStream_t s, const Views& v, int blocks) {
 void launch_system(hipStream_t s, const Views& v, int blocks, const SolveScalars& k) {
   hipLaunchKernelGGL(k_system, dim3(blocks), dim3(kBlock), 0, s, v, k);
 }
-void launch_outer_first(hipStream_t s, const Views& v, int blocks, int n_slots, const SolveScalars& k, int store) {
-  hipLaunchKernelGGL(k_outer_first, dim3(blocks), dim3(kBlock), 0, s, v, n_slots, k, store);
+void launch_outer_first(hipStream_t s, const Views& v, int blocks, int n_slots, const SolveScalars& k, int store,
+                        int img) {
+  auto fn = img == kImgF16    ? k_outer_first<kImgF16>
+            : img == kImgBF16 ? k_outer_first<kImgBF16>
+                              : k_outer_first<kImgF64>;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, s, v, n_slots, k, store);
 }
 void launch_relax_plain(hipStream_t s, const Views& v, int blocks, int colour, int slot_base, int n_prev_checks,
                         int check_slot) {
   hipLaunchKernelGGL(k_relax_plain, dim3(blocks), dim3(256), 0, s, v, colour, slot_base, n_prev_checks, check_slot);
 }
-void launch_update_warp(hipStream_t s, const Views& v, int blocks, int want_stat) {
-  hipLaunchKernelGGL(k_update_warp, dim3(blocks), dim3(kBlock), 0, s, v, want_stat);
+void launch_update_warp(hipStream_t s, const Views& v, int blocks, int want_stat, int img) {
+  auto fn = img == kImgF16 ? k_update_warp<kImgF16> : img == kImgBF16 ? k_update_warp<kImgBF16> : k_update_warp<kImgF64>;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, s, v, want_stat);
 }
 void launch_stat_final(hipStream_t s, const Views& v, int outer, int decide) {
   hipLaunchKernelGGL(k_stat_final, dim3(v.n), dim3(kBlock), 0, s, v, outer, decide);

@@ -29,6 +29,26 @@ __device__ __forceinline__ unsigned short f32_to_f16_bits(float x) {
 }
 __device__ __forceinline__ float f16_bits_to_f32(unsigned short h) { return __half2float(__ushort_as_half(h)); }
 
+// Feature images ([kFeat][N] planes of I1, I2, Iw) in their storage S: f64,
+// or 16 bits (FP16 / BF16 of the value rounded to f32) in the first quarter
+// of the same buffer.  Loads return f64.  S is a compile-time constant, so
+// the f64 kernels are the reference's loads and stores.
+template <int S>
+__device__ __forceinline__ double img_ld(const double* base, size_t i) {
+  if (S == kImgF64) return gp(base)[i];
+  const unsigned short h = gp(reinterpret_cast<const unsigned short*>(base))[i];
+  return S == kImgF16 ? (double)f16_bits_to_f32(h) : (double)bf16_to_f32(h);
+}
+template <int S>
+__device__ __forceinline__ void img_st(double* base, size_t i, double x) {
+  if (S == kImgF64) {
+    gp(base)[i] = x;
+    return;
+  }
+  const float f = (float)x;
+  gp(reinterpret_cast<unsigned short*>(base))[i] = S == kImgF16 ? f32_to_f16_bits(f) : f32_to_bf16(f);
+}
+
 // Power-of-two scale 2^k as a float (k within the normal range).
 __device__ __forceinline__ float pow2f(int k) { return __uint_as_float((unsigned)(k + 127) << 23); }
 

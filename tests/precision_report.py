@@ -7,7 +7,9 @@ Workload: a batch of mixed-size synthetic pairs (sizes drawn from 320x240,
 segments), 5-level pyramids at ratio 0.5, 15 outer x 50 inner iterations,
 boundary-probability regularisation.  Variants: the storage of the per-pixel
 systems (f32 as the reference, f16 with a per-pixel power-of-two scale, bf16;
-the arithmetic is f32 in every variant) times the relaxation's update rule
+the arithmetic is f32 in every variant), with the feature images (frame 1,
+frame 2, warped frame 2) in f64 as the reference or in the same 16-bit
+format, times the relaxation's update rule
 (the reference's: factor 0.7, a pixel's unknowns from each other's old
 values; point SOR: factor 1.8, u then v then w from the newest values).
 
@@ -81,14 +83,18 @@ def main():
     rules = {"reference": dict(omega=0.7, pixel_update=R.PIXEL_JACOBI),
              "point_sor_1.8": dict(omega=1.8, pixel_update=R.PIXEL_GS)}
     stores = {"f32": R.STORE_F32, "f16": R.STORE_F16, "bf16": R.STORE_BF16}
+    # (system records, feature images) of the batch variants
+    batch_variants = {"f32": (R.STORE_F32, R.IMAGES_F64), "f16": (R.STORE_F16, R.IMAGES_F64),
+                      "bf16": (R.STORE_BF16, R.IMAGES_F64), "f16+f16_images": (R.STORE_F16, R.IMAGES_F16),
+                      "bf16+bf16_images": (R.STORE_BF16, R.IMAGES_BF16)}
     ctx = R.Context(0)
     for p in pairs[:4]:
         for (lw, lh) in R.pyramid_levels(p["W"], p["H"], 0.5, p["min_width"]):
             ctx.autotune_relax(lw, lh, batch=max(1, len(pairs) // 4))
     results, flows = [], {}
     for rname, rule in rules.items():
-        for sname, st in stores.items():
-            p = R.make_params(640, 480, storage=st, **rule, **base)
+        for sname, (st, img) in batch_variants.items():
+            p = R.make_params(640, 480, storage=st, image_storage=img, **rule, **base)
             ctx.batch_load(p, pairs)
             ctx.batch_run(p, stats=True)
             ctx.batch_run(p, stats=False)

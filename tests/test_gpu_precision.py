@@ -7,7 +7,7 @@ these variants change the reference's arithmetic on purpose)."""
 import numpy as np
 import pytest
 
-from helpers import CONFIGS, hpb_for, load_package, make_case, oracle
+from helpers import CONFIGS, hpb_for, level_inputs, load_package, make_case, oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -161,3 +161,44 @@ def test_relax_storage_schedules_agree(ctx, R, system, storage):
     # bf16 keeps 8 significand bits (relative rounding 2^-9), f16 with its scale 11
     rel = 1e-3 if storage == "f16" else 1e-2
     assert np.median(np.abs(a - f)) < rel * np.median(np.abs(f)) + 1e-7
+
+
+# ------------------------------------------------ FP16 / BF16 feature images
+@pytest.mark.parametrize("img", ["f16", "bf16"])
+def test_image_storage_matches_oracle(ctx, R, img):
+    """Feature images (frame 1, frame 2, warped frame 2) stored in FP16 /
+    BF16: the device equals the oracle run with the same rounding, bit for
+    bit, with both relaxation schedules and as sceneFlowWithInit; the flow
+    stays close to the f64-image result (distances reported, loose bound:
+    the variant changes the reference's arithmetic on purpose).  The
+    single-level entry points, which take f64 images, refuse the variant."""
+    O = oracle()
+    W, H = 160, 120
+    pair, im1, d1, im2, d2, sigma, hpb = _case(W, H)
+    kw = dict(n_outer=4, n_sor=30, min_width=20)
+    st = R.IMAGES_F16 if img == "f16" else R.IMAGES_BF16
+    ua, va, wa, sa = O.multiscale(im1, d1, im2, d2, sigma, hpb, O.make_params(W, H, image_storage=st, **kw))
+    for variant in (R.RELAX_PLAIN, R.RELAX_TILED):
+        print(f"{img} images, relax variant {variant}", flush=True)
+        ub, vb, wb, sb = ctx.multiscale(R.make_params(W, H, relax_variant=variant, image_storage=st, **kw),
+                                        im1, d1, im2, d2, sigma, hpb)
+        assert [sb.sweeps_exec[k] for k in range(sa.n_levels)] == [sa.sweeps_exec[k] for k in range(sa.n_levels)]
+        assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
+    ref = ctx.multiscale(R.make_params(W, H, relax_variant=R.RELAX_TILED, **kw), im1, d1, im2, d2, sigma, hpb)
+    epe = np.sqrt((ub - ref[0]) ** 2 + (vb - ref[1]) ** 2)
+    uf, vf, wf, _ = O.multiscale(im1, d1, im2, d2, sigma, hpb, O.make_params(W, H, precision=O.PREC_F64, **kw))
+    epe64 = np.sqrt((ub - uf) ** 2 + (vb - vf) ** 2)
+    print(f"{img} images: EPE vs f64 images median {np.median(epe):.4g} p99 {np.percentile(epe, 99):.4g}; "
+          f"vs FP64 oracle median {np.median(epe64):.4g}; |dw| median {np.median(np.abs(wb - ref[2])):.3g} m")
+    assert np.median(epe) < 0.05 and np.isfinite(ub).all()
+    # sceneFlowWithInit from an estimate
+    est = pair["flow"].astype(np.float64) * 0.7
+    po = O.make_params(W, H, n_outer=3, n_sor=30, image_storage=st)
+    ua, va, wa, _ = O.with_init(im1, d1, im2, d2, sigma, None, est, po)
+    pw = R.make_params(W, H, n_outer=3, n_sor=30, relax_variant=R.RELAX_TILED, image_storage=st)
+    ub, vb, wb, _ = ctx.with_init(pw, im1, d1, im2, d2, est, sigma, None)
+    assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
+    li, (u, v, w), _ = level_inputs(seed=9, W=96, H=72)
+    with pytest.raises(R.RgbdFlowError) as e:
+        ctx.solve_level(li, R.make_params(96, 72, image_storage=st), True, u, v, w)
+    assert e.value.code == R.ERR_UNSUPPORTED

@@ -516,3 +516,36 @@ def test_level_solve_residual_exit():
     _, _, _, st0 = O.level_solve(li, O.make_params(32, 24, residual_exit=True, residual_tol=0.0, **kw), False,
                                  u, v, w)
     assert st0.sweeps_exec == 3 * 200 and st0.residual_checks == 3 * 8 and st0.last_residual > 0
+
+
+def _img_values(rng):
+    """Feature-like values plus the hard cases of a 16-bit rounding: FP16
+    subnormals, exact ties between neighbouring FP16 / BF16 numbers, the FP16
+    overflow boundary, zeros and both signs."""
+    v = [rng.uniform(-1, 1, 2000), rng.uniform(-1e-3, 1e-3, 500), rng.uniform(-1e-6, 1e-6, 500),
+         np.ldexp(rng.integers(1, 2048, 300).astype(np.float64) + 0.5, -24),     # FP16 subnormal ties
+         np.ldexp(rng.integers(1024, 2048, 300).astype(np.float64) + 0.5, -10),  # FP16 ties in [1, 2)
+         np.ldexp(rng.integers(128, 256, 300).astype(np.float64) + 0.5, -7),     # BF16 ties in [1, 2)
+         np.array([0.0, -0.0, 65504.0, 65519.0, 65520.0, -65520.0, 1e5, 2.0 ** -25, 2.0 ** -24, 3 * 2.0 ** -26])]
+    x = np.concatenate(v)
+    return np.concatenate([x, -x])
+
+
+def test_image_storage_rounding():
+    """The feature-image storage variants round like numpy's float16 and
+    torch's bfloat16 conversions of the f32 value (round to nearest even,
+    FP16 subnormals and overflow included); f64 storage leaves values as they
+    are."""
+    import torch
+    O = oracle()
+    x = _img_values(np.random.default_rng(5))
+    f32 = x.astype(np.float32)
+    want16 = f32.astype(np.float16).astype(np.float64)
+    got16 = O.img_round(x, O.IMG_F16)
+    assert np.array_equal(got16, want16, equal_nan=True)
+    wantb = torch.from_numpy(f32).to(torch.bfloat16).to(torch.float64).numpy()
+    gotb = O.img_round(x, O.IMG_BF16)
+    assert np.array_equal(gotb, wantb)
+    assert np.array_equal(O.img_round(x, O.IMG_F64), x)
+    # a planted mistake fails: truncation instead of rounding differs on the ties
+    assert not np.array_equal(np.trunc(f32.astype(np.float64) * 2.0 ** 10) / 2.0 ** 10, got16)
