@@ -353,9 +353,9 @@ def main():
         spec = [(W, H)] * args.served_streams
         frames = serve_frames(R, spec, args.frames)
         served = serve_run(R, spec, frames, args.fps, args.served_seconds, args.served_max_batch, "f16", True,
-                           args.served_warm_outer, autotune=not args.no_autotune)
+                           args.served_warm_outer, autotune=not args.no_autotune, images="f16")
         served.update(streams=args.served_streams, fps_per_stream=args.fps, max_batch=args.served_max_batch,
-                      storage="f16", warm_start=True, warm_n_outer=args.served_warm_outer or 15)
+                      storage="f16", images="f16", warm_start=True, warm_n_outer=args.served_warm_outer or 15)
         served["speedup_vs_plain_single"] = round(served["pairs_per_s"] / plain_pps, 2) if plain_pps else None
         print(f"bench: served {served}", file=sys.stderr, flush=True)
 
