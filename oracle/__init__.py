@@ -344,6 +344,14 @@ def hpb_exp(x):
     return np.array([f(float(t)) for t in np.asarray(x, np.float32).ravel()], np.float32)
 
 
+def hpb_smallest_eigvec(a):
+    """The normals' eigenvector of the smallest eigenvalue of a symmetric PSD 3x3 (f64)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = np.zeros(3)
+    lib().orc_hpb_smallest_eigvec(_p(a), _p(n))
+    return n
+
+
 def system_residual(coef13, x):
     """(relative residual ||r - M x|| / ||r||, energy r'x - 1/2 x'Mx) in f64."""
     c = _f64(coef13)

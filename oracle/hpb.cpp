@@ -25,8 +25,9 @@
  * Third-party arithmetic restated, not copied: the normal of a window is the
  * eigenvector of the smallest eigenvalue of its point covariance
  * (pcl::NormalEstimation::computePointNormal, PCL unpinned version) -- here a
- * two-pass f64 covariance and a cyclic Jacobi eigen-solver (Numerical
- * Recipes, 3rd ed., sec. 11.1), so normals agree with PCL's to rounding, not
+ * two-pass f64 covariance and the smallest eigenvector by power iteration on
+ * the adjugate (PCL's eigen33 takes cross products of rows of A - l0 I, the
+ * adjugate's columns at l0), so normals agree with PCL's to rounding, not
  * bit for bit.  exp() is evaluated by hpb_exp (Cody-Waite reduction by ln 2
  * and a degree-11 Taylor polynomial in f64, rounded to f32 once), which the
  * HIP path restates with the same operations.
@@ -87,47 +88,38 @@ F3 normalized(F3 v) {  // Eigen's normalize(): v / sqrt(|v|^2) when |v|^2 > 0
   return v;
 }
 
-// Cyclic Jacobi on a symmetric 3x3 (f64), at most 12 sweeps, ending before
-// a sweep once the off-diagonal mass is below 1e-15 of the diagonal's;
-// returns the eigenvector of the smallest eigenvalue in n.
+// Eigenvector of the smallest eigenvalue of a symmetric positive
+// semi-definite 3x3 matrix (f64).  The adjugate adj(A) has A's eigenvectors,
+// with eigenvalues l1 l2, l0 l2, l0 l1 (l0 <= l1 <= l2), so A's smallest
+// eigenvector is adj(A)'s dominant one: start from adj(A)'s column of largest
+// norm (the null vector itself when A has rank 2) and apply 8 normalised
+// power iterations of adj(A), each shrinking the error by l0 / l1.
 void smallest_eigvec(double a[3][3], double n[3]) {
-  double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
-  for (int sweep = 0; sweep < 12; sweep++) {
-    // converged: the off-diagonal mass is below 1e-15 of the diagonal's
-    // (the eigenvector then moves by far less than an f32 ulp)
-    const double off = (std::fabs(a[0][1]) + std::fabs(a[0][2])) + std::fabs(a[1][2]);
-    const double dia = (std::fabs(a[0][0]) + std::fabs(a[1][1])) + std::fabs(a[2][2]);
-    if (off <= 1e-15 * dia) break;
-    for (int r = 0; r < 3; r++) {
-      const int p = P[r], q = Q[r];
-      const double apq = a[p][q];
-      if (apq == 0.0) continue;
-      const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
-      const double at = std::fabs(theta);
-      double t = 1.0 / (at + std::sqrt(theta * theta + 1.0));
-      if (theta < 0) t = -t;
-      const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-      const double tau = s / (1.0 + c);
-      a[p][p] = a[p][p] - t * apq;
-      a[q][q] = a[q][q] + t * apq;
-      a[p][q] = a[q][p] = 0.0;
-      for (int k = 0; k < 3; k++) {
-        if (k != p && k != q) {
-          const double akp = a[k][p], akq = a[k][q];
-          a[k][p] = a[p][k] = akp - s * (akq + tau * akp);
-          a[k][q] = a[q][k] = akq + s * (akp - tau * akq);
-        }
-        const double vkp = v[k][p], vkq = v[k][q];
-        v[k][p] = vkp - s * (vkq + tau * vkp);
-        v[k][q] = vkq + s * (vkp - tau * vkq);
-      }
+  double c[3][3];
+  c[0][0] = a[1][1] * a[2][2] - a[1][2] * a[1][2];
+  c[1][1] = a[0][0] * a[2][2] - a[0][2] * a[0][2];
+  c[2][2] = a[0][0] * a[1][1] - a[0][1] * a[0][1];
+  c[0][1] = c[1][0] = a[0][2] * a[1][2] - a[0][1] * a[2][2];
+  c[0][2] = c[2][0] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+  c[1][2] = c[2][1] = a[0][1] * a[0][2] - a[0][0] * a[1][2];
+  int k = 0;
+  double best = -1.0;
+  for (int j = 0; j < 3; j++) {
+    const double s = (c[0][j] * c[0][j] + c[1][j] * c[1][j]) + c[2][j] * c[2][j];
+    if (s > best) {
+      best = s;
+      k = j;
     }
   }
-  int m = 0;
-  if (a[1][1] < a[m][m]) m = 1;
-  if (a[2][2] < a[m][m]) m = 2;
-  for (int k = 0; k < 3; k++) n[k] = v[k][m];
+  double x[3] = {c[0][k], c[1][k], c[2][k]};
+  for (int it = 0; it < 8; it++) {
+    double y[3];
+    for (int i = 0; i < 3; i++) y[i] = (c[i][0] * x[0] + c[i][1] * x[1]) + c[i][2] * x[2];
+    const double s = std::sqrt((y[0] * y[0] + y[1] * y[1]) + y[2] * y[2]);
+    if (!(s > 0)) break;
+    for (int i = 0; i < 3; i++) x[i] = y[i] / s;
+  }
+  for (int i = 0; i < 3; i++) n[i] = x[i];
 }
 
 }  // namespace
@@ -369,5 +361,10 @@ void orc_heuristic_pb(const double* bgr, const double* depth_in, int W, int H, f
 }
 
 float orc_hpb_exp(float x) { return hpb_exp(x); }
+void orc_hpb_smallest_eigvec(const double* a9, double* n3) {
+  double a[3][3];
+  for (int i = 0; i < 9; i++) a[i / 3][i % 3] = a9[i];
+  smallest_eigvec(a, n3);
+}
 
 }  // extern "C"

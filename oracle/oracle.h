@@ -159,6 +159,7 @@ void orc_heuristic_pb(const double* bgr, const double* depth, int W, int H, floa
 /* the normals it uses: [N][3] (0 where none) and their validity flags [N] */
 void orc_hpb_normals(const double* depth, int W, int H, float* normals, unsigned* valid);
 float orc_hpb_exp(float x);
+void orc_hpb_smallest_eigvec(const double* a9, double* n3);
 
 /* Threads of the level solve's row partitions (1: the scalar restatement,
  * the default).  Results are identical for every count. */

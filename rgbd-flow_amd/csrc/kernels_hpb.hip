@@ -11,8 +11,8 @@
 //   k_hpb_normals   normal of each valid point: the points of its 9x9 window
 //                   (staged in LDS per 16x16 pixels)
 //                   within 0.05 x stereoErrorRatio(z) in depth (at least 27),
-//                   f64 two-pass covariance, cyclic Jacobi eigen-solver
-//                   (Numerical Recipes 3rd ed. 11.1; PCL's solver is not
+//                   f64 two-pass covariance, smallest eigenvector by power
+//                   iteration on its adjugate (PCL's solver is not
 //                   restated), flipped towards the camera
 //   k_hpb_rows      running row sums of the 7 channels (colour, normal,
 //                   normal validity), left to right in f32 / u32, through
@@ -87,45 +87,38 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
   }
 }
 
+// Eigenvector of the smallest eigenvalue of a symmetric positive
+// semi-definite 3x3 matrix (f64).  The adjugate adj(A) has A's eigenvectors,
+// with eigenvalues l1 l2, l0 l2, l0 l1 (l0 <= l1 <= l2), so A's smallest
+// eigenvector is adj(A)'s dominant one: start from adj(A)'s column of largest
+// norm (the null vector itself when A has rank 2) and apply 8 normalised
+// power iterations of adj(A), each shrinking the error by l0 / l1.
 __device__ __forceinline__ void smallest_eigvec(double a[3][3], double n[3]) {
-  double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  for (int sweep = 0; sweep < 12; sweep++) {
-    // converged: the off-diagonal mass is below 1e-15 of the diagonal's
-    // (the eigenvector then moves by far less than an f32 ulp)
-    const double off = (fabs(a[0][1]) + fabs(a[0][2])) + fabs(a[1][2]);
-    const double dia = (fabs(a[0][0]) + fabs(a[1][1])) + fabs(a[2][2]);
-    if (off <= 1e-15 * dia) break;
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-      const int p = r < 2 ? 0 : 1, q = r == 0 ? 1 : 2;
-      const double apq = a[p][q];
-      if (apq == 0.0) continue;
-      const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
-      const double at = fabs(theta);
-      double t = 1.0 / (at + sqrt(theta * theta + 1.0));
-      if (theta < 0) t = -t;
-      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-      const double tau = s / (1.0 + c);
-      a[p][p] = a[p][p] - t * apq;
-      a[q][q] = a[q][q] + t * apq;
-      a[p][q] = a[q][p] = 0.0;
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        if (k != p && k != q) {
-          const double akp = a[k][p], akq = a[k][q];
-          a[k][p] = a[p][k] = akp - s * (akq + tau * akp);
-          a[k][q] = a[q][k] = akq + s * (akp - tau * akq);
-        }
-        const double vkp = v[k][p], vkq = v[k][q];
-        v[k][p] = vkp - s * (vkq + tau * vkp);
-        v[k][q] = vkq + s * (vkp - tau * vkq);
-      }
+  double c[3][3];
+  c[0][0] = a[1][1] * a[2][2] - a[1][2] * a[1][2];
+  c[1][1] = a[0][0] * a[2][2] - a[0][2] * a[0][2];
+  c[2][2] = a[0][0] * a[1][1] - a[0][1] * a[0][1];
+  c[0][1] = c[1][0] = a[0][2] * a[1][2] - a[0][1] * a[2][2];
+  c[0][2] = c[2][0] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+  c[1][2] = c[2][1] = a[0][1] * a[0][2] - a[0][0] * a[1][2];
+  int k = 0;
+  double best = -1.0;
+  for (int j = 0; j < 3; j++) {
+    const double s = (c[0][j] * c[0][j] + c[1][j] * c[1][j]) + c[2][j] * c[2][j];
+    if (s > best) {
+      best = s;
+      k = j;
     }
   }
-  int m = 0;
-  if (a[1][1] < a[m][m]) m = 1;
-  if (a[2][2] < a[m][m]) m = 2;
-  for (int k = 0; k < 3; k++) n[k] = v[k][m];
+  double x[3] = {c[0][k], c[1][k], c[2][k]};
+  for (int it = 0; it < 8; it++) {
+    double y[3];
+    for (int i = 0; i < 3; i++) y[i] = (c[i][0] * x[0] + c[i][1] * x[1]) + c[i][2] * x[2];
+    const double s = sqrt((y[0] * y[0] + y[1] * y[1]) + y[2] * y[2]);
+    if (!(s > 0)) break;
+    for (int i = 0; i < 3; i++) x[i] = y[i] / s;
+  }
+  for (int i = 0; i < 3; i++) n[i] = x[i];
 }
 
 struct Cam {

@@ -31,7 +31,8 @@ def _maps(R, S, pair, W, H, mw, shift):
 
 
 def _alone(R, ctx, p, W, H, mw, f1, f2, hpb1):
-    q = R.make_params(W, H, n_outer=p.n_outer, n_sor=p.n_sor, min_width=mw, relax_variant=p.relax_variant)
+    q = R.make_params(W, H, n_outer=p.n_outer, n_sor=p.n_sor, min_width=mw, relax_variant=p.relax_variant,
+                      storage=p.storage, image_storage=p.image_storage)
     im1 = f1[0].astype(np.float64) / 255.0
     im2 = f2[0].astype(np.float64) / 255.0
     u, v, w, _ = ctx.multiscale(q, im1, f1[1].astype(np.float64), im2, f2[1].astype(np.float64), None, hpb1)
@@ -39,15 +40,20 @@ def _alone(R, ctx, p, W, H, mw, f1, f2, hpb1):
 
 
 @pytest.mark.parametrize("max_batch,lanes", [(8, 2), (2, 2), (1, 1)])
-def test_serving_results_equal_alone(R, max_batch, lanes):
+@pytest.mark.parametrize("mode", ["maps", "device_maps", "fp16"])
+def test_serving_results_equal_alone(R, max_batch, lanes, mode):
+    """mode: boundary maps submitted with the frames; made on the device
+    (frames submitted without maps; the pair alone is loaded without them
+    too); or FP16 records and feature images with device maps."""
     S = synth()
     sizes = [(128, 96, 16), (160, 120, 20), (96, 72, 12)]
-    p = R.make_params(128, 96, n_outer=3, n_sor=25, min_width=16, relax_variant=R.RELAX_TILED)
+    extra = dict(storage=R.STORE_F16, image_storage=R.IMAGES_F16) if mode == "fp16" else {}
+    p = R.make_params(128, 96, n_outer=3, n_sor=25, min_width=16, relax_variant=R.RELAX_TILED, **extra)
     streams = []
     with R.Server(p, max_batch=max_batch, lanes=lanes) as srv:
         for k, (W, H, mw) in enumerate(sizes):
             frames, pair = _stream_frames(S, W, H, 200 + k)
-            maps = [_maps(R, S, pair, W, H, mw, sh) for sh in (0, 3)]
+            maps = [_maps(R, S, pair, W, H, mw, sh) if mode == "maps" else None for sh in (0, 3)]
             sid = srv.open(W, H, R.make_params(W, H).cam, mw)
             streams.append((sid, W, H, mw, frames, maps))
         # frames A, B, A per stream, interleaved across streams
@@ -90,7 +96,8 @@ def test_serving_budget_and_errors(R):
 
 
 @pytest.mark.parametrize("max_batch,lanes", [(8, 2), (1, 1)])
-def test_serving_warm_start_equals_with_init(R, max_batch, lanes):
+@pytest.mark.parametrize("mode", ["maps", "device_maps"])
+def test_serving_warm_start_equals_with_init(R, max_batch, lanes, mode):
     """With warm starts a stream's first pair is solved coarse to fine and
     every later pair as sceneFlowWithInit from the stream's previous result
     (here with 2 outer iterations): each result equals rgbdflow_with_init of
@@ -104,7 +111,7 @@ def test_serving_warm_start_equals_with_init(R, max_batch, lanes):
     with R.Server(p, max_batch=max_batch, lanes=lanes, warm_start=True, warm_n_outer=2) as srv:
         for k, (W, H, mw) in enumerate(sizes):
             frames, pair = _stream_frames(S, W, H, 400 + k)
-            maps = [_maps(R, S, pair, W, H, mw, sh) for sh in (0, 3)]
+            maps = [_maps(R, S, pair, W, H, mw, sh) if mode == "maps" else None for sh in (0, 3)]
             sid = srv.open(W, H, R.make_params(W, H).cam, mw)
             streams.append((sid, W, H, mw, frames, maps))
         tickets = []
@@ -129,6 +136,7 @@ def test_serving_warm_start_equals_with_init(R, max_batch, lanes):
                 im1 = frames[a][0].astype(np.float64) / 255.0
                 im2 = frames[b][0].astype(np.float64) / 255.0
                 u, v, w, st = ctx.with_init(q, im1, frames[a][1].astype(np.float64), im2,
-                                            frames[b][1].astype(np.float64), est, None, maps[a][: W * H * 2])
+                                            frames[b][1].astype(np.float64), est, None,
+                                            maps[a][: W * H * 2] if maps[a] is not None else None)
                 ub, vb, wb = results[(sid, t)]
                 assert np.array_equal(u, ub) and np.array_equal(v, vb) and np.array_equal(w, wb), (sid, t)

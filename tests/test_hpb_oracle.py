@@ -34,6 +34,30 @@ def test_exp_is_correctly_rounded_almost_everywhere():
     assert O.hpb_exp([0.0])[0] == 1.0 and O.hpb_exp([-200.0])[0] == 0.0
 
 
+def test_smallest_eigenvector_matches_eigh():
+    """The normals' eigen-solver against numpy's eigh on random symmetric PSD
+    matrices shaped like point covariances (two spreads, one small normal
+    spread, eigenvalue ratio l0 / l1 up to 0.2), plus the rank-2 case; a
+    planted mistake (the dominant instead of the smallest eigenvector)
+    fails."""
+    rng = np.random.default_rng(3)
+    worst = 0.0
+    for t in range(400):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        l1, l2 = rng.uniform(1e-4, 1e-2, 2)
+        l0 = rng.uniform(0, 0.2) * min(l1, l2) if t % 10 else 0.0
+        a = (q * np.array([l0, l1, l2])) @ q.T
+        a = (a + a.T) / 2
+        n = O.hpb_smallest_eigvec(a)
+        w, v = np.linalg.eigh(a)
+        ang = np.arccos(min(1.0, abs(float(n @ v[:, 0])) / np.linalg.norm(n)))
+        worst = max(worst, ang)
+        assert abs(np.linalg.norm(n) - 1) < 1e-12
+        assert np.arccos(min(1.0, abs(float(n @ v[:, 2])))) > 0.1   # not the dominant one
+    assert worst < 1e-6, worst
+    assert not O.hpb_smallest_eigvec(np.zeros((3, 3))).any()
+
+
 @pytest.mark.parametrize("n", [(0, 0, -1.0), (0.3, -0.2, -0.93), (-0.5, 0.1, -0.86)])
 def test_normals_of_a_plane(n):
     W, H = 64, 48
