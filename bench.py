@@ -180,6 +180,8 @@ def main():
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage run of the batch")
     ap.add_argument("--images", choices=["f64", "f16", "bf16"], default="f64",
                     help="serving: storage of the feature images (f64: the reference's precision)")
+    ap.add_argument("--fill-holes", action="store_true",
+                    help="serving: hole-fill each frame's depth on the device (the frames here have no holes)")
     ap.add_argument("--device-hpb", action="store_true",
                     help="serving: submit no boundary maps; the server makes them on the device")
     ap.add_argument("--lanes2", action="store_true", help="also time the batch as two halves on two streams")
@@ -524,7 +526,7 @@ def serve_frames(R, spec, n_frames):
 
 
 def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_outer, autotune=True, lanes=2, seed=7,
-              score_every=8, warmup=1.0, device_hpb=False, images="f64"):
+              score_every=8, warmup=1.0, device_hpb=False, images="f64", fill_holes=False):
     """Streams submit frames at Poisson arrival times (rate fps each); stream k
     walks its size's sequence back and forth from its own starting frame.
     Arrivals run for warmup + seconds; requests that arrive in the first
@@ -538,7 +540,8 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
     st = {"f32": R.STORE_F32, "f16": R.STORE_F16, "bf16": R.STORE_BF16}[storage]
     p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED, graph=True,
                       storage=st, image_storage={"f64": R.IMAGES_F64, "f16": R.IMAGES_F16, "bf16": R.IMAGES_BF16}[images])
-    srv = R.Server(p, max_batch=max_batch, lanes=lanes, warm_start=warm, warm_n_outer=warm_outer)
+    srv = R.Server(p, max_batch=max_batch, lanes=lanes, warm_start=warm, warm_n_outer=warm_outer,
+                   fill_holes=fill_holes)
     if autotune:
         with R.Context(0) as tctx:
             for (w, h) in sizes:
@@ -623,7 +626,8 @@ def serve_main(R, args):
     frames = serve_frames(R, spec, args.frames)
     print(f"bench: serving {len(spec)} streams over {args.seconds} s", file=sys.stderr, flush=True)
     r = serve_run(R, spec, frames, args.fps, args.seconds, args.max_batch, args.storage, args.warm,
-                  args.warm_outer, autotune=not args.no_autotune, device_hpb=args.device_hpb, images=args.images)
+                  args.warm_outer, autotune=not args.no_autotune, device_hpb=args.device_hpb, images=args.images,
+                  fill_holes=args.fill_holes)
     out = {
         "metric": "served scene-flow pairs/s and request latency",
         "value": r["pairs_per_s"],
@@ -634,7 +638,7 @@ def serve_main(R, args):
         "data": "synthetic (seeded frame sequences, smooth per-segment SE(3) motion)",
         "config": dict(streams=args.serve, fps_per_stream=args.fps, arrival_seconds=args.seconds,
                        max_batch=args.max_batch, lanes=2, warm_start=args.warm, warm_n_outer=args.warm_outer,
-                       storage=args.storage, images=args.images, **r),
+                       storage=args.storage, images=args.images, fill_holes=args.fill_holes, **r),
     }
     print(json.dumps(out), flush=True)
 

@@ -166,6 +166,19 @@ int rgbdflow_solve_level(rgbdflow_ctx* ctx, int W, int H, float focal_level, con
                          int nOuter, int nInner, int nSOR, int allowEarlyTermination, const rgbdflow_params* opts,
                          rgbdflow_stats* stats);
 
+/* Depth hole fill on the device: the parallel variant of recursiveMedianFilter
+ * (recursiveMedianFiltering.cpp:34-100, halfwidth 1; sceneFlowMultiscale.cpp:54,61
+ * applies it to both depth maps).  The reference fills one invalid pixel
+ * (depth <= 0) at a time, always one with the most known 3x3 neighbours,
+ * with the upper median of their values rounded to float, in an order its
+ * hash container decides; here every pixel with the most known neighbours
+ * is filled at once, round after round.  Equal to the reference wherever
+ * no two pixels filled in one round are neighbours (isolated holes),
+ * otherwise a deterministic order of its own.  depth, out: [N] metres
+ * (may be the same array); unfilled (may be NULL): the pixels left invalid
+ * because no known pixel surrounds them (the reference asserts there). */
+int rgbdflow_fill_depth_holes(rgbdflow_ctx* ctx, const double* depth, int W, int H, double* out, int* unfilled);
+
 /* computeDepthMapStdevsPointwise (staticDepthNoiseModeling.h:40, .cpp:225-258)
  * with sigma(1 m) = stereoError(1).  depth, stdevs: [N] float. */
 int rgbdflow_depth_stdevs_pointwise(rgbdflow_ctx* ctx, const float* depth, int W, int H, int max_dist,
@@ -326,7 +339,8 @@ int rgbdflow_time_relax_schedule(rgbdflow_ctx* ctx, int W, int H, int batch, con
  * and a worker thread.  Each stream keeps its last frame on the device; every
  * submitted frame after the first forms the pair (last frame, frame), solved
  * by sceneFlowMultiscale exactly as rgbdflow_multiscale would (depth stdevs
- * computed on the device, no hole fill), in whatever batch the worker puts it:
+ * computed on the device; holes filled only with fill_holes), in whatever
+ * batch the worker puts it:
  * the result does not depend on the batch.  With warm_start, every pair after
  * a stream's first is instead solved as rgbdflow_with_init would solve it
  * with the stream's previous result as the estimate. */
@@ -341,7 +355,11 @@ typedef struct rgbdflow_server_config {
                                      (ceLiuOpticalFlow.cpp:200-204) */
   unsigned long long hbm_budget;  /* bytes for stream state and in-flight pairs (0: no limit) */
   int warm_n_outer;               /* outer iterations of warm-started pairs (<= 0: params' n_outer) */
-  int reserved;
+  int fill_holes;                 /* 1: each frame's depth is hole-filled on the device by
+                                     rgbdflow_fill_depth_holes's fill before it is used, as
+                                     sceneFlowMultiscale.cpp:54,61 fills both depth maps; the depth
+                                     stdevs still come from the depth as submitted
+                                     (runFlowTwoFrames.cpp:89-91) */
 } rgbdflow_server_config;
 typedef struct rgbdflow_request_info {
   int has_flow;       /* 0 for a stream's first frame */

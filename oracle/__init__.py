@@ -257,6 +257,14 @@ def median_fill(depth, halfwidth=1):
     return d
 
 
+def median_fill_rounds(depth):
+    """The device's parallel-rounds hole fill: (filled depth, pixels left invalid)."""
+    d = _f64(depth).copy()
+    H, W = d.shape
+    left = lib().orc_median_fill_rounds(_p(d), W, H)
+    return d, left
+
+
 # ------------------------------------------------------------------ solvers
 class LevelInputs:
     """Arrays of one pyramid level, in the reference's DImage layouts."""

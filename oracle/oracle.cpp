@@ -418,6 +418,60 @@ void median_fill(double* data, int W, int H, int hw) {
   }
 }
 
+/* The device's parallel variant of median_fill (halfwidth 1), restated:
+ * rounds r = 1, 2, ...; in each, M = the largest number of known 8-neighbours
+ * (valid, or filled in an earlier round) over the pixels still invalid, and
+ * every such pixel with M known neighbours is filled at once with the upper
+ * median of their values (sorted as float, vals[n / 2], as median_fill);
+ * stops when none is left or M is 0.  Equal to median_fill wherever no two
+ * pixels filled in one round are neighbours (isolated holes); elsewhere the
+ * sequential fill's order (which depends on a hash container) is replaced by
+ * whole rounds.  Returns the number of pixels left invalid. */
+int median_fill_rounds(double* d, int W, int H) {
+  const size_t N = (size_t)W * H;
+  const int kUnfilled = 0x7fffffff;
+  vector<int> st(N);
+  vector<size_t> todo;
+  for (size_t l = 0; l < N; l++) {
+    st[l] = d[l] > 0 ? 0 : kUnfilled;
+    if (d[l] <= 0) todo.push_back(l);
+  }
+  auto known = [&](int i, int j, int r, float* vals) {
+    int n = 0;
+    for (int ii = std::max(0, i - 1); ii <= std::min(H - 1, i + 1); ii++)
+      for (int jj = std::max(0, j - 1); jj <= std::min(W - 1, j + 1); jj++) {
+        const size_t q = (size_t)ii * W + jj;
+        if (st[q] < r) {
+          if (vals) vals[n] = (float)d[q];
+          n++;
+        }
+      }
+    return n;
+  };
+  size_t left = todo.size();
+  for (int r = 1; left > 0; r++) {
+    int M = 0;
+    for (size_t l : todo)
+      if (st[l] == kUnfilled) M = std::max(M, known((int)(l / W), (int)(l % W), r, nullptr));
+    if (M == 0) break;
+    vector<std::pair<size_t, double>> fills;
+    for (size_t l : todo) {
+      if (st[l] != kUnfilled) continue;
+      float vals[9];
+      const int n = known((int)(l / W), (int)(l % W), r, vals);
+      if (n != M) continue;
+      std::sort(vals, vals + n);
+      fills.push_back({l, (double)vals[n / 2]});
+    }
+    for (const auto& f : fills) {
+      d[f.first] = f.second;
+      st[f.first] = r;
+    }
+    left -= fills.size();
+  }
+  return (int)left;
+}
+
 /* staticDepthNoiseModeling.cpp:85-113: depth-boundary mask, then the exact L1
  * distance to the nearest boundary pixel (what cv::distanceTransform(L1)
  * returns), capped: distances above `cap` are reported as cap+1. */
@@ -1037,6 +1091,7 @@ void orc_reg_weights(const float* hpb, const double* depth, int W, int H, int ty
   reg_weights(hpb, depth, W, H, type, out);
 }
 void orc_median_fill(double* depth, int W, int H, int hw) { median_fill(depth, W, H, hw); }
+int orc_median_fill_rounds(double* depth, int W, int H) { return median_fill_rounds(depth, W, H); }
 
 int orc_level_solve(const orc_level_in* in, const orc_params* p, int allow_early, double* u, double* v, double* w,
                     orc_level_stats* st) {

@@ -86,6 +86,8 @@ void orc_depth_mag_weights(const double* depth_level, int Wl, int Hl, int Wfull,
                            float f_full, double* out);
 void orc_reg_weights(const float* hpb, const double* depth, int W, int H, int type, double* out);
 void orc_median_fill(double* depth, int W, int H, int halfwidth);
+/* the device's parallel rounds variant of it (halfwidth 1); returns the pixels left invalid */
+int orc_median_fill_rounds(double* depth, int W, int H);
 
 /* ---- one level: sceneFlowSingleScale (src/sceneFlowSingleScaleJacobiCUDA.cu:660-969) ---- */
 typedef struct {

@@ -29,14 +29,15 @@ IMAGES_F64, IMAGES_F16, IMAGES_BF16 = 0, 1, 2
 EXPORTS = [
     "rgbdflow_default_params", "rgbdflow_create", "rgbdflow_destroy", "rgbdflow_last_error",
     "rgbdflow_bytes_in_use", "rgbdflow_synchronize", "rgbdflow_set_relax_timing", "rgbdflow_pyramid_levels",
-    "rgbdflow_solve_level", "rgbdflow_depth_stdevs_pointwise", "rgbdflow_pair_load", "rgbdflow_pair_run",
-    "rgbdflow_pair_read", "rgbdflow_multiscale", "rgbdflow_with_init", "rgbdflow_gaussian_pyramid",
-    "rgbdflow_depth_map_pyramid", "rgbdflow_im2feature", "rgbdflow_warp_by_3d_flow", "rgbdflow_upsample_flow",
-    "rgbdflow_level_system", "rgbdflow_relax", "rgbdflow_relax_residual", "rgbdflow_heuristic_pb", "rgbdflow_autotune_relax", "rgbdflow_batch_load",
-    "rgbdflow_batch_run", "rgbdflow_batch_read", "rgbdflow_set_relax_schedule",
-    "rgbdflow_time_relax_schedule", "rgbdflow_server_create", "rgbdflow_server_destroy",
-    "rgbdflow_server_last_error", "rgbdflow_stream_open", "rgbdflow_stream_submit", "rgbdflow_stream_poll",
-    "rgbdflow_stream_close", "rgbdflow_server_set_relax_schedule",
+    "rgbdflow_solve_level", "rgbdflow_depth_stdevs_pointwise", "rgbdflow_fill_depth_holes",
+    "rgbdflow_pair_load", "rgbdflow_pair_run", "rgbdflow_pair_read", "rgbdflow_multiscale",
+    "rgbdflow_with_init", "rgbdflow_gaussian_pyramid", "rgbdflow_depth_map_pyramid", "rgbdflow_im2feature",
+    "rgbdflow_warp_by_3d_flow", "rgbdflow_upsample_flow", "rgbdflow_level_system", "rgbdflow_relax",
+    "rgbdflow_relax_residual", "rgbdflow_heuristic_pb", "rgbdflow_autotune_relax", "rgbdflow_batch_load",
+    "rgbdflow_batch_run", "rgbdflow_batch_read", "rgbdflow_set_relax_schedule", "rgbdflow_time_relax_schedule",
+    "rgbdflow_server_create", "rgbdflow_server_destroy", "rgbdflow_server_last_error", "rgbdflow_stream_open",
+    "rgbdflow_stream_submit", "rgbdflow_stream_poll", "rgbdflow_stream_close",
+    "rgbdflow_server_set_relax_schedule",
 ]
 
 
@@ -77,7 +78,7 @@ class PairInput(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("max_batch", C.c_int), ("lanes", C.c_int), ("warm_start", C.c_int),
-                ("hbm_budget", C.c_ulonglong), ("warm_n_outer", C.c_int), ("reserved", C.c_int)]
+                ("hbm_budget", C.c_ulonglong), ("warm_n_outer", C.c_int), ("fill_holes", C.c_int)]
 
 
 class RequestInfo(C.Structure):
@@ -276,6 +277,16 @@ class Context:
             off += w * h
         return out
 
+    def fill_depth_holes(self, depth):
+        """The device's parallel hole fill of a (H, W) depth map (metres, <= 0
+        invalid): (filled f64 map, pixels left invalid)."""
+        d = _f64(depth)
+        H, W = d.shape
+        out = np.empty((H, W))
+        left = C.c_int()
+        self._check(lib().rgbdflow_fill_depth_holes(self.h, _p(d), W, H, _p(out), C.byref(left)))
+        return out, left.value
+
     def heuristic_pb(self, bgr, depth):
         """computeHeuristicPb of one image on the device: (H, W, 3) BGR in [0,1],
         (H, W) metres -> (H, W, 2) f32."""
@@ -446,11 +457,13 @@ class Context:
 class Server:
     """A serving runtime (rgbdflow_server_*): streams of frames, continuous batching."""
 
-    def __init__(self, params, device=0, max_batch=16, lanes=2, hbm_budget=0, warm_start=False, warm_n_outer=0):
+    def __init__(self, params, device=0, max_batch=16, lanes=2, hbm_budget=0, warm_start=False, warm_n_outer=0,
+                 fill_holes=False):
         """warm_start: every pair after a stream's first is solved as
         sceneFlowWithInit from the stream's previous result (warm_n_outer > 0:
-        with that many outer iterations)."""
-        cfg = ServerConfig(device, max_batch, lanes, int(warm_start), hbm_budget, warm_n_outer, 0)
+        with that many outer iterations).  fill_holes: each frame's depth is
+        hole-filled on the device (depth stdevs from the depth as submitted)."""
+        cfg = ServerConfig(device, max_batch, lanes, int(warm_start), hbm_budget, warm_n_outer, int(fill_holes))
         h = C.c_void_p()
         rc = lib().rgbdflow_server_create(C.byref(h), C.byref(cfg), C.byref(params))
         if rc != OK:

@@ -1028,9 +1028,10 @@ int load_pair_device(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const P
   if (!in.f1.im || !in.f1.dm || !in.f2.im || !in.f2.dm)
     return fail(c, RGBDFLOW_ERR_ARG, "bad device frames of pair " + std::to_string(idx));
   TRY(alloc_pair(c, p, idx, in.W, in.H, in.cam, in.min_width, in.f1.hpb != nullptr, in.init != nullptr, ps));
-  ps.sigma_given = false;
+  ps.sigma_given = in.f1.sigma != nullptr;
   const size_t N = (size_t)in.W * in.H, rest = pyramid_elems(ps.L, 1);
   hipStream_t s = c->stream;
+  if (in.f1.sigma) HIP_TRY(c, hipMemcpyAsync(ps.sigma, in.f1.sigma, N * 4, hipMemcpyDeviceToDevice, s));
   HIP_TRY(c, hipMemcpyAsync(ps.im1, in.f1.im, 3 * N * 8, hipMemcpyDeviceToDevice, s));
   HIP_TRY(c, hipMemcpyAsync(ps.dm1, in.f1.dm, N * 8, hipMemcpyDeviceToDevice, s));
   HIP_TRY(c, hipMemcpyAsync(ps.im2, in.f2.im, 3 * N * 8, hipMemcpyDeviceToDevice, s));
@@ -1459,6 +1460,28 @@ int rgbdflow_depth_stdevs_pointwise(rgbdflow_ctx* c, const float* depth, int W, 
   launch_depth_sigma(c->stream, d, W, H, max_dist, s);
   TRY(download(c, out, s, N * 4));
   return sync(c);
+}
+
+int rgbdflow_fill_depth_holes(rgbdflow_ctx* c, const double* depth, int W, int H, double* out, int* unfilled) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (bad_dims(W, H) || !depth || !out) return fail(c, RGBDFLOW_ERR_ARG, "bad hole-fill arguments");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const size_t N = (size_t)W * H;
+  double* d;
+  int *st, *todo, *left;
+  TRY(get(c, "fill.d", N, &d));
+  TRY(get(c, "fill.st", N, &st));
+  TRY(get(c, "fill.todo", 2 * N, &todo));
+  TRY(get(c, "fill.left", 1, &left));
+  TRY(upload(c, d, depth, N * 8));
+  launch_median_fill(c->stream, d, W, H, st, todo, left);
+  HIP_TRY(c, hipGetLastError());
+  TRY(download(c, out, d, N * 8));
+  int l = 0;
+  TRY(download(c, &l, left, sizeof(int)));
+  TRY(sync(c));
+  if (unfilled) *unfilled = l;
+  return RGBDFLOW_OK;
 }
 
 int rgbdflow_gaussian_pyramid(rgbdflow_ctx* c, const double* img, int W, int H, int C, double ratio, int min_width,
@@ -1998,8 +2021,8 @@ int batch_load_device(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const Pa
   // captured graph of it) stays valid: only the frames are copied in.
   std::string sig;
   for (int i = 0; i < n; i++) {
-    const int f[6] = {in[i].W, in[i].H, in[i].min_width, in[i].f1.hpb != nullptr, in[i].init != nullptr,
-                      in[i].f1.pim != nullptr && in[i].f1.pd != nullptr};
+    const int f[7] = {in[i].W, in[i].H, in[i].min_width, in[i].f1.hpb != nullptr, in[i].init != nullptr,
+                      in[i].f1.pim != nullptr && in[i].f1.pd != nullptr, in[i].f1.sigma != nullptr};
     sig.append(reinterpret_cast<const char*>(f), sizeof(f));
     sig.append(reinterpret_cast<const char*>(&in[i].cam), sizeof(in[i].cam));
   }
@@ -2023,6 +2046,7 @@ int batch_load_device(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const Pa
 int pair_outputs(rgbdflow_ctx* c, int i, FrameDev* f2, const double** flow, size_t* pyr_elems) {
   if (!c || !c->ran || i < 0 || i >= (int)c->pairs.size()) return RGBDFLOW_ERR_STATE;
   const PairSlot& ps = c->pairs[i];
+  *f2 = FrameDev{};
   f2->im = ps.im2;
   f2->dm = ps.dm2;
   f2->pim = ps.pim2;

@@ -16,6 +16,8 @@ struct FrameDev {
   const double* pim;  // colour pyramid levels >= 1 ([3][N_k] per level, finest first), or null
   const double* pd;   // depth pyramid levels >= 1, or null
   const float* hpb;   // boundary maps of every level ([N_k][2], finest first), or null
+  const float* sigma; // [N] depth stdevs as frame 1 (from the depth before any hole fill), or null:
+                      // computed from dm
 };
 
 // One pair whose frames are on the device.  Frame 1's pyramids, when given,

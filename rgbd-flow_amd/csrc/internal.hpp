@@ -323,5 +323,8 @@ void launch_depth_sigma(hipStream_t s, const float* depth, int W, int H, int max
 void launch_depth_map_level(hipStream_t s, const double* larger, int LW, int LH, double* dst, int W, int H);
 void launch_features_planes(hipStream_t s, const double* bgr, int W, int H, double* out5);
 void launch_ingest(hipStream_t s, const unsigned char* bgr, const float* depth, int N, double* im, double* dm);
+// depth hole fill in place (the rounds variant); st: [N], todo: [2N] int scratch;
+// left: [1] the pixels left invalid (none is known around them)
+void launch_median_fill(hipStream_t s, double* depth, int W, int H, int* st, int* todo, int* left);
 
 }  // namespace rgbdflow

@@ -365,6 +365,95 @@ __global__ void __launch_bounds__(kBlock) k_ingest(const unsigned char* bgr, con
   dm[l] = depth[l];
 }
 
+// Depth hole fill, the parallel variant of recursiveMedianFilter
+// (recursiveMedianFiltering.cpp:34-100, halfwidth 1).  The reference fills
+// one pixel at a time, always one with the most known 3x3 neighbours, with
+// the upper median of their values (as float), in an order its hash
+// container decides.  Here, in rounds r = 1, 2, ...: M = the most known
+// neighbours (valid, or filled in an earlier round) of any pixel still
+// invalid; every such pixel with M known neighbours is filled at once.  Equal
+// to the reference wherever no two pixels filled in one round touch
+// (isolated holes).  One workgroup per frame: the invalid pixels are listed
+// once, then each round is two passes over the list of the pixels still
+// unfilled, with a barrier between; the second pass moves the pixels it
+// leaves to the next round's list (st[l]: 0 valid, r filled in round r,
+// kUnfilled).  Every round with M > 0 fills at least the pixel that has M, so
+// the rounds end.
+constexpr int kFillThreads = 1024;
+constexpr int kUnfilled = 0x7fffffff;
+
+__device__ __forceinline__ int fill_known(const double* d, const int* st, int W, int H, int l, int r, float* vals) {
+  const int i = l / W, j = l - i * W;
+  int n = 0;
+  for (int ii = max(0, i - 1); ii <= min(H - 1, i + 1); ii++)
+    for (int jj = max(0, j - 1); jj <= min(W - 1, j + 1); jj++) {
+      const int q = ii * W + jj;
+      if (gp(st)[q] < r) {
+        if (vals) vals[n] = (float)gp(d)[q];
+        n++;
+      }
+    }
+  return n;
+}
+
+__global__ void __launch_bounds__(kFillThreads) k_median_fill(double* d, int W, int H, int* st, int* todo,
+                                                              int* left_out) {
+  __shared__ int s_n, s_m, s_max;
+  const int N = W * H;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  for (int l = threadIdx.x; l < N; l += kFillThreads) {
+    // known: depth > 0; filled: depth <= 0 (a NaN is neither, as in the reference)
+    const double x = gp(d)[l];
+    gp(st)[l] = x > 0 ? 0 : kUnfilled;
+    if (x <= 0) gp(todo)[atomicAdd(&s_n, 1)] = l;
+  }
+  __syncthreads();
+  // todo holds two lists of N: the pixels still unfilled, and the next round's
+  int n = s_n, cur = 0;
+  const int n0 = n;
+  for (int r = 1; n > 0 && r <= n0; r++) {
+    const int* list = todo + (size_t)cur * N;
+    int* next = todo + (size_t)(1 - cur) * N;
+    if (threadIdx.x == 0) {
+      s_max = 0;
+      s_m = 0;
+    }
+    __syncthreads();
+    int mx = 0;
+    for (int t = threadIdx.x; t < n; t += kFillThreads) mx = max(mx, fill_known(d, st, W, H, gp(list)[t], r, nullptr));
+    atomicMax(&s_max, mx);
+    __syncthreads();
+    const int M = s_max;
+    if (M == 0) break;
+    for (int t = threadIdx.x; t < n; t += kFillThreads) {
+      const int l = gp(list)[t];
+      float vals[9];
+      const int k = fill_known(d, st, W, H, l, r, vals);
+      if (k != M) {
+        gp(next)[atomicAdd(&s_m, 1)] = l;  // still unfilled: into the next round's list
+        continue;
+      }
+      for (int a = 1; a < k; a++) {  // insertion sort, ascending
+        const float x = vals[a];
+        int b = a - 1;
+        while (b >= 0 && vals[b] > x) {
+          vals[b + 1] = vals[b];
+          b--;
+        }
+        vals[b + 1] = x;
+      }
+      gp(d)[l] = (double)vals[k / 2];
+      gp(st)[l] = r;
+    }
+    __syncthreads();
+    n = s_m;
+    cur = 1 - cur;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *gp(left_out) = n;
+}
+
 inline int blocks_for(size_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -400,6 +489,9 @@ void launch_level_weights(hipStream_t s, const Views& v, int blocks) {
 }
 void launch_depth_sigma(hipStream_t s, const float* depth, int W, int H, int max_dist, float* sigma) {
   hipLaunchKernelGGL(k_depth_sigma, dim3(blocks_for((size_t)W * H)), dim3(kBlock), 0, s, depth, W, H, max_dist, sigma);
+}
+void launch_median_fill(hipStream_t s, double* depth, int W, int H, int* st, int* todo, int* left) {
+  hipLaunchKernelGGL(k_median_fill, dim3(1), dim3(kFillThreads), 0, s, depth, W, H, st, todo, left);
 }
 void launch_depth_map_level(hipStream_t s, const double* larger, int LW, int LH, double* dst, int W, int H) {
   hipLaunchKernelGGL(k_depth_map_level, dim3(blocks_for((size_t)W * H)), dim3(kBlock), 0, s, larger, LW, LH, dst, W,

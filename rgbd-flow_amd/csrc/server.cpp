@@ -49,6 +49,7 @@ constexpr double kPairBytesPerPixel = 640.0;
 struct Frame {
   double *im = nullptr, *dm = nullptr, *pim = nullptr, *pd = nullptr;
   float* hpb = nullptr;
+  float* sigma = nullptr;  // fill_holes: depth stdevs of the depth as submitted
   bool pyr = false;  // pim / pd hold this frame's pyramids
 };
 
@@ -58,6 +59,7 @@ struct StreamState {
   size_t N = 0, rest = 0, hpb_n = 0;
   Frame f[2];
   double* flow = nullptr;  // [3][N] the stream's last result (warm starts)
+  int *fill_st = nullptr, *fill_todo = nullptr, *fill_left = nullptr;  // fill_holes scratch
   bool has_flow = false;
   int cur = -1;          // index of the last frame, -1 before the first
   bool busy = false;     // a request of this stream is in a batch
@@ -111,6 +113,7 @@ struct rgbdflow_server {
   int max_batch = 16;
   rgbdflow_params params{};
   bool warm = false;
+  bool fill = false;              // hole-fill each frame's depth on the device
   rgbdflow_params warm_params{};  // params of warm-started pairs
   unsigned long long budget = 0;
   unsigned long long state_bytes = 0;
@@ -184,7 +187,14 @@ void dispatch(rgbdflow_server* S, Lane& ln) {
     Request& q = ln.reqs[r];
     StreamState& st = *q.st;
     Frame& F = st.f[q.next];
-    launch_ingest(hs, q.in.p, reinterpret_cast<const float*>(q.in.p + q.in.depth_off), (int)st.N, F.im, F.dm);
+    const float* d32 = reinterpret_cast<const float*>(q.in.p + q.in.depth_off);
+    launch_ingest(hs, q.in.p, d32, (int)st.N, F.im, F.dm);
+    if (S->fill) {
+      // the depth stdevs of the depth as submitted (runFlowTwoFrames.cpp:89-91),
+      // then the holes filled (sceneFlowMultiscale.cpp:54,61)
+      launch_depth_sigma(hs, d32, st.W, st.H, 4, F.sigma);
+      launch_median_fill(hs, F.dm, st.W, st.H, st.fill_st, st.fill_todo, st.fill_left);
+    }
     if (q.in.hpb_n) hipMemcpyAsync(F.hpb, q.in.p + q.in.hpb_off, q.in.hpb_n * 4, hipMemcpyDeviceToDevice, hs);
     F.pyr = false;
     if (st.cur >= 0) {
@@ -194,8 +204,9 @@ void dispatch(rgbdflow_server* S, Lane& ln) {
       pd.H = st.H;
       pd.min_width = st.min_width;
       pd.cam = st.cam;
-      pd.f1 = FrameDev{P.im, P.dm, P.pyr ? P.pim : nullptr, P.pyr ? P.pd : nullptr, q.in.hpb_n ? P.hpb : nullptr};
-      pd.f2 = FrameDev{F.im, F.dm, nullptr, nullptr, nullptr};
+      pd.f1 = FrameDev{P.im, P.dm, P.pyr ? P.pim : nullptr, P.pyr ? P.pd : nullptr, q.in.hpb_n ? P.hpb : nullptr,
+                       S->fill ? P.sigma : nullptr};
+      pd.f2 = FrameDev{F.im, F.dm, nullptr, nullptr, nullptr, nullptr};
       pd.init = q.warm ? st.flow : nullptr;
       ln.pair_of[r] = (int)pairs.size();
       pairs.push_back(pd);
@@ -366,6 +377,7 @@ int rgbdflow_server_create(rgbdflow_server** out, const rgbdflow_server_config* 
   S->budget = cfg->hbm_budget;
   S->params = *params;
   S->warm = cfg->warm_start != 0;
+  S->fill = cfg->fill_holes != 0;
   S->warm_params = *params;
   if (cfg->warm_n_outer > 0) S->warm_params.n_outer = cfg->warm_n_outer;
   const int n_lanes = cfg->lanes > 0 ? cfg->lanes : 2;
@@ -434,7 +446,8 @@ int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera
   // one allocation: two frames (planes, pyramids, boundary maps) and the last flow
   const size_t frame_bytes = r256(3 * st->N * 8) + r256(st->N * 8) + r256(3 * st->rest * 8 + 8) +
                              r256(st->rest * 8 + 8) + r256(st->hpb_n * 4);
-  const size_t bytes = 2 * frame_bytes + (S->warm ? r256(3 * st->N * 8) : 0);
+  const size_t fill_bytes = S->fill ? 2 * r256(st->N * 4) + r256(st->N * 4) + r256(2 * st->N * 4) + r256(4) : 0;
+  const size_t bytes = 2 * frame_bytes + (S->warm ? r256(3 * st->N * 8) : 0) + fill_bytes;
   {
     std::lock_guard<std::mutex> lk(S->m);
     if (S->budget && S->state_bytes + bytes > S->budget)
@@ -457,6 +470,12 @@ int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera
     F.hpb = reinterpret_cast<float*>(take(st->hpb_n * 4));
   }
   if (S->warm) st->flow = reinterpret_cast<double*>(take(3 * st->N * 8));
+  if (S->fill) {
+    for (Frame& F : st->f) F.sigma = reinterpret_cast<float*>(take(st->N * 4));
+    st->fill_st = reinterpret_cast<int*>(take(st->N * 4));
+    st->fill_todo = reinterpret_cast<int*>(take(2 * st->N * 4));
+    st->fill_left = reinterpret_cast<int*>(take(4));
+  }
   std::lock_guard<std::mutex> lk(S->m);
   S->state_bytes += st->bytes;
   *stream_id = S->next_stream++;

@@ -140,3 +140,50 @@ def test_serving_warm_start_equals_with_init(R, max_batch, lanes, mode):
                                             maps[a][: W * H * 2] if maps[a] is not None else None)
                 ub, vb, wb = results[(sid, t)]
                 assert np.array_equal(u, ub) and np.array_equal(v, vb) and np.array_equal(w, wb), (sid, t)
+
+
+@pytest.mark.parametrize("max_batch,lanes", [(8, 2), (1, 1)])
+def test_serving_fill_holes_equals_alone(R, max_batch, lanes):
+    """fill_holes: frames whose depth has holes; each result equals
+    rgbdflow_multiscale of the pair alone with both depth maps filled by
+    rgbdflow_fill_depth_holes and frame 1's depth stdevs computed from its
+    depth as submitted (holes included), bit for bit."""
+    S = synth()
+    sizes = [(128, 96, 16), (96, 72, 12)]
+    p = R.make_params(128, 96, n_outer=3, n_sor=25, min_width=16, relax_variant=R.RELAX_TILED)
+    streams = []
+    with R.Server(p, max_batch=max_batch, lanes=lanes, fill_holes=True) as srv:
+        for k, (W, H, mw) in enumerate(sizes):
+            frames, pair = _stream_frames(S, W, H, 500 + k)
+            rng = np.random.default_rng(k)
+            holed = []
+            for img, d in frames:
+                d = d.copy()
+                d[rng.random(d.shape) < 0.03] = 0.0
+                d[H // 3:H // 3 + 5, W // 4:W // 4 + 7] = 0.0
+                holed.append((img, d))
+            maps = [_maps(R, S, pair, W, H, mw, sh) for sh in (0, 3)]
+            sid = srv.open(W, H, R.make_params(W, H).cam, mw)
+            streams.append((sid, W, H, mw, holed, maps))
+        tickets = []
+        for t, fi in enumerate([0, 1, 0]):
+            for (sid, W, H, mw, frames, maps) in streams:
+                tickets.append((sid, t, srv.submit(sid, frames[fi][0], frames[fi][1], maps[fi])))
+        results = {}
+        for (sid, t, tk) in tickets:
+            print(f"poll stream {sid} frame {t}", flush=True)
+            info, flow = srv.poll(tk, wait_ms=60000)
+            assert info["status"] == 0 and info["has_flow"] == (t > 0)
+            results[(sid, t)] = flow
+    with R.Context(0) as ctx:
+        for (sid, W, H, mw, frames, maps) in streams:
+            q = R.make_params(W, H, n_outer=p.n_outer, n_sor=p.n_sor, min_width=mw, relax_variant=p.relax_variant)
+            for t, (a, b) in [(1, (0, 1)), (2, (1, 0))]:
+                fa, la = ctx.fill_depth_holes(frames[a][1].astype(np.float64))
+                fb, lb = ctx.fill_depth_holes(frames[b][1].astype(np.float64))
+                assert la == 0 and lb == 0
+                sa = ctx.depth_stdevs(frames[a][1])
+                u, v, w, _ = ctx.multiscale(q, frames[a][0].astype(np.float64) / 255.0, fa,
+                                            frames[b][0].astype(np.float64) / 255.0, fb, sa, maps[a])
+                ub, vb, wb = results[(sid, t)]
+                assert np.array_equal(u, ub) and np.array_equal(v, vb) and np.array_equal(w, wb), (sid, t)

@@ -549,3 +549,48 @@ def test_image_storage_rounding():
     assert np.array_equal(O.img_round(x, O.IMG_F64), x)
     # a planted mistake fails: truncation instead of rounding differs on the ties
     assert not np.array_equal(np.trunc(f32.astype(np.float64) * 2.0 ** 10) / 2.0 ** 10, got16)
+
+
+def _plane(W, H, rng):
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    return 1.5 + 0.002 * xs - 0.001 * ys + rng.normal(0, 1e-4, (H, W))
+
+
+def test_median_fill_rounds():
+    """The parallel-rounds hole fill: equal to the sequential fill
+    (median_fill, the reference's order) on isolated holes; on blobs and
+    border strips of a noisy plane it fills everything with values near the
+    plane, as the sequential fill does; hole-free input is left as it is and
+    an all-invalid frame stays invalid."""
+    O = oracle()
+    rng = np.random.default_rng(9)
+    W, H = 64, 48
+    d = _plane(W, H, rng)
+    iso = d.copy()
+    ys, xs = np.mgrid[1:H - 1:3, 1:W - 1:3]
+    iso[ys.ravel(), xs.ravel()] = 0.0            # holes 3 apart: no two touch
+    a = O.median_fill(iso)
+    b, left = O.median_fill_rounds(iso)
+    assert left == 0 and np.array_equal(a, b)
+    blob = d.copy()
+    blob[10:20, 10:25] = 0.0
+    blob[:, :3] = 0.0
+    blob[30:33, 40:60] = -1.0
+    blob[rng.random((H, W)) < 0.05] = 0.0
+    a = O.median_fill(blob)
+    b, left = O.median_fill_rounds(blob)
+    hole = blob <= 0
+    assert left == 0 and (b > 0).all() and np.array_equal(b[~hole], blob[~hole])
+    assert np.abs(b - d)[hole].max() < 0.05 and np.abs(a - d)[hole].max() < 0.05
+    assert np.array_equal(b[hole], b[hole].astype(np.float32).astype(np.float64))   # medians are floats
+    same, left = O.median_fill_rounds(d)
+    assert left == 0 and np.array_equal(same, d)
+    z, left = O.median_fill_rounds(np.zeros((H, W)))
+    assert left == W * H and not z.any()
+    # a 3 x 3 hole in a constant-gradient field fills ring by ring: the
+    # centre (no known neighbour at first) comes last, from its 8 neighbours
+    g = np.add.outer(np.arange(7.0), np.arange(7.0)) + 1.0
+    h = g.copy()
+    h[2:5, 2:5] = 0.0
+    b, left = O.median_fill_rounds(h)
+    assert left == 0 and b[3, 3] == np.float32(np.sort(b[2:5, 2:5].ravel()[[0, 1, 2, 3, 5, 6, 7, 8]])[4])
