@@ -196,7 +196,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: the scene-flow solve runs on one GPU "
-              "(pairs are independent; see DESIGN.md on sharding)", file=sys.stderr)
+              "(frame pairs are independent: one server or batch per GPU)", file=sys.stderr)
         sys.exit(2)
 
     R = _package()

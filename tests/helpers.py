@@ -38,9 +38,9 @@ def oracle():
 
 
 # The workloads of the benchmark configurations, as parity cases.
-CONFIGS = {
-    "cfg0_320x240": dict(W=320, H=240, f=262.5, cx=159.5, cy=119.5, segments=4, min_width=40, n_outer=10, n_sor=30),
-    "cfg1_640x480": dict(W=640, H=480, f=525.0, cx=319.5, cy=239.5, segments=6, min_width=40, n_outer=15, n_sor=50),
+WORKLOADS = {
+    "320x240_10x30": dict(W=320, H=240, f=262.5, cx=159.5, cy=119.5, segments=4, min_width=40, n_outer=10, n_sor=30),
+    "640x480_15x50": dict(W=640, H=480, f=525.0, cx=319.5, cy=239.5, segments=6, min_width=40, n_outer=15, n_sor=50),
 }
 
 

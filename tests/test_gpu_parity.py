@@ -12,7 +12,7 @@ reported.
 import numpy as np
 import pytest
 
-from helpers import CONFIGS, hpb_for, level_inputs, load_package, make_case, oracle
+from helpers import WORKLOADS, hpb_for, level_inputs, load_package, make_case, oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -141,7 +141,7 @@ def test_level_solve(ctx, R, allow_early, exits):
 
 def _pair_parity(ctx, R, name, seed, exits=True):
     O = oracle()
-    cfg = CONFIGS[name]
+    cfg = WORKLOADS[name]
     W, H = cfg["W"], cfg["H"]
     pair, im1, d1, im2, d2, sigma = make_case(W, H, seed, cfg["segments"], cfg["f"], cfg["cx"], cfg["cy"])
     hpb, lv = hpb_for(pair, W, H, 0.5, cfg["min_width"])
@@ -167,16 +167,16 @@ def _pair_parity(ctx, R, name, seed, exits=True):
     return (ua, va, wa), (ub, vb, wb), pair
 
 
-def test_multiscale_cfg0(ctx, R):
-    _pair_parity(ctx, R, "cfg0_320x240", seed=11)
+def test_multiscale_320x240(ctx, R):
+    _pair_parity(ctx, R, "320x240_10x30", seed=11)
 
 
-def test_multiscale_cfg0_no_exits(ctx, R):
-    _pair_parity(ctx, R, "cfg0_320x240", seed=12, exits=False)
+def test_multiscale_320x240_no_exits(ctx, R):
+    _pair_parity(ctx, R, "320x240_10x30", seed=12, exits=False)
 
 
-def test_multiscale_cfg1(ctx, R):
-    ref, got, pair = _pair_parity(ctx, R, "cfg1_640x480", seed=21)
+def test_multiscale_640x480(ctx, R):
+    ref, got, pair = _pair_parity(ctx, R, "640x480_15x50", seed=21)
     gt = pair["flow"]
     epe = np.sqrt((got[0] - gt[..., 0]) ** 2 + (got[1] - gt[..., 1]) ** 2)
     # the solve recovers the synthetic motion (occluded slivers at segment

@@ -7,7 +7,7 @@ these variants change the reference's arithmetic on purpose)."""
 import numpy as np
 import pytest
 
-from helpers import CONFIGS, hpb_for, level_inputs, load_package, make_case, oracle
+from helpers import WORKLOADS, hpb_for, level_inputs, load_package, make_case, oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -46,7 +46,7 @@ def test_omega_matches_oracle(ctx, R, variant):
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 def test_storage_variants(ctx, R, storage):
     O = oracle()
-    cfg = CONFIGS["cfg0_320x240"]
+    cfg = WORKLOADS["320x240_10x30"]
     W, H = cfg["W"], cfg["H"]
     pair, im1, d1, im2, d2, sigma = make_case(W, H, 11, cfg["segments"], cfg["f"], cfg["cx"], cfg["cy"])
     hpb, lv = hpb_for(pair, W, H, 0.5, cfg["min_width"])

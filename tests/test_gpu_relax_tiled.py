@@ -7,7 +7,7 @@ iterations when the halo is 2T wide, and the per-pixel arithmetic is shared.
 import numpy as np
 import pytest
 
-from helpers import CONFIGS, hpb_for, level_inputs, load_package, make_case, oracle
+from helpers import WORKLOADS, hpb_for, level_inputs, load_package, make_case, oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -77,10 +77,10 @@ def test_tiled_level_solve_exits(ctx, R):
     assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
 
 
-@pytest.mark.parametrize("name,seed", [("cfg0_320x240", 11), ("cfg1_640x480", 21)])
+@pytest.mark.parametrize("name,seed", [("320x240_10x30", 11), ("640x480_15x50", 21)])
 def test_tiled_multiscale_matches_oracle(ctx, R, name, seed):
     O = oracle()
-    cfg = CONFIGS[name]
+    cfg = WORKLOADS[name]
     W, H = cfg["W"], cfg["H"]
     pair, im1, d1, im2, d2, sigma = make_case(W, H, seed, cfg["segments"], cfg["f"], cfg["cx"], cfg["cy"])
     hpb, lv = hpb_for(pair, W, H, 0.5, cfg["min_width"])
