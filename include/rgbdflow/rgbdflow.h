@@ -179,6 +179,21 @@ int rgbdflow_solve_level(rgbdflow_ctx* ctx, int W, int H, float focal_level, con
  * because no known pixel surrounds them (the reference asserts there). */
 int rgbdflow_fill_depth_holes(rgbdflow_ctx* ctx, const double* depth, int W, int H, double* out, int* unfilled);
 
+/* Colour-guided fill of small holes on the device: the parallel variant of
+ * recursiveMedianFilterSmallInvalidRegionsUsingColor
+ * (recursiveMedianFiltering.cpp:127-222, halfwidth 1;
+ * sceneFlowMultiscale.cpp:78-83 applies it, max_region 60, to the depth the
+ * boundary maps' depth pyramid is built from).  Only 4-connected invalid
+ * regions of at most max_region (<= 63) pixels are filled, each pixel with
+ * the mean of its known 3x3 neighbours weighted by
+ * exp(-|colour difference|^2 / 0.6); the reference goes region by region,
+ * pixel by pixel in hash-container order, here every region's pixels with the
+ * region's most known neighbours are filled at once, round after round.
+ * bgr: [N][3] in [0, 1]; depth, out: [N] metres; unfilled (may be NULL): the
+ * pixels of small regions left invalid (no known neighbour). */
+int rgbdflow_fill_small_holes_colour(rgbdflow_ctx* ctx, const double* depth, const double* bgr, int W, int H,
+                                     int max_region, double* out, int* unfilled);
+
 /* computeDepthMapStdevsPointwise (staticDepthNoiseModeling.h:40, .cpp:225-258)
  * with sigma(1 m) = stereoError(1).  depth, stdevs: [N] float. */
 int rgbdflow_depth_stdevs_pointwise(rgbdflow_ctx* ctx, const float* depth, int W, int H, int max_dist,
@@ -239,6 +254,9 @@ typedef struct rgbdflow_pair_input {
                           OpticalFlow::sceneFlowWithInit (sceneFlowWithInit.cpp:7-81, as
                           rgbdflow_with_init) -- full resolution only, from est, no outer
                           early exit; hpb then needs only its level-0 map */
+  const double* hpb_dm1; /* [N] the depth that boundary maps made on the device start from (the
+                            reference's colour-filled raw depth, sceneFlowMultiscale.cpp:78-83, e.g.
+                            rgbdflow_fill_small_holes_colour of the depth as measured), or NULL: dm1 */
 } rgbdflow_pair_input;
 int rgbdflow_batch_load(rgbdflow_ctx* ctx, const rgbdflow_params* params, int n, const rgbdflow_pair_input* pairs);
 int rgbdflow_batch_run(rgbdflow_ctx* ctx, const rgbdflow_params* params, rgbdflow_stats* stats);

@@ -257,6 +257,23 @@ def median_fill(depth, halfwidth=1):
     return d
 
 
+def colour_fill_rounds(depth, bgr, max_region=60):
+    """The device's rounds variant of the colour-guided small-hole fill:
+    (filled depth, pixels of small regions left invalid).  bgr: (H, W, 3)."""
+    d = _f64(depth).copy()
+    H, W = d.shape
+    planes = np.ascontiguousarray(np.moveaxis(_f64(bgr), -1, 0))
+    left = lib().orc_colour_fill_rounds(_p(d), _p(planes), W, H, max_region)
+    return d, left
+
+
+def fill_exp(x):
+    f = lib().orc_fill_exp
+    f.restype = C.c_double
+    f.argtypes = [C.c_double]
+    return np.array([f(float(t)) for t in np.ravel(x)])
+
+
 def median_fill_rounds(depth):
     """The device's parallel-rounds hole fill: (filled depth, pixels left invalid)."""
     d = _f64(depth).copy()

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <functional>
 #include <thread>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -463,6 +464,111 @@ int median_fill_rounds(double* d, int W, int H) {
       std::sort(vals, vals + n);
       fills.push_back({l, (double)vals[n / 2]});
     }
+    for (const auto& f : fills) {
+      d[f.first] = f.second;
+      st[f.first] = r;
+    }
+    left -= fills.size();
+  }
+  return (int)left;
+}
+
+/* exp(x) of the device's colour fill, the same operations: Cody-Waite
+ * reduction by ln 2, Taylor series to degree 13 in f64, times 2^n. */
+static double fill_exp(double x) {
+  if (!(x > -700.0)) return 0.0;
+  const double n = std::rint(x * 1.4426950408889634);
+  const double r = (x - n * 0.693145751953125) - n * 1.4286068203094173e-06;
+  double p = 1.0 / 6227020800.0;
+  p = p * r + 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return std::ldexp(p, (int)n);
+}
+
+/* The device's parallel variant of the colour-guided fill of small holes
+ * (recursiveMedianFiltering.cpp:127-222, halfwidth 1), restated: the
+ * 4-connected invalid regions of at most max_region pixels are filled
+ * together in rounds; in each, a region's pixels with the region's largest
+ * number of known 3x3 neighbours (valid, or filled in an earlier round; at
+ * least one) are filled at once with the mean of those neighbours' depths
+ * weighted by exp(-|colour difference|^2 / 0.6) (float sums, in window
+ * order, as the reference); ends when a round fills nothing.  bgr: [3][N]
+ * planes.  Returns the pixels of small regions left invalid. */
+int colour_fill_rounds(double* d, const double* bgr, int W, int H, int max_region) {
+  const size_t N = (size_t)W * H;
+  const int kUnfilled = 0x7fffffff;
+  // regions: 4-connected invalid pixels, labelled by their smallest index
+  vector<int> label(N, -2);
+  vector<int> stack;
+  for (size_t l0 = 0; l0 < N; l0++) {
+    if (!(d[l0] <= 0) || label[l0] != -2) continue;
+    vector<int> comp;
+    stack.assign(1, (int)l0);
+    label[l0] = (int)l0;
+    while (!stack.empty()) {
+      const int p = stack.back();
+      stack.pop_back();
+      comp.push_back(p);
+      const int i = p / W, j = p % W;
+      const int nb[4] = {i > 0 ? p - W : -1, j > 0 ? p - 1 : -1, j < W - 1 ? p + 1 : -1, i < H - 1 ? p + W : -1};
+      for (int m : nb)
+        if (m >= 0 && d[m] <= 0 && label[m] == -2) {
+          label[m] = (int)l0;
+          stack.push_back(m);
+        }
+    }
+    if ((int)comp.size() > max_region)
+      for (int p : comp) label[p] = -1;
+  }
+  vector<int> st(N);
+  vector<size_t> todo;
+  for (size_t l = 0; l < N; l++) {
+    st[l] = d[l] > 0 ? 0 : kUnfilled;
+    if (label[l] >= 0) todo.push_back(l);
+  }
+  auto known = [&](size_t l, int r) {
+    const int i = (int)(l / W), j = (int)(l % W);
+    int n = 0;
+    for (int ii = std::max(0, i - 1); ii <= std::min(H - 1, i + 1); ii++)
+      for (int jj = std::max(0, j - 1); jj <= std::min(W - 1, j + 1); jj++) n += st[(size_t)ii * W + jj] < r;
+    return n;
+  };
+  size_t left = todo.size();
+  for (int r = 1; left > 0; r++) {
+    std::unordered_map<int, int> cmax;
+    for (size_t l : todo)
+      if (st[l] == kUnfilled) cmax[label[l]] = std::max(cmax[label[l]], known(l, r));
+    vector<std::pair<size_t, double>> fills;
+    for (size_t l : todo) {
+      if (st[l] != kUnfilled) continue;
+      const int k = known(l, r);
+      if (k == 0 || k != cmax[label[l]]) continue;
+      const int i = (int)(l / W), j = (int)(l % W);
+      float zs = 0.f, ws = 0.f;
+      for (int ii = std::max(0, i - 1); ii <= std::min(H - 1, i + 1); ii++)
+        for (int jj = std::max(0, j - 1); jj <= std::min(W - 1, j + 1); jj++) {
+          const size_t q = (size_t)ii * W + jj;
+          if (st[q] < r) {
+            const double e0 = bgr[l] - bgr[q], e1 = bgr[N + l] - bgr[N + q], e2 = bgr[2 * N + l] - bgr[2 * N + q];
+            const float w = (float)fill_exp(-(e0 * e0 + e1 * e1 + e2 * e2) / .6);
+            zs = (float)((double)zs + d[q] * w);
+            ws += w;
+          }
+        }
+      fills.push_back({l, (double)(zs / ws)});
+    }
+    if (fills.empty()) break;
     for (const auto& f : fills) {
       d[f.first] = f.second;
       st[f.first] = r;
@@ -1092,6 +1198,10 @@ void orc_reg_weights(const float* hpb, const double* depth, int W, int H, int ty
 }
 void orc_median_fill(double* depth, int W, int H, int hw) { median_fill(depth, W, H, hw); }
 int orc_median_fill_rounds(double* depth, int W, int H) { return median_fill_rounds(depth, W, H); }
+int orc_colour_fill_rounds(double* depth, const double* bgr, int W, int H, int max_region) {
+  return colour_fill_rounds(depth, bgr, W, H, max_region);
+}
+double orc_fill_exp(double x) { return fill_exp(x); }
 
 int orc_level_solve(const orc_level_in* in, const orc_params* p, int allow_early, double* u, double* v, double* w,
                     orc_level_stats* st) {

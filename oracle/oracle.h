@@ -88,6 +88,9 @@ void orc_reg_weights(const float* hpb, const double* depth, int W, int H, int ty
 void orc_median_fill(double* depth, int W, int H, int halfwidth);
 /* the device's parallel rounds variant of it (halfwidth 1); returns the pixels left invalid */
 int orc_median_fill_rounds(double* depth, int W, int H);
+/* the device's rounds variant of the colour-guided fill of small holes; bgr [3][N] planes */
+int orc_colour_fill_rounds(double* depth, const double* bgr, int W, int H, int max_region);
+double orc_fill_exp(double x);
 
 /* ---- one level: sceneFlowSingleScale (src/sceneFlowSingleScaleJacobiCUDA.cu:660-969) ---- */
 typedef struct {

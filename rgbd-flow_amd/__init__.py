@@ -29,7 +29,7 @@ IMAGES_F64, IMAGES_F16, IMAGES_BF16 = 0, 1, 2
 EXPORTS = [
     "rgbdflow_default_params", "rgbdflow_create", "rgbdflow_destroy", "rgbdflow_last_error",
     "rgbdflow_bytes_in_use", "rgbdflow_synchronize", "rgbdflow_set_relax_timing", "rgbdflow_pyramid_levels",
-    "rgbdflow_solve_level", "rgbdflow_depth_stdevs_pointwise", "rgbdflow_fill_depth_holes",
+    "rgbdflow_solve_level", "rgbdflow_depth_stdevs_pointwise", "rgbdflow_fill_depth_holes", "rgbdflow_fill_small_holes_colour",
     "rgbdflow_pair_load", "rgbdflow_pair_run", "rgbdflow_pair_read", "rgbdflow_multiscale",
     "rgbdflow_with_init", "rgbdflow_gaussian_pyramid", "rgbdflow_depth_map_pyramid", "rgbdflow_im2feature",
     "rgbdflow_warp_by_3d_flow", "rgbdflow_upsample_flow", "rgbdflow_level_system", "rgbdflow_relax",
@@ -73,7 +73,7 @@ class Stats(C.Structure):
 class PairInput(C.Structure):
     _fields_ = [("W", C.c_int), ("H", C.c_int), ("im1", C.c_void_p), ("dm1", C.c_void_p), ("im2", C.c_void_p),
                 ("dm2", C.c_void_p), ("sigma1", C.c_void_p), ("hpb", C.c_void_p), ("cam", Camera),
-                ("min_width", C.c_int), ("reserved", C.c_int), ("est", C.c_void_p)]
+                ("min_width", C.c_int), ("reserved", C.c_int), ("est", C.c_void_p), ("hpb_dm1", C.c_void_p)]
 
 
 class ServerConfig(C.Structure):
@@ -287,6 +287,17 @@ class Context:
         self._check(lib().rgbdflow_fill_depth_holes(self.h, _p(d), W, H, _p(out), C.byref(left)))
         return out, left.value
 
+    def fill_small_holes_colour(self, depth, bgr, max_region=60):
+        """The device's colour-guided fill of small holes: (filled f64 map,
+        pixels of small regions left invalid).  bgr: (H, W, 3) in [0, 1]."""
+        d, b = _f64(depth), _f64(bgr)
+        H, W = d.shape
+        out = np.empty((H, W))
+        left = C.c_int()
+        self._check(lib().rgbdflow_fill_small_holes_colour(self.h, _p(d), _p(b), W, H, max_region, _p(out),
+                                                          C.byref(left)))
+        return out, left.value
+
     def heuristic_pb(self, bgr, depth):
         """computeHeuristicPb of one image on the device: (H, W, 3) BGR in [0,1],
         (H, W) metres -> (H, W, 2) f32."""
@@ -400,20 +411,22 @@ class Context:
     # ---------------------------------------------------------------- batches
     def batch_load(self, params, pairs):
         """pairs: list of dicts with im1, dm1, im2, dm2 and optionally sigma1,
-        hpb, cam (a Camera), min_width and est (an (H, W, 3) flow estimate: the
-        pair is then solved as sceneFlowWithInit)."""
+        hpb, cam (a Camera), min_width, est (an (H, W, 3) flow estimate: the
+        pair is then solved as sceneFlowWithInit) and hpb_dm1 (the depth the
+        boundary maps made on the device start from)."""
         keep, arr = [], (PairInput * len(pairs))()
         for i, pr in enumerate(pairs):
             im1, dm1, im2, dm2 = _f64(pr["im1"]), _f64(pr["dm1"]), _f64(pr["im2"]), _f64(pr["dm2"])
             s1 = _f32(pr["sigma1"]) if pr.get("sigma1") is not None else None
             h = _f32(pr["hpb"]) if pr.get("hpb") is not None else None
             est = _f64(pr["est"]) if pr.get("est") is not None else None
-            keep.append((im1, dm1, im2, dm2, s1, h, est))
+            hd = _f64(pr["hpb_dm1"]) if pr.get("hpb_dm1") is not None else None
+            keep.append((im1, dm1, im2, dm2, s1, h, est, hd))
             H, W = dm1.shape
             a = arr[i]
             a.W, a.H = W, H
             a.im1, a.dm1, a.im2, a.dm2 = _p(im1), _p(dm1), _p(im2), _p(dm2)
-            a.sigma1, a.hpb, a.est = _p(s1), _p(h), _p(est)
+            a.sigma1, a.hpb, a.est, a.hpb_dm1 = _p(s1), _p(h), _p(est), _p(hd)
             if pr.get("cam") is not None:
                 a.cam = pr["cam"]
             a.min_width = int(pr.get("min_width", 0))

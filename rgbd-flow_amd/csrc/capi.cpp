@@ -62,6 +62,7 @@ struct PairSlot {
   double *pim1 = nullptr, *pim2 = nullptr, *pd1 = nullptr, *pd2 = nullptr, *pdel = nullptr, *psm = nullptr;
   float *sigma = nullptr, *hpb = nullptr;
   double* pdm = nullptr;  // depth-map pyramid of dm1, levels >= 1 (hpb_device)
+  double* dmh = nullptr;  // its level 0 when it is not dm1 (the colour-filled depth), else null
   Work wk{};
 };
 
@@ -810,6 +811,11 @@ int load_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const rgbdflow
   TRY(upload(c, ps.dm2, in.dm2, N * 8));
   if (in.sigma1) TRY(upload(c, ps.sigma, in.sigma1, N * 4));
   if (in.hpb) TRY(upload(c, ps.hpb, in.hpb, pyramid_elems(ps.L, 0) * 2 * 4));
+  ps.dmh = nullptr;
+  if (in.hpb_dm1 && ps.hpb_device) {
+    TRY(get(c, "b" + std::to_string(idx) + ".dmh", N, &ps.dmh));
+    TRY(upload(c, ps.dmh, in.hpb_dm1, N * 8));
+  }
   return sync(c);  // the caller may release its arrays after return
 }
 
@@ -918,10 +924,12 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
     size_t scratch = 0, used = 0;
     for (const PairSlot& ps : c->pairs) {
       if (!ps.hpb_device) continue;
+      // level 0: dm1, or the depth given for the maps (the colour-filled raw depth)
+      double* d0 = ps.dmh ? ps.dmh : ps.dm1;
       for (int k = 0; k < (int)ps.L.size(); k++) {
         if (k > 0)
-          bp.dm_steps.push_back({level_of(ps, ps.dm1, ps.pdm, 1, k - 1), ps.L[k - 1].W, ps.L[k - 1].H,
-                                 level_of(ps, ps.dm1, ps.pdm, 1, k), ps.L[k].W, ps.L[k].H});
+          bp.dm_steps.push_back({level_of(ps, d0, ps.pdm, 1, k - 1), ps.L[k - 1].W, ps.L[k - 1].H,
+                                 level_of(ps, d0, ps.pdm, 1, k), ps.L[k].W, ps.L[k].H});
         const size_t need = (hpb_scratch_floats(ps.L[k].W, ps.L[k].H) + 63) & ~size_t(63);
         if (groups.empty() || groups.back().size() == (size_t)kHpbMaxItems ||
             (used > 0 && used + need > kHpbGroupFloats)) {
@@ -941,7 +949,8 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
       for (const Lv& e : g) {
         const PairSlot& ps = *e.ps;
         hpb_batch_add(hb, hpb_level(ps.L[e.k].W, ps.L[e.k].H, level_of(ps, ps.im1, ps.pim1, 3, e.k),
-                                    level_of(ps, ps.dm1, ps.pdm, 1, e.k), ps.hpb + ps.hoff[e.k], hs + e.off));
+                                    level_of(ps, ps.dmh ? ps.dmh : ps.dm1, ps.pdm, 1, e.k), ps.hpb + ps.hoff[e.k],
+                                    hs + e.off));
       }
       bp.hpb_groups.push_back(hb);
     }
@@ -1032,6 +1041,11 @@ int load_pair_device(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const P
   const size_t N = (size_t)in.W * in.H, rest = pyramid_elems(ps.L, 1);
   hipStream_t s = c->stream;
   if (in.f1.sigma) HIP_TRY(c, hipMemcpyAsync(ps.sigma, in.f1.sigma, N * 4, hipMemcpyDeviceToDevice, s));
+  ps.dmh = nullptr;
+  if (in.f1.hpb_dm && ps.hpb_device) {
+    TRY(get(c, "b" + std::to_string(idx) + ".dmh", N, &ps.dmh));
+    HIP_TRY(c, hipMemcpyAsync(ps.dmh, in.f1.hpb_dm, N * 8, hipMemcpyDeviceToDevice, s));
+  }
   HIP_TRY(c, hipMemcpyAsync(ps.im1, in.f1.im, 3 * N * 8, hipMemcpyDeviceToDevice, s));
   HIP_TRY(c, hipMemcpyAsync(ps.dm1, in.f1.dm, N * 8, hipMemcpyDeviceToDevice, s));
   HIP_TRY(c, hipMemcpyAsync(ps.im2, in.f2.im, 3 * N * 8, hipMemcpyDeviceToDevice, s));
@@ -1475,6 +1489,35 @@ int rgbdflow_fill_depth_holes(rgbdflow_ctx* c, const double* depth, int W, int H
   TRY(get(c, "fill.left", 1, &left));
   TRY(upload(c, d, depth, N * 8));
   launch_median_fill(c->stream, d, W, H, st, todo, left);
+  HIP_TRY(c, hipGetLastError());
+  TRY(download(c, out, d, N * 8));
+  int l = 0;
+  TRY(download(c, &l, left, sizeof(int)));
+  TRY(sync(c));
+  if (unfilled) *unfilled = l;
+  return RGBDFLOW_OK;
+}
+
+int rgbdflow_fill_small_holes_colour(rgbdflow_ctx* c, const double* depth, const double* bgr, int W, int H,
+                                     int max_region, double* out, int* unfilled) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (bad_dims(W, H) || !depth || !bgr || !out) return fail(c, RGBDFLOW_ERR_ARG, "bad hole-fill arguments");
+  if (max_region < 1 || max_region > 63)
+    return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "max_region must be in [1, 63]");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const size_t N = (size_t)W * H;
+  double *d, *b;
+  int *label, *st, *todo, *cmax, *left;
+  TRY(get(c, "fill.d", N, &d));
+  TRY(get(c, "fill.bgr", 3 * N, &b));
+  TRY(get(c, "fill.label", N, &label));
+  TRY(get(c, "fill.st", N, &st));
+  TRY(get(c, "fill.todo", 2 * N, &todo));
+  TRY(get(c, "fill.cmax", N, &cmax));
+  TRY(get(c, "fill.left", 1, &left));
+  TRY(upload(c, d, depth, N * 8));
+  TRY(upload_planes(c, bgr, N, 3, b));
+  launch_colour_fill(c->stream, d, b, W, H, max_region, label, st, todo, cmax, left);
   HIP_TRY(c, hipGetLastError());
   TRY(download(c, out, d, N * 8));
   int l = 0;
@@ -2021,8 +2064,9 @@ int batch_load_device(rgbdflow_ctx* c, const rgbdflow_params* p, int n, const Pa
   // captured graph of it) stays valid: only the frames are copied in.
   std::string sig;
   for (int i = 0; i < n; i++) {
-    const int f[7] = {in[i].W, in[i].H, in[i].min_width, in[i].f1.hpb != nullptr, in[i].init != nullptr,
-                      in[i].f1.pim != nullptr && in[i].f1.pd != nullptr, in[i].f1.sigma != nullptr};
+    const int f[8] = {in[i].W, in[i].H, in[i].min_width, in[i].f1.hpb != nullptr, in[i].init != nullptr,
+                      in[i].f1.pim != nullptr && in[i].f1.pd != nullptr, in[i].f1.sigma != nullptr,
+                      in[i].f1.hpb_dm != nullptr};
     sig.append(reinterpret_cast<const char*>(f), sizeof(f));
     sig.append(reinterpret_cast<const char*>(&in[i].cam), sizeof(in[i].cam));
   }

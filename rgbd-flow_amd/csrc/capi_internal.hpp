@@ -18,6 +18,7 @@ struct FrameDev {
   const float* hpb;   // boundary maps of every level ([N_k][2], finest first), or null
   const float* sigma; // [N] depth stdevs as frame 1 (from the depth before any hole fill), or null:
                       // computed from dm
+  const double* hpb_dm; // [N] depth the boundary maps made on the device start from, or null: dm
 };
 
 // One pair whose frames are on the device.  Frame 1's pyramids, when given,

@@ -326,5 +326,10 @@ void launch_ingest(hipStream_t s, const unsigned char* bgr, const float* depth, 
 // depth hole fill in place (the rounds variant); st: [N], todo: [2N] int scratch;
 // left: [1] the pixels left invalid (none is known around them)
 void launch_median_fill(hipStream_t s, double* depth, int W, int H, int* st, int* todo, int* left);
+// the colour-guided fill of invalid regions of at most max_region (<= 63)
+// pixels, in place; bgr: [3][N] planes in [0, 1]; label, st, cmax: [N],
+// todo: [2N] int scratch; left: [1] the pixels of small regions left invalid
+void launch_colour_fill(hipStream_t s, double* depth, const double* bgr, int W, int H, int max_region, int* label,
+                        int* st, int* todo, int* cmax, int* left);
 
 }  // namespace rgbdflow

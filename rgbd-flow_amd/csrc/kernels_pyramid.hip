@@ -454,6 +454,162 @@ __global__ void __launch_bounds__(kFillThreads) k_median_fill(double* d, int W, 
   if (threadIdx.x == 0) *gp(left_out) = n;
 }
 
+// The colour-guided fill of small holes, the parallel variant of
+// recursiveMedianFilterSmallInvalidRegionsUsingColor
+// (recursiveMedianFiltering.cpp:127-222, halfwidth 1), which fills only the
+// 4-connected invalid regions of at most max_region pixels, each pixel with
+// the mean of its known 3x3 neighbours weighted by
+// exp(-|colour difference|^2 / 0.6), region by region in an order its hash
+// containers decide.  Here all small regions at once, in rounds: a region's
+// pixels with the region's largest number of known neighbours (valid, or
+// filled in an earlier round) are filled together.
+constexpr int kMaxRegion = 63;
+
+// A region's label: its smallest pixel index (max_region or fewer pixels),
+// -1 (larger), -2 (the pixel is not invalid).  Each invalid pixel walks its
+// own region breadth-first, at most max_region + 1 pixels: the queue and an
+// open-addressing set of the pixels seen (128 slots, at most 64 used) are
+// rows of LDS (odd pitches: lanes at the same position hit distinct banks).
+constexpr int kRegionThreads = 128;
+constexpr int kQPitch = kMaxRegion + 2, kSeen = 128, kSeenPitch = kSeen + 1;
+__global__ void __launch_bounds__(kRegionThreads) k_small_regions(const double* d, int W, int H, int max_region,
+                                                                  int* label) {
+  __shared__ int qs[kRegionThreads * kQPitch];
+  __shared__ int ss[kRegionThreads * kSeenPitch];
+  const int l = blockIdx.x * kRegionThreads + threadIdx.x;
+  if (l >= W * H) return;
+  if (!(gp(d)[l] <= 0)) {
+    gp(label)[l] = -2;
+    return;
+  }
+  int* q = qs + threadIdx.x * kQPitch;
+  int* seen = ss + threadIdx.x * kSeenPitch;
+  for (int t = 0; t < kSeen; t++) seen[t] = -1;
+  // inserts m into the set; false when it was there already
+  auto insert = [&](int m) {
+    unsigned h = ((unsigned)m * 2654435761u) >> 25;
+    for (;;) {
+      const int v = seen[h];
+      if (v == m) return false;
+      if (v < 0) {
+        seen[h] = m;
+        return true;
+      }
+      h = (h + 1) & (kSeen - 1);
+    }
+  };
+  int head = 0, tail = 1, lo = l;
+  q[0] = l;
+  insert(l);
+  bool large = false;
+  while (head < tail && !large) {
+    const int p = q[head++];
+    const int i = p / W, j = p - i * W;
+    const int nb[4] = {i > 0 ? p - W : -1, j > 0 ? p - 1 : -1, j < W - 1 ? p + 1 : -1, i < H - 1 ? p + W : -1};
+    for (int k = 0; k < 4 && !large; k++) {
+      const int m = nb[k];
+      if (m < 0 || !(gp(d)[m] <= 0) || !insert(m)) continue;
+      if (tail == max_region) {
+        large = true;  // a (max_region + 1)-th pixel
+        break;
+      }
+      q[tail++] = m;
+      lo = min(lo, m);
+    }
+  }
+  gp(label)[l] = large ? -1 : lo;
+}
+
+// exp(x) for the colour weights, in f64: x = n ln 2 + r (Cody-Waite, ln 2 in
+// two parts), e^r by its Taylor series to degree 13 (|r| <= 0.35: error below
+// 1e-17 relative), times 2^n.  The oracle evaluates the same operations.
+__device__ __forceinline__ double fill_exp(double x) {
+  if (!(x > -700.0)) return 0.0;
+  const double n = rint(x * 1.4426950408889634);
+  const double r = (x - n * 0.693145751953125) - n * 1.4286068203094173e-06;
+  double p = 1.0 / 6227020800.0;
+  p = p * r + 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)n);
+}
+
+// One workgroup per frame, as k_median_fill; cmax[label]: the region's
+// largest number of known neighbours this round.
+__global__ void __launch_bounds__(kFillThreads) k_colour_fill(double* d, const double* bgr, int W, int H,
+                                                              const int* label, int* st, int* todo, int* cmax,
+                                                              int* left_out) {
+  __shared__ int s_n, s_m, s_filled;
+  const int N = W * H;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  for (int l = threadIdx.x; l < N; l += kFillThreads) {
+    gp(st)[l] = gp(d)[l] > 0 ? 0 : kUnfilled;
+    if (gp(label)[l] >= 0) gp(todo)[atomicAdd(&s_n, 1)] = l;
+  }
+  __syncthreads();
+  int n = s_n, cur = 0;
+  const int n0 = n;
+  for (int r = 1; n > 0 && r <= n0; r++) {
+    const int* list = todo + (size_t)cur * N;
+    int* next = todo + (size_t)(1 - cur) * N;
+    if (threadIdx.x == 0) {
+      s_m = 0;
+      s_filled = 0;
+    }
+    for (int t = threadIdx.x; t < n; t += kFillThreads) gp(cmax)[gp(label)[gp(list)[t]]] = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += kFillThreads) {
+      const int l = gp(list)[t];
+      atomicMax(cmax + gp(label)[l], fill_known(d, st, W, H, l, r, nullptr));
+    }
+    __syncthreads();
+    int filled = 0;
+    for (int t = threadIdx.x; t < n; t += kFillThreads) {
+      const int l = gp(list)[t];
+      const int k = fill_known(d, st, W, H, l, r, nullptr);
+      if (k == 0 || k != gp(cmax)[gp(label)[l]]) {
+        gp(next)[atomicAdd(&s_m, 1)] = l;
+        continue;
+      }
+      const int i = l / W, j = l - i * W;
+      const double c0 = gp(bgr)[l], c1 = gp(bgr)[N + l], c2 = gp(bgr)[2 * (size_t)N + l];
+      float zs = 0.f, ws = 0.f;
+      for (int ii = max(0, i - 1); ii <= min(H - 1, i + 1); ii++)
+        for (int jj = max(0, j - 1); jj <= min(W - 1, j + 1); jj++) {
+          const int q = ii * W + jj;
+          if (gp(st)[q] < r) {
+            const double e0 = c0 - gp(bgr)[q], e1 = c1 - gp(bgr)[N + q], e2 = c2 - gp(bgr)[2 * (size_t)N + q];
+            const float w = (float)fill_exp(-(e0 * e0 + e1 * e1 + e2 * e2) / .6);
+            zs = (float)((double)zs + gp(d)[q] * w);
+            ws += w;
+          }
+        }
+      gp(d)[l] = (double)(zs / ws);
+      gp(st)[l] = r;
+      filled++;
+    }
+    atomicAdd(&s_filled, filled);
+    __syncthreads();
+    n = s_m;
+    cur = 1 - cur;
+    const bool stuck = s_filled == 0;
+    __syncthreads();
+    if (stuck) break;  // regions with no known neighbour (the reference asserts there)
+  }
+  if (threadIdx.x == 0) *gp(left_out) = n;
+}
+
 inline int blocks_for(size_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -489,6 +645,12 @@ void launch_level_weights(hipStream_t s, const Views& v, int blocks) {
 }
 void launch_depth_sigma(hipStream_t s, const float* depth, int W, int H, int max_dist, float* sigma) {
   hipLaunchKernelGGL(k_depth_sigma, dim3(blocks_for((size_t)W * H)), dim3(kBlock), 0, s, depth, W, H, max_dist, sigma);
+}
+void launch_colour_fill(hipStream_t s, double* depth, const double* bgr, int W, int H, int max_region, int* label,
+                        int* st, int* todo, int* cmax, int* left) {
+  hipLaunchKernelGGL(k_small_regions, dim3((W * H + kRegionThreads - 1) / kRegionThreads), dim3(kRegionThreads), 0, s,
+                     depth, W, H, max_region, label);
+  hipLaunchKernelGGL(k_colour_fill, dim3(1), dim3(kFillThreads), 0, s, depth, bgr, W, H, label, st, todo, cmax, left);
 }
 void launch_median_fill(hipStream_t s, double* depth, int W, int H, int* st, int* todo, int* left) {
   hipLaunchKernelGGL(k_median_fill, dim3(1), dim3(kFillThreads), 0, s, depth, W, H, st, todo, left);

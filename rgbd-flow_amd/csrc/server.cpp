@@ -50,6 +50,7 @@ struct Frame {
   double *im = nullptr, *dm = nullptr, *pim = nullptr, *pd = nullptr;
   float* hpb = nullptr;
   float* sigma = nullptr;  // fill_holes: depth stdevs of the depth as submitted
+  double* dmh = nullptr;   // fill_holes: the depth with small holes colour-filled (boundary maps)
   bool pyr = false;  // pim / pd hold this frame's pyramids
 };
 
@@ -60,6 +61,7 @@ struct StreamState {
   Frame f[2];
   double* flow = nullptr;  // [3][N] the stream's last result (warm starts)
   int *fill_st = nullptr, *fill_todo = nullptr, *fill_left = nullptr;  // fill_holes scratch
+  int *fill_label = nullptr, *fill_cmax = nullptr;
   bool has_flow = false;
   int cur = -1;          // index of the last frame, -1 before the first
   bool busy = false;     // a request of this stream is in a batch
@@ -190,9 +192,13 @@ void dispatch(rgbdflow_server* S, Lane& ln) {
     const float* d32 = reinterpret_cast<const float*>(q.in.p + q.in.depth_off);
     launch_ingest(hs, q.in.p, d32, (int)st.N, F.im, F.dm);
     if (S->fill) {
-      // the depth stdevs of the depth as submitted (runFlowTwoFrames.cpp:89-91),
-      // then the holes filled (sceneFlowMultiscale.cpp:54,61)
+      // the depth stdevs of the depth as submitted (runFlowTwoFrames.cpp:89-91);
+      // small holes colour-filled for the boundary maps (sceneFlowMultiscale.cpp:78-83);
+      // every hole filled for the rest (sceneFlowMultiscale.cpp:54,61)
       launch_depth_sigma(hs, d32, st.W, st.H, 4, F.sigma);
+      hipMemcpyAsync(F.dmh, F.dm, st.N * 8, hipMemcpyDeviceToDevice, hs);
+      launch_colour_fill(hs, F.dmh, F.im, st.W, st.H, 60, st.fill_label, st.fill_st, st.fill_todo, st.fill_cmax,
+                         st.fill_left);
       launch_median_fill(hs, F.dm, st.W, st.H, st.fill_st, st.fill_todo, st.fill_left);
     }
     if (q.in.hpb_n) hipMemcpyAsync(F.hpb, q.in.p + q.in.hpb_off, q.in.hpb_n * 4, hipMemcpyDeviceToDevice, hs);
@@ -205,8 +211,8 @@ void dispatch(rgbdflow_server* S, Lane& ln) {
       pd.min_width = st.min_width;
       pd.cam = st.cam;
       pd.f1 = FrameDev{P.im, P.dm, P.pyr ? P.pim : nullptr, P.pyr ? P.pd : nullptr, q.in.hpb_n ? P.hpb : nullptr,
-                       S->fill ? P.sigma : nullptr};
-      pd.f2 = FrameDev{F.im, F.dm, nullptr, nullptr, nullptr, nullptr};
+                       S->fill ? P.sigma : nullptr, S->fill ? P.dmh : nullptr};
+      pd.f2 = FrameDev{F.im, F.dm, nullptr, nullptr, nullptr, nullptr, nullptr};
       pd.init = q.warm ? st.flow : nullptr;
       ln.pair_of[r] = (int)pairs.size();
       pairs.push_back(pd);
@@ -446,7 +452,8 @@ int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera
   // one allocation: two frames (planes, pyramids, boundary maps) and the last flow
   const size_t frame_bytes = r256(3 * st->N * 8) + r256(st->N * 8) + r256(3 * st->rest * 8 + 8) +
                              r256(st->rest * 8 + 8) + r256(st->hpb_n * 4);
-  const size_t fill_bytes = S->fill ? 2 * r256(st->N * 4) + r256(st->N * 4) + r256(2 * st->N * 4) + r256(4) : 0;
+  const size_t fill_bytes =
+      S->fill ? 2 * (r256(st->N * 4) + r256(st->N * 8)) + 3 * r256(st->N * 4) + r256(2 * st->N * 4) + r256(4) : 0;
   const size_t bytes = 2 * frame_bytes + (S->warm ? r256(3 * st->N * 8) : 0) + fill_bytes;
   {
     std::lock_guard<std::mutex> lk(S->m);
@@ -471,8 +478,13 @@ int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera
   }
   if (S->warm) st->flow = reinterpret_cast<double*>(take(3 * st->N * 8));
   if (S->fill) {
-    for (Frame& F : st->f) F.sigma = reinterpret_cast<float*>(take(st->N * 4));
+    for (Frame& F : st->f) {
+      F.sigma = reinterpret_cast<float*>(take(st->N * 4));
+      F.dmh = reinterpret_cast<double*>(take(st->N * 8));
+    }
     st->fill_st = reinterpret_cast<int*>(take(st->N * 4));
+    st->fill_label = reinterpret_cast<int*>(take(st->N * 4));
+    st->fill_cmax = reinterpret_cast<int*>(take(st->N * 4));
     st->fill_todo = reinterpret_cast<int*>(take(2 * st->N * 4));
     st->fill_left = reinterpret_cast<int*>(take(4));
   }

@@ -63,3 +63,23 @@ def test_fill_matches_oracle(ctx, R, W, H, pattern):
         assert lb == 0 and np.median(diff) < 1e-3 and diff.max() <= 0.002 * (W // 6 + 2) + 0.01
     if pattern == "all":
         assert lb == W * H and not b.any()
+
+
+@pytest.mark.parametrize("W,H,holes", [(64, 48, 0.02), (160, 120, 0.05), (640, 480, 0.05), (97, 61, 0.4)])
+def test_colour_fill_matches_oracle(ctx, R, W, H, holes):
+    """rgbdflow_fill_small_holes_colour: bit for bit the oracle's rounds, on
+    random holes (many small regions, some merging into regions above 60
+    pixels that stay invalid), a block too large to fill and random colours."""
+    O = oracle()
+    d, rng = _depth(W, H, 3 * W + H)
+    col = rng.uniform(0, 1, (H, W, 3))
+    d[rng.random((H, W)) < holes] = 0.0
+    d[H // 4:H // 4 + 9, W // 5:W // 5 + 9] = 0.0            # 81 pixels: left as it is
+    print(f"colour fill {W}x{H}: {(d <= 0).sum()} invalid", flush=True)
+    a, la = O.colour_fill_rounds(d, col)
+    b, lb = ctx.fill_small_holes_colour(d, col)
+    assert la == lb and np.array_equal(a, b)
+    assert (b[H // 4 + 4, W // 5 + 4] <= 0)
+    with pytest.raises(R.RgbdFlowError) as e:
+        ctx.fill_small_holes_colour(d, col, max_region=64)
+    assert e.value.code == R.ERR_UNSUPPORTED

@@ -143,11 +143,14 @@ def test_serving_warm_start_equals_with_init(R, max_batch, lanes, mode):
 
 
 @pytest.mark.parametrize("max_batch,lanes", [(8, 2), (1, 1)])
-def test_serving_fill_holes_equals_alone(R, max_batch, lanes):
-    """fill_holes: frames whose depth has holes; each result equals
-    rgbdflow_multiscale of the pair alone with both depth maps filled by
-    rgbdflow_fill_depth_holes and frame 1's depth stdevs computed from its
-    depth as submitted (holes included), bit for bit."""
+@pytest.mark.parametrize("mode", ["maps", "device_maps"])
+def test_serving_fill_holes_equals_alone(R, max_batch, lanes, mode):
+    """fill_holes: frames whose depth has holes; each result equals the pair
+    solved alone with both depth maps filled by rgbdflow_fill_depth_holes,
+    frame 1's depth stdevs computed from its depth as submitted (holes
+    included) and -- boundary maps made on the device -- those maps made from
+    frame 1's depth with its small holes filled by
+    rgbdflow_fill_small_holes_colour, bit for bit."""
     S = synth()
     sizes = [(128, 96, 16), (96, 72, 12)]
     p = R.make_params(128, 96, n_outer=3, n_sor=25, min_width=16, relax_variant=R.RELAX_TILED)
@@ -161,8 +164,9 @@ def test_serving_fill_holes_equals_alone(R, max_batch, lanes):
                 d = d.copy()
                 d[rng.random(d.shape) < 0.03] = 0.0
                 d[H // 3:H // 3 + 5, W // 4:W // 4 + 7] = 0.0
+                d[H // 2:H // 2 + 9, W // 2:W // 2 + 9] = 0.0      # above 60 pixels: no colour fill
                 holed.append((img, d))
-            maps = [_maps(R, S, pair, W, H, mw, sh) for sh in (0, 3)]
+            maps = [_maps(R, S, pair, W, H, mw, sh) if mode == "maps" else None for sh in (0, 3)]
             sid = srv.open(W, H, R.make_params(W, H).cam, mw)
             streams.append((sid, W, H, mw, holed, maps))
         tickets = []
@@ -183,7 +187,15 @@ def test_serving_fill_holes_equals_alone(R, max_batch, lanes):
                 fb, lb = ctx.fill_depth_holes(frames[b][1].astype(np.float64))
                 assert la == 0 and lb == 0
                 sa = ctx.depth_stdevs(frames[a][1])
-                u, v, w, _ = ctx.multiscale(q, frames[a][0].astype(np.float64) / 255.0, fa,
-                                            frames[b][0].astype(np.float64) / 255.0, fb, sa, maps[a])
+                im_a = frames[a][0].astype(np.float64) / 255.0
+                pr = dict(im1=im_a, dm1=fa, im2=frames[b][0].astype(np.float64) / 255.0, dm2=fb, sigma1=sa,
+                          hpb=maps[a], cam=R.make_params(W, H).cam, min_width=mw)
+                if maps[a] is None:
+                    hd, _ = ctx.fill_small_holes_colour(frames[a][1].astype(np.float64), im_a)
+                    assert (hd[H // 2 + 4, W // 2 + 4] <= 0) and (hd[H // 3 + 2, W // 4 + 3] > 0)
+                    pr["hpb_dm1"] = hd
+                ctx.batch_load(q, [pr])
+                ctx.batch_run(q, stats=False)
+                u, v, w = ctx.batch_read(0)
                 ub, vb, wb = results[(sid, t)]
                 assert np.array_equal(u, ub) and np.array_equal(v, vb) and np.array_equal(w, wb), (sid, t)

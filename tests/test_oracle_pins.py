@@ -594,3 +594,45 @@ def test_median_fill_rounds():
     h[2:5, 2:5] = 0.0
     b, left = O.median_fill_rounds(h)
     assert left == 0 and b[3, 3] == np.float32(np.sort(b[2:5, 2:5].ravel()[[0, 1, 2, 3, 5, 6, 7, 8]])[4])
+
+
+def test_colour_fill_rounds():
+    """The parallel-rounds variant of the colour-guided small-hole fill:
+    its exp against numpy's; equal colours give the plain mean of the known
+    neighbours; across a colour edge the weights pick the hole's own side;
+    regions of max_region pixels are filled, larger ones left as they are;
+    a 3 x 3 hole fills its ring before its centre."""
+    O = oracle()
+    x = -np.concatenate([np.linspace(0, 10, 2001), [1e-12, 0.5 / 0.6, 3 / 0.6]])
+    assert np.abs(O.fill_exp(x) / np.exp(x) - 1).max() < 4e-16
+    W, H = 40, 30
+    rng = np.random.default_rng(4)
+    d = rng.uniform(1, 2, (H, W))
+    col = np.full((H, W, 3), 0.5)
+    h = d.copy()
+    h[10, 10] = 0.0
+    f, left = O.colour_fill_rounds(h, col)
+    nb = np.delete(d[9:12, 9:12].ravel(), 4)
+    assert left == 0 and abs(f[10, 10] - nb.mean()) < 1e-6 and np.array_equal(f[h > 0], d[h > 0])
+    # colour edge at column 20: depth 1 on the left (dark), 2 on the right (bright)
+    d2 = np.where(np.arange(W)[None, :] < 20, 1.0, 2.0) * np.ones((H, 1))
+    col2 = np.where(np.arange(W)[None, :, None] < 20, 0.1, 0.9) * np.ones((H, 1, 3))
+    h2 = d2.copy()
+    h2[15, 19] = h2[5, 20] = 0.0
+    f2, _ = O.colour_fill_rounds(h2, col2)
+    w = np.exp(-3 * 0.8 ** 2 / 0.6)         # the other side's weight; one's own side weighs 1
+    assert abs(f2[15, 19] - (5 * 1.0 + 3 * w * 2.0) / (5 + 3 * w)) < 1e-6
+    assert abs(f2[5, 20] - (5 * 2.0 + 3 * w * 1.0) / (5 + 3 * w)) < 1e-6
+    # a straight 4-connected run of 60 pixels is filled, one of 61 is not
+    h3 = d.copy()
+    h3[2, 0:30] = 0.0
+    h3[3, 0:30] = 0.0            # 60 pixels
+    h3[20, 0:31] = 0.0
+    h3[21, 0:30] = 0.0           # 61 pixels
+    f3, left = O.colour_fill_rounds(h3, col)
+    assert left == 0 and (f3[2:4, 0:30] > 0).all() and (f3[20, 0:31] <= 0).all() and (f3[21, 0:30] <= 0).all()
+    # the ring before the centre: the centre's neighbours are all filled values when it is filled
+    h4 = d.copy()
+    h4[5:8, 5:8] = 0.0
+    f4, left = O.colour_fill_rounds(h4, col)
+    assert left == 0 and abs(f4[6, 6] - np.delete(f4[5:8, 5:8].ravel(), 4).mean()) < 1e-6
