@@ -349,13 +349,14 @@ def main():
             "batch_p99_ms": round(float(np.percentile(lat16i, 99)), 3)}
         print(f"bench: fp16 storage and images {f16['with_fp16_images']}", file=sys.stderr, flush=True)
 
-    # served: 640x480 streams at 30 fps, FP16 storage, warm starts, continuous batching
+    # served: 640x480 streams at 30 fps, FP16 storage, warm starts, continuous batching, boundary maps
+    # made on the device
     served = None
     if args.served_streams > 0:
         spec = [(W, H)] * args.served_streams
         frames = serve_frames(R, spec, args.frames)
         served = serve_run(R, spec, frames, args.fps, args.served_seconds, args.served_max_batch, "f16", True,
-                           args.served_warm_outer, autotune=not args.no_autotune, images="f16")
+                           args.served_warm_outer, autotune=not args.no_autotune, images="f16", device_hpb=True)
         served.update(streams=args.served_streams, fps_per_stream=args.fps, max_batch=args.served_max_batch,
                       storage="f16", images="f16", warm_start=True, warm_n_outer=args.served_warm_outer or 15)
         served["speedup_vs_plain_single"] = round(served["pairs_per_s"] / plain_pps, 2) if plain_pps else None
