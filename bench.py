@@ -309,6 +309,22 @@ def main():
             c.close()
         print(f"bench: two lanes {two_lanes}", file=sys.stderr, flush=True)
 
+    # the same f32 batch with the boundary maps made on the device (every level's
+    # computeHeuristicPb inside the timed step, as sceneFlowMultiscale.cpp:116 does)
+    dev_maps = None
+    if args.storage == "f32" and args.variant == "tiled":
+        ctx.batch_load(p, [dict(pr, hpb=None) for pr in batch])
+        std = ctx.batch_run(p, stats=True)
+        for _ in range(args.warmup):
+            ctx.batch_run(p, stats=False)
+        eld = timed_runs(ctx, p, args.steps)
+        L = std[0].n_levels
+        dev_maps = {"pairs_per_s": round(B * args.steps / eld, 3), "ms_per_step": round(eld / args.steps * 1e3, 3),
+                    "speedup_vs_plain_single": round(B * args.steps / eld / plain_pps, 2) if plain_pps else None,
+                    "sweeps_exec_pair0": [std[0].sweeps_exec[k] for k in range(L)],
+                    "outer_exec_pair0": [std[0].outer_exec[k] for k in range(L)]}
+        print(f"bench: device boundary maps {dev_maps}", file=sys.stderr, flush=True)
+
     # the same batch with the systems stored in FP16 (f32 arithmetic)
     f16 = None
     if args.storage == "f32" and not args.no_f16 and args.variant == "tiled":
@@ -406,6 +422,7 @@ def main():
             "autotuned": tuned,
             "two_lanes": two_lanes,
             "fp16_storage": f16,
+            "with_device_boundary_maps": dev_maps,
             "served_640x480": served,
         },
         "roofline": {
