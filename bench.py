@@ -178,6 +178,8 @@ def main():
                     help="serving: solve each stream's later pairs as sceneFlowWithInit from its last flow")
     ap.add_argument("--warm-outer", type=int, default=0, help="serving: outer iterations of warm starts (0: 15)")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage run of the batch")
+    ap.add_argument("--no-device-maps", action="store_true",
+                    help="skip the run of the batch with its boundary maps made on the device")
     ap.add_argument("--images", choices=["f64", "f16", "bf16"], default="f64",
                     help="serving: storage of the feature images (f64: the reference's precision)")
     ap.add_argument("--fill-holes", action="store_true",
@@ -312,7 +314,7 @@ def main():
     # the same f32 batch with the boundary maps made on the device (every level's
     # computeHeuristicPb inside the timed step, as sceneFlowMultiscale.cpp:116 does)
     dev_maps = None
-    if args.storage == "f32" and args.variant == "tiled":
+    if args.storage == "f32" and args.variant == "tiled" and not args.no_device_maps:
         ctx.batch_load(p, [dict(pr, hpb=None) for pr in batch])
         std = ctx.batch_run(p, stats=True)
         for _ in range(args.warmup):
