@@ -315,6 +315,13 @@ def main():
     # computeHeuristicPb inside the timed step, as sceneFlowMultiscale.cpp:116 does)
     dev_maps = None
     if args.storage == "f32" and args.variant == "tiled" and not args.no_device_maps:
+        # the batch's schedules again (the single-pair runs installed their own)
+        sched_b = raw.get(B)
+        if args.load_schedules:
+            with open(args.load_schedules) as f:
+                sched_b = json.load(f)[str(B)]
+        for ch in sched_b or []:
+            ctx.set_relax_schedule(ch["W"], ch["H"], ch)
         ctx.batch_load(p, [dict(pr, hpb=None) for pr in batch])
         std = ctx.batch_run(p, stats=True)
         for _ in range(args.warmup):
