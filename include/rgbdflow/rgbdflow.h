@@ -179,6 +179,22 @@ int rgbdflow_solve_level(rgbdflow_ctx* ctx, int W, int H, float focal_level, con
  * because no known pixel surrounds them (the reference asserts there). */
 int rgbdflow_fill_depth_holes(rgbdflow_ctx* ctx, const double* depth, int W, int H, double* out, int* unfilled);
 
+/* rgbdflow_fill_depth_holes on device memory, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  depth, out: [N] metres in device memory (may be the
+ * same array); unfilled: one device int that receives the count of pixels
+ * left invalid, or NULL.  stream: a hipStream_t, or NULL for the context's
+ * stream; the call only enqueues (no host synchronisation), and the result
+ * is the same as rgbdflow_fill_depth_holes's, bit for bit.  The fill's
+ * scratch (per-pixel state and the lists of unfilled pixels) is one
+ * fixed array per device that the library holds until the process ends,
+ * so N = W*H is limited to 1920*1088 (larger frames: RGBDFLOW_ERR_ARG).
+ * Every call on a device shares that array: calls that overlap on one
+ * device (on different streams, or from several threads) race. */
+int rgbdflow_fill_depth_holes_noalloc(rgbdflow_ctx* ctx, void* stream, const double* depth, int W, int H,
+                                      double* out, int* unfilled);
+
 /* Colour-guided fill of small holes on the device: the parallel variant of
  * recursiveMedianFilterSmallInvalidRegionsUsingColor
  * (recursiveMedianFiltering.cpp:127-222, halfwidth 1;

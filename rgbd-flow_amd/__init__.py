@@ -37,7 +37,7 @@ EXPORTS = [
     "rgbdflow_batch_run", "rgbdflow_batch_read", "rgbdflow_set_relax_schedule", "rgbdflow_time_relax_schedule",
     "rgbdflow_server_create", "rgbdflow_server_destroy", "rgbdflow_server_last_error", "rgbdflow_stream_open",
     "rgbdflow_stream_submit", "rgbdflow_stream_poll", "rgbdflow_stream_close",
-    "rgbdflow_server_set_relax_schedule",
+    "rgbdflow_server_set_relax_schedule", "rgbdflow_fill_depth_holes_noalloc",
 ]
 
 
@@ -286,6 +286,14 @@ class Context:
         left = C.c_int()
         self._check(lib().rgbdflow_fill_depth_holes(self.h, _p(d), W, H, _p(out), C.byref(left)))
         return out, left.value
+
+    def fill_depth_holes_noalloc(self, depth_ptr, W, H, out_ptr, unfilled_ptr=None, stream=None):
+        """rgbdflow_fill_depth_holes_noalloc on device pointers (ints): enqueues
+        the fill on `stream` (a hipStream_t as an int, None: the context's
+        stream) and returns without waiting."""
+        self._check(lib().rgbdflow_fill_depth_holes_noalloc(
+            self.h, C.c_void_p(stream) if stream else None, C.c_void_p(depth_ptr), int(W), int(H),
+            C.c_void_p(out_ptr), C.c_void_p(unfilled_ptr) if unfilled_ptr else None))
 
     def fill_small_holes_colour(self, depth, bgr, max_region=60):
         """The device's colour-guided fill of small holes: (filled f64 map,

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -150,6 +151,39 @@ struct rgbdflow_ctx {
 namespace {
 
 const char* kNoCtx = "null context";
+
+// Scratch of rgbdflow_fill_depth_holes_noalloc: the fill's per-pixel state
+// [N], its two lists of unfilled pixels [2N] and, at kNoAllocLeft, the count
+// of pixels left unfilled, for frames of at most kNoAllocMaxPixels pixels
+// (25 MB).  Each device holds its own copy until the process ends.  Host
+// code never uses the array's name as an address: it uses the per-device
+// address that hipGetSymbolAddress returns at context creation.
+constexpr size_t kNoAllocMaxPixels = (size_t)1920 * 1088;
+constexpr size_t kNoAllocLeft = 3 * kNoAllocMaxPixels;
+__device__ __align__(256) int scratch[kNoAllocLeft + 64];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// Called with `device` current (rgbdflow_create); looks the address up once per device.
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  if (e == hipSuccess) g_scratch_ptr[device] = static_cast<int*>(p);
+}
+
+int* scratch_of(int device, hipError_t* err) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  *err = g_scratch_err[device];
+  return g_scratch_ptr[device];
+}
 
 int fail(rgbdflow_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -1238,6 +1272,7 @@ int rgbdflow_create(rgbdflow_ctx** out, int device, unsigned long long budget) {
     delete c;
     return RGBDFLOW_ERR_HIP;
   }
+  scratch_setup(device);
   int v = 0;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0) c->cu_count = v;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
@@ -1495,6 +1530,26 @@ int rgbdflow_fill_depth_holes(rgbdflow_ctx* c, const double* depth, int W, int H
   TRY(download(c, &l, left, sizeof(int)));
   TRY(sync(c));
   if (unfilled) *unfilled = l;
+  return RGBDFLOW_OK;
+}
+
+int rgbdflow_fill_depth_holes_noalloc(rgbdflow_ctx* c, void* stream, const double* depth, int W, int H, double* out,
+                                      int* unfilled) {
+  if (!c) return RGBDFLOW_ERR_ARG;
+  if (bad_dims(W, H) || !depth || !out) return fail(c, RGBDFLOW_ERR_ARG, "bad hole-fill arguments");
+  const size_t N = (size_t)W * H;
+  if (N > kNoAllocMaxPixels)
+    return fail(c, RGBDFLOW_ERR_ARG, "frame of " + std::to_string(N) + " pixels exceeds the no-allocation limit of " +
+                                         std::to_string(kNoAllocMaxPixels) + " pixels");
+  hipError_t e = hipSuccess;
+  int* sc = scratch_of(c->device, &e);
+  if (!sc)
+    return fail(c, RGBDFLOW_ERR_HIP, std::string("hipGetSymbolAddress of the fill scratch: ") + hipGetErrorString(e));
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  if (out != depth) launch_deinterleave(s, depth, (int)N, 1, out);  // one plane: a copy
+  launch_median_fill(s, out, W, H, sc, sc + N, unfilled ? unfilled : sc + kNoAllocLeft);
+  HIP_TRY(c, hipGetLastError());
   return RGBDFLOW_OK;
 }
 
