@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
 PREC_REF, PREC_F64 = 0, 1
 REG_ISOTROPIC, REG_LETOUZEY, REG_HPB = 0, 1, 2
 IMG_F64, IMG_F16, IMG_BF16 = 0, 1, 2
+REC_F32, REC_F16, REC_BF16 = 0, 1, 2
 
 
 class Camera(C.Structure):
@@ -33,7 +34,7 @@ class Params(C.Structure):
                 ("n_inner", C.c_int), ("n_sor", C.c_int), ("cam", Camera), ("precision", C.c_int),
                 ("disable_inner_exit", C.c_int), ("disable_outer_exit", C.c_int), ("omega", C.c_float),
                 ("residual_exit", C.c_int), ("residual_tol", C.c_double), ("pixel_gs", C.c_int),
-                ("image_storage", C.c_int)]
+                ("image_storage", C.c_int), ("record_storage", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -91,7 +92,7 @@ def _f32(a):
 def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
                 ratio=0.5, min_width=None, reg=REG_HPB, precision=PREC_REF, cw=1.0, dvc=1.0,
                 disable_inner_exit=False, disable_outer_exit=False, omega=0.7, residual_exit=False,
-                residual_tol=1e-4, pixel_gs=False, image_storage=IMG_F64) -> Params:
+                residual_tol=1e-4, pixel_gs=False, image_storage=IMG_F64, record_storage=0) -> Params:
     p = Params()
     p.regularization_type = reg
     p.color_data_weight = cw
@@ -113,6 +114,7 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.residual_tol = residual_tol
     p.pixel_gs = int(pixel_gs)
     p.image_storage = image_storage
+    p.record_storage = record_storage
     return p
 
 
@@ -122,6 +124,16 @@ def img_round(x, storage):
     f.restype = C.c_double
     f.argtypes = [C.c_double, C.c_int]
     return np.array([f(float(v), storage) for v in np.ravel(x)]).reshape(np.shape(x))
+
+
+def record_round(c, storage):
+    """Rows of 13 f32 system coefficients rounded as that record storage holds them."""
+    a = np.array(c, dtype=np.float32).reshape(-1, 13).copy()
+    f = lib().orc_record_round
+    f.argtypes = [C.c_void_p, C.c_int]
+    for row in a:
+        f(row.ctypes.data, storage)
+    return a
 
 
 # ---------------------------------------------------------------- pyramids

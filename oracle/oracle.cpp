@@ -324,6 +324,26 @@ static double img_round(double xd, int storage) {
   const double r = std::nearbyint((double)x / q) * q;
   return std::fabs(r) > 65504.0 ? std::copysign(INFINITY, r) : r;
 }
+/* The relaxation's per-pixel systems stored in 16 bits (a precision variant,
+ * not in the reference): BF16 of each f32 coefficient, or FP16 with a
+ * per-pixel power-of-two scale -- the 13 coefficients times 2^-e with
+ * e = ilogb(max |c|) - 14 (clamped to [-100, 100]; 0 when the largest is 0
+ * or not finite), so the largest lands in [2^14, 2^15) inside FP16's range,
+ * rounded to FP16 and decoded as value times 2^e.  Both round to nearest,
+ * ties to even.  c: the 13 coefficients of one pixel, rounded in place. */
+static void record_round(float* c, int storage) {
+  if (storage == ORC_REC_BF16) {
+    for (int k = 0; k < 13; k++) c[k] = (float)img_round(c[k], ORC_IMG_BF16);
+    return;
+  }
+  float m = 0.f;
+  for (int k = 0; k < 13; k++) m = std::max(m, std::fabs(c[k]));
+  int e = 0;
+  if (m > 0.f && m <= 3.4e38f) e = std::min(std::max(std::ilogb(m) - 14, -100), 100);
+  const float down = std::ldexp(1.0f, -e), up = std::ldexp(1.0f, e);
+  for (int k = 0; k < 13; k++) c[k] = (float)img_round((double)(c[k] * down), ORC_IMG_F16) * up;
+}
+
 static void img_round_all(double* a, size_t n, int storage) {
   if (storage != ORC_IMG_F64)
     for (size_t i = 0; i < n; i++) a[i] = img_round(a[i], storage);
@@ -853,6 +873,16 @@ struct Solver {
     });
   }
 
+  /* the records as the 16-bit storage holds them (record_round) */
+  void store_records() {
+    for (size_t l = 0; l < N; l++) {
+      float c[13];
+      for (int k = 0; k < 13; k++) c[k] = (float)coef[k * N + l];
+      record_round(c, p->record_storage);
+      for (int k = 0; k < 13; k++) coef[k * N + l] = (CT)c[k];
+    }
+  }
+
   void build_system() {
     derivatives();
     occlusion();
@@ -1005,6 +1035,7 @@ struct Solver {
       for (int hh = 0; hh < p->n_inner; hh++) {
         penalties();
         assemble();
+        if (REF && p->record_storage != ORC_REC_F32) store_records();
         for (int it = 0; it < p->n_sor; it++) {
           const float om = p->omega > 0 ? p->omega : .7f;
           const CT m0 = relax_colour(coef.data(), W, H, 0, du.data(), om, p->pixel_gs != 0);
@@ -1175,6 +1206,10 @@ void orc_depth_map_pyramid(const double* depth, int W, int H, double ratio, int 
   for (auto& L : P) { std::memcpy(out + off, L.d.data(), L.d.size() * sizeof(double)); off += L.d.size(); }
 }
 void orc_im2feature(const double* bgr, int W, int H, double* out5) { im2feature(bgr, W, H, out5); }
+void orc_record_round(float* c13, int storage) {
+  if (storage != ORC_REC_F32) record_round(c13, storage);
+}
+
 double orc_img_round(double x, int storage) { return img_round(x, storage); }
 void orc_warp_cpu(const double* I1, const double* D1, const double* I2, const double* D2, const double* u,
                   const double* v, const double* w, int W, int H, int C, double* wI, double* wD) {

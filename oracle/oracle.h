@@ -47,7 +47,13 @@ typedef struct {
                              (point SOR order; convergent for any factor in (0, 2)) */
   int image_storage;      /* ORC_IMG_*: the feature images (frame 1, frame 2, warped frame 2)
                              rounded, wherever they are produced, to f32 and then to 16 bits */
+  int record_storage;     /* ORC_REC_*: the level solve's per-pixel systems rounded, right after
+                             assembly, as 16-bit records hold them (reference precision only) */
 } orc_params;
+
+enum { ORC_REC_F32 = 0, ORC_REC_F16 = 1, ORC_REC_BF16 = 2 };
+/* one pixel's 13 system coefficients rounded as that record storage holds them */
+void orc_record_round(float* c13, int storage);
 
 enum { ORC_IMG_F64 = 0, ORC_IMG_F16 = 1, ORC_IMG_BF16 = 2 };
 /* x rounded as a feature image of that storage holds it (nearest-even at each step) */

@@ -70,6 +70,30 @@ def test_storage_variants(ctx, R, storage):
     assert np.median(epe) < 0.05 and np.isfinite(tiled[0]).all()
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_record_storage_matches_oracle(ctx, R, storage):
+    """16-bit system records: the device equals the oracle run with the same
+    record rounding (FP16 with a per-pixel power-of-two scale, or BF16),
+    bit for bit and with the same iterations, in both relaxation schedules;
+    with the point-SOR order too."""
+    O = oracle()
+    W, H = 160, 120
+    pair, im1, d1, im2, d2, sigma, hpb = _case(W, H, seed=63)
+    st = R.STORE_F16 if storage == "f16" else R.STORE_BF16
+    for gs in (False, True):
+        kw = dict(n_outer=4, n_sor=40, min_width=20, omega=1.8 if gs else 0.7)
+        ua, va, wa, sa = O.multiscale(im1, d1, im2, d2, sigma, hpb,
+                                      O.make_params(W, H, record_storage=st, pixel_gs=gs, **kw))
+        for variant in (R.RELAX_PLAIN, R.RELAX_TILED):
+            print(f"{storage} records, point SOR {gs}, relax variant {variant}", flush=True)
+            pg = R.make_params(W, H, relax_variant=variant, storage=st,
+                               pixel_update=R.PIXEL_GS if gs else R.PIXEL_JACOBI, **kw)
+            ub, vb, wb, sb = ctx.multiscale(pg, im1, d1, im2, d2, sigma, hpb)
+            L = sa.n_levels
+            assert [sb.sweeps_exec[k] for k in range(L)] == [sa.sweeps_exec[k] for k in range(L)]
+            assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
+
+
 # ----------------------------------------- point SOR order and the FP64 residual
 @pytest.fixture(scope="module")
 def system():

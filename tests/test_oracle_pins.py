@@ -636,3 +636,34 @@ def test_colour_fill_rounds():
     h4[5:8, 5:8] = 0.0
     f4, left = O.colour_fill_rounds(h4, col)
     assert left == 0 and abs(f4[6, 6] - np.delete(f4[5:8, 5:8].ravel(), 4).mean()) < 1e-6
+
+
+def test_record_storage_rounding():
+    """The 16-bit system records: BF16 equals torch's bfloat16 conversion of
+    each f32 coefficient; FP16 scales the pixel's 13 coefficients by 2^-e so
+    that the largest is in [2^14, 2^15), rounds with numpy's float16
+    conversion and scales back -- values far below the largest lose their
+    low bits (or flush to zero) as FP16's subnormals do.  A planted mistake
+    (no scale: FP16 overflow above 65504) fails."""
+    import torch
+    rng = np.random.default_rng(8)
+    rows = [rng.normal(size=13) * 10.0 ** rng.uniform(-8, 8) for _ in range(300)]
+    rows += [np.r_[1e6, rng.normal(size=12) * 1e-6], np.r_[-3e-30, rng.normal(size=12) * 1e-31], np.zeros(13),
+             np.r_[65504.0 * 3, np.ones(12)], np.r_[2.0 ** 14, 2.0 ** 15 - 1, np.full(11, 2.0 ** -11 * 3)]]
+    c = np.array(rows, np.float32)
+    got = O.record_round(c, O.REC_F16)
+    m = np.abs(c).max(1)
+    _, ex = np.frexp(m)
+    e = np.where(m > 0, np.clip(ex - 1 - 14, -100, 100), 0)
+    down = np.ldexp(np.float32(1), -e).astype(np.float32)[:, None]
+    up = np.ldexp(np.float32(1), e).astype(np.float32)[:, None]
+    want = ((c * down).astype(np.float16).astype(np.float32) * up).astype(np.float32)
+    assert np.array_equal(got, want)
+    big = (np.abs(got).max(1) > 0) & (np.abs(e) < 100)      # scale not clamped
+    scaled = np.abs(got[big] * down[big]).max(1)
+    assert np.all(scaled >= 2.0 ** 14) and np.all(scaled <= 2.0 ** 15)   # 2^15 - 1 rounds up to 2^15
+    assert (np.abs(e) == 100).any()
+    assert not np.array_equal(got, c.astype(np.float16).astype(np.float32))          # unscaled: overflows
+    gotb = O.record_round(c, O.REC_BF16)
+    assert np.array_equal(gotb, torch.from_numpy(c).to(torch.bfloat16).to(torch.float32).numpy())
+    assert np.array_equal(O.record_round(c, O.REC_F32), c)
