@@ -75,6 +75,7 @@ struct StepPlan {
   Views vpix{}, vplain{}, vtiled{}, votile{};
   bool tiled_ok = false;
   int cfg = 0, slots = 0, T = 0;  // tiled launch shape: workgroup shape, LDS slots, iterations
+  int grid = 0;                    // tiled launch grid (tiled_prepare, when the step is packed)
   int storage = 0;                 // system record storage of the step's items
   int img = 0;                     // feature-image storage of the step's items
   int relax_mode = kRelaxFixed;    // RelaxMode of the step's items
@@ -555,6 +556,7 @@ void pack_step(std::vector<char>& host, const std::vector<LevelView>& items, Ste
   st.plain = plain[st.n];
   st.tiled = tiled[st.n];
   st.otile = otile[st.n];
+  st.grid = st.tiled_ok && st.tiled > 0 ? tiled_prepare(st.cfg, st.storage, st.relax_mode, st.slots, st.tiled) : 0;
   st.any_early = false;
   for (const LevelView& v : items) st.any_early = st.any_early || v.allow_early;
   st.items_off = pack(host, items.data(), items.size() * sizeof(LevelView));
@@ -629,8 +631,8 @@ void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, i
       const int chk = check_of(last);
       const int slot = chk >= 0 ? slot_base + chk : -1;
       relax_event(c);
-      launch_relax_tiled(s, st.vtiled, st.tiled, st.cfg, st.storage, st.relax_mode, st.slots, flip, T, slot_base,
-                         n_prev, resid ? -1 : slot);
+      launch_relax_tiled(s, st.vtiled, st.tiled, st.grid, st.cfg, st.storage, st.relax_mode, st.slots, flip, T,
+                         slot_base, n_prev, resid ? -1 : slot);
       relax_event(c);
       // the launch read du (flip 0) or du_alt (flip 1) and wrote the other
       if (resid && chk >= 0) launch_residual(s, st.vpix, st.pix, flip ? 0 : 1, slot_base, chk, slot, k.residual_tol);

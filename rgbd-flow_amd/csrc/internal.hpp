@@ -303,7 +303,10 @@ inline int tiled_slots(const TileSched& t, int W, int H, int T) {
 }
 // mode (RelaxMode): kRelaxFixed when every item relaxes as the reference (factor 0.7, old
 // values within the pixel), kRelaxGS when every item uses the point SOR order, else kRelaxFree.
-void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int storage, int mode,
+// grid: tiled_prepare's for the same (config, storage, mode, slots, blocks),
+// obtained when the launch was planned (outside any stream capture)
+int tiled_prepare(int config, int storage, int mode, int slots, int blocks);
+void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int grid, int config, int storage, int mode,
                         int slots, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot);
 void launch_copy_du(hipStream_t s, const Views& v, int blocks);
 void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H, int storage);

@@ -22,6 +22,9 @@
 #include "relax_pixel.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 
 namespace rgbdflow {
 
@@ -344,40 +347,51 @@ void launch_copy_du(hipStream_t s, const Views& v, int blocks) {
   hipLaunchKernelGGL(k_copy_du, dim3(blocks), dim3(kBlock), 0, s, v);
 }
 
-void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int config, int storage, int mode,
-                        int slots, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot) {
+// Grid of a persistent tiled launch (resident workgroups, a multiple of 8,
+// at most one per tile), and the kernel's dynamic-LDS limit raised to what
+// it needs.  Called when a launch is planned, never while a stream is being
+// captured: the capture then holds kernel launches only.  The occupancy of
+// each (device, kernel, LDS size) is queried once per process.
+int tiled_prepare(int config, int storage, int mode, int slots, int blocks) {
   const TiledConfig& c = kConfigs[config];
   const TiledFn fn = c.fn[storage][mode];
   const int key = (config * 3 + storage) * 3 + mode;
   const size_t lds = tiled_lds_bytes(slots);
-  // per-thread caches: launches may come from several host threads (one per context)
-  thread_local size_t raised[160] = {};
-  if (lds > 65536 && lds > raised[key]) {
+  int dev = 0;
+  hipGetDevice(&dev);
+  static std::mutex mu;
+  static std::map<int, size_t> raised;                     // (device, kernel) -> LDS limit set
+  static std::map<std::pair<int, long long>, int> occ;      // (device, kernel x LDS) -> workgroups per CU
+  static std::map<int, int> cus;                           // device -> compute units
+  std::lock_guard<std::mutex> lk(mu);
+  const int fkey = dev * 1024 + key;
+  if (lds > 65536 && lds > raised[fkey]) {
     hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    raised[key] = lds;
+    raised[fkey] = lds;
   }
-  // resident workgroups per CU for this shape (cached per kernel and LDS size)
-  thread_local int occ_cfg[160] = {}, occ_lds[160] = {}, occ_val[160] = {}, cus = 0;
-  if (occ_lds[key] != (int)lds || occ_cfg[key] == 0) {
+  const std::pair<int, long long> okey(fkey, (long long)lds);
+  auto it = occ.find(okey);
+  if (it == occ.end()) {
     int n = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(fn), c.nt, lds) !=
             hipSuccess ||
         n < 1)
       n = 1;
-    occ_val[key] = n;
-    occ_lds[key] = (int)lds;
-    occ_cfg[key] = 1;
+    it = occ.emplace(okey, n).first;
   }
-  if (cus == 0) {
-    int dev = 0;
-    hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
-  }
+  int& cu = cus[dev];
+  if (cu == 0 && (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu < 1))
+    cu = 256;
   const int per_xcd = (blocks + 7) / 8;
-  int grid = std::min(8 * per_xcd, cus * occ_val[key]);
-  grid = std::max(8, grid & ~7);
-  hipLaunchKernelGGL(fn, dim3(grid), dim3(c.nt), lds, s, v, blocks, flip, n_iter, slot_base, n_prev_checks,
-                     check_slot);
+  const int grid = std::min(8 * per_xcd, cu * it->second);
+  return std::max(8, grid & ~7);
+}
+
+void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int grid, int config, int storage, int mode,
+                        int slots, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot) {
+  const TiledConfig& c = kConfigs[config];
+  hipLaunchKernelGGL(c.fn[storage][mode], dim3(grid), dim3(c.nt), tiled_lds_bytes(slots), s, v, blocks, flip, n_iter,
+                     slot_base, n_prev_checks, check_slot);
 }
 
 }  // namespace rgbdflow

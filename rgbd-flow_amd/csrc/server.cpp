@@ -15,6 +15,14 @@
 // worker only launches work and device-to-device copies.  Results do not depend on how
 // requests are batched: every pair's solve is the same whatever else is in
 // its batch.
+//
+// Every HIP API call the server makes, in any thread, runs under one mutex
+// (hip_mu): the worker holds it while it enqueues a batch -- including the
+// capture, instantiation and launch of the batch's HIP graph -- and the
+// clients while they allocate, free or enqueue a copy.  Only waits (stream
+// synchronisation, event queries) run outside it.  Host frames and results
+// pass through pinned host buffers, so a client holds the mutex only for
+// the time it takes to enqueue a DMA copy.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -51,6 +59,7 @@ struct Frame {
   float* hpb = nullptr;
   float* sigma = nullptr;  // fill_holes: depth stdevs of the depth as submitted
   double* dmh = nullptr;   // fill_holes: the depth with small holes colour-filled (boundary maps)
+  double* raw = nullptr;   // fill_holes with warm starts: the depth as submitted (a warm pair's maps)
   bool pyr = false;  // pim / pd hold this frame's pyramids
 };
 
@@ -133,6 +142,8 @@ struct rgbdflow_server {
   std::string err;
   hipStream_t copy_stream = nullptr;       // host<->device copies of submit and poll
   std::multimap<size_t, void*> free_bufs;  // device staging and result buffers by size (lock m)
+  std::multimap<size_t, void*> free_pinned;  // pinned host buffers by size (lock m)
+  std::mutex hip_mu;                       // every HIP API call except waits (see the file comment)
 };
 
 namespace {
@@ -155,9 +166,35 @@ void* buf_get(rgbdflow_server* S, size_t bytes) {
     }
   }
   void* p = nullptr;
+  std::lock_guard<std::mutex> hk(S->hip_mu);
   hipSetDevice(S->device);
   if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
   return p;
+}
+
+// A pinned host buffer of exactly `bytes` from the server's pool, else a new
+// one (no lock held).
+void* pinned_get(rgbdflow_server* S, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(S->m);
+    auto it = S->free_pinned.find(bytes);
+    if (it != S->free_pinned.end()) {
+      void* p = it->second;
+      S->free_pinned.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  std::lock_guard<std::mutex> hk(S->hip_mu);
+  hipSetDevice(S->device);
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+
+void pinned_put(rgbdflow_server* S, void* p, size_t bytes) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(S->m);
+  S->free_pinned.emplace(bytes, p);
 }
 
 void buf_put(rgbdflow_server* S, void* p, size_t bytes) {
@@ -191,6 +228,7 @@ void dispatch(rgbdflow_server* S, Lane& ln) {
     Frame& F = st.f[q.next];
     const float* d32 = reinterpret_cast<const float*>(q.in.p + q.in.depth_off);
     launch_ingest(hs, q.in.p, d32, (int)st.N, F.im, F.dm);
+    if (F.raw) hipMemcpyAsync(F.raw, F.dm, st.N * 8, hipMemcpyDeviceToDevice, hs);
     if (S->fill) {
       // the depth stdevs of the depth as submitted (runFlowTwoFrames.cpp:89-91);
       // small holes colour-filled for the boundary maps (sceneFlowMultiscale.cpp:78-83);
@@ -210,8 +248,12 @@ void dispatch(rgbdflow_server* S, Lane& ln) {
       pd.H = st.H;
       pd.min_width = st.min_width;
       pd.cam = st.cam;
+      // the boundary maps of frame 1 (made on the device unless submitted) start from
+      // the colour-filled depth for a coarse-to-fine pair (sceneFlowMultiscale.cpp:78-83)
+      // and from the depth as submitted for a warm start (sceneFlowWithInit.cpp:47)
+      const double* hpb_dm = S->fill ? (q.warm ? P.raw : P.dmh) : nullptr;
       pd.f1 = FrameDev{P.im, P.dm, P.pyr ? P.pim : nullptr, P.pyr ? P.pd : nullptr, q.in.hpb_n ? P.hpb : nullptr,
-                       S->fill ? P.sigma : nullptr, S->fill ? P.dmh : nullptr};
+                       S->fill ? P.sigma : nullptr, hpb_dm};
       pd.f2 = FrameDev{F.im, F.dm, nullptr, nullptr, nullptr, nullptr, nullptr};
       pd.init = q.warm ? st.flow : nullptr;
       ln.pair_of[r] = (int)pairs.size();
@@ -250,18 +292,19 @@ void finish(rgbdflow_server* S, Lane& ln) {
       continue;
     }
     Frame& F = st.f[q.next];
-    if (rest > 0 && !q.warm) {
-      hipMemcpyAsync(F.pim, f2.pim, 3 * rest * 8, hipMemcpyDeviceToDevice, hs);
-      hipMemcpyAsync(F.pd, f2.pd, rest * 8, hipMemcpyDeviceToDevice, hs);
-    }
     F.pyr = !q.warm;  // a warm start builds no pyramids
-    if (S->warm) hipMemcpyAsync(st.flow, flow, 3 * st.N * 8, hipMemcpyDeviceToDevice, hs);
     res.bytes = 3 * st.N * 8;
     res.flow = static_cast<double*>(buf_get(S, res.bytes));
     if (!res.flow) {
       res.info.status = RGBDFLOW_ERR_HIP;
       continue;
     }
+    std::lock_guard<std::mutex> hk(S->hip_mu);
+    if (rest > 0 && !q.warm) {
+      hipMemcpyAsync(F.pim, f2.pim, 3 * rest * 8, hipMemcpyDeviceToDevice, hs);
+      hipMemcpyAsync(F.pd, f2.pd, rest * 8, hipMemcpyDeviceToDevice, hs);
+    }
+    if (S->warm) hipMemcpyAsync(st.flow, flow, 3 * st.N * 8, hipMemcpyDeviceToDevice, hs);
     hipMemcpyAsync(res.flow, flow, res.bytes, hipMemcpyDeviceToDevice, hs);
     res.info.has_flow = 1;
   }
@@ -341,7 +384,12 @@ void worker_main(rgbdflow_server* S) {
     for (Lane& ln : S->lanes) {
       if (!ln.busy) continue;
       idle = false;
-      if (hipEventQuery(ln.done) == hipSuccess) {
+      hipError_t q;
+      {
+        std::lock_guard<std::mutex> hk(S->hip_mu);
+        q = hipEventQuery(ln.done);
+      }
+      if (q == hipSuccess) {
         lk.unlock();
         finish(S, ln);
         lk.lock();
@@ -355,7 +403,10 @@ void worker_main(rgbdflow_server* S) {
       ln.busy = true;
       idle = false;
       lk.unlock();
-      dispatch(S, ln);
+      {
+        std::lock_guard<std::mutex> hk(S->hip_mu);
+        dispatch(S, ln);
+      }
       lk.lock();
       progressed = true;
     }
@@ -428,6 +479,7 @@ int rgbdflow_server_destroy(rgbdflow_server* S) {
   for (const Request& q : S->pending)
     if (q.in.p) hipFree(q.in.p);
   for (auto& kv : S->free_bufs) hipFree(kv.second);
+  for (auto& kv : S->free_pinned) hipHostFree(kv.second);
   if (S->copy_stream) hipStreamDestroy(S->copy_stream);
   delete S;
   return RGBDFLOW_OK;
@@ -453,15 +505,20 @@ int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera
   const size_t frame_bytes = r256(3 * st->N * 8) + r256(st->N * 8) + r256(3 * st->rest * 8 + 8) +
                              r256(st->rest * 8 + 8) + r256(st->hpb_n * 4);
   const size_t fill_bytes =
-      S->fill ? 2 * (r256(st->N * 4) + r256(st->N * 8)) + 3 * r256(st->N * 4) + r256(2 * st->N * 4) + r256(4) : 0;
+      S->fill ? 2 * (r256(st->N * 4) + r256(st->N * 8) + (S->warm ? r256(st->N * 8) : 0)) + 3 * r256(st->N * 4) +
+                    r256(2 * st->N * 4) + r256(4)
+              : 0;
   const size_t bytes = 2 * frame_bytes + (S->warm ? r256(3 * st->N * 8) : 0) + fill_bytes;
   {
     std::lock_guard<std::mutex> lk(S->m);
     if (S->budget && S->state_bytes + bytes > S->budget)
       return sfail(S, RGBDFLOW_ERR_BUDGET, "stream state exceeds the HBM budget");
   }
-  hipSetDevice(S->device);
-  if (hipMalloc(&st->mem, bytes) != hipSuccess) return sfail(S, RGBDFLOW_ERR_HIP, "hipMalloc of the stream state");
+  {
+    std::lock_guard<std::mutex> hk(S->hip_mu);
+    hipSetDevice(S->device);
+    if (hipMalloc(&st->mem, bytes) != hipSuccess) return sfail(S, RGBDFLOW_ERR_HIP, "hipMalloc of the stream state");
+  }
   st->bytes = bytes;
   char* p = static_cast<char*>(st->mem);
   auto take = [&](size_t n) {
@@ -481,6 +538,7 @@ int rgbdflow_stream_open(rgbdflow_server* S, int W, int H, const rgbdflow_camera
     for (Frame& F : st->f) {
       F.sigma = reinterpret_cast<float*>(take(st->N * 4));
       F.dmh = reinterpret_cast<double*>(take(st->N * 8));
+      if (S->warm) F.raw = reinterpret_cast<double*>(take(st->N * 8));
     }
     st->fill_st = reinterpret_cast<int*>(take(st->N * 4));
     st->fill_label = reinterpret_cast<int*>(take(st->N * 4));
@@ -516,11 +574,23 @@ int rgbdflow_stream_submit(rgbdflow_server* S, int stream_id, const unsigned cha
   in.bytes = in.hpb_off + r256(in.hpb_n * 4);
   in.p = static_cast<unsigned char*>(buf_get(S, in.bytes));
   if (!in.p) return sfail(S, RGBDFLOW_ERR_HIP, "hipMalloc of a frame staging buffer");
-  hipSetDevice(S->device);
-  hipError_t e = hipMemcpyAsync(in.p, bgr, 3 * N, hipMemcpyHostToDevice, S->copy_stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(in.p + in.depth_off, depth, N * 4, hipMemcpyHostToDevice, S->copy_stream);
-  if (e == hipSuccess && hpb) e = hipMemcpyAsync(in.p + in.hpb_off, hpb, hn * 4, hipMemcpyHostToDevice, S->copy_stream);
+  // the frame into a pinned buffer (host copy, no lock), then one DMA copy
+  unsigned char* h = static_cast<unsigned char*>(pinned_get(S, in.bytes));
+  if (!h) {
+    buf_put(S, in.p, in.bytes);
+    return sfail(S, RGBDFLOW_ERR_HIP, "hipHostMalloc of a frame staging buffer");
+  }
+  std::memcpy(h, bgr, 3 * N);
+  std::memcpy(h + in.depth_off, depth, N * 4);
+  if (hpb) std::memcpy(h + in.hpb_off, hpb, hn * 4);
+  hipError_t e;
+  {
+    std::lock_guard<std::mutex> hk(S->hip_mu);
+    hipSetDevice(S->device);
+    e = hipMemcpyAsync(in.p, h, in.bytes, hipMemcpyHostToDevice, S->copy_stream);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(S->copy_stream);
+  pinned_put(S, h, in.bytes);
   if (e != hipSuccess) {
     buf_put(S, in.p, in.bytes);
     return sfail(S, RGBDFLOW_ERR_HIP, std::string("frame upload: ") + hipGetErrorString(e));
@@ -560,12 +630,20 @@ int rgbdflow_stream_poll(rgbdflow_server* S, unsigned long long ticket, int wait
   if (r.flow && status == RGBDFLOW_OK && r.info.has_flow) {
     // the result leaves the device here, in the caller's thread
     const size_t n = r.bytes / 24;
-    hipSetDevice(S->device);
-    hipError_t e = hipSuccess;
-    double* out[3] = {u, v, w};
-    for (int c = 0; c < 3 && e == hipSuccess; c++)
-      if (out[c]) e = hipMemcpyAsync(out[c], r.flow + c * n, n * 8, hipMemcpyDeviceToHost, S->copy_stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(S->copy_stream);
+    double* h = static_cast<double*>(pinned_get(S, r.bytes));
+    hipError_t e = h ? hipSuccess : hipErrorOutOfMemory;
+    if (h) {
+      {
+        std::lock_guard<std::mutex> hk(S->hip_mu);
+        hipSetDevice(S->device);
+        e = hipMemcpyAsync(h, r.flow, r.bytes, hipMemcpyDeviceToHost, S->copy_stream);
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(S->copy_stream);
+      double* out[3] = {u, v, w};
+      for (int c = 0; c < 3 && e == hipSuccess; c++)
+        if (out[c]) std::memcpy(out[c], h + c * n, n * 8);
+      pinned_put(S, h, r.bytes);
+    }
     if (e != hipSuccess) status = RGBDFLOW_ERR_HIP;
   }
   buf_put(S, r.flow, r.bytes);
@@ -587,6 +665,7 @@ int rgbdflow_server_set_relax_schedule(rgbdflow_server* S, int W, int H, const r
       if (ln.busy) return false;
     return true;
   });
+  std::lock_guard<std::mutex> hk(S->hip_mu);
   for (Lane& ln : S->lanes) {
     const int rc = rgbdflow_set_relax_schedule(ln.ctx, W, H, choice);
     if (rc != RGBDFLOW_OK) return sfail(S, rc, rgbdflow_last_error(ln.ctx));
@@ -603,7 +682,10 @@ int rgbdflow_stream_close(rgbdflow_server* S, int stream_id) {
   st->closing = true;
   S->cv_done.wait(lk, [&] { return !st->busy && st->queued == 0; });
   S->state_bytes -= st->bytes;
-  free_stream(*st);
+  {
+    std::lock_guard<std::mutex> hk(S->hip_mu);
+    free_stream(*st);
+  }
   S->streams.erase(it);
   return RGBDFLOW_OK;
 }

@@ -199,3 +199,47 @@ def test_serving_fill_holes_equals_alone(R, max_batch, lanes, mode):
                 u, v, w = ctx.batch_read(0)
                 ub, vb, wb = results[(sid, t)]
                 assert np.array_equal(u, ub) and np.array_equal(v, vb) and np.array_equal(w, wb), (sid, t)
+
+
+def test_serving_warm_fill_holes_maps_from_the_raw_depth(R):
+    """Warm starts with fill_holes and boundary maps made on the device: a
+    warm pair is sceneFlowWithInit, whose maps come from frame 1's depth as
+    measured (computeHeuristicPb(Im1, dm1), sceneFlowWithInit.cpp:47), while
+    everything else uses the hole-filled depths (:19-21).  Each warm result
+    equals the pair solved alone that way, bit for bit."""
+    S = synth()
+    W, H, mw = 128, 96, 16
+    p = R.make_params(W, H, n_outer=3, n_sor=25, min_width=mw, relax_variant=R.RELAX_TILED, graph=True)
+    frames, pair = _stream_frames(S, W, H, 610)
+    rng = np.random.default_rng(6)
+    holed = []
+    for img, d in frames:
+        d = d.copy()
+        d[rng.random(d.shape) < 0.03] = 0.0
+        d[H // 3:H // 3 + 5, W // 4:W // 4 + 7] = 0.0
+        holed.append((img, d))
+    order = [0, 1, 0, 1]
+    with R.Server(p, max_batch=4, lanes=2, warm_start=True, warm_n_outer=2, fill_holes=True) as srv:
+        sid = srv.open(W, H, R.make_params(W, H).cam, mw)
+        tickets = [(t, srv.submit(sid, holed[fi][0], holed[fi][1], None)) for t, fi in enumerate(order)]
+        results = {}
+        for (t, tk) in tickets:
+            print(f"poll frame {t}", flush=True)
+            info, flow = srv.poll(tk, wait_ms=60000)
+            assert info["status"] == 0 and info["has_flow"] == (t > 0)
+            results[t] = flow
+    q = R.make_params(W, H, n_outer=2, n_sor=25, min_width=mw, relax_variant=R.RELAX_TILED)
+    with R.Context(0) as ctx:
+        for t in range(2, len(order)):
+            a, b = order[t - 1], order[t]
+            fa, _ = ctx.fill_depth_holes(holed[a][1].astype(np.float64))
+            fb, _ = ctx.fill_depth_holes(holed[b][1].astype(np.float64))
+            pr = dict(im1=holed[a][0].astype(np.float64) / 255.0, dm1=fa, im2=holed[b][0].astype(np.float64) / 255.0,
+                      dm2=fb, sigma1=ctx.depth_stdevs(holed[a][1]), hpb=None, cam=R.make_params(W, H).cam,
+                      min_width=mw, est=np.stack(results[t - 1], axis=-1),
+                      hpb_dm1=holed[a][1].astype(np.float64))
+            ctx.batch_load(q, [pr])
+            ctx.batch_run(q, stats=False)
+            u, v, w = ctx.batch_read(0)
+            ub, vb, wb = results[t]
+            assert np.array_equal(u, ub) and np.array_equal(v, vb) and np.array_equal(w, wb), t
