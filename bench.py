@@ -553,7 +553,8 @@ def serve_frames(R, spec, n_frames):
 
 
 def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_outer, autotune=True, lanes=2, seed=7,
-              score_every=8, warmup=1.0, device_hpb=False, images="f64", fill_holes=False):
+              score_every=8, warmup=1.0, device_hpb=False, images="f64", fill_holes=False, graph=True,
+              save_schedules=None, load_schedules=None):
     """Streams submit frames at Poisson arrival times (rate fps each); stream k
     walks its size's sequence back and forth from its own starting frame.
     Arrivals run for warmup + seconds; requests that arrive in the first
@@ -565,16 +566,25 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
     import rgbdflow_amd.synth as S
     sizes = sorted(set(spec), key=lambda t: -t[0])
     st = {"f32": R.STORE_F32, "f16": R.STORE_F16, "bf16": R.STORE_BF16}[storage]
-    p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED, graph=True,
+    p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED, graph=graph,
                       storage=st, image_storage={"f64": R.IMAGES_F64, "f16": R.IMAGES_F16, "bf16": R.IMAGES_BF16}[images])
     srv = R.Server(p, max_batch=max_batch, lanes=lanes, warm_start=warm, warm_n_outer=warm_outer,
                    fill_holes=fill_holes)
-    if autotune:
+    if load_schedules:
+        with open(load_schedules) as f:
+            for ch in json.load(f)["served"]:
+                srv.set_relax_schedule(ch["W"], ch["H"], ch)
+    elif autotune:
+        tuned = []
         with R.Context(0) as tctx:
             for (w, h) in sizes:
                 for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
-                    srv.set_relax_schedule(lw, lh, tctx.autotune_relax(lw, lh, batch=max(1, min(4, max_batch)),
-                                                                       params=p))
+                    ch = tctx.autotune_relax(lw, lh, batch=max(1, min(4, max_batch)), params=p)
+                    srv.set_relax_schedule(lw, lh, ch)
+                    tuned.append(dict(W=lw, H=lh, **ch))
+        if save_schedules:
+            with open(save_schedules, "w") as f:
+                json.dump({"served": tuned}, f)
     sids = [srv.open(w, h, R.make_params(w, h).cam, w // 16) for (w, h) in spec]
     rng = np.random.default_rng(seed)
     events = []
@@ -654,7 +664,8 @@ def serve_main(R, args):
     print(f"bench: serving {len(spec)} streams over {args.seconds} s", file=sys.stderr, flush=True)
     r = serve_run(R, spec, frames, args.fps, args.seconds, args.max_batch, args.storage, args.warm,
                   args.warm_outer, autotune=not args.no_autotune, device_hpb=args.device_hpb, images=args.images,
-                  fill_holes=args.fill_holes)
+                  fill_holes=args.fill_holes, graph=not args.no_graph, save_schedules=args.save_schedules,
+                  load_schedules=args.load_schedules)
     out = {
         "metric": "served scene-flow pairs/s and request latency",
         "value": r["pairs_per_s"],
@@ -665,7 +676,8 @@ def serve_main(R, args):
         "data": "synthetic (seeded frame sequences, smooth per-segment SE(3) motion)",
         "config": dict(streams=args.serve, fps_per_stream=args.fps, arrival_seconds=args.seconds,
                        max_batch=args.max_batch, lanes=2, warm_start=args.warm, warm_n_outer=args.warm_outer,
-                       storage=args.storage, images=args.images, fill_holes=args.fill_holes, **r),
+                       storage=args.storage, images=args.images, fill_holes=args.fill_holes,
+                       graph=not args.no_graph, **r),
     }
     print(json.dumps(out), flush=True)
 
