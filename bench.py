@@ -187,6 +187,9 @@ def main():
     ap.add_argument("--device-hpb", action="store_true",
                     help="serving: submit no boundary maps; the server makes them on the device")
     ap.add_argument("--lanes2", action="store_true", help="also time the batch as two halves on two streams")
+    ap.add_argument("--relax-level", default="",
+                    help="instead: time the tiled relaxation alone on --batch levels of this size ('WxH'), "
+                         "one check interval (25 iterations) per timing, --steps timings")
     ap.add_argument("--hpb", default="",
                     help="instead: time computeHeuristicPb on the device for frames of these sizes ('WxH,...')")
     ap.add_argument("--served-streams", type=int, default=8,
@@ -208,6 +211,8 @@ def main():
         return serve_main(R, args)
     if args.hpb:
         return hpb_main(R, args)
+    if args.relax_level:
+        return relax_main(R, args)
     pairs, levels = workload(R, max(1, min(args.distinct, args.batch)))
     B = args.batch
     batch = [pairs[i % len(pairs)] for i in range(B)]
@@ -498,6 +503,41 @@ def mixed_main(R, args):
                    "graph": not args.no_graph},
     }
     ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def relax_main(R, args):
+    """The tiled relaxation alone: --batch levels of one size, systems of the
+    autotuner's synthetic kind stored as --storage, the autotuned (or loaded)
+    schedule; device time per red-black iteration from HIP events around one
+    check interval (25 iterations), median of --steps timings, and the
+    algorithmic rate at 72 (f32) / 48 (16-bit) bytes per pixel and iteration."""
+    w, h = (int(x) for x in args.relax_level.split("x"))
+    B = args.batch
+    st = {"f32": R.STORE_F32, "f16": R.STORE_F16, "bf16": R.STORE_BF16}[args.storage]
+    p = R.make_params(w, h, relax_variant=R.RELAX_TILED, storage=st)
+    ctx = R.Context(0)
+    if args.load_schedules:
+        with open(args.load_schedules) as f:
+            ch = json.load(f)["relax"]
+    else:
+        ch = ctx.autotune_relax(w, h, batch=B, params=p)
+        if args.save_schedules:
+            with open(args.save_schedules, "w") as f:
+                json.dump({"relax": ch}, f)
+    print(f"bench: relaxation of {B} x {w}x{h}, schedule {ch}", file=sys.stderr, flush=True)
+    us = sorted(ctx.time_relax_schedule(w, h, ch, batch=B, params=p) for _ in range(max(1, args.steps)))
+    med = us[len(us) // 2]
+    bpp = 72.0 if args.storage == "f32" else 48.0
+    gbps = B * w * h * bpp / (med * 1e-6) / 1e9
+    ctx.close()
+    out = {"metric": "tiled relaxation, device time per red-black iteration", "value": round(med, 3),
+           "unit": "us per iteration", "n_gpus": 1, "steps": args.steps, "higher_is_better": False,
+           "dtype": "f32" if args.storage == "f32" else f"f32 compute, {args.storage} system storage",
+           "data": "synthetic systems (the autotuner's)",
+           "config": {"workload": f"{B} x {w}x{h} levels", "schedule": ch,
+                      "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(gbps / HBM_PEAK_GBS, 4), "bytes_per_px_iter": bpp}}}
     print(json.dumps(out), flush=True)
 
 

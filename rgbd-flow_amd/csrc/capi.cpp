@@ -1398,9 +1398,10 @@ int rgbdflow_level_system(rgbdflow_ctx* c, int W, int H, float focal_level, cons
   std::vector<float> rec((size_t)kRec * 4 * N);
   TRY(download(c, rec.data(), wk.coef, rec.size() * sizeof(float)));
   TRY(sync(c));
-  // record order (eRxy, eRz, eDxy, eDz, Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw, ...) is the plane order
+  // record order (eRxy, eRz, eDxy, eDz, Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw, ...) is the plane order;
+  // value k of pixel l is component k % 4 of part k / 4, parts being planes of N float4
   for (size_t l = 0; l < N; l++)
-    for (int k = 0; k < kCoef; k++) coef13[(size_t)k * N + l] = rec[l * 16 + k];
+    for (int k = 0; k < kCoef; k++) coef13[(size_t)k * N + l] = rec[((k / 4) * N + l) * 4 + k % 4];
   return RGBDFLOW_OK;
 }
 
@@ -1415,12 +1416,14 @@ int stage_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, const doubl
   TRY(get_work(c, "d.", N, wk));
   // planes -> records (the reciprocals are IEEE single-precision divisions, as on the device)
   std::vector<float> rec((size_t)kRec * 4 * N), hx(4 * N, 0.f);
+  // (parts of the records are planes of N float4: value k of pixel l is component k % 4 of part k / 4)
   for (size_t l = 0; l < N; l++) {
-    float* r = rec.data() + l * 16;
+    float r[16];
     for (int k = 0; k < kCoef; k++) r[k] = (float)coef13[(size_t)k * N + l];
     r[13] = 1.0f / r[kBuu];
     r[14] = 1.0f / r[kBvv];
     r[15] = 1.0f / r[kBww];
+    for (int k = 0; k < 16; k++) rec[((k / 4) * N + l) * 4 + k % 4] = r[k];
     for (int k = 0; k < 3; k++) hx[l * 4 + k] = (float)x[(size_t)k * N + l];
   }
   const int storage = sched ? sched->storage : RGBDFLOW_STORE_F32;

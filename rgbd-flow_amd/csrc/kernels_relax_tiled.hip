@@ -171,7 +171,7 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
       PixRegs r{};
       if (valid) {
         float cv[kCoef], yv[3];
-        load_record(v.coef, ST, (size_t)l, cv, yv);
+        load_record(v.coef, ST, (size_t)v.N, (size_t)l, cv, yv);
         // a missing neighbour's coefficient is exactly zero (its value is a zero pad)
         const bool hr = gc < W - 1, hd = gy < H - 1;
         r.rxy = hr ? cv[kERxy] : 0.f;
@@ -319,7 +319,7 @@ __global__ void k_fill_test_system(float4* coef, float4* du, int W, int H, int s
     const float er = j < W - 1 ? 0.1f : 0.f, ed = i < H - 1 ? 0.1f : 0.f;
     const float c[kCoef] = {er, er, ed, ed, -1.f, -1.f, -1.f, 0.01f, 0.01f, 0.01f,
                             1e-3f * (l % 7), 1e-3f * (l % 5), 1e-3f * (l % 3)};
-    store_record(coef, storage, (size_t)l, c);
+    store_record(coef, storage, (size_t)N, (size_t)l, c);
     du[l] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
@@ -328,8 +328,8 @@ __global__ void k_fill_test_system(float4* coef, float4* du, int W, int H, int s
 __global__ void k_encode_records(const float4* src, float4* dst, int N, int storage) {
   for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < N; l += gridDim.x * blockDim.x) {
     float c[kCoef], y[3];
-    load_record(src, kStoreF32, (size_t)l, c, y);
-    store_record(dst, storage, (size_t)l, c);
+    load_record(src, kStoreF32, (size_t)N, (size_t)l, c, y);
+    store_record(dst, storage, (size_t)N, (size_t)l, c);
   }
 }
 

@@ -293,7 +293,7 @@ __device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars&
   t = (float)(-(P * Dt + alpha * -wL));
   const float rw = (float)((double)t + -(-gamma * m * u2));
   const float rec[kCoef] = {eRxy, eRz, eDxy, eDz, Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw};
-  store_record(v.coef, v.storage, (size_t)l, rec);
+  store_record(v.coef, v.storage, (size_t)v.N, (size_t)l, rec);
 }
 
 // The system of an inner iteration after the first, from the stored
@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
   if (i < H && j < W) {
     const int l = i * W + j;
     const bool hl = j > 0, hu = i > 0, hr = j < W - 1, hd = i < H - 1;
-    const PixCoef c = load_coef(v.coef, v.storage, W, l, hl, hu);
+    const PixCoef c = load_coef(v.coef, v.storage, W, v.N, l, hl, hu);
     float4* x = v.du;
     const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 o = gload(x + l), L = hl ? gload(x + l - 1) : zero, U = hu ? gload(x + l - W) : zero,
@@ -609,7 +609,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(Views wl, int alt, int slot
   if (l < N) {
     const int i = l / W, j = l - i * W;
     const bool hl = j > 0, hu = i > 0, hr = j < W - 1, hd = i < H - 1;
-    const PixCoef c = load_coef(v.coef, v.storage, W, l, hl, hu);
+    const PixCoef c = load_coef(v.coef, v.storage, W, v.N, l, hl, hu);
     const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 o = gload(x + l), L = hl ? gload(x + l - 1) : zero, U = hu ? gload(x + l - W) : zero,
                  R = hr ? gload(x + l + 1) : zero, D = hd ? gload(x + l + W) : zero;

@@ -7,6 +7,10 @@
 //   two exponent e (values are stored as c * 2^-e so the largest is in
 //   [2^14, 2^15), inside FP16's range); the reciprocals are recomputed from the
 //   decoded diagonal when a record is loaded.  Arithmetic stays f32.
+// Layout: the 16-byte parts of the records are planes of N (structure of
+// arrays): part q of pixel l is at base[q * N + l] (float4 units for F32,
+// uint4 units for F16 / BF16), so the lanes of a wave that load part q of
+// neighbouring pixels read contiguous memory.
 #pragma once
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -52,14 +56,14 @@ __device__ __forceinline__ void img_st(double* base, size_t i, double x) {
 // Power-of-two scale 2^k as a float (k within the normal range).
 __device__ __forceinline__ float pow2f(int k) { return __uint_as_float((unsigned)(k + 127) << 23); }
 
-// Writes pixel l's record; c: the 13 values in Coef order.
-__device__ __forceinline__ void store_record(float4* base, int storage, size_t l, const float (&c)[kCoef]) {
+// Writes pixel l's record (N: pixels of the level); c: the 13 values in Coef order.
+__device__ __forceinline__ void store_record(float4* base, int storage, size_t N, size_t l,
+                                             const float (&c)[kCoef]) {
   if (storage == kStoreF32) {
-    float4* C = base + l * kRec;
-    gstore(C + 0, make_float4(c[0], c[1], c[2], c[3]));
-    gstore(C + 1, make_float4(c[4], c[5], c[6], c[7]));
-    gstore(C + 2, make_float4(c[8], c[9], c[10], c[11]));
-    gstore(C + 3, make_float4(c[12], 1.0f / c[kBuu], 1.0f / c[kBvv], 1.0f / c[kBww]));
+    gstore(base + l, make_float4(c[0], c[1], c[2], c[3]));
+    gstore(base + N + l, make_float4(c[4], c[5], c[6], c[7]));
+    gstore(base + 2 * N + l, make_float4(c[8], c[9], c[10], c[11]));
+    gstore(base + 3 * N + l, make_float4(c[12], 1.0f / c[kBuu], 1.0f / c[kBvv], 1.0f / c[kBww]));
     return;
   }
   unsigned short h[16];
@@ -76,19 +80,20 @@ __device__ __forceinline__ void store_record(float4* base, int storage, size_t l
     h[13] = 0;
   }
   h[14] = h[15] = 0;
-  uint4* R = reinterpret_cast<uint4*>(base) + l * 2;
-  gstore(R + 0, make_uint4(h[0] | (unsigned)h[1] << 16, h[2] | (unsigned)h[3] << 16, h[4] | (unsigned)h[5] << 16,
+  uint4* R = reinterpret_cast<uint4*>(base);
+  gstore(R + l, make_uint4(h[0] | (unsigned)h[1] << 16, h[2] | (unsigned)h[3] << 16, h[4] | (unsigned)h[5] << 16,
                            h[6] | (unsigned)h[7] << 16));
-  gstore(R + 1, make_uint4(h[8] | (unsigned)h[9] << 16, h[10] | (unsigned)h[11] << 16,
-                           h[12] | (unsigned)h[13] << 16, h[14] | (unsigned)h[15] << 16));
+  gstore(R + N + l, make_uint4(h[8] | (unsigned)h[9] << 16, h[10] | (unsigned)h[11] << 16,
+                               h[12] | (unsigned)h[13] << 16, h[14] | (unsigned)h[15] << 16));
 }
 
-// Reads pixel l's record: the 13 values and the reciprocals of the diagonal.
-__device__ __forceinline__ void load_record(const float4* base, int storage, size_t l, float (&c)[kCoef],
+// Reads pixel l's record (N: pixels of the level): the 13 values and the
+// reciprocals of the diagonal.
+__device__ __forceinline__ void load_record(const float4* base, int storage, size_t N, size_t l, float (&c)[kCoef],
                                             float (&y)[3]) {
   if (storage == kStoreF32) {
-    const float4* C = base + l * kRec;
-    const float4 r0 = gload(C + 0), r1 = gload(C + 1), r2 = gload(C + 2), r3 = gload(C + 3);
+    const float4 r0 = gload(base + l), r1 = gload(base + N + l), r2 = gload(base + 2 * N + l),
+                 r3 = gload(base + 3 * N + l);
     c[0] = r0.x; c[1] = r0.y; c[2] = r0.z; c[3] = r0.w;
     c[4] = r1.x; c[5] = r1.y; c[6] = r1.z; c[7] = r1.w;
     c[8] = r2.x; c[9] = r2.y; c[10] = r2.z; c[11] = r2.w;
@@ -96,8 +101,8 @@ __device__ __forceinline__ void load_record(const float4* base, int storage, siz
     y[0] = r3.y; y[1] = r3.z; y[2] = r3.w;
     return;
   }
-  const uint4* R = reinterpret_cast<const uint4*>(base) + l * 2;
-  const uint4 a = gload(R + 0), b = gload(R + 1);
+  const uint4* R = reinterpret_cast<const uint4*>(base);
+  const uint4 a = gload(R + l), b = gload(R + N + l);
   const unsigned w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   if (storage == kStoreF16) {
     const int e = (short)(unsigned short)(w[6] >> 16);

@@ -23,13 +23,13 @@ struct PixCoef {
 // A pixel's coefficients from the system records; the left and up edge
 // weights are the left neighbour's right and the upper neighbour's down ones
 // (zero where the neighbour does not exist).
-__device__ __forceinline__ PixCoef load_coef(const float4* C, int storage, int W, int l, bool hl, bool hu) {
+__device__ __forceinline__ PixCoef load_coef(const float4* C, int storage, int W, int N, int l, bool hl, bool hu) {
   PixCoef c;
   float r[kCoef], y[3];
-  load_record(C, storage, (size_t)l, r, y);
+  load_record(C, storage, (size_t)N, (size_t)l, r, y);
   float L[kCoef] = {}, U[kCoef] = {};
-  if (hl) load_record(C, storage, (size_t)(l - 1), L, y);
-  if (hu) load_record(C, storage, (size_t)(l - W), U, y);
+  if (hl) load_record(C, storage, (size_t)N, (size_t)(l - 1), L, y);
+  if (hu) load_record(C, storage, (size_t)N, (size_t)(l - W), U, y);
   c.lxy = hl ? L[kERxy] : 0.f;
   c.lz = hl ? L[kERz] : 0.f;
   c.uxy = hu ? U[kEDxy] : 0.f;
@@ -48,6 +48,14 @@ __device__ __forceinline__ PixCoef load_coef(const float4* C, int storage, int W
   c.rv = r[kRv];
   c.rw = r[kRw];
   return c;
+}
+
+// The reference's step limit (.cu:463-467): x clamped to xo +- 0.3, as two
+// selects (branch-free; a NaN x passes through, as in the reference).
+__device__ __forceinline__ float clamp_step(float r, float xo) {
+  const float lo = xo - .3f, hi = xo + .3f;
+  r = r < lo ? lo : r;
+  return r > hi ? hi : r;
 }
 
 // One pixel's update.  h* say which neighbours exist in the image; x* hold the
@@ -86,10 +94,7 @@ __device__ __forceinline__ void relax_update(const PixCoef& c, float omega, floa
       nv -= c.Bvw * x1;
       nv /= c.Bww;
     }
-    float r = omega * nv + omega_c * xo[k];
-    if (r < xo[k] - .3f) r = xo[k] - .3f;
-    else if (r > xo[k] + .3f) r = xo[k] + .3f;
-    xn[k] = r;
+    xn[k] = clamp_step(omega * nv + omega_c * xo[k], xo[k]);
   }
 }
 
@@ -97,9 +102,11 @@ __device__ __forceinline__ void relax_update(const PixCoef& c, float omega, floa
 // q0 = RN(a y), r = a - b q0 (exact by FMA), RN(q0 + r y) = RN(a/b) whenever
 // no intermediate underflows or overflows, which holds for |b| in
 // [2^-60, 2^60] and a = 0 or |a| in [2^-60, 2^60].
+// (on the bit pattern: positive floats order like their bits, so |a| in
+// [2^-60, 2^60] is one unsigned range test; NaN and infinity fall outside it)
 __device__ __forceinline__ bool recip_div_ok(float a) {
-  const float aa = fabsf(a);
-  return a == 0.f || (aa >= 0x1p-60f && aa <= 0x1p60f);
+  const unsigned ua = __float_as_uint(a) & 0x7fffffffu;
+  return (ua == 0u) | (ua - 0x21800000u <= 0x5d800000u - 0x21800000u);
 }
 __device__ __forceinline__ bool recip_den_ok(float b) {
   const float ab = fabsf(b);
@@ -151,7 +158,7 @@ __device__ __forceinline__ void relax_update_dense(const PixCoef& c, float omega
     }
     nv[k] = a;
   }
-  if (__builtin_expect(den_ok && recip_div_ok(nv[0]) && recip_div_ok(nv[1]) && recip_div_ok(nv[2]), 1)) {
+  if (__builtin_expect(den_ok & recip_div_ok(nv[0]) & recip_div_ok(nv[1]) & recip_div_ok(nv[2]), 1)) {
     nv[0] = recip_div(nv[0], c.Buu, y3[0]);
     nv[1] = recip_div(nv[1], c.Bvv, y3[1]);
     nv[2] = recip_div(nv[2], c.Bww, y3[2]);
@@ -161,12 +168,7 @@ __device__ __forceinline__ void relax_update_dense(const PixCoef& c, float omega
     nv[2] /= c.Bww;
   }
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    float r = omega * nv[k] + omega_c * xo[k];
-    if (r < xo[k] - .3f) r = xo[k] - .3f;
-    else if (r > xo[k] + .3f) r = xo[k] + .3f;
-    xn[k] = r;
-  }
+  for (int k = 0; k < 3; k++) xn[k] = clamp_step(omega * nv[k] + omega_c * xo[k], xo[k]);
 }
 
 // relax_update_dense in the point SOR order (relax_update with gs): each
@@ -204,11 +206,9 @@ __device__ __forceinline__ void relax_update_dense_gs(const PixCoef& c, float om
       a -= c.Buw * xn[0];
       a -= c.Bvw * xn[1];
     }
-    const float q = (den_ok && recip_div_ok(a)) ? recip_div(a, den[k], y3[k]) : a / den[k];
-    float r = omega * q + omega_c * xo[k];
-    if (r < xo[k] - .3f) r = xo[k] - .3f;
-    else if (r > xo[k] + .3f) r = xo[k] + .3f;
-    xn[k] = r;
+    float q = recip_div(a, den[k], y3[k]);
+    if (__builtin_expect(!(den_ok & recip_div_ok(a)), 0)) q = a / den[k];
+    xn[k] = clamp_step(omega * q + omega_c * xo[k], xo[k]);
   }
 }
 

@@ -12,7 +12,10 @@ and produces the short text summaries kept with the project:
   python rgbd-flow_amd/profiling.py traffic --fetch <db> --write <db> --copy-fetch <db>
       --copy-write <db> --copy-mib M --kernel K --batch B --variant V --out <json> --command C
       HBM bytes per launch of the kernels whose name contains K, each counter
-      scaled by the copy calibration (bench.py reads the JSON as roofline.traffic).
+      scaled by the copy calibration (bench.py reads the JSON as roofline.traffic);
+  python rgbd-flow_amd/profiling.py counters <db> [<db> ...] --kernel K --out <json> --command C
+      per-dispatch averages of the counters of --pmc runs and the derived
+      utilisations (VALU busy, LDS bank conflicts, waves per SIMD).
 """
 from __future__ import annotations
 
@@ -59,6 +62,49 @@ def counter_per_dispatch(db_path, counter, kernel_like):
     rows = db.execute("select dispatch_id, sum(value) from counters_collection where counter_name = ? and "
                       "kernel_name like ? group by dispatch_id", (counter, f"%{kernel_like}%")).fetchall()
     return [r[1] for r in rows]
+
+
+def counters(a):
+    """Average per-dispatch value of every counter in the given --pmc runs for
+    the kernels whose name contains a.kernel, and the derived utilisations
+    (CU count from a.cus; each SIMD issues one VALU instruction of a wave in
+    4 cycles)."""
+    vals = {}
+    n = {}
+    for db_path in a.dbs:
+        db = sqlite3.connect(db_path)
+        names = [r[0] for r in db.execute("select distinct counter_name from counters_collection")]
+        for c in names:
+            v = counter_per_dispatch(db_path, c, a.kernel)
+            if v:
+                vals[c] = sum(v) / len(v)
+                n[c] = len(v)
+    cus = a.cus
+    d = {}
+    g = vals.get("GRBM_GUI_ACTIVE")
+    if g:
+        if "SQ_ACTIVE_INST_VALU" in vals:
+            d["valu_busy_pct"] = 100 * vals["SQ_ACTIVE_INST_VALU"] * 4 / (4 * cus) / g
+        if "SQ_LDS_BANK_CONFLICT" in vals:
+            d["lds_bank_conflict_pct_of_cycles"] = 100 * vals["SQ_LDS_BANK_CONFLICT"] / g / cus
+        if "SQ_ACTIVE_INST_LDS" in vals:
+            d["lds_inst_busy_pct"] = 100 * vals["SQ_ACTIVE_INST_LDS"] * 4 / (4 * cus) / g
+        if "SQ_ACTIVE_INST_VMEM" in vals:
+            d["vmem_inst_busy_pct"] = 100 * vals["SQ_ACTIVE_INST_VMEM"] * 4 / (4 * cus) / g
+        if "TA_ADDR_STALLED_BY_TC_CYCLES" in vals and "TA_TA_BUSY" in vals and vals["TA_TA_BUSY"]:
+            d["ta_addr_stalled_by_tc_pct_of_busy"] = 100 * vals["TA_ADDR_STALLED_BY_TC_CYCLES"] / vals["TA_TA_BUSY"]
+    if "SQ_WAVE_CYCLES" in vals and "SQ_BUSY_CYCLES" in vals and vals["SQ_BUSY_CYCLES"]:
+        d["mean_waves_per_sq_while_busy"] = vals["SQ_WAVE_CYCLES"] / vals["SQ_BUSY_CYCLES"]
+    if "SQ_WAIT_INST_LDS" in vals and "SQ_WAVE_CYCLES" in vals and vals["SQ_WAVE_CYCLES"]:
+        d["wave_cycles_waiting_lds_pct"] = 100 * vals["SQ_WAIT_INST_LDS"] / vals["SQ_WAVE_CYCLES"]
+    if "SQ_WAIT_ANY" in vals and "SQ_WAVE_CYCLES" in vals and vals["SQ_WAVE_CYCLES"]:
+        d["wave_cycles_waiting_any_pct"] = 100 * vals["SQ_WAIT_ANY"] / vals["SQ_WAVE_CYCLES"]
+    out = {"kernel": a.kernel, "dispatches": max(n.values()) if n else 0, "cus": cus,
+           "per_dispatch": {k: round(v, 1) for k, v in sorted(vals.items())},
+           "derived": {k: round(v, 2) for k, v in d.items()}, "command": a.command}
+    with open(a.out, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
 
 
 def traffic(a):
@@ -108,9 +154,17 @@ def main(argv=None):
         t.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
     t.add_argument("--copy-mib", dest="copy_mib", type=int, required=True)
     t.add_argument("--batch", type=int, required=True)
+    k = sub.add_parser("counters")
+    k.add_argument("dbs", nargs="+")
+    k.add_argument("--kernel", required=True)
+    k.add_argument("--cus", type=int, default=256)
+    k.add_argument("--out", required=True)
+    k.add_argument("--command", required=True)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         write_stats(a.db, a.out, a.title, a.command, a.detail)
+    elif a.cmd == "counters":
+        counters(a)
     elif a.cmd == "copy":
         copy(a.mib)
     else:
