@@ -37,18 +37,22 @@ __global__ void k_ctl_reset(Views w) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
-// x / 12 in f64, bit-equal to the IEEE division: Markstein's correction with
-// the correctly rounded reciprocal of 12 (IBM J. Res. Dev. 34(1), 1990) away
-// from zero, underflow and overflow; the division itself elsewhere.
-__device__ __forceinline__ double div12(double x) {
+// x / D in f64 for a small integer constant D, bit-equal to the IEEE
+// division: Markstein's correction with the correctly rounded reciprocal of D
+// (IBM J. Res. Dev. 34(1), 1990) away from zero, underflow and overflow; the
+// division itself elsewhere (the CPU check of this recipe for D = 12, 5 and 3
+// is oracle/divcheck.cpp).
+template <int D>
+__device__ __forceinline__ double div_const(double x) {
   const double ax = fabs(x);
   if (__builtin_expect(ax >= 0x1p-960 && ax <= 0x1p960, 1)) {
-    const double y = 1.0 / 12.0;
+    constexpr double y = 1.0 / D;
     const double q0 = x * y;
-    return __builtin_fma(__builtin_fma(-12.0, q0, x), y, q0);
+    return __builtin_fma(__builtin_fma(-(double)D, q0, x), y, q0);
   }
-  return x / 12;
+  return x / D;
 }
+__device__ __forceinline__ double div12(double x) { return div_const<12>(x); }
 
 // Image and depth derivatives of one pixel (.cu:84-147): x/y derivatives of
 // 0.4*I1 + 0.6*warped I2 with [1,-8,0,8,-1]/12, clamp-to-edge; It = Iw - I1;
@@ -208,24 +212,30 @@ __device__ __forceinline__ void load_nbr(const LevelView& v, int l, int i, int j
 
 // .cu:240-333: phi- and r-weighted Laplacian of component c of u (not u+du),
 // normalised by the mean regularisation weight of the existing edges.  pC,
-// pL, pU: phi at the pixel, its left and its upper neighbour.
-__device__ __forceinline__ double laplacian(const PixNbr& nb, int c, int W, int H, int i, int j, double pC, double pL,
-                                            double pU) {
-  const double uc = nb.uC[c];
-  double lap = 0;
-  if (j < W - 1) lap -= (nb.uR[c] - uc) * pC * (double)nb.rR;
-  if (j > 0) lap += (uc - nb.uL[c]) * pL * (double)nb.rRl;
-  if (i < H - 1) lap -= (nb.uD[c] - uc) * pC * (double)nb.rD;
-  if (i > 0) lap += (uc - nb.uU[c]) * pU * (double)nb.rDu;
+// pL, pU: phi at the pixel, its left and its upper neighbour.  The mean
+// (norm / count, the same for the three components) comes from
+// laplacian_mean: 0 when |norm| < 1e-5 (the Laplacian is then 0).
+__device__ __forceinline__ double laplacian_mean(const PixNbr& nb, int W, int H, int i, int j) {
   double norm = 0;
   int count = 0;
   if (j < W - 1) { norm += nb.rR; count++; }
   if (j > 0) { norm += nb.rRl; count++; }
   if (i < H - 1) { norm += nb.rD; count++; }
   if (i > 0) { norm += nb.rDu; count++; }
-  if (fabs(norm) < 1e-5) lap = 0;
-  else lap /= (norm / count);
-  return lap;
+  if (fabs(norm) < 1e-5) return 0.0;
+  // norm / count: exact scalings for 2 and 4 (norm >= 1e-5: no underflow),
+  // the exact constant division for 3
+  return count == 3 ? div_const<3>(norm) : count == 2 ? norm * 0.5 : count == 4 ? norm * 0.25 : norm;
+}
+__device__ __forceinline__ double laplacian(const PixNbr& nb, int c, int W, int H, int i, int j, double pC, double pL,
+                                            double pU, double mean) {
+  const double uc = nb.uC[c];
+  double lap = 0;
+  if (j < W - 1) lap -= (nb.uR[c] - uc) * pC * (double)nb.rR;
+  if (j > 0) lap += (uc - nb.uL[c]) * pL * (double)nb.rRl;
+  if (i < H - 1) lap -= (nb.uD[c] - uc) * pC * (double)nb.rD;
+  if (i > 0) lap += (uc - nb.uU[c]) * pU * (double)nb.rDu;
+  return mean == 0.0 ? 0.0 : lap / mean;
 }
 
 // .cu:356-369 (psi', pi'), .cu:189-230 (products), .cu:477-575 (assembly),
@@ -250,8 +260,9 @@ __device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars&
   }
   for (int k = 0; k < kFeat; k++) stx += psi[k] * dv.Ix[k] * dv.It[k];
   for (int k = 0; k < kFeat; k++) sty += psi[k] * dv.Iy[k] * dv.It[k];
-  const double ixy = sxy / kFeat, ix2 = sxx / kFeat, iy2 = syy / kFeat;
-  const double itx = stx * ql * ql / kFeat, ity = sty * ql * ql / kFeat;
+  static_assert(kFeat == 5, "channel means divide by 5");
+  const double ixy = div_const<5>(sxy), ix2 = div_const<5>(sxx), iy2 = div_const<5>(syy);
+  const double itx = div_const<5>(stx * ql * ql), ity = div_const<5>(sty * ql * ql);
 
   const double Dx = dv.Dx, Dy = dv.Dy, Dt = dv.Dt;
   const double t2 = Dt + Dx * d0 + Dy * d1 - d2;
@@ -259,8 +270,9 @@ __device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars&
   const double dxy = P * Dx * Dy, dx2 = P * Dx * Dx, dy2 = P * Dy * Dy;
   const double dtx = P * Dx * Dt, dty = P * Dy * Dt;
 
-  const double uL = laplacian(nb, 0, W, H, i, j, pC, pL, pU), vL = laplacian(nb, 1, W, H, i, j, pC, pL, pU);
-  const double wL = (double)nb.sC * laplacian(nb, 2, W, H, i, j, pC, pL, pU);
+  const double mean = laplacian_mean(nb, W, H, i, j);
+  const double uL = laplacian(nb, 0, W, H, i, j, pC, pL, pU, mean), vL = laplacian(nb, 1, W, H, i, j, pC, pL, pU, mean);
+  const double wL = (double)nb.sC * laplacian(nb, 2, W, H, i, j, pC, pL, pU, mean);
 
   double wsXY = 0, wsZ = 0;
   float eRxy = 0.f, eRz = 0.f, eDxy = 0.f, eDz = 0.f;

@@ -81,7 +81,8 @@ def counters(a):
                 n[c] = len(v)
     cus = a.cus
     d = {}
-    g = vals.get("GRBM_GUI_ACTIVE")
+    # GRBM_GUI_ACTIVE is summed over the XCDs; the utilisations use one XCD's cycles
+    g = vals.get("GRBM_GUI_ACTIVE") / a.xcds if vals.get("GRBM_GUI_ACTIVE") else None
     if g:
         if "SQ_ACTIVE_INST_VALU" in vals:
             d["valu_busy_pct"] = 100 * vals["SQ_ACTIVE_INST_VALU"] * 4 / (4 * cus) / g
@@ -158,6 +159,7 @@ def main(argv=None):
     k.add_argument("dbs", nargs="+")
     k.add_argument("--kernel", required=True)
     k.add_argument("--cus", type=int, default=256)
+    k.add_argument("--xcds", type=int, default=8)
     k.add_argument("--out", required=True)
     k.add_argument("--command", required=True)
     a = ap.parse_args(argv)
