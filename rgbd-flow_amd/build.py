@@ -59,12 +59,14 @@ def build(verbose: bool = False) -> str:
         _run(["g++", *CXX_FLAGS, *inc, "-shared", "-o", tmp, *cxx, "-L", HERE, "-lrgbdflow",
               f"-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, LIB_CXX)
-        # the drop-in API's test driver (run by tests/test_gpu_dropin.py)
-        drv_src = os.path.join(ROOT, "tests", "cxx", "test_dropin.cpp")
-        if os.path.exists(drv_src):
+        # the drop-in API's test drivers (run by tests/test_gpu_dropin.py and tests/test_host_cxx.py)
+        for name in ("test_dropin", "test_host"):
+            drv_src = os.path.join(ROOT, "tests", "cxx", name + ".cpp")
+            if not os.path.exists(drv_src):
+                continue
             drv_dir = os.path.join(ROOT, "tests", "cxx", "build")
             os.makedirs(drv_dir, exist_ok=True)
-            drv = os.path.join(drv_dir, "test_dropin")
+            drv = os.path.join(drv_dir, name)
             _run(["g++", *CXX_FLAGS, *inc, "-I", os.path.join(ROOT, "include"), "-o", drv + ".tmp", drv_src,
                   "-L", HERE, "-lrgbdflow_cxx", "-lrgbdflow", "-Wl,-rpath,$ORIGIN/../../../rgbd-flow_amd"])
             os.replace(drv + ".tmp", drv)
