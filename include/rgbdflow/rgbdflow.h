@@ -57,9 +57,14 @@ extern "C" {
  * these systems.  GS updates u, then v from the new u, then w from the new u
  * and v: with the red-black pixel order this is point SOR, which converges
  * for any factor in (0, 2) (the systems are symmetric definite) to the same
- * fixed point. */
+ * fixed point.  BLOCK solves the pixel's three unknowns together from its
+ * 3x3 block (x = B^-1 (rhs - neighbour terms), the inverse from the adjugate
+ * in f32), then relaxes and clamps as the reference: block SOR, convergent
+ * for any factor in (0, 2), to the same fixed point.  GS and BLOCK are not
+ * in the reference. */
 #define RGBDFLOW_PIXEL_JACOBI 0
 #define RGBDFLOW_PIXEL_GS 1
+#define RGBDFLOW_PIXEL_BLOCK 2
 
 /* Storage of the per-pixel systems the relaxation reads.  F32 is the
  * reference's precision; F16 (with a per-pixel power-of-two scale) and BF16

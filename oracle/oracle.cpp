@@ -893,7 +893,26 @@ struct Solver {
   /* .cu:382-470: one colour of the red-black relaxation; returns the largest
    * |change| over the updated pixels (the check of .cu:914-918 splits exactly
    * into the two colours' changes). */
-  static CT relax_colour(const CT* coef, int W, int H, int colour, CT* x, float omega_f = .7f, bool gs = false) {
+  /* The inverse of a pixel's symmetric 3x3 block from its adjugate, in the
+   * working precision and this order: cofactors, det = Buu c_uu + Buv c_uv +
+   * Buw c_uw, 1 / det, then each cofactor times it.  inv: (uu, vv, ww, uv,
+   * uw, vw), the block's own order. */
+  static void block_inverse(CT Buu, CT Bvv, CT Bww, CT Buv, CT Buw, CT Bvw, CT* inv) {
+    const CT cuu = Bvv * Bww - Bvw * Bvw, cvv = Buu * Bww - Buw * Buw, cww = Buu * Bvv - Buv * Buv;
+    const CT cuv = Buw * Bvw - Buv * Bww, cuw = Buv * Bvw - Buw * Bvv, cvw = Buv * Buw - Buu * Bvw;
+    const CT det = Buu * cuu + Buv * cuv + Buw * cuw;
+    const CT id = (CT)1 / det;
+    inv[0] = cuu * id; inv[1] = cvv * id; inv[2] = cww * id;
+    inv[3] = cuv * id; inv[4] = cuw * id; inv[5] = cvw * id;
+  }
+
+  /* mode 0: the reference's order inside a pixel (each unknown from the
+   * others' old values, .cu:431-459); 1: point SOR (u, then v from the new
+   * u, then w from both); 2: the pixel's 3x3 block solved as a whole
+   * (block SOR: x^ = B^-1 (rhs - sum_n A_n x_n), B^-1 from the adjugate in
+   * the working precision, block_inverse) -- not in the reference. */
+  static CT relax_colour(const CT* coef, int W, int H, int colour, CT* x, float omega_f = .7f, int mode = 0) {
+    const bool gs = mode == 1;
     const size_t N = (size_t)W * H;
     const CT* eRxy = coef + 0 * N; const CT* eRz = coef + 1 * N;
     const CT* eDxy = coef + 2 * N; const CT* eDz = coef + 3 * N;
@@ -911,6 +930,30 @@ struct Solver {
         const size_t l = (size_t)i * W + j;
         CT xo[3] = {x[l], x[N + l], x[2 * N + l]};
         CT xn[3];
+        if (mode == 2) {
+          CT a[3];
+          for (int k = 0; k < 3; k++) {
+            const CT* xk = x + k * N;
+            CT nv = 0;
+            if (j > 0) nv -= (k < 2 ? eRxy[l - 1] : eRz[l - 1]) * xk[l - 1];
+            if (i > 0) nv -= (k < 2 ? eDxy[l - W] : eDz[l - W]) * xk[l - W];
+            if (j < W - 1) nv -= (k < 2 ? eRxy[l] : eRz[l]) * xk[l + 1];
+            if (i < H - 1) nv -= (k < 2 ? eDxy[l] : eDz[l]) * xk[l + W];
+            nv += rhs[k][l];
+            a[k] = nv;
+          }
+          CT inv[6];
+          block_inverse(Buu[l], Bvv[l], Bww[l], Buv[l], Buw[l], Bvw[l], inv);
+          const CT xh[3] = {inv[0] * a[0] + inv[3] * a[1] + inv[4] * a[2],
+                            inv[3] * a[0] + inv[1] * a[1] + inv[5] * a[2],
+                            inv[4] * a[0] + inv[5] * a[1] + inv[2] * a[2]};
+          for (int k = 0; k < 3; k++) {
+            CT r = omega * xh[k] + (1 - omega) * xo[k];
+            if (r < xo[k] - step) r = xo[k] - step;
+            else if (r > xo[k] + step) r = xo[k] + step;
+            xn[k] = r;
+          }
+        } else
         for (int k = 0; k < 3; k++) {
           const CT* xk = x + k * N;
           CT nv = 0;
@@ -1038,8 +1081,8 @@ struct Solver {
         if (REF && p->record_storage != ORC_REC_F32) store_records();
         for (int it = 0; it < p->n_sor; it++) {
           const float om = p->omega > 0 ? p->omega : .7f;
-          const CT m0 = relax_colour(coef.data(), W, H, 0, du.data(), om, p->pixel_gs != 0);
-          const CT m1 = relax_colour(coef.data(), W, H, 1, du.data(), om, p->pixel_gs != 0);
+          const CT m0 = relax_colour(coef.data(), W, H, 0, du.data(), om, p->pixel_gs);
+          const CT m1 = relax_colour(coef.data(), W, H, 1, du.data(), om, p->pixel_gs);
           sweeps++;
           if (it % checkEvery == checkEvery - 1) {
             const CT md = std::max(m0, m1);
@@ -1145,7 +1188,7 @@ int level_solve_dispatch(const orc_level_in* in, const orc_params* p, int allow_
 
 /* ========================================================================== */
 template <class CT, bool REF>
-static void relax_residual_t(const double* coef13, int W, int H, float omega, bool gs, int max_iter, double tol,
+static void relax_residual_t(const double* coef13, int W, int H, float omega, int gs, int max_iter, double tol,
                              double* x, int* iters, double* residual, double* energy) {
   const size_t N = (size_t)W * H;
   vector<CT> c(coef13, coef13 + 13 * N), xx(x, x + 3 * N);
@@ -1321,9 +1364,9 @@ void orc_relax_residual(const double* coef13, int W, int H, int precision, float
                         double tol, double* x, int* iters, double* residual, double* energy) {
   if (omega <= 0) omega = .7f;
   if (precision == ORC_PREC_REF)
-    relax_residual_t<float, true>(coef13, W, H, omega, pixel_gs != 0, max_iter, tol, x, iters, residual, energy);
+    relax_residual_t<float, true>(coef13, W, H, omega, pixel_gs, max_iter, tol, x, iters, residual, energy);
   else
-    relax_residual_t<double, false>(coef13, W, H, omega, pixel_gs != 0, max_iter, tol, x, iters, residual, energy);
+    relax_residual_t<double, false>(coef13, W, H, omega, pixel_gs, max_iter, tol, x, iters, residual, energy);
 }
 
 /* sceneFlowMultiscale.cpp:30-201 */

@@ -44,7 +44,8 @@ typedef struct {
   double residual_tol;
   int pixel_gs;           /* 0: a pixel's unknowns are updated from each other's old values
                              (.cu:431-459); 1: u, then v from the new u, then w from both
-                             (point SOR order; convergent for any factor in (0, 2)) */
+                             (point SOR order; convergent for any factor in (0, 2)); 2: the
+                             pixel's 3x3 block solved as a whole (block SOR, likewise) */
   int image_storage;      /* ORC_IMG_*: the feature images (frame 1, frame 2, warped frame 2)
                              rounded, wherever they are produced, to f32 and then to 16 bits */
   int record_storage;     /* ORC_REC_*: the level solve's per-pixel systems rounded, right after

@@ -22,7 +22,7 @@ VARIANT_FULL = 2
 RELAX_PLAIN, RELAX_TILED = 0, 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 EXIT_NONE, EXIT_REFERENCE, EXIT_RESIDUAL = 0, 1, 2
-PIXEL_JACOBI, PIXEL_GS = 0, 1
+PIXEL_JACOBI, PIXEL_GS, PIXEL_BLOCK = 0, 1, 2
 IMAGES_F64, IMAGES_F16, IMAGES_BF16 = 0, 1, 2
 
 # Every symbol include/rgbdflow/rgbdflow.h declares.

@@ -538,12 +538,13 @@ void pack_step(std::vector<char>& host, const std::vector<LevelView>& items, Ste
   st.n = (int)items.size();
   st.storage = items.empty() ? 0 : items[0].storage;
   st.img = items.empty() ? 0 : items[0].img;
-  bool fixed = true, gs = !items.empty();
+  bool fixed = true, gs = !items.empty(), block = !items.empty();
   for (const LevelView& v : items) {
-    fixed = fixed && v.omega == .7f && !v.pixel_gs;
-    gs = gs && v.pixel_gs;
+    fixed = fixed && v.omega == .7f && v.pixel_gs == 0;
+    gs = gs && v.pixel_gs == 1;
+    block = block && v.pixel_gs == 2;
   }
-  st.relax_mode = fixed ? kRelaxFixed : gs ? kRelaxGS : kRelaxFree;
+  st.relax_mode = fixed ? kRelaxFixed : gs ? kRelaxGS : block ? kRelaxBlock : kRelaxFree;
   std::vector<int> pix(st.n + 1), plain(st.n + 1), tiled(st.n + 1), otile(st.n + 1);
   pix[0] = plain[0] = tiled[0] = otile[0] = 0;
   for (int i = 0; i < st.n; i++) {
@@ -1061,7 +1062,7 @@ int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
     return fail(c, RGBDFLOW_ERR_ARG, "bad system storage");
   if (p->inner_exit < RGBDFLOW_EXIT_NONE || p->inner_exit > RGBDFLOW_EXIT_RESIDUAL)
     return fail(c, RGBDFLOW_ERR_ARG, "bad inner_exit");
-  if (p->pixel_update != RGBDFLOW_PIXEL_JACOBI && p->pixel_update != RGBDFLOW_PIXEL_GS)
+  if (p->pixel_update < RGBDFLOW_PIXEL_JACOBI || p->pixel_update > RGBDFLOW_PIXEL_BLOCK)
     return fail(c, RGBDFLOW_ERR_ARG, "bad pixel_update");
   if (p->image_storage < RGBDFLOW_IMAGES_F64 || p->image_storage > RGBDFLOW_IMAGES_BF16)
     return fail(c, RGBDFLOW_ERR_ARG, "bad image_storage");
@@ -1335,7 +1336,7 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
   TRY(validate_iters(c, nOuter, nInner, nSOR));
   if (opts && (opts->inner_exit < RGBDFLOW_EXIT_NONE || opts->inner_exit > RGBDFLOW_EXIT_RESIDUAL))
     return fail(c, RGBDFLOW_ERR_ARG, "bad inner_exit");
-  if (opts && opts->pixel_update != RGBDFLOW_PIXEL_JACOBI && opts->pixel_update != RGBDFLOW_PIXEL_GS)
+  if (opts && (opts->pixel_update < RGBDFLOW_PIXEL_JACOBI || opts->pixel_update > RGBDFLOW_PIXEL_BLOCK))
     return fail(c, RGBDFLOW_ERR_ARG, "bad pixel_update");
   if (opts && opts->image_storage != RGBDFLOW_IMAGES_F64)
     return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "image_storage: the single-level entry points take f64 images");

@@ -106,7 +106,8 @@ struct LevelView {
   float omega_c;        // 1 - omega, rounded to float as the update computes it
   int img;              // feature-image storage of I1, I2, Iw (ImgStorage; uniform over a launch)
   int pixel_gs;         // 0: a pixel's unknowns update from each other's old values (.cu:431-459);
-                        // 1: u, then v from the new u, then w from both (point SOR order)
+                        // 1: u, then v from the new u, then w from both (point SOR order);
+                        // 2: the pixel's 3x3 block solved as a whole (block SOR)
   int W_full;           // camera xRes (full size)
   float f_full;         // full-size focal length
   float f_level;        // focal length of the level
@@ -150,14 +151,15 @@ __device__ __forceinline__ void gstore(uint4* p, const uint4& v) {
 inline void set_omega(LevelView& v, float omega, int pixel_gs = 0) {
   v.omega = omega > 0 ? omega : .7f;
   v.omega_c = 1.f - v.omega;
-  v.pixel_gs = pixel_gs ? 1 : 0;
+  v.pixel_gs = pixel_gs;
 }
 
 // Feature-image storage (records.hpp: img_ld / img_st).
 enum ImgStorage { kImgF64 = 0, kImgF16 = 1, kImgBF16 = 2 };
 
 // Tiled relaxation kernel variants by update rule.
-enum RelaxMode { kRelaxFree = 0, kRelaxFixed = 1, kRelaxGS = 2 };
+enum RelaxMode { kRelaxFree = 0, kRelaxFixed = 1, kRelaxGS = 2, kRelaxBlock = 3 };
+constexpr int kRelaxModes = 4;
 
 // A launch's work-list: items plus the first block of each item (n + 1 entries).
 template <class T>
@@ -302,7 +304,8 @@ inline int tiled_slots(const TileSched& t, int W, int H, int T) {
   return (PR + 2) * (RH + 2);
 }
 // mode (RelaxMode): kRelaxFixed when every item relaxes as the reference (factor 0.7, old
-// values within the pixel), kRelaxGS when every item uses the point SOR order, else kRelaxFree.
+// values within the pixel), kRelaxGS / kRelaxBlock when every item uses the point SOR order /
+// the block update, else kRelaxFree.
 // grid: tiled_prepare's for the same (config, storage, mode, slots, blocks),
 // obtained when the launch was planned (outside any stream capture)
 int tiled_prepare(int config, int storage, int mode, int slots, int blocks);

@@ -35,7 +35,7 @@ namespace {
 // edge weights.  The left/up edge weights are the neighbours' right/down
 // ones, read from LDS.
 struct PixRegs {
-  float Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw;
+  float Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw;  // with kRelaxBlock: the block's inverse, then the rhs
   float yu, yv, yw;          // 1/Buu, 1/Bvv, 1/Bww
   float rxy, rz, dxy, dz;    // own right/down edge weights
 };
@@ -48,7 +48,7 @@ struct PixRegs {
 // dmax: pixels farther than dmax (Chebyshev distance, in pixels) from the
 // tile's interior are left as they are -- no value the interior ends with
 // depends on them any more.
-template <int P, int C, bool GS>
+template <int P, int C, int MODE>
 __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int PRp, float omega, float omega_c,
                                            const PixRegs (&pr)[P][2],
                                            const int (&qi)[P], const int (&par)[P], const unsigned (&fl)[P][2],
@@ -90,8 +90,14 @@ __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int P
     c.rw = r.rw;
     const float y3[3] = {r.yu, r.yv, r.yw};
     float xn[3];
-    if (GS) relax_update_dense_gs(c, omega, omega_c, y3, f & 64u, L, U, R, D, o, xn);
-    else relax_update_dense(c, omega, omega_c, y3, f & 64u, L, U, R, D, o, xn);
+    if (MODE == kRelaxBlock) {
+      const float inv[6] = {r.Buu, r.Bvv, r.Bww, r.Buv, r.Buw, r.Bvw};
+      relax_update_dense_block(c, inv, omega, omega_c, L, U, R, D, o, xn);
+    } else if (MODE == kRelaxGS) {
+      relax_update_dense_gs(c, omega, omega_c, y3, f & 64u, L, U, R, D, o, xn);
+    } else {
+      relax_update_dense(c, omega, omega_c, y3, f & 64u, L, U, R, D, o, xn);
+    }
     Ac[idx] = make_float4(xn[0], xn[1], xn[2], r.rxy);
     if (last && (f & 32u))
       change = fmaxf(change, fmaxf(fmaxf(fabsf(xn[0] - o.x), fabsf(xn[1] - o.y)), fabsf(xn[2] - o.z)));
@@ -191,6 +197,17 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
         r.yv = yv[1];
         r.yw = yv[2];
         if (recip_den_ok(r.Buu) && recip_den_ok(r.Bvv) && recip_den_ok(r.Bww)) f |= 64u;
+        if (MODE == kRelaxBlock) {
+          // the block's inverse once per launch, in place of the block
+          float inv[6];
+          block_inverse(r.Buu, r.Bvv, r.Bww, r.Buv, r.Buw, r.Bvw, inv);
+          r.Buu = inv[0];
+          r.Bvv = inv[1];
+          r.Bww = inv[2];
+          r.Buv = inv[3];
+          r.Buw = inv[4];
+          r.Bvw = inv[5];
+        }
         const float4 x = gload(src + l);
         XA[c * Si + qi[s]] = make_float4(x.x, x.y, x.z, r.rxy);
         XB[c * Si + qi[s]] = make_float4(r.rz, r.dxy, r.dz, 0.f);
@@ -206,7 +223,7 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
   float change = 0.f;
   // kRelaxFixed: the reference's factor 0.7 as constants (a factor held in
   // registers costs the widest configurations spills in the iteration loop)
-  constexpr bool FIXED = MODE == kRelaxFixed, GS = MODE == kRelaxGS;
+  constexpr bool FIXED = MODE == kRelaxFixed;
   const float om = FIXED ? .7f : v.omega, om_c = FIXED ? 1.f - .7f : v.omega_c;
   // After iteration it only pixels within 2 * (n_iter - 1 - it) of the
   // interior still matter: the second colour's pass there reads the first
@@ -218,9 +235,9 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
     const bool last = it == n_iter - 1;
     const int d = 2 * (n_iter - 1 - it);
     __syncthreads();
-    tiled_pass<P, 0, GS>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, d + 1, last, change);
+    tiled_pass<P, 0, MODE>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, d + 1, last, change);
     __syncthreads();
-    tiled_pass<P, 1, GS>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, d, last, change);
+    tiled_pass<P, 1, MODE>(XA, XB, Si, PRp, om, om_c, pr, qi, par, fl, d, last, change);
   }
   __syncthreads();
 #pragma unroll
@@ -276,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
 typedef void (*TiledFn)(Views, int, int, int, int, int, int);
 struct TiledConfig {
   int nt, p;
-  TiledFn fn[3][3];  // per system storage (records.hpp) and RelaxMode
+  TiledFn fn[3][kRelaxModes];  // per system storage (records.hpp) and RelaxMode
 };
 
 // (threads, pairs per thread, workgroups per CU the registers must allow);
@@ -284,7 +301,7 @@ struct TiledConfig {
 // compiled on its own
 #define RGBDFLOW_TILED_ST(NT, P, MB, ST) \
   {k_relax_tiled<NT, P, MB, ST, kRelaxFree>, k_relax_tiled<NT, P, MB, ST, kRelaxFixed>, \
-   k_relax_tiled<NT, P, MB, ST, kRelaxGS>}
+   k_relax_tiled<NT, P, MB, ST, kRelaxGS>, k_relax_tiled<NT, P, MB, ST, kRelaxBlock>}
 #define RGBDFLOW_TILED(NT, P, MB)                                                                    \
   TiledConfig {                                                                                      \
     NT, P, {                                                                                         \
@@ -355,20 +372,21 @@ void launch_copy_du(hipStream_t s, const Views& v, int blocks) {
 int tiled_prepare(int config, int storage, int mode, int slots, int blocks) {
   const TiledConfig& c = kConfigs[config];
   const TiledFn fn = c.fn[storage][mode];
-  const int key = (config * 3 + storage) * 3 + mode;
+  const int key = (config * 3 + storage) * kRelaxModes + mode;
   const size_t lds = tiled_lds_bytes(slots);
   int dev = 0;
-  hipGetDevice(&dev);
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
   static std::mutex mu;
   static std::map<int, size_t> raised;                     // (device, kernel) -> LDS limit set
   static std::map<std::pair<int, long long>, int> occ;      // (device, kernel x LDS) -> workgroups per CU
   static std::map<int, int> cus;                           // device -> compute units
   std::lock_guard<std::mutex> lk(mu);
   const int fkey = dev * 1024 + key;
-  if (lds > 65536 && lds > raised[fkey]) {
-    hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  // (a refused limit is not recorded; the launch then reports the error)
+  if (lds > 65536 && lds > raised[fkey] &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds) == hipSuccess)
     raised[fkey] = lds;
-  }
   const std::pair<int, long long> okey(fkey, (long long)lds);
   auto it = occ.find(okey);
   if (it == occ.end()) {

@@ -521,7 +521,8 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
     const float xo[3] = {o.x, o.y, o.z}, xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z},
                 xd[3] = {D.x, D.y, D.z};
     float xn[3];
-    relax_update(c, v.omega, v.omega_c, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn, v.pixel_gs != 0);
+    if (v.pixel_gs == 2) relax_update_block(c, v.omega, v.omega_c, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn);
+    else relax_update(c, v.omega, v.omega_c, hl, hu, hr, hd, xl, xu, xr, xd, xo, xn, v.pixel_gs == 1);
     gstore(x + l, make_float4(xn[0], xn[1], xn[2], 0.f));
     change = fmaxf(fmaxf(fabsf(xn[0] - xo[0]), fabsf(xn[1] - xo[1])), fabsf(xn[2] - xo[2]));
   }

@@ -98,6 +98,79 @@ __device__ __forceinline__ void relax_update(const PixCoef& c, float omega, floa
   }
 }
 
+// The inverse of a pixel's symmetric 3x3 block from its adjugate, in f32 and
+// this order: the cofactors, det = Buu c_uu + Buv c_uv + Buw c_uw, 1 / det
+// (IEEE), then each cofactor times it.  inv: (uu, vv, ww, uv, uw, vw).
+__device__ __forceinline__ void block_inverse(float Buu, float Bvv, float Bww, float Buv, float Buw, float Bvw,
+                                              float (&inv)[6]) {
+  const float cuu = Bvv * Bww - Bvw * Bvw, cvv = Buu * Bww - Buw * Buw, cww = Buu * Bvv - Buv * Buv;
+  const float cuv = Buw * Bvw - Buv * Bww, cuw = Buv * Bvw - Buw * Bvv, cvw = Buv * Buw - Buu * Bvw;
+  const float det = Buu * cuu + Buv * cuv + Buw * cuw;
+  const float id = 1.0f / det;
+  inv[0] = cuu * id;
+  inv[1] = cvv * id;
+  inv[2] = cww * id;
+  inv[3] = cuv * id;
+  inv[4] = cuw * id;
+  inv[5] = cvw * id;
+}
+
+// The block update's last half: x^ = B^-1 a, relaxed and clamped.
+__device__ __forceinline__ void block_step(const float (&inv)[6], const float (&a)[3], float omega, float omega_c,
+                                           const float xo[3], float xn[3]) {
+  const float xh[3] = {inv[0] * a[0] + inv[3] * a[1] + inv[4] * a[2],
+                       inv[3] * a[0] + inv[1] * a[1] + inv[5] * a[2],
+                       inv[4] * a[0] + inv[5] * a[1] + inv[2] * a[2]};
+#pragma unroll
+  for (int k = 0; k < 3; k++) xn[k] = clamp_step(omega * xh[k] + omega_c * xo[k], xo[k]);
+}
+
+// Block SOR (not in the reference): the pixel's three unknowns solved together
+// from its 3x3 block, x^ = B^-1 (rhs - sum_n A_n x_n), then over-relaxed and
+// clamped as the reference's update.  Convergent for any factor in (0, 2) on a
+// symmetric positive definite system.
+__device__ __forceinline__ void relax_update_block(const PixCoef& c, float omega, float omega_c, bool hl, bool hu,
+                                                   bool hr, bool hd, const float xl[3], const float xu[3],
+                                                   const float xr[3], const float xd[3], const float xo[3],
+                                                   float xn[3]) {
+  float a[3];
+  const float rhs[3] = {c.ru, c.rv, c.rw};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float nv = 0.f;
+    if (hl) nv -= (k < 2 ? c.lxy : c.lz) * xl[k];
+    if (hu) nv -= (k < 2 ? c.uxy : c.uz) * xu[k];
+    if (hr) nv -= (k < 2 ? c.rxy : c.rz) * xr[k];
+    if (hd) nv -= (k < 2 ? c.dxy : c.dz) * xd[k];
+    a[k] = nv + rhs[k];
+  }
+  float inv[6];
+  block_inverse(c.Buu, c.Bvv, c.Bww, c.Buv, c.Buw, c.Bvw, inv);
+  block_step(inv, a, omega, omega_c, xo, xn);
+}
+
+// relax_update_block with every neighbour term evaluated (as
+// relax_update_dense; the same bits) and the block's inverse given.
+__device__ __forceinline__ void relax_update_dense_block(const PixCoef& c, const float (&inv)[6], float omega,
+                                                         float omega_c, const float4& L, const float4& U,
+                                                         const float4& R, const float4& D, const float4& o,
+                                                         float xn[3]) {
+  const float xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z}, xd[3] = {D.x, D.y, D.z};
+  const float xo[3] = {o.x, o.y, o.z};
+  const float rhs[3] = {c.ru, c.rv, c.rw};
+  float a[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float nv = 0.f;
+    nv -= (k < 2 ? c.lxy : c.lz) * xl[k];
+    nv -= (k < 2 ? c.uxy : c.uz) * xu[k];
+    nv -= (k < 2 ? c.rxy : c.rz) * xr[k];
+    nv -= (k < 2 ? c.dxy : c.dz) * xd[k];
+    a[k] = nv + rhs[k];
+  }
+  block_step(inv, a, omega, omega_c, xo, xn);
+}
+
 // Markstein's division (IBM J. Res. Dev. 34(1), 1990): with y = RN(1/b),
 // q0 = RN(a y), r = a - b q0 (exact by FMA), RN(q0 + r y) = RN(a/b) whenever
 // no intermediate underflows or overflows, which holds for |b| in

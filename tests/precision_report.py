@@ -11,7 +11,8 @@ the arithmetic is f32 in every variant), with the feature images (frame 1,
 frame 2, warped frame 2) in f64 as the reference or in the same 16-bit
 format, times the relaxation's update rule
 (the reference's: factor 0.7, a pixel's unknowns from each other's old
-values; point SOR: factor 1.8, u then v then w from the newest values).
+values; point SOR: factor 1.8, u then v then w from the newest values;
+block SOR: factor 1.8, the pixel's 3x3 block solved as a whole).
 
 Reported per variant: batch pairs/s; flow distance to the f32 reference-rule
 result (EPE of u, v in px; |dw| in m); flow distance to the FP64 oracle of
@@ -81,7 +82,8 @@ def main():
     print(f"precision: {len(pairs)} pairs " + ", ".join(f"{p['W']}x{p['H']}" for p in pairs), flush=True)
     base = dict(n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED, graph=True)
     rules = {"reference": dict(omega=0.7, pixel_update=R.PIXEL_JACOBI),
-             "point_sor_1.8": dict(omega=1.8, pixel_update=R.PIXEL_GS)}
+             "point_sor_1.8": dict(omega=1.8, pixel_update=R.PIXEL_GS),
+             "block_sor_1.8": dict(omega=1.8, pixel_update=R.PIXEL_BLOCK)}
     stores = {"f32": R.STORE_F32, "f16": R.STORE_F16, "bf16": R.STORE_BF16}
     # (system records, feature images) of the batch variants
     batch_variants = {"f32": (R.STORE_F32, R.IMAGES_F64), "f16": (R.STORE_F16, R.IMAGES_F64),
@@ -116,7 +118,7 @@ def main():
         for p in pairs:
             po = O.make_params(p["W"], p["H"], f=p["f"], n_outer=15, n_sor=50, min_width=p["min_width"],
                                precision=O.PREC_F64, omega=rule["omega"],
-                               pixel_gs=rule["pixel_update"] == R.PIXEL_GS)
+                               pixel_gs=rule["pixel_update"])
             u, v, w, _ = O.multiscale(p["im1"], p["dm1"], p["im2"], p["dm2"], p["sigma1"], p["hpb"], po)
             out.append((u, v, w))
         f64[rname] = out
@@ -142,9 +144,10 @@ def main():
                                  reached=bool(0 <= rs <= 1e-4), residual=rs, energy=e))
         print(f"precision: convergence on the {w}x{h} system done", flush=True)
     for c in conv:
-        best = [d["energy"] for d in conv if d["level"] == c["level"] and d["rule"] == "point_sor_1.8" and
-                d["storage"] == "f32"][0]
-        c["energy_rel_to_sor_f32"] = abs(c["energy"] - best) / abs(best)
+        for rn in ("point_sor_1.8", "reference"):
+            e0 = [d["energy"] for d in conv if d["level"] == c["level"] and d["rule"] == rn and
+                  d["storage"] == "f32"][0]
+            c["energy_rel_to_" + ("sor" if rn == "point_sor_1.8" else rn) + "_f32"] = abs(c["energy"] - e0) / abs(e0)
     ctx.close()
     report = dict(workload=f"{len(pairs)} mixed-size pairs (" + ", ".join(f"{p['W']}x{p['H']}" for p in pairs) +
                   "), 5-level pyramids ratio 0.5, 15 outer x 50 inner, HPB regularisation",

@@ -80,14 +80,13 @@ def test_record_storage_matches_oracle(ctx, R, storage):
     W, H = 160, 120
     pair, im1, d1, im2, d2, sigma, hpb = _case(W, H, seed=63)
     st = R.STORE_F16 if storage == "f16" else R.STORE_BF16
-    for gs in (False, True):
-        kw = dict(n_outer=4, n_sor=40, min_width=20, omega=1.8 if gs else 0.7)
+    for mode in (R.PIXEL_JACOBI, R.PIXEL_GS, R.PIXEL_BLOCK):
+        kw = dict(n_outer=4, n_sor=40, min_width=20, omega=0.7 if mode == R.PIXEL_JACOBI else 1.8)
         ua, va, wa, sa = O.multiscale(im1, d1, im2, d2, sigma, hpb,
-                                      O.make_params(W, H, record_storage=st, pixel_gs=gs, **kw))
+                                      O.make_params(W, H, record_storage=st, pixel_gs=mode, **kw))
         for variant in (R.RELAX_PLAIN, R.RELAX_TILED):
-            print(f"{storage} records, point SOR {gs}, relax variant {variant}", flush=True)
-            pg = R.make_params(W, H, relax_variant=variant, storage=st,
-                               pixel_update=R.PIXEL_GS if gs else R.PIXEL_JACOBI, **kw)
+            print(f"{storage} records, pixel update {mode}, relax variant {variant}", flush=True)
+            pg = R.make_params(W, H, relax_variant=variant, storage=st, pixel_update=mode, **kw)
             ub, vb, wb, sb = ctx.multiscale(pg, im1, d1, im2, d2, sigma, hpb)
             L = sa.n_levels
             assert [sb.sweeps_exec[k] for k in range(L)] == [sa.sweeps_exec[k] for k in range(L)]
@@ -111,51 +110,55 @@ SCHEDULES = {
 
 
 @pytest.mark.parametrize("sched", list(SCHEDULES))
-@pytest.mark.parametrize("gs,omega", [(False, 0.7), (True, 1.8), (True, 1.0), (False, 1.3)])
-def test_relax_residual_matches_oracle(ctx, R, system, sched, gs, omega):
+@pytest.mark.parametrize("mode,omega", [(0, 0.7), (1, 1.8), (1, 1.0), (0, 1.3), (2, 1.8), (2, 1.0)])
+def test_relax_residual_matches_oracle(ctx, R, system, sched, mode, omega):
     """The FP64 residual exit on the device: the same iterations, the same
     increments bit for bit, and residual / energy equal to the oracle's f64
-    reduction up to summation order."""
+    reduction up to summation order.  mode: the pixel update (0 the
+    reference's, 1 point SOR, 2 the 3x3 block solve)."""
     O = oracle()
     x0 = np.zeros((3, 72, 96))
-    xa, ia, ra, ea = O.relax_residual(system, 600, 1e-4, x0, omega=omega, pixel_gs=gs)
-    p = R.make_params(96, 72, omega=omega, pixel_update=R.PIXEL_GS if gs else R.PIXEL_JACOBI, **SCHEDULES[sched])
+    xa, ia, ra, ea = O.relax_residual(system, 600, 1e-4, x0, omega=omega, pixel_gs=mode)
+    p = R.make_params(96, 72, omega=omega, pixel_update=mode, **SCHEDULES[sched])
     xb, ib, rb, eb = ctx.relax_residual(system, 600, 1e-4, x0, p)
-    print(f"{sched} gs={gs} omega={omega}: {ib} iterations, residual {rb:.3e}, energy {eb:.9g}")
+    print(f"{sched} mode={mode} omega={omega}: {ib} iterations, residual {rb:.3e}, energy {eb:.9g}")
     assert ia == ib
     assert np.array_equal(xa, xb), float(np.abs(xa - xb).max())
     assert abs(ra - rb) <= 1e-10 * ra and abs(ea - eb) <= 1e-10 * abs(ea)
 
 
-def test_sor_converged_energy_agrees(ctx, R, system):
-    """Red-black point SOR with omega = 1.8 reaches the reference iteration's
-    converged energy (omega = 0.7, old values within the pixel) within 1e-4
-    relative, in far fewer iterations (device residuals and energies).  The
-    f32 increments floor the relative residual near 5e-6 here, so the target
-    is 2e-5."""
+@pytest.mark.parametrize("mode", [1, 2])
+def test_sor_converged_energy_agrees(ctx, R, system, mode):
+    """Red-black SOR with omega = 1.8 -- point (mode 1) or 3x3 block (mode 2)
+    -- reaches the reference iteration's converged energy (omega = 0.7, old
+    values within the pixel) within 1e-4 relative, in far fewer iterations
+    (device residuals and energies).  The f32 increments floor the relative
+    residual near 5e-6 here, so the target is 2e-5."""
     x0 = np.zeros((3, 72, 96))
     tiled = dict(relax_variant=R.RELAX_TILED)
     _, i_ref, r_ref, e_ref = ctx.relax_residual(system, 40000, 2e-5, x0, R.make_params(96, 72, **tiled))
     _, i_sor, r_sor, e_sor = ctx.relax_residual(system, 40000, 2e-5, x0,
-                                                R.make_params(96, 72, omega=1.8, pixel_update=R.PIXEL_GS, **tiled))
-    print(f"reference order omega 0.7: {i_ref} its, energy {e_ref:.12g}; point SOR 1.8: {i_sor} its, {e_sor:.12g}")
+                                                R.make_params(96, 72, omega=1.8, pixel_update=mode, **tiled))
+    print(f"reference order omega 0.7: {i_ref} its, energy {e_ref:.12g}; mode {mode} 1.8: {i_sor} its, {e_sor:.12g}")
     assert r_ref <= 2e-5 and r_sor <= 2e-5
     assert abs(e_sor - e_ref) <= 1e-4 * abs(e_ref)
     assert i_sor * 3 < i_ref
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("variant", ["plain", "tiled"])
-def test_point_sor_pair_matches_oracle(ctx, R, variant):
-    """Whole pairs with the point SOR order (omega 1.8): bit-exact with the
-    oracle, and with the FP64 residual exit the same iterations per level."""
+def test_point_sor_pair_matches_oracle(ctx, R, variant, mode):
+    """Whole pairs with the point SOR order or the 3x3 block solve (omega
+    1.8): bit-exact with the oracle, and with the FP64 residual exit the same
+    iterations per level."""
     O = oracle()
     W, H = 160, 120
     pair, im1, d1, im2, d2, sigma, hpb = _case(W, H)
     kw = dict(n_outer=4, n_sor=60, min_width=20, omega=1.8)
     rv = R.RELAX_TILED if variant == "tiled" else R.RELAX_PLAIN
     for resid in (False, True):
-        po = O.make_params(W, H, pixel_gs=True, residual_exit=resid, residual_tol=1e-3, **kw)
-        pg = R.make_params(W, H, pixel_update=R.PIXEL_GS, relax_variant=rv, residual_tol=1e-3,
+        po = O.make_params(W, H, pixel_gs=mode, residual_exit=resid, residual_tol=1e-3, **kw)
+        pg = R.make_params(W, H, pixel_update=mode, relax_variant=rv, residual_tol=1e-3,
                            inner_exit=R.EXIT_RESIDUAL if resid else R.EXIT_REFERENCE, **kw)
         ua, va, wa, sa = O.multiscale(im1, d1, im2, d2, sigma, hpb, po)
         ub, vb, wb, sb = ctx.multiscale(pg, im1, d1, im2, d2, sigma, hpb)

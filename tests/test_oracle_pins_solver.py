@@ -295,6 +295,89 @@ def test_pixel_update_uses_the_old_values_and_the_factor():
     assert not np.allclose(got, sor, rtol=1e-6)              # the two orders really differ here
 
 
+def test_block_update_solves_each_pixel_s_3x3_system():
+    """The block update (pixel mode 2, not in the reference): with the
+    couplings removed one iteration is x + w (B^-1 r - x) on every pixel, B^-1 r
+    by numpy's LU solve -- in f64 to rounding, and in the f32 working
+    precision (the inverse from the adjugate) to f32 accuracy."""
+    W, H = 8, 6
+    coef = _level_system_f64(43, W, H, scale=0.1)
+    coef[0:4] = 0.0
+    rng = np.random.default_rng(43)
+    x0 = rng.normal(scale=0.005, size=(3, H, W))
+    B, r = _blocks(coef)
+    X = x0.reshape(3, -1).T
+    xh = np.linalg.solve(B, r[..., None])[..., 0]
+    for om in (1.0, float(np.float32(0.7)), float(np.float32(1.8))):
+        want = om * xh + (1 - om) * X
+        assert np.abs(want - X).max() < 0.3
+        got, it, _, _ = O.relax_residual(coef, 1, 0.0, x0, omega=om, precision=O.PREC_F64, pixel_gs=2)
+        assert it == 1 and np.allclose(got.reshape(3, -1).T, want, rtol=1e-10, atol=1e-15)
+        g32, it, _, _ = O.relax_residual(coef, 1, 0.0, x0, omega=om, precision=O.PREC_REF, pixel_gs=2)
+        assert it == 1 and np.abs(g32.reshape(3, -1).T - want).max() < 2e-5 * np.abs(want).max()
+    # the point-SOR and reference orders stop short of the block's solution in one step
+    gs, _, _, _ = O.relax_residual(coef, 1, 0.0, x0, omega=1.0, precision=O.PREC_F64, pixel_gs=1)
+    assert not np.allclose(gs.reshape(3, -1).T, xh, rtol=1e-6)
+
+
+def test_block_update_reads_the_neighbours_red_then_black():
+    """One block iteration with the couplings: red pixels solve
+    B_p x_p = r_p - (E x0)_p (E: the couplings, assembled by scipy), black
+    pixels the same from the new red values; each then relaxed by w."""
+    W, H = 10, 8
+    coef = _level_system_f64(44, W, H)
+    M, b = _sparse_system(coef)
+    N = W * H
+    B, r = _blocks(coef)
+    # E = M minus its pixel blocks
+    Bd = sp.block_diag(list(B)).tocsr()
+    perm = np.arange(3 * N).reshape(3, N).T.reshape(-1)        # (k, p) -> (p, k) order
+    P = sp.csr_matrix((np.ones(3 * N), (np.arange(3 * N), perm)), shape=(3 * N, 3 * N))
+    E = M - P.T @ Bd @ P
+    rng = np.random.default_rng(44)
+    x0 = rng.normal(scale=0.002, size=(3, H, W))
+    red = (np.add.outer(np.arange(H), np.arange(W)) % 2 == 0).ravel()
+    for om in (1.0, float(np.float32(1.8))):
+        x = x0.reshape(-1).copy()
+        for mask in (red, ~red):
+            a = (b - E @ x).reshape(3, N).T
+            xh = np.linalg.solve(B, a[..., None])[..., 0]
+            xo = x.reshape(3, N).T
+            xn = np.where(mask[:, None], om * xh + (1 - om) * xo, xo)
+            assert np.abs(xn - xo).max() < 0.3
+            x = xn.T.reshape(-1)
+        got, it, _, _ = O.relax_residual(coef, 1, 0.0, x0, omega=om, precision=O.PREC_F64, pixel_gs=2)
+        assert it == 1 and np.allclose(got.reshape(-1), x, rtol=1e-10, atol=1e-15)
+
+
+def test_block_sor_converges_to_the_direct_solve():
+    """Block SOR at w = 1.8 reaches M x = b (scipy's direct solve) in f64, and
+    in f32 its converged energy x.b - x.Mx/2 is the direct solution's b.x*/2
+    within 1e-4 -- the energy the reference's own update reaches too."""
+    import scipy.sparse.linalg as spla
+    from helpers import level_inputs
+    W, H = 14, 12
+    li, (u, v, w), _ = level_inputs(seed=5, W=W, H=H)
+    coef = O.level_system(li, O.make_params(W, H, precision=O.PREC_F64), u, v, w)
+    coef[10:13] *= 1e-2
+    M, b = _sparse_system(coef)
+    xs = spla.spsolve(M.tocsc(), b)
+    e_star = 0.5 * b @ xs
+    x, it, res, en = O.relax_residual(coef, 20000, 1e-10, np.zeros((3, H, W)), omega=float(np.float32(1.8)),
+                                      precision=O.PREC_F64, pixel_gs=2)
+    assert 0 < it < 20000 and res < 1e-10
+    assert np.max(np.abs(x.reshape(-1) - xs)) < 1e-7 * np.abs(xs).max()
+    assert abs(en - e_star) < 1e-9 * abs(e_star)
+    x32, it32, res32, en32 = O.relax_residual(coef, 20000, 1e-4, np.zeros((3, H, W)), omega=float(np.float32(1.8)),
+                                              precision=O.PREC_REF, pixel_gs=2)
+    assert 0 < it32 < 20000 and res32 < 1e-4
+    assert abs(en32 - e_star) < 1e-4 * abs(e_star)
+    xr, itr, _, enr = O.relax_residual(coef, 20000, 1e-4, np.zeros((3, H, W)), precision=O.PREC_REF)
+    assert abs(enr - e_star) < 1e-4 * abs(e_star)
+    # block SOR at 1.8 needs fewer iterations than the reference's damped update
+    assert it32 < itr
+
+
 # ------------------------------------------------------------------ with_init
 def _shifted_pair(W, H, dx, dy, seed):
     """Frame 2 is frame 1 moved by the integer (dx, dy): I2(x + dx, y + dy) =
