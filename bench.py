@@ -2,14 +2,19 @@
 
 Workload: 640x480 synthetic RGB-D frame pairs (f = 525, c = (319.5, 239.5),
 6 moving planar segments, seeds 21, 22, ...), 5-level pyramid at ratio 0.5,
-15 outer x 50 inner iterations, boundary-probability regularisation (maps from
-the generator's segment labels), the reference's mixed precision (f64 images,
+15 outer x 50 inner iterations, boundary-probability regularisation with each
+level's maps made from the frame on the device inside the step, as
+sceneFlowMultiscale.cpp:116 does (--label-maps: maps from the generator's
+segment labels instead), the reference's mixed precision (f64 images,
 derivatives and flow; f32 system and increments).  A step is one batch of
---batch pairs solved together (depth stdevs, the six pyramids, per-level
-features / upsample / warp / weights and the level solves of every pair),
-with the inputs resident in HBM.  Beside it: the plain single-pair GPU path
-(one pair, one launch per colour per relaxation iteration, as the reference
-schedules it) and the tiled single-pair path.
+--batch pairs solved together (depth stdevs, the six pyramids, boundary maps,
+per-level features / upsample / warp / weights and the level solves of every
+pair), with the inputs resident in HBM.  Beside it, as sub-objects of the same
+line: the plain single-pair GPU path (one pair, one launch per colour per
+relaxation iteration, as the reference schedules it), the tiled single pair,
+the batch with the other boundary maps, the FP16-storage batch, a batch of 64
+mixed-size pairs (320x240 to 1280x720) and 16 served streams of those sizes
+at 30 fps with warm starts of 4 and of 15 outer iterations.
 
 Prints one JSON line.  --gpus above 1 is refused: the solve runs on one GPU.
 """
@@ -34,6 +39,12 @@ B_SWEEP_16 = 48.0  # the same with the 12 system values stored in 16 bits
 B_STAGE = 128.0    # algorithmic bytes per pixel per outer iteration (warp, gradients, weights, assembly)
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "relax_tiled_traffic.json")
+T_START = time.perf_counter()
+
+
+def log(msg):
+    """A progress line on stderr, with the seconds since the start."""
+    print(f"bench [{time.perf_counter() - T_START:6.1f} s]: {msg}", file=sys.stderr, flush=True)
 
 
 def _package():
@@ -178,8 +189,6 @@ def main():
                     help="serving: solve each stream's later pairs as sceneFlowWithInit from its last flow")
     ap.add_argument("--warm-outer", type=int, default=0, help="serving: outer iterations of warm starts (0: 15)")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage run of the batch")
-    ap.add_argument("--no-device-maps", action="store_true",
-                    help="skip the run of the batch with its boundary maps made on the device")
     ap.add_argument("--images", choices=["f64", "f16", "bf16"], default="f64",
                     help="serving: storage of the feature images (f64: the reference's precision)")
     ap.add_argument("--fill-holes", action="store_true",
@@ -192,12 +201,20 @@ def main():
                          "one check interval (25 iterations) per timing, --steps timings")
     ap.add_argument("--hpb", default="",
                     help="instead: time computeHeuristicPb on the device for frames of these sizes ('WxH,...')")
-    ap.add_argument("--served-streams", type=int, default=8,
-                    help="640x480 streams of the served measurement beside the batch (0: skip)")
+    ap.add_argument("--served-streams", type=int, default=16,
+                    help="streams of the served measurement beside the batch, a quarter of each of 320x240, "
+                         "640x480, 960x540 and 1280x720 (0: skip)")
     ap.add_argument("--served-seconds", type=float, default=4.0)
-    ap.add_argument("--served-max-batch", type=int, default=4)
-    ap.add_argument("--served-warm-outer", type=int, default=5,
-                    help="outer iterations of the served measurement's warm starts")
+    ap.add_argument("--served-max-batch", type=int, default=8)
+    ap.add_argument("--served-warm-outer", type=int, default=4,
+                    help="outer iterations of the served measurement's warm starts (also run at 15)")
+    ap.add_argument("--served-compare", type=int, default=4,
+                    help="served warm pairs per size solved again at the configured iterations")
+    ap.add_argument("--mixed-pairs", type=int, default=64, help="pairs of the mixed-size batch beside it (0: skip)")
+    ap.add_argument("--mixed-steps", type=int, default=3)
+    ap.add_argument("--label-maps", action="store_true",
+                    help="boundary maps from the generator's segment labels instead of made on the device")
+    ap.add_argument("--no-other-maps", action="store_true", help="skip the batch with the other boundary maps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: the scene-flow solve runs on one GPU "
@@ -215,45 +232,47 @@ def main():
         return relax_main(R, args)
     pairs, levels = workload(R, max(1, min(args.distinct, args.batch)))
     B = args.batch
-    batch = [pairs[i % len(pairs)] for i in range(B)]
+    label_batch = [pairs[i % len(pairs)] for i in range(B)]
+    # sceneFlowMultiscale.cpp:116 makes every level's boundary maps from the frame: the step does
+    # the same on the device; --label-maps takes the generator's segment-label maps instead
+    dev_batch = [dict(pr, hpb=None) for pr in label_batch]
+    batch = label_batch if args.label_maps else dev_batch
+    maps = "generator labels" if args.label_maps else "device (computeHeuristicPb per level, in the step)"
     p = params_for(R, args.variant, not args.no_graph, args.storage)
     ctx = R.Context(0)
+    raw = {}
 
-    def autotune(nb):
+    def autotune(nb, params, record=True):
         out = []
         for (w, h) in levels:
-            ch = ctx.autotune_relax(w, h, batch=nb, params=p)
-            raw.setdefault(nb, []).append(dict(W=w, H=h, **ch))
+            ch = ctx.autotune_relax(w, h, batch=nb, params=params)
+            if record:
+                raw.setdefault(nb, []).append(dict(W=w, H=h, **ch))
             out.append(dict(level=f"{w}x{h}", tile=f"{ch['tile_w']}x{ch['tile_h']}", iters=ch["tile_iters"],
                             threads=ch["threads"], pairs_per_thread=ch["pairs_per_thread"],
                             us_per_iter=round(ch["us_per_iter"], 2)))
         return out
 
-    tuned, raw = [], {}
+    def install(nb):
+        """The schedules of batch size nb again (a later search installs its own)."""
+        sched = raw.get(nb)
+        if args.load_schedules:
+            with open(args.load_schedules) as f:
+                sched = json.load(f).get(str(nb))
+        for ch in sched or []:
+            ctx.set_relax_schedule(ch["W"], ch["H"], ch)
+
+    tuned = []
     if args.variant == "tiled" and args.load_schedules:
-        with open(args.load_schedules) as f:
-            for ch in json.load(f)[str(B)]:
-                ctx.set_relax_schedule(ch["W"], ch["H"], ch)
-                tuned.append(dict(level=f"{ch['W']}x{ch['H']}", tile=f"{ch['tile_w']}x{ch['tile_h']}",
-                                  iters=ch["tile_iters"], threads=ch["threads"], loaded=True))
+        install(B)
+        tuned = [dict(level=f"{ch['W']}x{ch['H']}", loaded=True) for ch in json.load(open(args.load_schedules))[str(B)]]
     elif args.variant == "tiled" and not args.no_autotune:
         # one-time schedule search per level shape (outside the timed region)
-        tuned = autotune(B)
-        print(f"bench: autotuned for batch {B}: {tuned}", file=sys.stderr, flush=True)
-    ctx.batch_load(p, batch)   # stdevs computed on the device every step
-    print(f"bench: {B} x 640x480 pairs, {len(levels)} levels, 15x50, variant {args.variant}", file=sys.stderr,
-          flush=True)
-    st = ctx.batch_run(p, stats=True)
-    for _ in range(args.warmup):
-        ctx.batch_run(p, stats=False)
-
-    # timed region: K batches of resident pairs
-    elapsed = timed_runs(ctx, p, args.steps)
-    pairs_per_s = B * args.steps / elapsed
-    print(f"bench: timed {args.steps} batches in {elapsed:.3f} s", file=sys.stderr, flush=True)
-
-    # batch latency (HIP events around each batch)
-    lat = np.array([ctx.batch_run(p, stats=True)[0].device_ms for _ in range(args.latency_steps)])
+        tuned = autotune(B, p)
+        log(f"autotuned for batch {B}: {tuned}")
+    log(f"{B} x 640x480 pairs, {len(levels)} levels, 15x50, variant {args.variant}, maps {maps}")
+    pairs_per_s, elapsed, lat, st = run_batch(ctx, p, batch, args.steps, args.warmup, args.latency_steps)
+    log(f"timed {args.steps} batches in {elapsed:.3f} s")
 
     # dominant kernel: the relaxation, timed launch by launch with HIP events
     ctx.set_relax_timing(True)
@@ -265,7 +284,7 @@ def main():
     achieved = sweep_bytes / (relax_ms / 1e3) / 1e9 if relax_ms > 0 else 0.0
     avg_launch_us = relax_ms * 1e3 / max(launches, 1)
 
-    # the plain single-pair GPU path and the tiled single-pair path, for comparison
+    # the plain single-pair GPU path (the reference's schedule) and the tiled single pair, same maps
     plain_pps = single_pps = None
     single_lat = None
     if args.plain_steps > 0:
@@ -273,17 +292,21 @@ def main():
         ctx.batch_load(pp, batch[:1])
         ctx.batch_run(pp, stats=True)
         plain_pps = args.plain_steps / timed_runs(ctx, pp, args.plain_steps)
+        log(f"plain single pair: {plain_pps:.2f} pairs/s")
     if args.single_steps > 0 and args.variant == "tiled":
         if args.load_schedules:
-            with open(args.load_schedules) as f:
-                for ch in json.load(f).get("1", []):
-                    ctx.set_relax_schedule(ch["W"], ch["H"], ch)
+            install(1)
         elif not args.no_autotune:
-            autotune(1)
+            autotune(1, p)
         ctx.batch_load(p, batch[:1])
         ctx.batch_run(p, stats=True)
         single_pps = args.single_steps / timed_runs(ctx, p, args.single_steps)
         single_lat = np.array([ctx.batch_run(p, stats=True)[0].device_ms for _ in range(10)])
+        log(f"tiled single pair: {single_pps:.2f} pairs/s")
+    install(B)
+
+    def speedup(v):
+        return round(v / plain_pps, 2) if plain_pps else None
 
     # the same pairs as two half batches on two contexts (two HIP streams), so
     # one half's coarse, latency-bound levels overlap the other's fine levels
@@ -314,83 +337,72 @@ def main():
                      "lanes": 2, "pairs_per_lane": half}
         for c in ctxs:
             c.close()
-        print(f"bench: two lanes {two_lanes}", file=sys.stderr, flush=True)
+        log(f"two lanes {two_lanes}")
 
-    # the same f32 batch with the boundary maps made on the device (every level's
-    # computeHeuristicPb inside the timed step, as sceneFlowMultiscale.cpp:116 does)
-    dev_maps = None
-    if args.storage == "f32" and args.variant == "tiled" and not args.no_device_maps:
-        # the batch's schedules again (the single-pair runs installed their own)
-        sched_b = raw.get(B)
-        if args.load_schedules:
-            with open(args.load_schedules) as f:
-                sched_b = json.load(f)[str(B)]
-        for ch in sched_b or []:
-            ctx.set_relax_schedule(ch["W"], ch["H"], ch)
-        ctx.batch_load(p, [dict(pr, hpb=None) for pr in batch])
-        std = ctx.batch_run(p, stats=True)
-        for _ in range(args.warmup):
-            ctx.batch_run(p, stats=False)
-        eld = timed_runs(ctx, p, args.steps)
-        L = std[0].n_levels
-        dev_maps = {"pairs_per_s": round(B * args.steps / eld, 3), "ms_per_step": round(eld / args.steps * 1e3, 3),
-                    "speedup_vs_plain_single": round(B * args.steps / eld / plain_pps, 2) if plain_pps else None,
-                    "sweeps_exec_pair0": [std[0].sweeps_exec[k] for k in range(L)],
-                    "outer_exec_pair0": [std[0].outer_exec[k] for k in range(L)]}
-        print(f"bench: device boundary maps {dev_maps}", file=sys.stderr, flush=True)
+    # the same f32 batch with the other boundary maps
+    other_maps = None
+    if args.storage == "f32" and args.variant == "tiled" and not args.no_other_maps:
+        ob = dev_batch if args.label_maps else label_batch
+        opps, oel, olat, ost = run_batch(ctx, p, ob, args.steps, args.warmup, 0)
+        L = ost[0].n_levels
+        other_maps = {"boundary_maps": "device" if args.label_maps else "generator labels",
+                      "pairs_per_s": round(opps, 3), "ms_per_step": round(oel / args.steps * 1e3, 3),
+                      "speedup_vs_plain_single": speedup(opps),
+                      "sweeps_exec_pair0": [ost[0].sweeps_exec[k] for k in range(L)],
+                      "outer_exec_pair0": [ost[0].outer_exec[k] for k in range(L)]}
+        log(f"other boundary maps {other_maps}")
 
     # the same batch with the systems stored in FP16 (f32 arithmetic)
     f16 = None
     if args.storage == "f32" and not args.no_f16 and args.variant == "tiled":
         p16 = params_for(R, args.variant, not args.no_graph, "f16")
         if not args.no_autotune and not args.load_schedules:
-            for (w, h) in levels:
-                ctx.autotune_relax(w, h, batch=B, params=p16)
-        ctx.batch_load(p16, batch)
-        ctx.batch_run(p16, stats=True)
-        for _ in range(args.warmup):
-            ctx.batch_run(p16, stats=False)
-        el16 = timed_runs(ctx, p16, args.steps)
-        lat16 = np.array([ctx.batch_run(p16, stats=True)[0].device_ms for _ in range(max(5, args.latency_steps // 2))])
+            autotune(B, p16, record=False)
+        f16pps, el16, lat16, _ = run_batch(ctx, p16, batch, args.steps, args.warmup, max(5, args.latency_steps // 2))
         ctx.set_relax_timing(True)
         r16 = ctx.batch_run(p16, stats=True)
         ctx.set_relax_timing(False)
         sb16, _ = algorithmic_bytes(r16, B_SWEEP_16)
         a16 = sb16 / (r16[0].relax_ms / 1e3) / 1e9 if r16[0].relax_ms > 0 else 0.0
-        f16 = {"pairs_per_s": round(B * args.steps / el16, 3),
-               "speedup_vs_plain_single": round(B * args.steps / el16 / plain_pps, 2) if plain_pps else None,
+        f16 = {"pairs_per_s": round(f16pps, 3), "speedup_vs_plain_single": speedup(f16pps),
                "batch_p99_ms": round(float(np.percentile(lat16, 99)), 3),
                "relax_roofline": {"achieved": round(a16, 1), "frac": round(a16 / HBM_PEAK_GBS, 4),
                                   "bytes_per_px_iter": B_SWEEP_16},
-               "relax_avg_launch_us": round(r16[0].relax_ms * 1e3 / max(r16[0].relax_launches, 1), 3)}
-        print(f"bench: fp16 storage {f16}", file=sys.stderr, flush=True)
+               "relax_avg_launch_us": round(r16[0].relax_ms * 1e3 / max(r16[0].relax_launches, 1), 3),
+               "stage_ms_per_batch": round(r16[0].device_ms - r16[0].relax_ms, 3)}
+        log(f"fp16 storage {f16}")
         # and the feature images (frame 1, frame 2, warped frame 2) in FP16 as well
         p16.image_storage = R.IMAGES_F16
-        ctx.batch_load(p16, batch)
-        ctx.batch_run(p16, stats=True)
-        for _ in range(args.warmup):
-            ctx.batch_run(p16, stats=False)
-        el16i = timed_runs(ctx, p16, args.steps)
-        lat16i = np.array([ctx.batch_run(p16, stats=True)[0].device_ms
-                           for _ in range(max(5, args.latency_steps // 2))])
-        f16["with_fp16_images"] = {
-            "pairs_per_s": round(B * args.steps / el16i, 3),
-            "speedup_vs_plain_single": round(B * args.steps / el16i / plain_pps, 2) if plain_pps else None,
-            "batch_p99_ms": round(float(np.percentile(lat16i, 99)), 3)}
-        print(f"bench: fp16 storage and images {f16['with_fp16_images']}", file=sys.stderr, flush=True)
+        f16ipps, _, lat16i, _ = run_batch(ctx, p16, batch, args.steps, args.warmup, max(5, args.latency_steps // 2))
+        f16["with_fp16_images"] = {"pairs_per_s": round(f16ipps, 3), "speedup_vs_plain_single": speedup(f16ipps),
+                                   "batch_p99_ms": round(float(np.percentile(lat16i, 99)), 3)}
+        log(f"fp16 storage and images {f16['with_fp16_images']}")
+    ctx.close()
 
-    # served: 640x480 streams at 30 fps, FP16 storage, warm starts, continuous batching, boundary maps
-    # made on the device
+    # 64 mixed-size pairs in one batch, f32 and FP16 records and images
+    mixed = None
+    if args.mixed_pairs > 0 and args.variant == "tiled":
+        mixed = mixed_batch(R, args, args.mixed_pairs, label_maps=args.label_maps)
+        log(f"mixed batch {mixed}")
+
+    # 16 served streams of four sizes at 30 fps, FP16 records and images, boundary maps made on the
+    # device, warm starts of --served-warm-outer and of the configured 15 outer iterations
     served = None
     if args.served_streams > 0:
-        spec = [(W, H)] * args.served_streams
-        frames = serve_frames(R, spec, args.frames)
-        served = serve_run(R, spec, frames, args.fps, args.served_seconds, args.served_max_batch, "f16", True,
-                           args.served_warm_outer, autotune=not args.no_autotune, images="f16", device_hpb=True)
-        served.update(streams=args.served_streams, fps_per_stream=args.fps, max_batch=args.served_max_batch,
-                      storage="f16", images="f16", warm_start=True, warm_n_outer=args.served_warm_outer or 15)
-        served["speedup_vs_plain_single"] = round(served["pairs_per_s"] / plain_pps, 2) if plain_pps else None
-        print(f"bench: served {served}", file=sys.stderr, flush=True)
+        per = max(1, args.served_streams // len(MIXED_SIZES))
+        spec = [sz for sz in MIXED_SIZES for _ in range(per)]
+        frames = serve_frames(R, spec, args.frames, maps=False)
+        served, sched = {}, []
+        for wo in (args.served_warm_outer, 15):
+            r = serve_run(R, spec, frames, args.fps, args.served_seconds, args.served_max_batch, "f16", True, wo,
+                          autotune=not args.no_autotune, images="f16", device_hpb=True, schedules=sched or None,
+                          tuned_out=sched, compare=args.served_compare if wo != 15 else 0)
+            r.update(warm_n_outer=wo)
+            served[f"warm_{wo}_outer"] = r
+            log(f"served, warm {wo} outer: {r}")
+        served.update(streams=",".join(f"{w}x{h}:{per}" for (w, h) in MIXED_SIZES), fps_per_stream=args.fps,
+                      max_batch=args.served_max_batch, lanes=2, storage="f16", images="f16",
+                      boundary_maps="device")
 
     if args.save_schedules:
         with open(args.save_schedules, "w") as f:
@@ -418,7 +430,7 @@ def main():
         "data": "synthetic (seeded generator, ground-truth flow)",
         "config": {
             "workload": f"{B} x 640x480 RGB-D pairs per step ({len(pairs)} distinct), 5-level pyramid ratio 0.5, "
-                        "15 outer x 50 inner, HPB regularisation",
+                        f"15 outer x 50 inner, HPB regularisation, boundary maps: {maps}",
             "batch": B,
             "precision": "reference mixed: f64 images/derivatives/flow, f32 system/increments",
             "relax": args.variant,
@@ -429,15 +441,17 @@ def main():
             "algorithmic_GBps": round((sweep_bytes + stage_bytes) / (np.percentile(lat, 50) / 1e3) / 1e9, 1),
             "relax_launches_per_batch": launches,
             "relax_avg_launch_us": round(avg_launch_us, 3),
+            "stage_ms_per_batch": round(rst[0].device_ms - relax_ms, 3),
             "plain_single_pairs_per_s": round(plain_pps, 3) if plain_pps else None,
-            "speedup_vs_plain_single": round(pairs_per_s / plain_pps, 2) if plain_pps else None,
+            "speedup_vs_plain_single": speedup(pairs_per_s),
             "tiled_single_pairs_per_s": round(single_pps, 3) if single_pps else None,
             "tiled_single_p99_ms": round(float(np.percentile(single_lat, 99)), 3) if single_lat is not None else None,
             "autotuned": tuned,
             "two_lanes": two_lanes,
             "fp16_storage": f16,
-            "with_device_boundary_maps": dev_maps,
-            "served_640x480": served,
+            "other_boundary_maps": other_maps,
+            "mixed_64": mixed,
+            "served_16_mixed": served,
         },
         "roofline": {
             "bound": "hbm",
@@ -450,59 +464,71 @@ def main():
     }
     if not args.no_cpu_baseline:
         nt = cpu_threads()
-        v1, n1, el1 = cpu_baseline(pairs[0], args.cpu_budget / 2, 1)
-        vn, nn, eln = cpu_baseline(pairs[0], args.cpu_budget / 2, nt)
+        cpu_pair = dict(pairs[0], hpb=None if not args.label_maps else pairs[0]["hpb"])
+        v1, n1, el1 = cpu_baseline(cpu_pair, args.cpu_budget / 2, 1)
+        vn, nn, eln = cpu_baseline(cpu_pair, args.cpu_budget / 2, nt)
+        log(f"cpu baseline: {vn:.3f} pairs/s on {nt} threads, {v1:.3f} on 1")
         out["cpu_baseline"] = {"value": round(vn, 4), "unit": "pairs/s", "cores": nt, "kind": "port",
                                "sample": f"{nn} whole pairs of the same 640x480 workload through the oracle "
-                                         f"(reference precision, level-solve rows on {nt} threads, {eln:.1f} s); "
-                                         f"1 thread: {n1} pairs in {el1:.1f} s",
+                                         f"(reference precision, boundary maps {'from labels' if args.label_maps else 'made per level'}, "
+                                         f"level-solve "
+                                         f"rows on {nt} threads, {eln:.1f} s); 1 thread: {n1} pairs in {el1:.1f} s",
                                "value_1_thread": round(v1, 4)}
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 
-def mixed_main(R, args):
-    """The mixed-size batch: one batch of --mixed pairs, every level step of
-    every pair in one launch per kernel, schedules autotuned per level shape."""
-    pairs, sizes = mixed_workload(R, args.mixed, max(1, args.distinct // 2))
-    print(f"bench: mixed batch of {len(pairs)}: " +
-          ", ".join(f"{w}x{h}: {sizes.count((w, h))}" for (w, h) in MIXED_SIZES), file=sys.stderr, flush=True)
-    ctx = R.Context(0)
-    p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED,
-                      graph=not args.no_graph)
-    if not args.no_autotune:
-        for (w, h) in sorted(set(sizes), key=lambda t: -t[0]):
-            n = sizes.count((w, h))
-            for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
-                ctx.autotune_relax(lw, lh, batch=n)
-    ctx.batch_load(p, pairs)
+def run_batch(ctx, p, batch, steps, warmup, latency_steps):
+    """Loads the batch, runs it once with statistics and warmup times, then
+    times `steps` batches; then `latency_steps` batches one by one (device
+    time from HIP events).  Returns (pairs/s, seconds, latencies in ms,
+    statistics of the first run)."""
+    ctx.batch_load(p, batch)
     st = ctx.batch_run(p, stats=True)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         ctx.batch_run(p, stats=False)
-    elapsed = timed_runs(ctx, p, args.steps)
-    lat = np.array([ctx.batch_run(p, stats=True)[0].device_ms for _ in range(max(3, args.latency_steps // 4))])
-    npx = sum(w * h for (w, h) in sizes)
-    out = {
-        "metric": "mixed-size scene-flow pairs/s (one batch)",
-        "value": round(len(pairs) * args.steps / elapsed, 3),
-        "unit": "pairs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded generator)",
-        "config": {"workload": f"{len(pairs)} pairs, sizes uniform over 320x240/640x480/960x540/1280x720, "
-                               "5-level pyramids, 15 outer x 50 inner, HPB regularisation",
-                   "sizes": {f"{w}x{h}": sizes.count((w, h)) for (w, h) in MIXED_SIZES},
-                   "megapixels_per_s": round(npx * args.steps / elapsed / 1e6, 2),
-                   "batch_p50_ms": round(float(np.percentile(lat, 50)), 3),
-                   "graph": not args.no_graph},
-    }
-    ctx.close()
+    el = timed_runs(ctx, p, steps)
+    lat = np.array([ctx.batch_run(p, stats=True)[0].device_ms for _ in range(latency_steps)])
+    return len(batch) * steps / el, el, lat, st
+
+
+def mixed_batch(R, args, n, label_maps=False):
+    """n pairs of mixed sizes in one batch (every level step of every pair in
+    one launch per kernel), schedules autotuned per level shape; f32, then
+    FP16 records, then FP16 records and images (the same schedules)."""
+    pairs, sizes = mixed_workload(R, n, max(1, args.distinct // 2))
+    if not label_maps:
+        pairs = [dict(pr, hpb=None) for pr in pairs]
+    out = {"workload": f"{len(pairs)} pairs, sizes uniform over 320x240/640x480/960x540/1280x720 (intrinsics "
+                       "scaled by W/640, 2-8 segments), 5-level pyramids, 15 outer x 50 inner, boundary maps "
+                       + ("from generator labels" if label_maps else "made on the device"),
+           "sizes": {f"{w}x{h}": sizes.count((w, h)) for (w, h) in MIXED_SIZES}}
+    steps = max(1, args.mixed_steps)
+    with R.Context(0) as ctx:
+        p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED,
+                          graph=not args.no_graph)
+        if not args.no_autotune:
+            for (w, h) in sorted(set(sizes), key=lambda t: -t[0]):
+                for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
+                    ctx.autotune_relax(lw, lh, batch=sizes.count((w, h)), params=p)
+        npx = sum(w * h for (w, h) in sizes)
+        for name, st, img in (("f32", R.STORE_F32, R.IMAGES_F64), ("fp16", R.STORE_F16, R.IMAGES_F64),
+                              ("fp16_images", R.STORE_F16, R.IMAGES_F16)):
+            p.storage, p.image_storage = st, img
+            pps, el, lat, _ = run_batch(ctx, p, pairs, steps, 1, 2)
+            out[name] = {"pairs_per_s": round(pps, 3), "ms_per_step": round(el / steps * 1e3, 3),
+                         "megapixels_per_s": round(npx * steps / el / 1e6, 2),
+                         "batch_p50_ms": round(float(np.percentile(lat, 50)), 3)}
+    return out
+
+
+def mixed_main(R, args):
+    """The mixed-size batch alone (--mixed pairs, --steps timed batches)."""
+    args.mixed_steps = args.steps
+    r = mixed_batch(R, args, args.mixed, label_maps=args.label_maps)
+    out = {"metric": "mixed-size scene-flow pairs/s (one batch)", "value": r["f32"]["pairs_per_s"], "unit": "pairs/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": r["f32"]["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded generator)", "config": dict(graph=not args.no_graph, **r)}
     print(json.dumps(out), flush=True)
 
 
@@ -525,7 +551,7 @@ def relax_main(R, args):
         if args.save_schedules:
             with open(args.save_schedules, "w") as f:
                 json.dump({"relax": ch}, f)
-    print(f"bench: relaxation of {B} x {w}x{h}, schedule {ch}", file=sys.stderr, flush=True)
+    log(f"relaxation of {B} x {w}x{h}, schedule {ch}")
     us = sorted(ctx.time_relax_schedule(w, h, ch, batch=B, params=p) for _ in range(max(1, args.steps)))
     med = us[len(us) // 2]
     bpp = 72.0 if args.storage == "f32" else 48.0
@@ -568,7 +594,7 @@ def hpb_main(R, args):
             O.heuristic_pb(im1, dm)
             row["oracle_ms_1_thread"] = round((time.perf_counter() - t0) * 1e3, 1)
         rows[tok] = row
-        print(f"bench: hpb {tok}: {row}", file=sys.stderr, flush=True)
+        log(f"hpb {tok}: {row}")
     ctx.close()
     out = {"metric": "computeHeuristicPb frames/s on the device", "unit": "ms per frame", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "dtype": "f32 (f64 normals)",
@@ -577,9 +603,10 @@ def hpb_main(R, args):
     print(json.dumps(out), flush=True)
 
 
-def serve_frames(R, spec, n_frames):
+def serve_frames(R, spec, n_frames, maps=True):
     """Per stream size: a synthetic frame sequence (every segment on a smooth
-    constant-velocity SE(3) trajectory), with each frame's boundary maps."""
+    constant-velocity SE(3) trajectory), with each frame's boundary maps from
+    the segment labels (None without maps)."""
     import rgbdflow_amd.synth as S
     frames = {}
     for (w, h) in sorted(set(spec), key=lambda t: -t[0]):
@@ -587,21 +614,29 @@ def serve_frames(R, spec, n_frames):
         seq = S.make_sequence(w, h, 500 + w, n_frames, n_segments=6, cam=S.Camera(w, h, f, f, (w - 1) / 2, (h - 1) / 2))
         lv = R.pyramid_levels(w, h, 0.5, w // 16)
         frames[(w, h)] = ([(fr["img"], (fr["depth"].astype(np.float64) * .001).astype(np.float32),
-                            S.segment_boundary_maps(fr["seg"], [a for a, b in lv], [b for a, b in lv]))
+                            S.segment_boundary_maps(fr["seg"], [a for a, b in lv], [b for a, b in lv]) if maps
+                            else None)
                            for fr in seq["frames"]], seq)
     return frames
 
 
 def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_outer, autotune=True, lanes=2, seed=7,
               score_every=8, warmup=1.0, device_hpb=False, images="f64", fill_holes=False, graph=True,
-              save_schedules=None, load_schedules=None):
+              save_schedules=None, load_schedules=None, schedules=None, tuned_out=None, compare=0):
     """Streams submit frames at Poisson arrival times (rate fps each); stream k
     walks its size's sequence back and forth from its own starting frame.
     Arrivals run for warmup + seconds; requests that arrive in the first
     `warmup` seconds (while the server's buffers and plans are first made)
     are served but not counted.  Returns the summary of the counted window:
     throughput, request latencies, and the end-point error against the true
-    flow of every score_every-th served pair."""
+    flow of every score_every-th served pair.
+    schedules: relaxation schedules to install (else autotuned; the tuned
+    ones are appended to tuned_out).  compare > 0 (warm starts): up to that
+    many scored warm pairs per size are solved again after the run, as
+    sceneFlowWithInit from the estimate the server used, with the served
+    outer iterations (the served flow must come out the same) and with the
+    configured 15, and coarse to fine at 15 x 50; the served flow's distance
+    to the last two is reported."""
     import threading
     import rgbdflow_amd.synth as S
     sizes = sorted(set(spec), key=lambda t: -t[0])
@@ -612,8 +647,10 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
                    fill_holes=fill_holes)
     if load_schedules:
         with open(load_schedules) as f:
-            for ch in json.load(f)["served"]:
-                srv.set_relax_schedule(ch["W"], ch["H"], ch)
+            schedules = json.load(f)["served"]
+    if schedules:
+        for ch in schedules:
+            srv.set_relax_schedule(ch["W"], ch["H"], ch)
     elif autotune:
         tuned = []
         with R.Context(0) as tctx:
@@ -622,10 +659,13 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
                     ch = tctx.autotune_relax(lw, lh, batch=max(1, min(4, max_batch)), params=p)
                     srv.set_relax_schedule(lw, lh, ch)
                     tuned.append(dict(W=lw, H=lh, **ch))
+        if tuned_out is not None:
+            tuned_out.extend(tuned)
         if save_schedules:
             with open(save_schedules, "w") as f:
                 json.dump({"served": tuned}, f)
-    sids = [srv.open(w, h, R.make_params(w, h).cam, w // 16) for (w, h) in spec]
+    cams = [R.make_params(w, h).cam for (w, h) in spec]
+    sids = [srv.open(w, h, cams[k], w // 16) for k, (w, h) in enumerate(spec)]
     rng = np.random.default_rng(seed)
     events = []
     for k, (w, h) in enumerate(spec):
@@ -659,7 +699,9 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
     th = threading.Thread(target=submitter)
     th.start()
     # results are collected while frames still arrive, in submission order
-    lat, qms, bsz, scored = [], [], [], []
+    lat, qms, bsz, scored, again = [], [], [], [], []
+    last = {}                    # stream -> its latest result (the next pair's warm-start estimate)
+    per_size = {}
     polled = 0
     t_last = t0 + warmup + 1e-9
     while polled < len(tickets) or th.is_alive():
@@ -676,6 +718,11 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
             bsz.append(info["batch_pairs"])
             if len(lat) % score_every == 0:
                 scored.append((k, n, flow))
+                if compare and warm and n >= 2 and k in last and per_size.get(spec[k], 0) < compare:
+                    per_size[spec[k]] = per_size.get(spec[k], 0) + 1
+                    again.append((k, n, flow, last[k]))
+        if info["has_flow"]:
+            last[k] = flow
     th.join()
     wall = t_last - (t0 + warmup)
     srv.close()
@@ -683,12 +730,58 @@ def serve_run(R, spec, frames, fps, seconds, max_batch, storage, warm, warm_oute
     for (k, n, flow) in scored:   # the pair is (frame n-1, frame n) of stream k
         gt = S.sequence_flow(frames[spec[k]][1], index_of(k, n - 1), index_of(k, n))
         epe.append(float(np.median(np.hypot(flow[0] - gt[..., 0], flow[1] - gt[..., 1]))))
-    return dict(pairs_per_s=round(len(lat) / wall, 3), offered_pairs_per_s=round(len(lat) / seconds, 2),
-                warmup_s=warmup,
-                p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
-                queue_p50_ms=round(float(np.percentile(qms, 50)), 2), mean_batch_pairs=round(float(np.mean(bsz)), 2),
-                requests=len(lat), wall_s=round(wall, 2), boundary_maps="device" if device_hpb else "generator labels",
-                epe_vs_truth_median_px=round(float(np.median(epe)), 4) if epe else None, scored_pairs=len(epe))
+    out = dict(pairs_per_s=round(len(lat) / wall, 3), offered_pairs_per_s=round(len(lat) / seconds, 2),
+               warmup_s=warmup,
+               p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
+               queue_p50_ms=round(float(np.percentile(qms, 50)), 2), mean_batch_pairs=round(float(np.mean(bsz)), 2),
+               requests=len(lat), wall_s=round(wall, 2), boundary_maps="device" if device_hpb else "generator labels",
+               epe_vs_truth_median_px=round(float(np.median(epe)), 4) if epe else None, scored_pairs=len(epe))
+    if again:
+        out["warm_vs_configured"] = resolve_served(R, p, spec, frames, again, index_of, cams, warm_outer or 15,
+                                                   device_hpb)
+    return out
+
+
+def flow_distance(flows, refs):
+    """EPE (u, v; px) and |dw| (m) over every pixel of the pairs."""
+    e = np.concatenate([np.hypot(a[0] - b[0], a[1] - b[1]).ravel() for a, b in zip(flows, refs)])
+    dw = np.concatenate([np.abs(a[2] - b[2]).ravel() for a, b in zip(flows, refs)])
+    return dict(epe_median_px=float(np.median(e)), epe_mean_px=float(e.mean()), epe_p99_px=float(np.percentile(e, 99)),
+                dw_median_m=float(np.median(dw)), dw_p99_m=float(np.percentile(dw, 99)))
+
+
+def resolve_served(R, p, spec, frames, again, index_of, cams, served_outer, device_hpb):
+    """The served warm pairs solved again in one batch each way: as
+    sceneFlowWithInit from the estimate the server used, with the served
+    outer iterations (bit-equal to the served flow) and with 15
+    (sceneFlowWithInit.cpp:74-80 takes nOuterFPIterations), and coarse to
+    fine at 15 x 50 from scratch."""
+    def inputs(k, n, est):
+        w, h = spec[k]
+        fr = frames[spec[k]][0]
+        a, b = fr[index_of(k, n - 1)], fr[index_of(k, n)]
+        d = dict(im1=a[0].astype(np.float64) / 255.0, dm1=a[1].astype(np.float64), im2=b[0].astype(np.float64) / 255.0,
+                 dm2=b[1].astype(np.float64), hpb=None if device_hpb else a[2], cam=cams[k], min_width=w // 16)
+        if est is not None:
+            d["est"] = np.stack(est, axis=-1)
+        return d
+
+    def solve(n_outer, warm):
+        q = R.make_params(640, 480, n_outer=n_outer, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED,
+                          graph=False, storage=p.storage, image_storage=p.image_storage)
+        with R.Context(0) as c:
+            c.batch_load(q, [inputs(k, n, est if warm else None) for (k, n, _, est) in again])
+            c.batch_run(q, stats=False)
+            return [c.batch_read(i) for i in range(len(again))]
+
+    served = [f for (_, _, f, _) in again]
+    same = solve(served_outer, True)
+    w15 = solve(15, True)
+    cold = solve(15, False)
+    equal = all(all(np.array_equal(x, y) for x, y in zip(a, b)) for a, b in zip(served, same))
+    return {"pairs": len(again), "served_equal_resolved": bool(equal),
+            "vs_warm_15_outer": flow_distance(served, w15), "vs_cold_15x50": flow_distance(served, cold),
+            "warm_15_vs_cold_15x50": flow_distance(w15, cold)}
 
 
 def serve_main(R, args):
@@ -700,12 +793,12 @@ def serve_main(R, args):
         size, cnt = tok.split(":")
         w, h = (int(x) for x in size.split("x"))
         spec += [(w, h)] * int(cnt)
-    frames = serve_frames(R, spec, args.frames)
-    print(f"bench: serving {len(spec)} streams over {args.seconds} s", file=sys.stderr, flush=True)
+    frames = serve_frames(R, spec, args.frames, maps=not args.device_hpb)
+    log(f"serving {len(spec)} streams over {args.seconds} s")
     r = serve_run(R, spec, frames, args.fps, args.seconds, args.max_batch, args.storage, args.warm,
                   args.warm_outer, autotune=not args.no_autotune, device_hpb=args.device_hpb, images=args.images,
                   fill_holes=args.fill_holes, graph=not args.no_graph, save_schedules=args.save_schedules,
-                  load_schedules=args.load_schedules)
+                  load_schedules=args.load_schedules, compare=args.served_compare if args.warm else 0)
     out = {
         "metric": "served scene-flow pairs/s and request latency",
         "value": r["pairs_per_s"],
