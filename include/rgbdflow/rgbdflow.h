@@ -132,11 +132,16 @@ typedef struct rgbdflow_stats {
   int sweeps_exec[16];           /* red-black iterations run per level (all outers) */
   double last_stat[16];          /* last outer convergence statistic per level (-1: none) */
   double device_ms;              /* device time of the call (HIP events) */
-  double relax_ms;               /* summed device time of the relaxation launches, if timed */
+  double relax_ms;               /* device time of the relaxation launches, if timed: HIP events
+                                    around each run of back-to-back launches (one inner
+                                    iteration's; each launch where residual checks interleave) */
   int relax_launches;            /* relaxation launches timed */
   int residual_checks[16];       /* FP64 residual evaluations per level (RGBDFLOW_EXIT_RESIDUAL) */
   double residual[16];           /* last relative residual per level (-1: none evaluated) */
   double energy[16];             /* energy r'du - 1/2 du'M du of the last system at that evaluation */
+  double stat_margin[16];        /* min over the level's outer iterations of |stat - 4e-4| / 4e-4:
+                                    how near the early-exit statistic (.cu:948-951, summed in f64
+                                    here, in f32 by the reference) came to the threshold (-1: no test) */
 } rgbdflow_stats;
 
 typedef struct rgbdflow_ctx rgbdflow_ctx;
@@ -151,7 +156,8 @@ int rgbdflow_destroy(rgbdflow_ctx* ctx);
 const char* rgbdflow_last_error(const rgbdflow_ctx* ctx);
 unsigned long long rgbdflow_bytes_in_use(const rgbdflow_ctx* ctx);
 int rgbdflow_synchronize(rgbdflow_ctx* ctx);
-/* Time each relaxation launch with HIP events (stats.relax_ms / relax_launches). */
+/* Time the relaxation launches with HIP events (stats.relax_ms / relax_launches); the
+ * timed runs launch every kernel (no graph replay). */
 int rgbdflow_set_relax_timing(rgbdflow_ctx* ctx, int on);
 
 /* Level count and sizes of GaussianPyramid::ConstructPyramid (GaussianPyramid.cpp:20-63);

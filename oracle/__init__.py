@@ -330,6 +330,16 @@ def warp_gpu_style(I1, D1, I2, D2, u, v, w, precision=PREC_REF):
     return wI, wD
 
 
+def warp_gpu_style_f32(I1, D1, I2, D2, u, v, w, image_storage):
+    I1, D1, I2, D2, u, v, w = map(_f64, (I1, D1, I2, D2, u, v, w))
+    H, W = D1.shape
+    wI = np.empty_like(I1)
+    wD = np.empty_like(D1)
+    lib().orc_warp_gpu_style_f32(_p(I1), _p(D1), _p(I2), _p(D2), _p(u), _p(v), _p(w), W, H, image_storage, _p(wI),
+                                 _p(wD))
+    return wI, wD
+
+
 def level_weights(li: LevelInputs, params: Params, u, v, w):
     u, v, w = _f64(u), _f64(v), _f64(w)
     phi = np.empty((li.H, li.W)); psi = np.empty((li.H, li.W, 5)); pi = np.empty((li.H, li.W))
@@ -475,4 +485,17 @@ def check_div_const(n, seed, d, recipe=0):
     f.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_double)]
     x = C.c_double()
     bad = f(n, seed, d, recipe, C.byref(x))
+    return int(bad), x.value
+
+
+def check_div_const_f32(d, recipe=0):
+    """Every f32 x of the binade [1, 2) (both signs) through the f32 division
+    by the constant d (divcheck.cpp); returns (mismatches against IEEE x / d,
+    first x)."""
+    L = lib()
+    f = L.orc_check_div_const_f32
+    f.restype = C.c_uint64
+    f.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_float)]
+    x = C.c_float()
+    bad = f(d, recipe, C.byref(x))
     return int(bad), x.value

@@ -125,3 +125,35 @@ uint64_t orc_check_div_const(uint64_t n, uint64_t seed, int d, int recipe, doubl
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// The f32 stage of the 16-bit image variants divides f32 values by the same
+// constants: q0 = RN(x * RN(1/d)), r = fma(-d, q0, x), q = fma(r, RN(1/d), q0)
+// for |x| in [2^-100, 2^100], the IEEE division otherwise.  In that domain the
+// recipe commutes with scaling x by a power of two (q0, r and q scale with it
+// and stay normal or exact), so every binade behaves as [1, 2): this checks
+// each of its 2^23 significands with both signs, bit for bit against x / d
+// (recipe 1: the uncorrected product, a negative control).  Returns the
+// mismatches; the first is stored in *ex.
+uint64_t orc_check_div_const_f32(int d, int recipe, float* ex) {
+  const float df = (float)d;
+  const float y = 1.0f / df;
+  uint64_t bad = 0;
+  for (uint32_t m = 0; m < (1u << 23); m++)
+    for (uint32_t sgn = 0; sgn < 2; sgn++) {
+      const uint32_t bits = (sgn << 31) | (127u << 23) | m;
+      float x;
+      std::memcpy(&x, &bits, 4);
+      const float q0 = x * y;
+      const float q = recipe == 1 ? q0 : std::fmaf(std::fmaf(-df, q0, x), y, q0);
+      const float ref = x / df;
+      if (std::memcmp(&q, &ref, 4) != 0) {
+        if (bad == 0 && ex) *ex = x;
+        bad++;
+      }
+    }
+  return bad;
+}
+
+}  // extern "C"

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <functional>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -657,9 +658,15 @@ void depth_stdevs_pointwise(const float* depth, int W, int H, int max_dist, floa
  * type of the device weight copies (float in the reference); FSQ selects the
  * reference's float squares (sqrCU, .cu:44-47).
  */
-template <class CT, bool REF>
+/* ST: the arithmetic of the per-outer stage (derivatives, occlusion, penalty
+ * derivatives, Laplacians, assembly and the in-solve warp).  double: the
+ * reference's (f64 with its f32 roundings); float (with REF, the 16-bit image
+ * variants): every stage formula in f32 on f32-rounded inputs, in the same
+ * order -- not in the reference. */
+template <class CT, bool REF, class ST = double>
 struct Solver {
   typedef typename std::conditional<REF, float, double>::type WT;
+  static constexpr bool F32 = std::is_same<ST, float>::value;
 
   static inline double sq(double x) {
     if (REF) { float f = (float)x; return (double)(float)(f * f); }
@@ -702,6 +709,7 @@ struct Solver {
 
   /* .cu:84-147 imgDerivativesKernel / ...WithClamp */
   void derivatives() {
+    if (F32) { derivatives_f32(); return; }
     static const double g[5] = {1, -8, 0, 8, -1};
     const double* I1 = in->I1;
     pool().rows(H, [&](int i0, int i1, int) {
@@ -741,10 +749,56 @@ struct Solver {
     });
   }
 
+  /* derivatives in f32: stencil inputs .4f I1 + .6f Iw of the f32-rounded
+   * values, sums in the stencil order, / 12 and the depth clamps in f32 */
+  void derivatives_f32() {
+    static const float g[5] = {1, -8, 0, 8, -1};
+    const double* I1 = in->I1;
+    const double* D1 = in->D1;
+    pool().rows(H, [&](int i0, int i1, int) {
+    for (int i = i0; i < i1; i++)
+      for (int j = 0; j < W; j++) {
+        const size_t l = (size_t)i * W + j;
+        for (int k = 0; k < C; k++) {
+          float sx = 0, sy = 0;
+          for (int c = j - 2, m = 0; c <= j + 2; c++, m++) {
+            const size_t o = ((size_t)i * W + std::max(0, std::min(W - 1, c))) * C + k;
+            sx += g[m] * (.4f * (float)I1[o] + .6f * (float)Iw[o]);
+          }
+          for (int r = i - 2, m = 0; r <= i + 2; r++, m++) {
+            const size_t o = ((size_t)std::max(0, std::min(H - 1, r)) * W + j) * C + k;
+            sy += g[m] * (.4f * (float)I1[o] + .6f * (float)Iw[o]);
+          }
+          Ix[l * C + k] = sx / 12.f;
+          Iy[l * C + k] = sy / 12.f;
+          It[l * C + k] = (float)Iw[l * C + k] - (float)I1[l * C + k];
+        }
+        const float mg = .04f;
+        float sx = 0, sy = 0;
+        for (int c = j - 2, m = 0; c <= j + 2; c++, m++) {
+          const size_t o = (size_t)i * W + std::max(0, std::min(W - 1, c));
+          sx += g[m] * (.4f * (float)D1[o] + .6f * (float)Dw[o]);
+        }
+        for (int r = i - 2, m = 0; r <= i + 2; r++, m++) {
+          const size_t o = (size_t)std::max(0, std::min(H - 1, r)) * W + j;
+          sy += g[m] * (.4f * (float)D1[o] + .6f * (float)Dw[o]);
+        }
+        Dx[l] = std::min(std::max(sx / 12.f, -mg), mg);
+        Dy[l] = std::min(std::max(sy / 12.f, -mg), mg);
+        Dt[l] = std::min(std::max((float)Dw[l] - (float)D1[l], -mg), mg);
+      }
+    });
+  }
+
   /* .cu:640-648 fillOcclusionPenaltyKernel (unwarped D2) */
   void occlusion() {
     pool().rows(H, [&](int i0, int i1, int) {
       for (size_t l = (size_t)i0 * W; l < (size_t)i1 * W; l++) {
+        if (F32) {
+          const float d1 = (float)in->D1[l];
+          q[l] = (d1 + (float)u[2 * N + l] < (float)in->D2[l] + .02f * (d1 * d1)) ? (WT)1 : (WT).01;
+          continue;
+        }
         const double d1 = in->D1[l];
         q[l] = (d1 + u[2 * N + l] < in->D2[l] + .02 * sq(d1)) ? (WT)1 : (WT).01;
       }
@@ -756,6 +810,11 @@ struct Solver {
     const size_t l = (size_t)i * W + j;
     const double* uc = u.data() + c * N;
     const CT* dc = du.data() + c * N;
+    if (F32) {
+      gy = (i == H - 1) ? 0.f : ((float)uc[l + W] + (float)dc[l + W] - (float)uc[l] - (float)dc[l]);
+      gx = (j == W - 1) ? 0.f : ((float)uc[l + 1] + (float)dc[l + 1] - (float)uc[l] - (float)dc[l]);
+      return;
+    }
     gy = (i == H - 1) ? 0.0 : (uc[l + W] + (double)dc[l + W] - uc[l] - (double)dc[l]);
     gx = (j == W - 1) ? 0.0 : (uc[l + 1] + (double)dc[l + 1] - uc[l] - (double)dc[l]);
   }
@@ -772,6 +831,21 @@ struct Solver {
         flow_grad(0, i, j, ux, uy);
         flow_grad(1, i, j, vx, vy);
         flow_grad(2, i, j, wx, wy);
+        if (F32) {
+          const float fux = (float)ux, fuy = (float)uy, fvx = (float)vx, fvy = (float)vy, fwx = (float)wx,
+                      fwy = (float)wy;
+          const float a = fux * fux + fuy * fuy + fvx * fvx + fvy * fvy + (float)sW[l] * (fwx * fwx + fwy * fwy);
+          phi[l] = .5f / std::sqrt(a + 1e-6f);
+          const float d0 = (float)du[l], d1 = (float)du[N + l], d2 = (float)du[2 * N + l];
+          for (int k = 0; k < C; k++) {
+            const size_t o = l * C + k;
+            const float tt = ((float)It[o] + (float)Ix[o] * d0 + (float)Iy[o] * d1) * (float)q[l];
+            psi[o] = .5f / std::sqrt(tt * tt + 1e-6f) * (float)cw;
+          }
+          const float t2 = (float)Dt[l] + (float)Dx[l] * d0 + (float)Dy[l] * d1 - d2;
+          pi[l] = .5f / std::sqrt(t2 * t2 + 1e-10f) * (float)dW[l];
+          continue;
+        }
         double t;
         if (REF) {
           const float a = (float)sq(ux) + (float)sq(uy) + (float)sq(vx) + (float)sq(vy) +
@@ -796,6 +870,7 @@ struct Solver {
   /* .cu:240-333 uv/w flow Laplacians (on u only, normalised by the mean
    * regularisation weight of the existing edges) */
   double laplacian(int c, int i, int j) const {
+    if (F32) return laplacian_f32(c, i, j);
     const size_t l = (size_t)i * W + j;
     const double* uc = u.data() + c * N;
     double lap = 0;
@@ -814,6 +889,25 @@ struct Solver {
     return lap;
   }
 
+  float laplacian_f32(int c, int i, int j) const {
+    const size_t l = (size_t)i * W + j;
+    const double* uc = u.data() + c * N;
+    const float ul = (float)uc[l];
+    float lap = 0;
+    if (j < W - 1) lap -= ((float)uc[l + 1] - ul) * (float)phi[l] * (float)rW[l * 2 + 0];
+    if (j > 0) lap += (ul - (float)uc[l - 1]) * (float)phi[l - 1] * (float)rW[(l - 1) * 2 + 0];
+    if (i < H - 1) lap -= ((float)uc[l + W] - ul) * (float)phi[l] * (float)rW[l * 2 + 1];
+    if (i > 0) lap += (ul - (float)uc[l - W]) * (float)phi[l - W] * (float)rW[(l - W) * 2 + 1];
+    float norm = 0;
+    int count = 0;
+    if (j < W - 1) { norm += (float)rW[l * 2 + 0]; count++; }
+    if (j > 0) { norm += (float)rW[(l - 1) * 2 + 0]; count++; }
+    if (i < H - 1) { norm += (float)rW[l * 2 + 1]; count++; }
+    if (i > 0) { norm += (float)rW[(l - W) * 2 + 1]; count++; }
+    if (std::fabs(norm) < 1e-5f) return 0.f;
+    return lap / (norm / (float)count);
+  }
+
   /* .cu:189-230 products, .cu:477-575 fillSystemMtxKernel, written into the
    * compact form: own right/down edge weights (xy and z), the symmetric 3x3
    * block and the rhs. */
@@ -824,6 +918,7 @@ struct Solver {
     CT* Buu = coef.data() + 4 * N; CT* Bvv = coef.data() + 5 * N; CT* Bww = coef.data() + 6 * N;
     CT* Buv = coef.data() + 7 * N; CT* Buw = coef.data() + 8 * N; CT* Bvw = coef.data() + 9 * N;
     CT* ru = coef.data() + 10 * N; CT* rv = coef.data() + 11 * N; CT* rw = coef.data() + 12 * N;
+    if (F32) { assemble_f32(); return; }
     pool().rows(H, [&](int i0, int i1, int) {
     for (int i = i0; i < i1; i++)
       for (int j = 0; j < W; j++) {
@@ -869,6 +964,64 @@ struct Solver {
         t = (CT)(-(-ity - dty + alpha * -vL)); rv[l] = (CT)((double)t + -(-gamma * u1));
         t = (CT)(-(P + alpha * wsZ)); Bww[l] = (CT)((double)t + -(gamma * (double)mW[l]));
         t = (CT)(-(P * Dt[l] + alpha * -wL)); rw[l] = (CT)((double)t + -(-gamma * (double)mW[l] * u2));
+      }
+    });
+  }
+
+  /* assemble in f32: the same sums and products, alpha and gamma rounded to
+   * f32, the channel means as f32 divisions by 5 */
+  void assemble_f32() {
+    const float alpha = (float)p->alpha, gamma = (float)p->gamma;
+    CT* eRxy = coef.data() + 0 * N; CT* eRz = coef.data() + 1 * N;
+    CT* eDxy = coef.data() + 2 * N; CT* eDz = coef.data() + 3 * N;
+    CT* Buu = coef.data() + 4 * N; CT* Bvv = coef.data() + 5 * N; CT* Bww = coef.data() + 6 * N;
+    CT* Buv = coef.data() + 7 * N; CT* Buw = coef.data() + 8 * N; CT* Bvw = coef.data() + 9 * N;
+    CT* ru = coef.data() + 10 * N; CT* rv = coef.data() + 11 * N; CT* rw = coef.data() + 12 * N;
+    pool().rows(H, [&](int i0, int i1, int) {
+    for (int i = i0; i < i1; i++)
+      for (int j = 0; j < W; j++) {
+        const size_t l = (size_t)i * W + j;
+        float sxy = 0, sxx = 0, syy = 0, stx = 0, sty = 0;
+        for (int k = 0; k < C; k++) {
+          const size_t o = l * C + k;
+          const float ps = (float)psi[o], ix = (float)Ix[o], iy = (float)Iy[o];
+          sxy += ps * ix * iy;
+          sxx += ps * ix * ix;
+          syy += ps * iy * iy;
+        }
+        for (int k = 0; k < C; k++) { const size_t o = l * C + k; stx += (float)psi[o] * (float)Ix[o] * (float)It[o]; }
+        for (int k = 0; k < C; k++) { const size_t o = l * C + k; sty += (float)psi[o] * (float)Iy[o] * (float)It[o]; }
+        const float ql = (float)q[l];
+        const float ixy = sxy / 5.f, ix2 = sxx / 5.f, iy2 = syy / 5.f;
+        const float itx = stx * ql * ql / 5.f, ity = sty * ql * ql / 5.f;
+        const float P = (float)pi[l], dx = (float)Dx[l], dy = (float)Dy[l], dt = (float)Dt[l];
+        const float dxy = P * dx * dy, dx2 = P * dx * dx, dy2 = P * dy * dy;
+        const float dtx = P * dx * dt, dty = P * dy * dt;
+        const float uL = laplacian_f32(0, i, j), vL = laplacian_f32(1, i, j);
+        const float wL = (float)sW[l] * laplacian_f32(2, i, j);
+        float wsXY = 0, wsZ = 0;
+        if (j > 0) { const float a = (float)phi[l - 1] * (float)rW[(l - 1) * 2 + 0], b = a * (float)sW[l - 1]; wsXY += a; wsZ += b; }
+        if (i > 0) { const float a = (float)phi[l - W] * (float)rW[(l - W) * 2 + 1], b = a * (float)sW[l - W]; wsXY += a; wsZ += b; }
+        if (j < W - 1) {
+          const float a = (float)phi[l] * (float)rW[l * 2 + 0], b = a * (float)sW[l];
+          eRxy[l] = alpha * a; eRz[l] = alpha * b; wsXY += a; wsZ += b;
+        }
+        if (i < H - 1) {
+          const float a = (float)phi[l] * (float)rW[l * 2 + 1], b = a * (float)sW[l];
+          eDxy[l] = alpha * a; eDz[l] = alpha * b; wsXY += a; wsZ += b;
+        }
+        const float u0 = (float)u[l], u1 = (float)u[N + l], u2 = (float)u[2 * N + l];
+        const float m = (float)mW[l];
+        float t;
+        t = -(ix2 + dx2 + alpha * wsXY); Buu[l] = t + -gamma;
+        Buv[l] = -ixy - dxy;
+        Buw[l] = P * dx;
+        t = -(-itx - dtx + alpha * -uL); ru[l] = t + -(-gamma * u0);
+        t = -(iy2 + dy2 + alpha * wsXY); Bvv[l] = t + -gamma;
+        Bvw[l] = P * dy;
+        t = -(-ity - dty + alpha * -vL); rv[l] = t + -(-gamma * u1);
+        t = -(P + alpha * wsZ); Bww[l] = t + -(gamma * m);
+        t = -(P * dt + alpha * -wL); rw[l] = t + -(-gamma * m * u2);
       }
     });
   }
@@ -1044,13 +1197,26 @@ struct Solver {
         const double x = j + u[l], y = i + u[N + l];
         if (x < 0 || x > W - 1 || y < 0 || y > H - 1) {
           for (int k = 0; k < C; k++) Iw[l * C + k] = I1[l * C + k];
-          Dw[l] = in->D1[l] - u[2 * N + l];
+          Dw[l] = F32 ? (double)((float)in->D1[l] - (float)u[2 * N + l]) : in->D1[l] - u[2 * N + l];
         } else {
           const int x0 = std::max(0, std::min(W - 1, (int)std::floor(x)));
           const int y0 = std::max(0, std::min(H - 1, (int)std::floor(y)));
           const int x1 = std::min(W - 1, x0 + 1), y1 = std::min(H - 1, y0 + 1);
           const AT ax = x - x0, ay = y - y0;
           const AT w00 = (1 - ax) * (1 - ay), w01 = (1 - ax) * ay, w10 = ax * (1 - ay), w11 = ax * ay;
+          if (F32) {
+            for (int k = 0; k < C; k++) {
+              const float s = w00 * (float)I2[((size_t)y0 * W + x0) * C + k] +
+                              w01 * (float)I2[((size_t)y1 * W + x0) * C + k] +
+                              w10 * (float)I2[((size_t)y0 * W + x1) * C + k] +
+                              w11 * (float)I2[((size_t)y1 * W + x1) * C + k];
+              Iw[l * C + k] = img_round(s, p->image_storage);
+            }
+            const double* D2 = in->D2;
+            Dw[l] = w00 * (float)D2[(size_t)y0 * W + x0] + w01 * (float)D2[(size_t)y1 * W + x0] +
+                    w10 * (float)D2[(size_t)y0 * W + x1] + w11 * (float)D2[(size_t)y1 * W + x1] - (float)u[2 * N + l];
+            continue;
+          }
           for (int k = 0; k < C; k++)
             Iw[l * C + k] = img_round(w00 * I2[((size_t)y0 * W + x0) * C + k] + w01 * I2[((size_t)y1 * W + x0) * C + k] +
                                       w10 * I2[((size_t)y0 * W + x1) * C + k] + w11 * I2[((size_t)y1 * W + x1) * C + k],
@@ -1121,10 +1287,10 @@ struct Solver {
   }
 };
 
-template <class CT, bool REF>
+template <class CT, bool REF, class ST = double>
 void level_solve_t(const orc_level_in* in, const orc_params* p, int allow_early, double* u, double* v,
                    double* w, orc_level_stats* st) {
-  Solver<CT, REF> S(in, p);
+  Solver<CT, REF, ST> S(in, p);
   const size_t N = S.N;
   for (size_t l = 0; l < N; l++) { S.u[l] = u[l]; S.u[N + l] = v[l]; S.u[2 * N + l] = w[l]; }
   S.run(allow_early, st);
@@ -1179,7 +1345,9 @@ void reg_weights(const float* hpb, const double* depth, int W, int H, int type, 
 
 int level_solve_dispatch(const orc_level_in* in, const orc_params* p, int allow_early, double* u, double* v,
                          double* w, orc_level_stats* st) {
-  if (p->precision == ORC_PREC_REF) level_solve_t<float, true>(in, p, allow_early, u, v, w, st);
+  if (p->precision == ORC_PREC_REF && p->image_storage != ORC_IMG_F64)
+    level_solve_t<float, true, float>(in, p, allow_early, u, v, w, st);
+  else if (p->precision == ORC_PREC_REF) level_solve_t<float, true>(in, p, allow_early, u, v, w, st);
   else level_solve_t<double, false>(in, p, allow_early, u, v, w, st);
   return 0;
 }
@@ -1289,7 +1457,12 @@ int orc_level_solve(const orc_level_in* in, const orc_params* p, int allow_early
 void orc_level_system(const orc_level_in* in, const orc_params* p, const double* u, const double* v,
                       const double* w, double* coef13) {
   const size_t N = (size_t)in->W * in->H;
-  if (p->precision == ORC_PREC_REF) {
+  if (p->precision == ORC_PREC_REF && p->image_storage != ORC_IMG_F64) {
+    Solver<float, true, float> S(in, p);
+    for (size_t l = 0; l < N; l++) { S.u[l] = u[l]; S.u[N + l] = v[l]; S.u[2 * N + l] = w[l]; }
+    S.build_system();
+    for (size_t k = 0; k < 13 * N; k++) coef13[k] = S.coef[k];
+  } else if (p->precision == ORC_PREC_REF) {
     Solver<float, true> S(in, p);
     for (size_t l = 0; l < N; l++) { S.u[l] = u[l]; S.u[N + l] = v[l]; S.u[2 * N + l] = w[l]; }
     S.build_system();
@@ -1321,6 +1494,24 @@ void orc_warp_gpu_style(const double* I1, const double* D1, const double* I2, co
   else { Solver<double, false> S(&in, &p); run(S); }
 }
 
+void orc_warp_gpu_style_f32(const double* I1, const double* D1, const double* I2, const double* D2, const double* u,
+                            const double* v, const double* w, int W, int H, int image_storage, double* wI,
+                            double* wD) {
+  const size_t N = (size_t)W * H;
+  std::vector<double> z(N, 0.0), one(N * 2, 1.0);
+  orc_level_in in;
+  in.W = W; in.H = H; in.focal_level = 1;
+  in.I1 = I1; in.I2 = I2; in.warpI2 = I2; in.D1 = D1; in.D2 = D2; in.warpD2 = D2;
+  in.delta = z.data(); in.s = z.data(); in.m = z.data(); in.r = one.data();
+  orc_params p{};
+  p.image_storage = image_storage;
+  Solver<float, true, float> S(&in, &p);
+  for (size_t l = 0; l < N; l++) { S.u[l] = u[l]; S.u[N + l] = v[l]; S.u[2 * N + l] = w[l]; }
+  S.warp_gpu();
+  std::memcpy(wI, S.Iw.data(), N * 5 * sizeof(double));
+  std::memcpy(wD, S.Dw.data(), N * sizeof(double));
+}
+
 void orc_level_weights(const orc_level_in* in, const orc_params* p, const double* u, const double* v,
                        const double* w, double* phi, double* psi, double* pi) {
   const size_t N = (size_t)in->W * in->H;
@@ -1333,7 +1524,8 @@ void orc_level_weights(const orc_level_in* in, const orc_params* p, const double
     std::memcpy(psi, S.psi.data(), N * 5 * sizeof(double));
     std::memcpy(pi, S.pi.data(), N * sizeof(double));
   };
-  if (p->precision == ORC_PREC_REF) { Solver<float, true> S(in, p); run(S); }
+  if (p->precision == ORC_PREC_REF && p->image_storage != ORC_IMG_F64) { Solver<float, true, float> S(in, p); run(S); }
+  else if (p->precision == ORC_PREC_REF) { Solver<float, true> S(in, p); run(S); }
   else { Solver<double, false> S(in, p); run(S); }
 }
 

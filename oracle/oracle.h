@@ -131,6 +131,11 @@ void orc_level_system(const orc_level_in* in, const orc_params* p, const double*
 void orc_warp_gpu_style(const double* I1, const double* D1, const double* I2, const double* D2, const double* u,
                         const double* v, const double* w, int W, int H, int precision, double* warpI2,
                         double* warpD2);
+/* The same warp in the f32 stage of the 16-bit image variants (image_storage
+ * ORC_IMG_F16 or ORC_IMG_BF16: f32 sums, the warped features rounded to it). */
+void orc_warp_gpu_style_f32(const double* I1, const double* D1, const double* I2, const double* D2, const double* u,
+                            const double* v, const double* w, int W, int H, int image_storage, double* warpI2,
+                            double* warpD2);
 /* The robust weights at du = 0 (.cu:340-371): phi [N], psi [N][5], pi [N]. */
 void orc_level_weights(const orc_level_in* in, const orc_params* p, const double* u, const double* v,
                        const double* w, double* phi, double* psi, double* pi);

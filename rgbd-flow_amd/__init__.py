@@ -62,12 +62,13 @@ class Stats(C.Structure):
     _fields_ = [("n_levels", C.c_int), ("width", C.c_int * 16), ("height", C.c_int * 16),
                 ("outer_exec", C.c_int * 16), ("sweeps_exec", C.c_int * 16), ("last_stat", C.c_double * 16),
                 ("device_ms", C.c_double), ("relax_ms", C.c_double), ("relax_launches", C.c_int),
-                ("residual_checks", C.c_int * 16), ("residual", C.c_double * 16), ("energy", C.c_double * 16)]
+                ("residual_checks", C.c_int * 16), ("residual", C.c_double * 16), ("energy", C.c_double * 16),
+                ("stat_margin", C.c_double * 16)]
 
     def levels(self):
         return [dict(W=self.width[k], H=self.height[k], outer=self.outer_exec[k], sweeps=self.sweeps_exec[k],
                      stat=self.last_stat[k], residual_checks=self.residual_checks[k], residual=self.residual[k],
-                     energy=self.energy[k]) for k in range(self.n_levels)]
+                     energy=self.energy[k], stat_margin=self.stat_margin[k]) for k in range(self.n_levels)]
 
 
 class PairInput(C.Structure):

@@ -137,6 +137,7 @@ struct rgbdflow_ctx {
   bool time_relax = false;
   std::vector<hipEvent_t> relax_ev;
   int relax_ev_used = 0;
+  int relax_n = 0;  // relaxation launches inside the timed event pairs
   // loaded batch
   bool loaded = false, ran = false;
   double load_ratio = 0;
@@ -623,6 +624,9 @@ void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, i
   const bool resid = k.inner_exit == RGBDFLOW_EXIT_RESIDUAL;
   auto check_of = [&](int last) { return (last % kCheckEvery == kCheckEvery - 1) ? last / kCheckEvery : -1; };
   if (st.tiled_ok) {
+    // timed: one event pair around the launches of this (outer, inner), which
+    // run back to back (around each launch where residual checks come between)
+    if (!resid && k.n_sor > 0) relax_event(c);
     int it = 0, flip = 0;
     while (it < k.n_sor) {
       const int boundary = (it / kCheckEvery + 1) * kCheckEvery;
@@ -631,15 +635,17 @@ void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, i
       const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
       const int chk = check_of(last);
       const int slot = chk >= 0 ? slot_base + chk : -1;
-      relax_event(c);
+      if (resid) relax_event(c);
       launch_relax_tiled(s, st.vtiled, st.tiled, st.grid, st.cfg, st.storage, st.relax_mode, st.slots, flip, T,
                          slot_base, n_prev, resid ? -1 : slot);
-      relax_event(c);
+      if (resid) relax_event(c);
+      c->relax_n += c->time_relax ? 1 : 0;
       // the launch read du (flip 0) or du_alt (flip 1) and wrote the other
       if (resid && chk >= 0) launch_residual(s, st.vpix, st.pix, flip ? 0 : 1, slot_base, chk, slot, k.residual_tol);
       flip ^= 1;
       it += T;
     }
+    if (!resid && k.n_sor > 0) relax_event(c);
     if (k.n_sor > 0) launch_copy_du(s, st.vpix, st.pix);
     return;
   }
@@ -653,6 +659,7 @@ void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, i
     relax_event(c);
     launch_relax_plain(s, st.vplain, st.plain, 1, slot_base, n_prev, resid ? -1 : slot);
     relax_event(c);
+    c->relax_n += c->time_relax ? 2 : 0;
     if (resid && chk >= 0) launch_residual(s, st.vpix, st.pix, 0, slot_base, chk, slot, k.residual_tol);
   }
 }
@@ -668,8 +675,8 @@ void enqueue_level_solve(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars
       if (h == 0) {
         launch_outer_first(s, st.votile, st.otile, k.n_inner * checks, k, k.n_inner > 1 ? 1 : 0, st.img);
       } else {
-        launch_phi(s, st.vpix, st.pix);
-        launch_system(s, st.vpix, st.pix, k);
+        launch_phi(s, st.vpix, st.pix, st.img);
+        launch_system(s, st.vpix, st.pix, k, st.img);
       }
       enqueue_relax(c, st, k, h * checks);
     }
@@ -679,10 +686,17 @@ void enqueue_level_solve(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars
 }
 
 void ctl_to_stats(const SolveCtl& ctl, int n_outer, bool allow_early, int& outer_exec, int& sweeps,
-                  double& last_stat) {
+                  double& last_stat, double& margin) {
   outer_exec = ctl.outer_done ? ctl.outer_exit_at + 1 : n_outer;
   sweeps = ctl.sweeps_exec;
   last_stat = (allow_early && outer_exec > 0) ? ctl.stat[outer_exec - 1] : -1.0;
+  // the statistic's closest approach to the early-exit threshold (.cu:948-951)
+  margin = -1.0;
+  if (allow_early)
+    for (int o = 0; o < outer_exec; o++) {
+      const double m = std::fabs(ctl.stat[o] - 4e-4) / 4e-4;
+      if (margin < 0 || m < margin) margin = m;
+    }
 }
 
 void ctl_residual_stats(const SolveCtl& ctl, rgbdflow_stats* st, int k) {
@@ -726,16 +740,12 @@ int download_planes(rgbdflow_ctx* c, const double* src, size_t N, int C, double*
 void collect_relax_timing(rgbdflow_ctx* c, rgbdflow_stats* st) {
   if (!c->time_relax || !st) return;
   double tot = 0;
-  int n = 0;
   for (int i = 0; i + 1 < c->relax_ev_used; i += 2) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, c->relax_ev[i], c->relax_ev[i + 1]) == hipSuccess) {
-      tot += ms;
-      n++;
-    }
+    if (hipEventElapsedTime(&ms, c->relax_ev[i], c->relax_ev[i + 1]) == hipSuccess) tot += ms;
   }
   st->relax_ms = tot;
-  st->relax_launches = n;
+  st->relax_launches = c->relax_n;
 }
 
 bool bad_dims(int W, int H) { return W < 2 || H < 2 || (long long)W * H > (1LL << 28); }
@@ -1154,6 +1164,7 @@ int batch_run(rgbdflow_ctx* c, const rgbdflow_params* p, rgbdflow_stats* st) {
   hipStream_t s = c->stream;
   const SolveScalars k = scalars_from(p, p->color_data_weight, p->alpha, p->gamma, p->n_outer, p->n_inner, p->n_sor);
   c->relax_ev_used = 0;
+  c->relax_n = 0;
   BatchPlan& bpm = c->plan;
   const bool graph = p->graph && !c->time_relax;
   if (graph && !bpm.exec && bpm.eager_runs > 0) {
@@ -1194,7 +1205,8 @@ int batch_run(rgbdflow_ctx* c, const rgbdflow_params* p, rgbdflow_stats* st) {
     for (int l = 0; l < nL; l++) {
       o.width[l] = ps.L[l].W;
       o.height[l] = ps.L[l].H;
-      ctl_to_stats(ctl[i][l], k.n_outer, l < nL - 1, o.outer_exec[l], o.sweeps_exec[l], o.last_stat[l]);
+      ctl_to_stats(ctl[i][l], k.n_outer, l < nL - 1, o.outer_exec[l], o.sweeps_exec[l], o.last_stat[l],
+                   o.stat_margin[l]);
       ctl_residual_stats(ctl[i][l], &o, l);
     }
     o.device_ms = ms;
@@ -1224,7 +1236,8 @@ void fill_stats(rgbdflow_ctx* c, const SolveCtl& ctl, const LevelView& view, int
   st->n_levels = 1;
   st->width[0] = view.W;
   st->height[0] = view.H;
-  ctl_to_stats(ctl, n_outer, view.allow_early != 0, st->outer_exec[0], st->sweeps_exec[0], st->last_stat[0]);
+  ctl_to_stats(ctl, n_outer, view.allow_early != 0, st->outer_exec[0], st->sweeps_exec[0], st->last_stat[0],
+               st->stat_margin[0]);
   ctl_residual_stats(ctl, st, 0);
   float ms = 0;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
@@ -1357,6 +1370,7 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
   TRY(one_step(c, "d.desc", items, opts, sp));
   hipStream_t s = c->stream;
   c->relax_ev_used = 0;
+  c->relax_n = 0;
   HIP_TRY(c, hipEventRecord(c->ev0, s));
   launch_level_weights(s, sp.vpix, sp.pix);
   enqueue_level_solve(c, sp, k);
@@ -1394,8 +1408,8 @@ int rgbdflow_level_system(rgbdflow_ctx* c, int W, int H, float focal_level, cons
   launch_level_weights(s, sp.vpix, sp.pix);
   launch_ctl_reset(s, sp.vpix);
   launch_derivs(s, sp.vpix, sp.pix, 0, sp.img);
-  launch_phi(s, sp.vpix, sp.pix);
-  launch_system(s, sp.vpix, sp.pix, k);
+  launch_phi(s, sp.vpix, sp.pix, sp.img);
+  launch_system(s, sp.vpix, sp.pix, k, sp.img);
   std::vector<float> rec((size_t)kRec * 4 * N);
   TRY(download(c, rec.data(), wk.coef, rec.size() * sizeof(float)));
   TRY(sync(c));
@@ -2093,6 +2107,7 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   const SolveScalars k = scalars_from(p, p->color_data_weight, p->alpha, p->gamma, p->n_outer, p->n_inner, p->n_sor);
   hipStream_t s = c->stream;
   c->relax_ev_used = 0;
+  c->relax_n = 0;
   HIP_TRY(c, hipEventRecord(c->ev0, s));
   launch_pair_prep(s, pwl, pstart[1]);
   // sceneFlowWithInit.cpp:51: computeHeuristicPb(Im1, dm1) when no map is given

@@ -268,8 +268,8 @@ __host__ __device__ inline int otile_blocks(int W, int H) { return ((W + kOtW - 
 // kernels_solve.hip (blocks = total blocks of the list for that family)
 void launch_ctl_reset(hipStream_t s, const Views& v);
 void launch_derivs(hipStream_t s, const Views& v, int blocks, int n_slots, int img);
-void launch_phi(hipStream_t s, const Views& v, int blocks);
-void launch_system(hipStream_t s, const Views& v, int blocks, const SolveScalars& k);
+void launch_phi(hipStream_t s, const Views& v, int blocks, int img);
+void launch_system(hipStream_t s, const Views& v, int blocks, const SolveScalars& k, int img);
 // derivatives + phi + system of the first inner iteration in one pass
 // (store: keep derivatives, occlusion weights and phi for later inner iterations)
 void launch_outer_first(hipStream_t s, const Views& v, int blocks, int n_slots, const SolveScalars& k, int store,

@@ -6,6 +6,10 @@
 // the re-warp.  Arithmetic follows the reference's precision exactly: images,
 // derivatives, products and u in f64; the system, du, q and the device weight
 // copies in f32; the squares inside the robust weights in f32 (sqrCU, .cu:44-47).
+// With the feature images in 16 bits (a precision variant, not in the
+// reference) every per-outer stage formula -- derivatives, occlusion weight,
+// penalty derivatives, Laplacians, assembly, the in-solve warp -- is evaluated
+// in f32 on f32-rounded inputs, in the same order (StageOf).
 // Built with -ffp-contract=off so every rounding happens where the source says.
 //
 // Every kernel runs over a work-list of (pair, level) items: block b belongs
@@ -13,6 +17,8 @@
 // device: a kernel reads its item's SolveCtl at entry and returns when that
 // pair's outer loop or current relaxation has stopped, so the host enqueues
 // whole levels of whole batches without waiting.
+#include <type_traits>
+
 #include "internal.hpp"
 #include "records.hpp"
 #include "relax_pixel.hpp"
@@ -28,6 +34,20 @@ __device__ __forceinline__ float sqr_f32(double x) {
 }
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// The stage arithmetic of a feature-image storage S: f64 for the reference's
+// f64 images, f32 for the 16-bit ones.
+template <int S> struct StageOf { typedef double T; };
+template <> struct StageOf<kImgF16> { typedef float T; };
+template <> struct StageOf<kImgBF16> { typedef float T; };
+template <class T> constexpr bool kF32 = std::is_same<T, float>::value;
+
+// 0.5 / sqrt(x^2 + eps) (.cu:356-369): f64 from the f32 square (sqrCU), or all f32.
+template <class T>
+__device__ __forceinline__ T half_rsqrt_sq(T x, double eps) {
+  if constexpr (kF32<T>) return .5f / sqrtf(x * x + (float)eps);
+  else return .5 / sqrt((double)sqr_f32(x) + eps);
+}
 
 // One block per item: zero the item's control block.
 __global__ void k_ctl_reset(Views w) {
@@ -52,60 +72,94 @@ __device__ __forceinline__ double div_const(double x) {
   }
   return x / D;
 }
-__device__ __forceinline__ double div12(double x) { return div_const<12>(x); }
+// The same in f32: Markstein's correction for |x| in [2^-100, 2^100] (the
+// quotient, the product and the remainder stay normal or exact there), the
+// IEEE division elsewhere; oracle/divcheck.cpp checks it bit for bit over a
+// whole binade, which the power-of-two scaling carries to the domain.
+template <int D>
+__device__ __forceinline__ float div_const(float x) {
+  const float ax = fabsf(x);
+  if (__builtin_expect(ax >= 0x1p-100f && ax <= 0x1p100f, 1)) {
+    constexpr float y = 1.0f / D;
+    const float q0 = x * y;
+    return __builtin_fmaf(__builtin_fmaf(-(float)D, q0, x), y, q0);
+  }
+  return x / (float)D;
+}
+template <class T>
+__device__ __forceinline__ T div12(T x) { return div_const<12>(x); }
 
 // Image and depth derivatives of one pixel (.cu:84-147): x/y derivatives of
 // 0.4*I1 + 0.6*warped I2 with [1,-8,0,8,-1]/12, clamp-to-edge; It = Iw - I1;
 // depth derivatives clamped to +-0.04.
+template <class T>
 struct PixDerivs {
-  double Ix[kFeat], Iy[kFeat], It[kFeat];
-  double Dx, Dy, Dt;
+  T Ix[kFeat], Iy[kFeat], It[kFeat];
+  T Dx, Dy, Dt;
 };
 
-template <int S>
-__device__ __forceinline__ void derivs_at(const LevelView& v, int l, int i, int j, PixDerivs& d) {
+// The stencil input 0.4 I1 + 0.6 Iw of one value pair.
+template <class T>
+__device__ __forceinline__ T blend(double a, double b) { return (T).4 * (T)a + (T).6 * (T)b; }
+template <class T>
+__device__ __forceinline__ T clamp_mg(T x) {
+  const T mg = (T).04;
+  return fmin(fmax(x, -mg), mg);
+}
+
+template <int S, class T = typename StageOf<S>::T>
+__device__ __forceinline__ void derivs_at(const LevelView& v, int l, int i, int j, PixDerivs<T>& d) {
   const int W = v.W, H = v.H, N = v.N;
-  const double g[5] = {1, -8, 0, 8, -1};
+  const T g[5] = {1, -8, 0, 8, -1};
   for (int k = 0; k < kFeat; k++) {
     const size_t kN = (size_t)k * N;
-    double sx = 0, sy = 0;
+    T sx = 0, sy = 0;
     for (int m = 0; m < 5; m++) {
       const size_t o = kN + i * W + clampi(j - 2 + m, 0, W - 1);
-      sx += g[m] * (.4 * img_ld<S>(v.I1, o) + .6 * img_ld<S>(v.Iw, o));
+      sx += g[m] * blend<T>(img_ld<S>(v.I1, o), img_ld<S>(v.Iw, o));
     }
     for (int m = 0; m < 5; m++) {
       const size_t o = kN + clampi(i - 2 + m, 0, H - 1) * W + j;
-      sy += g[m] * (.4 * img_ld<S>(v.I1, o) + .6 * img_ld<S>(v.Iw, o));
+      sy += g[m] * blend<T>(img_ld<S>(v.I1, o), img_ld<S>(v.Iw, o));
     }
     d.Ix[k] = div12(sx);
     d.Iy[k] = div12(sy);
-    d.It[k] = img_ld<S>(v.Iw, kN + l) - img_ld<S>(v.I1, kN + l);
+    d.It[k] = (T)img_ld<S>(v.Iw, kN + l) - (T)img_ld<S>(v.I1, kN + l);
   }
-  const double mg = .04;
-  double sx = 0, sy = 0;
+  T sx = 0, sy = 0;
   for (int m = 0; m < 5; m++) {
     const int o = i * W + clampi(j - 2 + m, 0, W - 1);
-    sx += g[m] * (.4 * gp(v.D1)[o] + .6 * gp(v.Dw)[o]);
+    sx += g[m] * blend<T>(gp(v.D1)[o], gp(v.Dw)[o]);
   }
   for (int m = 0; m < 5; m++) {
     const int o = clampi(i - 2 + m, 0, H - 1) * W + j;
-    sy += g[m] * (.4 * gp(v.D1)[o] + .6 * gp(v.Dw)[o]);
+    sy += g[m] * blend<T>(gp(v.D1)[o], gp(v.Dw)[o]);
   }
-  d.Dx = fmin(fmax(div12(sx), -mg), mg);
-  d.Dy = fmin(fmax(div12(sy), -mg), mg);
-  d.Dt = fmin(fmax(gp(v.Dw)[l] - gp(v.D1)[l], -mg), mg);
+  d.Dx = clamp_mg(div12(sx));
+  d.Dy = clamp_mg(div12(sy));
+  d.Dt = clamp_mg((T)gp(v.Dw)[l] - (T)gp(v.D1)[l]);
 }
 
-// .cu:640-648: occlusion weight from the UNWARPED D2.
+// .cu:640-648: occlusion weight from the UNWARPED D2 (d1, u2, d2 at the pixel).
+template <class T>
+__device__ __forceinline__ float occlusion_of(double d1, double u2, double d2) {
+  if constexpr (kF32<T>) {
+    const float f = (float)d1;
+    return (f + (float)u2 < (float)d2 + .02f * (f * f)) ? 1.f : (float).01;
+  } else {
+    return (d1 + u2 < d2 + .02 * (double)sqr_f32(d1)) ? 1.f : (float).01;
+  }
+}
+template <class T>
 __device__ __forceinline__ float occlusion_at(const LevelView& v, int l) {
-  const double d1 = gp(v.D1)[l];
-  return (d1 + gp(v.u)[2 * v.N + l] < gp(v.D2)[l] + .02 * (double)sqr_f32(d1)) ? 1.f : (float).01;
+  return occlusion_of<T>(gp(v.D1)[l], gp(v.u)[2 * v.N + l], gp(v.D2)[l]);
 }
 
 // Derivatives + occlusion stored for the inner iterations, du = 0 (.cu:776);
 // also zeroes the item's check slots of this outer iteration.
 template <int S>
 __global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
+  typedef typename StageOf<S>::T T;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int it = work_item(w, blk);
   const LevelView& v = w.items[it];
@@ -116,7 +170,7 @@ __global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
   if (l >= v.N) return;
   const int N = v.N;
   const int i = l / v.W, j = l - i * v.W;
-  PixDerivs d;
+  PixDerivs<T> d;
   derivs_at<S>(v, l, i, j, d);
   for (int k = 0; k < kFeat; k++) {
     gp(v.Ix)[(size_t)k * N + l] = d.Ix[k];
@@ -127,7 +181,7 @@ __global__ void __launch_bounds__(kBlock) k_derivs(Views w, int n_slots) {
   gp(v.Dy)[l] = d.Dy;
   gp(v.Dt)[l] = d.Dt;
   gstore(v.du + l, make_float4(0.f, 0.f, 0.f, 0.f));
-  gp(v.q)[l] = occlusion_at(v, l);
+  gp(v.q)[l] = occlusion_at<T>(v, l);
 }
 
 __device__ __forceinline__ float du_comp(const float4& d, int c) { return c == 0 ? d.x : (c == 1 ? d.y : d.z); }
@@ -135,35 +189,41 @@ __device__ __forceinline__ float du_comp(const float4& d, int c) { return c == 0
 // .cu:156-182: forward differences of u + du (zero on the last row/column).
 // zero_du: du is known to be zero (first inner iteration), the same sums
 // with du's terms written as 0.
-template <bool zero_du>
-__device__ __forceinline__ void flow_grad(const LevelView& v, int c, int l, int i, int j, double& gx, double& gy) {
+template <class T, bool zero_du>
+__device__ __forceinline__ void flow_grad(const LevelView& v, int c, int l, int i, int j, T& gx, T& gy) {
   const RGBDFLOW_GLOBAL double* uc = gp(v.u) + (size_t)c * v.N;
-  const double dl = zero_du ? 0.0 : (double)du_comp(gload(v.du + l), c);
-  const double dd = zero_du || i == v.H - 1 ? 0.0 : (double)du_comp(gload(v.du + l + v.W), c);
-  const double dr = zero_du || j == v.W - 1 ? 0.0 : (double)du_comp(gload(v.du + l + 1), c);
-  gy = (i == v.H - 1) ? 0.0 : (uc[l + v.W] + dd - uc[l] - dl);
-  gx = (j == v.W - 1) ? 0.0 : (uc[l + 1] + dr - uc[l] - dl);
+  const T dl = zero_du ? (T)0 : (T)du_comp(gload(v.du + l), c);
+  const T dd = zero_du || i == v.H - 1 ? (T)0 : (T)du_comp(gload(v.du + l + v.W), c);
+  const T dr = zero_du || j == v.W - 1 ? (T)0 : (T)du_comp(gload(v.du + l + 1), c);
+  gy = (i == v.H - 1) ? (T)0 : ((T)uc[l + v.W] + dd - (T)uc[l] - dl);
+  gx = (j == v.W - 1) ? (T)0 : ((T)uc[l + 1] + dr - (T)uc[l] - dl);
 }
 
 // .cu:354-355: smoothness weight phi' of the Charbonnier penalty from the
 // forward differences (x, y) of the three flow components and the pixel's
 // depth-smoothness weight.
-__device__ __forceinline__ double phi_from_grads(double ux, double uy, double vx, double vy, double wx, double wy,
-                                                 float sw) {
-  const float t = sqr_f32(ux) + sqr_f32(uy) + sqr_f32(vx) + sqr_f32(vy) + sw * (sqr_f32(wx) + sqr_f32(wy));
-  return 0.5 / sqrt((double)t + 1e-6);
+template <class T>
+__device__ __forceinline__ T phi_from_grads(T ux, T uy, T vx, T vy, T wx, T wy, float sw) {
+  if constexpr (kF32<T>) {
+    const float t = ux * ux + uy * uy + vx * vx + vy * vy + sw * (wx * wx + wy * wy);
+    return .5f / sqrtf(t + 1e-6f);
+  } else {
+    const float t = sqr_f32(ux) + sqr_f32(uy) + sqr_f32(vx) + sqr_f32(vy) + sw * (sqr_f32(wx) + sqr_f32(wy));
+    return 0.5 / sqrt((double)t + 1e-6);
+  }
 }
 
 // phi' at pixel l.
-template <bool zero_du>
-__device__ __forceinline__ double phi_at(const LevelView& v, int l, int i, int j) {
-  double ux, uy, vx, vy, wx, wy;
-  flow_grad<zero_du>(v, 0, l, i, j, ux, uy);
-  flow_grad<zero_du>(v, 1, l, i, j, vx, vy);
-  flow_grad<zero_du>(v, 2, l, i, j, wx, wy);
-  return phi_from_grads(ux, uy, vx, vy, wx, wy, gp(v.sW)[l]);
+template <class T, bool zero_du>
+__device__ __forceinline__ T phi_at(const LevelView& v, int l, int i, int j) {
+  T ux, uy, vx, vy, wx, wy;
+  flow_grad<T, zero_du>(v, 0, l, i, j, ux, uy);
+  flow_grad<T, zero_du>(v, 1, l, i, j, vx, vy);
+  flow_grad<T, zero_du>(v, 2, l, i, j, wx, wy);
+  return phi_from_grads<T>(ux, uy, vx, vy, wx, wy, gp(v.sW)[l]);
 }
 
+template <int S>
 __global__ void __launch_bounds__(kBlock) k_phi(Views w) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int it = work_item(w, blk);
@@ -172,29 +232,31 @@ __global__ void __launch_bounds__(kBlock) k_phi(Views w) {
   const int l = (blk - w.start[it]) * kBlock + threadIdx.x;
   if (l >= v.N) return;
   const int i = l / v.W, j = l - i * v.W;
-  gp(v.phi)[l] = phi_at<false>(v, l, i, j);
+  gp(v.phi)[l] = phi_at<typename StageOf<S>::T, false>(v, l, i, j);
 }
 
 // The values around one pixel that the Laplacians and the assembly read
 // (zeros for neighbours outside the image, which are never used).
+template <class T>
 struct PixNbr {
-  double uC[3], uL[3], uR[3], uU[3], uD[3];  // u at the pixel and its left, right, upper, lower neighbours
+  T uC[3], uL[3], uR[3], uU[3], uD[3];  // u at the pixel and its left, right, upper, lower neighbours
   float rR, rRl, rD, rDu;  // right weight at the pixel and its left neighbour; down weight at the pixel and above
   float sC, sL, sU;        // depth-smoothness weights at the pixel, its left and its upper neighbour
   float dW, mW;            // depth-data and depth-magnitude weights at the pixel
 };
 
-__device__ __forceinline__ void load_nbr(const LevelView& v, int l, int i, int j, PixNbr& nb) {
+template <class T>
+__device__ __forceinline__ void load_nbr(const LevelView& v, int l, int i, int j, PixNbr<T>& nb) {
   const int W = v.W, H = v.H, N = v.N;
   const bool hl = j > 0, hr = j < W - 1, hu = i > 0, hd = i < H - 1;
   const RGBDFLOW_GLOBAL double* u = gp(v.u);
   for (int c = 0; c < 3; c++) {
     const size_t o = (size_t)c * N + l;
-    nb.uC[c] = u[o];
-    nb.uL[c] = hl ? u[o - 1] : 0.0;
-    nb.uR[c] = hr ? u[o + 1] : 0.0;
-    nb.uU[c] = hu ? u[o - W] : 0.0;
-    nb.uD[c] = hd ? u[o + W] : 0.0;
+    nb.uC[c] = (T)u[o];
+    nb.uL[c] = hl ? (T)u[o - 1] : (T)0;
+    nb.uR[c] = hr ? (T)u[o + 1] : (T)0;
+    nb.uU[c] = hu ? (T)u[o - W] : (T)0;
+    nb.uD[c] = hd ? (T)u[o + W] : (T)0;
   }
   const RGBDFLOW_GLOBAL float* rR = gp(v.rW);
   const RGBDFLOW_GLOBAL float* rD = gp(v.rW) + N;
@@ -215,43 +277,45 @@ __device__ __forceinline__ void load_nbr(const LevelView& v, int l, int i, int j
 // pL, pU: phi at the pixel, its left and its upper neighbour.  The mean
 // (norm / count, the same for the three components) comes from
 // laplacian_mean: 0 when |norm| < 1e-5 (the Laplacian is then 0).
-__device__ __forceinline__ double laplacian_mean(const PixNbr& nb, int W, int H, int i, int j) {
-  double norm = 0;
+template <class T>
+__device__ __forceinline__ T laplacian_mean(const PixNbr<T>& nb, int W, int H, int i, int j) {
+  T norm = 0;
   int count = 0;
   if (j < W - 1) { norm += nb.rR; count++; }
   if (j > 0) { norm += nb.rRl; count++; }
   if (i < H - 1) { norm += nb.rD; count++; }
   if (i > 0) { norm += nb.rDu; count++; }
-  if (fabs(norm) < 1e-5) return 0.0;
+  if (fabs(norm) < (T)1e-5) return (T)0;
   // norm / count: exact scalings for 2 and 4 (norm >= 1e-5: no underflow),
   // the exact constant division for 3
-  return count == 3 ? div_const<3>(norm) : count == 2 ? norm * 0.5 : count == 4 ? norm * 0.25 : norm;
+  return count == 3 ? div_const<3>(norm) : count == 2 ? norm * (T)0.5 : count == 4 ? norm * (T)0.25 : norm;
 }
-__device__ __forceinline__ double laplacian(const PixNbr& nb, int c, int W, int H, int i, int j, double pC, double pL,
-                                            double pU, double mean) {
-  const double uc = nb.uC[c];
-  double lap = 0;
-  if (j < W - 1) lap -= (nb.uR[c] - uc) * pC * (double)nb.rR;
-  if (j > 0) lap += (uc - nb.uL[c]) * pL * (double)nb.rRl;
-  if (i < H - 1) lap -= (nb.uD[c] - uc) * pC * (double)nb.rD;
-  if (i > 0) lap += (uc - nb.uU[c]) * pU * (double)nb.rDu;
-  return mean == 0.0 ? 0.0 : lap / mean;
+template <class T>
+__device__ __forceinline__ T laplacian(const PixNbr<T>& nb, int c, int W, int H, int i, int j, T pC, T pL, T pU,
+                                       T mean) {
+  const T uc = nb.uC[c];
+  T lap = 0;
+  if (j < W - 1) lap -= (nb.uR[c] - uc) * pC * (T)nb.rR;
+  if (j > 0) lap += (uc - nb.uL[c]) * pL * (T)nb.rRl;
+  if (i < H - 1) lap -= (nb.uD[c] - uc) * pC * (T)nb.rD;
+  if (i > 0) lap += (uc - nb.uU[c]) * pU * (T)nb.rDu;
+  return mean == (T)0 ? (T)0 : lap / mean;
 }
 
 // .cu:356-369 (psi', pi'), .cu:189-230 (products), .cu:477-575 (assembly),
 // written as the compact per-pixel system (Coef).  d0, d1, d2: du at the
 // pixel; pC, pL, pU: phi at the pixel, its left and upper neighbours (pL, pU
 // unused where the neighbour does not exist).
-__device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars& s, int l, int i, int j, double d0,
-                                         double d1, double d2, const PixDerivs& dv, double ql, double pC, double pL,
-                                         double pU, const PixNbr& nb) {
+template <class T>
+__device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars& s, int l, int i, int j, T d0, T d1,
+                                         T d2, const PixDerivs<T>& dv, T ql, T pC, T pL, T pU, const PixNbr<T>& nb) {
   const int W = v.W, H = v.H;
-  const double alpha = s.alpha, gamma = s.gamma;
-  double sxy = 0, sxx = 0, syy = 0, stx = 0, sty = 0;
-  double psi[kFeat];
+  const T alpha = (T)s.alpha, gamma = (T)s.gamma;
+  T sxy = 0, sxx = 0, syy = 0, stx = 0, sty = 0;
+  T psi[kFeat];
   for (int k = 0; k < kFeat; k++) {
-    const double tt = (dv.It[k] + dv.Ix[k] * d0 + dv.Iy[k] * d1) * ql;
-    psi[k] = .5 / sqrt((double)sqr_f32(tt) + 1e-6) * s.color_weight;
+    const T tt = (dv.It[k] + dv.Ix[k] * d0 + dv.Iy[k] * d1) * ql;
+    psi[k] = half_rsqrt_sq<T>(tt, 1e-6) * (T)s.color_weight;
   }
   for (int k = 0; k < kFeat; k++) {
     sxy += psi[k] * dv.Ix[k] * dv.Iy[k];
@@ -261,56 +325,58 @@ __device__ __forceinline__ void assemble(const LevelView& v, const SolveScalars&
   for (int k = 0; k < kFeat; k++) stx += psi[k] * dv.Ix[k] * dv.It[k];
   for (int k = 0; k < kFeat; k++) sty += psi[k] * dv.Iy[k] * dv.It[k];
   static_assert(kFeat == 5, "channel means divide by 5");
-  const double ixy = div_const<5>(sxy), ix2 = div_const<5>(sxx), iy2 = div_const<5>(syy);
-  const double itx = div_const<5>(stx * ql * ql), ity = div_const<5>(sty * ql * ql);
+  const T ixy = div_const<5>(sxy), ix2 = div_const<5>(sxx), iy2 = div_const<5>(syy);
+  const T itx = div_const<5>(stx * ql * ql), ity = div_const<5>(sty * ql * ql);
 
-  const double Dx = dv.Dx, Dy = dv.Dy, Dt = dv.Dt;
-  const double t2 = Dt + Dx * d0 + Dy * d1 - d2;
-  const double P = .5 / sqrt((double)sqr_f32(t2) + 1e-10) * (double)nb.dW;
-  const double dxy = P * Dx * Dy, dx2 = P * Dx * Dx, dy2 = P * Dy * Dy;
-  const double dtx = P * Dx * Dt, dty = P * Dy * Dt;
+  const T Dx = dv.Dx, Dy = dv.Dy, Dt = dv.Dt;
+  const T t2 = Dt + Dx * d0 + Dy * d1 - d2;
+  const T P = half_rsqrt_sq<T>(t2, 1e-10) * (T)nb.dW;
+  const T dxy = P * Dx * Dy, dx2 = P * Dx * Dx, dy2 = P * Dy * Dy;
+  const T dtx = P * Dx * Dt, dty = P * Dy * Dt;
 
-  const double mean = laplacian_mean(nb, W, H, i, j);
-  const double uL = laplacian(nb, 0, W, H, i, j, pC, pL, pU, mean), vL = laplacian(nb, 1, W, H, i, j, pC, pL, pU, mean);
-  const double wL = (double)nb.sC * laplacian(nb, 2, W, H, i, j, pC, pL, pU, mean);
+  const T mean = laplacian_mean(nb, W, H, i, j);
+  const T uL = laplacian(nb, 0, W, H, i, j, pC, pL, pU, mean), vL = laplacian(nb, 1, W, H, i, j, pC, pL, pU, mean);
+  const T wL = (T)nb.sC * laplacian(nb, 2, W, H, i, j, pC, pL, pU, mean);
 
-  double wsXY = 0, wsZ = 0;
+  T wsXY = 0, wsZ = 0;
   float eRxy = 0.f, eRz = 0.f, eDxy = 0.f, eDz = 0.f;
-  if (j > 0) { const double a = pL * (double)nb.rRl, b = a * (double)nb.sL; wsXY += a; wsZ += b; }
-  if (i > 0) { const double a = pU * (double)nb.rDu, b = a * (double)nb.sU; wsXY += a; wsZ += b; }
+  if (j > 0) { const T a = pL * (T)nb.rRl, b = a * (T)nb.sL; wsXY += a; wsZ += b; }
+  if (i > 0) { const T a = pU * (T)nb.rDu, b = a * (T)nb.sU; wsXY += a; wsZ += b; }
   if (j < W - 1) {
-    const double a = pC * (double)nb.rR, b = a * (double)nb.sC;
+    const T a = pC * (T)nb.rR, b = a * (T)nb.sC;
     eRxy = (float)(alpha * a); eRz = (float)(alpha * b); wsXY += a; wsZ += b;
   }
   if (i < H - 1) {
-    const double a = pC * (double)nb.rD, b = a * (double)nb.sC;
+    const T a = pC * (T)nb.rD, b = a * (T)nb.sC;
     eDxy = (float)(alpha * a); eDz = (float)(alpha * b); wsXY += a; wsZ += b;
   }
-  const double u0 = nb.uC[0], u1 = nb.uC[1], u2 = nb.uC[2];
-  const double m = (double)nb.mW;
+  const T u0 = nb.uC[0], u1 = nb.uC[1], u2 = nb.uC[2];
+  const T m = (T)nb.mW;
   float t;
   t = (float)(-(ix2 + dx2 + alpha * wsXY));
-  const float Buu = (float)((double)t + -(gamma));
+  const float Buu = (float)((T)t + -(gamma));
   const float Buv = (float)(-ixy - dxy);
   const float Buw = (float)(P * Dx);
   t = (float)(-(-itx - dtx + alpha * -uL));
-  const float ru = (float)((double)t + -(-gamma * u0));
+  const float ru = (float)((T)t + -(-gamma * u0));
   t = (float)(-(iy2 + dy2 + alpha * wsXY));
-  const float Bvv = (float)((double)t + -(gamma));
+  const float Bvv = (float)((T)t + -(gamma));
   const float Bvw = (float)(P * Dy);
   t = (float)(-(-ity - dty + alpha * -vL));
-  const float rv = (float)((double)t + -(-gamma * u1));
+  const float rv = (float)((T)t + -(-gamma * u1));
   t = (float)(-(P + alpha * wsZ));
-  const float Bww = (float)((double)t + -(gamma * m));
+  const float Bww = (float)((T)t + -(gamma * m));
   t = (float)(-(P * Dt + alpha * -wL));
-  const float rw = (float)((double)t + -(-gamma * m * u2));
+  const float rw = (float)((T)t + -(-gamma * m * u2));
   const float rec[kCoef] = {eRxy, eRz, eDxy, eDz, Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw};
   store_record(v.coef, v.storage, (size_t)v.N, (size_t)l, rec);
 }
 
 // The system of an inner iteration after the first, from the stored
 // derivatives, occlusion weights and phi (k_phi) and the current du.
+template <int S>
 __global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
+  typedef typename StageOf<S>::T T;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int it = work_item(w, blk);
   const LevelView& v = w.items[it];
@@ -319,20 +385,20 @@ __global__ void __launch_bounds__(kBlock) k_system(Views w, SolveScalars s) {
   if (l >= v.N) return;
   const int W = v.W, N = v.N;
   const int i = l / W, j = l - i * W;
-  PixDerivs d;
+  PixDerivs<T> d;
   for (int k = 0; k < kFeat; k++) {
-    d.Ix[k] = gp(v.Ix)[(size_t)k * N + l];
-    d.Iy[k] = gp(v.Iy)[(size_t)k * N + l];
-    d.It[k] = gp(v.It)[(size_t)k * N + l];
+    d.Ix[k] = (T)gp(v.Ix)[(size_t)k * N + l];
+    d.Iy[k] = (T)gp(v.Iy)[(size_t)k * N + l];
+    d.It[k] = (T)gp(v.It)[(size_t)k * N + l];
   }
-  d.Dx = gp(v.Dx)[l];
-  d.Dy = gp(v.Dy)[l];
-  d.Dt = gp(v.Dt)[l];
-  const double pC = gp(v.phi)[l], pL = j > 0 ? gp(v.phi)[l - 1] : 0.0, pU = i > 0 ? gp(v.phi)[l - W] : 0.0;
+  d.Dx = (T)gp(v.Dx)[l];
+  d.Dy = (T)gp(v.Dy)[l];
+  d.Dt = (T)gp(v.Dt)[l];
+  const T pC = (T)gp(v.phi)[l], pL = j > 0 ? (T)gp(v.phi)[l - 1] : (T)0, pU = i > 0 ? (T)gp(v.phi)[l - W] : (T)0;
   const float4 du = gload(v.du + l);
-  PixNbr nb;
+  PixNbr<T> nb;
   load_nbr(v, l, i, j, nb);
-  assemble(v, s, l, i, j, (double)du.x, (double)du.y, (double)du.z, d, (double)gp(v.q)[l], pC, pL, pU, nb);
+  assemble<T>(v, s, l, i, j, (T)du.x, (T)du.y, (T)du.z, d, (T)gp(v.q)[l], pC, pL, pU, nb);
 }
 
 // The first inner iteration of an outer iteration in one pass: derivatives,
@@ -354,12 +420,16 @@ constexpr int kRw = kOtW + 4, kRh = kOtH + 4;  // stencil region (2-pixel border
 constexpr int kUw = kOtW + 2, kUh = kOtH + 2;  // neighbourhood region (1-pixel border)
 constexpr int kPw = kOtW + 1, kPh = kOtH + 1;  // phi region (tile, left column, upper row)
 
+// (the f32 stage asks for 6 waves per SIMD: its registers fit without spills,
+// and its halved LDS tiles allow them; the f64 stage is held to 4 by its LDS)
 template <int S>
-__global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, SolveScalars s, int store) {
-  __shared__ double Ms[kFeat + 1][kRh * kRw];  // 0.4*I1 + 0.6*Iw (5 features, depth)
-  __shared__ double Us[3][kUh * kUw];          // u
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(S == kImgF64 ? 1 : 6)))
+k_outer_first(Views w, int n_slots, SolveScalars s, int store) {
+  typedef typename StageOf<S>::T T;
+  __shared__ T Ms[kFeat + 1][kRh * kRw];  // 0.4*I1 + 0.6*Iw (5 features, depth)
+  __shared__ T Us[3][kUh * kUw];          // u
   __shared__ float Ss[kUh * kUw], RRs[kUh * kUw], RDs[kUh * kUw];  // sW, rW right, rW down
-  __shared__ double Ps[kPh * kPw];             // phi
+  __shared__ T Ps[kPh * kPw];             // phi
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int it = work_item(w, blk);
   const LevelView& v = w.items[it];
@@ -379,12 +449,13 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
   const RGBDFLOW_GLOBAL double* D1 = gp(v.D1);
   const RGBDFLOW_GLOBAL double* Dw = gp(v.Dw);
   // the pixel's own values
-  double i1c[kFeat], iwc[kFeat], d1c = 0, dwc = 0, d2c = 0;
+  T i1c[kFeat], iwc[kFeat];
+  double d1c = 0, dwc = 0, d2c = 0;
   float dwt = 0.f, mwt = 0.f;
   if (inside) {
     for (int k = 0; k < kFeat; k++) {
-      i1c[k] = img_ld<S>(v.I1, (size_t)k * N + l);
-      iwc[k] = img_ld<S>(v.Iw, (size_t)k * N + l);
+      i1c[k] = (T)img_ld<S>(v.I1, (size_t)k * N + l);
+      iwc[k] = (T)img_ld<S>(v.Iw, (size_t)k * N + l);
     }
     d1c = D1[l];
     dwc = Dw[l];
@@ -396,20 +467,20 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     const int ry = t / kRw, rx = t - ry * kRw;
     const int o = clampi(y0 - 2 + ry, 0, H - 1) * W + clampi(x0 - 2 + rx, 0, W - 1);
     for (int k = 0; k < kFeat; k++)
-      Ms[k][t] = .4 * img_ld<S>(v.I1, (size_t)k * N + o) + .6 * img_ld<S>(v.Iw, (size_t)k * N + o);
-    Ms[kFeat][t] = .4 * D1[o] + .6 * Dw[o];
+      Ms[k][t] = blend<T>(img_ld<S>(v.I1, (size_t)k * N + o), img_ld<S>(v.Iw, (size_t)k * N + o));
+    Ms[kFeat][t] = blend<T>(D1[o], Dw[o]);
   }
   for (int t = threadIdx.x; t < kUh * kUw; t += kBlock) {
     const int ry = t / kUw, rx = t - ry * kUw;
     const int gy = y0 - 1 + ry, gx = x0 - 1 + rx;
     if (gy >= 0 && gx >= 0 && gy < H && gx < W) {
       const int o = gy * W + gx;
-      for (int c = 0; c < 3; c++) Us[c][t] = gp(v.u)[(size_t)c * N + o];
+      for (int c = 0; c < 3; c++) Us[c][t] = (T)gp(v.u)[(size_t)c * N + o];
       Ss[t] = gp(v.sW)[o];
       RRs[t] = gp(v.rW)[o];
       RDs[t] = gp(v.rW)[(size_t)N + o];
     } else {
-      for (int c = 0; c < 3; c++) Us[c][t] = 0.0;
+      for (int c = 0; c < 3; c++) Us[c][t] = (T)0;
       Ss[t] = RRs[t] = RDs[t] = 0.f;
     }
   }
@@ -418,27 +489,27 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
   for (int t = threadIdx.x; t < kPh * kPw; t += kBlock) {
     const int py = t / kPw, px = t - py * kPw;
     const int pi = y0 - 1 + py, pj = x0 - 1 + px;
-    double r = 0.0;
+    T r = 0;
     if (pi >= 0 && pj >= 0 && pi < H && pj < W) {
       const int c = py * kUw + px;  // (pi, pj) in the neighbourhood region
       const bool last_row = pi == H - 1, last_col = pj == W - 1;
-      double gx[3], gy[3];
+      T gx[3], gy[3];
       for (int k = 0; k < 3; k++) {
-        gy[k] = last_row ? 0.0 : (Us[k][c + kUw] + 0.0 - Us[k][c] - 0.0);
-        gx[k] = last_col ? 0.0 : (Us[k][c + 1] + 0.0 - Us[k][c] - 0.0);
+        gy[k] = last_row ? (T)0 : (Us[k][c + kUw] + (T)0 - Us[k][c] - (T)0);
+        gx[k] = last_col ? (T)0 : (Us[k][c + 1] + (T)0 - Us[k][c] - (T)0);
       }
-      r = phi_from_grads(gx[0], gy[0], gx[1], gy[1], gx[2], gy[2], Ss[c]);
+      r = phi_from_grads<T>(gx[0], gy[0], gx[1], gy[1], gx[2], gy[2], Ss[c]);
     }
     Ps[t] = r;
   }
   __syncthreads();
   if (!inside) return;
   // derivs_at, with the stencil inputs from LDS (same values, same order)
-  PixDerivs d;
-  const double g[5] = {1, -8, 0, 8, -1};
+  PixDerivs<T> d;
+  const T g[5] = {1, -8, 0, 8, -1};
   const int c0 = (ty + 2) * kRw + txx + 2;
   for (int k = 0; k < kFeat; k++) {
-    double sx = 0, sy = 0;
+    T sx = 0, sy = 0;
     for (int m = 0; m < 5; m++) sx += g[m] * Ms[k][c0 - 2 + m];
     for (int m = 0; m < 5; m++) sy += g[m] * Ms[k][c0 + (m - 2) * kRw];
     d.Ix[k] = div12(sx);
@@ -446,15 +517,14 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     d.It[k] = iwc[k] - i1c[k];
   }
   {
-    const double mg = .04;
-    double sx = 0, sy = 0;
+    T sx = 0, sy = 0;
     for (int m = 0; m < 5; m++) sx += g[m] * Ms[kFeat][c0 - 2 + m];
     for (int m = 0; m < 5; m++) sy += g[m] * Ms[kFeat][c0 + (m - 2) * kRw];
-    d.Dx = fmin(fmax(div12(sx), -mg), mg);
-    d.Dy = fmin(fmax(div12(sy), -mg), mg);
-    d.Dt = fmin(fmax(dwc - d1c, -mg), mg);
+    d.Dx = clamp_mg(div12(sx));
+    d.Dy = clamp_mg(div12(sy));
+    d.Dt = clamp_mg((T)dwc - (T)d1c);
   }
-  PixNbr nb;
+  PixNbr<T> nb;
   const int cu = (ty + 1) * kUw + txx + 1;
   for (int k = 0; k < 3; k++) {
     nb.uC[k] = Us[k][cu];
@@ -473,10 +543,10 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
   nb.dW = dwt;
   nb.mW = mwt;
   // occlusion_at (.cu:640-648): the unwarped D2
-  const float q = (d1c + nb.uC[2] < d2c + .02 * (double)sqr_f32(d1c)) ? 1.f : (float).01;
+  const float q = occlusion_of<T>(d1c, (double)nb.uC[2], d2c);
   gstore(v.du + l, make_float4(0.f, 0.f, 0.f, 0.f));
   const int p0 = (ty + 1) * kPw + txx + 1;
-  const double pC = Ps[p0], pL = Ps[p0 - 1], pU = Ps[p0 - kPw];
+  const T pC = Ps[p0], pL = Ps[p0 - 1], pU = Ps[p0 - kPw];
   if (store) {
     for (int k = 0; k < kFeat; k++) {
       gp(v.Ix)[(size_t)k * N + l] = d.Ix[k];
@@ -489,7 +559,7 @@ __global__ void __launch_bounds__(kBlock) k_outer_first(Views w, int n_slots, So
     gp(v.q)[l] = q;
     gp(v.phi)[l] = pC;
   }
-  assemble(v, s, l, i, j, 0.0, 0.0, 0.0, d, (double)q, pC, pL, pU, nb);
+  assemble<T>(v, s, l, i, j, (T)0, (T)0, (T)0, d, (T)q, pC, pL, pU, nb);
 }
 
 // .cu:382-470: one colour of the red-black relaxation, one thread per pixel of
@@ -538,6 +608,7 @@ __global__ void __launch_bounds__(256) k_relax_plain(Views w, int colour, int sl
 // sum(du_k^2 * c_k) (.cu:948), accumulated in f64 in a fixed order.
 template <int S>
 __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat) {
+  typedef typename StageOf<S>::T T;
   __shared__ double red[kBlock];
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int item = work_item(wl, blk);
@@ -561,7 +632,7 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat)
     const double x = j + u0, y = i + u1;
     if (x < 0 || x > W - 1 || y < 0 || y > H - 1) {
       for (int k = 0; k < kFeat; k++) img_st<S>(v.Iw, (size_t)k * N + l, img_ld<S>(v.I1, (size_t)k * N + l));
-      gp(v.Dw)[l] = gp(v.D1)[l] - u2;
+      gp(v.Dw)[l] = (T)gp(v.D1)[l] - (T)u2;
     } else {
       const int x0 = clampi((int)floor(x), 0, W - 1), y0 = clampi((int)floor(y), 0, H - 1);
       const int x1 = min(W - 1, x0 + 1), y1 = min(H - 1, y0 + 1);
@@ -571,10 +642,11 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat)
       for (int k = 0; k < kFeat; k++) {
         const size_t kN = (size_t)k * N;
         img_st<S>(v.Iw, kN + l,
-                  w00 * img_ld<S>(v.I2, kN + o00) + w01 * img_ld<S>(v.I2, kN + o01) +
-                      w10 * img_ld<S>(v.I2, kN + o10) + w11 * img_ld<S>(v.I2, kN + o11));
+                  w00 * (T)img_ld<S>(v.I2, kN + o00) + w01 * (T)img_ld<S>(v.I2, kN + o01) +
+                      w10 * (T)img_ld<S>(v.I2, kN + o10) + w11 * (T)img_ld<S>(v.I2, kN + o11));
       }
-      gp(v.Dw)[l] = w00 * gp(v.D2)[o00] + w01 * gp(v.D2)[o01] + w10 * gp(v.D2)[o10] + w11 * gp(v.D2)[o11] - u2;
+      gp(v.Dw)[l] = w00 * (T)gp(v.D2)[o00] + w01 * (T)gp(v.D2)[o01] + w10 * (T)gp(v.D2)[o10] +
+                    w11 * (T)gp(v.D2)[o11] - (T)u2;
     }
     if (stat) {
       // sqrCU(du_k) * c_k in f32 (thrustSqrMultFunctor), c = (1, 1, f/D1)
@@ -728,11 +800,13 @@ void launch_derivs(hipStream_t s, const Views& v, int blocks, int n_slots, int i
   auto fn = img == kImgF16 ? k_derivs<kImgF16> : img == kImgBF16 ? k_derivs<kImgBF16> : k_derivs<kImgF64>;
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, s, v, n_slots);
 }
-void launch_phi(hipStream_t s, const Views& v, int blocks) {
-  hipLaunchKernelGGL(k_phi, dim3(blocks), dim3(kBlock), 0, s, v);
+void launch_phi(hipStream_t s, const Views& v, int blocks, int img) {
+  auto fn = img == kImgF16 ? k_phi<kImgF16> : img == kImgBF16 ? k_phi<kImgBF16> : k_phi<kImgF64>;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, s, v);
 }
-void launch_system(hipStream_t s, const Views& v, int blocks, const SolveScalars& k) {
-  hipLaunchKernelGGL(k_system, dim3(blocks), dim3(kBlock), 0, s, v, k);
+void launch_system(hipStream_t s, const Views& v, int blocks, const SolveScalars& k, int img) {
+  auto fn = img == kImgF16 ? k_system<kImgF16> : img == kImgBF16 ? k_system<kImgBF16> : k_system<kImgF64>;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, s, v, k);
 }
 void launch_outer_first(hipStream_t s, const Views& v, int blocks, int n_slots, const SolveScalars& k, int store,
                         int img) {

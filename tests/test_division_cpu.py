@@ -30,3 +30,15 @@ def test_division_by_constant_is_ieee(d):
     assert bad == 0, x
     bad, _ = oracle().check_div_const(200_000, 4 + d, d, recipe=1)
     assert bad > 1000
+
+
+@pytest.mark.parametrize("d", [12, 5, 3])
+def test_f32_division_by_constant_is_ieee(d):
+    """The f32 stage of the 16-bit image variants: the same recipe in f32 is
+    bit-equal to IEEE x / d for every significand of a binade (which the
+    power-of-two scaling carries to the whole domain [2^-100, 2^100]); the
+    bare product is not."""
+    bad, x = oracle().check_div_const_f32(d)
+    assert bad == 0, x
+    bad, _ = oracle().check_div_const_f32(d, recipe=1)
+    assert bad > 1000

@@ -155,6 +155,15 @@ def _pair_parity(ctx, R, name, seed, exits=True):
     assert sb.n_levels == L
     assert [sb.outer_exec[k] for k in range(L)] == [sa.outer_exec[k] for k in range(L)]
     assert [sb.sweeps_exec[k] for k in range(L)] == [sa.sweeps_exec[k] for k in range(L)]
+    # the outer early-exit statistic's closest approach to 4e-4 (the decisions above agree; the
+    # margin says how far an f32 summation, as the reference's, would have to move it)
+    for k in range(L):
+        m = sb.stat_margin[k]
+        if k < L - 1:   # the coarsest level has no early-exit test (sceneFlowMultiscale.cpp:175)
+            assert 0 <= m <= abs(sb.last_stat[k] - 4e-4) / 4e-4 + 1e-12
+        else:
+            assert m == -1
+    print(f"{name} seed {seed}: outer statistic margin per level {[round(sb.stat_margin[k], 4) for k in range(L)]}")
     du, dv, dw = _maxdiff(ua, ub), _maxdiff(va, vb), _maxdiff(wa, wb)
     assert du <= TOL_UV and dv <= TOL_UV and dw <= TOL_W, (du, dv, dw)
     # error against the FP64 oracle (reported; not a bound on the GPU path)
