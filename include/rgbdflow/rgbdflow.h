@@ -60,8 +60,11 @@ extern "C" {
  * fixed point.  BLOCK solves the pixel's three unknowns together from its
  * 3x3 block (x = B^-1 (rhs - neighbour terms), the inverse from the adjugate
  * in f32), then relaxes and clamps as the reference: block SOR, convergent
- * for any factor in (0, 2), to the same fixed point.  GS and BLOCK are not
- * in the reference. */
+ * for any factor in (0, 2), to the same fixed point.  BLOCK needs f32 system
+ * records (RGBDFLOW_ERR_UNSUPPORTED otherwise: rounded to 16 bits many of
+ * these nearly singular blocks are no longer definite); a pixel whose stored
+ * block is not negative definite with a finite inverse keeps its values in
+ * that pass.  GS and BLOCK are not in the reference. */
 #define RGBDFLOW_PIXEL_JACOBI 0
 #define RGBDFLOW_PIXEL_GS 1
 #define RGBDFLOW_PIXEL_BLOCK 2

@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <thread>
 #include <type_traits>
 #include <unordered_map>
@@ -1043,22 +1044,25 @@ struct Solver {
     assemble();
   }
 
-  /* .cu:382-470: one colour of the red-black relaxation; returns the largest
-   * |change| over the updated pixels (the check of .cu:914-918 splits exactly
-   * into the two colours' changes). */
   /* The inverse of a pixel's symmetric 3x3 block from its adjugate, in the
    * working precision and this order: cofactors, det = Buu c_uu + Buv c_uv +
    * Buw c_uw, 1 / det, then each cofactor times it.  inv: (uu, vv, ww, uv,
-   * uw, vw), the block's own order. */
-  static void block_inverse(CT Buu, CT Bvv, CT Bww, CT Buv, CT Buw, CT Bvw, CT* inv) {
+   * uw, vw), the block's own order.  Returns whether the block is negative
+   * definite with a finite inverse (Buu < 0, c_ww > 0, det < 0, 1 / det
+   * finite); a pixel whose block fails keeps its old values as x^. */
+  static bool block_inverse(CT Buu, CT Bvv, CT Bww, CT Buv, CT Buw, CT Bvw, CT* inv) {
     const CT cuu = Bvv * Bww - Bvw * Bvw, cvv = Buu * Bww - Buw * Buw, cww = Buu * Bvv - Buv * Buv;
     const CT cuv = Buw * Bvw - Buv * Bww, cuw = Buv * Bvw - Buw * Bvv, cvw = Buv * Buw - Buu * Bvw;
     const CT det = Buu * cuu + Buv * cuv + Buw * cuw;
     const CT id = (CT)1 / det;
     inv[0] = cuu * id; inv[1] = cvv * id; inv[2] = cww * id;
     inv[3] = cuv * id; inv[4] = cuw * id; inv[5] = cvw * id;
+    return Buu < 0 && cww > 0 && det < 0 && id >= -std::numeric_limits<CT>::max();
   }
 
+  /* .cu:382-470: one colour of the red-black relaxation; returns the largest
+   * |change| over the updated pixels (the check of .cu:914-918 splits exactly
+   * into the two colours' changes). */
   /* mode 0: the reference's order inside a pixel (each unknown from the
    * others' old values, .cu:431-459); 1: point SOR (u, then v from the new
    * u, then w from both); 2: the pixel's 3x3 block solved as a whole
@@ -1096,10 +1100,11 @@ struct Solver {
             a[k] = nv;
           }
           CT inv[6];
-          block_inverse(Buu[l], Bvv[l], Bww[l], Buv[l], Buw[l], Bvw[l], inv);
-          const CT xh[3] = {inv[0] * a[0] + inv[3] * a[1] + inv[4] * a[2],
-                            inv[3] * a[0] + inv[1] * a[1] + inv[5] * a[2],
-                            inv[4] * a[0] + inv[5] * a[1] + inv[2] * a[2]};
+          const bool ok = block_inverse(Buu[l], Bvv[l], Bww[l], Buv[l], Buw[l], Bvw[l], inv);
+          CT xh[3] = {inv[0] * a[0] + inv[3] * a[1] + inv[4] * a[2],
+                      inv[3] * a[0] + inv[1] * a[1] + inv[5] * a[2],
+                      inv[4] * a[0] + inv[5] * a[1] + inv[2] * a[2]};
+          if (!ok) for (int k = 0; k < 3; k++) xh[k] = xo[k];
           for (int k = 0; k < 3; k++) {
             CT r = omega * xh[k] + (1 - omega) * xo[k];
             if (r < xo[k] - step) r = xo[k] - step;

@@ -1064,6 +1064,15 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
   return RGBDFLOW_OK;
 }
 
+// The block update needs f32 records: the 3x3 blocks are negative definite
+// as assembled, but their condition reaches ~1e6 where the depth term's
+// weight is large, and rounded to 16 bits many are not definite any more.
+int check_block_storage(rgbdflow_ctx* c, int pixel_update, int storage) {
+  if (pixel_update == RGBDFLOW_PIXEL_BLOCK && storage != RGBDFLOW_STORE_F32)
+    return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "pixel_update BLOCK needs f32 system records");
+  return RGBDFLOW_OK;
+}
+
 int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
   if (!p) return fail(c, RGBDFLOW_ERR_ARG, "null params");
   if (p->variant != RGBDFLOW_VARIANT_FULL) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "bad scene flow variant");
@@ -1074,6 +1083,7 @@ int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
     return fail(c, RGBDFLOW_ERR_ARG, "bad inner_exit");
   if (p->pixel_update < RGBDFLOW_PIXEL_JACOBI || p->pixel_update > RGBDFLOW_PIXEL_BLOCK)
     return fail(c, RGBDFLOW_ERR_ARG, "bad pixel_update");
+  TRY(check_block_storage(c, p->pixel_update, p->storage));
   if (p->image_storage < RGBDFLOW_IMAGES_F64 || p->image_storage > RGBDFLOW_IMAGES_BF16)
     return fail(c, RGBDFLOW_ERR_ARG, "bad image_storage");
   return validate_iters(c, p->n_outer, p->n_inner, p->n_sor);
@@ -1351,6 +1361,7 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
     return fail(c, RGBDFLOW_ERR_ARG, "bad inner_exit");
   if (opts && (opts->pixel_update < RGBDFLOW_PIXEL_JACOBI || opts->pixel_update > RGBDFLOW_PIXEL_BLOCK))
     return fail(c, RGBDFLOW_ERR_ARG, "bad pixel_update");
+  if (opts) TRY(check_block_storage(c, opts->pixel_update, opts->storage));
   if (opts && opts->image_storage != RGBDFLOW_IMAGES_F64)
     return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "image_storage: the single-level entry points take f64 images");
   HIP_TRY(c, hipSetDevice(c->device));
@@ -1444,6 +1455,7 @@ int stage_relax(rgbdflow_ctx* c, const double* coef13, int W, int H, const doubl
   const int storage = sched ? sched->storage : RGBDFLOW_STORE_F32;
   if (storage < RGBDFLOW_STORE_F32 || storage > RGBDFLOW_STORE_BF16)
     return fail(c, RGBDFLOW_ERR_ARG, "bad system storage");
+  if (sched) TRY(check_block_storage(c, sched->pixel_update, storage));
   if (storage == RGBDFLOW_STORE_F32) {
     TRY(upload(c, wk.coef, rec.data(), rec.size() * 4));
   } else {

@@ -92,7 +92,7 @@ __device__ __forceinline__ void tiled_pass(float4* XA, float4* XB, int Si, int P
     float xn[3];
     if (MODE == kRelaxBlock) {
       const float inv[6] = {r.Buu, r.Bvv, r.Bww, r.Buv, r.Buw, r.Bvw};
-      relax_update_dense_block(c, inv, omega, omega_c, L, U, R, D, o, xn);
+      relax_update_dense_block(c, inv, (f & 128u) != 0, omega, omega_c, L, U, R, D, o, xn);
     } else if (MODE == kRelaxGS) {
       relax_update_dense_gs(c, omega, omega_c, y3, f & 64u, L, U, R, D, o, xn);
     } else {
@@ -153,7 +153,8 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
   int qi[P], par[P];
   const int np = PR * RH;
   // bits: 16 pixel valid; 32 interior; 64 diagonals in the reciprocal
-  // division's domain; 8-15 distance to the interior (Chebyshev, capped)
+  // division's domain; 128 (kRelaxBlock) the block passes block_inverse's
+  // test; 8-15 distance to the interior (Chebyshev, capped)
   unsigned fl[P][2];
 #pragma unroll
   for (int s = 0; s < P; s++) {
@@ -200,7 +201,7 @@ __device__ __forceinline__ void relax_tile(float4* XA, const Views& wl, int t, i
         if (MODE == kRelaxBlock) {
           // the block's inverse once per launch, in place of the block
           float inv[6];
-          block_inverse(r.Buu, r.Bvv, r.Bww, r.Buv, r.Buw, r.Bvw, inv);
+          if (block_inverse(r.Buu, r.Bvv, r.Bww, r.Buv, r.Buw, r.Bvw, inv)) f |= 128u;
           r.Buu = inv[0];
           r.Bvv = inv[1];
           r.Bww = inv[2];

@@ -101,7 +101,10 @@ __device__ __forceinline__ void relax_update(const PixCoef& c, float omega, floa
 // The inverse of a pixel's symmetric 3x3 block from its adjugate, in f32 and
 // this order: the cofactors, det = Buu c_uu + Buv c_uv + Buw c_uw, 1 / det
 // (IEEE), then each cofactor times it.  inv: (uu, vv, ww, uv, uw, vw).
-__device__ __forceinline__ void block_inverse(float Buu, float Bvv, float Bww, float Buv, float Buw, float Bvw,
+// Returns whether the stored block is negative definite with a finite
+// inverse (Buu < 0, c_ww > 0, det < 0, 1 / det finite): the assembled block
+// is, but rounding a nearly singular one can break it.
+__device__ __forceinline__ bool block_inverse(float Buu, float Bvv, float Bww, float Buv, float Buw, float Bvw,
                                               float (&inv)[6]) {
   const float cuu = Bvv * Bww - Bvw * Bvw, cvv = Buu * Bww - Buw * Buw, cww = Buu * Bvv - Buv * Buv;
   const float cuv = Buw * Bvw - Buv * Bww, cuw = Buv * Bvw - Buw * Bvv, cvw = Buv * Buw - Buu * Bvw;
@@ -113,16 +116,21 @@ __device__ __forceinline__ void block_inverse(float Buu, float Bvv, float Bww, f
   inv[3] = cuv * id;
   inv[4] = cuw * id;
   inv[5] = cvw * id;
+  return (Buu < 0.f) & (cww > 0.f) & (det < 0.f) & (id >= -0x1.fffffep127f);
 }
 
-// The block update's last half: x^ = B^-1 a, relaxed and clamped.
-__device__ __forceinline__ void block_step(const float (&inv)[6], const float (&a)[3], float omega, float omega_c,
-                                           const float xo[3], float xn[3]) {
-  const float xh[3] = {inv[0] * a[0] + inv[3] * a[1] + inv[4] * a[2],
-                       inv[3] * a[0] + inv[1] * a[1] + inv[5] * a[2],
-                       inv[4] * a[0] + inv[5] * a[1] + inv[2] * a[2]};
+// The block update's last half: x^ = B^-1 a (ok: block_inverse's test; a
+// pixel whose block fails it keeps x^ = its old values), relaxed and clamped.
+__device__ __forceinline__ void block_step(const float (&inv)[6], const float (&a)[3], bool ok, float omega,
+                                           float omega_c, const float xo[3], float xn[3]) {
+  float xh[3] = {inv[0] * a[0] + inv[3] * a[1] + inv[4] * a[2],
+                 inv[3] * a[0] + inv[1] * a[1] + inv[5] * a[2],
+                 inv[4] * a[0] + inv[5] * a[1] + inv[2] * a[2]};
 #pragma unroll
-  for (int k = 0; k < 3; k++) xn[k] = clamp_step(omega * xh[k] + omega_c * xo[k], xo[k]);
+  for (int k = 0; k < 3; k++) {
+    xh[k] = ok ? xh[k] : xo[k];
+    xn[k] = clamp_step(omega * xh[k] + omega_c * xo[k], xo[k]);
+  }
 }
 
 // Block SOR (not in the reference): the pixel's three unknowns solved together
@@ -145,14 +153,14 @@ __device__ __forceinline__ void relax_update_block(const PixCoef& c, float omega
     a[k] = nv + rhs[k];
   }
   float inv[6];
-  block_inverse(c.Buu, c.Bvv, c.Bww, c.Buv, c.Buw, c.Bvw, inv);
-  block_step(inv, a, omega, omega_c, xo, xn);
+  const bool ok = block_inverse(c.Buu, c.Bvv, c.Bww, c.Buv, c.Buw, c.Bvw, inv);
+  block_step(inv, a, ok, omega, omega_c, xo, xn);
 }
 
 // relax_update_block with every neighbour term evaluated (as
-// relax_update_dense; the same bits) and the block's inverse given.
-__device__ __forceinline__ void relax_update_dense_block(const PixCoef& c, const float (&inv)[6], float omega,
-                                                         float omega_c, const float4& L, const float4& U,
+// relax_update_dense; the same bits) and the block's inverse and test given.
+__device__ __forceinline__ void relax_update_dense_block(const PixCoef& c, const float (&inv)[6], bool ok,
+                                                         float omega, float omega_c, const float4& L, const float4& U,
                                                          const float4& R, const float4& D, const float4& o,
                                                          float xn[3]) {
   const float xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z}, xd[3] = {D.x, D.y, D.z};
@@ -168,7 +176,7 @@ __device__ __forceinline__ void relax_update_dense_block(const PixCoef& c, const
     nv -= (k < 2 ? c.dxy : c.dz) * xd[k];
     a[k] = nv + rhs[k];
   }
-  block_step(inv, a, omega, omega_c, xo, xn);
+  block_step(inv, a, ok, omega, omega_c, xo, xn);
 }
 
 // Markstein's division (IBM J. Res. Dev. 34(1), 1990): with y = RN(1/b),

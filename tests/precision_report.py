@@ -22,6 +22,11 @@ systems of each size, the iterations each variant needs to bring the
 relative residual ||r - M du|| / ||r|| to 1e-4 (the FP64 residual exit, at
 most 20000) and the energy it reaches.
 
+Then the served warm starts (four streams, one per size, FP16 records and
+images, warm starts of 4 outer iterations): each warm pair's served flow
+against the FP64 oracle of the configured work from the same estimate (15
+outer iterations) and coarse to fine (15 x 50), and against the truth.
+
 This script is the checker's view of the variants: it uses the oracle, so
 it lives with the tests, not with the product.
 """
@@ -68,16 +73,74 @@ def epe_stats(flows, refs):
                 dw_median=float(np.median(dw)), dw_p99=float(np.percentile(dw, 99)))
 
 
+def served_vs_fp64(R, O, warm_outer=4, n_frames=5, seed=900):
+    """Four served streams, one per size, FP16 records and images, boundary
+    maps made on the device, warm starts of warm_outer outer iterations: every
+    warm pair's served flow against the FP64 oracle of the same pair solved
+    as sceneFlowWithInit from the estimate the server used with the
+    configured 15 outer iterations (sceneFlowWithInit.cpp:74-80), and coarse
+    to fine at 15 x 50; the truth beside them."""
+    import rgbdflow_amd.synth as S
+    p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED, graph=True,
+                      storage=R.STORE_F16, image_storage=R.IMAGES_F16)
+    streams, flows = [], {}
+    with R.Server(p, max_batch=4, lanes=2, warm_start=True, warm_n_outer=warm_outer) as srv:
+        for k, (w, h) in enumerate(SIZES):
+            f = 525.0 * w / 640
+            seq = S.make_sequence(w, h, seed + k, n_frames, n_segments=6,
+                                  cam=S.Camera(w, h, f, f, (w - 1) / 2, (h - 1) / 2))
+            fr = [(x["img"], (x["depth"].astype(np.float64) * .001).astype(np.float32)) for x in seq["frames"]]
+            sid = srv.open(w, h, R.make_params(w, h).cam, w // 16)
+            streams.append((sid, w, h, seq, fr))
+        tickets = [(k, n, srv.submit(st[0], st[4][n][0], st[4][n][1])) for n in range(n_frames)
+                   for k, st in enumerate(streams)]
+        for (k, n, tk) in tickets:
+            info, flow = srv.poll(tk, wait_ms=120000)
+            if info["has_flow"]:
+                flows[(k, n)] = flow
+    served, w15, cold, truth = [], [], [], []
+    for k, (sid, w, h, seq, fr) in enumerate(streams):
+        for n in range(2, n_frames):
+            a, b = fr[n - 1], fr[n]
+            im1, im2 = a[0].astype(np.float64) / 255.0, b[0].astype(np.float64) / 255.0
+            d1, d2 = a[1].astype(np.float64), b[1].astype(np.float64)
+            sig = O.depth_stdevs(a[1], 4)
+            est = np.stack(flows[(k, n - 1)], axis=-1)
+            kw = dict(n_outer=15, n_sor=50, min_width=w // 16, precision=O.PREC_F64)
+            served.append(flows[(k, n)])
+            w15.append(O.with_init(im1, d1, im2, d2, sig, None, est, O.make_params(w, h, **kw))[:3])
+            cold.append(O.multiscale(im1, d1, im2, d2, sig, None, O.make_params(w, h, **kw))[:3])
+            gt = S.sequence_flow(seq, n - 1, n)
+            truth.append((gt[..., 0], gt[..., 1], gt[..., 2]))
+        print(f"precision: served {w}x{h} compared", flush=True)
+    return dict(workload=f"4 served streams ({', '.join(f'{w}x{h}' for w, h in SIZES)}), {n_frames} frames each, "
+                         f"FP16 records and images, device boundary maps, warm starts of {warm_outer} outer "
+                         f"iterations; {len(served)} warm pairs",
+                vs_fp64_warm_15_outer=epe_stats(served, w15), vs_fp64_cold_15x50=epe_stats(served, cold),
+                vs_truth=epe_stats(served, truth), fp64_warm_15_vs_truth=epe_stats(w15, truth),
+                fp64_cold_vs_truth=epe_stats(cold, truth))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--pairs", type=int, default=8)
     ap.add_argument("--out", default="")
+    ap.add_argument("--no-served", action="store_true", help="skip the served warm-start comparison")
+    ap.add_argument("--served-only", action="store_true", help="only the served warm-start comparison")
     ap.add_argument("--threads", type=int, default=0, help="oracle threads (0: the cores this process may use)")
     args = ap.parse_args()
     R = load_package()
     O = oracle()
     nt = args.threads or min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "64") or 64))
     O.set_threads(max(1, nt))
+    if args.served_only:
+        rep = served_vs_fp64(R, O)
+        txt = json.dumps(rep, indent=1)
+        print(txt)
+        if args.out:
+            with open(args.out, "w") as f:
+                f.write(txt + "\n")
+        return
     pairs = workload(R, args.pairs)
     print(f"precision: {len(pairs)} pairs " + ", ".join(f"{p['W']}x{p['H']}" for p in pairs), flush=True)
     base = dict(n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED, graph=True)
@@ -96,6 +159,8 @@ def main():
     results, flows = [], {}
     for rname, rule in rules.items():
         for sname, (st, img) in batch_variants.items():
+            if rule["pixel_update"] == R.PIXEL_BLOCK and st != R.STORE_F32:
+                continue      # the block update needs f32 records
             p = R.make_params(640, 480, storage=st, image_storage=img, **rule, **base)
             ctx.batch_load(p, pairs)
             ctx.batch_run(p, stats=True)
@@ -138,6 +203,8 @@ def main():
         x0 = np.zeros((3, h, w))
         for rname, rule in rules.items():
             for sname, st in stores.items():
+                if rule["pixel_update"] == R.PIXEL_BLOCK and st != R.STORE_F32:
+                    continue
                 sched = R.make_params(w, h, relax_variant=R.RELAX_TILED, storage=st, **rule)
                 _, it, rs, e = ctx.relax_residual(coef, 20000, 1e-4, x0, sched)
                 conv.append(dict(level=f"{w}x{h}", rule=rname, storage=sname, iterations=it,
@@ -149,9 +216,10 @@ def main():
                   d["storage"] == "f32"][0]
             c["energy_rel_to_" + ("sor" if rn == "point_sor_1.8" else rn) + "_f32"] = abs(c["energy"] - e0) / abs(e0)
     ctx.close()
+    served = None if args.no_served else served_vs_fp64(R, O)
     report = dict(workload=f"{len(pairs)} mixed-size pairs (" + ", ".join(f"{p['W']}x{p['H']}" for p in pairs) +
                   "), 5-level pyramids ratio 0.5, 15 outer x 50 inner, HPB regularisation",
-                  oracle_threads=nt, variants=results, convergence_to_residual_1e_4=conv)
+                  oracle_threads=nt, variants=results, convergence_to_residual_1e_4=conv, served_warm_starts=served)
     txt = json.dumps(report, indent=1)
     print(txt)
     if args.out:

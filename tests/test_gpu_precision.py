@@ -75,12 +75,16 @@ def test_record_storage_matches_oracle(ctx, R, storage):
     """16-bit system records: the device equals the oracle run with the same
     record rounding (FP16 with a per-pixel power-of-two scale, or BF16),
     bit for bit and with the same iterations, in both relaxation schedules;
-    with the point-SOR order too."""
+    with the point-SOR order too.  The block update refuses them."""
     O = oracle()
     W, H = 160, 120
     pair, im1, d1, im2, d2, sigma, hpb = _case(W, H, seed=63)
     st = R.STORE_F16 if storage == "f16" else R.STORE_BF16
-    for mode in (R.PIXEL_JACOBI, R.PIXEL_GS, R.PIXEL_BLOCK):
+    with pytest.raises(R.RgbdFlowError) as e:     # the block update needs f32 records
+        ctx.multiscale(R.make_params(W, H, storage=st, pixel_update=R.PIXEL_BLOCK, omega=1.8), im1, d1, im2, d2,
+                       sigma, hpb)
+    assert e.value.code == R.ERR_UNSUPPORTED
+    for mode in (R.PIXEL_JACOBI, R.PIXEL_GS):
         kw = dict(n_outer=4, n_sor=40, min_width=20, omega=0.7 if mode == R.PIXEL_JACOBI else 1.8)
         ua, va, wa, sa = O.multiscale(im1, d1, im2, d2, sigma, hpb,
                                       O.make_params(W, H, record_storage=st, pixel_gs=mode, **kw))
@@ -125,6 +129,23 @@ def test_relax_residual_matches_oracle(ctx, R, system, sched, mode, omega):
     assert ia == ib
     assert np.array_equal(xa, xb), float(np.abs(xa - xb).max())
     assert abs(ra - rb) <= 1e-10 * ra and abs(ea - eb) <= 1e-10 * abs(ea)
+
+
+@pytest.mark.parametrize("sched", list(SCHEDULES))
+def test_block_update_non_definite_pixels_match_oracle(ctx, R, system, sched):
+    """Pixels whose block is not negative definite (Buu made positive at every
+    37th pixel) keep their values in the block update, on the device as in
+    the oracle: bit for bit, with the same iterations."""
+    O = oracle()
+    c = system.copy()
+    c[4].ravel()[::37] = np.abs(c[4].ravel()[::37])
+    x0 = np.zeros((3, 72, 96))
+    xa, ia, ra, ea = O.relax_residual(c, 100, 0.0, x0, omega=1.8, pixel_gs=2)
+    xb, ib, rb, eb = ctx.relax_residual(c, 100, 0.0, x0, R.make_params(96, 72, omega=1.8, pixel_update=2,
+                                                                       **SCHEDULES[sched]))
+    assert ia == ib == 100
+    assert np.array_equal(xa, xb), float(np.abs(xa - xb).max())
+    assert np.isfinite(xb).all()
 
 
 @pytest.mark.parametrize("mode", [1, 2])

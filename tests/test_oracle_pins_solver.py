@@ -320,6 +320,32 @@ def test_block_update_solves_each_pixel_s_3x3_system():
     assert not np.allclose(gs.reshape(3, -1).T, xh, rtol=1e-6)
 
 
+def test_block_update_keeps_a_pixel_whose_block_is_not_definite():
+    """A pixel whose stored block is not negative definite (here Buu made
+    positive, or the block made singular) keeps its values in the block
+    update (x^ = its old values, so with factor 1 it is unchanged); its
+    neighbours still update from it."""
+    W, H = 8, 6
+    coef = _level_system_f64(45, W, H, scale=0.1)
+    rng = np.random.default_rng(45)
+    x0 = rng.normal(scale=0.005, size=(3, H, W))
+    bad = (2, 3)
+    c = coef.copy()
+    c[4][bad] = abs(c[4][bad])                       # Buu > 0
+    c2 = coef.copy()
+    c2[5][bad] = c2[7][bad] ** 2 / c2[4][bad]        # Buu Bvv = Buv^2: the 2x2 minor vanishes
+    for cc in (c, c2):
+        for prec in (O.PREC_F64, O.PREC_REF):
+            got, _, _, _ = O.relax_residual(cc, 1, 0.0, x0, omega=1.0, precision=prec, pixel_gs=2)
+            xo = x0[:, bad[0], bad[1]].astype(np.float32 if prec == O.PREC_REF else np.float64)
+            assert np.array_equal(got[:, bad[0], bad[1]], xo.astype(np.float64))
+            ref, _, _, _ = O.relax_residual(coef, 1, 0.0, x0, omega=1.0, precision=prec, pixel_gs=2)
+            assert not np.array_equal(ref[:, bad[0], bad[1]], got[:, bad[0], bad[1]])
+            far = np.ones((H, W), bool)
+            far[max(bad[0] - 2, 0):bad[0] + 3, max(bad[1] - 2, 0):bad[1] + 3] = False
+            assert np.array_equal(got[:, far], ref[:, far])  # pixels two away see no difference
+
+
 def test_block_update_reads_the_neighbours_red_then_black():
     """One block iteration with the couplings: red pixels solve
     B_p x_p = r_p - (E x0)_p (E: the couplings, assembled by scipy), black
