@@ -1060,6 +1060,12 @@ struct Solver {
     return Buu < 0 && cww > 0 && det < 0 && id >= -std::numeric_limits<CT>::max();
   }
 
+  /* .cu:463-467: the step limit, r kept within xo +- 0.3, as max then min:
+   * the reference's two comparisons for every r but a NaN (the reference's
+   * CHECK_FLOATS error), which this clamps into the band instead of passing
+   * on; the device's two-instruction form. */
+  static CT clamp_step(CT r, CT xo, CT step) { return std::fmin(std::fmax(r, xo - step), xo + step); }
+
   /* .cu:382-470: one colour of the red-black relaxation; returns the largest
    * |change| over the updated pixels (the check of .cu:914-918 splits exactly
    * into the two colours' changes). */
@@ -1107,8 +1113,7 @@ struct Solver {
           if (!ok) for (int k = 0; k < 3; k++) xh[k] = xo[k];
           for (int k = 0; k < 3; k++) {
             CT r = omega * xh[k] + (1 - omega) * xo[k];
-            if (r < xo[k] - step) r = xo[k] - step;
-            else if (r > xo[k] + step) r = xo[k] + step;
+            r = clamp_step(r, xo[k], step);
             xn[k] = r;
           }
         } else
@@ -1127,8 +1132,7 @@ struct Solver {
           else if (k == 1) { nv -= Buv[l] * x0; nv -= Bvw[l] * xo[2]; nv /= Bvv[l]; }
           else { nv -= Buw[l] * x0; nv -= Bvw[l] * x1; nv /= Bww[l]; }
           CT r = omega * nv + (1 - omega) * xo[k];
-          if (r < xo[k] - step) r = xo[k] - step;
-          else if (r > xo[k] + step) r = xo[k] + step;
+          r = clamp_step(r, xo[k], step);
           xn[k] = r;
         }
         for (int k = 0; k < 3; k++) {

@@ -50,12 +50,13 @@ __device__ __forceinline__ PixCoef load_coef(const float4* C, int storage, int W
   return c;
 }
 
-// The reference's step limit (.cu:463-467): x clamped to xo +- 0.3, as two
-// selects (branch-free; a NaN x passes through, as in the reference).
+// The reference's step limit (.cu:463-467): x clamped to xo +- 0.3, as max
+// then min (two instructions): the reference's two comparisons for every x
+// but a NaN -- the reference's CHECK_FLOATS error -- which this clamps into
+// the band (max returns the non-NaN operand) instead of passing on.
 __device__ __forceinline__ float clamp_step(float r, float xo) {
   const float lo = xo - .3f, hi = xo + .3f;
-  r = r < lo ? lo : r;
-  return r > hi ? hi : r;
+  return fminf(fmaxf(r, lo), hi);
 }
 
 // One pixel's update.  h* say which neighbours exist in the image; x* hold the
