@@ -362,7 +362,16 @@ typedef struct rgbdflow_tile_choice {
   int single;                    /* 1: one tile covers the level */
   int candidates;                /* schedules timed */
   double us_per_iter;            /* measured device time per red-black iteration */
+  int streamed;                  /* out: > 0 for a streamed workgroup shape (the tile swept through a ring
+                                    of rows; up to this many iterations per launch), 0 otherwise */
 } rgbdflow_tile_choice;
+
+/* Workgroup shape `index` of the tiled relaxation (the tile_config values):
+ * threads, pixel pairs per thread, and for a streamed shape (the tile swept
+ * top to bottom through a ring of region rows in LDS, one wave per ring-row
+ * pair) the most iterations per launch, else 0.  RGBDFLOW_ERR_ARG past the
+ * last shape.  No counterpart in the reference (which has one schedule). */
+int rgbdflow_relax_shape(int index, int* threads, int* pairs_per_thread, int* streamed);
 
 /* Times candidate schedules of the tiled relaxation (workgroup shape, tile
  * shape, iterations per launch) on `batch` W x H levels relaxed by one launch

@@ -37,7 +37,7 @@ EXPORTS = [
     "rgbdflow_batch_run", "rgbdflow_batch_read", "rgbdflow_set_relax_schedule", "rgbdflow_time_relax_schedule",
     "rgbdflow_server_create", "rgbdflow_server_destroy", "rgbdflow_server_last_error", "rgbdflow_stream_open",
     "rgbdflow_stream_submit", "rgbdflow_stream_poll", "rgbdflow_stream_close",
-    "rgbdflow_server_set_relax_schedule", "rgbdflow_fill_depth_holes_noalloc",
+    "rgbdflow_server_set_relax_schedule", "rgbdflow_fill_depth_holes_noalloc", "rgbdflow_relax_shape",
 ]
 
 
@@ -93,13 +93,25 @@ class RequestInfo(C.Structure):
 class TileChoice(C.Structure):
     _fields_ = [("tile_w", C.c_int), ("tile_h", C.c_int), ("tile_iters", C.c_int), ("tile_config", C.c_int),
                 ("threads", C.c_int), ("pairs_per_thread", C.c_int), ("single", C.c_int), ("candidates", C.c_int),
-                ("us_per_iter", C.c_double)]
+                ("us_per_iter", C.c_double), ("streamed", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 _lib = None
+
+
+def relax_shapes():
+    """Workgroup shapes of the tiled relaxation (tile_config index order):
+    dicts of threads, pairs_per_thread and streamed (the most iterations per
+    launch of a streamed shape, else 0)."""
+    out = []
+    while True:
+        t, p, s = C.c_int(), C.c_int(), C.c_int()
+        if lib().rgbdflow_relax_shape(len(out), C.byref(t), C.byref(p), C.byref(s)) != 0:
+            return out
+        out.append(dict(threads=t.value, pairs_per_thread=p.value, streamed=s.value))
 
 
 def lib():

@@ -414,6 +414,18 @@ void relax_event(rgbdflow_ctx* c) {
   hipEventRecord(c->relax_ev[c->relax_ev_used++], c->stream);
 }
 
+// Whether every tile of width TW (and its halo of 2T at cut edges) of a
+// W-wide level fits a streamed region row.
+bool stream_width_ok(int TW, int W, int T) {
+  if (TW >= W) return (W + 1) / 2 <= stream_pairs();
+  const int h = 2 * T;
+  for (int ox = 0; ox < W; ox += TW) {
+    const int rx0 = std::max(0, ox - h) & ~1, rx1 = std::min(W, std::min(ox + TW, W) + h);
+    if ((rx1 - rx0 + 1) / 2 > stream_pairs()) return false;
+  }
+  return true;
+}
+
 // Tile schedule of the tiled relaxation for a W x H level.  Explicit tile_*
 // parameters win; otherwise a cost model picks the workgroup shape, the tile
 // and the iterations per launch: a tile covering the whole level runs a whole
@@ -428,6 +440,35 @@ TileSched plan_tiles(const rgbdflow_ctx* c, int W, int H, int req_T, int req_w, 
     if (req_cfg >= 0 && cfg != req_cfg) continue;
     int nt, p;
     tiled_config_shape(cfg, &nt, &p);
+    const int tr = tiled_config_stream(cfg);
+    if (tr > 0) {
+      // streamed: a tile's height is free; its width plus the halo must fit
+      // a region row.  Cost ~ launches x (launch overhead + waves of tiles x
+      // steps per tile x time per step).
+      if (req_T > tr) continue;
+      const int T = req_T > 0 ? req_T : tr;
+      const int h = 2 * T;
+      const double step_us = 0.5 * ((tr * 2 + 1 + 3) / 4 * 0.142 + 0.05);
+      const int tw0 = req_w > 0 ? req_w + (req_w & 1) : 8, tw1 = req_w > 0 ? tw0 : W + 7;
+      for (int TW = tw0; TW <= tw1; TW += 2) {
+        if (req_w <= 0 && (TW & 7) && TW < W) continue;
+        if (!stream_width_ok(TW, W, T)) continue;
+        const int th0 = req_h > 0 ? req_h : 4, th1 = req_h > 0 ? req_h : H + 3;
+        for (int TH = th0; TH <= th1; TH += 4) {
+          const int rh = std::min(TH + 2 * h, H);
+          const int tiles = ((W + TW - 1) / TW) * ((H + TH - 1) / TH);
+          const double waves = std::ceil((double)tiles / c->cu_count);
+          const double per_wg = (rh + 4 * T - 1) * step_us;
+          const double launches = std::ceil((double)kCheckEvery / T) * 2;
+          const double cost = launches * (4.0 + waves * per_wg);
+          if (cost < best_cost) {
+            best_cost = cost;
+            best = TileSched{TW, TH, (W + TW - 1) / TW, (H + TH - 1) / TH, 0, stream_pairs(), cfg, T};
+          }
+        }
+      }
+      continue;
+    }
     const int cap = nt * p;
     const long long pairs_img = (long long)((W + 1) / 2) * H;
     if (pairs_img <= cap && req_w == 0) {
@@ -1914,6 +1955,16 @@ int schedule_from_choice(rgbdflow_ctx* c, int W, int H, const rgbdflow_tile_choi
   if (ch->tile_iters < 1 || ch->tile_iters > kCheckEvery) return fail(c, RGBDFLOW_ERR_ARG, "bad iterations per launch");
   int nt, pp;
   tiled_config_shape(ch->tile_config, &nt, &pp);
+  const int tr = tiled_config_stream(ch->tile_config);
+  if (tr > 0) {
+    const int TW = ch->tile_w, TH = ch->tile_h;
+    if (ch->single || ch->tile_iters > tr) return fail(c, RGBDFLOW_ERR_ARG, "bad iterations for a streamed schedule");
+    if (TW < 2 || (TW & 1) || TH < 1) return fail(c, RGBDFLOW_ERR_ARG, "bad tile shape");
+    if (!stream_width_ok(TW, W, ch->tile_iters))
+      return fail(c, RGBDFLOW_ERR_ARG, "tile and halo exceed a streamed region row");
+    *out = TileSched{TW, TH, (W + TW - 1) / TW, (H + TH - 1) / TH, 0, stream_pairs(), ch->tile_config, ch->tile_iters};
+    return RGBDFLOW_OK;
+  }
   const int cap = nt * pp;
   TileSched t{};
   if (ch->single) {
@@ -1941,11 +1992,22 @@ void choice_from_schedule(const TileSched& t, int candidates, double us, rgbdflo
   out->single = t.single;
   out->candidates = candidates;
   out->us_per_iter = us;
+  out->streamed = tiled_config_stream(t.config);
 }
 
 }  // namespace
 
 extern "C" {
+
+int rgbdflow_relax_shape(int index, int* threads, int* pairs_per_thread, int* streamed) {
+  if (index < 0 || index >= tiled_config_count()) return RGBDFLOW_ERR_ARG;
+  int nt, pp;
+  tiled_config_shape(index, &nt, &pp);
+  if (threads) *threads = nt;
+  if (pairs_per_thread) *pairs_per_thread = pp;
+  if (streamed) *streamed = tiled_config_stream(index);
+  return RGBDFLOW_OK;
+}
 
 int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, const rgbdflow_params* p,
                             rgbdflow_tile_choice* out) {
@@ -1962,6 +2024,34 @@ int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, const rgbd
   for (int cfg = 0; cfg < tiled_config_count(); cfg++) {
     int nt, pp;
     tiled_config_shape(cfg, &nt, &pp);
+    const int tr = tiled_config_stream(cfg);
+    if (tr > 0) {
+      // streamed: the fewest columns of tiles whose region rows fit, and the
+      // rows of tiles that give the batch about 1..4 rounds of the resident
+      // workgroups (or a few rows more or less)
+      const int T = tr;
+      int TW = 0;
+      for (int k = 1; k <= 64 && TW == 0; k++) {
+        const int tw = k == 1 ? W + (W & 1) : ((W + k - 1) / k + 1) & ~1;
+        if (stream_width_ok(tw, W, T)) TW = tw;
+      }
+      if (TW == 0) continue;
+      const int tx = (W + TW - 1) / TW;
+      // resident workgroups per CU (the grid of a launch with many tiles)
+      const int occ = std::max(1, tiled_prepare(cfg, p ? p->storage : RGBDFLOW_STORE_F32, kRelaxFixed,
+                                                stream_slots(T), 1 << 20) / c->cu_count);
+      std::vector<int> ths;
+      for (int rounds = 1; rounds <= 4; rounds++) {
+        const int rows = std::max(1, (rounds * c->cu_count * occ + batch * tx - 1) / (batch * tx));
+        for (int d = -1; d <= 1; d++) {
+          const int r = std::max(1, rows + d);
+          const int th = (H + r - 1) / r;
+          if (th >= 2 && std::find(ths.begin(), ths.end(), th) == ths.end()) ths.push_back(th);
+        }
+      }
+      for (int TH : ths) cand.push_back(TileSched{TW, TH, tx, (H + TH - 1) / TH, 0, stream_pairs(), cfg, T});
+      continue;
+    }
     const int cap = nt * pp;
     if (pairs_img <= cap) {
       const TileSched t{W + (W & 1), H, 1, 1, 1, cap, cfg, kCheckEvery};

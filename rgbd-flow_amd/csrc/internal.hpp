@@ -128,6 +128,7 @@ __device__ __forceinline__ RGBDFLOW_GLOBAL T* gp(T* p) {
 // classes whose copies do not take address-space-qualified operands; the
 // native vector types do).
 typedef float gvec4f __attribute__((ext_vector_type(4)));
+typedef float gvec3f __attribute__((ext_vector_type(3)));
 typedef unsigned gvec4u __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 gload(const float4* p) {
   const gvec4f x = *(const RGBDFLOW_GLOBAL gvec4f*)p;
@@ -287,11 +288,19 @@ void launch_residual(hipStream_t s, const Views& v, int blocks, int alt, int slo
 // kernels_relax_tiled.hip
 int tiled_config_count();
 void tiled_config_shape(int idx, int* nt, int* p);
+// 0 for a k_relax_tiled configuration; TR for a streamed one (k_relax_stream:
+// a ring of region rows for up to TR iterations per launch, stream_pairs()
+// pixel pairs per region row, any number of rows)
+int tiled_config_stream(int idx);
+int stream_slots(int TR);
+int stream_pairs();
 size_t tiled_lds_bytes(int slots);
 // LDS slots per colour plane for a tile schedule run T iterations per launch:
 // the largest region (tile + halo of 2T, clipped to the level, in pixel pairs)
-// padded by one pair on every side.
+// padded by one pair on every side; for a streamed schedule, its ring.
 inline int tiled_slots(const TileSched& t, int W, int H, int T) {
+  const int tr = tiled_config_stream(t.config);
+  if (tr > 0) return stream_slots(tr);
   int PR, RH;
   if (t.single) {
     PR = (W + 1) / 2;

@@ -280,6 +280,373 @@ k_relax_tiled(Views wl, int total, int flip, int n_iter, int slot_base, int n_pr
   }
 }
 
+// ---------------------------------------------------------------- streamed
+// The streamed schedule: the same colour passes over the same tiles, with the
+// tile swept from top to bottom through a ring of 4n + 2 region rows in LDS
+// (n: iterations of the launch) instead of being held whole.  Colour pass p
+// (0 .. 2n-1, colour p & 1) reaches region row r at step r + 2p, so at any
+// step the rows being updated all have the step's parity and read only rows
+// of the other parity, which no pass of that step writes: one barrier per
+// step.  Row r enters the ring at step r - 2 (its x values written to LDS) and
+// leaves after its last pass at step r + 4n - 2.  A tile's height is then not
+// bounded by LDS or registers, and the halo is paid on its perimeter only.
+//
+// One wave per ring-row pair: wave g (g < G = 2n + 1) owns ring rows g and
+// g + G (opposite parities) and runs every pass on them (one pixel pair per
+// lane: 64 pairs, 128 columns per row), with the pixels' blocks, rhs and
+// reciprocals in registers and their values and edge weights in the ring.
+// Right after one of its rows enters the ring (x values written to LDS, the
+// records decoded) a wave issues the HBM loads of its next row (G ring rows
+// on), which enters G steps later.
+constexpr int kStreamPairs = 64;                // pairs per region row (one wave)
+constexpr int kStreamPitch = kStreamPairs + 2;  // LDS slots per ring row: a zero pair each side
+
+template <int TR>
+struct StreamShape {
+  static constexpr int waves = 2 * TR + 1;
+  static constexpr int threads = 64 * waves;
+  static constexpr int rows = 4 * TR + 2;           // ring rows
+  static constexpr int plane = rows * kStreamPitch;  // slots per LDS plane
+};
+
+// What a lane keeps of one pixel across the passes: the 3x3 block (or its
+// inverse, kRelaxBlock), the rhs and the diagonal's reciprocals.
+struct StreamPix {
+  float Buu, Bvv, Bww, Buv, Buw, Bvw, ru, rv, rw, yu, yv, yw;
+};
+
+template <int ST>
+struct StreamRegs {
+  StreamPix pr[2][2];      // [row parity][colour]
+  unsigned fl[2][2];       // flags as in relax_tile
+  RawRecord<ST> raw[2];    // [colour] records of the next row, in flight
+  gvec3f xr[2];            // [colour] x values of the next row, in flight
+};
+
+// A tile's region and the ring rows the wave owns.
+struct StreamTile {
+  const float4* src;
+  float4* dst;
+  const float4* coef;
+  int W, H, N, ox, oy, ex, ey, rx0, ry0, PR, RH, n, NR, G;
+  int q[2], up[2], dn[2], e[2];  // per row parity: the wave's ring row, the ring rows above / below, red offset
+};
+
+// Issues the loads of region row rho (the lane's pixel pair: x values and
+// records).  The loads are unconditional (a missing pixel loads pixel 0 of the
+// level, and reads as zero when the row enters the ring): registers with a
+// load in flight are then never merged with other values, which would make
+// the wave wait for the load right away.
+template <int ST>
+__device__ __forceinline__ void stream_fetch(StreamRegs<ST>& s, const StreamTile& T, int rho, int kk) {
+  const int gy = T.ry0 + rho;
+  const int e = (gy + T.rx0) & 1;
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int gc = T.rx0 + 2 * kk + (c == 0 ? e : 1 - e);
+    const bool valid = rho < T.RH && kk < T.PR && gc < T.W;
+    const size_t l = valid ? (size_t)gy * T.W + gc : 0;
+    record_fetch<ST>(s.raw[c], T.coef, (size_t)T.N, l);
+    s.xr[c] = *(const RGBDFLOW_GLOBAL gvec3f*)(T.src + l);  // x, y, z (the fourth value is unused)
+  }
+}
+
+// Enters the fetched x values of region row rho into the wave's ring row of
+// parity PI (zeros for missing pixels and rows past the region).
+template <int TR, int PI, int ST>
+__device__ __forceinline__ void stream_enter(const StreamRegs<ST>& s, const StreamTile& T, float4* X, int rho,
+                                             int kk) {
+  constexpr int SI = StreamShape<TR>::plane;
+  const int slot = T.q[PI] * kStreamPitch + kk + 1;
+  const int e = T.e[PI];
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int gc = T.rx0 + 2 * kk + (c == 0 ? e : 1 - e);
+    const bool valid = rho < T.RH && kk < T.PR && gc < T.W;
+    // (.w: the row's decode writes its edge weight there)
+    X[c * SI + slot] = valid ? make_float4(s.xr[c].x, s.xr[c].y, s.xr[c].z, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// Decodes the records fetched for region row rho into register set PI and
+// writes the pixels' edge weights into the ring (zeros for missing pixels).
+// Every lane decodes (a missing pixel's registers hold pixel 0's values,
+// which its flags never let a pass use): registers assigned under a lane
+// condition would have to be merged with their old values.
+template <int TR, int PI, int ST, int MODE>
+__device__ __forceinline__ void stream_decode(StreamRegs<ST>& s, const StreamTile& T, float4* X, int rho, int kk) {
+  constexpr int SI = StreamShape<TR>::plane;
+  const int gy = T.ry0 + rho;
+  const int e = T.e[PI];
+  const int slot = T.q[PI] * kStreamPitch + kk + 1;
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int gc = T.rx0 + 2 * kk + (c == 0 ? e : 1 - e);
+    const bool valid = rho < T.RH && kk < T.PR && gc < T.W;
+    const int dy = max(max(T.oy - gy, gy - (T.ey - 1)), 0), dx = max(max(T.ox - gc, gc - (T.ex - 1)), 0);
+    float cv[kCoef], yv[3];
+    record_decode<ST>(s.raw[c], cv, yv);
+    // a missing neighbour's coefficient is exactly zero (its value is a zero pad)
+    const bool hr = valid && gc < T.W - 1, hd = valid && gy < T.H - 1;
+    StreamPix& r = s.pr[PI][c];
+    r.Buu = cv[kBuu];
+    r.Bvv = cv[kBvv];
+    r.Bww = cv[kBww];
+    r.Buv = cv[kBuv];
+    r.Buw = cv[kBuw];
+    r.Bvw = cv[kBvw];
+    r.ru = cv[kRu];
+    r.rv = cv[kRv];
+    r.rw = cv[kRw];
+    r.yu = yv[0];
+    r.yv = yv[1];
+    r.yw = yv[2];
+    unsigned f = 16u | ((unsigned)min(max(dx, dy), 255) << 8) | (dx == 0 && dy == 0 ? 32u : 0u);
+    if (recip_den_ok(r.Buu) && recip_den_ok(r.Bvv) && recip_den_ok(r.Bww)) f |= 64u;
+    if (MODE == kRelaxBlock) {
+      // the block's inverse once per row, in place of the block
+      float inv[6];
+      if (block_inverse(r.Buu, r.Bvv, r.Bww, r.Buv, r.Buw, r.Bvw, inv)) f |= 128u;
+      r.Buu = inv[0];
+      r.Bvv = inv[1];
+      r.Bww = inv[2];
+      r.Buv = inv[3];
+      r.Buw = inv[4];
+      r.Bvw = inv[5];
+    }
+    s.fl[PI][c] = valid ? f : 0u;
+    // planes: A colour 0, A colour 1, B colour 0, B colour 1; A = (x, y, z, eRxy), B = (eRz, eDxy, eDz, 0)
+    reinterpret_cast<float*>(X + c * SI + slot)[3] = hr ? cv[kERxy] : 0.f;
+    X[(2 + c) * SI + slot] = make_float4(hr ? cv[kERz] : 0.f, hd ? cv[kEDxy] : 0.f, hd ? cv[kEDz] : 0.f, 0.f);
+  }
+}
+
+// Colour pass p (colour C = p & 1) over the wave's ring row of parity PI,
+// region row rho.  dmax: pixels farther than dmax from the tile's interior
+// are left as they are (no value the interior ends with depends on them).
+// final_: the colour's last pass of the launch (write the interior; track:
+// record the change of the check iteration).
+template <int TR, int PI, int C, int ST, int MODE>
+__device__ __forceinline__ void stream_pass(const StreamRegs<ST>& s, const StreamTile& T, float4* X, int rho, int kk,
+                                            int dmax, bool final_, bool track, float om, float om_c, float& change) {
+  constexpr int SI = StreamShape<TR>::plane;
+  constexpr int O = 1 - C;  // the colour read
+  const unsigned f = s.fl[PI][C];
+  // every lane computes (no lane branch); a missing pixel or one outside the
+  // cone writes back its old values
+  const bool act = (f & 16u) && (int)((f >> 8) & 255u) <= dmax;
+  const int e = T.e[PI];
+  const int own = T.q[PI] * kStreamPitch + kk + 1;
+  const int up = T.up[PI] * kStreamPitch + kk + 1, dn = T.dn[PI] * kStreamPitch + kk + 1;
+  const int il = C == 0 ? own - 1 + e : own - e;
+  const int ir = C == 0 ? own + e : own + 1 - e;
+  const float4* Ao = X + O * SI;
+  const float4* Bo = X + (2 + O) * SI;
+  const float4 o = X[C * SI + own], OB = X[(2 + C) * SI + own];
+  const float4 L = Ao[il], LB = Bo[il], U = Ao[up], UB = Bo[up], R = Ao[ir], D = Ao[dn];
+  const StreamPix& r = s.pr[PI][C];
+  PixCoef c;
+  c.lxy = L.w;
+  c.lz = LB.x;
+  c.uxy = UB.y;
+  c.uz = UB.z;
+  c.rxy = o.w;
+  c.rz = OB.x;
+  c.dxy = OB.y;
+  c.dz = OB.z;
+  c.Buu = r.Buu;
+  c.Bvv = r.Bvv;
+  c.Bww = r.Bww;
+  c.Buv = r.Buv;
+  c.Buw = r.Buw;
+  c.Bvw = r.Bvw;
+  c.ru = r.ru;
+  c.rv = r.rv;
+  c.rw = r.rw;
+  const float y3[3] = {r.yu, r.yv, r.yw};
+  float xn[3];
+  if (MODE == kRelaxBlock) {
+    const float inv[6] = {r.Buu, r.Bvv, r.Bww, r.Buv, r.Buw, r.Bvw};
+    relax_update_dense_block(c, inv, (f & 128u) != 0, om, om_c, L, U, R, D, o, xn);
+  } else if (MODE == kRelaxGS) {
+    relax_update_dense_gs(c, om, om_c, y3, f & 64u, L, U, R, D, o, xn);
+  } else {
+    relax_update_dense(c, om, om_c, y3, f & 64u, L, U, R, D, o, xn);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) xn[k] = act ? xn[k] : (k == 0 ? o.x : k == 1 ? o.y : o.z);
+  X[C * SI + own] = make_float4(xn[0], xn[1], xn[2], o.w);
+  if (final_ && (f & 32u)) {
+    const int gy = T.ry0 + rho, gc = T.rx0 + 2 * kk + (C == 0 ? e : 1 - e);
+    gstore(T.dst + (size_t)gy * T.W + gc, make_float4(xn[0], xn[1], xn[2], 0.f));
+    if (track) change = fmaxf(change, fmaxf(fmaxf(fabsf(xn[0] - o.x), fabsf(xn[1] - o.y)), fabsf(xn[2] - o.z)));
+  }
+}
+
+// The wave's work at one step of parity PI: ph counts the steps of that
+// parity since its ring row took region row rho (0: the row enters the ring
+// -- its x values and edge weights are written to LDS and its records decoded
+// into registers, two steps before its first pass; p + 1: colour pass p).
+// Returns the region row whose loads the wave issues next (after a row
+// enters, the wave's ring row of the other parity takes region row rho + G
+// next), else -1.
+template <int TR, int PI, int ST, int MODE>
+__device__ __forceinline__ int stream_step(StreamRegs<ST>& s, const StreamTile& T, float4* X, int rho, int ph,
+                                           int kk, bool track, float om, float om_c, float& change) {
+  // lane-dependent addresses and tests are recomputed at every step instead
+  // of being held in registers across the loop (the records need them)
+  asm volatile("" : "+v"(kk));
+  // independent parts (no early exits): the registers of the row are then
+  // either left alone or replaced, never merged between paths
+  const int p = ph - 1;
+  const bool in = rho >= 0 && rho < T.RH;
+  if (rho >= 0 && ph == 0) stream_enter<TR, PI, ST>(s, T, X, rho, kk);
+  if (in && ph == 0) stream_decode<TR, PI, ST, MODE>(s, T, X, rho, kk);
+  const int gy = T.ry0 + rho;
+  const int dy = max(max(T.oy - gy, gy - (T.ey - 1)), 0);
+  const int dmax = 2 * T.n - 1 - p;
+  if (in && p >= 0 && dy <= dmax) {
+    const bool final_ = p >= 2 * T.n - 2;
+    if (p & 1)
+      stream_pass<TR, PI, 1, ST, MODE>(s, T, X, rho, kk, dmax, final_, track, om, om_c, change);
+    else
+      stream_pass<TR, PI, 0, ST, MODE>(s, T, X, rho, kk, dmax, final_, track, om, om_c, change);
+  }
+  return rho >= 0 && ph == 0 ? rho + T.G : -1;
+}
+
+// Barrier for the ring: LDS writes before it are seen by every wave after it.
+// Global loads in flight are not waited for (the barrier does not need them),
+// so the prefetched rows keep loading across steps.
+__device__ __forceinline__ void ring_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int TR, int ST, int MODE>
+__device__ __forceinline__ void stream_tile(float4* X, const Views& wl, int t, int flip, int n_iter, int slot_base,
+                                            int n_prev_checks, int check_slot) {
+  constexpr int SI = StreamShape<TR>::plane;
+  const int item = work_item(wl, t);
+  const LevelView& v = wl.items[item];
+  if (v.ctl->outer_done) return;
+  if (relax_stopped(v.ctl, slot_base, n_prev_checks)) return;
+  const TileSched ts = v.ts;
+  StreamTile T;
+  T.src = flip ? v.du_alt : v.du;
+  T.dst = flip ? v.du : v.du_alt;
+  T.coef = v.coef;
+  T.W = v.W;
+  T.H = v.H;
+  T.N = v.N;
+  const int tb = t - wl.start[item];
+  const int bx = tb % ts.tiles_x, by = tb / ts.tiles_x;
+  T.ox = bx * ts.TW;
+  T.oy = by * ts.TH;
+  T.ex = min(T.ox + ts.TW, T.W);
+  T.ey = min(T.oy + ts.TH, T.H);
+  const int n = min(n_iter, TR);
+  const int h = 2 * n;
+  T.rx0 = max(0, T.ox - h) & ~1;
+  T.ry0 = max(0, T.oy - h);
+  T.PR = (min(T.W, T.ex + h) - T.rx0 + 1) >> 1;
+  T.RH = min(T.H, T.ey + h) - T.ry0;
+  T.n = n;
+  // (a row's loads are issued at the end of the loop iteration in which the
+  // wave's row before it entered, at most one step after; it enters G >= 3
+  // steps after that row)
+  T.G = 2 * n + 1;
+  T.NR = 2 * T.G;
+  // the wave index is uniform across the wave: kept in scalar registers, as
+  // is everything derived from it (ring rows, steps, the row of each step)
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), kk = threadIdx.x & 63;
+#pragma unroll
+  for (int pi = 0; pi < 2; pi++) {
+    const int q = (g & 1) == pi ? g : g + T.G;
+    T.q[pi] = q;
+    T.up[pi] = q == 0 ? T.NR - 1 : q - 1;
+    T.dn[pi] = q == T.NR - 1 ? 0 : q + 1;
+    T.e[pi] = (T.ry0 + T.rx0 + pi) & 1;  // ring rows of parity pi hold region rows of parity pi
+  }
+  // ring row NR - 1 stands for region row -1 until region row NR - 1 enters: zeros
+  for (int i = threadIdx.x; i < 4 * kStreamPairs; i += StreamShape<TR>::threads)
+    X[(i >> 6) * SI + (T.NR - 1) * kStreamPitch + (i & 63) + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+  constexpr bool FIXED = MODE == kRelaxFixed;
+  const float om = FIXED ? .7f : v.omega, om_c = FIXED ? 1.f - .7f : v.omega_c;
+  const bool track = check_slot >= 0;
+  float change = 0.f;
+  StreamRegs<ST> s;
+  const bool active = g < T.G;
+  // each wave's first row (ring row g) is fetched here; the later ones when
+  // the wave decodes the row before
+  if (active) stream_fetch<ST>(s, T, g, kk);
+  // per parity: the region row the wave's ring row holds and the steps of
+  // that parity since it entered (the first steps are -2 and -1; ring row q
+  // first enters at step q - 2)
+  int rho[2], ph[2];
+#pragma unroll
+  for (int pi = 0; pi < 2; pi++) {
+    const int u = pi - T.q[pi];
+    if (u >= 0) {
+      rho[pi] = T.q[pi];
+      ph[pi] = u >> 1;
+    } else {
+      rho[pi] = T.q[pi] - T.NR;
+      ph[pi] = (u + T.NR) >> 1;
+    }
+  }
+  const int s_end = T.RH + 4 * n - 3;  // the last pass of the last row
+  for (int S = -2; S <= s_end; S += 2) {
+    // at most one row is decoded per two steps (a wave's rows are G >= 3
+    // steps apart), so the loads of the next row are issued from one place:
+    // after both steps
+    int next = -1;
+    ring_barrier();
+    if (active) next = stream_step<TR, 0, ST, MODE>(s, T, X, rho[0], ph[0], kk, track, om, om_c, change);
+    ring_barrier();
+    if (active) next = max(next, stream_step<TR, 1, ST, MODE>(s, T, X, rho[1], ph[1], kk, track, om, om_c, change));
+    if (next >= 0) stream_fetch<ST>(s, T, next, kk);
+#pragma unroll
+    for (int pi = 0; pi < 2; pi++)
+      if (++ph[pi] == T.G) {
+        ph[pi] = 0;
+        rho[pi] += T.NR;
+      }
+  }
+  if (track) {
+    const float m = wave_max(change);
+    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(&v.ctl->maxdiff[check_slot], __float_as_uint(m));
+  }
+  if (tb == 0 && threadIdx.x == 0) {
+    v.ctl->sweeps_exec += n;
+    v.ctl->du_in_alt = flip ? 0 : 1;
+  }
+}
+
+// A persistent streamed launch (MB workgroups of 2 TR + 1 waves per CU, as
+// registers must allow), tiles assigned to XCDs as in k_relax_tiled.
+template <int TR, int MB, int ST, int MODE>
+__global__ void __launch_bounds__(StreamShape<TR>::threads)
+    __attribute__((amdgpu_waves_per_eu((StreamShape<TR>::waves * MB + 3) / 4)))
+    k_relax_stream(Views wl, int total, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot) {
+  extern __shared__ float4 X4[];
+  constexpr int SI = StreamShape<TR>::plane;
+  // the pad pairs at both ends of every ring row stay zero
+  for (int i = threadIdx.x; i < 4 * StreamShape<TR>::rows * 2; i += StreamShape<TR>::threads) {
+    const int plane = i / (2 * StreamShape<TR>::rows), r = (i >> 1) % StreamShape<TR>::rows;
+    X4[plane * SI + r * kStreamPitch + ((i & 1) ? kStreamPitch - 1 : 0)] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  const int per_xcd = (total + 7) / 8;
+  for (int b = blockIdx.x; b < 8 * per_xcd; b += gridDim.x) {
+    const int t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= total) continue;
+    __syncthreads();  // the previous tile's last reads of the ring
+    stream_tile<TR, ST, MODE>(X4, wl, t, flip, n_iter, slot_base, n_prev_checks, check_slot);
+  }
+}
+
 // Copies du_alt into du for every item whose last tiled launch wrote du_alt.
 __global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -294,6 +661,7 @@ __global__ void __launch_bounds__(kBlock) k_copy_du(Views wl) {
 typedef void (*TiledFn)(Views, int, int, int, int, int, int);
 struct TiledConfig {
   int nt, p;
+  int stream;                  // 0: k_relax_tiled; TR: k_relax_stream with a ring for TR iterations
   TiledFn fn[3][kRelaxModes];  // per system storage (records.hpp) and RelaxMode
 };
 
@@ -305,17 +673,30 @@ struct TiledConfig {
    k_relax_tiled<NT, P, MB, ST, kRelaxGS>, k_relax_tiled<NT, P, MB, ST, kRelaxBlock>}
 #define RGBDFLOW_TILED(NT, P, MB)                                                                    \
   TiledConfig {                                                                                      \
-    NT, P, {                                                                                         \
+    NT, P, 0, {                                                                                      \
       RGBDFLOW_TILED_ST(NT, P, MB, kStoreF32), RGBDFLOW_TILED_ST(NT, P, MB, kStoreF16),              \
           RGBDFLOW_TILED_ST(NT, P, MB, kStoreBF16)                                                   \
     }                                                                                                \
   }
+#define RGBDFLOW_STREAM_ST(TR, MB, ST) \
+  {k_relax_stream<TR, MB, ST, kRelaxFree>, k_relax_stream<TR, MB, ST, kRelaxFixed>, \
+   k_relax_stream<TR, MB, ST, kRelaxGS>, k_relax_stream<TR, MB, ST, kRelaxBlock>}
+#define RGBDFLOW_STREAM(TR, MB)                                                                     \
+  TiledConfig {                                                                                     \
+    StreamShape<TR>::threads, 1, TR, {                                                              \
+      RGBDFLOW_STREAM_ST(TR, MB, kStoreF32), RGBDFLOW_STREAM_ST(TR, MB, kStoreF16),                 \
+          RGBDFLOW_STREAM_ST(TR, MB, kStoreBF16)                                                    \
+    }                                                                                               \
+  }
 const TiledConfig kConfigs[] = {RGBDFLOW_TILED(256, 1, 1), RGBDFLOW_TILED(512, 1, 1), RGBDFLOW_TILED(1024, 1, 1),
                                 RGBDFLOW_TILED(256, 2, 1), RGBDFLOW_TILED(512, 2, 1), RGBDFLOW_TILED(1024, 1, 2),
                                 RGBDFLOW_TILED(512, 1, 4), RGBDFLOW_TILED(512, 2, 2), RGBDFLOW_TILED(1024, 2, 1),
-                                RGBDFLOW_TILED(256, 2, 4)};
+                                RGBDFLOW_TILED(256, 2, 4), RGBDFLOW_STREAM(3, 1), RGBDFLOW_STREAM(4, 1),
+                                RGBDFLOW_STREAM(5, 1), RGBDFLOW_STREAM(3, 2), RGBDFLOW_STREAM(2, 3)};
 #undef RGBDFLOW_TILED
 #undef RGBDFLOW_TILED_ST
+#undef RGBDFLOW_STREAM
+#undef RGBDFLOW_STREAM_ST
 
 }  // namespace
 
@@ -324,6 +705,9 @@ void tiled_config_shape(int idx, int* nt, int* p) {
   *nt = kConfigs[idx].nt;
   *p = kConfigs[idx].p;
 }
+int tiled_config_stream(int idx) { return kConfigs[idx].stream; }
+int stream_slots(int TR) { return (4 * TR + 2) * kStreamPitch; }
+int stream_pairs() { return kStreamPairs; }
 
 size_t tiled_lds_bytes(int slots) { return (size_t)4 * slots * sizeof(float4); }
 

@@ -87,33 +87,75 @@ __device__ __forceinline__ void store_record(float4* base, int storage, size_t N
                                h[12] | (unsigned)h[13] << 16, h[14] | (unsigned)h[15] << 16));
 }
 
-// Reads pixel l's record (N: pixels of the level): the 13 values and the
-// reciprocals of the diagonal.
-__device__ __forceinline__ void load_record(const float4* base, int storage, size_t N, size_t l, float (&c)[kCoef],
-                                            float (&y)[3]) {
-  if (storage == kStoreF32) {
-    const float4 r0 = gload(base + l), r1 = gload(base + N + l), r2 = gload(base + 2 * N + l),
-                 r3 = gload(base + 3 * N + l);
+// A record as it comes from memory, before decoding: the four f32 part
+// planes, or the two 16-byte parts of a 16-bit record.  Fetching and decoding
+// are separate steps so that a kernel can issue the loads well before it
+// needs the values.
+template <int ST>
+struct RawRecord {
+  uint4 q[2];
+};
+template <>
+struct RawRecord<kStoreF32> {
+  float4 q[4];
+};
+
+template <int ST>
+__device__ __forceinline__ void record_fetch(RawRecord<ST>& r, const float4* base, size_t N, size_t l) {
+  if constexpr (ST == kStoreF32) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) r.q[i] = gload(base + i * N + l);
+  } else {
+    const uint4* R = reinterpret_cast<const uint4*>(base);
+    r.q[0] = gload(R + l);
+    r.q[1] = gload(R + N + l);
+  }
+}
+
+// The 13 values and the reciprocals of the diagonal of a fetched record.
+template <int ST>
+__device__ __forceinline__ void record_decode(const RawRecord<ST>& r, float (&c)[kCoef], float (&y)[3]) {
+  if constexpr (ST == kStoreF32) {
+    const float4 r0 = r.q[0], r1 = r.q[1], r2 = r.q[2], r3 = r.q[3];
     c[0] = r0.x; c[1] = r0.y; c[2] = r0.z; c[3] = r0.w;
     c[4] = r1.x; c[5] = r1.y; c[6] = r1.z; c[7] = r1.w;
     c[8] = r2.x; c[9] = r2.y; c[10] = r2.z; c[11] = r2.w;
     c[12] = r3.x;
     y[0] = r3.y; y[1] = r3.z; y[2] = r3.w;
-    return;
-  }
-  const uint4* R = reinterpret_cast<const uint4*>(base);
-  const uint4 a = gload(R + l), b = gload(R + N + l);
-  const unsigned w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-  if (storage == kStoreF16) {
-    const int e = (short)(unsigned short)(w[6] >> 16);
-    const float s = pow2f(e);
-    for (int k = 0; k < kCoef; k++) c[k] = f16_bits_to_f32((unsigned short)(w[k >> 1] >> ((k & 1) * 16))) * s;
   } else {
-    for (int k = 0; k < kCoef; k++) c[k] = bf16_to_f32((unsigned short)(w[k >> 1] >> ((k & 1) * 16)));
+    const unsigned w[8] = {r.q[0].x, r.q[0].y, r.q[0].z, r.q[0].w, r.q[1].x, r.q[1].y, r.q[1].z, r.q[1].w};
+    if constexpr (ST == kStoreF16) {
+      const int e = (short)(unsigned short)(w[6] >> 16);
+      const float s = pow2f(e);
+#pragma unroll
+      for (int k = 0; k < kCoef; k++) c[k] = f16_bits_to_f32((unsigned short)(w[k >> 1] >> ((k & 1) * 16))) * s;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kCoef; k++) c[k] = bf16_to_f32((unsigned short)(w[k >> 1] >> ((k & 1) * 16)));
+    }
+    y[0] = 1.0f / c[kBuu];
+    y[1] = 1.0f / c[kBvv];
+    y[2] = 1.0f / c[kBww];
   }
-  y[0] = 1.0f / c[kBuu];
-  y[1] = 1.0f / c[kBvv];
-  y[2] = 1.0f / c[kBww];
+}
+
+// Reads pixel l's record (N: pixels of the level): the 13 values and the
+// reciprocals of the diagonal.
+__device__ __forceinline__ void load_record(const float4* base, int storage, size_t N, size_t l, float (&c)[kCoef],
+                                            float (&y)[3]) {
+  if (storage == kStoreF32) {
+    RawRecord<kStoreF32> r;
+    record_fetch<kStoreF32>(r, base, N, l);
+    record_decode<kStoreF32>(r, c, y);
+  } else if (storage == kStoreF16) {
+    RawRecord<kStoreF16> r;
+    record_fetch<kStoreF16>(r, base, N, l);
+    record_decode<kStoreF16>(r, c, y);
+  } else {
+    RawRecord<kStoreBF16> r;
+    record_fetch<kStoreBF16>(r, base, N, l);
+    record_decode<kStoreBF16>(r, c, y);
+  }
 }
 
 }  // namespace rgbdflow

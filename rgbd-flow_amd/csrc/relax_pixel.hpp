@@ -194,6 +194,20 @@ __device__ __forceinline__ bool recip_den_ok(float b) {
   const float ab = fabsf(b);
   return ab >= 0x1p-60f && ab <= 0x1p60f;
 }
+// recip_div_ok of three numerators at once, with the diagonals' test: a zero
+// numerator is mapped into the range, then the smallest and largest bit
+// patterns are tested (vector arithmetic; one lane condition in the end).
+__device__ __forceinline__ bool recip_div3_ok(const float (&a)[3], bool den_ok) {
+  unsigned lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const unsigned ua = __float_as_uint(a[k]) & 0x7fffffffu;
+    const unsigned z = ua == 0u ? 0x21800000u : ua;
+    lo = min(lo, z);
+    hi = max(hi, z);
+  }
+  return den_ok & (lo >= 0x21800000u) & (hi <= 0x5d800000u);
+}
 __device__ __forceinline__ float recip_div(float a, float b, float y) {
   const float q0 = a * y;
   const float r = __builtin_fmaf(-b, q0, a);
@@ -240,17 +254,19 @@ __device__ __forceinline__ void relax_update_dense(const PixCoef& c, float omega
     }
     nv[k] = a;
   }
-  if (__builtin_expect(den_ok & recip_div_ok(nv[0]) & recip_div_ok(nv[1]) & recip_div_ok(nv[2]), 1)) {
-    nv[0] = recip_div(nv[0], c.Buu, y3[0]);
-    nv[1] = recip_div(nv[1], c.Bvv, y3[1]);
-    nv[2] = recip_div(nv[2], c.Bww, y3[2]);
-  } else {
-    nv[0] /= c.Buu;
-    nv[1] /= c.Bvv;
-    nv[2] /= c.Bww;
+  // every lane divides by the reciprocal; only a wave with a lane outside
+  // its domain (rare) divides again, the IEEE way, for those lanes
+  const float den[3] = {c.Buu, c.Bvv, c.Bww};
+  float q[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) q[k] = recip_div(nv[k], den[k], y3[k]);
+  const bool ok = recip_div3_ok(nv, den_ok);
+  if (__builtin_expect(__any(!ok), 0)) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) q[k] = ok ? q[k] : nv[k] / den[k];
   }
 #pragma unroll
-  for (int k = 0; k < 3; k++) xn[k] = clamp_step(omega * nv[k] + omega_c * xo[k], xo[k]);
+  for (int k = 0; k < 3; k++) xn[k] = clamp_step(omega * q[k] + omega_c * xo[k], xo[k]);
 }
 
 // relax_update_dense in the point SOR order (relax_update with gs): each
@@ -289,7 +305,8 @@ __device__ __forceinline__ void relax_update_dense_gs(const PixCoef& c, float om
       a -= c.Bvw * xn[1];
     }
     float q = recip_div(a, den[k], y3[k]);
-    if (__builtin_expect(!(den_ok & recip_div_ok(a)), 0)) q = a / den[k];
+    const bool ok = den_ok & recip_div_ok(a);
+    if (__builtin_expect(__any(!ok), 0)) q = ok ? q : a / den[k];
     xn[k] = clamp_step(omega * q + omega_c * xo[k], xo[k]);
   }
 }
