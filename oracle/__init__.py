@@ -499,3 +499,17 @@ def check_div_const_f32(d, recipe=0):
     x = C.c_float()
     bad = f(d, recipe, C.byref(x))
     return int(bad), x.value
+
+
+def check_recip_short(bits, recipe=0):
+    """Every significand of `bits` significant bits of the binade [1, 2)
+    (both signs) through the kernels' refined reciprocal of a 16-bit record's
+    diagonal, from each y0 within one ulp of 1/c (divcheck.cpp); returns
+    (mismatches against IEEE 1.0f / c, first c)."""
+    L = lib()
+    f = L.orc_check_recip_short
+    f.restype = C.c_uint64
+    f.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_float)]
+    x = C.c_float()
+    bad = f(bits, recipe, C.byref(x))
+    return int(bad), x.value

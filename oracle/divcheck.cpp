@@ -157,3 +157,51 @@ uint64_t orc_check_div_const_f32(int d, int recipe, float* ex) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// The relaxation kernels take the reciprocals of a 16-bit record's decoded
+// diagonal (at most 11 significant bits: FP16 times a power of two; BF16: 8)
+// as y0 = v_rcp_f32(c), which is within one ulp of 1/c, refined by one Newton
+// step e = fma(-c, y0, 1), y = fma(e, y0, y0), instead of the IEEE division
+// 1.0f / c, for c normal with 1/c normal.  1/c then lies at least 2^-12 ulp
+// from a rounding midpoint (the significand ratio is 2^k / H with H an
+// integer below 2^11), farther than the refined value's error, so y is the
+// IEEE quotient.  Scaling c by a power of two scales every step exactly, so
+// each binade behaves as [1, 2): this checks every significand of `bits`
+// significant bits, both signs, with y0 each float within one ulp of the
+// IEEE quotient r (its neighbours and r itself), bit for bit against r.
+// recipe 1 returns y0 = the upper neighbour unrefined (a negative control);
+// recipe 2 refines a full 24-bit significand the same way (the method's
+// premise does not hold there).  Returns the mismatches; the first c is
+// stored in *ex.
+uint64_t orc_check_recip_short(int bits, int recipe, float* ex) {
+  uint64_t bad = 0;
+  const int sig = recipe == 2 ? 24 : bits;
+  const uint32_t step = 1u << (24 - sig);
+  for (uint32_t m = 0; m < (1u << 23); m += step)
+    for (uint32_t sgn = 0; sgn < 2; sgn++) {
+      const uint32_t cb = (sgn << 31) | (127u << 23) | m;
+      float c;
+      std::memcpy(&c, &cb, 4);
+      const float r = 1.0f / c;
+      const float cand[3] = {std::nextafter(r, -INFINITY), r, std::nextafter(r, INFINITY)};
+      for (int k = 0; k < 3; k++) {
+        const float y0 = cand[k];
+        float y;
+        if (recipe == 1) {
+          y = cand[2];
+        } else {
+          const float e = std::fmaf(-c, y0, 1.0f);
+          y = std::fmaf(e, y0, y0);
+        }
+        if (std::memcmp(&y, &r, 4) != 0) {
+          if (bad == 0 && ex) *ex = c;
+          bad++;
+        }
+      }
+    }
+  return bad;
+}
+
+}  // extern "C"

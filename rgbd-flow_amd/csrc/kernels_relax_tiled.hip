@@ -426,7 +426,7 @@ __device__ __forceinline__ void stream_decode(StreamRegs<ST>& s, const StreamTil
 // are left as they are (no value the interior ends with depends on them).
 // final_: the colour's last pass of the launch (write the interior; track:
 // record the change of the check iteration).
-template <int TR, int PI, int C, int ST, int MODE>
+template <int TR, int PI, int C, int ST, int MODE, bool WIDE>
 __device__ __forceinline__ void stream_pass(const StreamRegs<ST>& s, const StreamTile& T, float4* X, int rho, int kk,
                                             int dmax, bool final_, bool track, float om, float om_c, float& change) {
   constexpr int SI = StreamShape<TR>::plane;
@@ -444,6 +444,12 @@ __device__ __forceinline__ void stream_pass(const StreamRegs<ST>& s, const Strea
   const float4* Bo = X + (2 + O) * SI;
   const float4 o = X[C * SI + own], OB = X[(2 + C) * SI + own];
   const float4 L = Ao[il], LB = Bo[il], U = Ao[up], UB = Bo[up], R = Ao[ir], D = Ao[dn];
+  // WIDE: whole 16-byte slots are read (the unused values are marked as
+  // used): a wave's 16-byte reads at a 16-byte stride are free of bank
+  // conflicts, its 4-, 8- and 12-byte reads at that stride are not; it costs
+  // registers, which two workgroups per CU do not have
+  if constexpr (WIDE)
+    asm volatile("" ::"v"(U.w), "v"(R.w), "v"(D.w), "v"(OB.w), "v"(UB.x), "v"(UB.w), "v"(LB.y), "v"(LB.z), "v"(LB.w));
   const StreamPix& r = s.pr[PI][C];
   PixCoef c;
   c.lxy = L.w;
@@ -490,7 +496,7 @@ __device__ __forceinline__ void stream_pass(const StreamRegs<ST>& s, const Strea
 // Returns the region row whose loads the wave issues next (after a row
 // enters, the wave's ring row of the other parity takes region row rho + G
 // next), else -1.
-template <int TR, int PI, int ST, int MODE>
+template <int TR, int PI, int ST, int MODE, bool WIDE>
 __device__ __forceinline__ int stream_step(StreamRegs<ST>& s, const StreamTile& T, float4* X, int rho, int ph,
                                            int kk, bool track, float om, float om_c, float& change) {
   // lane-dependent addresses and tests are recomputed at every step instead
@@ -508,9 +514,9 @@ __device__ __forceinline__ int stream_step(StreamRegs<ST>& s, const StreamTile& 
   if (in && p >= 0 && dy <= dmax) {
     const bool final_ = p >= 2 * T.n - 2;
     if (p & 1)
-      stream_pass<TR, PI, 1, ST, MODE>(s, T, X, rho, kk, dmax, final_, track, om, om_c, change);
+      stream_pass<TR, PI, 1, ST, MODE, WIDE>(s, T, X, rho, kk, dmax, final_, track, om, om_c, change);
     else
-      stream_pass<TR, PI, 0, ST, MODE>(s, T, X, rho, kk, dmax, final_, track, om, om_c, change);
+      stream_pass<TR, PI, 0, ST, MODE, WIDE>(s, T, X, rho, kk, dmax, final_, track, om, om_c, change);
   }
   return rho >= 0 && ph == 0 ? rho + T.G : -1;
 }
@@ -524,7 +530,7 @@ __device__ __forceinline__ void ring_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int TR, int ST, int MODE>
+template <int TR, int ST, int MODE, bool WIDE>
 __device__ __forceinline__ void stream_tile(float4* X, const Views& wl, int t, int flip, int n_iter, int slot_base,
                                             int n_prev_checks, int check_slot) {
   constexpr int SI = StreamShape<TR>::plane;
@@ -604,9 +610,9 @@ __device__ __forceinline__ void stream_tile(float4* X, const Views& wl, int t, i
     // after both steps
     int next = -1;
     ring_barrier();
-    if (active) next = stream_step<TR, 0, ST, MODE>(s, T, X, rho[0], ph[0], kk, track, om, om_c, change);
+    if (active) next = stream_step<TR, 0, ST, MODE, WIDE>(s, T, X, rho[0], ph[0], kk, track, om, om_c, change);
     ring_barrier();
-    if (active) next = max(next, stream_step<TR, 1, ST, MODE>(s, T, X, rho[1], ph[1], kk, track, om, om_c, change));
+    if (active) next = max(next, stream_step<TR, 1, ST, MODE, WIDE>(s, T, X, rho[1], ph[1], kk, track, om, om_c, change));
     if (next >= 0) stream_fetch<ST>(s, T, next, kk);
 #pragma unroll
     for (int pi = 0; pi < 2; pi++)
@@ -643,7 +649,7 @@ __global__ void __launch_bounds__(StreamShape<TR>::threads)
     const int t = (b & 7) * per_xcd + (b >> 3);
     if (t >= total) continue;
     __syncthreads();  // the previous tile's last reads of the ring
-    stream_tile<TR, ST, MODE>(X4, wl, t, flip, n_iter, slot_base, n_prev_checks, check_slot);
+    stream_tile<TR, ST, MODE, MB == 1>(X4, wl, t, flip, n_iter, slot_base, n_prev_checks, check_slot);
   }
 }
 

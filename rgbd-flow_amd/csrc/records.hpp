@@ -87,6 +87,24 @@ __device__ __forceinline__ void store_record(float4* base, int storage, size_t N
                                h[12] | (unsigned)h[13] << 16, h[14] | (unsigned)h[15] << 16));
 }
 
+// 1.0f / c for a c of at most 11 significant bits (a 16-bit record's
+// decoded value): v_rcp_f32 (within one ulp of 1/c) refined by one Newton
+// step with FMAs.  1/c lies at least 2^-12 ulp from a rounding midpoint
+// (its significand ratio is 2^k / H, H an integer below 2^11), farther than
+// the refined value's error (under 2^-21 ulp), so the result is the IEEE
+// quotient; checked for every such significand and every start within one
+// ulp (oracle/divcheck.cpp).  It needs c and 1/c normal (exponent field 1 ..
+// 253); a wave with a lane outside that (zero, huge, infinite or NaN
+// diagonals: rare) divides again, the IEEE way, for those lanes.
+__device__ __forceinline__ float recip_short(float c) {
+  const float y0 = __builtin_amdgcn_rcpf(c);
+  float y = __builtin_fmaf(__builtin_fmaf(-c, y0, 1.0f), y0, y0);
+  const unsigned ex = (__float_as_uint(c) >> 23) & 255u;
+  const bool ok = ex - 1u <= 252u;
+  if (__builtin_expect(__any(!ok), 0)) y = ok ? y : 1.0f / c;
+  return y;
+}
+
 // A record as it comes from memory, before decoding: the four f32 part
 // planes, or the two 16-byte parts of a 16-bit record.  Fetching and decoding
 // are separate steps so that a kernel can issue the loads well before it
@@ -133,9 +151,9 @@ __device__ __forceinline__ void record_decode(const RawRecord<ST>& r, float (&c)
 #pragma unroll
       for (int k = 0; k < kCoef; k++) c[k] = bf16_to_f32((unsigned short)(w[k >> 1] >> ((k & 1) * 16)));
     }
-    y[0] = 1.0f / c[kBuu];
-    y[1] = 1.0f / c[kBvv];
-    y[2] = 1.0f / c[kBww];
+    y[0] = recip_short(c[kBuu]);
+    y[1] = recip_short(c[kBvv]);
+    y[2] = recip_short(c[kBww]);
   }
 }
 

@@ -42,3 +42,20 @@ def test_f32_division_by_constant_is_ieee(d):
     assert bad == 0, x
     bad, _ = oracle().check_div_const_f32(d, recipe=1)
     assert bad > 1000
+
+
+@pytest.mark.parametrize("bits", [11, 8])
+def test_refined_reciprocal_of_short_significands_is_ieee(bits):
+    """The kernels' reciprocals of a 16-bit record's decoded diagonal
+    (v_rcp_f32 within one ulp, then one Newton step with FMAs): for every
+    significand of 11 (FP16) or 8 (BF16) significant bits and every starting
+    value within one ulp, the IEEE quotient 1.0f / c bit for bit.  The
+    unrefined start (negative control) is caught, and so is the same
+    refinement of full 24-bit significands, where the method's premise (1/c
+    far from a rounding midpoint) does not hold."""
+    bad, c = oracle().check_recip_short(bits)
+    assert bad == 0, c
+    bad, _ = oracle().check_recip_short(bits, recipe=1)
+    assert bad > 100
+    bad, _ = oracle().check_recip_short(bits, recipe=2)
+    assert bad > 0
