@@ -660,7 +660,10 @@ int one_step(rgbdflow_ctx* c, const char* name, std::vector<LevelView>& items, c
 // RGBDFLOW_EXIT_RESIDUAL the check is the FP64 relative residual, evaluated
 // after the launch that ends at the check iteration (the relaxation launches
 // then record no change maxima).
-void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, int slot_base) {
+// copy_back: leave the result in du (a tiled schedule's last launch may have
+// written du_alt; SolveCtl::du_in_alt says which) -- not needed when the next
+// reader is k_update_warp, which reads whichever holds it.
+void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, int slot_base, bool copy_back = true) {
   hipStream_t s = c->stream;
   const bool resid = k.inner_exit == RGBDFLOW_EXIT_RESIDUAL;
   auto check_of = [&](int last) { return (last % kCheckEvery == kCheckEvery - 1) ? last / kCheckEvery : -1; };
@@ -687,7 +690,7 @@ void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, i
       it += T;
     }
     if (!resid && k.n_sor > 0) relax_event(c);
-    if (k.n_sor > 0) launch_copy_du(s, st.vpix, st.pix);
+    if (k.n_sor > 0 && copy_back) launch_copy_du(s, st.vpix, st.pix);
     return;
   }
   for (int it = 0; it < k.n_sor; it++) {
@@ -719,7 +722,7 @@ void enqueue_level_solve(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars
         launch_phi(s, st.vpix, st.pix, st.img);
         launch_system(s, st.vpix, st.pix, k, st.img);
       }
-      enqueue_relax(c, st, k, h * checks);
+      enqueue_relax(c, st, k, h * checks, h + 1 < k.n_inner);
     }
     launch_update_warp(s, st.vpix, st.pix, st.any_early ? 1 : 0, st.img);
     if (st.any_early) launch_stat_final(s, st.vpix, o, k.outer_exit);
@@ -947,7 +950,7 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
       q.dvc = p->depth_vs_color_data_weight;
       const double f0 = cam.focalLength;
       q.f2 = f0 * f0;
-      start[i + 1] = start[i] + pix_blocks(ps.W * ps.H);
+      start[i + 1] = start[i] + pair_prep_blocks(ps.W, ps.H);
     }
     bp.prep_items = pack(bp.host, items.data(), items.size() * sizeof(PairPrep));
     bp.prep_start = pack(bp.host, start.data(), start.size() * sizeof(int));
@@ -2177,7 +2180,7 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   const double f0 = p->cam.focalLength;
   pp.f2 = f0 * f0;
   std::vector<char> ph;
-  const int pstart[2] = {0, pix_blocks((int)N)};
+  const int pstart[2] = {0, pair_prep_blocks(W, H)};
   const size_t pio = pack(ph, &pp, sizeof(pp));
   const size_t pso = pack(ph, pstart, sizeof(pstart));
   char* pdev;

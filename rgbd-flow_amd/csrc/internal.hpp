@@ -261,6 +261,13 @@ struct SolveScalars {
 __host__ __device__ inline int pix_blocks(int N) { return (N + kBlock - 1) / kBlock; }
 __host__ __device__ inline int plain_blocks(int W, int H) { return (((W + 1) / 2 + 63) / 64) * ((H + 3) / 4); }
 __host__ __device__ inline int tiled_blocks(const TileSched& t) { return t.tiles_x * t.tiles_y; }
+// tiles of the per-pair preparation (kBlock threads, one pixel each) and the
+// radius of the depth stdevs' boundary search (staticDepthNoiseModeling.cpp's
+// maxDist, 4 in runFlowTwoFrames.cpp:90)
+constexpr int kPrepW = 32, kPrepH = kBlock / kPrepW, kPrepRadius = 4;
+__host__ __device__ inline int pair_prep_blocks(int W, int H) {
+  return ((W + kPrepW - 1) / kPrepW) * ((H + kPrepH - 1) / kPrepH);
+}
 // tiles of the fused first-inner kernel (kBlock threads, one pixel each)
 constexpr int kOtW = 32, kOtH = kBlock / kOtW;
 __host__ __device__ inline int otile_blocks(int W, int H) { return ((W + kOtW - 1) / kOtW) * ((H + kOtH - 1) / kOtH); }

@@ -300,19 +300,72 @@ __global__ void __launch_bounds__(kBlock) k_depth_sigma(const float* depth, int 
 // Per pair at the start of a solve: depth stdevs of frame 1 if not given
 // (runFlowTwoFrames.cpp:90, from the float depth), delta = stereoError(1) /
 // sigma * depthVsColorDataWeight (OpticalFlow.cpp:71-81), s = f^2 (:90-97).
+// One workgroup per kPrepW x kPrepH tile: the depth-boundary test of every
+// pixel within the search radius of the tile is made once, into LDS, and each
+// pixel searches its diamond there (the values and the minimum distance of
+// depth_sigma_at).
 __global__ void __launch_bounds__(kBlock) k_pair_prep(WorkList<PairPrep> wl) {
+  constexpr int R = kPrepRadius, BW = kPrepW + 2 * R, BH = kPrepH + 2 * R;
+  __shared__ float dz[(BH + 1) * (BW + 1)];  // the flags' region, one more column and row for their tests
+  __shared__ unsigned char bnd[BH * BW];
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int item = work_item(wl, blk);
   const PairPrep& p = wl.items[item];
-  const int l = (blk - wl.start[item]) * kBlock + threadIdx.x;
-  if (l >= p.W * p.H) return;
-  float sg;
+  const int W = p.W, H = p.H;
+  const int lb = blk - wl.start[item];
+  const int tx = (W + kPrepW - 1) / kPrepW;
+  const int x0 = (lb % tx) * kPrepW, y0 = (lb / tx) * kPrepH;
+  const int i = y0 + threadIdx.x / kPrepW, j = x0 + threadIdx.x % kPrepW;
+  const bool inside = i < H && j < W;
+  const int l = inside ? i * W + j : 0;
+  float sg = 0.f;
   if (p.compute_sigma) {
-    sg = depth_sigma_at(p.dm1, p.W, p.H, l, 4);
-    p.sigma[l] = sg;
-  } else {
+    for (int t = threadIdx.x; t < (BH + 1) * (BW + 1); t += kBlock) {
+      const int ri = y0 - R + t / (BW + 1), rj = x0 - R + t % (BW + 1);
+      dz[t] = ri >= 0 && rj >= 0 && ri < H && rj < W ? (float)p.dm1[(size_t)ri * W + rj] : 0.f;
+    }
+    __syncthreads();
+    // is_depth_boundary on the staged values
+    for (int t = threadIdx.x; t < BH * BW; t += kBlock) {
+      const int bi = t / BW, bj = t - bi * BW;
+      const int ri = y0 - R + bi, rj = x0 - R + bj;
+      bool b = false;
+      if (ri >= 0 && rj >= 0 && ri < H && rj < W) {
+        const float d = dz[bi * (BW + 1) + bj];
+        if (ri < H - 1) {
+          const float dn = dz[(bi + 1) * (BW + 1) + bj];
+          if (dn > 0 && fabsf(d - dn) > .02 * stereo_error_ratio(fminf(d, dn))) b = true;
+        }
+        if (rj < W - 1) {
+          const float dn = dz[bi * (BW + 1) + bj + 1];
+          if (dn > 0 && fabsf(d - dn) > .02 * stereo_error_ratio(fminf(d, dn))) b = true;
+        }
+      }
+      bnd[t] = b ? 1 : 0;
+    }
+    __syncthreads();
+    if (inside) {
+      const float sigmaAt1m = (float)stereo_error(1);
+      const float z = dz[(i - y0 + R) * (BW + 1) + (j - x0 + R)];
+      sg = z > 0 ? (float)fmax(1e-4, sigmaAt1m * stereo_error_ratio(z)) : 1.0f;
+      int best = R + 1;
+      for (int di = -R; di <= R; di++) {
+        const int rem = R - abs(di);
+        for (int dj = -rem; dj <= rem; dj++) {
+          const int dd = abs(di) + abs(dj);
+          if (dd < best && bnd[(i - y0 + R + di) * BW + (j - x0 + R + dj)]) best = dd;
+        }
+      }
+      if (best <= R) {
+        const float a = (float)best / R;
+        sg = (float)exp((1 - a) * log((double)(10 * sg)) + a * log((double)sg));
+      }
+      p.sigma[l] = sg;
+    }
+  } else if (inside) {
     sg = p.sigma[l];
   }
+  if (!inside) return;
   p.del0[l] = stereo_error(1) / sg * p.dvc;
   p.sm0[l] = p.f2;
 }

@@ -621,7 +621,8 @@ __global__ void __launch_bounds__(kBlock) k_update_warp(Views wl, int want_stat)
   double term = 0;
   if (l < N) {
     const int i = l / W, j = l - i * W;
-    const float4 e = gload(v.du + l);
+    // the relaxation's result: du, or du_alt if a tiled launch left it there
+    const float4 e = gload((v.ctl->du_in_alt ? v.du_alt : v.du) + l);
     const float e0 = e.x, e1 = e.y, e2 = e.z;
     const double u0 = gp(v.u)[l] + (double)e0;
     const double u1 = gp(v.u)[N + l] + (double)e1;
