@@ -126,6 +126,18 @@ typedef struct rgbdflow_params {
   int image_storage;             /* RGBDFLOW_IMAGES_*: storage of the feature images (pairs solved
                                     from frames; the single-level entry points, which take f64
                                     images, accept F64 only) */
+  /* Coarse-grid correction of the relaxation (not in the reference; 0 levels: off, the
+   * default).  After iteration coarse_at of every 25 red-black iterations, one V-cycle
+   * over up to coarse_levels levels of 2x2 aggregates of the level's system (a level is
+   * made while the finer one is at least 4x4): the finer level's residual summed into
+   * the coarse rhs, coarse_sweeps iterations of the same relaxation before and after the
+   * correction from the next coarser level, the coarse unknowns added to their 2x2 fine
+   * pixels.  Coarse edges are the mean of the fine edges crossing between aggregates;
+   * coarse blocks sum the fine data terms.  The relaxation's exits and step limit are
+   * unchanged. */
+  int coarse_levels;             /* 0..8 */
+  int coarse_sweeps;             /* >= 1 (default 2) */
+  int coarse_at;                 /* 0..23 (default 9) */
 } rgbdflow_params;
 
 typedef struct rgbdflow_stats {

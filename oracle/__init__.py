@@ -34,7 +34,8 @@ class Params(C.Structure):
                 ("n_inner", C.c_int), ("n_sor", C.c_int), ("cam", Camera), ("precision", C.c_int),
                 ("disable_inner_exit", C.c_int), ("disable_outer_exit", C.c_int), ("omega", C.c_float),
                 ("residual_exit", C.c_int), ("residual_tol", C.c_double), ("pixel_gs", C.c_int),
-                ("image_storage", C.c_int), ("record_storage", C.c_int)]
+                ("image_storage", C.c_int), ("record_storage", C.c_int), ("cg_levels", C.c_int),
+                ("cg_sweeps", C.c_int), ("cg_at", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -92,7 +93,8 @@ def _f32(a):
 def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=15, n_inner=1, n_sor=50,
                 ratio=0.5, min_width=None, reg=REG_HPB, precision=PREC_REF, cw=1.0, dvc=1.0,
                 disable_inner_exit=False, disable_outer_exit=False, omega=0.7, residual_exit=False,
-                residual_tol=1e-4, pixel_gs=False, image_storage=IMG_F64, record_storage=0) -> Params:
+                residual_tol=1e-4, pixel_gs=False, image_storage=IMG_F64, record_storage=0, cg_levels=0, cg_sweeps=2,
+                cg_at=9) -> Params:
     p = Params()
     p.regularization_type = reg
     p.color_data_weight = cw
@@ -115,6 +117,7 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.pixel_gs = int(pixel_gs)
     p.image_storage = image_storage
     p.record_storage = record_storage
+    p.cg_levels, p.cg_sweeps, p.cg_at = cg_levels, cg_sweeps, cg_at
     return p
 
 
@@ -397,6 +400,44 @@ def hpb_smallest_eigvec(a):
     n = np.zeros(3)
     lib().orc_hpb_smallest_eigvec(_p(a), _p(n))
     return n
+
+
+def cg_operator(coef13, precision=PREC_REF):
+    """The coarse-grid correction's coarse system of a compact system (rhs zero)."""
+    c = _f64(coef13)
+    _, H, W = c.shape
+    out = np.empty((13, (H + 1) // 2, (W + 1) // 2))
+    lib().orc_cg_operator(_p(c), W, H, precision, _p(out))
+    return out
+
+
+def cg_restrict(coef13, x, precision=PREC_REF):
+    """The coarse rhs: the residual at x ([3][H][W]) summed over 2x2 aggregates."""
+    c, xx = _f64(coef13), _f64(x)
+    _, H, W = c.shape
+    out = np.empty((3, (H + 1) // 2, (W + 1) // 2))
+    lib().orc_cg_restrict(_p(c), W, H, precision, _p(xx), _p(out))
+    return out
+
+
+def cg_prolong(x, xc, precision=PREC_REF):
+    """x + P xc."""
+    xx, cc = np.array(x, np.float64, copy=True), _f64(xc)
+    _, H, W = xx.shape
+    lib().orc_cg_prolong(_p(xx), W, H, precision, _p(cc))
+    return xx
+
+
+def relax_cg(coef13, n_iter, x, params: Params):
+    """n_iter relaxation iterations with params' coarse-grid correction; returns
+    (x, per-iteration max change, per-iteration f64 relative residual)."""
+    c = _f64(coef13)
+    _, H, W = c.shape
+    xx = np.array(x, np.float64, copy=True)
+    md = np.empty(n_iter)
+    rs = np.empty(n_iter)
+    lib().orc_relax_cg(_p(c), W, H, C.byref(params), n_iter, _p(xx), _p(md), _p(rs))
+    return xx, md, rs
 
 
 def system_residual(coef13, x):

@@ -1195,6 +1195,189 @@ struct Solver {
     *energy = 0.5 * ex;
   }
 
+  /* ---- coarse-grid correction: not in the reference (opt-in, cg_levels) ----
+   * Coarse level l+1 aggregates the 2x2 pixels (2I..2I+1, 2J..2J+1) of
+   * level l (one row or column where a size is odd): Wc = (W + 1) / 2.  P
+   * copies a coarse pixel's three unknowns to its fine pixels.  The coarse
+   * system has the fine system's form (5-point, a 3x3 block per pixel), so
+   * the relaxation runs on it unchanged:
+   *   g_right(I,J), g_down(I,J): the sums of the fine edge weights (xy and z
+   *     alike) that cross to the right / lower aggregate (P^T M P's edges);
+   *   edges: g / 2 -- the crossing weights' mean, the weight a smooth
+   *     field's coarse Laplacian has (P^T M P's are twice that: a piecewise
+   *     constant field pays for its jumps);
+   *   block: the children's blocks summed, plus 2 e for every edge inside
+   *     the aggregate on the diagonal (P^T M P's block), plus half the sum of
+   *     the four g around it (so that the diagonal keeps the children's
+   *     data terms and loses exactly the coarse edges).
+   * Sums run in the working precision, from zero: children in the order
+   * (2I,2J), (2I,2J+1), (2I+1,2J), (2I+1,2J+1); then the inner edges right
+   * of (2I,2J), right of (2I+1,2J), down of (2I,2J), down of (2I,2J+1), each
+   * added as 2 e; g over rows 2I, 2I+1 (right) or columns 2J, 2J+1 (down);
+   * the diagonal's last term as 0.5 * (((0 + g_left) + g_up) + g_right) +
+   * g_down), absent neighbours skipped.  A coarse level is made while the
+   * finer one is at least 4x4. */
+  static bool cg_has_coarse(int W, int H) { return W >= 4 && H >= 4; }
+  static bool cg_when_p(const orc_params* p, int it) { return it % 25 == p->cg_at; }
+  bool cg_when(int it) const { return cg_when_p(p, it); }
+
+  /* g_right (down = 0) or g_down (down = 1) of coarse pixel (I, J), edge plane z (0: xy, 1: z) */
+  static CT cg_cross(const CT* f, int W, int H, int I, int J, int down, int z) {
+    const size_t N = (size_t)W * H;
+    const int Wc = (W + 1) / 2, Hc = (H + 1) / 2;
+    CT s = 0;
+    if (!down) {
+      if (J + 1 >= Wc) return s;
+      const size_t a = (size_t)(2 * I) * W + 2 * J + 1;
+      s += f[z * N + a];
+      if (2 * I + 1 < H) s += f[z * N + a + W];
+    } else {
+      if (I + 1 >= Hc) return s;
+      const size_t a = (size_t)(2 * I + 1) * W + 2 * J;
+      s += f[(2 + z) * N + a];
+      if (2 * J + 1 < W) s += f[(2 + z) * N + a + 1];
+    }
+    return s;
+  }
+
+  /* the coarse operator (planes 0-9); the rhs planes are set to zero */
+  static void cg_operator(const CT* f, int W, int H, CT* c) {
+    const size_t N = (size_t)W * H;
+    const int Wc = (W + 1) / 2, Hc = (H + 1) / 2;
+    const size_t Nc = (size_t)Wc * Hc;
+    for (int I = 0; I < Hc; I++)
+      for (int J = 0; J < Wc; J++) {
+        const size_t L = (size_t)I * Wc + J;
+        const int i0 = 2 * I, j0 = 2 * J;
+        const bool r1 = i0 + 1 < H, c1 = j0 + 1 < W;
+        const size_t p00 = (size_t)i0 * W + j0;
+        size_t ch[4];
+        int n = 0;
+        ch[n++] = p00;
+        if (c1) ch[n++] = p00 + 1;
+        if (r1) ch[n++] = p00 + W;
+        if (r1 && c1) ch[n++] = p00 + W + 1;
+        CT around[2];
+        for (int z = 0; z < 2; z++) {
+          const CT gr = cg_cross(f, W, H, I, J, 0, z), gd = cg_cross(f, W, H, I, J, 1, z);
+          CT t = 0;
+          if (J > 0) t += cg_cross(f, W, H, I, J - 1, 0, z);
+          if (I > 0) t += cg_cross(f, W, H, I - 1, J, 1, z);
+          t += gr;
+          t += gd;
+          around[z] = t;
+          c[z * Nc + L] = (CT).5 * gr;
+          c[(2 + z) * Nc + L] = (CT).5 * gd;
+        }
+        for (int k = 4; k < 10; k++) {
+          CT s = 0;
+          for (int a = 0; a < n; a++) s += f[k * N + ch[a]];
+          if (k <= 6) {
+            /* Buu, Bvv: the xy edges (planes 0, 2); Bww: the z edges (1, 3) */
+            const int z = k < 6 ? 0 : 1;
+            if (c1) s += (CT)2 * f[z * N + p00];
+            if (c1 && r1) s += (CT)2 * f[z * N + p00 + W];
+            if (r1) s += (CT)2 * f[(2 + z) * N + p00];
+            if (r1 && c1) s += (CT)2 * f[(2 + z) * N + p00 + 1];
+            s += (CT).5 * around[z];
+          }
+          c[k * Nc + L] = s;
+        }
+        for (int k = 10; k < 13; k++) c[k * Nc + L] = 0;
+      }
+  }
+
+  /* The residual of pixel l's unknown k at x in the working precision:
+   * rhs - e_left x_left - e_up x_up - e_right x_right - e_down x_down -
+   * B_k0 x_0 - B_k1 x_1 - B_k2 x_2, in that order (missing neighbours
+   * skipped). */
+  static CT cg_residual(const CT* f, int W, int H, const CT* x, int i, int j, int k) {
+    const size_t N = (size_t)W * H, l = (size_t)i * W + j;
+    const CT* xk = x + k * N;
+    const int er = k < 2 ? 0 : 1, ed = k < 2 ? 2 : 3;
+    CT a = f[(10 + k) * N + l];
+    if (j > 0) a -= f[er * N + l - 1] * xk[l - 1];
+    if (i > 0) a -= f[ed * N + l - W] * xk[l - W];
+    if (j < W - 1) a -= f[er * N + l] * xk[l + 1];
+    if (i < H - 1) a -= f[ed * N + l] * xk[l + W];
+    static const int row[3][3] = {{4, 7, 8}, {7, 5, 9}, {8, 9, 6}}; /* (Buu Buv Buw) (Buv Bvv Bvw) (Buw Bvw Bww) */
+    for (int m = 0; m < 3; m++) a -= f[row[k][m] * N + l] * x[m * N + l];
+    return a;
+  }
+
+  /* the coarse rhs (planes 10-12): sums of the children's residuals at x */
+  static void cg_restrict(const CT* f, int W, int H, const CT* x, CT* c) {
+    const int Wc = (W + 1) / 2, Hc = (H + 1) / 2;
+    const size_t Nc = (size_t)Wc * Hc;
+    for (int I = 0; I < Hc; I++)
+      for (int J = 0; J < Wc; J++)
+        for (int k = 0; k < 3; k++) {
+          CT s = 0;
+          for (int di = 0; di < 2; di++)
+            for (int dj = 0; dj < 2; dj++) {
+              const int i = 2 * I + di, j = 2 * J + dj;
+              if (i < H && j < W) s += cg_residual(f, W, H, x, i, j, k);
+            }
+          c[(10 + k) * Nc + (size_t)I * Wc + J] = s;
+        }
+  }
+
+  /* x += P xc */
+  static void cg_prolong(CT* x, int W, int H, const CT* xc) {
+    const size_t N = (size_t)W * H;
+    const int Wc = (W + 1) / 2, Hc = (H + 1) / 2;
+    const size_t Nc = (size_t)Wc * Hc;
+    for (int i = 0; i < H; i++)
+      for (int j = 0; j < W; j++)
+        for (int k = 0; k < 3; k++) x[k * N + (size_t)i * W + j] += xc[k * Nc + (size_t)(i / 2) * Wc + j / 2];
+  }
+
+  /* The coarse levels of a system: sizes and operators (rhs zero). */
+  struct CgHier {
+    vector<int> Ws, Hs;
+    vector<vector<CT>> coef, x;
+    void build(const CT* f, int W, int H, int levels) {
+      Ws.assign(1, W); Hs.assign(1, H);
+      coef.clear(); x.clear();
+      coef.emplace_back(); x.emplace_back();
+      const CT* cur = f;
+      while ((int)Ws.size() <= levels && cg_has_coarse(Ws.back(), Hs.back())) {
+        const int w = Ws.back(), h = Hs.back(), wc = (w + 1) / 2, hc = (h + 1) / 2;
+        coef.emplace_back(13 * (size_t)wc * hc);
+        x.emplace_back(3 * (size_t)wc * hc);
+        cg_operator(cur, w, h, coef.back().data());
+        cur = coef.back().data();
+        Ws.push_back(wc); Hs.push_back(hc);
+      }
+    }
+    int levels() const { return (int)Ws.size() - 1; }
+  };
+
+  /* One V-cycle on the fine system (f, x): for l = 1..L, the rhs of level l
+   * from the residual of level l-1, x_l = 0, then `sweeps` red-black
+   * iterations at level l; for l = L..1, `sweeps` iterations at level l, then
+   * x_{l-1} += P x_l.  (The coarsest level gets 2 x sweeps.)  The
+   * relaxation is the level solve's own: factor, pixel order, step limit. */
+  static void cg_vcycle(CgHier& h, const CT* f, CT* x, int sweeps, float omega, int mode) {
+    const int L = h.levels();
+    auto relax = [&](int l) {
+      for (int s = 0; s < sweeps; s++) {
+        relax_colour(h.coef[l].data(), h.Ws[l], h.Hs[l], 0, h.x[l].data(), omega, mode);
+        relax_colour(h.coef[l].data(), h.Ws[l], h.Hs[l], 1, h.x[l].data(), omega, mode);
+      }
+    };
+    for (int l = 1; l <= L; l++) {
+      cg_restrict(l == 1 ? f : h.coef[l - 1].data(), h.Ws[l - 1], h.Hs[l - 1], l == 1 ? x : h.x[l - 1].data(),
+                  h.coef[l].data());
+      std::fill(h.x[l].begin(), h.x[l].end(), (CT)0);
+      relax(l);
+    }
+    for (int l = L; l >= 1; l--) {
+      relax(l);
+      cg_prolong(l == 1 ? x : h.x[l - 1].data(), h.Ws[l - 1], h.Hs[l - 1], h.x[l].data());
+    }
+  }
+
   /* .cu:579-635 GPU warps: floor base, explicit clamps, f32 weights (f64 in the FP64 oracle) */
   void warp_gpu() {
     typedef typename std::conditional<REF, float, double>::type AT;
@@ -1245,6 +1428,7 @@ struct Solver {
     int outer = 0, sweeps = 0, rchecks = 0;
     double lastStat = -1, lastRes = -1, lastEnergy = 0;
     float lastMax = -1;
+    CgHier cg;
     for (int count = 0; count < p->n_outer; count++) {
       outer++;
       derivatives();
@@ -1254,11 +1438,14 @@ struct Solver {
         penalties();
         assemble();
         if (REF && p->record_storage != ORC_REC_F32) store_records();
+        const float om = p->omega > 0 ? p->omega : .7f;
+        if (p->cg_levels > 0) cg.build(coef.data(), W, H, p->cg_levels);
         for (int it = 0; it < p->n_sor; it++) {
-          const float om = p->omega > 0 ? p->omega : .7f;
           const CT m0 = relax_colour(coef.data(), W, H, 0, du.data(), om, p->pixel_gs);
           const CT m1 = relax_colour(coef.data(), W, H, 1, du.data(), om, p->pixel_gs);
           sweeps++;
+          if (p->cg_levels > 0 && cg.levels() > 0 && cg_when(it))
+            cg_vcycle(cg, coef.data(), du.data(), p->cg_sweeps, om, p->pixel_gs);
           if (it % checkEvery == checkEvery - 1) {
             const CT md = std::max(m0, m1);
             lastMax = (float)md;
@@ -1384,6 +1571,29 @@ static void relax_residual_t(const double* coef13, int W, int H, float omega, in
   *iters = it;
   *residual = res;
   *energy = en;
+}
+
+template <class CT, bool REF>
+static void relax_cg_t(const double* coef13, int W, int H, const orc_params* p, int n_iter, double* x,
+                       double* maxd, double* res) {
+  typedef Solver<CT, REF> S;
+  const size_t N = (size_t)W * H;
+  vector<CT> c(coef13, coef13 + 13 * N), xx(x, x + 3 * N);
+  const float om = p->omega > 0 ? p->omega : .7f;
+  typename S::CgHier h;
+  if (p->cg_levels > 0) h.build(c.data(), W, H, p->cg_levels);
+  for (int it = 0; it < n_iter; it++) {
+    const CT m0 = S::relax_colour(c.data(), W, H, 0, xx.data(), om, p->pixel_gs);
+    const CT m1 = S::relax_colour(c.data(), W, H, 1, xx.data(), om, p->pixel_gs);
+    if (p->cg_levels > 0 && h.levels() > 0 && S::cg_when_p(p, it))
+      S::cg_vcycle(h, c.data(), xx.data(), p->cg_sweeps, om, p->pixel_gs);
+    if (maxd) maxd[it] = (double)std::max(m0, m1);
+    if (res) {
+      double e;
+      S::residual(c.data(), W, H, xx.data(), &res[it], &e);
+    }
+  }
+  for (size_t k = 0; k < 3 * N; k++) x[k] = xx[k];
 }
 
 extern "C" {
@@ -1554,6 +1764,50 @@ void orc_relax(const double* coef13, int W, int H, int precision, int n_iter, do
       Solver<double, false>::relax_colour(c.data(), W, H, 1, xx.data());
     }
     for (size_t k = 0; k < 3 * N; k++) x[k] = xx[k];
+  }
+}
+
+void orc_relax_cg(const double* coef13, int W, int H, const orc_params* p, int n_iter, double* x, double* maxd,
+                  double* res) {
+  if (p->precision == ORC_PREC_REF) relax_cg_t<float, true>(coef13, W, H, p, n_iter, x, maxd, res);
+  else relax_cg_t<double, false>(coef13, W, H, p, n_iter, x, maxd, res);
+}
+
+void orc_cg_operator(const double* coef13, int W, int H, int precision, double* out13) {
+  const int Wc = (W + 1) / 2, Hc = (H + 1) / 2;
+  const size_t N = (size_t)W * H, Nc = (size_t)Wc * Hc;
+  if (precision == ORC_PREC_REF) {
+    vector<float> f(coef13, coef13 + 13 * N), c(13 * Nc);
+    Solver<float, true>::cg_operator(f.data(), W, H, c.data());
+    for (size_t k = 0; k < 13 * Nc; k++) out13[k] = c[k];
+  } else {
+    Solver<double, false>::cg_operator(coef13, W, H, out13);
+  }
+}
+
+void orc_cg_restrict(const double* coef13, int W, int H, int precision, const double* x, double* rhs3) {
+  const int Wc = (W + 1) / 2, Hc = (H + 1) / 2;
+  const size_t N = (size_t)W * H, Nc = (size_t)Wc * Hc;
+  if (precision == ORC_PREC_REF) {
+    vector<float> f(coef13, coef13 + 13 * N), xx(x, x + 3 * N), c(13 * Nc);
+    Solver<float, true>::cg_restrict(f.data(), W, H, xx.data(), c.data());
+    for (size_t k = 0; k < 3 * Nc; k++) rhs3[k] = c[10 * Nc + k];
+  } else {
+    vector<double> c(13 * Nc);
+    Solver<double, false>::cg_restrict(coef13, W, H, x, c.data());
+    for (size_t k = 0; k < 3 * Nc; k++) rhs3[k] = c[10 * Nc + k];
+  }
+}
+
+void orc_cg_prolong(double* x, int W, int H, int precision, const double* xc) {
+  const int Wc = (W + 1) / 2, Hc = (H + 1) / 2;
+  const size_t N = (size_t)W * H, Nc = (size_t)Wc * Hc;
+  if (precision == ORC_PREC_REF) {
+    vector<float> xx(x, x + 3 * N), cc(xc, xc + 3 * Nc);
+    Solver<float, true>::cg_prolong(xx.data(), W, H, cc.data());
+    for (size_t k = 0; k < 3 * N; k++) x[k] = xx[k];
+  } else {
+    Solver<double, false>::cg_prolong(x, W, H, xc);
   }
 }
 

@@ -50,6 +50,12 @@ typedef struct {
                              rounded, wherever they are produced, to f32 and then to 16 bits */
   int record_storage;     /* ORC_REC_*: the level solve's per-pixel systems rounded, right after
                              assembly, as 16-bit records hold them (reference precision only) */
+  int cg_levels;          /* coarse-grid correction (not in the reference): 0 off; else a V-cycle
+                             over up to this many 2x2-aggregation levels of the relaxed system
+                             after iteration cg_at of every 25 (Solver::cg_vcycle) */
+  int cg_sweeps;          /* red-black iterations on each coarse level before and after the
+                             correction from the next coarser one */
+  int cg_at;              /* iteration (0..23) of each 25 after which the V-cycle runs */
 } orc_params;
 
 enum { ORC_REC_F32 = 0, ORC_REC_F16 = 1, ORC_REC_BF16 = 2 };
@@ -141,6 +147,19 @@ void orc_level_weights(const orc_level_in* in, const orc_params* p, const double
                        const double* w, double* phi, double* psi, double* pi);
 /* nIter red-black iterations on a compact system; x = [3][N] planes (in/out). */
 void orc_relax(const double* coef13, int W, int H, int precision, int n_iter, double* x);
+/* n_iter red-black iterations (factor, pixel order and precision of p) with
+ * p's coarse-grid correction, from x; per iteration (may be NULL): the
+ * largest |change| of its two colour passes and the f64 relative residual
+ * after it (and after the V-cycle that follows it, if one does). */
+void orc_relax_cg(const double* coef13, int W, int H, const orc_params* p, int n_iter, double* x, double* maxd,
+                  double* res);
+/* The coarse-grid correction's parts (Solver::cg_*; not in the reference), on
+ * a W x H compact system; the coarse level is ((W+1)/2) x ((H+1)/2):
+ * the coarse operator (13 planes, rhs zero); the coarse rhs (3 planes) from
+ * the residual at x ([3][N]); x += P xc. */
+void orc_cg_operator(const double* coef13, int W, int H, int precision, double* out13);
+void orc_cg_restrict(const double* coef13, int W, int H, int precision, const double* x, double* rhs3);
+void orc_cg_prolong(double* x, int W, int H, int precision, const double* xc);
 /* The residual of a compact system at x in f64: out[0] = ||r - M x||_2 / ||r||_2
  * (||r - M x||_2 when r = 0), out[1] = r'x - 1/2 x'Mx, where M is the operator
  * the relaxation inverts (.cu:404-459): (M x)_k = sum_n e_nk x_nk + sum_k' B_kk' x_k'. */

@@ -55,7 +55,8 @@ class Params(C.Structure):
                 ("relax_variant", C.c_int), ("inner_exit", C.c_int), ("outer_exit", C.c_int),
                 ("tile_iters", C.c_int), ("tile_w", C.c_int), ("tile_h", C.c_int), ("tile_config", C.c_int),
                 ("graph", C.c_int), ("storage", C.c_int), ("omega", C.c_float), ("residual_tol", C.c_float),
-                ("pixel_update", C.c_int), ("image_storage", C.c_int)]
+                ("pixel_update", C.c_int), ("image_storage", C.c_int), ("coarse_levels", C.c_int),
+                ("coarse_sweeps", C.c_int), ("coarse_at", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -149,7 +150,7 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
                 ratio=0.5, min_width=None, reg=REG_HPB, cw=1.0, dvc=1.0, inner_exit=True, outer_exit=True,
                 relax_variant=RELAX_PLAIN, tile_iters=0, tile_w=0, tile_h=0, tile_config=-1, graph=False,
                 storage=STORE_F32, omega=0.7, residual_tol=1e-4, pixel_update=PIXEL_JACOBI,
-                image_storage=IMAGES_F64) -> Params:
+                image_storage=IMAGES_F64, coarse_levels=0, coarse_sweeps=2, coarse_at=9) -> Params:
     """inner_exit: True / EXIT_REFERENCE (the reference's max-change test),
     False / EXIT_NONE, or EXIT_RESIDUAL (FP64 relative residual <= residual_tol)."""
     p = Params()
@@ -177,6 +178,7 @@ def make_params(W, H, f=None, cx=None, cy=None, alpha=0.02, gamma=1e-5, n_outer=
     p.residual_tol = residual_tol
     p.pixel_update = pixel_update
     p.image_storage = image_storage
+    p.coarse_levels, p.coarse_sweeps, p.coarse_at = coarse_levels, coarse_sweeps, coarse_at
     return p
 
 

@@ -16,8 +16,8 @@ LIB_CXX = os.path.join(HERE, "librgbdflow_cxx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RGBDFLOW_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels_solve.hip", "kernels_pyramid.hip", "kernels_relax_tiled.hip", "kernels_hpb.hip", "capi.cpp",
-               "server.cpp"]
+HIP_SOURCES = ["kernels_solve.hip", "kernels_pyramid.hip", "kernels_relax_tiled.hip", "kernels_hpb.hip",
+               "kernels_coarse.hip", "capi.cpp", "server.cpp"]
 CXX_SOURCES = ["dropin.cpp", "median_fill.cpp"]
 
 # -ffp-contract=off: products and sums round where the source says, which the

@@ -45,6 +45,10 @@ struct Work {
   float4 *du, *du_alt;  // [N] (du, dv, dw, unused)
   float4* coef;         // [N][kRec] system records
   SolveCtl* ctl;  // kMaxLevels entries, one per level
+  // coarse-grid correction (allocated when asked for): the coarse levels of
+  // the level being solved, packed level after level
+  float4 *cg_coef, *cg_du, *cg_du_alt;
+  SolveCtl* cg_ctl;  // kMaxCg entries, one per coarse level
 };
 
 // One loaded frame pair: inputs, pyramids (levels >= 1 packed finest first)
@@ -80,6 +84,9 @@ struct StepPlan {
   int img = 0;                     // feature-image storage of the step's items
   int relax_mode = kRelaxFixed;    // RelaxMode of the step's items
   bool any_early = false;
+  // coarse-grid correction: the coarse levels 1, 2, ... of the step's items
+  // (level l holds the items that have it)
+  std::vector<StepPlan> cg;
 };
 
 struct BatchPlan {
@@ -355,6 +362,32 @@ int get_work(rgbdflow_ctx* c, const std::string& pre, size_t N, Work& w) {
   TRY(get(c, pre + "q", N, &w.q));
   TRY(get(c, pre + "coef", (size_t)kRec * N, &w.coef));
   TRY(get(c, pre + "ctl", kMaxLevels, &w.ctl));
+  w.cg_coef = w.cg_du = w.cg_du_alt = nullptr;
+  w.cg_ctl = nullptr;
+  return RGBDFLOW_OK;
+}
+
+// Sizes of the coarse levels of a W x H level (at most `levels`).
+std::vector<LevelShape> cg_shapes(int W, int H, int levels) {
+  std::vector<LevelShape> L;
+  while ((int)L.size() < levels && cg_coarse_ok(W, H)) {
+    W = (W + 1) / 2;
+    H = (H + 1) / 2;
+    L.push_back({W, H});
+  }
+  return L;
+}
+
+// The coarse-grid correction's buffers of a work slot whose finest level is
+// W x H (every level of the slot uses a prefix of them).
+int get_cg_work(rgbdflow_ctx* c, const std::string& pre, int W, int H, Work& w) {
+  size_t n = 0;
+  for (const LevelShape& l : cg_shapes(W, H, kMaxCg)) n += (size_t)l.W * l.H;
+  n += 1;
+  TRY(get(c, pre + "cg_coef", (size_t)kRec * n, &w.cg_coef));
+  TRY(get(c, pre + "cg_du", n, &w.cg_du));
+  TRY(get(c, pre + "cg_du_alt", n, &w.cg_du_alt));
+  TRY(get(c, pre + "cg_ctl", kMaxCg, &w.cg_ctl));
   return RGBDFLOW_OK;
 }
 
@@ -609,12 +642,64 @@ void pack_step(std::vector<char>& host, const std::vector<LevelView>& items, Ste
   st.otile_off = pack(host, otile.data(), otile.size() * sizeof(int));
 }
 
+// The coarse levels of a step for the coarse-grid correction (p's
+// coarse_levels; none when 0): each item's levels over its work slot's
+// coarse buffers (works[i]: item i's), planned and packed level by level.
+void pack_coarse(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<char>& host,
+                 const std::vector<LevelView>& items, const std::vector<const Work*>& works, StepPlan& st) {
+  st.cg.clear();
+  if (!p || p->coarse_levels <= 0) return;
+  for (const Work* w : works)
+    if (!w->cg_coef) return;  // no buffers: the caller allocates them before planning
+  std::vector<std::vector<LevelView>> lv;
+  for (size_t i = 0; i < items.size(); i++) {
+    const LevelView& fv = items[i];
+    const Work& w = *works[i];
+    LevelView::Finer fin{fv.W, fv.H, fv.storage, fv.coef, fv.du, fv.du_alt, fv.ctl, fv.ctl, 1};
+    size_t off = 0;
+    const std::vector<LevelShape> L = cg_shapes(fv.W, fv.H, p->coarse_levels);
+    for (size_t l = 0; l < L.size(); l++) {
+      LevelView v;
+      std::memset(&v, 0, sizeof(v));
+      v.W = L[l].W;
+      v.H = L[l].H;
+      v.N = v.W * v.H;
+      v.coef = w.cg_coef + (size_t)kRec * off;
+      v.du = w.cg_du + off;
+      v.du_alt = w.cg_du_alt + off;
+      v.ctl = w.cg_ctl + l;
+      v.partials = fv.partials;
+      v.storage = RGBDFLOW_STORE_F32;
+      v.img = fv.img;
+      v.omega = fv.omega;
+      v.omega_c = fv.omega_c;
+      v.pixel_gs = fv.pixel_gs;
+      v.finer = fin;
+      fin = LevelView::Finer{v.W, v.H, RGBDFLOW_STORE_F32, v.coef, v.du, v.du_alt, v.ctl, fv.ctl, 0};
+      if (lv.size() <= l) lv.resize(l + 1);
+      lv[l].push_back(v);
+      off += (size_t)v.N;
+    }
+  }
+  // the coarse relaxation: the step's variant, launches of coarse_sweeps iterations
+  rgbdflow_params q = *p;
+  q.tile_iters = p->coarse_sweeps;
+  q.tile_w = q.tile_h = 0;
+  q.tile_config = -1;
+  for (std::vector<LevelView>& li : lv) {
+    st.cg.emplace_back();
+    plan_step_relax(c, &q, li, st.cg.back());
+    pack_step(host, li, st.cg.back());
+  }
+}
+
 void bind_step(char* dev, StepPlan& st) {
   const LevelView* items = reinterpret_cast<const LevelView*>(dev + st.items_off);
   st.vpix = Views{items, reinterpret_cast<const int*>(dev + st.pix_off), st.n};
   st.vplain = Views{items, reinterpret_cast<const int*>(dev + st.plain_off), st.n};
   st.vtiled = Views{items, reinterpret_cast<const int*>(dev + st.tiled_off), st.n};
   st.votile = Views{items, reinterpret_cast<const int*>(dev + st.otile_off), st.n};
+  for (StepPlan& cs : st.cg) bind_step(dev, cs);
 }
 
 int upload(rgbdflow_ctx* c, void* dst, const void* src, size_t bytes) {
@@ -644,10 +729,11 @@ int upload_desc(rgbdflow_ctx* c, const char* name, const std::vector<char>& host
 // A one-step plan over the given items (the single-level entry points and the
 // autotuner).  keep_ts: the items' tile schedules and st's launch shape are set.
 int one_step(rgbdflow_ctx* c, const char* name, std::vector<LevelView>& items, const rgbdflow_params* sched,
-             StepPlan& st, bool keep_ts = false) {
+             StepPlan& st, bool keep_ts = false, const Work* wk = nullptr) {
   if (!keep_ts) plan_step_relax(c, sched, items, st);
   std::vector<char> host;
   pack_step(host, items, st);
+  if (wk && items.size() == 1) pack_coarse(c, sched, host, items, std::vector<const Work*>{wk}, st);
   char* dev;
   TRY(upload_desc(c, name, host, &dev));
   bind_step(dev, st);
@@ -663,33 +749,85 @@ int one_step(rgbdflow_ctx* c, const char* name, std::vector<LevelView>& items, c
 // copy_back: leave the result in du (a tiled schedule's last launch may have
 // written du_alt; SolveCtl::du_in_alt says which) -- not needed when the next
 // reader is k_update_warp, which reads whichever holds it.
+// n red-black iterations of a coarse level from the unknowns in du_alt (alt)
+// or du; returns where they end.  No exits: the relaxed level's stop
+// (SolveCtl::outer_done of the coarse level, set by k_cg_restrict) skips them.
+int coarse_sweeps(rgbdflow_ctx* c, const StepPlan& cs, int n, int alt) {
+  hipStream_t s = c->stream;
+  if (cs.tiled_ok) {
+    for (int it = 0; it < n;) {
+      const int T = std::min(cs.T, n - it);
+      launch_relax_tiled(s, cs.vtiled, cs.tiled, cs.grid, cs.cfg, cs.storage, cs.relax_mode, cs.slots, alt, T, 0, 0,
+                         -1);
+      alt ^= 1;
+      it += T;
+    }
+    return alt;
+  }
+  for (int it = 0; it < n; it++) {
+    launch_relax_plain(s, cs.vplain, cs.plain, 0, 0, 0, -1);
+    launch_relax_plain(s, cs.vplain, cs.plain, 1, 0, 0, -1);
+  }
+  return 0;
+}
+
+// One V-cycle of the coarse-grid correction on the relaxed level's unknowns
+// (in du_alt if alt): down, the coarse rhs from the finer level's residual
+// and the coarse level's sweeps; up, the sweeps again and the correction
+// added to the finer level.  n_prev: checks before it (the stop test).
+void enqueue_vcycle(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, int slot_base, int n_prev, int alt) {
+  hipStream_t s = c->stream;
+  const int L = (int)st.cg.size();
+  int where[kMaxCg + 1];
+  where[0] = alt;
+  for (int l = 1; l <= L; l++) {
+    const StepPlan& cs = st.cg[l - 1];
+    launch_cg_restrict(s, cs.vpix, cs.pix, where[l - 1], slot_base, n_prev);
+    where[l] = coarse_sweeps(c, cs, k.cg_sweeps, 0);
+  }
+  for (int l = L; l >= 1; l--) {
+    const StepPlan& cs = st.cg[l - 1];
+    where[l] = coarse_sweeps(c, cs, k.cg_sweeps, where[l]);
+    launch_cg_prolong(s, cs.vpix, cs.pix, where[l], where[l - 1]);
+  }
+}
+
 void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, int slot_base, bool copy_back = true) {
   hipStream_t s = c->stream;
   const bool resid = k.inner_exit == RGBDFLOW_EXIT_RESIDUAL;
   auto check_of = [&](int last) { return (last % kCheckEvery == kCheckEvery - 1) ? last / kCheckEvery : -1; };
+  // coarse-grid correction: a V-cycle after iteration cg_at of every kCheckEvery
+  const bool cg = !st.cg.empty() && k.n_sor > k.cg_at;
+  if (cg)
+    for (const StepPlan& cs : st.cg) launch_cg_operator(s, cs.vpix, cs.pix);
   if (st.tiled_ok) {
     // timed: one event pair around the launches of this (outer, inner), which
-    // run back to back (around each launch where residual checks come between)
-    if (!resid && k.n_sor > 0) relax_event(c);
+    // run back to back (around each launch where residual checks or V-cycles
+    // come between)
+    const bool each = resid || cg;
+    if (!each && k.n_sor > 0) relax_event(c);
     int it = 0, flip = 0;
     while (it < k.n_sor) {
       const int boundary = (it / kCheckEvery + 1) * kCheckEvery;
-      const int T = std::min(std::min(st.T, boundary - it), k.n_sor - it);
+      int T = std::min(std::min(st.T, boundary - it), k.n_sor - it);
+      const int cg_it = it - it % kCheckEvery + k.cg_at;
+      if (cg && cg_it >= it) T = std::min(T, cg_it - it + 1);
       const int last = it + T - 1;
       const int n_prev = k.inner_exit ? it / kCheckEvery : 0;
       const int chk = check_of(last);
       const int slot = chk >= 0 ? slot_base + chk : -1;
-      if (resid) relax_event(c);
+      if (each) relax_event(c);
       launch_relax_tiled(s, st.vtiled, st.tiled, st.grid, st.cfg, st.storage, st.relax_mode, st.slots, flip, T,
                          slot_base, n_prev, resid ? -1 : slot);
-      if (resid) relax_event(c);
+      if (each) relax_event(c);
       c->relax_n += c->time_relax ? 1 : 0;
       // the launch read du (flip 0) or du_alt (flip 1) and wrote the other
       if (resid && chk >= 0) launch_residual(s, st.vpix, st.pix, flip ? 0 : 1, slot_base, chk, slot, k.residual_tol);
       flip ^= 1;
       it += T;
+      if (cg && last == cg_it) enqueue_vcycle(c, st, k, slot_base, k.inner_exit ? it / kCheckEvery : 0, flip);
     }
-    if (!resid && k.n_sor > 0) relax_event(c);
+    if (!each && k.n_sor > 0) relax_event(c);
     if (k.n_sor > 0 && copy_back) launch_copy_du(s, st.vpix, st.pix);
     return;
   }
@@ -705,6 +843,7 @@ void enqueue_relax(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, i
     relax_event(c);
     c->relax_n += c->time_relax ? 2 : 0;
     if (resid && chk >= 0) launch_residual(s, st.vpix, st.pix, 0, slot_base, chk, slot, k.residual_tol);
+    if (cg && it % kCheckEvery == k.cg_at) enqueue_vcycle(c, st, k, slot_base, k.inner_exit ? (it + 1) / kCheckEvery : 0, 0);
   }
 }
 
@@ -761,6 +900,8 @@ SolveScalars scalars_from(const rgbdflow_params* p, double cw, double alpha, dou
   k.inner_exit = p ? p->inner_exit : 1;
   k.outer_exit = p ? p->outer_exit : 1;
   k.residual_tol = (p && p->residual_tol > 0) ? p->residual_tol : 1e-4;
+  k.cg_sweeps = p ? p->coarse_sweeps : 2;
+  k.cg_at = p ? p->coarse_at : 9;
   return k;
 }
 
@@ -886,7 +1027,9 @@ int alloc_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, int W, int H,
     }
     TRY(get(c, pre + "pdm", pyramid_elems(ps.L, 1) + 1, &ps.pdm));
   }
-  return get_work(c, pre, N, ps.wk);
+  TRY(get_work(c, pre, N, ps.wk));
+  if (p->coarse_levels > 0) TRY(get_cg_work(c, pre, W, H, ps.wk));
+  return RGBDFLOW_OK;
 }
 
 int load_pair(rgbdflow_ctx* c, const rgbdflow_params* p, int idx, const rgbdflow_pair_input& in, PairSlot& ps) {
@@ -932,6 +1075,12 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
   bp.dm_steps.clear();
   bp.hpb_groups.clear();
   const int B = (int)c->pairs.size();
+  // the coarse-grid correction's buffers (pairs loaded without it have none)
+  if (p->coarse_levels > 0)
+    for (int i = 0; i < B; i++) {
+      PairSlot& ps = c->pairs[i];
+      if (!ps.wk.cg_coef) TRY(get_cg_work(c, "b" + std::to_string(i) + ".", ps.W, ps.H, ps.wk));
+    }
   // per-pair preparation
   {
     std::vector<PairPrep> items(B);
@@ -1052,10 +1201,12 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
   for (int s = 0; s < S; s++) {
     const int k = S - 1 - s;
     std::vector<LevelView> items;
+    std::vector<const Work*> works;
     for (int i = 0; i < B; i++) {
       PairSlot& ps = c->pairs[i];
       const int nL = (int)ps.L.size();
       if (k >= nL) continue;
+      works.push_back(&ps.wk);
       const rgbdflow_camera& cam = ps.cam_given ? ps.cam : p->cam;
       const int Wk = ps.L[k].W, Hk = ps.L[k].H;
       const int slot = (nL - 1 - k) & 1;  // flow buffers alternate level by level
@@ -1091,7 +1242,8 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
     StepPlan st;
     plan_step_relax(c, p, items, st);
     pack_step(bp.host, items, st);
-    bp.steps.push_back(st);
+    pack_coarse(c, p, bp.host, items, works, st);
+    bp.steps.push_back(std::move(st));
   }
   char* dev;
   TRY(upload_desc(c, "desc", bp.host, &dev));
@@ -1117,6 +1269,15 @@ int check_block_storage(rgbdflow_ctx* c, int pixel_update, int storage) {
   return RGBDFLOW_OK;
 }
 
+int check_coarse(rgbdflow_ctx* c, const rgbdflow_params* p) {
+  if (p->coarse_levels < 0 || p->coarse_levels > kMaxCg) return fail(c, RGBDFLOW_ERR_ARG, "bad coarse_levels");
+  if (p->coarse_levels > 0 && (p->coarse_sweeps < 1 || p->coarse_sweeps > 64))
+    return fail(c, RGBDFLOW_ERR_ARG, "bad coarse_sweeps");
+  if (p->coarse_levels > 0 && (p->coarse_at < 0 || p->coarse_at >= kCheckEvery - 1))
+    return fail(c, RGBDFLOW_ERR_ARG, "bad coarse_at");
+  return RGBDFLOW_OK;
+}
+
 int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
   if (!p) return fail(c, RGBDFLOW_ERR_ARG, "null params");
   if (p->variant != RGBDFLOW_VARIANT_FULL) return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "bad scene flow variant");
@@ -1130,6 +1291,7 @@ int check_run_params(rgbdflow_ctx* c, const rgbdflow_params* p) {
   TRY(check_block_storage(c, p->pixel_update, p->storage));
   if (p->image_storage < RGBDFLOW_IMAGES_F64 || p->image_storage > RGBDFLOW_IMAGES_BF16)
     return fail(c, RGBDFLOW_ERR_ARG, "bad image_storage");
+  TRY(check_coarse(c, p));
   return validate_iters(c, p->n_outer, p->n_inner, p->n_sor);
 }
 
@@ -1324,6 +1486,9 @@ int rgbdflow_default_params(rgbdflow_params* p) {
   p->tile_config = -1;
   p->storage = RGBDFLOW_STORE_F32;
   p->omega = .7f;
+  p->coarse_levels = 0;
+  p->coarse_sweeps = 2;
+  p->coarse_at = 9;
   return RGBDFLOW_OK;
 }
 
@@ -1408,12 +1573,14 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
   if (opts) TRY(check_block_storage(c, opts->pixel_update, opts->storage));
   if (opts && opts->image_storage != RGBDFLOW_IMAGES_F64)
     return fail(c, RGBDFLOW_ERR_UNSUPPORTED, "image_storage: the single-level entry points take f64 images");
+  if (opts) TRY(check_coarse(c, opts));
   HIP_TRY(c, hipSetDevice(c->device));
   const size_t N = (size_t)W * H;
   Work wk;
   LevelView view;
   TRY(stage_level_inputs(c, W, H, focal_level, Im1, dm1, Im2, dm2, warpIm2, warpDm2, u, v, w, dW, sW, mW, rW, wk,
                          view));
+  if (opts && opts->coarse_levels > 0) TRY(get_cg_work(c, "d.", W, H, wk));
   view.allow_early = allowEarly ? 1 : 0;
   if (opts) {
     view.storage = opts->storage;
@@ -1422,7 +1589,7 @@ int rgbdflow_solve_level(rgbdflow_ctx* c, int W, int H, float focal_level, const
   const SolveScalars k = scalars_from(opts, cw, alpha, gamma, nOuter, nInner, nSOR);
   std::vector<LevelView> items{view};
   StepPlan sp;
-  TRY(one_step(c, "d.desc", items, opts, sp));
+  TRY(one_step(c, "d.desc", items, opts, sp, false, &wk));
   hipStream_t s = c->stream;
   c->relax_ev_used = 0;
   c->relax_n = 0;
@@ -2146,6 +2313,7 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   const size_t N = (size_t)W * H;
   Work wk;
   TRY(get_work(c, "d.", N, wk));
+  if (p->coarse_levels > 0) TRY(get_cg_work(c, "d.", W, H, wk));
   double *i1, *i2, *d1, *d2, *del0, *sm0;
   float *s1, *h = nullptr;
   TRY(get(c, "wi.im1", 3 * N, &i1));
@@ -2208,7 +2376,7 @@ int rgbdflow_with_init(rgbdflow_ctx* c, int W, int H, const double* im1, const d
   view.f_level = p->cam.focalLength;
   std::vector<LevelView> items{view};
   StepPlan sp;
-  TRY(one_step(c, "d.desc", items, p, sp));
+  TRY(one_step(c, "d.desc", items, p, sp, false, &wk));
   const SolveScalars k = scalars_from(p, p->color_data_weight, p->alpha, p->gamma, p->n_outer, p->n_inner, p->n_sor);
   hipStream_t s = c->stream;
   c->relax_ev_used = 0;

@@ -20,6 +20,7 @@ constexpr int kMaxLevels = 16;
 constexpr int kCheckEvery = 25;     // relaxation convergence check period (.cu:905)
 constexpr int kMaxChecks = 4096;    // check slots per outer iteration
 constexpr int kMaxOuter = 1024;
+constexpr int kMaxCg = 8;          // coarse levels of the coarse-grid correction
 constexpr int kBlock = 256;         // threads of the per-pixel kernels
 
 // The compact system of one pixel p, in the order of the 13-plane export
@@ -112,6 +113,17 @@ struct LevelView {
   float f_full;         // full-size focal length
   float f_level;        // focal length of the level
   TileSched ts;
+  // coarse-grid correction: on the view of a coarse level (2x2 aggregates of
+  // the next finer level), that finer level's system and unknowns
+  struct Finer {
+    int W, H, storage;
+    const float4* coef;
+    float4* du;
+    float4* du_alt;
+    const SolveCtl* ctl;      // the finer level's control (coarse levels: outer_done = skip)
+    const SolveCtl* top_ctl;  // the relaxed level's control
+    int first;                // 1: the finer level is the relaxed level itself
+  } finer;
 };
 
 // Device code reaches the level buffers through global-address-space
@@ -255,6 +267,7 @@ struct SolveScalars {
   int inner_exit;           // relaxation exits: 0 none, 1 the reference's (max change), 2 FP64 residual
   double residual_tol;      // inner_exit 2: stop once the relative residual is <= this
   int outer_exit;           // outer early exits enabled
+  int cg_sweeps, cg_at;     // coarse-grid correction (when the step has coarse levels)
 };
 
 // Block counts of one item for each kernel family.
@@ -328,6 +341,12 @@ int tiled_prepare(int config, int storage, int mode, int slots, int blocks);
 void launch_relax_tiled(hipStream_t s, const Views& v, int blocks, int grid, int config, int storage, int mode,
                         int slots, int flip, int n_iter, int slot_base, int n_prev_checks, int check_slot);
 void launch_copy_du(hipStream_t s, const Views& v, int blocks);
+// kernels_coarse.hip: the coarse-grid correction (v: views of one coarse
+// level; blocks: pix_blocks of the coarse sizes)
+void launch_cg_operator(hipStream_t s, const Views& v, int blocks);
+void launch_cg_restrict(hipStream_t s, const Views& v, int blocks, int finer_alt, int slot_base, int n_prev_checks);
+void launch_cg_prolong(hipStream_t s, const Views& v, int blocks, int alt, int finer_alt);
+__host__ __device__ inline int cg_coarse_ok(int W, int H) { return W >= 4 && H >= 4; }
 void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H, int storage);
 // f32 records -> records in `storage` (src and dst distinct)
 void launch_encode_records(hipStream_t s, const float4* src, float4* dst, int N, int storage);
