@@ -65,8 +65,15 @@ def workload(R, n_distinct):
         pair = S.make_pair(W, H, 21 + i, n_segments=6, cam=cam)
         im1, d1, im2, d2 = S.to_solver_inputs(pair)
         hpb = S.segment_boundary_maps(pair["seg1"], [w for w, h in levels], [h for w, h in levels])
-        pairs.append(dict(im1=im1, dm1=d1.astype(np.float64), im2=im2, dm2=d2.astype(np.float64), hpb=hpb))
+        pairs.append(dict(im1=im1, dm1=d1.astype(np.float64), im2=im2, dm2=d2.astype(np.float64), hpb=hpb,
+                          gt=pair["flow"].astype(np.float64)))
     return pairs, levels
+
+
+def flow_error(flow, gt):
+    """Mean end-point error of (u, v) in px and mean |dw| in m against gt (H, W, 3)."""
+    u, v, w = flow
+    return (float(np.mean(np.hypot(u - gt[..., 0], v - gt[..., 1]))), float(np.mean(np.abs(w - gt[..., 2]))))
 
 
 MIXED_SIZES = [(320, 240), (640, 480), (960, 540), (1280, 720)]
@@ -216,6 +223,8 @@ def main():
     ap.add_argument("--label-maps", action="store_true",
                     help="boundary maps from the generator's segment labels instead of made on the device")
     ap.add_argument("--no-other-maps", action="store_true", help="skip the batch with the other boundary maps")
+    ap.add_argument("--coarse-levels", type=int, default=8,
+                    help="levels of the coarse-grid correction run beside the headline (device maps; 0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: the scene-flow solve runs on one GPU "
@@ -275,6 +284,7 @@ def main():
     log(f"{B} x 640x480 pairs, {len(levels)} levels, 15x50, variant {args.variant}, maps {maps}")
     pairs_per_s, elapsed, lat, st = run_batch(ctx, p, batch, args.steps, args.warmup, args.latency_steps)
     log(f"timed {args.steps} batches in {elapsed:.3f} s")
+    flow0 = ctx.batch_read(0)
 
     # dominant kernel: the relaxation, timed launch by launch with HIP events
     ctx.set_relax_timing(True)
@@ -353,6 +363,33 @@ def main():
                       "sweeps_exec_pair0": [ost[0].sweeps_exec[k] for k in range(L)],
                       "outer_exec_pair0": [ost[0].outer_exec[k] for k in range(L)]}
         log(f"other boundary maps {other_maps}")
+
+    # the device-map batch with the coarse-grid correction (opt-in): time, sweeps, and pair 0's
+    # distance to the generator's truth beside the reference rule's
+    coarse = None
+    if args.storage == "f32" and args.variant == "tiled" and args.coarse_levels > 0:
+        pc = params_for(R, args.variant, not args.no_graph, args.storage)
+        pc.image_storage = p.image_storage
+        pc.coarse_levels = args.coarse_levels
+        cpps, cel, _, cst = run_batch(ctx, pc, dev_batch, args.steps, args.warmup, 0)
+        cflow = ctx.batch_read(0)
+        if args.label_maps:
+            ctx.batch_load(p, dev_batch)
+            rsw = ctx.batch_run(p, stats=True)[0]
+            rflow = ctx.batch_read(0)
+        else:
+            rflow, rsw = flow0, rst[0]
+        gt = label_batch[0]["gt"]
+        e_ref, e_cg = flow_error(rflow, gt), flow_error(cflow, gt)
+        L = cst[0].n_levels
+        coarse = {"coarse_levels": args.coarse_levels, "coarse_sweeps": pc.coarse_sweeps, "coarse_at": pc.coarse_at,
+                  "boundary_maps": "device", "pairs_per_s": round(cpps, 3), "ms_per_step": round(cel / args.steps * 1e3, 3),
+                  "sweeps_exec_pair0": [cst[0].sweeps_exec[k] for k in range(L)],
+                  "sweeps_exec_pair0_reference_rule": [rsw.sweeps_exec[k] for k in range(L)],
+                  "pair0_epe_px_vs_truth": round(e_cg[0], 4), "pair0_abs_dw_m_vs_truth": round(e_cg[1], 6),
+                  "pair0_epe_px_vs_truth_reference_rule": round(e_ref[0], 4),
+                  "pair0_abs_dw_m_vs_truth_reference_rule": round(e_ref[1], 6)}
+        log(f"coarse-grid correction {coarse}")
 
     # the same batch with the systems stored in FP16 (f32 arithmetic)
     f16 = None
@@ -465,6 +502,7 @@ def main():
             "two_lanes": two_lanes,
             "fp16_storage": f16,
             "other_boundary_maps": other_maps,
+            "coarse_correction": coarse,
             "mixed_64": mixed,
             "served_16_mixed": served,
         },

@@ -70,7 +70,7 @@ def epe_stats(flows, refs):
     e = np.concatenate([np.sqrt((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2).ravel() for a, b in zip(flows, refs)])
     dw = np.concatenate([np.abs(a[2] - b[2]).ravel() for a, b in zip(flows, refs)])
     return dict(epe_median=float(np.median(e)), epe_mean=float(e.mean()), epe_p99=float(np.percentile(e, 99)),
-                dw_median=float(np.median(dw)), dw_p99=float(np.percentile(dw, 99)))
+                dw_median=float(np.median(dw)), dw_mean=float(dw.mean()), dw_p99=float(np.percentile(dw, 99)))
 
 
 def served_vs_fp64(R, O, warm_outer=4, n_frames=5, seed=900):
@@ -121,20 +121,80 @@ def served_vs_fp64(R, O, warm_outer=4, n_frames=5, seed=900):
                 fp64_cold_vs_truth=epe_stats(cold, truth))
 
 
+def coarse_report(R, O, n=4, steps=3):
+    """The coarse-grid correction on 640x480 pairs (seeds 21.., 6 segments)
+    with boundary maps made from the frames, as sceneFlowMultiscale.cpp:116
+    makes them: per rule with and without the correction, batch pairs/s,
+    pair 0's sweeps per level, and the flow's distance to the FP64 oracle of
+    the reference rule, to the FP64 oracle of the same variant and to the
+    truth."""
+    import rgbdflow_amd.synth as S
+    W, H = 640, 480
+    cam = S.Camera(W, H, 525.0, 525.0, 319.5, 239.5)
+    pairs = []
+    for i in range(n):
+        pr = S.make_pair(W, H, 21 + i, n_segments=6, cam=cam)
+        im1, d1, im2, d2 = S.to_solver_inputs(pr)
+        pairs.append(dict(im1=im1, dm1=d1.astype(np.float64), im2=im2, dm2=d2.astype(np.float64), hpb=None,
+                          sigma1=O.depth_stdevs(d1.astype(np.float32), 4), truth=pr["flow"].astype(np.float64)))
+    variants = {"reference": (dict(omega=0.7, pixel_update=R.PIXEL_JACOBI), dict(omega=0.7, pixel_gs=0), 0),
+                "reference+coarse8": (dict(omega=0.7, pixel_update=R.PIXEL_JACOBI), dict(omega=0.7, pixel_gs=0), 8),
+                "block_sor_1.8": (dict(omega=1.8, pixel_update=R.PIXEL_BLOCK), dict(omega=1.8, pixel_gs=2), 0),
+                "block_sor_1.8+coarse8": (dict(omega=1.8, pixel_update=R.PIXEL_BLOCK), dict(omega=1.8, pixel_gs=2), 8)}
+    base = dict(f=525.0, cx=319.5, cy=239.5, n_outer=15, n_sor=50, min_width=40)
+    ctx = R.Context(0)
+    for (lw, lh) in R.pyramid_levels(W, H, 0.5, 40):
+        ctx.autotune_relax(lw, lh, batch=n)
+    truth = [(p["truth"][..., 0], p["truth"][..., 1], p["truth"][..., 2]) for p in pairs]
+    out, f64 = [], {}
+    for name, (grule, orule, levels) in variants.items():
+        p = R.make_params(W, H, relax_variant=R.RELAX_TILED, graph=True, coarse_levels=levels, **grule, **base)
+        ctx.batch_load(p, pairs)
+        st = ctx.batch_run(p, stats=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.batch_run(p, stats=False)
+        ctx.synchronize()
+        pps = steps * n / (time.perf_counter() - t0)
+        flows = [ctx.batch_read(i) for i in range(n)]
+        t0 = time.perf_counter()
+        o = []
+        for pr in pairs:
+            po = O.make_params(W, H, precision=O.PREC_F64, cg_levels=levels, **orule, **base)
+            u, v, w, _ = O.multiscale(pr["im1"], pr["dm1"], pr["im2"], pr["dm2"], pr["sigma1"], None, po)
+            o.append((u, v, w))
+        f64[name] = o
+        L = st[0].n_levels
+        out.append({"variant": name, "pairs_per_s": round(pps, 2),
+                    "sweeps_exec_pair0": [st[0].sweeps_exec[k] for k in range(L)],
+                    "vs_fp64_oracle_same_variant": epe_stats(flows, o), "vs_truth": epe_stats(flows, truth)})
+        print(f"coarse: {name}: {pps:.1f} pairs/s, FP64 oracle {time.perf_counter() - t0:.1f} s", flush=True)
+    for r in out:
+        r["vs_fp64_oracle_reference_rule"] = epe_stats([f for f in f64[r["variant"]]], f64["reference"])
+    ctx.close()
+    return dict(workload=f"{n} x 640x480 pairs (seeds 21..{20 + n}, 6 segments), 5-level pyramids ratio 0.5, "
+                         "15 outer x 50 inner, HPB regularisation, boundary maps made per level from the frames",
+                note="vs_fp64_oracle_reference_rule compares the variant's FP64 oracle flow with the reference "
+                     "rule's FP64 oracle flow (what the variant changes, free of f32 rounding)", variants=out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--pairs", type=int, default=8)
     ap.add_argument("--out", default="")
     ap.add_argument("--no-served", action="store_true", help="skip the served warm-start comparison")
     ap.add_argument("--served-only", action="store_true", help="only the served warm-start comparison")
+    ap.add_argument("--coarse-only", action="store_true",
+                    help="only the coarse-grid correction report (boundary maps made from the frames)")
     ap.add_argument("--threads", type=int, default=0, help="oracle threads (0: the cores this process may use)")
     args = ap.parse_args()
     R = load_package()
     O = oracle()
     nt = args.threads or min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "64") or 64))
     O.set_threads(max(1, nt))
-    if args.served_only:
-        rep = served_vs_fp64(R, O)
+    if args.served_only or args.coarse_only:
+        rep = served_vs_fp64(R, O) if args.served_only else coarse_report(R, O)
         txt = json.dumps(rep, indent=1)
         print(txt)
         if args.out:
