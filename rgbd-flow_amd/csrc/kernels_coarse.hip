@@ -33,64 +33,64 @@ __device__ __forceinline__ void coefs(const float4* base, int storage, size_t N,
   load_record(base, storage, N, l, c, y);
 }
 
-// g_right (down = false) or g_down (down = true) of coarse pixel (I, J), xy
-// (z = 0) and z (z = 1) weights: rows 2I, 2I+1 of column 2J+1's right edges,
-// or columns 2J, 2J+1 of row 2I+1's down edges; zero at the level's edge.
-__device__ __forceinline__ void cross(const LevelView::Finer& f, int Wc, int Hc, int I, int J, bool down,
-                                      float (&g)[2]) {
-  const size_t N = (size_t)f.W * f.H;
-  g[0] = g[1] = 0.f;
-  float a[kCoef];
-  if (!down) {
-    if (J + 1 >= Wc) return;
-    const size_t p = (size_t)(2 * I) * f.W + 2 * J + 1;
-    coefs(f.coef, f.storage, N, p, a);
-    g[0] += a[kERxy];
-    g[1] += a[kERz];
-    if (2 * I + 1 < f.H) {
-      coefs(f.coef, f.storage, N, p + f.W, a);
-      g[0] += a[kERxy];
-      g[1] += a[kERz];
-    }
-  } else {
-    if (I + 1 >= Hc) return;
-    const size_t p = (size_t)(2 * I + 1) * f.W + 2 * J;
-    coefs(f.coef, f.storage, N, p, a);
-    g[0] += a[kEDxy];
-    g[1] += a[kEDz];
-    if (2 * J + 1 < f.W) {
-      coefs(f.coef, f.storage, N, p + 1, a);
-      g[0] += a[kEDxy];
-      g[1] += a[kEDz];
-    }
+// The four edge weights (eRxy, eRz, eDxy, eDz) of pixel l: the first part
+// plane of an f32 record, else the decoded record.
+__device__ __forceinline__ void edges(const float4* base, int storage, size_t N, size_t l, float (&e)[4]) {
+  if (storage == kStoreF32) {
+    const float4 q = gload(base + l);
+    e[0] = q.x;
+    e[1] = q.y;
+    e[2] = q.z;
+    e[3] = q.w;
+    return;
   }
+  float c[kCoef];
+  coefs(base, storage, N, l, c);
+  e[0] = c[kERxy];
+  e[1] = c[kERz];
+  e[2] = c[kEDxy];
+  e[3] = c[kEDz];
 }
 
-// The residual r - sum_n e_n x_n - B x of a pixel of the finer level for its
-// three unknowns, each in f32 in the order left, up, right, down, then the
-// block's row (missing neighbours skipped).
-__device__ __forceinline__ void residual3(const LevelView::Finer& f, const float4* x, int i, int j, float (&res)[3]) {
-  const int W = f.W, H = f.H, N = W * H, l = i * W + j;
-  const bool hl = j > 0, hu = i > 0, hr = j < W - 1, hd = i < H - 1;
-  const PixCoef c = load_coef(f.coef, f.storage, W, N, l, hl, hu);
-  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-  const float4 o = gload(x + l), L = hl ? gload(x + l - 1) : zero, U = hu ? gload(x + l - W) : zero,
-               R = hr ? gload(x + l + 1) : zero, D = hd ? gload(x + l + W) : zero;
-  const float xo[3] = {o.x, o.y, o.z}, xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z}, xr[3] = {R.x, R.y, R.z},
-              xd[3] = {D.x, D.y, D.z};
-  const float B[3][3] = {{c.Buu, c.Buv, c.Buw}, {c.Buv, c.Bvv, c.Bvw}, {c.Buw, c.Bvw, c.Bww}};
-  const float rhs[3] = {c.ru, c.rv, c.rw};
+// An aggregate's children (2I,2J), (2I,2J+1), (2I+1,2J), (2I+1,2J+1) with
+// their records (zero where a child is past the level's edge) and the edge
+// weights reaching them from outside: the right edges of column 2J-1 (rows
+// 2I, 2I+1) and the down edges of row 2I-1 (columns 2J, 2J+1), zero where
+// those pixels do not exist.
+struct Aggregate {
+  bool r1, c1;         // second row / column present
+  size_t p00;          // pixel index of (2I, 2J)
+  float a[4][kCoef];
+  float left[2][2];    // [row][xy, z]: right edges of (2I + row, 2J - 1)
+  float up[2][2];      // [column][xy, z]: down edges of (2I - 1, 2J + column)
+};
+
+__device__ __forceinline__ void load_aggregate(const LevelView::Finer& f, int I, int J, Aggregate& g) {
+  const size_t N = (size_t)f.W * f.H;
+  const int i0 = 2 * I, j0 = 2 * J;
+  g.r1 = i0 + 1 < f.H;
+  g.c1 = j0 + 1 < f.W;
+  g.p00 = (size_t)i0 * f.W + j0;
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    float a = rhs[k];
-    if (hl) a -= (k < 2 ? c.lxy : c.lz) * xl[k];
-    if (hu) a -= (k < 2 ? c.uxy : c.uz) * xu[k];
-    if (hr) a -= (k < 2 ? c.rxy : c.rz) * xr[k];
-    if (hd) a -= (k < 2 ? c.dxy : c.dz) * xd[k];
-    a -= B[k][0] * xo[0];
-    a -= B[k][1] * xo[1];
-    a -= B[k][2] * xo[2];
-    res[k] = a;
+  for (int d = 0; d < 4; d++) {
+    const bool has = ((d & 1) ? g.c1 : true) && ((d & 2) ? g.r1 : true);
+    if (has) {
+      coefs(f.coef, f.storage, N, g.p00 + (d >> 1) * f.W + (d & 1), g.a[d]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kCoef; k++) g.a[d][k] = 0.f;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    float e[4] = {0.f, 0.f, 0.f, 0.f};
+    if (j0 > 0 && (r == 0 || g.r1)) edges(f.coef, f.storage, N, g.p00 + r * f.W - 1, e);
+    g.left[r][0] = e[0];
+    g.left[r][1] = e[1];
+    float u[4] = {0.f, 0.f, 0.f, 0.f};
+    if (i0 > 0 && (r == 0 || g.c1)) edges(f.coef, f.storage, N, g.p00 - f.W + r, u);
+    g.up[r][0] = u[2];
+    g.up[r][1] = u[3];
   }
 }
 
@@ -116,21 +116,32 @@ __global__ void __launch_bounds__(kBlock) k_cg_operator(Views w) {
   const LevelView::Finer& f = v->finer;
   if (f.top_ctl->outer_done) return;
   const int Wc = v->W, Hc = v->H;
-  const size_t N = (size_t)f.W * f.H;
-  const int i0 = 2 * I, j0 = 2 * J;
-  const bool r1 = i0 + 1 < f.H, c1 = j0 + 1 < f.W;
-  const size_t p00 = (size_t)i0 * f.W + j0;
-  // children (2I,2J), (2I,2J+1), (2I+1,2J), (2I+1,2J+1)
-  float a00[kCoef], a01[kCoef] = {}, a10[kCoef] = {}, a11[kCoef] = {};
-  coefs(f.coef, f.storage, N, p00, a00);
-  if (c1) coefs(f.coef, f.storage, N, p00 + 1, a01);
-  if (r1) coefs(f.coef, f.storage, N, p00 + f.W, a10);
-  if (r1 && c1) coefs(f.coef, f.storage, N, p00 + f.W + 1, a11);
-  float gr[2], gd[2], gl[2] = {0.f, 0.f}, gu[2] = {0.f, 0.f};
-  cross(f, Wc, Hc, I, J, false, gr);
-  cross(f, Wc, Hc, I, J, true, gd);
-  if (J > 0) cross(f, Wc, Hc, I, J - 1, false, gl);
-  if (I > 0) cross(f, Wc, Hc, I - 1, J, true, gu);
+  Aggregate g;
+  load_aggregate(f, I, J, g);
+  const bool r1 = g.r1, c1 = g.c1;
+  // g_right: the right edges of the children in column 2J+1; g_down: the
+  // down edges of the children in row 2I+1 (zero at the level's right /
+  // lower edge); g_left, g_up: the left / upper aggregates' g_right, g_down
+  float gr[2], gd[2], gl[2], gu[2];
+#pragma unroll
+  for (int z = 0; z < 2; z++) {
+    gr[z] = 0.f;
+    if (J + 1 < Wc) {
+      gr[z] += g.a[1][kERxy + z];
+      if (r1) gr[z] += g.a[3][kERxy + z];
+    }
+    gd[z] = 0.f;
+    if (I + 1 < Hc) {
+      gd[z] += g.a[2][kEDxy + z];
+      if (c1) gd[z] += g.a[3][kEDxy + z];
+    }
+    gl[z] = 0.f;
+    gl[z] += g.left[0][z];
+    if (r1) gl[z] += g.left[1][z];
+    gu[z] = 0.f;
+    gu[z] += g.up[0][z];
+    if (c1) gu[z] += g.up[1][z];
+  }
   float c[kCoef];
   float around[2];
 #pragma unroll
@@ -151,18 +162,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_operator(Views w) {
   for (int b = 0; b < 6; b++) {
     const int k = blk[b];
     float s = 0.f;
-    s += a00[k];
-    if (c1) s += a01[k];
-    if (r1) s += a10[k];
-    if (r1 && c1) s += a11[k];
+    s += g.a[0][k];
+    if (c1) s += g.a[1][k];
+    if (r1) s += g.a[2][k];
+    if (r1 && c1) s += g.a[3][k];
     if (b < 3) {
       // Buu, Bvv: the xy edges; Bww: the z edges.  Inner edges: right of
       // (2I,2J), right of (2I+1,2J), down of (2I,2J), down of (2I,2J+1)
       const int er = b < 2 ? kERxy : kERz, ed = b < 2 ? kEDxy : kEDz;
-      if (c1) s += 2.f * a00[er];
-      if (c1 && r1) s += 2.f * a10[er];
-      if (r1) s += 2.f * a00[ed];
-      if (r1 && c1) s += 2.f * a01[ed];
+      if (c1) s += 2.f * g.a[0][er];
+      if (c1 && r1) s += 2.f * g.a[2][er];
+      if (r1) s += 2.f * g.a[0][ed];
+      if (r1 && c1) s += 2.f * g.a[1][ed];
       s += 0.5f * around[b < 2 ? 0 : 1];
     }
     c[k] = s;
@@ -186,16 +197,50 @@ __global__ void __launch_bounds__(kBlock) k_cg_restrict(Views w, int finer_alt, 
   if (blk == w.start[work_item(w, blk)] && threadIdx.x == 0) v->ctl->outer_done = stopped ? 1 : 0;
   if (!in || stopped) return;
   const float4* x = finer_alt ? f.du_alt : f.du;
+  Aggregate g;
+  load_aggregate(f, I, J, g);
+  const int W = f.W, H = f.H, i0 = 2 * I, j0 = 2 * J;
+  // the unknowns the children's stencils reach: rows i0-1 .. i0+2, columns
+  // j0-1 .. j0+2 (not the corners), zero outside the level
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 X[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = i0 - 1 + r, j = j0 - 1 + q;
+      const bool corner = (r == 0 || r == 3) && (q == 0 || q == 3);
+      X[r][q] = (!corner && i >= 0 && i < H && j >= 0 && j < W) ? gload(x + (size_t)i * W + j) : zero;
+    }
+  // each child's residual r - sum_n e_n x_n - B x for its three unknowns, in
+  // f32 in the order left, up, right, down, then the block's row (missing
+  // neighbours skipped), summed over the children in order
   float s[3] = {0.f, 0.f, 0.f};
 #pragma unroll
   for (int d = 0; d < 4; d++) {
-    const int i = 2 * I + (d >> 1), j = 2 * J + (d & 1);
-    if (i < f.H && j < f.W) {
-      float r[3];
-      residual3(f, x, i, j, r);
-      s[0] += r[0];
-      s[1] += r[1];
-      s[2] += r[2];
+    const int dr = d >> 1, dc = d & 1, i = i0 + dr, j = j0 + dc;
+    if (i >= H || j >= W) continue;
+    const float(&c)[kCoef] = g.a[d];
+    const bool hl = j > 0, hu = i > 0, hr = j < W - 1, hd = i < H - 1;
+    // the left / up edges: a sibling's right / down edges, or the outside ones
+    const float lxy = dc ? g.a[d - 1][kERxy] : g.left[dr][0], lz = dc ? g.a[d - 1][kERz] : g.left[dr][1];
+    const float uxy = dr ? g.a[d - 2][kEDxy] : g.up[dc][0], uz = dr ? g.a[d - 2][kEDz] : g.up[dc][1];
+    const float4 o = X[1 + dr][1 + dc], L = X[1 + dr][dc], U = X[dr][1 + dc], R = X[1 + dr][2 + dc],
+                 D = X[2 + dr][1 + dc];
+    const float xo[3] = {o.x, o.y, o.z}, xl[3] = {L.x, L.y, L.z}, xu[3] = {U.x, U.y, U.z},
+                xr[3] = {R.x, R.y, R.z}, xd[3] = {D.x, D.y, D.z};
+    const float B[3][3] = {{c[kBuu], c[kBuv], c[kBuw]}, {c[kBuv], c[kBvv], c[kBvw]}, {c[kBuw], c[kBvw], c[kBww]}};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      float a = c[kRu + k];
+      if (hl) a -= (k < 2 ? lxy : lz) * xl[k];
+      if (hu) a -= (k < 2 ? uxy : uz) * xu[k];
+      if (hr) a -= (k < 2 ? c[kERxy] : c[kERz]) * xr[k];
+      if (hd) a -= (k < 2 ? c[kEDxy] : c[kEDz]) * xd[k];
+      a -= B[k][0] * xo[0];
+      a -= B[k][1] * xo[1];
+      a -= B[k][2] * xo[2];
+      s[k] += a;
     }
   }
   const size_t L = (size_t)I * v->W + J, Nc = (size_t)v->N;
