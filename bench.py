@@ -38,7 +38,7 @@ B_SWEEP = 72.0     # algorithmic bytes per pixel per red-black iteration (f32 sy
 B_SWEEP_16 = 48.0  # the same with the 12 system values stored in 16 bits
 B_STAGE = 128.0    # algorithmic bytes per pixel per outer iteration (warp, gradients, weights, assembly)
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "relax_tiled_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "relax_traffic.json")
 T_START = time.perf_counter()
 
 
