@@ -563,11 +563,21 @@ def mixed_batch(R, args, n, label_maps=False):
             for (w, h) in sorted(set(sizes), key=lambda t: -t[0]):
                 for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
                     ctx.autotune_relax(lw, lh, batch=sizes.count((w, h)), params=p)
+                log(f"mixed batch: f32 schedules of the {w}x{h} levels tuned")
         npx = sum(w * h for (w, h) in sizes)
         for name, st, img in (("f32", R.STORE_F32, R.IMAGES_F64), ("fp16", R.STORE_F16, R.IMAGES_F64),
                               ("fp16_images", R.STORE_F16, R.IMAGES_F16)):
+            if name == "fp16" and not args.no_autotune:
+                # the FP16 records' own schedules (the streamed kernel suits f32 records only)
+                p16 = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED,
+                                    storage=R.STORE_F16)
+                for (w, h) in sorted(set(sizes), key=lambda t: -t[0]):
+                    for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
+                        ctx.autotune_relax(lw, lh, batch=sizes.count((w, h)), params=p16)
+                    log(f"mixed batch: FP16 schedules of the {w}x{h} levels tuned")
             p.storage, p.image_storage = st, img
             pps, el, lat, _ = run_batch(ctx, p, pairs, steps, 1, 2)
+            log(f"mixed batch, {name}: {pps:.2f} pairs/s")
             out[name] = {"pairs_per_s": round(pps, 3), "ms_per_step": round(el / steps * 1e3, 3),
                          "megapixels_per_s": round(npx * steps / el / 1e6, 2),
                          "batch_p50_ms": round(float(np.percentile(lat, 50)), 3)}

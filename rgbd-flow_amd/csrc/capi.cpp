@@ -465,6 +465,9 @@ bool stream_width_ok(int TW, int W, int T) {
 // check interval per launch; otherwise cost ~ launches x (launch overhead +
 // waves x per-workgroup work), where the per-workgroup work grows with the
 // region (tile + halo of 2 T) per thread.  cap == 0: nothing fits.
+// A plan request for any non-streamed workgroup shape.
+constexpr int kAnyTiledConfig = -2;
+
 TileSched plan_tiles(const rgbdflow_ctx* c, int W, int H, int req_T, int req_w, int req_h, int req_cfg) {
   TileSched best{};
   double best_cost = 1e300;
@@ -474,6 +477,7 @@ TileSched plan_tiles(const rgbdflow_ctx* c, int W, int H, int req_T, int req_w, 
     int nt, p;
     tiled_config_shape(cfg, &nt, &p);
     const int tr = tiled_config_stream(cfg);
+    if (req_cfg == kAnyTiledConfig && tr > 0) continue;
     if (tr > 0) {
       // streamed: a tile's height is free; its width plus the halo must fit
       // a region row.  Cost ~ launches x (launch overhead + waves of tiles x
@@ -576,7 +580,14 @@ void plan_step_relax(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<Leve
   for (size_t i = 1; i < items.size(); i++)
     if (items[i].N > items[lead].N) lead = i;
   const int LW = items[lead].W, LH = items[lead].H;
-  const TileSched lt = cached_plan(c, LW, LH, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
+  TileSched lt = cached_plan(c, LW, LH, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
+  // items of several sizes: tiled schedules only (a streamed workgroup's
+  // time is its tiles' rows plus a ring fill and drain, which smaller items'
+  // shorter tiles do not amortise)
+  bool uniform = true;
+  for (const LevelView& v : items) uniform = uniform && v.W == LW && v.H == LH;
+  if (!uniform && lt.cap != 0 && tiled_config_stream(lt.config) > 0)
+    lt = cached_plan(c, LW, LH, p->tile_iters, p->tile_w, p->tile_h, kAnyTiledConfig);
   if (lt.cap == 0) return;
   std::vector<TileSched> ts(items.size());
   for (size_t i = 0; i < items.size(); i++) {
@@ -2250,8 +2261,8 @@ int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, const rgbd
   hipEvent_t e0, e1;
   HIP_TRY(c, hipEventCreate(&e0));
   HIP_TRY(c, hipEventCreate(&e1));
-  double best_us = 1e300;
-  TileSched best{};
+  double best_us = 1e300, best_tiled_us = 1e300;
+  TileSched best{}, best_tiled{};
   int rc = RGBDFLOW_OK;
   for (const TileSched& ts : cand) {
     double us = 0;
@@ -2261,11 +2272,17 @@ int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, const rgbd
       best_us = us;
       best = ts;
     }
+    if (tiled_config_stream(ts.config) == 0 && us < best_tiled_us) {
+      best_tiled_us = us;
+      best_tiled = ts;
+    }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (rc != RGBDFLOW_OK) return rc;
   c->plans[plan_key(W, H, 0, 0, 0, -1)] = best;
+  // the best tiled schedule too, for steps whose items have several sizes
+  if (best_tiled.cap != 0) c->plans[plan_key(W, H, 0, 0, 0, kAnyTiledConfig)] = best_tiled;
   c->plans_gen++;
   if (out) choice_from_schedule(best, (int)cand.size(), best_us, out);
   return RGBDFLOW_OK;
@@ -2296,6 +2313,7 @@ int rgbdflow_set_relax_schedule(rgbdflow_ctx* c, int W, int H, const rgbdflow_ti
   TileSched t;
   TRY(schedule_from_choice(c, W, H, ch, &t));
   c->plans[plan_key(W, H, 0, 0, 0, -1)] = t;
+  if (tiled_config_stream(t.config) == 0) c->plans[plan_key(W, H, 0, 0, 0, kAnyTiledConfig)] = t;
   c->plans_gen++;
   return RGBDFLOW_OK;
 }
