@@ -85,8 +85,13 @@ struct StepPlan {
   int relax_mode = kRelaxFixed;    // RelaxMode of the step's items
   bool any_early = false;
   // coarse-grid correction: the coarse levels 1, 2, ... of the step's items
-  // (level l holds the items that have it)
-  std::vector<StepPlan> cg;
+  // (level l holds the items that have it; their operators), the same
+  // levels' items that relax level by level (the rest of each item's levels
+  // fit in LDS and run in k_cg_bottom), and the k_cg_bottom work-list
+  std::vector<StepPlan> cg, cg_nf;
+  size_t cg_bottom_off = 0;
+  int cg_bottom_n = 0;
+  Views vbottom{};
 };
 
 struct BatchPlan {
@@ -659,16 +664,25 @@ void pack_step(std::vector<char>& host, const std::vector<LevelView>& items, Ste
 void pack_coarse(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<char>& host,
                  const std::vector<LevelView>& items, const std::vector<const Work*>& works, StepPlan& st) {
   st.cg.clear();
+  st.cg_nf.clear();
+  st.cg_bottom_n = 0;
   if (!p || p->coarse_levels <= 0) return;
   for (const Work* w : works)
     if (!w->cg_coef) return;  // no buffers: the caller allocates them before planning
-  std::vector<std::vector<LevelView>> lv;
+  std::vector<std::vector<LevelView>> lv, nf;
+  std::vector<LevelView> bottom;
   for (size_t i = 0; i < items.size(); i++) {
     const LevelView& fv = items[i];
     const Work& w = *works[i];
-    LevelView::Finer fin{fv.W, fv.H, fv.storage, fv.coef, fv.du, fv.du_alt, fv.ctl, fv.ctl, 1};
+    LevelView::Finer fin{fv.W, fv.H, fv.storage, fv.coef, fv.du, fv.du_alt, fv.ctl, fv.ctl, 1, 0, 0};
     size_t off = 0;
     const std::vector<LevelShape> L = cg_shapes(fv.W, fv.H, p->coarse_levels);
+    // the levels from `fused` down fit in one workgroup's LDS
+    size_t fused = L.size(), tail = 0;
+    while (fused > 0 && tail + (size_t)L[fused - 1].W * L[fused - 1].H <= (size_t)kCgFusePx) {
+      fused--;
+      tail += (size_t)L[fused].W * L[fused].H;
+    }
     for (size_t l = 0; l < L.size(); l++) {
       LevelView v;
       std::memset(&v, 0, sizeof(v));
@@ -686,9 +700,17 @@ void pack_coarse(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<char>& h
       v.omega_c = fv.omega_c;
       v.pixel_gs = fv.pixel_gs;
       v.finer = fin;
-      fin = LevelView::Finer{v.W, v.H, RGBDFLOW_STORE_F32, v.coef, v.du, v.du_alt, v.ctl, fv.ctl, 0};
+      fin = LevelView::Finer{v.W, v.H, RGBDFLOW_STORE_F32, v.coef, v.du, v.du_alt, v.ctl, fv.ctl, 0, (int)l + 1, 0};
       if (lv.size() <= l) lv.resize(l + 1);
       lv[l].push_back(v);
+      if (l < fused) {
+        if (nf.size() <= l) nf.resize(l + 1);
+        nf[l].push_back(v);
+      } else if (l == fused) {
+        LevelView b = v;
+        b.finer.fused = (int)(L.size() - fused);
+        bottom.push_back(b);
+      }
       off += (size_t)v.N;
     }
   }
@@ -699,9 +721,15 @@ void pack_coarse(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<char>& h
   q.tile_config = -1;
   for (std::vector<LevelView>& li : lv) {
     st.cg.emplace_back();
-    plan_step_relax(c, &q, li, st.cg.back());
     pack_step(host, li, st.cg.back());
   }
+  for (std::vector<LevelView>& li : nf) {
+    st.cg_nf.emplace_back();
+    plan_step_relax(c, &q, li, st.cg_nf.back());
+    pack_step(host, li, st.cg_nf.back());
+  }
+  st.cg_bottom_n = (int)bottom.size();
+  if (!bottom.empty()) st.cg_bottom_off = pack(host, bottom.data(), bottom.size() * sizeof(LevelView));
 }
 
 void bind_step(char* dev, StepPlan& st) {
@@ -711,6 +739,8 @@ void bind_step(char* dev, StepPlan& st) {
   st.vtiled = Views{items, reinterpret_cast<const int*>(dev + st.tiled_off), st.n};
   st.votile = Views{items, reinterpret_cast<const int*>(dev + st.otile_off), st.n};
   for (StepPlan& cs : st.cg) bind_step(dev, cs);
+  for (StepPlan& cs : st.cg_nf) bind_step(dev, cs);
+  st.vbottom = Views{reinterpret_cast<const LevelView*>(dev + st.cg_bottom_off), nullptr, st.cg_bottom_n};
 }
 
 int upload(rgbdflow_ctx* c, void* dst, const void* src, size_t bytes) {
@@ -784,20 +814,24 @@ int coarse_sweeps(rgbdflow_ctx* c, const StepPlan& cs, int n, int alt) {
 
 // One V-cycle of the coarse-grid correction on the relaxed level's unknowns
 // (in du_alt if alt): down, the coarse rhs from the finer level's residual
-// and the coarse level's sweeps; up, the sweeps again and the correction
-// added to the finer level.  n_prev: checks before it (the stop test).
+// and the coarse level's sweeps; the levels that fit in LDS in one
+// k_cg_bottom launch; up, the sweeps again and the correction added to the
+// finer level.  n_prev: checks before it (the stop test).
 void enqueue_vcycle(rgbdflow_ctx* c, const StepPlan& st, const SolveScalars& k, int slot_base, int n_prev, int alt) {
   hipStream_t s = c->stream;
-  const int L = (int)st.cg.size();
+  const int L = (int)st.cg_nf.size();
   int where[kMaxCg + 1];
   where[0] = alt;
+  int mask = alt;
   for (int l = 1; l <= L; l++) {
-    const StepPlan& cs = st.cg[l - 1];
+    const StepPlan& cs = st.cg_nf[l - 1];
     launch_cg_restrict(s, cs.vpix, cs.pix, where[l - 1], slot_base, n_prev);
     where[l] = coarse_sweeps(c, cs, k.cg_sweeps, 0);
+    mask |= where[l] << l;
   }
+  launch_cg_bottom(s, st.vbottom, mask, slot_base, n_prev, k.cg_sweeps);
   for (int l = L; l >= 1; l--) {
-    const StepPlan& cs = st.cg[l - 1];
+    const StepPlan& cs = st.cg_nf[l - 1];
     where[l] = coarse_sweeps(c, cs, k.cg_sweeps, where[l]);
     launch_cg_prolong(s, cs.vpix, cs.pix, where[l], where[l - 1]);
   }

@@ -21,6 +21,8 @@ constexpr int kCheckEvery = 25;     // relaxation convergence check period (.cu:
 constexpr int kMaxChecks = 4096;    // check slots per outer iteration
 constexpr int kMaxOuter = 1024;
 constexpr int kMaxCg = 8;          // coarse levels of the coarse-grid correction
+constexpr int kCgFusePx = 2048;    // pixels of the coarse levels one workgroup relaxes in LDS
+constexpr int kCgBottomThreads = 1024;
 constexpr int kBlock = 256;         // threads of the per-pixel kernels
 
 // The compact system of one pixel p, in the order of the 13-plane export
@@ -123,6 +125,8 @@ struct LevelView {
     const SolveCtl* ctl;      // the finer level's control (coarse levels: outer_done = skip)
     const SolveCtl* top_ctl;  // the relaxed level's control
     int first;                // 1: the finer level is the relaxed level itself
+    int level;                // the finer level's index (0: the relaxed level, l: coarse level l)
+    int fused;                // on a view of k_cg_bottom: the coarse levels it covers (this and below)
   } finer;
 };
 
@@ -346,6 +350,10 @@ void launch_copy_du(hipStream_t s, const Views& v, int blocks);
 void launch_cg_operator(hipStream_t s, const Views& v, int blocks);
 void launch_cg_restrict(hipStream_t s, const Views& v, int blocks, int finer_alt, int slot_base, int n_prev_checks);
 void launch_cg_prolong(hipStream_t s, const Views& v, int blocks, int alt, int finer_alt);
+// the coarse levels from the first whose remaining levels fit kCgFusePx down,
+// one workgroup per item (v.n items); alt_mask bit l: level l's unknowns are
+// in du_alt
+void launch_cg_bottom(hipStream_t s, const Views& v, int alt_mask, int slot_base, int n_prev_checks, int sweeps);
 __host__ __device__ inline int cg_coarse_ok(int W, int H) { return W >= 4 && H >= 4; }
 void launch_fill_test_system(hipStream_t s, float4* coef, float4* du, int W, int H, int storage);
 // f32 records -> records in `storage` (src and dst distinct)
