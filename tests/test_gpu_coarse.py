@@ -144,6 +144,46 @@ def test_uneven_batch_and_graph_replays(ctx, R):
             assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb), (run, i)
 
 
+@pytest.mark.parametrize("case", ["two_inner", "residual_exit"])
+def test_level_solve_inner_iterations_and_residual_exit(ctx, R, case):
+    """Several inner iterations (the coarse operators made again for each
+    assembled system) and the FP64 residual exit (its checks between the
+    V-cycles)."""
+    O = oracle()
+    W, H = 96, 72
+    li, (u, v, w), _ = level_inputs(seed=31, W=W, H=H)
+    if case == "two_inner":
+        okw, gkw = dict(n_outer=3, n_inner=2, n_sor=30), dict(n_outer=3, n_inner=2, n_sor=30)
+    else:
+        okw = dict(n_outer=3, n_sor=75, residual_exit=True, residual_tol=0.05)
+        gkw = dict(n_outer=3, n_sor=75, inner_exit=R.EXIT_RESIDUAL, residual_tol=0.05)
+    po = O.make_params(W, H, cg_levels=8, **okw)
+    pg = R.make_params(W, H, coarse_levels=8, relax_variant=R.RELAX_TILED, **gkw)
+    print(f"level solve {W}x{H} {case}", flush=True)
+    ua, va, wa, sa = O.level_solve(li, po, True, u, v, w)
+    ub, vb, wb, sb = ctx.solve_level(li, pg, True, u, v, w)
+    assert sb.sweeps_exec[0] == sa.sweeps_exec
+    assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
+
+
+def test_with_init_matches_oracle(ctx, R):
+    """sceneFlowWithInit (one full-resolution level from an estimate) with
+    the correction."""
+    O = oracle()
+    W, H = 128, 96
+    pair, im1, d1, im2, d2, sigma = make_case(W, H, 14, 4)
+    hpb, _ = hpb_for(pair, W, H, 0.5, 16)
+    hpb0 = hpb[: W * H * 2]
+    est = pair["flow"].astype(np.float64) * 0.9
+    po = O.make_params(W, H, n_outer=3, n_sor=40, cg_levels=8)
+    pg = R.make_params(W, H, n_outer=3, n_sor=40, coarse_levels=8, relax_variant=R.RELAX_TILED)
+    print(f"with_init {W}x{H}", flush=True)
+    ua, va, wa, sa = O.with_init(im1, d1, im2, d2, sigma, hpb0, est, po)
+    ub, vb, wb, sb = ctx.with_init(pg, im1, d1, im2, d2, est, sigma, hpb0)
+    assert sb.sweeps_exec[0] == sa.sweeps_exec[0]
+    assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
+
+
 def test_bad_coarse_params_are_rejected(ctx, R):
     li, (u, v, w), _ = level_inputs(seed=9, W=32, H=24)
     for kw in (dict(coarse_levels=9), dict(coarse_levels=-1), dict(coarse_levels=2, coarse_sweeps=0),

@@ -32,7 +32,8 @@ def _maps(R, S, pair, W, H, mw, shift):
 
 def _alone(R, ctx, p, W, H, mw, f1, f2, hpb1):
     q = R.make_params(W, H, n_outer=p.n_outer, n_sor=p.n_sor, min_width=mw, relax_variant=p.relax_variant,
-                      storage=p.storage, image_storage=p.image_storage)
+                      storage=p.storage, image_storage=p.image_storage, coarse_levels=p.coarse_levels,
+                      coarse_sweeps=p.coarse_sweeps, coarse_at=p.coarse_at)
     im1 = f1[0].astype(np.float64) / 255.0
     im2 = f2[0].astype(np.float64) / 255.0
     u, v, w, _ = ctx.multiscale(q, im1, f1[1].astype(np.float64), im2, f2[1].astype(np.float64), None, hpb1)
@@ -40,14 +41,18 @@ def _alone(R, ctx, p, W, H, mw, f1, f2, hpb1):
 
 
 @pytest.mark.parametrize("max_batch,lanes", [(8, 2), (2, 2), (1, 1)])
-@pytest.mark.parametrize("mode", ["maps", "device_maps", "fp16"])
+@pytest.mark.parametrize("mode", ["maps", "device_maps", "fp16", "coarse"])
 def test_serving_results_equal_alone(R, max_batch, lanes, mode):
     """mode: boundary maps submitted with the frames; made on the device
     (frames submitted without maps; the pair alone is loaded without them
-    too); or FP16 records and feature images with device maps."""
+    too); FP16 records and feature images with device maps; or the
+    coarse-grid correction with device maps (batches of several sizes share
+    its launches)."""
     S = synth()
     sizes = [(128, 96, 16), (160, 120, 20), (96, 72, 12)]
     extra = dict(storage=R.STORE_F16, image_storage=R.IMAGES_F16) if mode == "fp16" else {}
+    if mode == "coarse":
+        extra = dict(coarse_levels=8, coarse_sweeps=2, coarse_at=4)
     p = R.make_params(128, 96, n_outer=3, n_sor=25, min_width=16, relax_variant=R.RELAX_TILED, **extra)
     streams = []
     with R.Server(p, max_batch=max_batch, lanes=lanes) as srv:
