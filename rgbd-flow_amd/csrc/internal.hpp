@@ -364,7 +364,16 @@ void launch_deinterleave(hipStream_t s, const double* src, int N, int C, double*
 void launch_interleave(hipStream_t s, const double* src, int N, int C, double* dst);
 void launch_pair_prep(hipStream_t s, const WorkList<PairPrep>& w, int blocks);
 void launch_pyramid_jobs(hipStream_t s, const WorkList<PyrJob>& w, int blocks);
-inline int pyramid_job_blocks(const PyrJob& j) { return pix_blocks(j.Wd * j.Hd * j.C); }
+// a ratio-0.5 level is made in output tiles of kPyrTW x kPyrTH (one per
+// block and channel), other ratios one output sample per thread
+constexpr int kPyrTW = 32, kPyrTH = kBlock / kPyrTW;
+__host__ __device__ inline bool pyr_halving(const PyrJob& j) {
+  return j.ratio == 0.5 && 2 * j.Hd <= j.Hs + 1 && 2 * j.Wd <= j.Ws + 1;
+}
+inline int pyramid_job_blocks(const PyrJob& j) {
+  return pyr_halving(j) ? ((j.Wd + kPyrTW - 1) / kPyrTW) * ((j.Hd + kPyrTH - 1) / kPyrTH) * j.C
+                        : pix_blocks(j.Wd * j.Hd * j.C);
+}
 void launch_features(hipStream_t s, const Views& v, int blocks, int img);
 void launch_level_entry(hipStream_t s, const Views& v, int blocks, int img);
 void launch_level_weights(hipStream_t s, const Views& v, int blocks);

@@ -66,12 +66,47 @@ __global__ void k_interleave(const double* src, int N, int C, double* dst) {
 // (ImageProcessing.h:203-222).  Only the smoothed samples the resize reads are
 // evaluated; taps of zero bilinear weight are skipped (adding x*0 leaves the
 // sum unchanged).
+// At ratio 0.5 the resize reads one smoothed sample per output, (2i+1, 2j+1)
+// with weight 1 (the others' weights are 0), so a block makes a 32 x 8 output
+// tile of one channel in two passes: the horizontal sums at the outputs'
+// columns for every source row the tile's vertical sums reach (into LDS),
+// then the vertical sums -- the same sums in the same order as per sample.
 __global__ void __launch_bounds__(kBlock) k_pyramid_jobs(WorkList<PyrJob> w) {
+  __shared__ double th[(2 * kPyrTH + kMaxTaps - 1) * kPyrTW];
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int item = work_item(w, blk);
   const PyrJob& J = w.items[item];
-  const int l = (blk - w.start[item]) * kBlock + threadIdx.x;
   const int Ws = J.Ws, Hs = J.Hs, Wd = J.Wd, Hd = J.Hd, fs = J.fs;
+  if (pyr_halving(J)) {
+    const int lb = blk - w.start[item];
+    const int tx = (Wd + kPyrTW - 1) / kPyrTW, nt = tx * ((Hd + kPyrTH - 1) / kPyrTH);
+    const int c = lb / nt, tt = lb - c * nt;
+    const int j0 = (tt % tx) * kPyrTW, i0 = (tt / tx) * kPyrTH;
+    const double* s = J.src + (size_t)c * Ws * Hs;
+    // rows 2 i0 - fs .. 2 (i0 + kPyrTH) - 1 + fs (each clamped as the per-sample sums clamp it)
+    const int rbase = 2 * i0 - fs, nr = 2 * kPyrTH + 2 * fs;
+    for (int t = threadIdx.x; t < nr * kPyrTW; t += kBlock) {
+      const int rr = t / kPyrTW, jj = t - rr * kPyrTW, j = j0 + jj;
+      double h = 0;
+      if (j < Wd) {
+        const int ii = enforce_range(rbase + rr, Hs), uu = enforce_range(2 * j + 1, Ws);
+        for (int lh = -fs; lh <= fs; lh++) h += s[(size_t)ii * Ws + enforce_range(uu + lh, Ws)] * J.taps[lh + fs];
+      }
+      th[t] = h;
+    }
+    __syncthreads();
+    const int ty = threadIdx.x / kPyrTW, jj = threadIdx.x - ty * kPyrTW;
+    const int i = i0 + ty, j = j0 + jj;
+    if (i >= Hd || j >= Wd) return;
+    const int vv = enforce_range(2 * i + 1, Hs);
+    double sm = 0;
+    for (int lv = -fs; lv <= fs; lv++) sm += th[(vv + lv - rbase) * kPyrTW + jj] * J.taps[lv + fs];
+    double acc = 0;
+    acc += sm * 1.0;
+    J.dst[(size_t)c * Wd * Hd + (size_t)i * Wd + j] = acc;
+    return;
+  }
+  const int l = (blk - w.start[item]) * kBlock + threadIdx.x;
   const int Nd = Wd * Hd;
   if (l >= Nd * J.C) return;
   const int c = l / Nd;

@@ -37,7 +37,8 @@ def _maxdiff(a, b):
 
 
 @pytest.mark.parametrize("shape,ratio,min_width", [((97, 73, 3), 0.5, 10), ((64, 48, 1), 0.75, 12),
-                                                    ((160, 120, 3), 0.5, 20)])
+                                                    ((160, 120, 3), 0.5, 20), ((1280, 720, 3), 0.5, 40),
+                                                    ((33, 17, 1), 0.5, 3), ((131, 67, 1), 0.5, 4)])
 def test_gaussian_pyramid_exact(ctx, shape, ratio, min_width):
     O = oracle()
     W, H, Cn = shape
