@@ -547,7 +547,8 @@ def run_batch(ctx, p, batch, steps, warmup, latency_steps):
 def mixed_batch(R, args, n, label_maps=False):
     """n pairs of mixed sizes in one batch (every level step of every pair in
     one launch per kernel), schedules autotuned per level shape; f32, then
-    FP16 records, then FP16 records and images (the same schedules)."""
+    FP16 records, then FP16 records and images (the same schedules: a step of
+    several sizes takes the tuned tiled ones)."""
     pairs, sizes = mixed_workload(R, n, max(1, args.distinct // 2))
     if not label_maps:
         pairs = [dict(pr, hpb=None) for pr in pairs]
@@ -567,14 +568,6 @@ def mixed_batch(R, args, n, label_maps=False):
         npx = sum(w * h for (w, h) in sizes)
         for name, st, img in (("f32", R.STORE_F32, R.IMAGES_F64), ("fp16", R.STORE_F16, R.IMAGES_F64),
                               ("fp16_images", R.STORE_F16, R.IMAGES_F16)):
-            if name == "fp16" and not args.no_autotune:
-                # the FP16 records' own schedules (the streamed kernel suits f32 records only)
-                p16 = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED,
-                                    storage=R.STORE_F16)
-                for (w, h) in sorted(set(sizes), key=lambda t: -t[0]):
-                    for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
-                        ctx.autotune_relax(lw, lh, batch=sizes.count((w, h)), params=p16)
-                    log(f"mixed batch: FP16 schedules of the {w}x{h} levels tuned")
             p.storage, p.image_storage = st, img
             pps, el, lat, _ = run_batch(ctx, p, pairs, steps, 1, 2)
             log(f"mixed batch, {name}: {pps:.2f} pairs/s")
