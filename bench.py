@@ -223,6 +223,8 @@ def main():
     ap.add_argument("--label-maps", action="store_true",
                     help="boundary maps from the generator's segment labels instead of made on the device")
     ap.add_argument("--no-other-maps", action="store_true", help="skip the batch with the other boundary maps")
+    ap.add_argument("--latency-batch", type=int, default=6,
+                    help="pairs of the latency-bounded batch beside the headline (0: skip)")
     ap.add_argument("--coarse-levels", type=int, default=8,
                     help="levels of the coarse-grid correction run beside the headline (device maps; 0: skip)")
     args = ap.parse_args()
@@ -425,6 +427,23 @@ def main():
                                    "relax_roofline": {"achieved": round(a16i, 1), "frac": round(a16i / HBM_PEAK_GBS, 4),
                                                       "bytes_per_px_iter": B_SWEEP_16}}
         log(f"fp16 storage and images {f16['with_fp16_images']}")
+
+    # a smaller batch of the same pairs, whose every pair finishes within a frame interval at
+    # 30 fps (batch p99 <= 33 ms): f32, and FP16 records and images
+    lat_bounded = None
+    if args.latency_batch > 0 and args.variant == "tiled":
+        Bl = args.latency_batch
+        lat_bounded = {"batch": Bl, "boundary_maps": "device" if not args.label_maps else "generator labels"}
+        for name, st, img in (("f32", "f32", R.IMAGES_F64), ("fp16_images", "f16", R.IMAGES_F16)):
+            pl = params_for(R, args.variant, not args.no_graph, st)
+            pl.image_storage = img
+            if not args.no_autotune and not args.load_schedules:
+                autotune(Bl, pl, record=False)
+            lpps, lel, llat, _ = run_batch(ctx, pl, batch[:Bl], args.steps, args.warmup, max(5, args.latency_steps))
+            lat_bounded[name] = {"pairs_per_s": round(lpps, 3), "speedup_vs_plain_single": speedup(lpps),
+                                 "batch_p50_ms": round(float(np.percentile(llat, 50)), 3),
+                                 "batch_p99_ms": round(float(np.percentile(llat, 99)), 3)}
+        log(f"latency-bounded batch {lat_bounded}")
     ctx.close()
 
     # 64 mixed-size pairs in one batch, f32 and FP16 records and images
@@ -503,6 +522,7 @@ def main():
             "fp16_storage": f16,
             "other_boundary_maps": other_maps,
             "coarse_correction": coarse,
+            "latency_bounded": lat_bounded,
             "mixed_64": mixed,
             "served_16_mixed": served,
         },
