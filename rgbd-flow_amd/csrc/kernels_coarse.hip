@@ -16,8 +16,10 @@
 // finer level's residual summed over each aggregate into the coarse rhs and
 // the coarse unknowns zeroed (k_cg_restrict), relaxation; back up, more
 // relaxation and the coarse unknowns added to their fine pixels
-// (k_cg_prolong).  A coarse level's SolveCtl::outer_done says, for the current
-// V-cycle, that the relaxed level has stopped: then nothing is changed.
+// (k_cg_prolong); the levels that fit in LDS together (kCgFusePx pixels) do
+// their part of the V-cycle in one workgroup per pair (k_cg_bottom).  A
+// coarse level's SolveCtl::outer_done says, for the current V-cycle, that the
+// relaxed level has stopped: then nothing is changed.
 #include <hip/hip_runtime.h>
 
 #include "records.hpp"
