@@ -63,6 +63,31 @@ def test_level_solve_odd_sizes(ctx, R, W, H, sweeps, at):
     assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
 
 
+@pytest.mark.parametrize("W,H", [(96, 72), (131, 67)])
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_level_solve_streamed_shapes(ctx, R, W, H, storage):
+    """Every streamed workgroup shape (the autotuner's usual choice at the
+    finest levels) with launches of 1, 2 and its most iterations, the
+    V-cycle splitting the launches of each inner iteration."""
+    O = oracle()
+    li, (u, v, w), _ = level_inputs(seed=41, W=W, H=H)
+    kw = dict(n_outer=3, n_sor=50)
+    okw = dict(record_storage=1) if storage == "f16" else {}
+    gkw = dict(storage=R.STORE_F16) if storage == "f16" else {}
+    po = O.make_params(W, H, cg_levels=8, cg_sweeps=2, cg_at=7, **okw, **kw)
+    ua, va, wa, sa = O.level_solve(li, po, True, u, v, w)
+    shapes = [(i, sh["streamed"]) for i, sh in enumerate(R.relax_shapes()) if sh["streamed"] > 0]
+    assert shapes
+    for cfg, tr in shapes:
+        for ti in sorted({1, 2, tr}):
+            print(f"level solve {W}x{H} {storage} streamed shape {cfg} {ti} per launch", flush=True)
+            pg = R.make_params(W, H, coarse_levels=8, coarse_sweeps=2, coarse_at=7, relax_variant=R.RELAX_TILED,
+                               tile_config=cfg, tile_iters=ti, **gkw, **kw)
+            ub, vb, wb, sb = ctx.solve_level(li, pg, True, u, v, w)
+            assert sb.sweeps_exec[0] == sa.sweeps_exec, (cfg, ti)
+            assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb), (cfg, ti)
+
+
 @pytest.mark.parametrize("rule", ["gs", "block", "f16", "bf16"])
 def test_level_solve_rules_and_storage(ctx, R, rule):
     """The coarse levels relax with the level's own rule; with 16-bit records
@@ -87,7 +112,7 @@ def test_level_solve_rules_and_storage(ctx, R, rule):
     assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb)
 
 
-@pytest.mark.parametrize("variant", ["plain", "tiled"])
+@pytest.mark.parametrize("variant", ["plain", "tiled", "streamed"])
 def test_pair_with_device_maps_matches_oracle(ctx, R, variant):
     """A whole 320x240 pair (10 x 30) with boundary maps made on the device,
     as sceneFlowMultiscale.cpp:116 makes them."""
@@ -98,6 +123,10 @@ def test_pair_with_device_maps_matches_oracle(ctx, R, variant):
     kw = dict(f=cfg["f"], cx=cfg["cx"], cy=cfg["cy"], n_outer=cfg["n_outer"], n_sor=cfg["n_sor"],
               min_width=cfg["min_width"])
     po = O.make_params(W, H, cg_levels=8, **kw)
+    if variant == "streamed":
+        # the largest streamed shape at every level
+        shape = max((sh["streamed"], i) for i, sh in enumerate(R.relax_shapes()) if sh["streamed"] > 0)[1]
+        kw = dict(kw, tile_config=shape)
     pg = R.make_params(W, H, coarse_levels=8, relax_variant=_variant(R, variant), **kw)
     print(f"pair {W}x{H} device maps {variant}", flush=True)
     ua, va, wa, sa = O.multiscale(im1, d1, im2, d2, sigma, None, po)
