@@ -13,9 +13,10 @@ and produces the short text summaries kept with the project:
       --copy-write <db> --copy-mib M --kernel K --batch B --variant V --out <json> --command C
       HBM bytes per launch of the kernels whose name contains K, each counter
       scaled by the copy calibration (bench.py reads the JSON as roofline.traffic);
-  python rgbd-flow_amd/profiling.py counters <db> [<db> ...] --kernel K --out <json> --command C
+  python rgbd-flow_amd/profiling.py counters <db> [<db> ...] --kernel K --out <json> --command C [--largest N]
       per-dispatch averages of the counters of --pmc runs and the derived
-      utilisations (VALU busy, LDS bank conflicts, waves per SIMD).
+      utilisations (VALU busy, LDS bank conflicts, waves per SIMD); with
+      --largest, over each counter's N largest dispatches.
 """
 from __future__ import annotations
 
@@ -66,9 +67,10 @@ def counter_per_dispatch(db_path, counter, kernel_like):
 
 def counters(a):
     """Average per-dispatch value of every counter in the given --pmc runs for
-    the kernels whose name contains a.kernel, and the derived utilisations
-    (CU count from a.cus; each SIMD issues one VALU instruction of a wave in
-    4 cycles)."""
+    the kernels whose name contains a.kernel (with a.largest: over the
+    a.largest largest values of each counter, the dispatches of the largest
+    level), and the derived utilisations (CU count from a.cus; each SIMD
+    issues one VALU instruction of a wave in 4 cycles)."""
     vals = {}
     n = {}
     for db_path in a.dbs:
@@ -76,6 +78,8 @@ def counters(a):
         names = [r[0] for r in db.execute("select distinct counter_name from counters_collection")]
         for c in names:
             v = counter_per_dispatch(db_path, c, a.kernel)
+            if a.largest:
+                v = sorted(v, reverse=True)[:a.largest]
             if v:
                 vals[c] = sum(v) / len(v)
                 n[c] = len(v)
@@ -100,7 +104,7 @@ def counters(a):
         d["wave_cycles_waiting_lds_pct"] = 100 * vals["SQ_WAIT_INST_LDS"] / vals["SQ_WAVE_CYCLES"]
     if "SQ_WAIT_ANY" in vals and "SQ_WAVE_CYCLES" in vals and vals["SQ_WAVE_CYCLES"]:
         d["wave_cycles_waiting_any_pct"] = 100 * vals["SQ_WAIT_ANY"] / vals["SQ_WAVE_CYCLES"]
-    out = {"kernel": a.kernel, "dispatches": max(n.values()) if n else 0, "cus": cus,
+    out = {"kernel": a.kernel, "dispatches": max(n.values()) if n else 0, "largest": a.largest, "cus": cus,
            "per_dispatch": {k: round(v, 1) for k, v in sorted(vals.items())},
            "derived": {k: round(v, 2) for k, v in d.items()}, "command": a.command}
     with open(a.out, "w") as f:
@@ -160,6 +164,7 @@ def main(argv=None):
     k.add_argument("--kernel", required=True)
     k.add_argument("--cus", type=int, default=256)
     k.add_argument("--xcds", type=int, default=8)
+    k.add_argument("--largest", type=int, default=0, help="average the largest N values of each counter")
     k.add_argument("--out", required=True)
     k.add_argument("--command", required=True)
     a = ap.parse_args(argv)
