@@ -617,6 +617,39 @@ void plan_step_relax(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<Leve
   st.T = lt.tmax;
 }
 
+// The steps of one level's items: each item size whose own schedule is
+// streamed gets a step of its own (a streamed launch has one tile shape, and
+// a step of several sizes plans tiled schedules only); the other items share
+// one step.  Items keep their order within a step.
+std::vector<std::vector<size_t>> level_steps(rgbdflow_ctx* c, const rgbdflow_params* p,
+                                             const std::vector<LevelView>& items) {
+  std::vector<std::vector<size_t>> out;
+  std::vector<std::pair<int, int>> shapes;  // distinct sizes, first-seen order
+  for (const LevelView& v : items)
+    if (std::find(shapes.begin(), shapes.end(), std::make_pair(v.W, v.H)) == shapes.end())
+      shapes.emplace_back(v.W, v.H);
+  std::vector<size_t> rest;
+  if (p && p->relax_variant == RGBDFLOW_RELAX_TILED && shapes.size() > 1) {
+    std::vector<std::pair<int, int>> own;
+    for (const auto& sh : shapes) {
+      const TileSched t = cached_plan(c, sh.first, sh.second, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
+      if (t.cap != 0 && tiled_config_stream(t.config) > 0) own.push_back(sh);
+    }
+    for (const auto& sh : own) {
+      std::vector<size_t> g;
+      for (size_t i = 0; i < items.size(); i++)
+        if (items[i].W == sh.first && items[i].H == sh.second) g.push_back(i);
+      out.push_back(g);
+    }
+    for (size_t i = 0; i < items.size(); i++)
+      if (std::find(own.begin(), own.end(), std::make_pair(items[i].W, items[i].H)) == own.end()) rest.push_back(i);
+  } else {
+    for (size_t i = 0; i < items.size(); i++) rest.push_back(i);
+  }
+  if (!rest.empty()) out.push_back(rest);
+  return out;
+}
+
 size_t pack(std::vector<char>& host, const void* p, size_t bytes) {
   const size_t off = (host.size() + 255) & ~size_t(255);
   host.resize(off + bytes);
@@ -1284,11 +1317,19 @@ int batch_plan(rgbdflow_ctx* c, const rgbdflow_params* p) {
       v.f_level = downsized_focal(cam, Wk);
       items.push_back(v);
     }
-    StepPlan st;
-    plan_step_relax(c, p, items, st);
-    pack_step(bp.host, items, st);
-    pack_coarse(c, p, bp.host, items, works, st);
-    bp.steps.push_back(std::move(st));
+    for (const std::vector<size_t>& g : level_steps(c, p, items)) {
+      std::vector<LevelView> gi;
+      std::vector<const Work*> gw;
+      for (size_t i : g) {
+        gi.push_back(items[i]);
+        gw.push_back(works[i]);
+      }
+      StepPlan st;
+      plan_step_relax(c, p, gi, st);
+      pack_step(bp.host, gi, st);
+      pack_coarse(c, p, bp.host, gi, gw, st);
+      bp.steps.push_back(std::move(st));
+    }
   }
   char* dev;
   TRY(upload_desc(c, "desc", bp.host, &dev));
