@@ -565,10 +565,10 @@ def run_batch(ctx, p, batch, steps, warmup, latency_steps):
 
 
 def mixed_batch(R, args, n, label_maps=False):
-    """n pairs of mixed sizes in one batch (every level step of every pair in
-    one launch per kernel), schedules autotuned per level shape; f32, then
-    FP16 records, then FP16 records and images (the same schedules: a step of
-    several sizes takes the tuned tiled ones)."""
+    """n pairs of mixed sizes in one batch, schedules autotuned per level
+    shape (f32); at each level, the sizes whose schedule is streamed relax in
+    steps of their own and the other sizes share one step.  f32, then FP16
+    records, then FP16 records and images, under the same schedules."""
     pairs, sizes = mixed_workload(R, n, max(1, args.distinct // 2))
     if not label_maps:
         pairs = [dict(pr, hpb=None) for pr in pairs]
