@@ -291,3 +291,30 @@ def test_batch_boundary_maps_two_groups(ctx, R):
         ub, vb, wb = ctx.batch_read(i)
         assert [sts[i].sweeps_exec[k] for k in range(4)] == [sa.sweeps_exec[k] for k in range(4)]
         assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb), i
+
+
+@pytest.mark.parametrize("schedule", ["streamed", "planned"])
+def test_batch_sizes_in_steps_of_their_own(ctx, R, schedule):
+    """Four or more pairs of one size whose schedule is streamed relax in a
+    step of their own at each level, the other sizes share one step; with
+    params.graph the steps are captured and replayed.  Every pair equals the
+    oracle's solve of it alone, every run."""
+    cases = [_case(160, 120, 61 + i, 20) for i in range(4)] + [_case(96, 72, 71, 12), _case(200, 150, 72, 25),
+                                                               _case(96, 72, 73, 12)]
+    kw = {}
+    if schedule == "streamed":
+        kw["tile_config"] = max((sh["streamed"], i) for i, sh in enumerate(R.relax_shapes()) if sh["streamed"] > 0)[1]
+    p = R.make_params(160, 120, n_outer=3, n_sor=30, min_width=20, graph=True, relax_variant=R.RELAX_TILED, **kw)
+    pairs = [dict(c, cam=R.make_params(c["W"], c["H"]).cam) for c in cases]
+    ctx.batch_load(p, pairs)
+    ref = [_oracle_solve(c, 3, 30) for c in cases]
+    for run in range(3):
+        print(f"{schedule} run {run}", flush=True)
+        st = ctx.batch_run(p, stats=True)
+        for i, c in enumerate(cases):
+            ua, va, wa, sa = ref[i]
+            ub, vb, wb = ctx.batch_read(i)
+            L = sa.n_levels
+            assert [st[i].outer_exec[k] for k in range(L)] == [sa.outer_exec[k] for k in range(L)], (run, i)
+            assert [st[i].sweeps_exec[k] for k in range(L)] == [sa.sweeps_exec[k] for k in range(L)], (run, i)
+            assert np.array_equal(ua, ub) and np.array_equal(va, vb) and np.array_equal(wa, wb), (run, i)
