@@ -617,12 +617,13 @@ void plan_step_relax(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<Leve
   st.T = lt.tmax;
 }
 
-// The steps of one level's items: each item size whose own schedule is
-// streamed and that kStepSplitItems items or more have gets a step of its
-// own (a streamed launch has one tile shape, and a step of several sizes
-// plans tiled schedules only); the other items share one step, so a few
-// pairs of several sizes (served batches) keep one launch per kernel and
-// their latency.  Items keep their order within a step.
+// The steps of one level's items: each item size that kStepSplitItems items
+// or more have (and that has a tiled schedule) gets a step of its own, so it
+// relaxes under its own schedule -- a step of several sizes plans its items
+// under its largest item's workgroup shape and iteration count, and tiled
+// schedules only (a streamed launch has one tile shape).  The other items
+// share one step, so a few pairs of several sizes (served batches) keep one
+// launch per kernel and their latency.  Items keep their order within a step.
 constexpr int kStepSplitItems = 4;
 std::vector<std::vector<size_t>> level_steps(rgbdflow_ctx* c, const rgbdflow_params* p,
                                              const std::vector<LevelView>& items) {
@@ -636,7 +637,7 @@ std::vector<std::vector<size_t>> level_steps(rgbdflow_ctx* c, const rgbdflow_par
     std::vector<std::pair<int, int>> own;
     for (const auto& sh : shapes) {
       const TileSched t = cached_plan(c, sh.first, sh.second, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
-      if (t.cap == 0 || tiled_config_stream(t.config) == 0) continue;
+      if (t.cap == 0) continue;
       long long count = 0;
       for (const LevelView& v : items) count += v.W == sh.first && v.H == sh.second;
       if (count >= kStepSplitItems) own.push_back(sh);
