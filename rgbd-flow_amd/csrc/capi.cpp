@@ -621,10 +621,11 @@ void plan_step_relax(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<Leve
 // or more have (and that has a tiled schedule) gets a step of its own, so it
 // relaxes under its own schedule -- a step of several sizes plans its items
 // under its largest item's workgroup shape and iteration count, and tiled
-// schedules only (a streamed launch has one tile shape).  The other items
-// share one step, so a few pairs of several sizes (served batches) keep one
-// launch per kernel and their latency.  Items keep their order within a step.
-constexpr int kStepSplitItems = 4;
+// schedules only (a streamed launch has one tile shape).  The sizes of one
+// item each share one step, so a served batch at light load (a pair or so of
+// each size) keeps one launch per kernel and its latency.  Items keep their
+// order within a step.
+constexpr int kStepSplitItems = 2;
 std::vector<std::vector<size_t>> level_steps(rgbdflow_ctx* c, const rgbdflow_params* p,
                                              const std::vector<LevelView>& items) {
   std::vector<std::vector<size_t>> out;

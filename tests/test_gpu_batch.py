@@ -295,10 +295,11 @@ def test_batch_boundary_maps_two_groups(ctx, R):
 
 @pytest.mark.parametrize("schedule", ["streamed", "planned"])
 def test_batch_sizes_in_steps_of_their_own(ctx, R, schedule):
-    """Four or more pairs of one size relax in a step of their own at each
-    level (under a streamed schedule, or the planned one), the other sizes
-    share one step; with params.graph the steps are captured and replayed.
-    Every pair equals the oracle's solve of it alone, every run."""
+    """Two or more pairs of one size relax in a step of their own at each
+    level (under a streamed schedule, or the planned one), a size of one pair
+    shares a step with the other such sizes; with params.graph the steps are
+    captured and replayed.  Every pair equals the oracle's solve of it alone,
+    every run."""
     cases = [_case(160, 120, 61 + i, 20) for i in range(4)] + [_case(96, 72, 71, 12), _case(200, 150, 72, 25),
                                                                _case(96, 72, 73, 12)]
     kw = {}
