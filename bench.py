@@ -564,7 +564,7 @@ def run_batch(ctx, p, batch, steps, warmup, latency_steps):
     return len(batch) * steps / el, el, lat, st
 
 
-def mixed_batch(R, args, n, label_maps=False):
+def mixed_batch(R, args, n, label_maps=False, schedules_in=None, schedules_out=None):
     """n pairs of mixed sizes in one batch, schedules autotuned per level
     shape (f32); at each level, the sizes whose schedule is streamed relax in
     steps of their own and the other sizes share one step.  f32, then FP16
@@ -580,11 +580,21 @@ def mixed_batch(R, args, n, label_maps=False):
     with R.Context(0) as ctx:
         p = R.make_params(640, 480, n_outer=15, n_sor=50, min_width=40, relax_variant=R.RELAX_TILED,
                           graph=not args.no_graph)
-        if not args.no_autotune:
+        if schedules_in:
+            # schedules tuned by an earlier run (e.g. an unprofiled one)
+            with open(schedules_in) as f:
+                for ch in json.load(f)["mixed"]:
+                    ctx.set_relax_schedule(ch["W"], ch["H"], ch)
+            log("mixed batch: schedules loaded")
+        elif not args.no_autotune:
+            tuned = []
             for (w, h) in sorted(set(sizes), key=lambda t: -t[0]):
                 for (lw, lh) in R.pyramid_levels(w, h, 0.5, w // 16):
-                    ctx.autotune_relax(lw, lh, batch=sizes.count((w, h)), params=p)
+                    tuned.append(dict(ctx.autotune_relax(lw, lh, batch=sizes.count((w, h)), params=p), W=lw, H=lh))
                 log(f"mixed batch: f32 schedules of the {w}x{h} levels tuned")
+            if schedules_out:
+                with open(schedules_out, "w") as f:
+                    json.dump({"mixed": tuned}, f)
         npx = sum(w * h for (w, h) in sizes)
         for name, st, img in (("f32", R.STORE_F32, R.IMAGES_F64), ("fp16", R.STORE_F16, R.IMAGES_F64),
                               ("fp16_images", R.STORE_F16, R.IMAGES_F16)):
@@ -600,7 +610,8 @@ def mixed_batch(R, args, n, label_maps=False):
 def mixed_main(R, args):
     """The mixed-size batch alone (--mixed pairs, --steps timed batches)."""
     args.mixed_steps = args.steps
-    r = mixed_batch(R, args, args.mixed, label_maps=args.label_maps)
+    r = mixed_batch(R, args, args.mixed, label_maps=args.label_maps, schedules_in=args.load_schedules,
+                    schedules_out=args.save_schedules)
     out = {"metric": "mixed-size scene-flow pairs/s (one batch)", "value": r["f32"]["pairs_per_s"], "unit": "pairs/s",
            "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": r["f32"]["ms_per_step"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
