@@ -389,13 +389,14 @@ int rgbdflow_relax_shape(int index, int* threads, int* pairs_per_thread, int* st
  * shape, iterations per launch) on `batch` W x H levels relaxed by one launch
  * each, with systems stored as params->storage (params may be NULL: f32), and
  * makes the fastest the context's schedule for that shape whenever tile_* are
- * left to the planner (tile_iters = tile_w = tile_h = 0, tile_config = -1).
- * out may be NULL. */
+ * left to the planner (tile_iters = tile_w = tile_h = 0, tile_config = -1):
+ * for batches of that record storage, and for the others until they have a
+ * schedule tuned with their own storage.  out may be NULL. */
 int rgbdflow_autotune_relax(rgbdflow_ctx* ctx, int W, int H, int batch, const rgbdflow_params* params,
                             rgbdflow_tile_choice* out);
 /* Makes a schedule (as autotune_relax reports it: tile_w, tile_h, tile_iters,
- * tile_config, single) the context's schedule for W x H levels, e.g. one
- * tuned earlier and stored by the caller. */
+ * tile_config, single) the context's schedule for W x H levels, every record
+ * storage, e.g. one tuned earlier and stored by the caller. */
 int rgbdflow_set_relax_schedule(rgbdflow_ctx* ctx, int W, int H, const rgbdflow_tile_choice* choice);
 /* Device time per red-black iteration of a schedule on `batch` W x H levels,
  * measured as autotune_relax measures its candidates. */

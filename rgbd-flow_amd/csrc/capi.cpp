@@ -565,9 +565,18 @@ std::string plan_key(int W, int H, int T, int tw, int th, int cfg) {
   return key;
 }
 
-// Cached plan_tiles.
-TileSched cached_plan(rgbdflow_ctx* c, int W, int H, int T, int tw, int th, int cfg) {
+// The key of a schedule the autotuner timed with one record storage.
+std::string storage_key(const std::string& key, int storage) { return key + "/s" + std::to_string(storage); }
+
+// Cached plan_tiles.  For a request without tile sizes (cfg -1 or
+// kAnyTiledConfig), a schedule tuned with the given record storage comes
+// first; else the one last tuned or installed for the shape.
+TileSched cached_plan(rgbdflow_ctx* c, int W, int H, int T, int tw, int th, int cfg, int storage = -1) {
   const std::string key = plan_key(W, H, T, tw, th, cfg);
+  if (storage >= 0 && T == 0 && tw == 0 && th == 0 && (cfg == -1 || cfg == kAnyTiledConfig)) {
+    auto st = c->plans.find(storage_key(key, storage));
+    if (st != c->plans.end()) return st->second;
+  }
   auto it = c->plans.find(key);
   if (it == c->plans.end()) it = c->plans.emplace(key, plan_tiles(c, W, H, T, tw, th, cfg)).first;
   return it->second;
@@ -585,14 +594,15 @@ void plan_step_relax(rgbdflow_ctx* c, const rgbdflow_params* p, std::vector<Leve
   for (size_t i = 1; i < items.size(); i++)
     if (items[i].N > items[lead].N) lead = i;
   const int LW = items[lead].W, LH = items[lead].H;
-  TileSched lt = cached_plan(c, LW, LH, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
+  const int storage = items[lead].storage;
+  TileSched lt = cached_plan(c, LW, LH, p->tile_iters, p->tile_w, p->tile_h, p->tile_config, storage);
   // items of several sizes: tiled schedules only (a streamed workgroup's
   // time is its tiles' rows plus a ring fill and drain, which smaller items'
   // shorter tiles do not amortise)
   bool uniform = true;
   for (const LevelView& v : items) uniform = uniform && v.W == LW && v.H == LH;
   if (!uniform && lt.cap != 0 && tiled_config_stream(lt.config) > 0)
-    lt = cached_plan(c, LW, LH, p->tile_iters, p->tile_w, p->tile_h, kAnyTiledConfig);
+    lt = cached_plan(c, LW, LH, p->tile_iters, p->tile_w, p->tile_h, kAnyTiledConfig, storage);
   if (lt.cap == 0) return;
   std::vector<TileSched> ts(items.size());
   for (size_t i = 0; i < items.size(); i++) {
@@ -637,7 +647,8 @@ std::vector<std::vector<size_t>> level_steps(rgbdflow_ctx* c, const rgbdflow_par
   if (p && p->relax_variant == RGBDFLOW_RELAX_TILED && shapes.size() > 1) {
     std::vector<std::pair<int, int>> own;
     for (const auto& sh : shapes) {
-      const TileSched t = cached_plan(c, sh.first, sh.second, p->tile_iters, p->tile_w, p->tile_h, p->tile_config);
+      const TileSched t =
+          cached_plan(c, sh.first, sh.second, p->tile_iters, p->tile_w, p->tile_h, p->tile_config, p->storage);
       if (t.cap == 0) continue;
       long long count = 0;
       for (const LevelView& v : items) count += v.W == sh.first && v.H == sh.second;
@@ -2363,9 +2374,16 @@ int rgbdflow_autotune_relax(rgbdflow_ctx* c, int W, int H, int batch, const rgbd
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (rc != RGBDFLOW_OK) return rc;
+  // the shape's schedule, and the one of this record storage (a batch of
+  // another storage takes it only if that storage has none of its own)
+  const int storage = p ? p->storage : RGBDFLOW_STORE_F32;
   c->plans[plan_key(W, H, 0, 0, 0, -1)] = best;
+  c->plans[storage_key(plan_key(W, H, 0, 0, 0, -1), storage)] = best;
   // the best tiled schedule too, for steps whose items have several sizes
-  if (best_tiled.cap != 0) c->plans[plan_key(W, H, 0, 0, 0, kAnyTiledConfig)] = best_tiled;
+  if (best_tiled.cap != 0) {
+    c->plans[plan_key(W, H, 0, 0, 0, kAnyTiledConfig)] = best_tiled;
+    c->plans[storage_key(plan_key(W, H, 0, 0, 0, kAnyTiledConfig), storage)] = best_tiled;
+  }
   c->plans_gen++;
   if (out) choice_from_schedule(best, (int)cand.size(), best_us, out);
   return RGBDFLOW_OK;
@@ -2397,6 +2415,11 @@ int rgbdflow_set_relax_schedule(rgbdflow_ctx* c, int W, int H, const rgbdflow_ti
   TRY(schedule_from_choice(c, W, H, ch, &t));
   c->plans[plan_key(W, H, 0, 0, 0, -1)] = t;
   if (tiled_config_stream(t.config) == 0) c->plans[plan_key(W, H, 0, 0, 0, kAnyTiledConfig)] = t;
+  // an installed schedule holds for every record storage
+  for (int s = RGBDFLOW_STORE_F32; s <= RGBDFLOW_STORE_BF16; s++) {
+    c->plans.erase(storage_key(plan_key(W, H, 0, 0, 0, -1), s));
+    c->plans.erase(storage_key(plan_key(W, H, 0, 0, 0, kAnyTiledConfig), s));
+  }
   c->plans_gen++;
   return RGBDFLOW_OK;
 }
